@@ -1,0 +1,185 @@
+// xc_fft.hpp -- LDS-staged Stockham FFT building blocks for CDNA4 (gfx950).
+//
+// One length-N complex transform is carried by TPF = N/E threads, each
+// holding E = min(8, N) elements in registers in the "canonical" layout
+//     reg[m] = x[t + TPF*m],  m in [0, E)
+// Passes are radix-8 with one final radix-4/2 pass.  A pass reads its R
+// inputs j + r*N/R from registers (they are exactly reg[b + r*E/R] for
+// butterfly j = t + TPF*b), multiplies by the Stockham twiddles, runs the
+// radix-R butterfly, and scatters the outputs to LDS at
+//     (j/Ns)*Ns*R + j%Ns + r*Ns
+// from which the next pass reloads the canonical layout.  The last pass's
+// outputs land back in canonical order in registers, so a forward and an
+// inverse transform chain without touching LDS (used to keep the cross
+// power in registers between the column forward and inverse transforms).
+//
+// No rocFFT, no MFMA: an FFT is not a dense contraction.  Twiddles come from
+// one host-generated FP64-accurate table of e^{-2 pi i m / XC_TW_N}.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xc {
+
+constexpr int XC_TW_N = 8192;          // twiddle table length (largest Nx)
+constexpr int XC_MAX_NH = 4096;        // largest complex transform length
+
+template <typename T> struct cpx { T x, y; };
+
+template <typename T> __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T> __device__ __forceinline__ cpx<T> cconj(cpx<T> a) { return {a.x, -a.y}; }
+template <typename T> __device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// Multiply by -i (DIR < 0, forward) or +i (DIR > 0, inverse).
+template <int DIR, typename T> __device__ __forceinline__ cpx<T> rot90(cpx<T> a)
+{
+    if constexpr (DIR < 0) return {a.y, -a.x};
+    else return {-a.y, a.x};
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void bfly2(cpx<T> *v)
+{
+    cpx<T> t = v[0] - v[1];
+    v[0] = v[0] + v[1];
+    v[1] = t;
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void bfly4(cpx<T> *v)
+{
+    cpx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+    cpx<T> t2 = v[1] + v[3], t3 = rot90<DIR>(v[1] - v[3]);
+    v[0] = t0 + t2;
+    v[2] = t0 - t2;
+    v[1] = t1 + t3;
+    v[3] = t1 - t3;
+}
+
+template <int DIR, typename T> __device__ __forceinline__ void bfly8(cpx<T> *v)
+{
+    const T s = (T)0.70710678118654752440;
+    cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    bfly4<DIR>(e);
+    bfly4<DIR>(o);
+    if constexpr (DIR < 0) {
+        o[1] = {s * (o[1].x + o[1].y), s * (o[1].y - o[1].x)};
+        o[2] = {o[2].y, -o[2].x};
+        o[3] = {s * (o[3].y - o[3].x), -s * (o[3].x + o[3].y)};
+    } else {
+        o[1] = {s * (o[1].x - o[1].y), s * (o[1].x + o[1].y)};
+        o[2] = {-o[2].y, o[2].x};
+        o[3] = {-s * (o[3].x + o[3].y), s * (o[3].x - o[3].y)};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        v[k] = e[k] + o[k];
+        v[k + 4] = e[k] - o[k];
+    }
+}
+
+template <int DIR, int R, typename T> __device__ __forceinline__ void bfly(cpx<T> *v)
+{
+    if constexpr (R == 8) bfly8<DIR>(v);
+    else if constexpr (R == 4) bfly4<DIR>(v);
+    else if constexpr (R == 2) bfly2<DIR>(v);
+}
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
+
+template <int N> struct Plan {
+    static constexpr int L = ilog2c(N);
+    static constexpr int E = N >= 8 ? 8 : N;
+    static constexpr int TPF = N / E;
+    static constexpr int NP = L == 0 ? 0 : (L + 2) / 3;
+    static constexpr int radix(int p) { return (p < L / 3) ? 8 : (1 << (L % 3)); }
+    static constexpr int stride_before(int p)
+    {
+        int s = 1;
+        for (int q = 0; q < p; ++q) s *= radix(q);
+        return s;
+    }
+};
+
+// Twiddle e^{DIR 2 pi i m / n} for n a power of two dividing XC_TW_N.
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> twiddle(const cpx<T> *__restrict__ tw, int m, int n)
+{
+    cpx<T> w = tw[(m & (n - 1)) * (XC_TW_N / n)];
+    if constexpr (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+// LDS view: element pos of this transform lives at base[pos * stride].
+template <typename T> struct LdsView {
+    cpx<T> *base;
+    int stride;
+    __device__ __forceinline__ cpx<T> &operator[](int pos) const { return base[pos * stride]; }
+};
+
+// One length-N transform in the canonical register layout.  Every thread
+// of the workgroup must call it (it contains __syncthreads), including
+// threads whose data is padding.
+template <int DIR, int N, typename T, int P = 0>
+__device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds,
+                                         const cpx<T> *__restrict__ tw)
+{
+    using PL = Plan<N>;
+    if constexpr (P < PL::NP) {
+        constexpr int E = PL::E, TPF = PL::TPF;
+        constexpr int R = PL::radix(P);
+        constexpr int NB = E / R;
+        constexpr int Ns = PL::stride_before(P);
+        if constexpr (P > 0) {
+            // reload canonical layout written by the previous pass
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) v[m] = lds[t + TPF * m];
+        }
+        cpx<T> u[NB][R];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int j = t + TPF * b;
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                u[b][r] = v[b + r * NB];
+                if constexpr (P > 0) {
+                    if (r > 0) u[b][r] = cmul(u[b][r], twiddle<DIR>(tw, r * k, Ns * R));
+                }
+            }
+            bfly<DIR, R>(u[b]);
+        }
+        if constexpr (P + 1 == PL::NP) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < R; ++r) v[b + r * NB] = u[b][r];
+        } else {
+            __syncthreads();  // everyone finished reading the previous contents
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = t + TPF * b;
+                const int k = j & (Ns - 1);
+                const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int r = 0; r < R; ++r) lds[base + r * Ns] = u[b][r];
+            }
+            fft_pass<DIR, N, T, P + 1>(v, t, lds, tw);
+        }
+    }
+}
+
+template <int DIR, int N, typename T>
+__device__ __forceinline__ void fft(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds,
+                                    const cpx<T> *__restrict__ tw)
+{
+    fft_pass<DIR, N, T, 0>(v, t, lds, tw);
+}
+
+}  // namespace xc

@@ -1,0 +1,103 @@
+// xc_host.hpp -- host-side context, workspace and launch helpers of libxcorr.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/xcorr.h"
+#include "xc_fft.hpp"
+#include "xc_types.hpp"
+
+namespace xc {
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+// Bump allocator over one grow-only device arena: every call of an entry
+// point lays its buffers out from offset 0 again.
+struct Arena {
+    DevBuf buf;
+    size_t used = 0;
+    std::vector<std::pair<size_t, size_t>> plan;  // (offset, bytes) of each reservation
+};
+
+struct TileRec;
+
+}  // namespace xc
+
+struct xc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    xc::cpx<double> *tw64 = nullptr;
+    xc::cpx<float> *tw32 = nullptr;
+    xc::DevBuf arena;       // per-call device workspace
+    void *pinned = nullptr; // pinned host staging
+    size_t pinned_bytes = 0;
+    int num_cu = 0;
+};
+
+namespace xc {
+
+// Status helpers ----------------------------------------------------------
+
+int set_err(xc_ctx *ctx, int code, const char *fmt, ...);
+
+#define XC_HIP(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return ::xc::set_err((ctx), XC_E_DEVICE, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                                            \
+    } while (0)
+
+// Workspace layout: reserve sizes, then materialise once.
+struct Layout {
+    std::vector<size_t> off;
+    size_t total = 0;
+    size_t add(size_t bytes)
+    {
+        size_t o = (total + 255) & ~(size_t)255;
+        off.push_back(o);
+        total = o + bytes;
+        return off.size() - 1;
+    }
+};
+
+int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes);
+int ensure_pinned(xc_ctx *ctx, size_t bytes);
+
+// Host mirror of the reference geometry helpers.
+struct IBox {
+    int L, R, B, T;
+};
+void bbox_from_points(IBox &B, const xc_point *p, int n);  // Geometry.cpp:78-106
+int ceil_pow2(int n);                                      // Maths.cpp:97-105
+
+RuleDev make_rule(const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, const uint8_t *dev_table);
+
+// Fill the derived geometry of a PairDev from (w1,h1,w2,h2) and the mode.
+int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode);
+
+// Kernel launchers (dispatch on transform length).
+template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work);
+template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nblocks, const PairDev *pd, const int2 *work);
+template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int2 *work);
+template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd);
+template <typename T> int row_group(int NH);   // rows per workgroup of the row kernels
+template <typename T> int col_group(int NY);   // columns per workgroup of the column kernel
+
+template <typename T> const cpx<T> *twiddles(xc_ctx *ctx);
+
+// Largest supported transform lengths for precision T.
+template <typename T> constexpr int max_nh() { return sizeof(T) == 8 ? 2048 : 4096; }
+template <typename T> constexpr int max_ny() { return 4096; }
+
+}  // namespace xc

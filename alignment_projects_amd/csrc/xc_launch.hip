@@ -1,0 +1,341 @@
+// xc_launch.hip -- context management, workspace and kernel dispatch of
+// libxcorr (everything the C-ABI entry points share).
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "xc_host.hpp"
+#include "xc_kernels.hpp"
+
+namespace xc {
+
+int set_err(xc_ctx *ctx, int code, const char *fmt, ...)
+{
+    if (ctx) {
+        char buf[1024];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes)
+{
+    if (bytes <= b.bytes) return XC_OK;
+    if (b.p) {
+        XC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        XC_HIP(ctx, hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 4;
+    if (hipMalloc(&b.p, want) != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError();
+        return set_err(ctx, XC_E_NOMEM, "hipMalloc(%zu) failed", want);
+    }
+    b.bytes = want;
+    return XC_OK;
+}
+
+int ensure_pinned(xc_ctx *ctx, size_t bytes)
+{
+    if (bytes <= ctx->pinned_bytes) return XC_OK;
+    if (ctx->pinned) {
+        XC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        XC_HIP(ctx, hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+    }
+    size_t want = bytes + bytes / 4 + 4096;
+    if (hipHostMalloc(&ctx->pinned, want, hipHostMallocDefault) != hipSuccess) {
+        ctx->pinned = nullptr;
+        (void)hipGetLastError();
+        return set_err(ctx, XC_E_NOMEM, "hipHostMalloc(%zu) failed", want);
+    }
+    ctx->pinned_bytes = want;
+    return XC_OK;
+}
+
+void bbox_from_points(IBox &B, const xc_point *p, int n)
+{
+    double xlo, xhi, ylo, yhi;
+    xlo = xhi = p[0].x;
+    ylo = yhi = p[0].y;
+    for (int i = 1; i < n; ++i) {
+        double t;
+        if ((t = p[i].x) < xlo) xlo = t;
+        else if (t > xhi) xhi = t;
+        if ((t = p[i].y) < ylo) ylo = t;
+        else if (t > yhi) yhi = t;
+    }
+    B.L = (int)floor(xlo);
+    B.R = (int)ceil(xhi);
+    B.B = (int)floor(ylo);
+    B.T = (int)ceil(yhi);
+}
+
+int ceil_pow2(int n)
+{
+    int p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+
+RuleDev make_rule(const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, const uint8_t *dev_table)
+{
+    RuleDev r;
+    memset(&r, 0, sizeof r);
+    if (rgn && rgn->enabled) {
+        r.rgn_enabled = 1;
+        r.min_w = rgn->min_w;
+        r.min_h = rgn->min_h;
+        r.exact_w = rgn->exact_w;
+        r.exact_h = rgn->exact_h;
+        r.or_w_ge = rgn->or_w_ge;
+        r.or_h_gt = rgn->or_h_gt;
+        r.min_area = rgn->min_area;
+        r.table_w = rgn->table_w;
+        r.table_h = rgn->table_h;
+        r.table = rgn->table ? dev_table : nullptr;
+    }
+    if (cnt && cnt->enabled) {
+        r.cnt_enabled = 1;
+        r.k1 = cnt->k1;
+        r.k2 = cnt->k2;
+    }
+    return r;
+}
+
+int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode)
+{
+    P.w1 = w1;
+    P.h1 = h1;
+    P.w2 = w2;
+    P.h2 = h2;
+    P.Nx_ref = ceil_pow2(w1 + w2 - 1);
+    P.Ny_ref = ceil_pow2(h1 + h2 - 1);
+    P.Nx = P.Nx_ref < 2 ? 2 : P.Nx_ref;
+    P.Ny = P.Ny_ref;
+    P.mode = mode;
+    P.lag_order = mode == PM_NCC ? LAG_RASTER : LAG_CENTRED;
+    P.nnegx = w1 - 1;
+    P.nposx = w2;
+    P.nnegy = h1 - 1;
+    P.nposy = h2;
+    P.wR = P.nnegx + P.nposx;
+    P.hR = P.nnegy + P.nposy;
+    return XC_OK;
+}
+
+template <> const cpx<double> *twiddles<double>(xc_ctx *ctx) { return ctx->tw64; }
+template <> const cpx<float> *twiddles<float>(xc_ctx *ctx) { return ctx->tw32; }
+
+// ---------------------------------------------------------------------------
+// Dispatch on power-of-two transform lengths
+// ---------------------------------------------------------------------------
+
+#define XC_POW2_CASES(F) \
+    F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048) F(4096)
+
+template <typename T> int row_group(int NH)
+{
+    switch (NH) {
+#define F(n) \
+    case n: return RowCfg<n>::G;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return 1;
+}
+
+template <typename T> int col_group(int NY)
+{
+    switch (NY) {
+#define F(n) \
+    case n: return ColCfg<T, n>::C;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return 1;
+}
+
+template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
+{
+    if constexpr (NH > max_nh<T>()) {
+        return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
+    } else {
+        const size_t smem = sizeof(cpx<T>) * RowCfg<NH>::G * NH;
+        hipLaunchKernelGGL((k_row_r2c<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
+                           twiddles<T>(ctx));
+        XC_HIP(ctx, hipGetLastError());
+        return XC_OK;
+    }
+}
+
+template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int4 *w)
+{
+    if (nb <= 0) return XC_OK;
+    switch (NH) {
+#define F(n) \
+    case n: return r2c_n<T, n>(ctx, nb, pd, w);
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
+}
+
+template <typename T, int NY> static int col_n(xc_ctx *ctx, bool whiten, int nb, const PairDev *pd, const int2 *w)
+{
+    if constexpr (NY > max_ny<T>()) {
+        return set_err(ctx, XC_E_SHAPE, "column transform %d too long", NY);
+    } else {
+        using CC = ColCfg<T, NY>;
+        const size_t smem = sizeof(cpx<T>) * CC::C * NY;
+        if (whiten)
+            hipLaunchKernelGGL((k_col_fused<T, NY, true>), dim3(nb), dim3(CC::THREADS), smem, ctx->stream, pd, w,
+                               twiddles<T>(ctx));
+        else
+            hipLaunchKernelGGL((k_col_fused<T, NY, false>), dim3(nb), dim3(CC::THREADS), smem, ctx->stream, pd, w,
+                               twiddles<T>(ctx));
+        XC_HIP(ctx, hipGetLastError());
+        return XC_OK;
+    }
+}
+
+template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nb, const PairDev *pd, const int2 *w)
+{
+    if (nb <= 0) return XC_OK;
+    switch (NY) {
+#define F(n) \
+    case n: return col_n<T, n>(ctx, whiten, nb, pd, w);
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "unsupported column length %d", NY);
+}
+
+template <typename T, int NH> static int c2r_n(xc_ctx *ctx, int nb, const PairDev *pd, const int2 *w)
+{
+    if constexpr (NH > max_nh<T>()) {
+        return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
+    } else {
+        const size_t smem = c2r_smem<T, NH>();
+        hipLaunchKernelGGL((k_row_c2r_eval<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
+                           twiddles<T>(ctx));
+        XC_HIP(ctx, hipGetLastError());
+        return XC_OK;
+    }
+}
+
+template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int2 *w)
+{
+    if (nb <= 0) return XC_OK;
+    switch (NH) {
+#define F(n) \
+    case n: return c2r_n<T, n>(ctx, nb, pd, w);
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
+}
+
+template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd)
+{
+    if (npairs <= 0) return XC_OK;
+    hipLaunchKernelGGL((k_finalize<T>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
+    XC_HIP(ctx, hipGetLastError());
+    return XC_OK;
+}
+
+template int launch_row_r2c<double>(xc_ctx *, int, int, const PairDev *, const int4 *);
+template int launch_row_r2c<float>(xc_ctx *, int, int, const PairDev *, const int4 *);
+template int launch_col<double>(xc_ctx *, int, bool, int, const PairDev *, const int2 *);
+template int launch_col<float>(xc_ctx *, int, bool, int, const PairDev *, const int2 *);
+template int launch_row_c2r<double>(xc_ctx *, int, int, const PairDev *, const int2 *);
+template int launch_row_c2r<float>(xc_ctx *, int, int, const PairDev *, const int2 *);
+template int launch_finalize<double>(xc_ctx *, int, const PairDev *);
+template int launch_finalize<float>(xc_ctx *, int, const PairDev *);
+template int row_group<double>(int);
+template int row_group<float>(int);
+template int col_group<double>(int);
+template int col_group<float>(int);
+
+}  // namespace xc
+
+using namespace xc;
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+extern "C" const char *xc_version(void) { return "xcorr 0.1 (gfx950)"; }
+
+extern "C" const char *xc_last_error(const xc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int xc_ctx_create(int device, xc_ctx **out)
+{
+    if (!out) return XC_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return XC_E_DEVICE;
+    }
+    if (device < 0 || device >= ndev) return XC_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return XC_E_DEVICE;
+    xc_ctx *ctx = new xc_ctx;
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return XC_E_DEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    // e^{-2 pi i m / XC_TW_N}, from long-double exact-angle cos/sin
+    std::vector<cpx<double>> t64(XC_TW_N);
+    std::vector<cpx<float>> t32(XC_TW_N);
+    for (int m = 0; m < XC_TW_N; ++m) {
+        long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)XC_TW_N;
+        t64[m].x = (double)cosl(a);
+        t64[m].y = (double)-sinl(a);
+        t32[m].x = (float)cosl(a);
+        t32[m].y = (float)-sinl(a);
+    }
+    if (hipMalloc(&ctx->tw64, sizeof(cpx<double>) * XC_TW_N) != hipSuccess ||
+        hipMalloc(&ctx->tw32, sizeof(cpx<float>) * XC_TW_N) != hipSuccess ||
+        hipMemcpy(ctx->tw64, t64.data(), sizeof(cpx<double>) * XC_TW_N, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(ctx->tw32, t32.data(), sizeof(cpx<float>) * XC_TW_N, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
+        xc_ctx_destroy(ctx);
+        return XC_E_DEVICE;
+    }
+    *out = ctx;
+    return XC_OK;
+}
+
+extern "C" int xc_ctx_destroy(xc_ctx *ctx)
+{
+    if (!ctx) return XC_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->arena.p) (void)hipFree(ctx->arena.p);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->tw64) (void)hipFree(ctx->tw64);
+    if (ctx->tw32) (void)hipFree(ctx->tw32);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return XC_OK;
+}
+
+extern "C" int xc_fft_size(int n1, int n2) { return ceil_pow2(n1 + n2 - 1); }
+
+extern "C" int xc_fft_size_sqr(int w1, int h1, int w2, int h2)
+{
+    int a = w1 + w2, b = h1 + h2;
+    return ceil_pow2((a > b ? a : b) - 1);
+}

@@ -1,0 +1,266 @@
+/*
+ * xcorr.h -- C-ABI of the MI355X-native FFT tile-pair cross-correlation library
+ * (libxcorr.so).  Plain C types only: pointers, sizes, POD structs.  No torch,
+ * no C++ types, so it can be bound from C, C++, ctypes or any FFI.
+ *
+ * Each entry point names the reference interface it replaces
+ * (billkarsh/Alignment_Projects, 0_GEN/Correlation.h and friends).  The C++
+ * drop-in that keeps the reference's exact signatures (std::vector<Point>,
+ * EvalType callbacks, vector<CD> caches) is declared in aln_corr_compat.h and
+ * is implemented on top of this header.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (XC_OK == 0, warnings > 0,
+ *    errors < 0).  These replace the reference's exit(42)/exit(44).
+ *  - "lag (x,y)" means B-coordinate = A-coordinate + (x,y), as in the
+ *    reference's CorrPatches.
+ *  - Padded FFT sizes follow FFTSize(): Nx = CeilPow2(w1+w2-1),
+ *    Ny = CeilPow2(h1+h2-1) (Correlation.cpp:344-347).
+ *  - Host pointers are plain host memory unless the name says "dev".
+ *  - A context owns one HIP stream; calls on one context are serialised by
+ *    the context.  Distinct contexts may be used from distinct threads.
+ */
+#ifndef XCORR_H
+#define XCORR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status codes                                                        */
+/* ------------------------------------------------------------------ */
+
+enum xc_status {
+    XC_OK            = 0,
+    XC_NO_PEAK       = 1,   /* no legal lag: CorrPatches returns 0.0 and leaves dx,dy untouched
+                               (Correlation.cpp:1074-1087); CorrImages* set dx=dy=0 (:3112-3119) */
+    XC_TOO_SMALL     = 2,   /* CCorImg::SetDims: a lag half-range < 6 (Correlation.cpp:1991-2000) */
+    XC_E_ODD_R       = -44, /* |r| > 1.0001: the reference calls exit(44) (Correlation.cpp:731-751, 1331-1336) */
+    XC_E_ARG         = -2,  /* malformed argument (null pointer, empty point list, bad enum) */
+    XC_E_SHAPE       = -3,  /* padded size outside what the kernels support */
+    XC_E_DEVICE      = -4,  /* HIP runtime error */
+    XC_E_NOMEM       = -5,  /* device allocation failed */
+    XC_E_BUDGET      = -6,  /* serving admission: request exceeds the HBM budget */
+    XC_E_UNSUPPORTED = -7,  /* feature not available on this path */
+    XC_E_TIMEOUT     = -8,  /* serving: request deadline passed before completion */
+    XC_E_CANCELLED   = -9,  /* serving: request cancelled by the caller */
+    XC_E_NOTREADY    = 3    /* serving poll: result not ready yet */
+};
+
+/* ------------------------------------------------------------------ */
+/* POD types                                                           */
+/* ------------------------------------------------------------------ */
+
+/* Same layout as the reference's Point {double x, y;} (CPoint.h:14-27). */
+typedef struct xc_point {
+    double x, y;
+} xc_point;
+
+/*
+ * Legal-overlap rule, the device form of an EvalType LegalRgn(olw, olh, arg)
+ * callback (Correlation.h:91).  A lag is legal iff every enabled clause holds:
+ *   olw >= min_w && olh >= min_h && (int64)olw*olh > min_area
+ *   && (exact_w <= 0 || olw == exact_w) && (exact_h <= 0 || olh == exact_h)
+ *   && (or_w_ge <= 0 || olw >= or_w_ge || olh > or_h_gt)
+ *   && (table == 0 || table[(olh-1)*table_w + (olw-1)] != 0)
+ * enabled == 0 is the NULL callback (everything legal).
+ * Production callbacks:
+ *   BigEnough    CThmScan.cpp:28-33   min_w=min_h=olap1D, min_area=reqArea
+ *   SomeArea     testnormcorr.cpp:11  min_w=min_h=2, min_area=-1
+ *   InSectionLegal InSectionOverlap.cpp:41-47  min_area=30000, or_w_ge=1500, or_h_gt=1500
+ *   lr / lr31x31 mos.cpp:1270, Inspect.cpp:631  exact_w=exact_h=21 / 31
+ * Any other callback depends only on (olw, olh) and can be tabulated by the
+ * caller into `table` (host pointer, table_w*table_h bytes, olw<=table_w,
+ * olh<=table_h; lags outside the table are illegal).
+ */
+typedef struct xc_rgn_rule {
+    int32_t        enabled;
+    int32_t        min_w, min_h;
+    int32_t        exact_w, exact_h;
+    int32_t        or_w_ge, or_h_gt;
+    int32_t        table_w, table_h;
+    int32_t        _pad;
+    int64_t        min_area;
+    const uint8_t *table;
+} xc_rgn_rule;
+
+/*
+ * Legal-count rule, the device form of EvalType LegalCnt(c1, c2, arg), where
+ * c1,c2 are the non-zero pixel counts of the two overlap boxes:
+ *   c1 > k1 && c2 > k2.   enabled == 0 is the NULL callback.
+ *   EnoughPoints CThmScan.cpp:39-42  k1=k2=reqArea
+ *   lc / lcTRUE  mos.cpp:1282, Inspect.cpp:640 -> enabled = 0
+ */
+typedef struct xc_cnt_rule {
+    int32_t enabled;
+    int32_t _pad;
+    int64_t k1, k2;
+} xc_cnt_rule;
+
+/* Result record of one correlation. */
+typedef struct xc_result {
+    double  r;          /* value the reference returns (NCC r, or the S-mode score) */
+    double  dx, dy;     /* displacement to add to A's points (reference output params) */
+    double  peak;       /* raw value at the integer peak (NCC r for R/NCC, S for S-mode) */
+    double  sub_x;      /* parabolic subpixel offsets that were applied */
+    double  sub_y;
+    int32_t status;     /* enum xc_status */
+    int32_t lag_x;      /* integer peak lag (signed, before parabola and bbox offsets) */
+    int32_t lag_y;
+    int32_t nx, ny;     /* padded FFT size */
+    int32_t _pad;
+} xc_result;
+
+/* Correlator modes (CorrImagesR / CorrImagesS, Correlation.cpp:3144, 3222). */
+enum xc_mode {
+    XC_MODE_NCC = 0,    /* CorrPatches: normalised cross-correlation, whole-grid wrapped scan */
+    XC_MODE_R   = 1,    /* CorrImagesR: centred R image, A-mask, straight max */
+    XC_MODE_S   = 2     /* CorrImagesS: whitened cross-power S image, score 10(r-mn)/(sd sqrt(n)) */
+};
+
+/* ------------------------------------------------------------------ */
+/* Context                                                             */
+/* ------------------------------------------------------------------ */
+
+typedef struct xc_ctx xc_ctx;
+
+/* Create a context on HIP device `device` (one stream, lazily grown workspace). */
+int xc_ctx_create(int device, xc_ctx **out);
+int xc_ctx_destroy(xc_ctx *ctx);
+/* Last error message of this context (static storage, never NULL). */
+const char *xc_last_error(const xc_ctx *ctx);
+/* Library version string. */
+const char *xc_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Plain FP64 path: one pair per call, parity with the reference FFTW   */
+/* path (integer peak bit-exact, r within 1e-9 relative).              */
+/* ------------------------------------------------------------------ */
+
+/* FFTSize (Correlation.cpp:344) and FFTSizeSQR (:356). */
+int xc_fft_size(int n1, int n2);
+int xc_fft_size_sqr(int w1, int h1, int w2, int h2);
+
+/*
+ * Replaces FFT_2D (Correlation_fft_fftw.cpp:38-56): forward 2-D r2c of the
+ * row-major real array in[Nslow][Nfast], unnormalised, into
+ * out[Nslow][Nfast/2+1] complex (interleaved re,im doubles).
+ * Returns XC_OK; *M_out (may be NULL) receives Nslow*(Nfast/2+1).
+ */
+int xc_fft2d_r2c_f64(xc_ctx *ctx, const double *in, int Nfast, int Nslow,
+                     double *out, int *M_out);
+
+/* Replaces IFT_2D (Correlation_fft_fftw.cpp:67-92): unnormalised 2-D c2r. */
+int xc_ift2d_c2r_f64(xc_ctx *ctx, const double *in, int Nfast, int Nslow,
+                     double *out);
+
+/*
+ * Replaces CorrPatches (Correlation.cpp:901-1170).  Points/values are host
+ * arrays.  On XC_OK, *dx,*dy receive the displacement and *r the maximum
+ * NCC; on XC_NO_PEAK *r = 0 and *dx,*dy are left untouched, exactly as the
+ * reference does.  `res` (may be NULL) receives the full record.
+ * The padded raster is built on the device (bilinear splat of the points).
+ */
+int xc_corr_patches_f64(xc_ctx *ctx,
+                        const xc_point *ip1, const double *iv1, int n1,
+                        const xc_point *ip2, const double *iv2, int n2,
+                        int Ox, int Oy, int radius,
+                        const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                        double *dx, double *dy, double *r, xc_result *res);
+
+/*
+ * Replaces CorrPatchToImage (Correlation.cpp:1186-1355): patch1 (integer
+ * points) against the caller-embedded square N x N image i2 (N = sqrt(n_i2)).
+ * bFilter applies the reference's Gaussian low-pass of the cross power.
+ */
+int xc_corr_patch_to_image_f64(xc_ctx *ctx,
+                               const xc_point *ip1, const double *iv1, int n1,
+                               const double *i2, int n_i2,
+                               int Ox, int Oy, int radius, int bFilter,
+                               double *dx, double *dy, double *r, xc_result *res);
+
+/*
+ * Replaces CorrImagesR / CorrImagesS (Correlation.cpp:3144-3290) with
+ * mode XC_MODE_R / XC_MODE_S.  mincor and nbmaxht keep their reference
+ * meaning (nbmaxht is a dummy slot in R/S modes).  Rx<=0 or Ry<=0 disables
+ * the oval search mask.  On XC_TOO_SMALL / XC_NO_PEAK, dx=dy=0 and r=0.
+ */
+int xc_corr_images_f64(xc_ctx *ctx, int mode,
+                       const xc_point *ip1, const double *iv1, int n1,
+                       const xc_point *ip2, const double *iv2, int n2,
+                       const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                       double mincor, double nbmaxht,
+                       int Ox, int Oy, int Rx, int Ry,
+                       double *dx, double *dy, double *r, xc_result *res);
+
+/* ------------------------------------------------------------------ */
+/* Device-resident tiles (the fft2 / ThmRec.ftc cache, made explicit).  */
+/* ------------------------------------------------------------------ */
+
+typedef struct xc_tile xc_tile;
+
+enum xc_dtype { XC_U8 = 0, XC_U16 = 1, XC_F32 = 2, XC_F64 = 3 };
+
+enum xc_tile_flags {
+    XC_TILE_NORMALIZE = 1,  /* Normalize() the valid pixels: mean 0, sd 1 (n-1), Maths.cpp:192-212 */
+    XC_TILE_DEVICE_SRC = 2  /* `data`/`mask` are device pointers (else host) */
+};
+
+/*
+ * Upload (or adopt from device memory) one w x h image with optional
+ * validity mask (non-zero = pixel present in the point list; absent pixels
+ * are 0 in the raster, as ImageFromValuesAndPoints leaves them).  The tile
+ * keeps FP32 values and its FP64 summed-area tables resident in HBM.
+ */
+int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int h, int stride,
+                const uint8_t *mask, int flags, xc_tile **out);
+int xc_tile_release(xc_ctx *ctx, xc_tile *tile);
+/* Bytes of device memory the tile holds (values, tables, cached spectra). */
+int64_t xc_tile_bytes(const xc_tile *tile);
+
+/* ------------------------------------------------------------------ */
+/* Batched FP32 path                                                   */
+/* ------------------------------------------------------------------ */
+
+/*
+ * One pair correlation of a batch.  mode XC_MODE_NCC follows CorrPatches
+ * (disc Ox,Oy,radius), XC_MODE_S follows CorrImagesS (oval Ox,Oy,Rx,Ry).
+ * angle_deg != 0 rotates A's pixel grid about the origin before the
+ * correlation, as CThmScan::RFromAngle does (CThmScan.cpp:329-366).
+ */
+typedef struct xc_pair {
+    const xc_tile *a;
+    const xc_tile *b;
+    int32_t        mode;
+    int32_t        Ox, Oy;
+    int32_t        radius;      /* NCC disc radius */
+    int32_t        Rx, Ry;      /* S/R oval semi-axes (<= 0: none) */
+    double         angle_deg;
+    xc_rgn_rule    rgn;
+    xc_cnt_rule    cnt;
+} xc_pair;
+
+/*
+ * Correlate n pairs in FP32 (FFTs, multiply) with FP64 summed-area tables
+ * and FP64 variance terms.  Blocks until results[0..n) are written.
+ */
+int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *results);
+
+/*
+ * Rotation sweep (CThmScan::AngleScan*, CThmScan.cpp:431-621): correlate
+ * A rotated by each of n_angles angles against B in one batched launch, in
+ * S mode with the given legality rules.  results[n_angles].
+ */
+int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
+                        const double *angles_deg, int n_angles,
+                        const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                        int Ox, int Oy, int Rx, int Ry, xc_result *results);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XCORR_H */

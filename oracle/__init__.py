@@ -1,0 +1,270 @@
+"""CPU oracle for the FFT tile-pair cross-correlation path -- TEST INFRASTRUCTURE.
+
+ctypes front end of oracle/liboracle.so (oracle/xc_oracle.c), a plain-C FP64
+restatement of billkarsh/Alignment_Projects' 0_GEN/Correlation.cpp path.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+it, and only as the checker or as the timed CPU baseline.  The product
+(alignment_projects_amd / libxcorr.so) never imports it.
+
+Parity pins (see tests/test_oracle.py): the reference's own harnesses
+(testnormcorr.cpp KAT, time_fft.cpp planted shift, fft.cpp DFT check), a
+direct DFT and numpy.fft for the transforms, and a brute-force (no FFT)
+normalised cross-correlation with the reference's scan order, zone rules and
+parabola quirks for the correlators.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from alignment_projects_amd.abi import XcCntRule, XcPoint, XcResult, XcRgnRule
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_pp = C.POINTER(XcPoint)
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        L.xo_corr_patches.argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(XcRgnRule), C.POINTER(XcCntRule), _dp, _dp, _dp,
+                                      C.POINTER(XcResult), _dp, C.c_int]
+        L.xo_corr_images.argtypes = [C.c_int, _pp, _dp, C.c_int, _pp, _dp, C.c_int,
+                                     C.POINTER(XcRgnRule), C.POINTER(XcCntRule), C.c_double, C.c_double,
+                                     C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp,
+                                     C.POINTER(XcResult), _dp, _dp, C.POINTER(C.c_uint8), C.c_int]
+        L.xo_corr_patch_to_image.argtypes = [_pp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, _dp, _dp, _dp, C.POINTER(XcResult)]
+        L.xo_rfromangle.argtypes = [C.c_double, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_int, C.c_long,
+                                    C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, C.POINTER(XcResult)]
+        L.xo_fft2d_r2c.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.xo_ift2d_c2r.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.xo_fft1d.argtypes = [_dp, C.c_int, C.c_int]
+        L.xo_stats.argtypes = [_dp, C.c_int, _dp, _dp]
+        L.xo_normalize.argtypes = [_dp, C.c_int]
+        L.xo_normalize.restype = C.c_double
+        L.xo_integrate_image.argtypes = [_dp, _dp, _ip, C.c_int, C.c_int, _dp, C.c_int]
+        L.xo_image_from_values_and_points.argtypes = [_dp, C.c_int, C.c_int, _dp, _pp, C.c_int, C.c_int, C.c_int]
+        L.xo_bbox_from_points.argtypes = [_pp, C.c_int, _ip]
+        L.xo_boxes_from_shifts.argtypes = [C.c_int] * 6 + [_ip, _ip]
+        L.xo_parab_peak_fft.argtypes = [_dp, _dp, C.c_int, _dp, C.c_int, C.c_int]
+        L.xo_decimate_vector.argtypes = [_pp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_set_fft_provider.argtypes = [C.c_int, C.c_char_p]
+        L.xo_fft_provider_name.restype = C.c_char_p
+        L.xo_rgn_ok.argtypes = [C.POINTER(XcRgnRule), C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _pts(p):
+    p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1, 2)
+    return p, p.ctypes.data_as(_pp)
+
+
+def _vals(v):
+    v = np.ascontiguousarray(v, dtype=np.float64).ravel()
+    return v, v.ctypes.data_as(_dp)
+
+
+def _ref(x):
+    return C.byref(x) if x is not None else None
+
+
+def corr_patches(ip1, iv1, ip2, iv2, Ox=0, Oy=0, radius=4096, rgn=None, cnt=None,
+                 dx0=0.0, dy0=0.0, want_rslt=False):
+    """CorrPatches (0_GEN/Correlation.cpp:901-1170). Returns (status, r, dx, dy, XcResult[, rslt])."""
+    p1, pp1 = _pts(ip1)
+    v1, pv1 = _vals(iv1)
+    p2, pp2 = _pts(ip2)
+    v2, pv2 = _vals(iv2)
+    dx, dy, r = C.c_double(dx0), C.c_double(dy0), C.c_double(0)
+    res = XcResult()
+    rs = None
+    if want_rslt:
+        b1 = bbox(p1)
+        b2 = bbox(p2)
+        Nx = fft_size(b1[1] - b1[0] + 1, b2[1] - b2[0] + 1)
+        Ny = fft_size(b1[3] - b1[2] + 1, b2[3] - b2[2] + 1)
+        rs = np.zeros(Nx * Ny)
+    st = lib().xo_corr_patches(pp1, pv1, len(v1), pp2, pv2, len(v2), Ox, Oy, radius,
+                               _ref(rgn), _ref(cnt), C.byref(dx), C.byref(dy), C.byref(r), C.byref(res),
+                               rs.ctypes.data_as(_dp) if rs is not None else None,
+                               rs.size if rs is not None else 0)
+    out = (st, r.value, dx.value, dy.value, res)
+    if want_rslt:
+        out = out + (rs.reshape(res.ny, res.nx) if res.nx else rs,)
+    return out
+
+
+def corr_images(mode, ip1, iv1, ip2, iv2, rgn=None, cnt=None, mincor=0.0, nbmaxht=0.0,
+                Ox=0, Oy=0, Rx=-1, Ry=-1, want_images=False):
+    """CorrImagesR/S (0_GEN/Correlation.cpp:3144-3290). Returns (status, r, dx, dy, XcResult[, R, S, A])."""
+    p1, pp1 = _pts(ip1)
+    v1, pv1 = _vals(iv1)
+    p2, pp2 = _pts(ip2)
+    v2, pv2 = _vals(iv2)
+    dx, dy, r = C.c_double(0), C.c_double(0), C.c_double(0)
+    res = XcResult()
+    R = S = A = None
+    cap = 0
+    if want_images:
+        b1, b2 = bbox(p1), bbox(p2)
+        wR = (b1[1] - b1[0]) + (b2[1] - b2[0] + 1)
+        hR = (b1[3] - b1[2]) + (b2[3] - b2[2] + 1)
+        cap = wR * hR
+        R, S, A = np.zeros(cap), np.zeros(cap), np.zeros(cap, np.uint8)
+    st = lib().xo_corr_images(mode, pp1, pv1, len(v1), pp2, pv2, len(v2), _ref(rgn), _ref(cnt),
+                              mincor, nbmaxht, Ox, Oy, Rx, Ry, C.byref(dx), C.byref(dy), C.byref(r),
+                              C.byref(res),
+                              R.ctypes.data_as(_dp) if R is not None else None,
+                              S.ctypes.data_as(_dp) if S is not None else None,
+                              A.ctypes.data_as(C.POINTER(C.c_uint8)) if A is not None else None, cap)
+    out = (st, r.value, dx.value, dy.value, res)
+    if want_images:
+        out = out + (R.reshape(hR, wR), S.reshape(hR, wR), A.reshape(hR, wR))
+    return out
+
+
+def corr_patch_to_image(ip1, iv1, i2, Ox=0, Oy=0, radius=4096, bfilter=False):
+    """CorrPatchToImage (0_GEN/Correlation.cpp:1186-1355). Returns (status, r, dx, dy, XcResult)."""
+    p1, pp1 = _pts(ip1)
+    v1, pv1 = _vals(iv1)
+    i2 = np.ascontiguousarray(i2, dtype=np.float64).ravel()
+    dx, dy, r = C.c_double(0), C.c_double(0), C.c_double(0)
+    res = XcResult()
+    st = lib().xo_corr_patch_to_image(pp1, pv1, len(v1), i2.ctypes.data_as(_dp), i2.size, Ox, Oy, radius,
+                                      int(bool(bfilter)), C.byref(dx), C.byref(dy), C.byref(r), C.byref(res))
+    return st, r.value, dx.value, dy.value, res
+
+
+def rfromangle(deg, ap, av, bp, bv, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """CThmScan::RFromAngle (0_GEN/CThmScan.cpp:329-366). Returns (status, R, X, Y, XcResult)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    X, Y, R = C.c_double(0), C.c_double(0), C.c_double(0)
+    res = XcResult()
+    st = lib().xo_rfromangle(deg, pp1, pv1, len(v1), pp2, pv2, len(v2), olap1d, req_area, Ox, Oy, Rx, Ry,
+                             C.byref(X), C.byref(Y), C.byref(R), C.byref(res))
+    return st, R.value, X.value, Y.value, res
+
+
+def fft2d_r2c(a):
+    """FFT_2D (0_GEN/Correlation_fft_fftw.cpp:38-56): a[Ny][Nx] -> [Ny][Nx/2+1] complex."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    Ny, Nx = a.shape
+    out = np.zeros((Ny, Nx // 2 + 1), np.complex128)
+    lib().xo_fft2d_r2c(a.ctypes.data_as(_dp), Nx, Ny, out.ctypes.data_as(_dp))
+    return out
+
+
+def ift2d_c2r(f, Nx):
+    """IFT_2D (0_GEN/Correlation_fft_fftw.cpp:67-92): unnormalised c2r."""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    Ny = f.shape[0]
+    out = np.zeros((Ny, Nx))
+    lib().xo_ift2d_c2r(f.ctypes.data_as(_dp), Nx, Ny, out.ctypes.data_as(_dp))
+    return out
+
+
+def fft1d(z, inverse=False):
+    z = np.array(z, dtype=np.complex128)
+    lib().xo_fft1d(z.ctypes.data_as(_dp), z.size, int(inverse))
+    return z
+
+
+def stats(v):
+    v, pv = _vals(v)
+    a, s = C.c_double(), C.c_double()
+    lib().xo_stats(pv, v.size, C.byref(a), C.byref(s))
+    return a.value, s.value
+
+
+def normalize(v):
+    v = np.array(v, dtype=np.float64)
+    lib().xo_normalize(v.ctypes.data_as(_dp), v.size)
+    return v
+
+
+def integrate_image(img, wS, hS):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    wI = img.shape[1]
+    S = np.zeros(wS * hS)
+    S2 = np.zeros(wS * hS)
+    nz = np.zeros(wS * hS, np.int32)
+    lib().xo_integrate_image(S.ctypes.data_as(_dp), S2.ctypes.data_as(_dp), nz.ctypes.data_as(_ip),
+                             wS, hS, img.ctypes.data_as(_dp), wI)
+    return S.reshape(hS, wS), S2.reshape(hS, wS), nz.reshape(hS, wS)
+
+
+def image_from_values_and_points(w, h, v, p, xmin, ymin):
+    p, pp = _pts(p)
+    v, pv = _vals(v)
+    out = np.zeros(w * h)
+    lib().xo_image_from_values_and_points(out.ctypes.data_as(_dp), w, h, pv, pp, len(v), xmin, ymin)
+    return out.reshape(h, w)
+
+
+def bbox(p):
+    p, pp = _pts(p)
+    o = (C.c_int * 4)()
+    lib().xo_bbox_from_points(pp, len(p), o)
+    return tuple(o)
+
+
+def boxes_from_shifts(w1, h1, w2, h2, x, y):
+    b1, b2 = (C.c_int * 4)(), (C.c_int * 4)()
+    lib().xo_boxes_from_shifts(w1, h1, w2, h2, x, y, b1, b2)
+    return tuple(b1), tuple(b2)
+
+
+def fft_size(n1, n2):
+    return lib().xo_fft_size(n1, n2)
+
+
+def ceil_pow2(n):
+    return lib().xo_ceil_pow2(n)
+
+
+def decimate_vector(p, v, w, h, lbin):
+    p = np.array(p, dtype=np.float64).reshape(-1, 2)
+    v = np.array(v, dtype=np.float64)
+    k = lib().xo_decimate_vector(p.ctypes.data_as(_pp), v.ctypes.data_as(_dp), len(v), w, h, lbin)
+    return p[:k].copy(), v[:k].copy()
+
+
+def rgn_ok(rule, olw, olh):
+    return bool(lib().xo_rgn_ok(C.byref(rule), olw, olh))
+
+
+MKL_CANDIDATES = ("/opt/conda/lib/libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so", "libfftw3.so.3")
+
+
+def set_fft_provider(which):
+    """0: the oracle's radix-2 FFT.  1: the FFTW3 API from an installed library
+    (libfftw3 or MKL's FFTW3 wrappers), called as the reference calls it."""
+    if which == 0:
+        lib().xo_set_fft_provider(0, None)
+        return provider_name()
+    for path in MKL_CANDIDATES:
+        if lib().xo_set_fft_provider(1, path.encode()) == 0:
+            return provider_name()
+    raise RuntimeError("no library exporting the FFTW3 API was found")
+
+
+def provider_name():
+    return lib().xo_fft_provider_name().decode()

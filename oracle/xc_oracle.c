@@ -1,0 +1,957 @@
+/*
+ * xc_oracle.c -- CPU oracle for the FFT tile-pair cross-correlation path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C and FP64, the
+ * algorithm of billkarsh/Alignment_Projects' correlation path so that the
+ * HIP library can be checked against it.  It is loaded only by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+ * checker or as the timed CPU baseline, never as the thing measured or
+ * shipped.  libxcorr.so does not link it.
+ *
+ * Every function cites the reference file:line it follows.  Compiled with
+ * -ffp-contract=off so that products and sums round exactly as the
+ * reference's x86-64 -O3 build (no FMA) rounds them.
+ *
+ * FFT: the reference calls FFTW3 (or MKL DFTI) -- third-party, absent from
+ * the reference tree.  The oracle carries its own radix-2 FP64 complex FFT
+ * (textbook Cooley-Tukey, exact-angle twiddles), pinned in tests against a
+ * direct DFT and numpy.fft.  For the CPU baseline, the FFTW3 API can be
+ * resolved at run time from an installed library exporting it (MKL's FFTW3
+ * wrappers) and called in the reference's own pattern: a plan per call with
+ * FFTW_ESTIMATE, executed under one mutex (Correlation_fft_fftw.cpp:10-92).
+ */
+#include "../include/xcorr.h"
+
+#include <dlfcn.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double re, im; } cd;
+
+/* GenDefs.h:16 -- the reference's truncated pi, used for deg->rad. */
+#define REF_PI 3.14159265358
+
+/* ------------------------------------------------------------------ */
+/* Complex FFT (oracle's own)                                          */
+/* ------------------------------------------------------------------ */
+
+static pthread_mutex_t tw_mutex = PTHREAD_MUTEX_INITIALIZER;
+static cd *tw_cache[32];
+
+/* e^{-2 pi i k / n}, k < n/2, from long-double exact-angle cos/sin. */
+static const cd *twiddles(int n)
+{
+    int lg = 0;
+    while ((1 << lg) < n) ++lg;
+    pthread_mutex_lock(&tw_mutex);
+    if (!tw_cache[lg]) {
+        int h = n / 2 > 0 ? n / 2 : 1;
+        cd *t = (cd *)malloc(sizeof(cd) * h);
+        for (int k = 0; k < h; ++k) {
+            long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+            t[k].re = (double)cosl(a);
+            t[k].im = (double)-sinl(a);
+        }
+        tw_cache[lg] = t;
+    }
+    const cd *r = tw_cache[lg];
+    pthread_mutex_unlock(&tw_mutex);
+    return r;
+}
+
+/* In-place unnormalised complex DFT of power-of-two length n.
+ * inverse != 0 uses e^{+2 pi i ...}. */
+static void fft_c(cd *a, int n, int inverse)
+{
+    if (n <= 1) return;
+    const cd *tw = twiddles(n);
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { cd t = a[i]; a[i] = a[j]; a[j] = t; }
+    }
+    for (int len = 2; len <= n; len <<= 1) {
+        int half = len >> 1, step = n / len;
+        for (int i = 0; i < n; i += len) {
+            for (int j = 0; j < half; ++j) {
+                cd w = tw[j * step];
+                if (inverse) w.im = -w.im;
+                cd u = a[i + j], v = a[i + j + half];
+                cd t;
+                t.re = v.re * w.re - v.im * w.im;
+                t.im = v.re * w.im + v.im * w.re;
+                a[i + j].re = u.re + t.re;  a[i + j].im = u.im + t.im;
+                a[i + j + half].re = u.re - t.re;  a[i + j + half].im = u.im - t.im;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Optional FFTW3-API provider (CPU baseline only)                     */
+/* ------------------------------------------------------------------ */
+
+typedef void *(*pf_plan_r2c)(int, int, double *, double (*)[2], unsigned);
+typedef void *(*pf_plan_c2r)(int, int, double (*)[2], double *, unsigned);
+typedef void (*pf_exec)(void *);
+typedef void (*pf_destroy)(void *);
+#define XO_FFTW_ESTIMATE (1U << 6)
+
+static struct {
+    int          active;
+    void        *lib;
+    pf_plan_r2c  r2c;
+    pf_plan_c2r  c2r;
+    pf_exec      exec;
+    pf_destroy   destroy;
+    char         name[256];
+} prov;
+
+/* mutex_fft (Correlation.cpp:26-27): serialises every FFTW call. */
+static pthread_mutex_t mutex_fft = PTHREAD_MUTEX_INITIALIZER;
+
+/* Select the FFT used by the 2-D transforms: 0 = oracle radix-2,
+ * 1 = FFTW3 API from `libpath` (dlopen).  Returns 0 on success. */
+int xo_set_fft_provider(int which, const char *libpath)
+{
+    if (which == 0) { prov.active = 0; strcpy(prov.name, "oracle radix-2 FP64"); return 0; }
+    if (!prov.lib) {
+        prov.lib = dlopen(libpath, RTLD_NOW | RTLD_LOCAL);
+        if (!prov.lib) return -1;
+        prov.r2c = (pf_plan_r2c)dlsym(prov.lib, "fftw_plan_dft_r2c_2d");
+        prov.c2r = (pf_plan_c2r)dlsym(prov.lib, "fftw_plan_dft_c2r_2d");
+        prov.exec = (pf_exec)dlsym(prov.lib, "fftw_execute");
+        prov.destroy = (pf_destroy)dlsym(prov.lib, "fftw_destroy_plan");
+        if (!prov.r2c || !prov.c2r || !prov.exec || !prov.destroy) {
+            dlclose(prov.lib); prov.lib = NULL; return -2;
+        }
+    }
+    prov.active = 1;
+    snprintf(prov.name, sizeof prov.name, "FFTW3 API via %s", libpath);
+    return 0;
+}
+
+const char *xo_fft_provider_name(void)
+{
+    if (!prov.name[0]) strcpy(prov.name, "oracle radix-2 FP64");
+    return prov.name;
+}
+
+/* ------------------------------------------------------------------ */
+/* 2-D real transforms: FFT_2D / IFT_2D semantics                      */
+/* ------------------------------------------------------------------ */
+
+/* FFT_2D (Correlation_fft_fftw.cpp:10-56): in[Nslow][Nfast] real ->
+ * out[Nslow][Nfast/2+1] complex, unnormalised. */
+int xo_fft2d_r2c(const double *in, int Nfast, int Nslow, double *out_)
+{
+    int H = Nfast / 2 + 1;
+    cd *out = (cd *)out_;
+    if (prov.active) {
+        pthread_mutex_lock(&mutex_fft);
+        void *p = prov.r2c(Nslow, Nfast, (double *)in, (double (*)[2])out, XO_FFTW_ESTIMATE);
+        prov.exec(p);
+        prov.destroy(p);
+        pthread_mutex_unlock(&mutex_fft);
+        return Nslow * H;
+    }
+    cd *row = (cd *)malloc(sizeof(cd) * (Nfast > Nslow ? Nfast : Nslow));
+    for (int y = 0; y < Nslow; ++y) {
+        const double *s = in + (size_t)y * Nfast;
+        int nz = 0;
+        for (int x = 0; x < Nfast; ++x) { row[x].re = s[x]; row[x].im = 0.0; nz |= (s[x] != 0.0); }
+        if (nz) fft_c(row, Nfast, 0);
+        else memset(row, 0, sizeof(cd) * Nfast);
+        memcpy(out + (size_t)y * H, row, sizeof(cd) * H);
+    }
+    for (int k = 0; k < H; ++k) {
+        for (int y = 0; y < Nslow; ++y) row[y] = out[(size_t)y * H + k];
+        fft_c(row, Nslow, 0);
+        for (int y = 0; y < Nslow; ++y) out[(size_t)y * H + k] = row[y];
+    }
+    free(row);
+    return Nslow * H;
+}
+
+/* IFT_2D (Correlation_fft_fftw.cpp:67-92): unnormalised c2r.  The input
+ * is not modified (the reference copies it before FFTW's c2r). */
+int xo_ift2d_c2r(const double *in_, int Nfast, int Nslow, double *out)
+{
+    int H = Nfast / 2 + 1;
+    const cd *in = (const cd *)in_;
+    if (prov.active) {
+        size_t M = (size_t)Nslow * H;
+        cd *copy = (cd *)malloc(sizeof(cd) * M);
+        memcpy(copy, in, sizeof(cd) * M);
+        pthread_mutex_lock(&mutex_fft);
+        void *p = prov.c2r(Nslow, Nfast, (double (*)[2])copy, out, XO_FFTW_ESTIMATE);
+        prov.exec(p);
+        prov.destroy(p);
+        pthread_mutex_unlock(&mutex_fft);
+        free(copy);
+        return 0;
+    }
+    cd *tmp = (cd *)malloc(sizeof(cd) * (size_t)Nslow * H);
+    cd *row = (cd *)malloc(sizeof(cd) * (Nfast > Nslow ? Nfast : Nslow));
+    for (int k = 0; k < H; ++k) {
+        for (int y = 0; y < Nslow; ++y) row[y] = in[(size_t)y * H + k];
+        fft_c(row, Nslow, 1);
+        for (int y = 0; y < Nslow; ++y) tmp[(size_t)y * H + k] = row[y];
+    }
+    for (int y = 0; y < Nslow; ++y) {
+        const cd *t = tmp + (size_t)y * H;
+        for (int k = 0; k < H; ++k) row[k] = t[k];
+        for (int k = H; k < Nfast; ++k) { row[k].re = t[Nfast - k].re; row[k].im = -t[Nfast - k].im; }
+        fft_c(row, Nfast, 1);
+        for (int x = 0; x < Nfast; ++x) out[(size_t)y * Nfast + x] = row[x].re;
+    }
+    free(row);
+    free(tmp);
+    return 0;
+}
+
+/* 1-D complex transform, exported for the DFT known-answer tests. */
+void xo_fft1d(double *a, int n, int inverse) { fft_c((cd *)a, n, inverse); }
+
+/* ------------------------------------------------------------------ */
+/* Geometry and maths helpers                                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct { int L, R, B, T; } ibox;
+
+/* CeilPow2 (Maths.cpp:97-105). */
+int xo_ceil_pow2(int n)
+{
+    int p = 1;
+    while (p < n) p *= 2;
+    return p;
+}
+
+/* FFTSize (Correlation.cpp:344-347), FFTSizeSQR (:356-359). */
+int xo_fft_size(int n1, int n2) { return xo_ceil_pow2(n1 + n2 - 1); }
+int xo_fft_size_sqr(int w1, int h1, int w2, int h2)
+{
+    int a = w1 + w2, b = h1 + h2;
+    return xo_ceil_pow2((a > b ? a : b) - 1);
+}
+
+/* BBoxFromPoints(IBox&) (Geometry.cpp:78-106), including its
+ * "else if" update order. */
+static void bbox_from_points(ibox *B, const xc_point *p, int n)
+{
+    double xlo, xhi, ylo, yhi;
+    xlo = xhi = p[0].x;
+    ylo = yhi = p[0].y;
+    for (int i = 1; i < n; ++i) {
+        double t;
+        if ((t = p[i].x) < xlo) xlo = t;
+        else if (t > xhi) xhi = t;
+        if ((t = p[i].y) < ylo) ylo = t;
+        else if (t > yhi) yhi = t;
+    }
+    B->L = (int)floor(xlo);
+    B->R = (int)ceil(xhi);
+    B->B = (int)floor(ylo);
+    B->T = (int)ceil(yhi);
+}
+
+void xo_bbox_from_points(const xc_point *p, int n, int *LRBT)
+{
+    ibox b;
+    bbox_from_points(&b, p, n);
+    LRBT[0] = b.L; LRBT[1] = b.R; LRBT[2] = b.B; LRBT[3] = b.T;
+}
+
+/* BoxesFromShifts (Geometry.cpp:150-176). */
+static void boxes_from_shifts(ibox *B1, ibox *B2, int w1, int h1, int w2, int h2, int x, int y)
+{
+    B2->L = x > 0 ? x : 0;
+    B2->R = (w2 - 1) < (x + w1 - 1) ? (w2 - 1) : (x + w1 - 1);
+    B2->B = y > 0 ? y : 0;
+    B2->T = (h2 - 1) < (y + h1 - 1) ? (h2 - 1) : (y + h1 - 1);
+    if (B2->R < B2->L) B2->R = B2->L;
+    if (B2->T < B2->B) B2->T = B2->B;
+    B1->L = B2->L - x;
+    B1->R = B2->R - x;
+    B1->B = B2->B - y;
+    B1->T = B2->T - y;
+}
+
+void xo_boxes_from_shifts(int w1, int h1, int w2, int h2, int x, int y, int *b1, int *b2)
+{
+    ibox B1, B2;
+    boxes_from_shifts(&B1, &B2, w1, h1, w2, h2, x, y);
+    b1[0] = B1.L; b1[1] = B1.R; b1[2] = B1.B; b1[3] = B1.T;
+    b2[0] = B2.L; b2[1] = B2.R; b2[2] = B2.B; b2[3] = B2.T;
+}
+
+/* DistributePixel (Maths.h:236-276): bilinear splat, in the reference's
+ * order of the four += updates. */
+static void distribute_pixel(double x, double y, double val, double *image, int w, int h)
+{
+    int    ix = (int)floor(x);
+    int    iy = (int)floor(y);
+    long   i = ix + (long)w * iy;
+    double alpha = x - ix;
+    double beta = y - iy;
+
+    if (ix >= 0 && ix < w) {
+        if (iy >= 0 && iy < h) image[i] += (1.0 - alpha) * (1.0 - beta) * val;
+        ++iy;
+        if (iy >= 0 && iy < h) image[i + w] += (1.0 - alpha) * beta * val;
+        --iy;
+    }
+    ++ix;
+    ++i;
+    if (ix >= 0 && ix < w) {
+        if (iy >= 0 && iy < h) image[i] += alpha * (1.0 - beta) * val;
+        ++iy;
+        if (iy >= 0 && iy < h) image[i + w] += alpha * beta * val;
+    }
+}
+
+/* ImageFromValuesAndPoints (Geometry.cpp:661-685): zero-filled w*h raster,
+ * each point splatted at (x - xmin, y - ymin). */
+void xo_image_from_values_and_points(double *I, int w, int h, const double *v,
+                                     const xc_point *p, int n, int xmin, int ymin)
+{
+    memset(I, 0, sizeof(double) * (size_t)w * h);
+    for (int i = 0; i < n; ++i)
+        distribute_pixel(p[i].x - xmin, p[i].y - ymin, v[i], I, w, h);
+}
+
+/* MeanStd::Stats (Maths.cpp:111-125, n-1 form) over v[0..n). */
+void xo_stats(const double *v, int n, double *avg, double *std)
+{
+    double sum = 0.0, sum2 = 0.0;
+    for (int i = 0; i < n; ++i) { sum += v[i]; sum2 += v[i] * v[i]; }
+    *avg = sum;
+    *std = 0.0;
+    if (n > 1) {
+        *avg /= n;
+        *std = sqrt(fmax(sum2 - n * (*avg) * (*avg), 0.0) / (n - 1.0));
+    }
+}
+
+/* Normalize (Maths.cpp:192-212): mean 0, sd 1; returns the sd. */
+double xo_normalize(double *v, int n)
+{
+    double avg, std;
+    xo_stats(v, n, &avg, &std);
+    for (int i = 0; i < n; ++i) v[i] = (v[i] - avg) / std;
+    return std;
+}
+
+/* IntegrateImage (Correlation.cpp:49-105): inclusive prefix sums of the
+ * wS x hS region of I (row stride wI): S, S^2 and the non-zero count. */
+void xo_integrate_image(double *S, double *S2, int *nz, int wS, int hS, const double *I, int wI)
+{
+    double t;
+    S[0] = (t = I[0]);
+    S2[0] = t * t;
+    nz[0] = (t != 0.0);
+    for (int x = 1; x < wS; ++x) {
+        S[x] = S[x - 1] + (t = I[x]);
+        S2[x] = S2[x - 1] + t * t;
+        nz[x] = nz[x - 1] + (t != 0.0);
+    }
+    for (int y = 1; y < hS; ++y) {
+        long j = (long)wS * y, k = (long)wI * y;
+        S[j] = S[j - wS] + (t = I[k]);
+        S2[j] = S2[j - wS] + t * t;
+        nz[j] = nz[j - wS] + (t != 0.0);
+    }
+    for (int y = 1; y < hS; ++y) {
+        long i = (long)wS * y, k = (long)wI * y;
+        for (int x = 1; x < wS; ++x) {
+            long j = i + x, m = k + x;
+            S[j] = S[j - 1] + S[j - wS] - S[j - wS - 1] + (t = I[m]);
+            S2[j] = S2[j - 1] + S2[j - wS] - S2[j - wS - 1] + t * t;
+            nz[j] = nz[j - 1] + nz[j - wS] - nz[j - wS - 1] + (t != 0.0);
+        }
+    }
+}
+
+/* IntegralTable (Correlation.cpp:160-201). */
+static double table_d(const double *t, int wS, const ibox *B)
+{
+    double r = t[B->R + (long)wS * B->T];
+    if (B->L > 0) r -= t[B->L - 1 + (long)wS * B->T];
+    if (B->B > 0) r -= t[B->R + (long)wS * (B->B - 1)];
+    if (B->L > 0 && B->B > 0) r += t[(B->L - 1) + (long)wS * (B->B - 1)];
+    return r;
+}
+
+static int table_i(const int *t, int wS, const ibox *B)
+{
+    int r = t[B->R + (long)wS * B->T];
+    if (B->L > 0) r -= t[B->L - 1 + (long)wS * B->T];
+    if (B->B > 0) r -= t[B->R + (long)wS * (B->B - 1)];
+    if (B->L > 0 && B->B > 0) r += t[(B->L - 1) + (long)wS * (B->B - 1)];
+    return r;
+}
+
+/* Legality rules: the descriptor form of the EvalType callbacks. */
+static int rgn_ok(const xc_rgn_rule *g, int olw, int olh)
+{
+    if (!g || !g->enabled) return 1;
+    if (olw < g->min_w || olh < g->min_h) return 0;
+    if (!((int64_t)olw * olh > g->min_area)) return 0;
+    if (g->exact_w > 0 && olw != g->exact_w) return 0;
+    if (g->exact_h > 0 && olh != g->exact_h) return 0;
+    if ((g->or_w_ge > 0 || g->or_h_gt > 0) && !(olw >= g->or_w_ge || olh > g->or_h_gt)) return 0;
+    if (g->table) {
+        if (olw > g->table_w || olh > g->table_h) return 0;
+        if (!g->table[(long)(olh - 1) * g->table_w + (olw - 1)]) return 0;
+    }
+    return 1;
+}
+
+static int cnt_ok(const xc_cnt_rule *c, int c1, int c2)
+{
+    if (!c || !c->enabled) return 1;
+    return (int64_t)c1 > c->k1 && (int64_t)c2 > c->k2;
+}
+
+int xo_rgn_ok(const xc_rgn_rule *g, int olw, int olh) { return rgn_ok(g, olw, olh); }
+
+/* ------------------------------------------------------------------ */
+/* Shared FFT correlation core                                         */
+/* ------------------------------------------------------------------ */
+
+/* rslt = IFT_2D(FFT_2D(i2) * conj(FFT_2D(i1))) (Correlation.cpp:965-989).
+ * If whitened != NULL it also receives IFT_2D of the cross power divided
+ * by sqrt(|.|) where |.| > 1e-10 (Correlation.cpp:2311-2319). */
+static void cross_power_inverse(const double *i1, const double *i2, int Nx, int Ny,
+                                double *rslt, double *whitened)
+{
+    int  H = Nx / 2 + 1;
+    long M = (long)Ny * H;
+    cd *f1 = (cd *)malloc(sizeof(cd) * M);
+    cd *f2 = (cd *)malloc(sizeof(cd) * M);
+    xo_fft2d_r2c(i2, Nx, Ny, (double *)f2);
+    xo_fft2d_r2c(i1, Nx, Ny, (double *)f1);
+    for (long i = 0; i < M; ++i) {
+        /* fft1[i] = fft2[i] * conj(fft1[i]) */
+        double a = f2[i].re, b = f2[i].im, c = f1[i].re, d = f1[i].im;
+        f1[i].re = a * c + b * d;
+        f1[i].im = b * c - a * d;
+    }
+    if (rslt) xo_ift2d_c2r((const double *)f1, Nx, Ny, rslt);
+    if (whitened) {
+        for (long i = 0; i < M; ++i) {
+            double mag = hypot(f1[i].re, f1[i].im);
+            if (mag > 1e-10) {
+                double s = sqrt(mag);
+                f1[i].re /= s;
+                f1[i].im /= s;
+            }
+        }
+        xo_ift2d_c2r((const double *)f1, Nx, Ny, whitened);
+    }
+    free(f1);
+    free(f2);
+}
+
+/* LkFFT (Correlation.cpp:287-298). */
+static double lk_fft(const double *fr, int nx, int ny, int x, int y)
+{
+    if (x < 0) x += nx;
+    if (y < 0) y += ny;
+    return fr[x + (long)nx * y];
+}
+
+/* ParabPeakFFT (Correlation.cpp:482-517): the refinement is applied only
+ * for signed lags in [d, n-d) and non-zero neighbours. */
+static void parab_peak_fft(double *xpk, double *ypk, int d, const double *I, int nx, int ny)
+{
+    int    ix = (int)*xpk, iy = (int)*ypk;
+    double z1 = lk_fft(I, nx, ny, ix, iy), z0, z2;
+
+    if (ix - d >= 0 && ix + d < nx &&
+        (z0 = lk_fft(I, nx, ny, ix - d, iy)) != 0.0 &&
+        (z2 = lk_fft(I, nx, ny, ix + d, iy)) != 0.0) {
+        z0 = d * (z0 - z2) / (2 * (z0 + z2 - z1 - z1));
+        if (fabs(z0) <= 8 * d) *xpk += z0;
+    }
+    if (iy - d >= 0 && iy + d < ny &&
+        (z0 = lk_fft(I, nx, ny, ix, iy - d)) != 0.0 &&
+        (z2 = lk_fft(I, nx, ny, ix, iy + d)) != 0.0) {
+        z0 = d * (z0 - z2) / (2 * (z0 + z2 - z1 - z1));
+        if (fabs(z0) <= 8 * d) *ypk += z0;
+    }
+}
+
+void xo_parab_peak_fft(double *xpk, double *ypk, int d, const double *I, int nx, int ny)
+{
+    parab_peak_fft(xpk, ypk, d, I, nx, ny);
+}
+
+static void fill_result(xc_result *res, int status, double r, double dx, double dy, double peak,
+                        int lx, int ly, double sx, double sy, int nx, int ny)
+{
+    if (!res) return;
+    memset(res, 0, sizeof *res);
+    res->status = status;
+    res->r = r;
+    res->dx = dx;
+    res->dy = dy;
+    res->peak = peak;
+    res->lag_x = lx;
+    res->lag_y = ly;
+    res->sub_x = sx;
+    res->sub_y = sy;
+    res->nx = nx;
+    res->ny = ny;
+}
+
+/* ------------------------------------------------------------------ */
+/* CorrPatches                                                         */
+/* ------------------------------------------------------------------ */
+
+/* CorrPatches (Correlation.cpp:901-1170) with CLinCorr (:592-760).
+ * rslt_out (optional, Nx*Ny doubles) receives the scanned R image: NCC at
+ * legal lags, 0 elsewhere, in wrapped raster order. */
+int xo_corr_patches(const xc_point *ip1, const double *iv1, int n1,
+                    const xc_point *ip2, const double *iv2, int n2,
+                    int Ox, int Oy, int radius,
+                    const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                    double *dx, double *dy, double *r_out, xc_result *res,
+                    double *rslt_out, int rslt_cap)
+{
+    if (n1 <= 0 || n2 <= 0) return XC_E_ARG;
+    ibox B1, B2;
+    bbox_from_points(&B1, ip1, n1);
+    bbox_from_points(&B2, ip2, n2);
+    int w1 = B1.R - B1.L + 1, h1 = B1.T - B1.B + 1;
+    int w2 = B2.R - B2.L + 1, h2 = B2.T - B2.B + 1;
+    int Nx = xo_fft_size(w1, w2), Ny = xo_fft_size(h1, h2);
+    long Nxy = (long)Nx * Ny;
+
+    double *i1 = (double *)malloc(sizeof(double) * Nxy);
+    double *i2 = (double *)malloc(sizeof(double) * Nxy);
+    double *rslt = (double *)malloc(sizeof(double) * Nxy);
+    xo_image_from_values_and_points(i1, Nx, Ny, iv1, ip1, n1, B1.L, B1.B);
+    xo_image_from_values_and_points(i2, Nx, Ny, iv2, ip2, n2, B2.L, B2.B);
+    cross_power_inverse(i1, i2, Nx, Ny, rslt, NULL);
+
+    /* CLinCorr::Initialize (:639-662) */
+    double *s1 = (double *)malloc(sizeof(double) * (long)w1 * h1);
+    double *q1 = (double *)malloc(sizeof(double) * (long)w1 * h1);
+    int    *z1 = (int *)malloc(sizeof(int) * (long)w1 * h1);
+    double *s2 = (double *)malloc(sizeof(double) * (long)w2 * h2);
+    double *q2 = (double *)malloc(sizeof(double) * (long)w2 * h2);
+    int    *z2 = (int *)malloc(sizeof(int) * (long)w2 * h2);
+    xo_integrate_image(s1, q1, z1, w1, h1, i1, Nx);
+    xo_integrate_image(s2, q2, z2, w2, h2, i2, Nx);
+
+    double biggest = -1.0E30;
+    int bigx = -1, bigy = -1;
+    int nnegx = w1 - 1, nnegy = h1 - 1, nposx = w2, nposy = h2;
+    int rsqr = radius * radius;
+    int status = XC_OK;
+
+    for (int iy = 0; iy < Ny && status == XC_OK; ++iy) {
+        int dyy, y = iy;
+        if (y >= Ny - nnegy) y -= Ny;
+        else if (y >= nposy) {
+            memset(&rslt[(long)Nx * iy], 0, Nx * sizeof(double));
+            continue;
+        }
+        dyy = y - Oy;
+        for (int ix = 0; ix < Nx; ++ix) {
+            int ddx, x = ix;
+            long i = ix + (long)Nx * iy;
+            if (x >= Nx - nnegx) x -= Nx;
+            else if (x >= nposx) { rslt[i] = 0.0; continue; }
+            ddx = x - Ox;
+            if (ddx * ddx + dyy * dyy > rsqr) { rslt[i] = 0.0; continue; }
+
+            /* CheckSize (:665-694) */
+            ibox OL1, OL2;
+            boxes_from_shifts(&OL1, &OL2, w1, h1, w2, h2, x, y);
+            int olw = OL1.R - OL1.L + 1, olh = OL1.T - OL1.B + 1;
+            if (!rgn_ok(rgn, olw, olh)) { rslt[i] = 0.0; continue; }
+            /* CheckDensity (:697-713) */
+            int i1c = table_i(z1, w1, &OL1), i2c = table_i(z2, w2, &OL2);
+            if (!cnt_ok(cnt, i1c, i2c)) { rslt[i] = 0.0; continue; }
+            /* GetCorr (:716-754) */
+            double n = (double)(olw * olh);
+            double im1sum = table_d(s1, w1, &OL1), im1sum2 = table_d(q1, w1, &OL1);
+            double im2sum = table_d(s2, w2, &OL2), im2sum2 = table_d(q2, w2, &OL2);
+            double num = n * rslt[i] / Nxy - im1sum * im2sum;
+            double d1 = n * im1sum2 - im1sum * im1sum;
+            double d2 = n * im2sum2 - im2sum * im2sum;
+            double d = d1 * d2;
+            double r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
+            if (fabs(r) > 1.0001) { status = XC_E_ODD_R; break; }
+            rslt[i] = r;
+            if (r > biggest) { biggest = r; bigx = x; bigy = y; }
+        }
+    }
+
+    double rv = 0.0;
+    if (status == XC_OK) {
+        if (biggest < -2.0) {
+            status = XC_NO_PEAK;
+            fill_result(res, status, 0.0, dx ? *dx : 0.0, dy ? *dy : 0.0, 0.0, 0, 0, 0, 0, Nx, Ny);
+        } else {
+            double px = bigx, py = bigy;
+            parab_peak_fft(&px, &py, 1, rslt, Nx, Ny);
+            double ox = px + (B2.L - B1.L), oy = py + (B2.B - B1.B);
+            if (dx) *dx = ox;
+            if (dy) *dy = oy;
+            rv = biggest;
+            fill_result(res, status, biggest, ox, oy, biggest, bigx, bigy, px - bigx, py - bigy, Nx, Ny);
+        }
+    } else {
+        fill_result(res, status, 0.0, 0, 0, 0, 0, 0, 0, 0, Nx, Ny);
+    }
+    if (r_out) *r_out = rv;
+    if (rslt_out && rslt_cap >= Nxy) memcpy(rslt_out, rslt, sizeof(double) * Nxy);
+
+    free(i1); free(i2); free(rslt);
+    free(s1); free(q1); free(z1); free(s2); free(q2); free(z2);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* CorrPatchToImage                                                    */
+/* ------------------------------------------------------------------ */
+
+/* CorrPatchToImage (Correlation.cpp:1186-1355). */
+int xo_corr_patch_to_image(const xc_point *ip1, const double *iv1, int np1,
+                           const double *i2, int Nsqr,
+                           int Ox, int Oy, int radius, int bFilter,
+                           double *dx, double *dy, double *r_out, xc_result *res)
+{
+    if (np1 <= 0 || Nsqr <= 0) return XC_E_ARG;
+    int N = (int)sqrt((double)Nsqr);
+    double norm = (double)np1 * Nsqr;
+    ibox B1;
+    bbox_from_points(&B1, ip1, np1);
+    int w1 = B1.R - B1.L + 1, h1 = B1.T - B1.B + 1;
+
+    int w2 = 0, h2 = 0;
+    for (int i = 0; i < Nsqr; ++i) {
+        if (i2[i] != 0.0) {
+            int y = i / N, x = i - N * y;
+            if (y > h2) h2 = y;
+            if (x > w2) w2 = x;
+        }
+    }
+    ++w2;
+    ++h2;
+
+    double *i1 = (double *)calloc((size_t)Nsqr, sizeof(double));
+    for (int i = 0; i < np1; ++i) {
+        int x = (int)ip1[i].x, y = (int)ip1[i].y;
+        i1[x + N * y] = iv1[i];
+    }
+
+    int  H = N / 2 + 1;
+    long M = (long)N * H;
+    cd *f1 = (cd *)malloc(sizeof(cd) * M);
+    cd *f2 = (cd *)malloc(sizeof(cd) * M);
+    double *rslt = (double *)malloc(sizeof(double) * Nsqr);
+    xo_fft2d_r2c(i2, N, N, (double *)f2);
+    xo_fft2d_r2c(i1, N, N, (double *)f1);
+    for (long i = 0; i < M; ++i) {
+        double a = f2[i].re, b = f2[i].im, c = f1[i].re, d = f1[i].im;
+        double pr = a * c + b * d, pi = b * c - a * d;
+        if (bFilter) {
+            /* :1255-1283 -- note the reference's x = i / yrow (row index). */
+            int yrow = N / 2 + 1;
+            int x = (int)(i / yrow);
+            int y = (int)(i - (long)yrow * x);
+            if (x >= N / 2) x -= N;
+            double rad2 = ((double)x * x + (double)y * y) / 10000.0;
+            double filt = rad2 > 10.0 ? 0.0 : exp(-rad2);
+            pr = pr * filt;
+            pi = pi * filt;
+        }
+        f1[i].re = pr;
+        f1[i].im = pi;
+    }
+    xo_ift2d_c2r((const double *)f1, N, N, rslt);
+
+    double biggest = -1.0E30;
+    int bigx = -1, bigy = -1;
+    int rsqr = radius * radius;
+    for (int i = 0; i < Nsqr; ++i) {
+        int y = i / N, x = i - N * y;
+        if (y > N - h1) y -= N;
+        else if (y >= h2) continue;
+        if (x > N - w1) x -= N;
+        else if (x >= w2) continue;
+        int ddx = x - Ox, ddy = y - Oy;
+        if (ddx * ddx + ddy * ddy > rsqr) continue;
+        if (rslt[i] > biggest) { biggest = rslt[i]; bigx = x; bigy = y; }
+    }
+    biggest /= norm;
+
+    int status = XC_OK;
+    if (biggest > 1.0001) {
+        status = XC_E_ODD_R;
+        fill_result(res, status, biggest, 0, 0, biggest, bigx, bigy, 0, 0, N, N);
+    } else {
+        double px = bigx, py = bigy;
+        parab_peak_fft(&px, &py, 1, rslt, N, N);
+        if (dx) *dx = px;
+        if (dy) *dy = py;
+        if (r_out) *r_out = biggest;
+        fill_result(res, status, biggest, px, py, biggest, bigx, bigy, px - bigx, py - bigy, N, N);
+    }
+    free(i1); free(f1); free(f2); free(rslt);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* CorrImagesR / CorrImagesS                                           */
+/* ------------------------------------------------------------------ */
+
+/* ParabPeak (Correlation.cpp:2975-3010): neighbours must be > 0. */
+static void parab_peak(double *xpk, double *ypk, int d, const double *I, int w, int h)
+{
+    int    ix = (int)*xpk, iy = (int)*ypk;
+    double z1 = I[ix + (long)w * iy], z0, z2;
+    if (ix - d >= 0 && ix + d < w &&
+        (z0 = I[ix - d + (long)w * iy]) > 0.0 &&
+        (z2 = I[ix + d + (long)w * iy]) > 0.0) {
+        z0 = d * (z0 - z2) / (2 * (z0 + z2 - z1 - z1));
+        if (fabs(z0) <= 8 * d) *xpk += z0;
+    }
+    if (iy - d >= 0 && iy + d < h &&
+        (z0 = I[ix + (long)w * (iy - d)]) > 0.0 &&
+        (z2 = I[ix + (long)w * (iy + d)]) > 0.0) {
+        z0 = d * (z0 - z2) / (2 * (z0 + z2 - z1 - z1));
+        if (fabs(z0) <= 8 * d) *ypk += z0;
+    }
+}
+
+/* CCorImg::SetDims / MakeRandA / MakeSandRandA / MaskA / OrderF /
+ * SimpleMax / ReturnR and CorrImagesR / CorrImagesS
+ * (Correlation.cpp:1949-2370, 2509-2569, 2939-3043, 3144-3290).
+ * mode: XC_MODE_R or XC_MODE_S.  Optional outputs (each nR = wR*hR):
+ * R_out, S_out (S mode), A_out.  On exact ties in OrderF's sort the oracle
+ * takes the lowest centred-image index (std::sort's order on exact ties is
+ * unspecified). */
+int xo_corr_images(int mode,
+                   const xc_point *ip1, const double *iv1, int n1,
+                   const xc_point *ip2, const double *iv2, int n2,
+                   const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                   double mincor, double nbmaxht,
+                   int Ox, int Oy, int Rx, int Ry,
+                   double *dx, double *dy, double *r_out, xc_result *res,
+                   double *R_out, double *S_out, uint8_t *A_out, int cap)
+{
+    (void)nbmaxht;
+    if (n1 <= 0 || n2 <= 0) return XC_E_ARG;
+    if (mode != XC_MODE_R && mode != XC_MODE_S) return XC_E_ARG;
+    ibox B1, B2;
+    bbox_from_points(&B1, ip1, n1);
+    bbox_from_points(&B2, ip2, n2);
+    int w1 = B1.R - B1.L + 1, h1 = B1.T - B1.B + 1;
+    int w2 = B2.R - B2.L + 1, h2 = B2.T - B2.B + 1;
+    int nnegx = w1 - 1, nposx = w2, nnegy = h1 - 1, nposy = h2;
+    int cx = nnegx, cy = nnegy, wR = nnegx + nposx, hR = nnegy + nposy;
+    long nR = (long)wR * hR;
+    int Nx = xo_fft_size(w1, w2), Ny = xo_fft_size(h1, h2);
+    long Nxy = (long)Nx * Ny;
+    const int mindelta = 11 / 2 + 1;
+
+    if (nnegx < mindelta || nposx < mindelta || nnegy < mindelta || nposy < mindelta) {
+        if (dx) *dx = 0.0;
+        if (dy) *dy = 0.0;
+        if (r_out) *r_out = 0.0;
+        fill_result(res, XC_TOO_SMALL, 0, 0, 0, 0, 0, 0, 0, 0, Nx, Ny);
+        return XC_TOO_SMALL;
+    }
+
+    double *i1 = (double *)malloc(sizeof(double) * Nxy);
+    double *i2 = (double *)malloc(sizeof(double) * Nxy);
+    double *rslt = (double *)malloc(sizeof(double) * Nxy);
+    double *wrs = mode == XC_MODE_S ? (double *)malloc(sizeof(double) * Nxy) : NULL;
+    xo_image_from_values_and_points(i1, Nx, Ny, iv1, ip1, n1, B1.L, B1.B);
+    xo_image_from_values_and_points(i2, Nx, Ny, iv2, ip2, n2, B2.L, B2.B);
+    cross_power_inverse(i1, i2, Nx, Ny, rslt, wrs);
+
+    /* RCalc::Initialize (:1771-1791) */
+    double *s1 = (double *)malloc(sizeof(double) * (long)w1 * h1);
+    double *q1 = (double *)malloc(sizeof(double) * (long)w1 * h1);
+    int    *z1 = (int *)malloc(sizeof(int) * (long)w1 * h1);
+    double *s2 = (double *)malloc(sizeof(double) * (long)w2 * h2);
+    double *q2 = (double *)malloc(sizeof(double) * (long)w2 * h2);
+    int    *z2 = (int *)malloc(sizeof(int) * (long)w2 * h2);
+    xo_integrate_image(s1, q1, z1, w1, h1, i1, Nx);
+    xo_integrate_image(s2, q2, z2, w2, h2, i2, Nx);
+
+    double  *R = (double *)malloc(sizeof(double) * nR);
+    double  *S = mode == XC_MODE_S ? (double *)malloc(sizeof(double) * nR) : NULL;
+    uint8_t *A = (uint8_t *)malloc(nR);
+
+    for (int y = -nnegy; y < nposy; ++y) {
+        long iy = (long)Nx * (y >= 0 ? y : Ny + y);
+        for (int x = -nnegx; x < nposx; ++x) {
+            long ix = (x >= 0 ? x : Nx + x);
+            long ir = cx + x + (long)wR * (cy + y);
+            /* RCalc::Valid (:1794-1826) */
+            ibox OL1, OL2;
+            boxes_from_shifts(&OL1, &OL2, w1, h1, w2, h2, x, y);
+            int olw = OL1.R - OL1.L + 1, olh = OL1.T - OL1.B + 1;
+            int ok = rgn_ok(rgn, olw, olh);
+            if (cnt && cnt->enabled) {
+                int i1c = table_i(z1, w1, &OL1), i2c = table_i(z2, w2, &OL2);
+                if (!cnt_ok(cnt, i1c, i2c)) ok = 0;
+            }
+            A[ir] = (uint8_t)ok;
+            /* RCalc::CalcR (:1829-1845) */
+            double n = (double)(olw * olh);
+            double im1sum = table_d(s1, w1, &OL1), im1sum2 = table_d(q1, w1, &OL1);
+            double im2sum = table_d(s2, w2, &OL2), im2sum2 = table_d(q2, w2, &OL2);
+            double num = n * rslt[ix + iy] / Nxy - im1sum * im2sum;
+            double d1 = n * im1sum2 - im1sum * im1sum;
+            double d2 = n * im2sum2 - im2sum * im2sum;
+            double d = d1 * d2;
+            double r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
+            R[ir] = (r > -1.0 && r < 1.0 ? r : 0.0);
+            /* CalcS (:1766-1767) */
+            if (S) S[ir] = wrs[ix + iy] / Nxy;
+        }
+    }
+
+    /* MaskA (:2011-2054) */
+    if (Rx > 0 && Ry > 0) {
+        int ox = Ox + B1.L - B2.L + cx, oy = Oy + B1.B - B2.B + cy;
+        int xmin = ox - Rx, xmax = ox + Rx, ymin = oy - Ry, ymax = oy + Ry;
+        double rx2 = Rx * Rx, asq = rx2 / ((double)Ry * Ry);
+        for (long i = 0; i < nR; ++i) {
+            int y = (int)(i / wR);
+            if (y < ymin || y > ymax) { A[i] = 0; continue; }
+            int x = (int)(i - (long)wR * y);
+            if (x < xmin || x > xmax) { A[i] = 0; continue; }
+            x -= ox;
+            y -= oy;
+            if (x * x + asq * y * y > rx2) A[i] = 0;
+        }
+    }
+
+    /* OrderF (:2509-2569) + SimpleMax (:2939-2948) */
+    const double *F = (mode == XC_MODE_S) ? S : R;
+    long best = -1;
+    for (long i = 0; i < nR; ++i) {
+        if (!A[i]) continue;
+        if (mincor > 0.0 && !(R[i] > mincor)) continue;
+        if (best < 0 || F[i] > F[best]) best = i;
+    }
+
+    int status = XC_OK;
+    if (best < 0) {
+        status = XC_NO_PEAK;
+        if (dx) *dx = 0.0;
+        if (dy) *dy = 0.0;
+        if (r_out) *r_out = 0.0;
+        fill_result(res, status, 0, 0, 0, 0, 0, 0, 0, 0, Nx, Ny);
+    } else {
+        int ry = (int)(best / wR), rx = (int)(best - (long)wR * ry);
+        /* ReturnR (:3016-3043) */
+        double bigR = F[best];
+        double px = rx, py = ry;
+        parab_peak(&px, &py, 1, F, wR, hR);
+        double ox = px + (B2.L - B1.L - cx), oy = py + (B2.B - B1.B - cy);
+        double rv = bigR;
+        if (mode == XC_MODE_S) {
+            /* Stats(S) then 10 (r - mn) / (sd sqrt(n)) (:3285-3289) */
+            double mn, sd;
+            xo_stats(S, (int)nR, &mn, &sd);
+            rv = 10.0 * (bigR - mn) / (sd * sqrt((double)nR));
+        }
+        if (dx) *dx = ox;
+        if (dy) *dy = oy;
+        if (r_out) *r_out = rv;
+        fill_result(res, status, rv, ox, oy, bigR, rx - cx, ry - cy, px - rx, py - ry, Nx, Ny);
+    }
+    if (cap >= nR) {
+        if (R_out) memcpy(R_out, R, sizeof(double) * nR);
+        if (S_out && S) memcpy(S_out, S, sizeof(double) * nR);
+        if (A_out) memcpy(A_out, A, nR);
+    }
+    free(i1); free(i2); free(rslt); free(wrs);
+    free(s1); free(q1); free(z1); free(s2); free(q2); free(z2);
+    free(R); free(S); free(A);
+    return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* CThmScan::RFromAngle                                                */
+/* ------------------------------------------------------------------ */
+
+/* RFromAngle (CThmScan.cpp:329-366) with identity Tdfm and Tptwk and no
+ * pretweak list: rotate A's points by R(deg*PI/180) (TAffine::NUSetRot,
+ * TAffine.cpp:116-122; Apply_R_Part :407-416), then CorrImagesS with
+ * BigEnough(olap1D, reqArea) and EnoughPoints(reqArea), mincor 0. */
+int xo_rfromangle(double deg, const xc_point *ap, const double *av, int na,
+                  const xc_point *bp, const double *bv, int nb,
+                  int olap1D, long reqArea, int Ox, int Oy, int Rx, int Ry,
+                  double *X, double *Y, double *Rv, xc_result *res)
+{
+    double c = cos(deg * REF_PI / 180.0), s = sin(deg * REF_PI / 180.0);
+    double t0 = c, t1 = -s, t3 = s, t4 = c;
+    xc_point *pts = (xc_point *)malloc(sizeof(xc_point) * na);
+    for (int i = 0; i < na; ++i) {
+        double x = ap[i].x * t0 + ap[i].y * t1;
+        double y = ap[i].x * t3 + ap[i].y * t4;
+        pts[i].x = x;
+        pts[i].y = y;
+    }
+    xc_rgn_rule g;
+    memset(&g, 0, sizeof g);
+    g.enabled = 1;
+    g.min_w = g.min_h = olap1D;
+    g.min_area = reqArea;
+    xc_cnt_rule k;
+    memset(&k, 0, sizeof k);
+    k.enabled = 1;
+    k.k1 = k.k2 = reqArea;
+    int st = xo_corr_images(XC_MODE_S, pts, av, na, bp, bv, nb, &g, &k, 0.0, 0.0,
+                            Ox, Oy, Rx, Ry, X, Y, Rv, res, NULL, NULL, NULL, 0);
+    free(pts);
+    return st;
+}
+
+/* DecimateVector (Maths.cpp:1315-1364): bin points (corner at 0,0) into
+ * lbin x lbin cells and average.  p/v are updated in place; returns the
+ * new count. */
+int xo_decimate_vector(xc_point *p, double *v, int np, int w, int h, int lbin)
+{
+    w = (w + lbin - 1) / lbin;
+    h = (h + lbin - 1) / lbin;
+    long nr = (long)w * h;
+    double *vr = (double *)calloc(nr, sizeof(double));
+    uint8_t *cr = (uint8_t *)calloc(nr, 1);
+    for (int ip = 0; ip < np; ++ip) {
+        int x = (int)(p[ip].x) / lbin, y = (int)(p[ip].y) / lbin;
+        long ir = x + (long)w * y;
+        vr[ir] += v[ip];
+        ++cr[ir];
+    }
+    int k = 0;
+    for (long ir = 0; ir < nr; ++ir) {
+        int cnt = cr[ir];
+        if (cnt) {
+            int y = (int)(ir / w), x = (int)(ir - (long)w * y);
+            v[k] = vr[ir] / cnt;
+            p[k].x = x;
+            p[k].y = y;
+            ++k;
+        }
+    }
+    free(vr);
+    free(cr);
+    return k;
+}

@@ -1,0 +1,89 @@
+"""Plain FP64 form on the GPU against the CPU oracle (CorrPatches / CorrImagesR / CorrImagesS).
+
+Bar: integer peak lag and status identical, r within 1e-9 relative, dx/dy within 1e-9 px.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from alignment_projects_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-9
+
+
+def _same(g, o, what):
+    gst, gr, gdx, gdy, gres = g
+    ost, orr, odx, ody, ores = o
+    assert gst == ost, f"{what}: status {gst} vs {ost}"
+    assert (gres.lag_x, gres.lag_y) == (ores.lag_x, ores.lag_y), f"{what}: lag {(gres.lag_x, gres.lag_y)} vs {(ores.lag_x, ores.lag_y)}"
+    assert (gres.nx, gres.ny) == (ores.nx, ores.ny)
+    assert abs(gr - orr) <= RTOL * max(1.0, abs(orr)), f"{what}: r {gr!r} vs {orr!r}"
+    assert abs(gdx - odx) <= 1e-9 and abs(gdy - ody) <= 1e-9, f"{what}: d {(gdx, gdy)} vs {(odx, ody)}"
+
+
+def test_testnormcorr_kat(ctx):
+    # Retired_Proj/4_TestNormCorr/testnormcorr.cpp:20-63: exact match at dx=-4, dy=+2, r=1
+    pb = [(0, 3), (2, 3), (3, 2), (3, 3), (3, 4), (4, 3), (5, 4)]
+    vb = [1, 2, 1, 3, 1, 2, 1]
+    pa = [(7, 2), (8, 2), (9, 2), (7, 1), (8, 1)]
+    va = [1, 0, 1, 3, 2]
+    g = ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=abi.rgn_some_area())
+    o = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=abi.rgn_some_area())
+    _same(g, o, "testnormcorr")
+    assert round(g[2]) == -4 and round(g[3]) == 2 and abs(g[1] - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_corr_patches_random(ctx, seed):
+    rng = np.random.default_rng(seed)
+    w1, h1, w2, h2 = (int(v) for v in rng.integers(2, 48, size=4))
+    A = rng.standard_normal((h1, w1))
+    B = rng.standard_normal((h2, w2))
+    pa, va = synth.raster_points(A, x0=int(rng.integers(-5, 5)), y0=int(rng.integers(-5, 5)))
+    pb, vb = synth.raster_points(B, x0=int(rng.integers(-5, 5)), y0=int(rng.integers(-5, 5)))
+    rules = [(None, None), (abi.rgn_some_area(), None), (abi.rgn_min(3, 3, 12), abi.cnt_enough_points(5))]
+    rgn, cnt = rules[seed % 3]
+    radius = [4096, 10, 25][seed % 3]
+    g = ctx.corr_patches(pa, va, pb, vb, 1, -1, radius, rgn=rgn, cnt=cnt)
+    o = oracle.corr_patches(pa, va, pb, vb, 1, -1, radius, rgn=rgn, cnt=cnt)
+    _same(g, o, f"seed {seed} {(w1, h1, w2, h2)}")
+
+
+def test_corr_patches_no_peak_leaves_dxdy(ctx):
+    pa, va = synth.raster_points(np.arange(12.0).reshape(3, 4))
+    pb, vb = synth.raster_points(np.arange(20.0).reshape(4, 5))
+    g = ctx.corr_patches(pa, va, pb, vb, rgn=abi.rgn_exact(99, 99), dx0=7.5, dy0=-3.25)
+    assert g[0] == abi.XC_NO_PEAK and g[1] == 0.0 and (g[2], g[3]) == (7.5, -3.25)
+
+
+def test_cfg1_plain_pair(ctx):
+    A, B = synth.plain_pair(seed=1)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    # With LegalRgn = NULL the scan's maximum is a 1 x 2 corner overlap (r = 1 for any two
+    # pixels); a BigEnough-style rule (CThmScan.cpp:28-33) restricts it to real overlaps.
+    rgn = abi.rgn_big_enough(128, 512 * 512 // 4)
+    g = ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+    o = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+    _same(g, o, "cfg1")
+    assert (g[4].lag_x, g[4].lag_y) == (37, -21)
+
+
+@pytest.mark.parametrize("mode", [abi.XC_MODE_R, abi.XC_MODE_S])
+@pytest.mark.parametrize("seed", range(8))
+def test_corr_images_random(ctx, mode, seed):
+    rng = np.random.default_rng(100 + seed)
+    w1, h1, w2, h2 = (int(v) for v in rng.integers(7, 60, size=4))
+    A = rng.standard_normal((h1, w1))
+    B = rng.standard_normal((h2, w2))
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B, x0=3, y0=-2)
+    req = int(0.25 * min(w1 * h1, w2 * h2))
+    rgn = abi.rgn_big_enough(4, req)
+    cnt = abi.cnt_enough_points(req)
+    Rx, Ry = (-1, -1) if seed % 2 == 0 else (12, 7)
+    g = ctx.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=2, Rx=Rx, Ry=Ry)
+    o = oracle.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=2, Rx=Rx, Ry=Ry)
+    _same(g, o, f"mode {mode} seed {seed}")
