@@ -13,14 +13,30 @@ pytestmark = pytest.mark.gpu
 RTOL = 1e-9
 
 
-def _same(g, o, what):
+def _same(g, o, what, ties=None):
+    """GPU result g against oracle result o.  `ties` (set of signed lags) lists the lags whose
+    oracle value lies within RTOL of the maximum: a near-tie is decided by FFT rounding, so the
+    GPU may pick any of them (then r must still agree and dx, dy are not compared)."""
     gst, gr, gdx, gdy, gres = g
     ost, orr, odx, ody, ores = o
     assert gst == ost, f"{what}: status {gst} vs {ost}"
-    assert (gres.lag_x, gres.lag_y) == (ores.lag_x, ores.lag_y), f"{what}: lag {(gres.lag_x, gres.lag_y)} vs {(ores.lag_x, ores.lag_y)}"
     assert (gres.nx, gres.ny) == (ores.nx, ores.ny)
     assert abs(gr - orr) <= RTOL * max(1.0, abs(orr)), f"{what}: r {gr!r} vs {orr!r}"
+    glag, olag = (gres.lag_x, gres.lag_y), (ores.lag_x, ores.lag_y)
+    if ties is not None and len(ties) > 1:
+        assert glag in ties, f"{what}: lag {glag} not among near-ties {sorted(ties)[:8]}"
+        return
+    assert glag == olag, f"{what}: lag {glag} vs {olag}"
     assert abs(gdx - odx) <= 1e-9 and abs(gdy - ody) <= 1e-9, f"{what}: d {(gdx, gdy)} vs {(odx, ody)}"
+
+
+def _ncc_ties(rslt, best, res, nnegx, nnegy):
+    """Signed lags of the oracle's scanned R image within RTOL of the maximum."""
+    Ny, Nx = rslt.shape
+    iy, ix = np.nonzero(rslt >= best - RTOL * max(1.0, abs(best)))
+    lx = np.where(ix >= Nx - nnegx, ix - Nx, ix)
+    ly = np.where(iy >= Ny - nnegy, iy - Ny, iy)
+    return set(zip(lx.tolist(), ly.tolist()))
 
 
 def test_testnormcorr_kat(ctx):
@@ -47,8 +63,9 @@ def test_corr_patches_random(ctx, seed):
     rgn, cnt = rules[seed % 3]
     radius = [4096, 10, 25][seed % 3]
     g = ctx.corr_patches(pa, va, pb, vb, 1, -1, radius, rgn=rgn, cnt=cnt)
-    o = oracle.corr_patches(pa, va, pb, vb, 1, -1, radius, rgn=rgn, cnt=cnt)
-    _same(g, o, f"seed {seed} {(w1, h1, w2, h2)}")
+    o = oracle.corr_patches(pa, va, pb, vb, 1, -1, radius, rgn=rgn, cnt=cnt, want_rslt=True)
+    ties = _ncc_ties(o[5], o[1], o[4], w1 - 1, h1 - 1) if o[0] == abi.XC_OK else None
+    _same(g, o[:5], f"seed {seed} {(w1, h1, w2, h2)}", ties)
 
 
 def test_corr_patches_no_peak_leaves_dxdy(ctx):
@@ -85,5 +102,12 @@ def test_corr_images_random(ctx, mode, seed):
     cnt = abi.cnt_enough_points(req)
     Rx, Ry = (-1, -1) if seed % 2 == 0 else (12, 7)
     g = ctx.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=2, Rx=Rx, Ry=Ry)
-    o = oracle.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=2, Rx=Rx, Ry=Ry)
-    _same(g, o, f"mode {mode} seed {seed}")
+    o = oracle.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=2, Rx=Rx, Ry=Ry, want_images=True)
+    ties = None
+    if o[0] == abi.XC_OK:
+        F = o[6] if mode == abi.XC_MODE_S else o[5]
+        A = o[7].astype(bool)
+        peak = F[A].max()
+        yy, xx = np.nonzero(A & (F >= peak - RTOL * max(1.0, abs(peak))))
+        ties = set(zip((xx - (w1 - 1)).tolist(), (yy - (h1 - 1)).tolist()))
+    _same(g, o[:5], f"mode {mode} seed {seed}", ties)
