@@ -1,0 +1,93 @@
+// xc_exec.hip -- enqueue of the prepared stages (image preparation, row
+// r2c, fused column pass, row c2r + evaluation, finalize).
+#include <algorithm>
+#include <map>
+#include <tuple>
+
+#include "xc_exec.hpp"
+#include "xc_kernels.hpp"
+
+namespace xc {
+
+int nstrips_of(int w) { return (w + TP_STRIP - 1) / TP_STRIP; }
+
+TileSizes tile_sizes(int w, int h)
+{
+    TileSizes t;
+    const size_t n = (size_t)w * h, ns = nstrips_of(w);
+    t.S = sizeof(double) * n;
+    t.Q = sizeof(double) * n;
+    t.Z = sizeof(int32_t) * n;
+    t.part = sizeof(double) * TP_PARTS * 3;
+    t.rowtot = sizeof(double) * 3 * ns * h;
+    t.carry = sizeof(double) * 3 * ns * h;
+    t.msd = sizeof(double) * 2;
+    return t;
+}
+
+void make_buckets(const std::vector<PairDev> &pairs, int, std::vector<int> &order, std::vector<Bucket> &buckets)
+{
+    std::map<std::tuple<int, int, int>, std::vector<int>> m;
+    for (int i = 0; i < (int)pairs.size(); ++i) {
+        const PairDev &P = pairs[i];
+        m[std::make_tuple(P.Nx / 2, P.Ny, P.mode == PM_S ? 1 : 0)].push_back(i);
+    }
+    order.clear();
+    buckets.clear();
+    for (auto &kv : m) {
+        Bucket b;
+        b.NH = std::get<0>(kv.first);
+        b.NY = std::get<1>(kv.first);
+        b.whiten = std::get<2>(kv.first) != 0;
+        b.first = (int)order.size();
+        b.count = (int)kv.second.size();
+        for (int i : kv.second) order.push_back(i);
+        buckets.push_back(b);
+    }
+}
+
+void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool prep_b, double bytes[ST_COUNT])
+{
+    const double H = P.Nx / 2 + 1;
+    const double n1 = (double)P.w1 * P.h1, n2 = (double)P.w2 * P.h2;
+    const double zb = P.need_cnt ? 4.0 : 0.0;
+    const double sat = (P.mode == PM_S) ? zb : 16.0 + zb;  // S mode needs only the count table
+    bytes[ST_PREP] = (prep_a ? ba * n1 + sat * n1 : 0.0) + (prep_b ? bb * n2 + sat * n2 : 0.0);
+    bytes[ST_R2C] = ba * n1 + bb * n2 + s * H * (P.h1 + P.h2);
+    bytes[ST_COL] = s * H * (P.h1 + P.h2) + s * H * P.hR;
+    bytes[ST_C2R] = s * H * P.hR + sat * (n1 + n2);
+    bytes[ST_FIN] = sizeof(RowRec) * (double)P.hR;
+}
+
+template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, int s1)
+{
+    hipStream_t s = ctx->stream;
+    int st;
+    if (s0 <= ST_PREP && ST_PREP < s1 && ex.ntiles > 0) {
+        hipLaunchKernelGGL(k_tp_stats, dim3(ex.ntiles * TP_PARTS), dim3(256), 0, s, ex.d_tiles);
+        hipLaunchKernelGGL(k_tp_stats_fin, dim3((ex.ntiles + 63) / 64), dim3(64), 0, s, ex.d_tiles, ex.ntiles);
+        hipLaunchKernelGGL(k_tp_rowtot<T>, dim3(ex.n_tp_work), dim3(256), 0, s, ex.d_tiles, ex.d_tp_work);
+        hipLaunchKernelGGL(k_tp_carry, dim3(ex.ntiles), dim3(256), 0, s, ex.d_tiles);
+        hipLaunchKernelGGL(k_tp_scan<T>, dim3(ex.n_tp_work), dim3(64), 0, s, ex.d_tiles, ex.d_tp_work);
+        XC_HIP(ctx, hipGetLastError());
+    }
+    for (const Bucket &b : ex.buckets) {
+        const PairDev *pd = ex.d_pairs;
+        if (s0 <= ST_R2C && ST_R2C < s1)
+            if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, pd, b.d_r2c))) return st;
+        if (s0 <= ST_COL && ST_COL < s1)
+            if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.n_col, pd, b.d_col))) return st;
+        if (s0 <= ST_C2R && ST_C2R < s1)
+            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, pd, b.d_c2r))) return st;
+        if (s0 <= ST_FIN && ST_FIN < s1)
+            if ((st = launch_finalize<T>(ctx, b.count, pd + b.first))) return st;
+    }
+    return XC_OK;
+}
+
+int enqueue(xc_ctx *ctx, const Exec &ex, int s0, int s1)
+{
+    return ex.fp64 ? enqueue_t<double>(ctx, ex, s0, s1) : enqueue_t<float>(ctx, ex, s0, s1);
+}
+
+}  // namespace xc

@@ -1,0 +1,59 @@
+// xc_exec.hpp -- the shared executor: a laid-out set of image preparations
+// and pair correlations, enqueued stage by stage on the context's stream.
+#pragma once
+
+#include <vector>
+
+#include "xc_host.hpp"
+
+namespace xc {
+
+enum Stage { ST_PREP = 0, ST_R2C = 1, ST_COL = 2, ST_C2R = 3, ST_FIN = 4, ST_COUNT = 5 };
+
+struct Bucket {
+    int NH = 0, NY = 0;
+    bool whiten = false;
+    int first = 0, count = 0;  // pair range in the PairDev array
+    const int4 *d_r2c = nullptr;
+    int n_r2c = 0;
+    const int2 *d_col = nullptr;
+    int n_col = 0;
+    const int2 *d_c2r = nullptr;
+    int n_c2r = 0;
+};
+
+// Device-resident execution description.  All pointers are device pointers
+// into memory owned by the caller (a plan or the per-call arena).
+struct Exec {
+    int ntiles = 0;
+    const TileDev *d_tiles = nullptr;
+    const int2 *d_tp_work = nullptr;  // (tile, strip) items
+    int n_tp_work = 0;
+    const PairDev *d_pairs = nullptr;
+    std::vector<Bucket> buckets;
+    bool fp64 = false;
+};
+
+// Enqueue stages [s0, s1) of `ex` on ctx->stream.
+int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
+
+// Host-side helpers that size and fill descriptors.
+
+// Bytes of device scratch/table memory one prepared image needs.
+struct TileSizes {
+    size_t S, Q, Z, part, rowtot, carry, msd;
+};
+TileSizes tile_sizes(int w, int h);
+int nstrips_of(int w);
+
+// Group pairs into (NH, NY, whiten) buckets; returns the order of pair
+// indices (bucket-major) and the bucket list (first/count filled).
+void make_buckets(const std::vector<PairDev> &pairs, int rows_g64, std::vector<int> &order,
+                  std::vector<Bucket> &buckets);
+
+// Algorithmic HBM bytes of each stage for one pair (the compulsory traffic
+// model: inputs once, each intermediate written once and read once).
+void pair_stage_bytes(const PairDev &P, int elem_bytes, int in_bytes_a, int in_bytes_b, bool prep_a, bool prep_b,
+                      double bytes[ST_COUNT]);
+
+}  // namespace xc

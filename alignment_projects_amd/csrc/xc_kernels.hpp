@@ -197,108 +197,194 @@ static __global__ void k_splat_points(const xc_point *__restrict__ pts, const do
 #pragma clang fp contract(on)
 
 // ------------------------------------------------------------------
-// Summed-area tables (IntegrateImage, Correlation.cpp:49-105)
+// Source pixels
 // ------------------------------------------------------------------
 
-// Row pass: inclusive prefix along x of v, v^2 and (v != 0).  One
-// 256-thread workgroup per row; FP64 block scan.
-template <typename T>
-__global__ void __launch_bounds__(256) k_sat_rows(const T *__restrict__ img, int w, int h,
-                                                  double *__restrict__ S, double *__restrict__ Q,
-                                                  int32_t *__restrict__ Z)
+__device__ __forceinline__ double load_raw(const ImgView &v, int x, int y)
 {
-    __shared__ double ss[256], sq[256];
-    __shared__ int32_t sz[256];
-    const int y = blockIdx.x;
-    if (y >= h) return;
-    const int tid = threadIdx.x;
-    // each thread owns a contiguous chunk
-    const int chunk = (w + 255) / 256;
-    const int x0 = tid * chunk, x1 = imin(w, x0 + chunk);
-    const T *row = img + (long long)y * w;
-    double a = 0.0, b = 0.0;
-    int32_t c = 0;
-    for (int x = x0; x < x1; ++x) {
-        double t = (double)row[x];
-        a += t;
-        b += t * t;
-        c += (t != 0.0);
-    }
-    ss[tid] = a;
-    sq[tid] = b;
-    sz[tid] = c;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over the 256 chunk totals
-    for (int off = 1; off < 256; off <<= 1) {
-        double va = tid >= off ? ss[tid - off] : 0.0;
-        double vb = tid >= off ? sq[tid - off] : 0.0;
-        int32_t vc = tid >= off ? sz[tid - off] : 0;
-        __syncthreads();
-        ss[tid] += va;
-        sq[tid] += vb;
-        sz[tid] += vc;
-        __syncthreads();
-    }
-    a = tid > 0 ? ss[tid - 1] : 0.0;
-    b = tid > 0 ? sq[tid - 1] : 0.0;
-    c = tid > 0 ? sz[tid - 1] : 0;
-    double *So = S + (long long)y * w;
-    double *Qo = Q + (long long)y * w;
-    int32_t *Zo = Z + (long long)y * w;
-    for (int x = x0; x < x1; ++x) {
-        double t = (double)row[x];
-        a += t;
-        b += t * t;
-        c += (t != 0.0);
-        So[x] = a;
-        Qo[x] = b;
-        Zo[x] = c;
+    const long long i = (long long)y * v.stride + x;
+    switch (v.dtype) {
+    case XC_U8: return (double)reinterpret_cast<const uint8_t *>(v.data)[i];
+    case XC_U16: return (double)reinterpret_cast<const uint16_t *>(v.data)[i];
+    case XC_F32: return (double)reinterpret_cast<const float *>(v.data)[i];
+    default: return reinterpret_cast<const double *>(v.data)[i];
     }
 }
 
-// Column pass: inclusive prefix along y, in place.  Workgroup = 64 columns
-// x 4 row segments; each thread scans its segment twice (totals, then the
-// carried prefix).
-static __global__ void __launch_bounds__(256) k_sat_cols(int w, int h, double *__restrict__ S,
-                                                  double *__restrict__ Q, int32_t *__restrict__ Z)
+__device__ __forceinline__ bool present(const ImgView &v, int x, int y)
 {
-    __shared__ double ts[4][64], tq[4][64];
-    __shared__ int32_t tz[4][64];
-    const int cx = threadIdx.x & 63, seg = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + cx;
-    const int rows = (h + 3) / 4;
-    const int y0 = seg * rows, y1 = imin(h, y0 + rows);
-    double a = 0.0, b = 0.0;
-    int32_t c = 0;
-    if (x < w) {
-        for (int y = y0; y < y1; ++y) {
-            long long i = x + (long long)w * y;
-            a += S[i];
-            b += Q[i];
-            c += Z[i];
+    return !v.mask || v.mask[(long long)y * v.mstride + x] != 0;
+}
+
+// Value of pixel (x, y) in transform precision: (raw - mean) / sd rounded to
+// T, 0 outside the image or where the mask says absent.
+template <typename T>
+__device__ __forceinline__ T load_val(const ImgView &v, double mean, double sd, int x, int y)
+{
+    if (x >= v.w || y >= v.h || !present(v, x, y)) return (T)0;
+    return (T)((load_raw(v, x, y) - mean) / sd);
+}
+
+// ------------------------------------------------------------------
+// Image preparation: statistics (Stats/Normalize, Maths.cpp:111-212) and
+// the summed-area tables of IntegrateImage (Correlation.cpp:49-105), built
+// in one write pass: 64-column strips scanned top to bottom by one
+// wavefront, with the left-of-strip contribution carried in from per-strip
+// row totals.
+// ------------------------------------------------------------------
+
+// Partial sums of present raw values: TP_PARTS workgroups per image.
+static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restrict__ tiles)
+{
+    __shared__ double ss[256], sq[256], sc[256];
+    const TileDev &D = tiles[blockIdx.x / TP_PARTS];
+    const int part = blockIdx.x % TP_PARTS, tid = threadIdx.x;
+    const long long n = (long long)D.v.w * D.v.h;
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (long long i = (long long)part * 256 + tid; i < n; i += (long long)TP_PARTS * 256) {
+        const int y = (int)(i / D.v.w), x = (int)(i - (long long)y * D.v.w);
+        if (present(D.v, x, y)) {
+            const double t = load_raw(D.v, x, y);
+            a += t;
+            b += t * t;
+            c += 1.0;
         }
     }
-    ts[seg][cx] = a;
-    tq[seg][cx] = b;
-    tz[seg][cx] = c;
+    ss[tid] = a;
+    sq[tid] = b;
+    sc[tid] = c;
     __syncthreads();
-    a = 0.0;
-    b = 0.0;
-    c = 0;
-    for (int s = 0; s < seg; ++s) {
-        a += ts[s][cx];
-        b += tq[s][cx];
-        c += tz[s][cx];
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            ss[tid] += ss[tid + off];
+            sq[tid] += sq[tid + off];
+            sc[tid] += sc[tid + off];
+        }
+        __syncthreads();
     }
-    if (x < w) {
-        for (int y = y0; y < y1; ++y) {
-            long long i = x + (long long)w * y;
-            a += S[i];
-            b += Q[i];
-            c += Z[i];
-            S[i] = a;
-            Q[i] = b;
-            Z[i] = c;
+    if (tid == 0) {
+        D.part[part * 3 + 0] = ss[0];
+        D.part[part * 3 + 1] = sq[0];
+        D.part[part * 3 + 2] = sc[0];
+    }
+}
+
+// (mean, sd) per image: MeanStd::Stats with the n-1 deviation, or (0, 1).
+static __global__ void k_tp_stats_fin(const TileDev *__restrict__ tiles, int ntiles)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ntiles) return;
+    const TileDev &D = tiles[i];
+    double mean = 0.0, sd = 1.0;
+    if (D.normalize) {
+        double s = 0.0, q = 0.0, c = 0.0;
+        for (int p = 0; p < TP_PARTS; ++p) {
+            s += D.part[p * 3 + 0];
+            q += D.part[p * 3 + 1];
+            c += D.part[p * 3 + 2];
+        }
+        mean = s;
+        sd = 0.0;
+        if (c > 1.0) {
+            mean = s / c;
+            sd = sqrt(fmax(q - c * mean * mean, 0.0) / (c - 1.0));
+        }
+    }
+    D.msd[0] = mean;
+    D.msd[1] = sd;
+}
+
+// Strip row totals of v, v^2, (v != 0): work item = (tile, strip).
+template <typename T>
+__global__ void __launch_bounds__(256) k_tp_rowtot(const TileDev *__restrict__ tiles, const int2 *__restrict__ work)
+{
+    const int2 wi = work[blockIdx.x];
+    const TileDev &D = tiles[wi.x];
+    const int s = wi.y, h = D.v.h, x0 = s * TP_STRIP, x1 = imin(D.v.w, x0 + TP_STRIP);
+    const double mean = D.msd[0], sd = D.msd[1];
+    for (int y = threadIdx.x; y < h; y += blockDim.x) {
+        double a = 0.0, b = 0.0, c = 0.0;
+        for (int x = x0; x < x1; ++x) {
+            const double t = (double)load_val<T>(D.v, mean, sd, x, y);
+            a += t;
+            b += t * t;
+            c += (t != 0.0);
+        }
+        const long long o = (long long)s * h + y, plane = (long long)D.nstrips * h;
+        D.rowtot[o] = a;
+        D.rowtot[plane + o] = b;
+        D.rowtot[2 * plane + o] = c;
+    }
+}
+
+// Carries: carry[s][y] = sum_{y' <= y} sum_{s' < s} rowtot[s'][y'].  One
+// workgroup per image, one thread per strip (<= 256 strips).
+static __global__ void __launch_bounds__(256) k_tp_carry(const TileDev *__restrict__ tiles)
+{
+    __shared__ double sa[256], sb[256], sc[256];
+    const TileDev &D = tiles[blockIdx.x];
+    const int s = threadIdx.x, ns = D.nstrips, h = D.v.h;
+    const long long plane = (long long)ns * h;
+    double aa = 0.0, ab = 0.0, ac = 0.0;
+    for (int y = 0; y < h; ++y) {
+        const long long o = (long long)s * h + y;
+        sa[s] = s < ns ? D.rowtot[o] : 0.0;
+        sb[s] = s < ns ? D.rowtot[plane + o] : 0.0;
+        sc[s] = s < ns ? D.rowtot[2 * plane + o] : 0.0;
+        __syncthreads();
+        double ea = 0.0, eb = 0.0, ec = 0.0;
+        for (int k = 0; k < s && k < ns; ++k) {
+            ea += sa[k];
+            eb += sb[k];
+            ec += sc[k];
+        }
+        aa += ea;
+        ab += eb;
+        ac += ec;
+        if (s < ns) {
+            D.carry[o] = aa;
+            D.carry[plane + o] = ab;
+            D.carry[2 * plane + o] = ac;
+        }
+        __syncthreads();
+    }
+}
+
+// Strip scan: one wavefront per (tile, strip); lane = column.
+template <typename T>
+__global__ void __launch_bounds__(64) k_tp_scan(const TileDev *__restrict__ tiles, const int2 *__restrict__ work)
+{
+    const int2 wi = work[blockIdx.x];
+    const TileDev &D = tiles[wi.x];
+    const int s = wi.y, lane = threadIdx.x, w = D.v.w, h = D.v.h;
+    const int x = s * TP_STRIP + lane;
+    const bool in = x < w;
+    const double mean = D.msd[0], sd = D.msd[1];
+    const long long plane = (long long)D.nstrips * h;
+    double cs = 0.0, cq = 0.0;
+    int cz = 0;
+    for (int y = 0; y < h; ++y) {
+        const double t = in ? (double)load_val<T>(D.v, mean, sd, x, y) : 0.0;
+        cs += t;
+        cq += t * t;
+        cz += (t != 0.0);
+        double ps = cs, pq = cq;
+        int pz = cz;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double us = __shfl_up(ps, off), uq = __shfl_up(pq, off);
+            const int uz = __shfl_up(pz, off);
+            if (lane >= off) {
+                ps += us;
+                pq += uq;
+                pz += uz;
+            }
+        }
+        if (in) {
+            const long long o = (long long)s * h + y, i = (long long)y * w + x;
+            D.S[i] = D.carry[o] + ps;
+            D.Q[i] = D.carry[plane + o] + pq;
+            D.Z[i] = (int32_t)D.carry[2 * plane + o] + pz;
         }
     }
 }
@@ -326,19 +412,18 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     const int4 wi = work[blockIdx.x];  // {pair, image, y0, -}
     const PairDev &P = pairs[wi.x];
     const int img = wi.y, y = wi.z + g;
-    const int w = img ? P.w2 : P.w1, h = img ? P.h2 : P.h1;
-    const T *src = reinterpret_cast<const T *>(img ? P.imgB : P.imgA) + (long long)y * w;
+    const ImgView &V = img ? P.B : P.A;
     cpx<T> *dst = reinterpret_cast<cpx<T> *>(img ? P.specB : P.specA);
-    const bool valid = y < h;
+    const bool valid = y < V.h;
     const int H = NH + 1;
+    const double mean = V.msd[0], sd = V.msd[1];
 
     cpx<T> v[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int p = t + TPF * m;
-        const int i0 = 2 * p, i1 = 2 * p + 1;
-        v[m].x = (valid && i0 < w) ? src[i0] : (T)0;
-        v[m].y = (valid && i1 < w) ? src[i1] : (T)0;
+        v[m].x = valid ? load_val<T>(V, mean, sd, 2 * p, y) : (T)0;
+        v[m].y = valid ? load_val<T>(V, mean, sd, 2 * p + 1, y) : (T)0;
     }
     LdsView<T> L{smem + g * NH, 1};
     fft<-1, NH>(v, t, L, tw);

@@ -6,27 +6,13 @@
 #include <algorithm>
 #include <vector>
 
+#include "xc_exec.hpp"
 #include "xc_host.hpp"
 #include "xc_kernels.hpp"
 
 using namespace xc;
 
 namespace {
-
-struct PlainBufs {
-    xc_point *pts;
-    double *vals;
-    double *imgA, *imgB;
-    double *SA, *QA, *SB, *QB;
-    int32_t *ZA, *ZB;
-    cpx<double> *specA, *specB, *lag;
-    RowRec *rows;
-    PairDev *pair;
-    int4 *w_r2c;
-    int2 *w_col, *w_c2r;
-    xc_result *res;
-    uint8_t *table;
-};
 
 // Run one FP64 correlation of point lists (A = ip1/iv1, B = ip2/iv2).
 int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int n1, const xc_point *ip2,
@@ -83,33 +69,59 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     const int nr2c = (h1 + G - 1) / G + (h2 + G - 1) / G;
     const int ncol = (H + C - 1) / C;
     const int nc2r = (P.hR + G - 1) / G;
+    const int ns1 = nstrips_of(w1), ns2 = nstrips_of(w2);
+    const TileSizes t1 = tile_sizes(w1, h1), t2 = tile_sizes(w2, h2);
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n), i_vals = lay.add(sizeof(double) * n);
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
-    size_t i_SA = lay.add(sizeof(double) * w1 * h1), i_QA = lay.add(sizeof(double) * w1 * h1);
-    size_t i_SB = lay.add(sizeof(double) * w2 * h2), i_QB = lay.add(sizeof(double) * w2 * h2);
-    size_t i_ZA = lay.add(sizeof(int32_t) * w1 * h1), i_ZB = lay.add(sizeof(int32_t) * w2 * h2);
+    size_t i_SA = lay.add(t1.S), i_QA = lay.add(t1.Q), i_ZA = lay.add(t1.Z);
+    size_t i_SB = lay.add(t2.S), i_QB = lay.add(t2.Q), i_ZB = lay.add(t2.Z);
+    size_t i_pA = lay.add(t1.part), i_rA = lay.add(t1.rowtot), i_cA = lay.add(t1.carry), i_mA = lay.add(t1.msd);
+    size_t i_pB = lay.add(t2.part), i_rB = lay.add(t2.rowtot), i_cB = lay.add(t2.carry), i_mB = lay.add(t2.msd);
     size_t i_spA = lay.add(sizeof(cpx<double>) * h1 * H), i_spB = lay.add(sizeof(cpx<double>) * h2 * H);
     size_t i_lag = lay.add(sizeof(cpx<double>) * (size_t)P.hR * H);
     size_t i_rows = lay.add(sizeof(RowRec) * P.hR);
-    // staged (uploaded) part: pair, work lists, table
+    size_t i_res = lay.add(sizeof(xc_result));
+    // uploaded block 2: pair, tiles, work lists, table
     size_t i_pair = lay.add(sizeof(PairDev));
+    size_t i_tiles = lay.add(sizeof(TileDev) * 2);
+    size_t i_tpw = lay.add(sizeof(int2) * (ns1 + ns2));
     size_t i_wr = lay.add(sizeof(int4) * nr2c), i_wc = lay.add(sizeof(int2) * ncol), i_wc2 = lay.add(sizeof(int2) * nc2r);
     size_t i_tbl = lay.add(tbl ? tbl : 1);
-    size_t i_res = lay.add(sizeof(xc_result));
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
     char *base = (char *)ctx->arena.p;
     auto at = [&](size_t i) { return base + lay.off[i]; };
 
-    P.imgA = at(i_iA);
-    P.imgB = at(i_iB);
-    P.SA = (double *)at(i_SA);
-    P.QA = (double *)at(i_QA);
-    P.SB = (double *)at(i_SB);
-    P.QB = (double *)at(i_QB);
-    P.ZA = (int32_t *)at(i_ZA);
-    P.ZB = (int32_t *)at(i_ZB);
+    TileDev T[2];
+    memset(T, 0, sizeof T);
+    for (int k = 0; k < 2; ++k) {
+        TileDev &D = T[k];
+        D.v.data = at(k ? i_iB : i_iA);
+        D.v.dtype = XC_F64;
+        D.v.w = k ? w2 : w1;
+        D.v.h = k ? h2 : h1;
+        D.v.stride = D.v.w;
+        D.v.mstride = D.v.w;
+        D.msd = (double *)at(k ? i_mB : i_mA);
+        D.v.msd = D.msd;
+        D.S = (double *)at(k ? i_SB : i_SA);
+        D.Q = (double *)at(k ? i_QB : i_QA);
+        D.Z = (int32_t *)at(k ? i_ZB : i_ZA);
+        D.part = (double *)at(k ? i_pB : i_pA);
+        D.rowtot = (double *)at(k ? i_rB : i_rA);
+        D.carry = (double *)at(k ? i_cB : i_cA);
+        D.nstrips = k ? ns2 : ns1;
+        D.normalize = 0;
+    }
+    P.A = T[0].v;
+    P.B = T[1].v;
+    P.SA = T[0].S;
+    P.QA = T[0].Q;
+    P.ZA = T[0].Z;
+    P.SB = T[1].S;
+    P.QB = T[1].Q;
+    P.ZB = T[1].Z;
     P.specA = at(i_spA);
     P.specB = at(i_spB);
     P.lag = at(i_lag);
@@ -118,66 +130,75 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     P.rule = make_rule(rgn, cnt, (const uint8_t *)at(i_tbl));
     P.need_cnt = P.rule.cnt_enabled;
 
-    // ---- host staging: points, values, pair, work lists, table ----
-    const size_t stage_lo = lay.off[i_pts];
-    const size_t stage_hi = lay.off[i_vals] + sizeof(double) * n;
-    const size_t stage2_lo = lay.off[i_pair];
-    const size_t stage2_hi = lay.off[i_tbl] + (tbl ? tbl : 1);
-    st = ensure_pinned(ctx, (stage_hi - stage_lo) + (stage2_hi - stage2_lo) + sizeof(xc_result));
+    // ---- host staging: points + values, then descriptors ----
+    const size_t s1_lo = lay.off[i_pts], s1_hi = lay.off[i_vals] + sizeof(double) * n;
+    const size_t s2_lo = lay.off[i_pair], s2_hi = lay.off[i_tbl] + (tbl ? tbl : 1);
+    st = ensure_pinned(ctx, (s1_hi - s1_lo) + (s2_hi - s2_lo) + sizeof(xc_result) + 64);
     if (st) return st;
     char *hs = (char *)ctx->pinned;
-    char *hs2 = hs + (stage_hi - stage_lo);
-    xc_result *hres = (xc_result *)(hs2 + (stage2_hi - stage2_lo));
-    memcpy(hs + (lay.off[i_pts] - stage_lo), ip1, sizeof(xc_point) * n1);
-    memcpy(hs + (lay.off[i_pts] - stage_lo) + sizeof(xc_point) * n1, ip2, sizeof(xc_point) * n2);
-    memcpy(hs + (lay.off[i_vals] - stage_lo), iv1, sizeof(double) * n1);
-    memcpy(hs + (lay.off[i_vals] - stage_lo) + sizeof(double) * n1, iv2, sizeof(double) * n2);
-    memcpy(hs2 + (lay.off[i_pair] - stage2_lo), &P, sizeof P);
-    int4 *wr = (int4 *)(hs2 + (lay.off[i_wr] - stage2_lo));
+    char *hs2 = hs + (s1_hi - s1_lo);
+    xc_result *hres = (xc_result *)(((uintptr_t)(hs2 + (s2_hi - s2_lo)) + 63) & ~(uintptr_t)63);
+    auto h1p = [&](size_t i) { return hs + (lay.off[i] - s1_lo); };
+    auto h2p = [&](size_t i) { return hs2 + (lay.off[i] - s2_lo); };
+    memcpy(h1p(i_pts), ip1, sizeof(xc_point) * n1);
+    memcpy(h1p(i_pts) + sizeof(xc_point) * n1, ip2, sizeof(xc_point) * n2);
+    memcpy(h1p(i_vals), iv1, sizeof(double) * n1);
+    memcpy(h1p(i_vals) + sizeof(double) * n1, iv2, sizeof(double) * n2);
+    memcpy(h2p(i_pair), &P, sizeof P);
+    memcpy(h2p(i_tiles), T, sizeof T);
+    int2 *tw = (int2 *)h2p(i_tpw);
+    for (int s = 0; s < ns1; ++s) tw[s] = make_int2(0, s);
+    for (int s = 0; s < ns2; ++s) tw[ns1 + s] = make_int2(1, s);
+    int4 *wr = (int4 *)h2p(i_wr);
     int k = 0;
     for (int y = 0; y < h1; y += G) wr[k++] = make_int4(0, 0, y, 0);
     for (int y = 0; y < h2; y += G) wr[k++] = make_int4(0, 1, y, 0);
-    int2 *wc = (int2 *)(hs2 + (lay.off[i_wc] - stage2_lo));
+    int2 *wc = (int2 *)h2p(i_wc);
     for (int i = 0; i < ncol; ++i) wc[i] = make_int2(0, i * C);
-    int2 *wc2 = (int2 *)(hs2 + (lay.off[i_wc2] - stage2_lo));
+    int2 *wc2 = (int2 *)h2p(i_wc2);
     for (int i = 0; i < nc2r; ++i) wc2[i] = make_int2(0, i * G);
-    if (tbl) memcpy(hs2 + (lay.off[i_tbl] - stage2_lo), rgn->table, tbl);
+    if (tbl) memcpy(h2p(i_tbl), rgn->table, tbl);
 
     hipStream_t s = ctx->stream;
-    XC_HIP(ctx, hipMemcpyAsync(base + stage_lo, hs, stage_hi - stage_lo, hipMemcpyHostToDevice, s));
-    XC_HIP(ctx, hipMemcpyAsync(base + stage2_lo, hs2, stage2_hi - stage2_lo, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(base + s1_lo, hs, s1_hi - s1_lo, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(base + s2_lo, hs2, s2_hi - s2_lo, hipMemcpyHostToDevice, s));
 
     // ---- rasters: zero then splat (ImageFromValuesAndPoints) ----
     {
         const long long nA = (long long)w1 * h1, nB = (long long)w2 * h2;
         const int fb = (int)std::min<long long>(1024, (nA + nB + 255) / 256);
-        hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)P.imgA, nA, 0.0);
-        hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)P.imgB, nB, 0.0);
+        hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)at(i_iA), nA, 0.0);
+        hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)at(i_iB), nB, 0.0);
         xc_point *dp = (xc_point *)at(i_pts);
         double *dv = (double *)at(i_vals);
         hipLaunchKernelGGL(k_splat_points, dim3((n1 + 255) / 256), dim3(256), 0, s, dp, dv, n1, (double)B1.L,
-                           (double)B1.B, (double *)P.imgA, w1, h1);
+                           (double)B1.B, (double *)at(i_iA), w1, h1);
         hipLaunchKernelGGL(k_splat_points, dim3((n2 + 255) / 256), dim3(256), 0, s, dp + n1, dv + n1, n2,
-                           (double)B2.L, (double)B2.B, (double *)P.imgB, w2, h2);
+                           (double)B2.L, (double)B2.B, (double *)at(i_iB), w2, h2);
         XC_HIP(ctx, hipGetLastError());
     }
-    // ---- summed-area tables ----
-    hipLaunchKernelGGL(k_sat_rows<double>, dim3(h1), dim3(256), 0, s, (const double *)P.imgA, w1, h1,
-                       (double *)P.SA, (double *)P.QA, (int32_t *)P.ZA);
-    hipLaunchKernelGGL(k_sat_rows<double>, dim3(h2), dim3(256), 0, s, (const double *)P.imgB, w2, h2,
-                       (double *)P.SB, (double *)P.QB, (int32_t *)P.ZB);
-    hipLaunchKernelGGL(k_sat_cols, dim3((w1 + 63) / 64), dim3(256), 0, s, w1, h1, (double *)P.SA, (double *)P.QA,
-                       (int32_t *)P.ZA);
-    hipLaunchKernelGGL(k_sat_cols, dim3((w2 + 63) / 64), dim3(256), 0, s, w2, h2, (double *)P.SB, (double *)P.QB,
-                       (int32_t *)P.ZB);
-    XC_HIP(ctx, hipGetLastError());
-
-    // ---- transforms and evaluation ----
-    const PairDev *dP = (const PairDev *)at(i_pair);
-    if ((st = launch_row_r2c<double>(ctx, NH, nr2c, dP, (const int4 *)at(i_wr)))) return st;
-    if ((st = launch_col<double>(ctx, NY, mode == PM_S, ncol, dP, (const int2 *)at(i_wc)))) return st;
-    if ((st = launch_row_c2r<double>(ctx, NH, nc2r, dP, (const int2 *)at(i_wc2)))) return st;
-    if ((st = launch_finalize<double>(ctx, 1, dP))) return st;
+    // ---- tables, transforms, evaluation ----
+    Exec ex;
+    ex.fp64 = true;
+    ex.ntiles = 2;
+    ex.d_tiles = (const TileDev *)at(i_tiles);
+    ex.d_tp_work = (const int2 *)at(i_tpw);
+    ex.n_tp_work = ns1 + ns2;
+    ex.d_pairs = (const PairDev *)at(i_pair);
+    Bucket b;
+    b.NH = NH;
+    b.NY = NY;
+    b.whiten = mode == PM_S;
+    b.first = 0;
+    b.count = 1;
+    b.d_r2c = (const int4 *)at(i_wr);
+    b.n_r2c = nr2c;
+    b.d_col = (const int2 *)at(i_wc);
+    b.n_col = ncol;
+    b.d_c2r = (const int2 *)at(i_wc2);
+    b.n_c2r = nc2r;
+    ex.buckets.push_back(b);
+    if ((st = enqueue(ctx, ex))) return st;
     XC_HIP(ctx, hipMemcpyAsync(hres, at(i_res), sizeof(xc_result), hipMemcpyDeviceToHost, s));
     XC_HIP(ctx, hipStreamSynchronize(s));
     *out = *hres;

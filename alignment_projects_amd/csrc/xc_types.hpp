@@ -35,6 +35,38 @@ struct RowRec {
     int32_t pad[2];
 };
 
+// A source image as the kernels read it: raw pixels of any supported type,
+// an optional presence mask, and a device (mean, sd) pair applied as
+// (v - mean) / sd before rounding to the transform precision (Normalize,
+// Maths.cpp:192-212; (0, 1) for raw values).  Absent pixels read as 0,
+// exactly as ImageFromValuesAndPoints leaves them.
+struct ImgView {
+    const void *data;
+    const uint8_t *mask;
+    const double *msd;   // device (mean, sd)
+    int32_t dtype;       // xc_dtype
+    int32_t w, h;
+    int32_t stride;      // elements per row of data
+    int32_t mstride;     // bytes per row of mask
+    int32_t pad;
+};
+
+// Per-image preparation record (statistics and summed-area tables).
+struct TileDev {
+    ImgView v;
+    double *msd;         // out: (mean, sd)
+    double *S, *Q;       // out: FP64 summed-area tables, w x h
+    int32_t *Z;          // out: non-zero count table, w x h
+    double *part;        // scratch: [TP_PARTS][3] statistics partials
+    double *rowtot;      // scratch: [3][nstrips][h] strip row totals
+    double *carry;       // scratch: [3][nstrips][h] carries
+    int32_t nstrips;
+    int32_t normalize;
+};
+
+constexpr int TP_PARTS = 16;  // statistics partials per image
+constexpr int TP_STRIP = 64;  // summed-area strip width (one wavefront)
+
 // One correlation, as the kernels see it.  Images are compact w x h rasters
 // (row stride w) whose origin is the point list's bounding-box corner; the
 // zero padding to Nx x Ny is implicit.
@@ -52,7 +84,7 @@ struct PairDev {
     int32_t pad0;
     double mincor;
     RuleDev rule;
-    const void *imgA, *imgB;          // T rasters
+    ImgView A, B;                     // source images
     const double *SA, *QA, *SB, *QB;  // FP64 summed-area tables (S, S^2)
     const int32_t *ZA, *ZB;           // non-zero count tables
     void *specA, *specB;              // cpx<T> [h][Nx/2+1] row spectra

@@ -67,6 +67,13 @@ _OPTIONAL = {
                      C.POINTER(C.c_void_p)], C.c_int),
     "xc_tile_release": ([C.c_void_p, C.c_void_p], C.c_int),
     "xc_tile_bytes": ([C.c_void_p], C.c_int64),
+    "xc_plan_create": ([C.c_void_p, C.POINTER(XcPair), C.c_int, C.c_int, C.POINTER(C.c_void_p)], C.c_int),
+    "xc_plan_launch": ([C.c_void_p], C.c_int),
+    "xc_plan_wait": ([C.c_void_p, C.POINTER(XcResult)], C.c_int),
+    "xc_plan_run": ([C.c_void_p, C.POINTER(XcResult)], C.c_int),
+    "xc_plan_stage_times": ([C.c_void_p, C.c_int, _dp, _dp], C.c_int),
+    "xc_plan_model_bytes": ([C.c_void_p], C.c_double),
+    "xc_plan_destroy": ([C.c_void_p], C.c_int),
     "xc_batch_corr_f32": ([C.c_void_p, C.POINTER(XcPair), C.c_int, C.POINTER(XcResult)], C.c_int),
     "xc_sweep_angles_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int, C.POINTER(XcRgnRule),
                              C.POINTER(XcCntRule), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(XcResult)],
@@ -145,6 +152,113 @@ class Context:
                                        C.byref(res))
         self._check(st, "xc_corr_images_f64")
         return st, r.value, dx.value, dy.value, res
+
+
+    # ---- tiles and batched plans ----
+
+    def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0):
+        """Upload (or adopt, with device_ptr) one image as a device-resident tile."""
+        flags = abi.XC_TILE_NORMALIZE if normalize else 0
+        if device_ptr is not None:
+            flags |= abi.XC_TILE_DEVICE_SRC
+            h, w = img
+            data = C.c_void_p(device_ptr)
+            mptr = C.c_void_p(mask) if mask is not None else None
+            dt = dtype
+        else:
+            img = np.ascontiguousarray(img)
+            h, w = img.shape
+            dt = {np.dtype(np.uint8): abi.XC_U8, np.dtype(np.uint16): abi.XC_U16,
+                  np.dtype(np.float32): abi.XC_F32, np.dtype(np.float64): abi.XC_F64}[img.dtype]
+            data = img.ctypes.data_as(C.c_void_p)
+            mptr = None
+            if mask is not None:
+                mask = np.ascontiguousarray(mask, dtype=np.uint8)
+                assert mask.shape == (h, w)
+                mptr = mask.ctypes.data_as(C.c_void_p)
+        t = C.c_void_p()
+        st = self.L.xc_tile_put(self.h, data, dt, w, h, stride, mptr, flags, C.byref(t))
+        self._check(st, "xc_tile_put", ok=(abi.XC_OK,))
+        return Tile(self, t, w, h)
+
+    def plan(self, pairs, refresh_tiles=False):
+        return Plan(self, pairs, refresh_tiles)
+
+    def batch_corr(self, pairs):
+        arr = (XcPair * len(pairs))(*pairs)
+        res = (XcResult * len(pairs))()
+        st = self.L.xc_batch_corr_f32(self.h, arr, len(pairs), res)
+        self._check(st, "xc_batch_corr_f32", ok=(abi.XC_OK,))
+        return list(res)
+
+
+class Tile:
+    def __init__(self, ctx, handle, w, h):
+        self.ctx, self.h, self.w, self.hgt = ctx, handle, w, h
+
+    @property
+    def nbytes(self):
+        return self.ctx.L.xc_tile_bytes(self.h)
+
+    def release(self):
+        if self.h:
+            self.ctx.L.xc_tile_release(self.ctx.h, self.h)
+            self.h = None
+
+
+def make_pair(a, b, mode=abi.XC_MODE_NCC, Ox=0, Oy=0, radius=4096, Rx=-1, Ry=-1, rgn=None, cnt=None, angle=0.0):
+    """xc_pair for tiles a, b (CorrPatches semantics for XC_MODE_NCC, CorrImages* for R/S)."""
+    return XcPair(a=a.h, b=b.h, mode=mode, Ox=Ox, Oy=Oy, radius=radius, Rx=Rx, Ry=Ry, angle_deg=angle,
+                  rgn=rgn if rgn is not None else XcRgnRule(), cnt=cnt if cnt is not None else XcCntRule())
+
+
+class Plan:
+    """A fixed batch of pairs, laid out once and replayed as one HIP graph (xc_plan)."""
+
+    def __init__(self, ctx, pairs, refresh_tiles=False):
+        self.ctx = ctx
+        self.n = len(pairs)
+        arr = (XcPair * self.n)(*pairs)
+        h = C.c_void_p()
+        st = ctx.L.xc_plan_create(ctx.h, arr, self.n, 1 if refresh_tiles else 0, C.byref(h))
+        ctx._check(st, "xc_plan_create", ok=(abi.XC_OK,))
+        self.h = h
+        self._res = (XcResult * self.n)()
+
+    def run(self, want=True):
+        st = self.ctx.L.xc_plan_run(self.h, self._res if want else None)
+        self.ctx._check(st, "xc_plan_run", ok=(abi.XC_OK,))
+        return list(self._res) if want else None
+
+    def launch(self):
+        self.ctx._check(self.ctx.L.xc_plan_launch(self.h), "xc_plan_launch", ok=(abi.XC_OK,))
+
+    def wait(self, want=False):
+        st = self.ctx.L.xc_plan_wait(self.h, self._res if want else None)
+        self.ctx._check(st, "xc_plan_wait", ok=(abi.XC_OK,))
+        return list(self._res) if want else None
+
+    def stage_times(self, iters=3):
+        ms = (C.c_double * 5)()
+        by = (C.c_double * 5)()
+        st = self.ctx.L.xc_plan_stage_times(self.h, iters, ms, by)
+        self.ctx._check(st, "xc_plan_stage_times", ok=(abi.XC_OK,))
+        return list(ms), list(by)
+
+    @property
+    def model_bytes(self):
+        return self.ctx.L.xc_plan_model_bytes(self.h)
+
+    def destroy(self):
+        if self.h:
+            self.ctx.L.xc_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 def fft_size(n1, n2):

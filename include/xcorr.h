@@ -250,6 +250,38 @@ typedef struct xc_pair {
 int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *results);
 
 /*
+ * Batch plans: a fixed batch of pairs laid out once (descriptors, work
+ * lists, spectra, lag rows) and replayed as one HIP graph.  This is the
+ * batched counterpart of the reference's per-thread CorrPatches calls
+ * (CThmScan::TCDGet, CThmScan.cpp:77-107), without per-call planning.
+ * XC_PLAN_REFRESH_TILES re-runs the tiles' statistics and summed-area tables
+ * inside every run (for tiles whose pixels are rewritten between runs).
+ */
+typedef struct xc_plan xc_plan;
+
+enum xc_plan_flags { XC_PLAN_REFRESH_TILES = 1 };
+
+/* Pipeline stages reported by xc_plan_stage_times. */
+enum xc_stage { XC_STAGE_PREP = 0, XC_STAGE_R2C = 1, XC_STAGE_COL = 2, XC_STAGE_C2R = 3, XC_STAGE_FIN = 4,
+                XC_STAGE_COUNT = 5 };
+
+int xc_plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan **out);
+/* Launch the plan's graph on the context stream (asynchronous). */
+int xc_plan_launch(xc_plan *plan);
+/* Wait for the last launch; copy results[n] back if results != NULL. */
+int xc_plan_wait(xc_plan *plan, xc_result *results);
+/* launch + wait. */
+int xc_plan_run(xc_plan *plan, xc_result *results);
+/* Run each stage separately `iters` times between HIP events on the
+ * context stream; ms[XC_STAGE_COUNT] receives the mean duration of each
+ * stage, bytes[XC_STAGE_COUNT] its algorithmic HBM bytes per run. */
+int xc_plan_stage_times(xc_plan *plan, int iters, double *ms, double *bytes);
+/* Algorithmic HBM bytes of one run of the whole plan (inputs once; row
+ * spectra, lag rows and FP64 tables written once and read once). */
+double xc_plan_model_bytes(const xc_plan *plan);
+int xc_plan_destroy(xc_plan *plan);
+
+/*
  * Rotation sweep (CThmScan::AngleScan*, CThmScan.cpp:431-621): correlate
  * A rotated by each of n_angles angles against B in one batched launch, in
  * S mode with the given legality rules.  results[n_angles].

@@ -1,0 +1,118 @@
+"""Batched FP32 form (tiles + plans) on the GPU against the FP64 oracle.
+
+Bar (BASELINE targets for the batched form): integer peak equal to the FP64 oracle's, subpixel error
+<= 0.05 px, correlation value within 1e-4 absolute; results bitwise identical alone and in a mix.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from alignment_projects_amd import abi, synth
+
+pytestmark = pytest.mark.gpu
+
+SUBPX = 0.05
+RTOL32 = 1e-4
+
+
+def _oracle_pair(A, B, rgn, cnt, Ma=None, Mb=None):
+    pa, va = synth.raster_points(A, Ma)
+    pb, vb = synth.raster_points(B, Mb)
+    return oracle.corr_patches(pa, synth.normalize(va), pb, synth.normalize(vb), 0, 0, 4096, rgn=rgn, cnt=cnt)
+
+
+def _strips(n, w, h, kmax, seed0):
+    out = []
+    for i in range(n):
+        a, b, s = synth.strip_pair(seed0 + i, w=w, h=h, kmax=kmax)
+        out.append((a, b, s))
+    return out
+
+
+def test_batch_strips_vs_oracle(ctx):
+    w, h = 512, 64
+    data = _strips(12, w, h, 20, 500)
+    rgn = abi.rgn_big_enough(h // 2, w * h // 4)
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data]
+    from alignment_projects_amd.xcorr import make_pair
+    pairs = [make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
+    res = ctx.batch_corr(pairs)
+    for (a, b, s), r in zip(data, res):
+        o = _oracle_pair(a, b, rgn, None)
+        assert r.status == o[0] == abi.XC_OK
+        assert (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y), (r.as_dict(), o[4].as_dict())
+        assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX, (r.dx, r.dy, o[2], o[3])
+        assert abs(r.r - o[1]) <= RTOL32, (r.r, o[1])
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
+
+
+def test_batch_modes_and_rules(ctx):
+    rng = np.random.default_rng(3)
+    from alignment_projects_amd.xcorr import make_pair
+    A = synth.quantize(synth.bandlimited(rng, 96, 160, 128, 40), np.uint8)
+    B = synth.quantize(np.roll(A.astype(float), (5, -9), axis=(0, 1)) + rng.normal(0, 6, A.shape), np.uint8)
+    ta, tb = ctx.tile_put(A), ctx.tile_put(B)
+    req = 96 * 160 // 4
+    cases = [
+        (abi.XC_MODE_NCC, abi.rgn_big_enough(24, req), abi.cnt_enough_points(req), 4096),
+        (abi.XC_MODE_NCC, abi.rgn_some_area(), None, 30),
+        (abi.XC_MODE_R, abi.rgn_big_enough(24, req), abi.cnt_enough_points(req), 0),
+        (abi.XC_MODE_S, abi.rgn_big_enough(24, req), abi.cnt_enough_points(req), 0),
+    ]
+    pairs = [make_pair(ta, tb, mode=m, rgn=g, cnt=c, radius=rad) for m, g, c, rad in cases]
+    res = ctx.batch_corr(pairs)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    va, vb = synth.normalize(va), synth.normalize(vb)
+    for (m, g, c, rad), r in zip(cases, res):
+        if m == abi.XC_MODE_NCC:
+            o = oracle.corr_patches(pa, va, pb, vb, 0, 0, rad, rgn=g, cnt=c)
+        else:
+            o = oracle.corr_images(m, pa, va, pb, vb, rgn=g, cnt=c)
+        assert r.status == o[0], (m, r.status, o[0])
+        assert (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y), (m, r.as_dict(), o[4].as_dict())
+        assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX, (m, r.dx, r.dy, o[2], o[3])
+        tol = RTOL32 * (50 if m == abi.XC_MODE_S else 1)
+        assert abs(r.r - o[1]) <= tol * max(1.0, abs(o[1])), (m, r.r, o[1])
+    ta.release()
+    tb.release()
+
+
+def test_batch_masks_vs_oracle(ctx):
+    from alignment_projects_amd.xcorr import make_pair
+    for seed in range(4):
+        a, ma, b, mb, s = synth.uneven_pair(900 + seed, lo=96, hi=200, smax=20)
+        h, w = a.shape
+        req = w * h // 4
+        rgn, cnt = abi.rgn_big_enough(min(w, h) // 4, req), abi.cnt_enough_points(req // 2)
+        ta, tb = ctx.tile_put(a, mask=ma), ctx.tile_put(b, mask=mb)
+        r = ctx.batch_corr([make_pair(ta, tb, rgn=rgn, cnt=cnt)])[0]
+        o = _oracle_pair(a, b, rgn, cnt, ma, mb)
+        assert r.status == o[0]
+        assert (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y), (seed, r.as_dict(), o[4].as_dict())
+        assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX
+        assert abs(r.r - o[1]) <= RTOL32
+        ta.release()
+        tb.release()
+
+
+def test_alone_vs_mix_bitwise(ctx):
+    from alignment_projects_amd.xcorr import make_pair
+    data = _strips(6, 384, 48, 16, 700)
+    rgn = abi.rgn_big_enough(24, 384 * 48 // 4)
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data]
+    pairs = [make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
+    mix = ctx.batch_corr(pairs)
+    plan = ctx.plan(pairs)
+    again = plan.run()
+    again2 = plan.run()
+    for i, p in enumerate(pairs):
+        alone = ctx.batch_corr([p])[0]
+        for k in ("r", "dx", "dy", "peak", "lag_x", "lag_y", "status"):
+            assert getattr(alone, k) == getattr(mix[i], k) == getattr(again[i], k) == getattr(again2[i], k), (i, k)
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
