@@ -135,6 +135,7 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
 // Lag-row index of raster row iy, or -1 for the dead zone.
 __device__ __forceinline__ int lag_row_of_raster(const PairDev &P, int iy)
 {
+    if (P.lag_order == LAG_ALL) return iy;
     int y;
     if (iy < P.nposy) y = iy;
     else if (iy >= P.Ny - P.nnegy) y = iy - P.Ny;
@@ -144,8 +145,15 @@ __device__ __forceinline__ int lag_row_of_raster(const PairDev &P, int iy)
 }
 
 // Signed lag y of stored lag row yl.
+constexpr int DEAD_ROW = -0x40000000;
+
 __device__ __forceinline__ int lag_y_of_row(const PairDev &P, int yl)
 {
+    if (P.lag_order == LAG_ALL) {
+        if (yl < P.nposy) return yl;
+        if (yl >= P.Ny - P.nnegy) return yl - P.Ny;
+        return DEAD_ROW;
+    }
     if (P.lag_order == LAG_RASTER) return yl < P.nposy ? yl : yl - P.nposy - P.nnegy;
     return yl - P.nnegy;
 }
@@ -487,13 +495,13 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int y = t + TPF * m;
-        a[m] = (kv && y < P.h1) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
     }
     fft<-1, NY>(a, t, L, tw);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int y = t + TPF * m;
-        b[m] = (kv && y < P.h2) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
     }
     __syncthreads();
     fft<-1, NY>(b, t, L, tw);
@@ -502,6 +510,16 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
         cpx<T> p;
         p.x = b[m].x * a[m].x + b[m].y * a[m].y;
         p.y = b[m].y * a[m].x - b[m].x * a[m].y;
+        if (P.filter) {
+            // CorrPatchToImage's low-pass (Correlation.cpp:1255-1283): with i the index
+            // into the [N][N/2+1] half spectrum, "x" = i / (N/2+1) is the row index.
+            int fx = t + TPF * m;
+            if (fx >= P.Ny / 2) fx -= P.Ny;
+            const double rad2 = ((double)fx * fx + (double)k * k) / 10000.0;
+            const T filt = rad2 > 10.0 ? (T)0 : (T)exp(-rad2);
+            p.x = p.x * filt;
+            p.y = p.y * filt;
+        }
         if constexpr (WHITEN) {
             T mag = hypot(p.x, p.y);
             if (mag > (T)1e-10) {
@@ -576,7 +594,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 
     // ---- evaluation ----
     const int y = lag_y_of_row(P, valid ? yl : 0);
-    const RowBox rb = row_box(P, y);
+    const RowBox rb = row_box(P, y == DEAD_ROW ? 0 : y);
     const bool centred = P.lag_order == LAG_CENTRED;
     const int dyd = y - P.Oy;
     const long long rsqr = (long long)P.radius * P.radius;
@@ -600,7 +618,14 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
             bool cand = false;
             bool dead;
             const int x = lag_x_of_raster(P, ix, &dead);
-            if (valid && !dead) {
+            if (P.mode == PM_P2I) {
+                // CorrPatchToImage scan (Correlation.cpp:1299-1327): raw surface, zones, disc
+                val = raw;
+                if (valid && !dead && y != DEAD_ROW) {
+                    const int dxd = x - P.Ox;
+                    cand = (long long)dxd * dxd + (long long)dyd * dyd <= rsqr;
+                }
+            } else if (valid && !dead) {
                 if (!centred) {
                     // CorrPatches scan: disc about (Ox, Oy), then the rules
                     const int dxd = x - P.Ox;
@@ -748,7 +773,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
     __shared__ int sf[256];
     const PairDev &P = pairs[blockIdx.x];
     const int tid = threadIdx.x;
-    const int wstride = P.lag_order == LAG_RASTER ? P.Nx : P.wR;
+    const int wstride = P.lag_order == LAG_CENTRED ? P.wR : P.Nx;
     double bv = -1.0E300, s1 = 0.0, s2 = 0.0;
     long long bk = 0x7fffffffffffffffLL;
     int fl = 0;
@@ -812,7 +837,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
         const RowRec rr = P.rows[yl];
         const int y = lag_y_of_row(P, yl);
         int x;
-        if (P.lag_order == LAG_RASTER) {
+        if (P.lag_order != LAG_CENTRED) {
             bool dead;
             x = lag_x_of_raster(P, kin, &dead);
         } else {
@@ -820,7 +845,27 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
         }
         double sx = 0.0, sy = 0.0;
         const double z1 = best;
-        if (P.mode == PM_NCC) {
+        if (P.mode == PM_P2I) {
+            // biggest /= norm; exit(44) above 1.0001; ParabPeakFFT on the raw surface
+            const double r = best / P.norm;
+            if (x - 1 >= 0 && x + 1 < P.Nx_ref && rr.left != 0.0 && rr.right != 0.0) {
+                const double d = 1.0 * (rr.left - rr.right) / (2 * (rr.left + rr.right - z1 - z1));
+                if (fabs(d) <= 8) sx = d;
+            }
+            if (y - 1 >= 0 && y + 1 < P.Ny_ref) {
+                const double z0 = point_c2r<T>(P, y - 1, x, tw, sv);
+                const double z2 = point_c2r<T>(P, y + 1, x, tw, sv);
+                if (z0 != 0.0 && z2 != 0.0) {
+                    const double d = 1.0 * (z0 - z2) / (2 * (z0 + z2 - z1 - z1));
+                    if (fabs(d) <= 8) sy = d;
+                }
+            }
+            res.status = r > 1.0001 ? XC_E_ODD_R : XC_OK;
+            res.r = r;
+            res.peak = best;
+            res.dx = x + sx;
+            res.dy = y + sy;
+        } else if (P.mode == PM_NCC) {
             // ParabPeakFFT (Correlation.cpp:482-517) on the scanned R image
             if (x - 1 >= 0 && x + 1 < P.Nx_ref && rr.left != 0.0 && rr.right != 0.0) {
                 const double z0 = rr.left, z2 = rr.right;

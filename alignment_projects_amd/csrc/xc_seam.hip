@@ -1,0 +1,347 @@
+// xc_seam.hip -- the FFT seam (FFT_2D / IFT_2D, Correlation_fft_fftw.cpp:10-92)
+// and CorrPatchToImage (Correlation.cpp:1186-1355) in FP64 on the device.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "xc_exec.hpp"
+#include "xc_host.hpp"
+#include "xc_kernels.hpp"
+
+namespace xc {
+
+// Row r2c of a dense real array in[rows][2*NH] into out[rows][NH+1].
+template <typename T, int NH>
+__global__ void __launch_bounds__(RowCfg<NH>::THREADS)
+    k_seam_r2c(const T *__restrict__ in, int rows, cpx<T> *__restrict__ out, const cpx<T> *__restrict__ tw)
+{
+    using PL = Plan<NH>;
+    constexpr int E = PL::E, TPF = PL::TPF, G = RowCfg<NH>::G;
+    __shared__ cpx<T> smem[G * NH];
+    const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
+    const int y = blockIdx.x * G + g;
+    const bool valid = y < rows;
+    const T *src = in + (long long)y * 2 * NH;
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int p = t + TPF * m;
+        v[m].x = valid ? src[2 * p] : (T)0;
+        v[m].y = valid ? src[2 * p + 1] : (T)0;
+    }
+    LdsView<T> L{smem + g * NH, 1};
+    fft<-1, NH>(v, t, L, tw);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) L[t + TPF * m] = v[m];
+    __syncthreads();
+    if (valid) {
+        cpx<T> *row = out + (long long)y * (NH + 1);
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int k = t + TPF * m;
+            const cpx<T> zk = v[m];
+            const cpx<T> zn = cconj(L[(NH - k) & (NH - 1)]);
+            const cpx<T> a = cscale(zk + zn, (T)0.5);
+            const cpx<T> b = rot90<-1>(cscale(zk - zn, (T)0.5));
+            row[k] = a + cmul(twiddle<-1>(tw, k, 2 * NH), b);
+        }
+        if (t == 0) row[NH] = cpx<T>{v[0].x - v[0].y, (T)0};
+    }
+}
+
+// In-place column transform of data[NY][H] (DIR -1 forward, +1 inverse).
+template <typename T, int NY, int DIR>
+__global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
+    k_seam_col(cpx<T> *__restrict__ data, int H, const cpx<T> *__restrict__ tw)
+{
+    using PL = Plan<NY>;
+    constexpr int E = PL::E, TPF = PL::TPF, C = ColCfg<T, NY>::C;
+    __shared__ cpx<T> smem[C * NY];
+    const int tid = threadIdx.x, c = tid % C, t = tid / C;
+    const int k = blockIdx.x * C + c;
+    const bool kv = k < H;
+    LdsView<T> L{smem + c, C};
+    cpx<T> a[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) a[m] = kv ? data[(long long)(t + TPF * m) * H + k] : cpx<T>{(T)0, (T)0};
+    fft<DIR, NY>(a, t, L, tw);
+    if (kv) {
+#pragma unroll
+        for (int m = 0; m < E; ++m) data[(long long)(t + TPF * m) * H + k] = a[m];
+    }
+}
+
+// Row c2r of in[rows][NH+1] into out[rows][2*NH] (unnormalised).
+template <typename T, int NH>
+__global__ void __launch_bounds__(RowCfg<NH>::THREADS)
+    k_seam_c2r(const cpx<T> *__restrict__ in, int rows, T *__restrict__ out, const cpx<T> *__restrict__ tw)
+{
+    using PL = Plan<NH>;
+    constexpr int E = PL::E, TPF = PL::TPF, G = RowCfg<NH>::G;
+    __shared__ cpx<T> smem[G * (NH + 1)];
+    const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
+    const int y = blockIdx.x * G + g;
+    const bool valid = y < rows;
+    cpx<T> *X = smem + g * (NH + 1);
+    for (int i = t; i < NH + 1; i += TPF) X[i] = valid ? in[(long long)y * (NH + 1) + i] : cpx<T>{(T)0, (T)0};
+    __syncthreads();
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int p = t + TPF * m;
+        const cpx<T> xk = X[p], xn = cconj(X[NH - p]);
+        v[m] = (xk + xn) + rot90<+1>(cmul(twiddle<+1>(tw, p, 2 * NH), xk - xn));
+    }
+    LdsView<T> L{X, 1};
+    fft<+1, NH>(v, t, L, tw);
+    if (valid) {
+        T *dst = out + (long long)y * 2 * NH;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int p = t + TPF * m;
+            dst[2 * p] = v[m].x;
+            dst[2 * p + 1] = v[m].y;
+        }
+    }
+}
+
+// CorrPatchToImage's raster of patch1: i1[x + N*y] = iv1[i] at the points'
+// truncated coordinates, later points overwriting earlier ones.
+__global__ void k_p2i_claim(const xc_point *__restrict__ pts, int n, int N, int *__restrict__ owner)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)pts[i].x, y = (int)pts[i].y;
+    if (x >= 0 && x < N && y >= 0 && y < N) atomicMax(&owner[x + (long long)N * y], i);
+}
+
+__global__ void k_p2i_assign(const xc_point *__restrict__ pts, const double *__restrict__ v, int n, int N,
+                             const int *__restrict__ owner, double *__restrict__ img)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)pts[i].x, y = (int)pts[i].y;
+    if (x >= 0 && x < N && y >= 0 && y < N && owner[x + (long long)N * y] == i) img[x + (long long)N * y] = v[i];
+}
+
+#define XC_POW2_CASES(F) \
+    F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048)
+
+static int seam_r2c(xc_ctx *ctx, int NH, const double *in, int rows, cpx<double> *out)
+{
+    switch (NH) {
+#define F(n)                                                                                                \
+    case n:                                                                                                 \
+        hipLaunchKernelGGL((k_seam_r2c<double, n>), dim3((rows + RowCfg<n>::G - 1) / RowCfg<n>::G),          \
+                           dim3(RowCfg<n>::THREADS), 0, ctx->stream, in, rows, out, ctx->tw64);             \
+        return XC_OK;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "row length %d unsupported", 2 * NH);
+}
+
+static int seam_c2r(xc_ctx *ctx, int NH, const cpx<double> *in, int rows, double *out)
+{
+    switch (NH) {
+#define F(n)                                                                                                \
+    case n:                                                                                                 \
+        hipLaunchKernelGGL((k_seam_c2r<double, n>), dim3((rows + RowCfg<n>::G - 1) / RowCfg<n>::G),          \
+                           dim3(RowCfg<n>::THREADS), 0, ctx->stream, in, rows, out, ctx->tw64);             \
+        return XC_OK;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "row length %d unsupported", 2 * NH);
+}
+
+template <int DIR> static int seam_col(xc_ctx *ctx, int NY, cpx<double> *data, int H)
+{
+    switch (NY) {
+#define F(n)                                                                                                     \
+    case n:                                                                                                      \
+        hipLaunchKernelGGL((k_seam_col<double, n, DIR>), dim3((H + ColCfg<double, n>::C - 1) / ColCfg<double, n>::C), \
+                           dim3(ColCfg<double, n>::THREADS), 0, ctx->stream, data, H, ctx->tw64);                \
+        return XC_OK;
+        XC_POW2_CASES(F) F(4096)
+#undef F
+    }
+    return set_err(ctx, XC_E_SHAPE, "column length %d unsupported", NY);
+}
+
+static bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+}  // namespace xc
+
+using namespace xc;
+
+extern "C" int xc_fft2d_r2c_f64(xc_ctx *ctx, const double *in, int Nfast, int Nslow, double *out, int *M_out)
+{
+    if (!ctx || !in || !out) return XC_E_ARG;
+    if (!pow2(Nfast) || Nfast < 2 || !pow2(Nslow) || Nfast / 2 > max_nh<double>() || Nslow > max_ny<double>())
+        return set_err(ctx, XC_E_SHAPE, "FFT_2D: %d x %d (powers of two, Nfast >= 2, <= %d x %d)", Nfast, Nslow,
+                       2 * max_nh<double>(), max_ny<double>());
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    const int NH = Nfast / 2, H = NH + 1;
+    const size_t nin = (size_t)Nfast * Nslow, M = (size_t)Nslow * H;
+    Layout lay;
+    size_t i_in = lay.add(sizeof(double) * nin), i_out = lay.add(sizeof(cpx<double>) * M);
+    int st = ensure_dev(ctx, ctx->arena, lay.total);
+    if (st) return st;
+    char *base = (char *)ctx->arena.p;
+    double *din = (double *)(base + lay.off[i_in]);
+    cpx<double> *dout = (cpx<double> *)(base + lay.off[i_out]);
+    XC_HIP(ctx, hipMemcpyAsync(din, in, sizeof(double) * nin, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = seam_r2c(ctx, NH, din, Nslow, dout))) return st;
+    if ((st = seam_col<-1>(ctx, Nslow, dout, H))) return st;
+    XC_HIP(ctx, hipGetLastError());
+    XC_HIP(ctx, hipMemcpyAsync(out, dout, sizeof(cpx<double>) * M, hipMemcpyDeviceToHost, ctx->stream));
+    XC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (M_out) *M_out = (int)M;
+    return XC_OK;
+}
+
+extern "C" int xc_ift2d_c2r_f64(xc_ctx *ctx, const double *in, int Nfast, int Nslow, double *out)
+{
+    if (!ctx || !in || !out) return XC_E_ARG;
+    if (!pow2(Nfast) || Nfast < 2 || !pow2(Nslow) || Nfast / 2 > max_nh<double>() || Nslow > max_ny<double>())
+        return set_err(ctx, XC_E_SHAPE, "IFT_2D: %d x %d unsupported", Nfast, Nslow);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    const int NH = Nfast / 2, H = NH + 1;
+    const size_t nout = (size_t)Nfast * Nslow, M = (size_t)Nslow * H;
+    Layout lay;
+    size_t i_in = lay.add(sizeof(cpx<double>) * M), i_out = lay.add(sizeof(double) * nout);
+    int st = ensure_dev(ctx, ctx->arena, lay.total);
+    if (st) return st;
+    char *base = (char *)ctx->arena.p;
+    cpx<double> *din = (cpx<double> *)(base + lay.off[i_in]);
+    double *dout = (double *)(base + lay.off[i_out]);
+    XC_HIP(ctx, hipMemcpyAsync(din, in, sizeof(cpx<double>) * M, hipMemcpyHostToDevice, ctx->stream));
+    if ((st = seam_col<+1>(ctx, Nslow, din, H))) return st;
+    if ((st = seam_c2r(ctx, NH, din, Nslow, dout))) return st;
+    XC_HIP(ctx, hipGetLastError());
+    XC_HIP(ctx, hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, ctx->stream));
+    XC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return XC_OK;
+}
+
+extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, const double *iv1, int n1,
+                                          const double *i2, int n_i2, int Ox, int Oy, int radius, int bFilter,
+                                          double *dx, double *dy, double *r, xc_result *res)
+{
+    if (!ctx || !ip1 || !iv1 || !i2 || n1 <= 0 || n_i2 <= 0) return set_err(ctx, XC_E_ARG, "null or empty input");
+    const int N = (int)sqrt((double)n_i2);
+    if (!pow2(N) || N < 2 || N / 2 > max_nh<double>() || N > max_ny<double>())
+        return set_err(ctx, XC_E_SHAPE, "CorrPatchToImage: N = %d must be a power of two in [2, %d]", N,
+                       2 * max_nh<double>());
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    IBox B1;
+    bbox_from_points(B1, ip1, n1);
+    const int w1 = B1.R - B1.L + 1, h1 = B1.T - B1.B + 1;
+    // effective extent of the caller-embedded image (Correlation.cpp:1215-1233)
+    int w2 = 0, h2 = 0;
+    for (int i = 0; i < n_i2; ++i)
+        if (i2[i] != 0.0) {
+            const int y = i / N, x = i - N * y;
+            h2 = std::max(h2, y);
+            w2 = std::max(w2, x);
+        }
+    ++w2;
+    ++h2;
+
+    PairDev P;
+    memset(&P, 0, sizeof P);
+    setup_pair_geometry(P, w1, h1, w2, h2, PM_P2I);
+    P.Nx = P.Nx_ref = N;
+    P.Ny = P.Ny_ref = N;
+    P.lag_order = LAG_ALL;
+    P.hR = N;
+    P.Ox = Ox;
+    P.Oy = Oy;
+    P.radius = radius;
+    P.filter = bFilter ? 1 : 0;
+    P.norm = (double)n1 * n_i2;
+    const int NH = N / 2, H = NH + 1, G = row_group<double>(NH), C = col_group<double>(N);
+    const int nr2c = 2 * ((N + G - 1) / G), ncol = (H + C - 1) / C, nc2r = (N + G - 1) / G;
+
+    Layout lay;
+    size_t i_pts = lay.add(sizeof(xc_point) * n1), i_vals = lay.add(sizeof(double) * n1);
+    size_t i_i2 = lay.add(sizeof(double) * n_i2), i_i1 = lay.add(sizeof(double) * n_i2);
+    size_t i_own = lay.add(sizeof(int) * n_i2), i_msd = lay.add(sizeof(double) * 2);
+    size_t i_spA = lay.add(sizeof(cpx<double>) * N * H), i_spB = lay.add(sizeof(cpx<double>) * N * H);
+    size_t i_lag = lay.add(sizeof(cpx<double>) * N * H), i_rows = lay.add(sizeof(RowRec) * N);
+    size_t i_res = lay.add(sizeof(xc_result)), i_pair = lay.add(sizeof(PairDev));
+    size_t i_wr = lay.add(sizeof(int4) * nr2c), i_wc = lay.add(sizeof(int2) * ncol), i_wc2 = lay.add(sizeof(int2) * nc2r);
+    int st = ensure_dev(ctx, ctx->arena, lay.total);
+    if (st) return st;
+    char *base = (char *)ctx->arena.p;
+    auto at = [&](size_t i) { return base + lay.off[i]; };
+    for (int k = 0; k < 2; ++k) {
+        ImgView &V = k ? P.B : P.A;
+        V.data = at(k ? i_i2 : i_i1);
+        V.dtype = XC_F64;
+        V.w = N;
+        V.h = N;
+        V.stride = N;
+        V.mstride = N;
+        V.msd = (const double *)at(i_msd);
+    }
+    P.specA = at(i_spA);
+    P.specB = at(i_spB);
+    P.lag = at(i_lag);
+    P.rows = (RowRec *)at(i_rows);
+    P.out = at(i_res);
+
+    const size_t up_lo = lay.off[i_pair], up_hi = lay.total;
+    st = ensure_pinned(ctx, (up_hi - up_lo) + 64 + sizeof(xc_result));
+    if (st) return st;
+    char *hs = (char *)ctx->pinned;
+    auto hp = [&](size_t i) { return hs + (lay.off[i] - up_lo); };
+    memcpy(hp(i_pair), &P, sizeof P);
+    int4 *wr = (int4 *)hp(i_wr);
+    int k = 0;
+    for (int img = 0; img < 2; ++img)
+        for (int y = 0; y < N; y += G) wr[k++] = make_int4(0, img, y, 0);
+    int2 *wc = (int2 *)hp(i_wc);
+    for (int i = 0; i < ncol; ++i) wc[i] = make_int2(0, i * C);
+    int2 *wc2 = (int2 *)hp(i_wc2);
+    for (int i = 0; i < nc2r; ++i) wc2[i] = make_int2(0, i * G);
+    xc_result *hres = (xc_result *)(((uintptr_t)(hs + (up_hi - up_lo)) + 63) & ~(uintptr_t)63);
+
+    hipStream_t s = ctx->stream;
+    const double msd[2] = {0.0, 1.0};
+    XC_HIP(ctx, hipMemcpyAsync(base + up_lo, hs, up_hi - up_lo, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_msd), msd, sizeof msd, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_pts), ip1, sizeof(xc_point) * n1, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_vals), iv1, sizeof(double) * n1, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_i2), i2, sizeof(double) * n_i2, hipMemcpyHostToDevice, s));
+    const int fb = (int)std::min<long long>(1024, ((long long)n_i2 + 255) / 256);
+    hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)at(i_i1), (long long)n_i2, 0.0);
+    hipLaunchKernelGGL(k_fill<int>, dim3(fb), dim3(256), 0, s, (int *)at(i_own), (long long)n_i2, -1);
+    hipLaunchKernelGGL(k_p2i_claim, dim3((n1 + 255) / 256), dim3(256), 0, s, (const xc_point *)at(i_pts), n1, N,
+                       (int *)at(i_own));
+    hipLaunchKernelGGL(k_p2i_assign, dim3((n1 + 255) / 256), dim3(256), 0, s, (const xc_point *)at(i_pts),
+                       (const double *)at(i_vals), n1, N, (const int *)at(i_own), (double *)at(i_i1));
+    XC_HIP(ctx, hipGetLastError());
+    const PairDev *dP = (const PairDev *)at(i_pair);
+    if ((st = launch_row_r2c<double>(ctx, NH, nr2c, dP, (const int4 *)at(i_wr)))) return st;
+    if ((st = launch_col<double>(ctx, N, false, ncol, dP, (const int2 *)at(i_wc)))) return st;
+    if ((st = launch_row_c2r<double>(ctx, NH, nc2r, dP, (const int2 *)at(i_wc2)))) return st;
+    if ((st = launch_finalize<double>(ctx, 1, dP))) return st;
+    XC_HIP(ctx, hipMemcpyAsync(hres, at(i_res), sizeof(xc_result), hipMemcpyDeviceToHost, s));
+    XC_HIP(ctx, hipStreamSynchronize(s));
+    xc_result R = *hres;
+    if (R.status == XC_OK) {
+        if (dx) *dx = R.dx;
+        if (dy) *dy = R.dy;
+        if (r) *r = R.r;
+    }
+    if (res) *res = R;
+    return R.status;
+}

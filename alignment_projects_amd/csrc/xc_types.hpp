@@ -11,9 +11,11 @@ namespace xc {
 //           first, then the negative zone), Correlation.cpp:1017-1070.
 //  CENTRED: CCorImg's centred image, signed y ascending from -nnegy,
 //           Correlation.cpp:2141-2161.
-enum LagOrder { LAG_RASTER = 0, LAG_CENTRED = 1 };
+enum LagOrder { LAG_RASTER = 0, LAG_CENTRED = 1, LAG_ALL = 2 };  // LAG_ALL: every raster row stored
 
-enum PairMode { PM_NCC = 0, PM_R = 1, PM_S = 2 };
+// PM_P2I: CorrPatchToImage (Correlation.cpp:1186-1355), raw correlation
+// surface at N x N without zero padding.
+enum PairMode { PM_NCC = 0, PM_R = 1, PM_S = 2, PM_P2I = 3 };
 
 struct RuleDev {
     int32_t rgn_enabled, min_w, min_h, exact_w, exact_h, or_w_ge, or_h_gt, table_w, table_h;
@@ -81,7 +83,8 @@ struct PairDev {
     int32_t offx, offy;      // B2.L - B1.L, B2.B - B1.B
     int32_t nnegx, nposx, nnegy, nposy, wR, hR;
     int32_t need_cnt;        // count predicate active (nz tables needed)
-    int32_t pad0;
+    int32_t filter;          // PM_P2I: Gaussian low-pass of the cross power (bFilter)
+    double norm;             // PM_P2I: np1 * N^2
     double mincor;
     RuleDev rule;
     ImgView A, B;                     // source images

@@ -154,6 +154,37 @@ class Context:
         return st, r.value, dx.value, dy.value, res
 
 
+    def fft_2d(self, a):
+        """FFT_2D: a[Nslow][Nfast] real -> [Nslow][Nfast/2+1] complex, unnormalised."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        Ny, Nx = a.shape
+        out = np.zeros((Ny, Nx // 2 + 1), np.complex128)
+        st = self.L.xc_fft2d_r2c_f64(self.h, a.ctypes.data_as(_dp), Nx, Ny, out.ctypes.data_as(_dp), None)
+        self._check(st, "xc_fft2d_r2c_f64", ok=(abi.XC_OK,))
+        return out
+
+    def ift_2d(self, f, Nfast):
+        """IFT_2D: unnormalised c2r of f[Nslow][Nfast/2+1]."""
+        f = np.ascontiguousarray(f, dtype=np.complex128)
+        Ny = f.shape[0]
+        out = np.zeros((Ny, Nfast))
+        st = self.L.xc_ift2d_c2r_f64(self.h, f.ctypes.data_as(_dp), Nfast, Ny, out.ctypes.data_as(_dp))
+        self._check(st, "xc_ift2d_c2r_f64", ok=(abi.XC_OK,))
+        return out
+
+    def corr_patch_to_image(self, ip1, iv1, i2, Ox=0, Oy=0, radius=4096, bfilter=False):
+        """CorrPatchToImage. Returns (status, r, dx, dy, XcResult)."""
+        p1, pp1 = _pts(ip1)
+        v1, pv1 = _vals(iv1)
+        i2 = np.ascontiguousarray(i2, dtype=np.float64).ravel()
+        dx, dy, r = C.c_double(0), C.c_double(0), C.c_double(0)
+        res = XcResult()
+        st = self.L.xc_corr_patch_to_image_f64(self.h, pp1, pv1, len(v1), i2.ctypes.data_as(_dp), i2.size, Ox, Oy,
+                                               radius, int(bool(bfilter)), C.byref(dx), C.byref(dy), C.byref(r),
+                                               C.byref(res))
+        self._check(st, "xc_corr_patch_to_image_f64", ok=(abi.XC_OK, abi.XC_NO_PEAK, abi.XC_E_ODD_R))
+        return st, r.value, dx.value, dy.value, res
+
     # ---- tiles and batched plans ----
 
     def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0):

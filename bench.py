@@ -1,0 +1,191 @@
+"""Benchmark: batched FP32 tile-pair correlations/s on one MI355X.
+
+Workload (default): 256 pairs of 2048 x 256 uint8 strips (band-limited noise,
+per-pair seeds, fractional shifts by bilinear resampling, N(0,10) noise),
+correlated with CorrPatches semantics (normalised cross-correlation, FP32 FFTs
+at 4096 x 512, FP64 summed-area tables and variance terms, BigEnough-style
+legal-overlap rule).  One step = one pass of the whole path over the batch,
+starting from the raw uint8 strips resident in HBM: statistics, summed-area
+tables, row r2c, fused column pass, row c2r + NCC + peak search, subpixel fit.
+
+Printed: one JSON line with the throughput, the plain FP64 GPU path on the
+same pairs (one pair per call), accuracy against the FP64 oracle, the
+roofline of the dominant kernel (HIP events on the library stream) and a CPU
+baseline (the oracle on the host cores, FFTW3 API, on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+STAGES = ["prep", "row_r2c", "col_fused", "row_c2r_eval", "finalize"]
+KERNELS = {"prep": "k_tp_*", "row_r2c": "k_row_r2c", "col_fused": "k_col_fused",
+           "row_c2r_eval": "k_row_c2r_eval", "finalize": "k_finalize"}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--width", type=int, default=2048)
+    ap.add_argument("--height", type=int, default=256)
+    ap.add_argument("--oracle-sample", type=int, default=16, help="pairs checked against the FP64 oracle")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="pairs timed on the CPU baseline")
+    ap.add_argument("--plain-sample", type=int, default=64, help="pairs run through the plain FP64 GPU path")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stage-iters", type=int, default=5)
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
+              "pairs shard independently across GPUs, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from alignment_projects_amd import abi, synth, xcorr
+
+    W, H, NP = args.width, args.height, args.pairs
+    t0 = time.time()
+    A, B, S = synth.strip_batch(NP, seed0=1000, w=W, h=H)
+    log(f"generated {NP} strip pairs {W}x{H} in {time.time() - t0:.1f}s")
+    rgn = abi.rgn_big_enough(H // 2, W * H // 4)
+
+    ctx = xcorr.Context(0)
+    tiles = [(ctx.tile_put(A[i]), ctx.tile_put(B[i])) for i in range(NP)]
+    pairs = [xcorr.make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
+    plan = ctx.plan(pairs, refresh_tiles=True)
+    log("plan ready")
+
+    # ---- warmup + timed steps ----
+    for _ in range(args.warmup):
+        plan.run(want=False)
+    lat = []
+    import torch
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        plan.run(want=False)
+        lat.append(time.perf_counter() - t1)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    res = plan.run(want=True)
+    value = NP * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    log(f"batched: {value:.1f} pairs/s, {ms_step:.3f} ms/step")
+
+    # ---- stage timing (HIP events on the library stream) and roofline ----
+    ms, by = plan.stage_times(args.stage_iters)
+    model = plan.model_bytes
+    dom = int(np.argmax(ms))
+    ach = by[dom] / (ms[dom] * 1e-3) / 1e9
+    stages = {STAGES[i]: {"ms": round(ms[i], 4), "alg_bytes": by[i],
+                          "GBs": round(by[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None}
+              for i in range(5)}
+    log("stages:", json.dumps(stages))
+    roofline = {"bound": "hbm", "kernel": KERNELS[STAGES[dom]], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": by[dom],
+                "pipeline": {"alg_bytes_per_step": model,
+                             "achieved_GBs": round(model / (ms_step * 1e-3) / 1e9, 1),
+                             "frac": round(model / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
+
+    # ---- plain FP64 GPU path on the same pairs (one pair per call) ----
+    nplain = min(args.plain_sample, NP)
+    pts = [(synth.raster_points(A[i]), synth.raster_points(B[i])) for i in range(nplain)]
+    norm = [((pa, synth.normalize(va)), (pb, synth.normalize(vb))) for (pa, va), (pb, vb) in pts]
+    ctx.corr_patches(norm[0][0][0], norm[0][0][1], norm[0][1][0], norm[0][1][1], 0, 0, 4096, rgn=rgn)
+    t1 = time.perf_counter()
+    plain = []
+    for (pa, va), (pb, vb) in norm:
+        plain.append(ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn))
+    plain_s = time.perf_counter() - t1
+    plain_rate = nplain / plain_s
+    log(f"plain FP64 GPU: {plain_rate:.1f} pairs/s")
+
+    def acc(ref):
+        agree, sub, rerr = 0, [], []
+        for i, o in enumerate(ref):
+            r = res[i]
+            if (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y):
+                agree += 1
+                sub.append(max(abs(r.dx - o[2]), abs(r.dy - o[3])))
+            rerr.append(abs(r.r - o[1]))
+        sub = np.array(sub) if sub else np.array([np.nan])
+        rerr = np.array(rerr)
+        return {"n": len(ref), "peak_agree": agree / len(ref), "subpx_max": float(sub.max()),
+                "subpx_p99": float(np.percentile(sub, 99)), "subpx_mean": float(sub.mean()),
+                "r_err_max": float(rerr.max()), "r_err_p99": float(np.percentile(rerr, 99))}
+
+    accuracy = {"vs_plain_fp64_gpu": acc(plain)}
+
+    # ---- FP64 oracle on a sample (checker) and CPU baseline ----
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        nor = min(args.oracle_sample, NP)
+        ref = [oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn) for (pa, va), (pb, vb) in norm[:nor]]
+        accuracy["vs_fp64_oracle"] = acc(ref)
+        truth = [float(np.hypot(res[i].dx - S[i][0], res[i].dy - S[i][1])) for i in range(NP)]
+        accuracy["vs_true_shift_px"] = {"max": max(truth), "mean": float(np.mean(truth))}
+        try:
+            prov = oracle.set_fft_provider(1)
+            kind = "port"
+        except RuntimeError:
+            prov = oracle.set_fft_provider(0)
+            kind = "port"
+        ncpu = min(args.cpu_sample, nplain)
+        oracle.corr_patches(norm[0][0][0], norm[0][0][1], norm[0][1][0], norm[0][1][1], 0, 0, 4096, rgn=rgn)
+        t1 = time.perf_counter()
+        for (pa, va), (pb, vb) in norm[:ncpu]:
+            oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+        cs = time.perf_counter() - t1
+        oracle.set_fft_provider(0)
+        cpu = {"value": round(ncpu / cs, 3), "unit": "pairs/s", "cores": 1, "kind": kind,
+               "sample": f"{ncpu} of the {NP} strip pairs, CorrPatches restatement, FFT: {prov}, 1 thread",
+               "seconds": round(cs, 2)}
+        log(f"cpu baseline: {cpu['value']} pairs/s")
+
+    line = {
+        "metric": "tile-pair correlations/sec (batched FP32, 2048x256 strip pairs)",
+        "value": round(value, 1), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched same-layer strip overlaps", "pairs": NP, "tile": f"{W}x{H} u8",
+                   "fft": f"{2 * W}x{2 * H}", "tables": "f64", "rule": f"BigEnough({H // 2}, {W * H // 4})"},
+        "latency_ms": {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
+                       "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"},
+        "plain_fp64_gpu": {"value": round(plain_rate, 1), "unit": "pairs/s", "pairs": nplain,
+                           "batched_over_plain": round(value / plain_rate, 1)},
+        "stages": stages,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "accuracy": accuracy,
+    }
+    if cpu:
+        line["batched_over_cpu"] = round(value / cpu["value"], 1)
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

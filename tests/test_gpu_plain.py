@@ -111,3 +111,37 @@ def test_corr_images_random(ctx, mode, seed):
         yy, xx = np.nonzero(A & (F >= peak - RTOL * max(1.0, abs(peak))))
         ties = set(zip((xx - (w1 - 1)).tolist(), (yy - (h1 - 1)).tolist()))
     _same(g, o[:5], f"mode {mode} seed {seed}", ties)
+
+
+@pytest.mark.parametrize("shape", [(1, 2), (8, 16), (64, 32), (512, 1024), (256, 4096)])
+def test_fft_seam_vs_oracle_and_numpy(ctx, shape):
+    rng = np.random.default_rng(sum(shape))
+    a = rng.standard_normal(shape)
+    f = ctx.fft_2d(a)
+    ref = np.fft.rfft2(a)
+    assert np.abs(f - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert np.abs(f - oracle.fft2d_r2c(a)).max() <= 1e-12 * np.abs(ref).max()
+    b = ctx.ift_2d(f, shape[1])
+    assert np.abs(b - oracle.ift2d_c2r(f, shape[1])).max() <= 1e-11 * a.size
+
+
+@pytest.mark.parametrize("bfilter", [False, True])
+@pytest.mark.parametrize("seed", range(4))
+def test_corr_patch_to_image(ctx, seed, bfilter):
+    rng = np.random.default_rng(40 + seed)
+    N = [64, 128, 256, 64][seed]
+    img = np.zeros((N, N))
+    h2, w2 = int(rng.integers(N // 4, N // 2)), int(rng.integers(N // 4, N // 2))
+    img[:h2, :w2] = rng.standard_normal((h2, w2)) + 2.0
+    ph, pw = int(rng.integers(4, h2 // 2)), int(rng.integers(4, w2 // 2))
+    x0, y0 = int(rng.integers(0, w2 - pw)), int(rng.integers(0, h2 - ph))
+    patch = img[y0:y0 + ph, x0:x0 + pw] + 0.05 * rng.standard_normal((ph, pw))
+    pa, va = synth.raster_points(patch)
+    g = ctx.corr_patch_to_image(pa, va, img, 0, 0, N, bfilter)
+    o = oracle.corr_patch_to_image(pa, va, img, 0, 0, N, bfilter)
+    assert g[0] == o[0]
+    assert (g[4].lag_x, g[4].lag_y) == (o[4].lag_x, o[4].lag_y)
+    assert abs(g[1] - o[1]) <= 1e-9 * max(1.0, abs(o[1]))
+    assert abs(g[2] - o[2]) <= 1e-9 and abs(g[3] - o[3]) <= 1e-9
+    if not bfilter:
+        assert (g[4].lag_x, g[4].lag_y) == (x0, y0)

@@ -1,0 +1,281 @@
+"""Pins of the CPU oracle (no GPU).
+
+Each check is chosen so that a plausible mistake in the oracle -- a dropped
+term, a wrong sign or index, a transposed operand -- fails it:
+  * the reference's own harnesses: testnormcorr.cpp (known answer), time_fft.cpp
+    (planted circular shift), fft.cpp (FFT against a direct DFT);
+  * numpy.fft (pocketfft) for the 2-D transforms, Parseval, the shift theorem;
+  * brute-force NCC (direct box sums, no FFT, no tables) for CorrPatches and
+    CorrImagesR/S, including scan order, zones, disc, rules and parabola quirks;
+  * closed forms for the geometry helpers.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from alignment_projects_amd import abi, synth
+from tests import bruteforce as bf
+
+
+# ---------------------------------------------------------------------------
+# FFT
+# ---------------------------------------------------------------------------
+
+def test_fft1d_vs_direct_dft():
+    # Retired_Proj/4_FFT/fft.cpp:14-65 compares FFTW against a naive DFT at N = 1024
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 8, 64, 1024):
+        z = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+        k = np.arange(n)
+        dft = np.exp(-2j * np.pi * np.outer(k, k) / n) @ z
+        got = oracle.fft1d(z)
+        assert np.abs(got - dft).max() <= 1e-9 * max(1.0, np.abs(dft).max())
+        inv = oracle.fft1d(got, inverse=True) / n
+        assert np.abs(inv - z).max() <= 1e-12
+
+
+@pytest.mark.parametrize("shape", [(1, 2), (4, 8), (16, 32), (64, 128), (256, 64)])
+def test_fft2d_vs_numpy(shape):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(shape)
+    f = oracle.fft2d_r2c(a)
+    ref = np.fft.rfft2(a)
+    assert np.abs(f - ref).max() <= 1e-12 * np.abs(ref).max()
+    b = oracle.ift2d_c2r(f, shape[1])
+    assert np.abs(b - a * a.size).max() <= 1e-11 * a.size
+    # Parseval on the half spectrum
+    w = np.full(f.shape[1], 2.0)
+    w[0] = 1.0
+    if shape[1] % 2 == 0:
+        w[-1] = 1.0
+    assert abs((np.abs(f) ** 2 * w).sum() / a.size - (a * a).sum()) <= 1e-9 * (a * a).sum()
+
+
+def test_fft2d_shift_theorem():
+    a = np.zeros((16, 32))
+    a[3, 5] = 1.0
+    f = oracle.fft2d_r2c(a)
+    ky, kx = np.mgrid[0:16, 0:17]
+    expect = np.exp(-2j * np.pi * (ky * 3 / 16 + kx * 5 / 32))
+    assert np.abs(f - expect).max() < 1e-13
+
+
+def test_time_fft_planted_shift():
+    # Retired_Proj/4_Time_FFT/time_fft.cpp:29-62: in2[(i + 2N + 3) % N^2] = in1[i],
+    # out = IFT(F(in1) * conj(F(in2))), argmax j -> (j / N, j % N), wrapped above N/2.
+    N = 64
+    rng = np.random.default_rng(2)
+    in1 = rng.integers(0, 2 ** 31, N * N).astype(np.float64) / 1e9
+    in2 = np.zeros(N * N)
+    in2[(np.arange(N * N) + 2 * N + 3) % (N * N)] = in1
+    f1 = oracle.fft2d_r2c(in1.reshape(N, N))
+    f2 = oracle.fft2d_r2c(in2.reshape(N, N))
+    out = oracle.ift2d_c2r(f1 * np.conj(f2), N)
+    j = int(np.argmax(out))
+    x, y = j // N, j - N * (j // N)
+    x = x - N if x > N // 2 else x
+    y = y - N if y > N // 2 else y
+    assert (x, y) == (-2, -3)
+
+
+# ---------------------------------------------------------------------------
+# Geometry / maths helpers
+# ---------------------------------------------------------------------------
+
+def test_ceil_pow2_and_fft_size():
+    for n in range(0, 5000, 7):
+        p = oracle.ceil_pow2(n)
+        assert p >= n and p & (p - 1) == 0 and (p == 1 or p // 2 < n)
+    assert oracle.fft_size(512, 512) == 1024 and oracle.fft_size(2048, 2048) == 4096
+
+
+def test_bbox_and_boxes():
+    pts = np.array([[2.5, -1.0], [7.0, 3.2], [-0.5, 0.0]])
+    assert oracle.bbox(pts) == (-1, 7, -1, 4)
+    for w1, h1, w2, h2 in [(5, 4, 7, 3), (3, 3, 3, 3)]:
+        for x in range(-w1 + 1, w2):
+            for y in range(-h1 + 1, h2):
+                b1, b2 = oracle.boxes_from_shifts(w1, h1, w2, h2, x, y)
+                sa, sb = bf.overlap(w1, h1, w2, h2, x, y)
+                assert (b2[0], b2[1] + 1, b2[2], b2[3] + 1) == (sb[1].start, sb[1].stop, sb[0].start, sb[0].stop)
+                assert (b1[0], b1[1] + 1, b1[2], b1[3] + 1) == (sa[1].start, sa[1].stop, sa[0].start, sa[0].stop)
+
+
+def test_splat_weights():
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(1, 8, size=(40, 2))
+    vals = rng.standard_normal(40)
+    img = oracle.image_from_values_and_points(10, 10, vals, pts, 0, 0)
+    assert abs(img.sum() - vals.sum()) < 1e-12          # bilinear weights sum to 1
+    ip = np.array([[1, 2], [3, 4], [5, 6]], float)
+    img = oracle.image_from_values_and_points(8, 8, [1.0, 2.0, 3.0], ip, 0, 0)
+    expect = np.zeros((8, 8))
+    expect[2, 1], expect[4, 3], expect[6, 5] = 1, 2, 3
+    assert np.array_equal(img, expect)                 # integer points reduce to a copy
+    img = oracle.image_from_values_and_points(4, 4, [1.0], [[1.25, 2.5]], 0, 0)
+    assert np.allclose(img[2:4, 1:3], [[0.375, 0.125], [0.375, 0.125]])
+
+
+def test_stats_normalize_decimate():
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(1000) * 3 + 7
+    m, s = oracle.stats(v)
+    assert abs(m - v.mean()) < 1e-12 and abs(s - v.std(ddof=1)) < 1e-12
+    nv = oracle.normalize(v)
+    assert np.allclose(nv, (v - v.mean()) / v.std(ddof=1), atol=1e-12)
+    img = rng.standard_normal((16, 24))
+    p, vals = synth.raster_points(img)
+    dp, dv = oracle.decimate_vector(p, vals, 24, 16, 8)
+    assert np.allclose(dv.reshape(2, 3), img.reshape(2, 8, 3, 8).mean(axis=(1, 3)))
+    assert np.array_equal(dp[:, 0], np.tile(np.arange(3), 2)) and np.array_equal(dp[:, 1], np.repeat(np.arange(2), 3))
+
+
+def test_integrate_image_box_sums():
+    rng = np.random.default_rng(5)
+    img = rng.integers(-3, 4, size=(9, 13)).astype(float)
+    S, S2, nz = oracle.integrate_image(np.pad(img, ((0, 3), (0, 5))), 13, 9)
+    assert np.array_equal(S, img.cumsum(0).cumsum(1))
+    assert np.array_equal(S2, (img * img).cumsum(0).cumsum(1))
+    assert np.array_equal(nz, (img != 0).cumsum(0).cumsum(1))
+
+
+# ---------------------------------------------------------------------------
+# Correlators
+# ---------------------------------------------------------------------------
+
+def test_testnormcorr_kat():
+    # Retired_Proj/4_TestNormCorr/testnormcorr.cpp:20-63: "exact match with delta y = 2, delta x = -4"
+    pb = [(0, 3), (2, 3), (3, 2), (3, 3), (3, 4), (4, 3), (5, 4)]
+    vb = [1, 2, 1, 3, 1, 2, 1]
+    pa = [(7, 2), (8, 2), (9, 2), (7, 1), (8, 1)]
+    va = [1, 0, 1, 3, 2]
+    st, r, dx, dy, res = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=abi.rgn_some_area())
+    assert st == abi.XC_OK and abs(r - 1.0) < 1e-12
+    assert (res.lag_x, res.lag_y) == (3, 1) and round(dx) == -4 and dy == 2.0
+    # the same answer through the provider that speaks the FFTW3 API, if one is installed
+    try:
+        oracle.set_fft_provider(1)
+        st2, r2, dx2, dy2, _ = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=abi.rgn_some_area())
+        assert (st2, round(dx2), dy2) == (st, round(dx), dy) and abs(r2 - r) < 1e-12
+    except RuntimeError:
+        pass
+    finally:
+        oracle.set_fft_provider(0)
+
+
+def _rand_case(seed, lo=2, hi=14):
+    rng = np.random.default_rng(seed)
+    w1, h1, w2, h2 = (int(v) for v in rng.integers(lo, hi, size=4))
+    A = rng.standard_normal((h1, w1)) + 1.5
+    B = rng.standard_normal((h2, w2)) - 0.5
+    if seed % 4 == 3:      # holes: zero-valued absent pixels
+        A[rng.random(A.shape) < 0.2] = 0.0
+    x1, y1, x2, y2 = (int(v) for v in rng.integers(-4, 5, size=4))
+    return A, B, (x1, y1), (x2, y2)
+
+
+RULES = [(None, None), (abi.rgn_some_area(), None), (abi.rgn_min(2, 3, 7), abi.cnt_enough_points(3)),
+         (abi.rgn_exact(3, 2), None)]
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_corr_patches_vs_bruteforce(seed):
+    A, B, (x1, y1), (x2, y2) = _rand_case(seed)
+    pa, va = synth.raster_points(A, x0=x1, y0=y1)
+    pb, vb = synth.raster_points(B, x0=x2, y0=y2)
+    if seed % 4 == 3:
+        keep = va != 0
+        pa, va = pa[keep], va[keep]
+        A = bf.raster(pa, va, A.shape[1], A.shape[0], x1, y1)
+    rgn, cnt = RULES[seed % 4]
+    Ox, Oy, rad = [(0, 0, 4096), (1, -1, 4), (0, 2, 6), (0, 0, 100)][seed % 4]
+    b1, b2 = oracle.bbox(pa), oracle.bbox(pb)
+    A = bf.raster(pa, va, b1[1] - b1[0] + 1, b1[3] - b1[2] + 1, b1[0], b1[2])
+    B = bf.raster(pb, vb, b2[1] - b2[0] + 1, b2[3] - b2[2] + 1, b2[0], b2[2])
+    st, r, dx, dy, res, R = oracle.corr_patches(pa, va, pb, vb, Ox, Oy, rad, rgn=rgn, cnt=cnt, want_rslt=True)
+    bst, br, bdx, bdy, blag, BR = bf.corr_patches(A, B, b2[0] - b1[0], b2[2] - b1[2], Ox, Oy, rad, rgn, cnt)
+    assert st == bst
+    assert np.abs(R - BR).max() <= 1e-9          # the whole scanned R image, lag by lag
+    if st == abi.XC_OK:
+        ties = np.sum(BR >= br - 1e-9)
+        assert abs(r - br) <= 1e-9
+        if ties == 1:
+            assert (res.lag_x, res.lag_y) == blag
+            assert abs(dx - bdx) <= 1e-9 and abs(dy - bdy) <= 1e-9
+
+
+def test_bruteforce_detects_transposed_operands():
+    # The brute-force check is sensitive to an A/B swap (a transposed operand).
+    A, B, _, _ = _rand_case(101, lo=6, hi=12)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    _, _, _, _, _, R = oracle.corr_patches(pb, vb, pa, va, want_rslt=True)
+    _, _, _, _, _, BR = bf.corr_patches(A, B, 0, 0)
+    assert R.shape != BR.shape or np.abs(R - BR).max() > 1e-3
+
+
+def test_ncc_formula_equals_pearson():
+    rng = np.random.default_rng(6)
+    for _ in range(20):
+        a, b = rng.standard_normal((2, 5, 7)) * 4 + 9
+        assert abs(bf.ncc_direct(a, b) - bf.pearson(a, b)) < 1e-10
+
+
+@pytest.mark.parametrize("mode", [abi.XC_MODE_R, abi.XC_MODE_S])
+@pytest.mark.parametrize("seed", range(8))
+def test_corr_images_vs_bruteforce(mode, seed):
+    rng = np.random.default_rng(200 + seed)
+    w1, h1, w2, h2 = (int(v) for v in rng.integers(7, 16, size=4))
+    A = rng.standard_normal((h1, w1))
+    B = rng.standard_normal((h2, w2))
+    pa, va = synth.raster_points(A, x0=2)
+    pb, vb = synth.raster_points(B, y0=-3)
+    req = int(0.2 * min(w1 * h1, w2 * h2))
+    # overlaps of one or two pixels have r = +-1 up to rounding (a threshold tie of CalcR's
+    # clamp), so the rule-free cases still exclude them
+    rgn = abi.rgn_big_enough(3, req) if seed % 2 else abi.rgn_min(2, 2, 3)
+    cnt = abi.cnt_enough_points(req) if seed % 2 else None
+    Rx, Ry = (5, 3) if seed % 3 == 0 else (-1, -1)
+    st, r, dx, dy, res = oracle.corr_images(mode, pa, va, pb, vb, rgn=rgn, cnt=cnt, Ox=1, Oy=0, Rx=Rx, Ry=Ry)
+    bst, br, bdx, bdy, blag = bf.corr_images(mode, A, B, 0 - 2, -3 - 0, rgn, cnt, 1, 0, Rx, Ry)
+    assert st == bst
+    if st == abi.XC_OK:
+        assert (res.lag_x, res.lag_y) == blag
+        assert abs(r - br) <= 1e-9 * max(1, abs(br))
+        assert abs(dx - bdx) <= 1e-9 and abs(dy - bdy) <= 1e-9
+
+
+def test_corr_images_too_small():
+    st, r, dx, dy, res = oracle.corr_images(abi.XC_MODE_S, [(0, 0), (1, 0)], [1, 2], [(0, 0), (5, 5)], [1, 2])
+    assert st == abi.XC_TOO_SMALL and (r, dx, dy) == (0.0, 0.0, 0.0)
+
+
+def test_rfromangle_zero_is_corr_images_s():
+    # RFromAngle at 0 deg rotates by the identity (cos 0 = 1, sin 0 = 0): plain CorrImagesS.
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((20, 24))
+    B = np.roll(A, (2, -3), axis=(0, 1))
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    req = 120
+    st, R, X, Y, res = oracle.rfromangle(0.0, pa, va, pb, vb, 6, req)
+    st2, r2, dx2, dy2, _ = oracle.corr_images(abi.XC_MODE_S, pa, va, pb, vb, rgn=abi.rgn_big_enough(6, req),
+                                              cnt=abi.cnt_enough_points(req))
+    assert (st, R, X, Y) == (st2, r2, dx2, dy2)
+    assert (res.lag_x, res.lag_y) == (-3, 2)
+
+
+def test_rfromangle_rotation_matches_closed_form():
+    # Rotating A's points by +theta (TAffine::NUSetRot, pos rotates CW in image coordinates) and
+    # correlating against B = A rotated the same way recovers a zero shift.
+    rng = np.random.default_rng(8)
+    A = synth.bandlimited(rng, 40, 40, 0.0, 1.0)
+    pa, va = synth.raster_points(A)
+    deg = 3.0
+    c, s = np.cos(deg * 3.14159265358 / 180), np.sin(deg * 3.14159265358 / 180)
+    rot = np.stack([pa[:, 0] * c - pa[:, 1] * s, pa[:, 0] * s + pa[:, 1] * c], axis=1)
+    b = oracle.bbox(rot)
+    Bimg = oracle.image_from_values_and_points(b[1] - b[0] + 1, b[3] - b[2] + 1, va, rot, b[0], b[2])
+    pb, vb = synth.raster_points(Bimg, x0=b[0], y0=b[2])
+    st, R, X, Y, res = oracle.rfromangle(deg, pa, va, pb, vb, 8, 200)
+    assert st == abi.XC_OK and abs(X) < 0.5 and abs(Y) < 0.5
