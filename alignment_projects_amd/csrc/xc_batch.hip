@@ -26,8 +26,6 @@ struct xc_tile {
     DevBuf mem;                // tables + scratch + descriptor
     TileDev dev;               // host copy, device pointers
     TileDev *d_dev = nullptr;  // device copy
-    int2 *d_work = nullptr;    // (0, strip) items
-    int nwork = 0;
     int64_t bytes = 0;
 };
 
@@ -75,11 +73,13 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     t->flags = flags;
     const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
     const TileSizes ts = tile_sizes(w, h);
-    const int ns = nstrips_of(w);
+    if (w > TP_MAX_W) {
+        delete t;
+        return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
+    }
     Layout lay;
     size_t iS = lay.add(ts.S), iQ = lay.add(ts.Q), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
-    size_t iR = lay.add(ts.rowtot), iC = lay.add(ts.carry), iM = lay.add(ts.msd);
-    size_t iD = lay.add(sizeof(TileDev)), iW = lay.add(sizeof(int2) * ns);
+    size_t iM = lay.add(ts.msd), iD = lay.add(sizeof(TileDev)), iL = lay.add(sizeof(int));
     size_t iRaw = dev_src ? 0 : lay.add((size_t)es * stride * h);
     size_t iMask = (!dev_src && mask) ? lay.add((size_t)w * h) : 0;
     int st = ensure_dev(ctx, t->mem, lay.total);
@@ -122,17 +122,11 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.Q = (double *)(base + lay.off[iQ]);
     D.Z = (int32_t *)(base + lay.off[iZ]);
     D.part = (double *)(base + lay.off[iP]);
-    D.rowtot = (double *)(base + lay.off[iR]);
-    D.carry = (double *)(base + lay.off[iC]);
-    D.nstrips = ns;
     D.normalize = (flags & XC_TILE_NORMALIZE) ? 1 : 0;
     t->d_dev = (TileDev *)(base + lay.off[iD]);
-    t->d_work = (int2 *)(base + lay.off[iW]);
-    t->nwork = ns;
-    std::vector<int2> wk(ns);
-    for (int s = 0; s < ns; ++s) wk[s] = make_int2(0, s);
+    const int zero = 0;
     if (hipMemcpy(t->d_dev, &D, sizeof D, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(t->d_work, wk.data(), sizeof(int2) * ns, hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(base + lay.off[iL], &zero, sizeof zero, hipMemcpyHostToDevice) != hipSuccess) {
         xc_tile_release(ctx, t);
         return set_err(ctx, XC_E_DEVICE, "tile descriptor upload failed");
     }
@@ -141,8 +135,12 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     Exec ex;
     ex.ntiles = 1;
     ex.d_tiles = t->d_dev;
-    ex.d_tp_work = t->d_work;
-    ex.n_tp_work = ns;
+    SatGroup sg;
+    sg.cpt = sat_cpt(w);
+    sg.threads = sat_threads(w);
+    sg.count = 1;
+    sg.list = (const int *)(base + lay.off[iL]);
+    ex.sat_groups.push_back(sg);
     ex.fp64 = false;
     st = enqueue(ctx, ex, ST_PREP, ST_PREP + 1);
     if (st == XC_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) st = set_err(ctx, XC_E_DEVICE, "tile prep");
@@ -225,9 +223,6 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
 
     // ---- layout ----
     const bool refresh = (flags & 1) != 0;
-    int ntp = 0;
-    if (refresh)
-        for (const xc_tile *t : tiles) ntp += t->nwork;
     Layout lay;
     std::vector<size_t> i_spA(n), i_spB(n), i_lag(n), i_rows(n);
     const size_t cs = sizeof(cpx<float>);
@@ -243,7 +238,14 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     // uploaded block: pairs, tiles, work lists
     size_t i_pairs = lay.add(sizeof(PairDev) * n);
     size_t i_tiles = lay.add(sizeof(TileDev) * (refresh ? tiles.size() : 0) + 16);
-    size_t i_tpw = lay.add(sizeof(int2) * ntp + 16);
+    std::vector<SatGroup> sgroups;
+    std::vector<int> sflat, soff;
+    if (refresh) {
+        std::vector<int> widths;
+        for (const xc_tile *t : tiles) widths.push_back(t->w);
+        make_sat_groups(widths, sgroups, sflat, soff);
+    }
+    size_t i_slist = lay.add(sizeof(int) * sflat.size() + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
@@ -284,13 +286,17 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         hp[k] = D;
     }
     if (refresh) {
+        // count tables only for images some pair tests with a LegalCnt rule
+        std::map<const xc_tile *, bool> needz;
+        for (int i = 0; i < n; ++i)
+            if (pairs[i].cnt.enabled) needz[pairs[i].a] = needz[pairs[i].b] = true;
         TileDev *ht = (TileDev *)hat(i_tiles);
-        int2 *hw = (int2 *)hat(i_tpw);
-        int w = 0;
         for (size_t ti = 0; ti < tiles.size(); ++ti) {
             ht[ti] = tiles[ti]->dev;
-            for (int s = 0; s < tiles[ti]->nwork; ++s) hw[w++] = make_int2((int)ti, s);
+            if (!needz[tiles[ti]]) ht[ti].Z = nullptr;
         }
+        if (!sflat.empty()) memcpy(hat(i_slist), sflat.data(), sizeof(int) * sflat.size());
+        for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];
     }
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
@@ -315,8 +321,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     p->ex.fp64 = false;
     p->ex.ntiles = refresh ? (int)tiles.size() : 0;
     p->ex.d_tiles = (const TileDev *)at(i_tiles);
-    p->ex.d_tp_work = (const int2 *)at(i_tpw);
-    p->ex.n_tp_work = ntp;
+    p->ex.sat_groups = sgroups;
     p->ex.d_pairs = (const PairDev *)at(i_pairs);
     p->ex.buckets = buckets;
 

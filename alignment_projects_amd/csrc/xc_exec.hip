@@ -9,18 +9,33 @@
 
 namespace xc {
 
-int nstrips_of(int w) { return (w + TP_STRIP - 1) / TP_STRIP; }
+void make_sat_groups(const std::vector<int> &widths, std::vector<SatGroup> &groups, std::vector<int> &flat,
+                     std::vector<int> &offsets)
+{
+    groups.clear();
+    flat.clear();
+    offsets.clear();
+    std::map<std::pair<int, int>, std::vector<int>> m;
+    for (int i = 0; i < (int)widths.size(); ++i) m[std::make_pair(sat_cpt(widths[i]), sat_threads(widths[i]))].push_back(i);
+    for (auto &kv : m) {
+        SatGroup g;
+        g.cpt = kv.first.first;
+        g.threads = kv.first.second;
+        g.count = (int)kv.second.size();
+        offsets.push_back((int)flat.size());
+        for (int i : kv.second) flat.push_back(i);
+        groups.push_back(g);
+    }
+}
 
 TileSizes tile_sizes(int w, int h)
 {
     TileSizes t;
-    const size_t n = (size_t)w * h, ns = nstrips_of(w);
+    const size_t n = (size_t)w * h;
     t.S = sizeof(double) * n;
     t.Q = sizeof(double) * n;
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
-    t.rowtot = sizeof(double) * 3 * ns * h;
-    t.carry = sizeof(double) * 3 * ns * h;
     t.msd = sizeof(double) * 2;
     return t;
 }
@@ -66,9 +81,24 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     if (s0 <= ST_PREP && ST_PREP < s1 && ex.ntiles > 0) {
         hipLaunchKernelGGL(k_tp_stats, dim3(ex.ntiles * TP_PARTS), dim3(256), 0, s, ex.d_tiles);
         hipLaunchKernelGGL(k_tp_stats_fin, dim3((ex.ntiles + 63) / 64), dim3(64), 0, s, ex.d_tiles, ex.ntiles);
-        hipLaunchKernelGGL(k_tp_rowtot<T>, dim3(ex.n_tp_work), dim3(256), 0, s, ex.d_tiles, ex.d_tp_work);
-        hipLaunchKernelGGL(k_tp_carry, dim3(ex.ntiles), dim3(256), 0, s, ex.d_tiles);
-        hipLaunchKernelGGL(k_tp_scan<T>, dim3(ex.n_tp_work), dim3(64), 0, s, ex.d_tiles, ex.d_tp_work);
+        for (const auto &g : ex.sat_groups) {
+            const int key = g.cpt * 10000 + g.threads;
+            switch (key) {
+#define XC_SAT(c, n)                                                                                                \
+    case c * 10000 + n:                                                                                             \
+        hipLaunchKernelGGL((k_tp_sat<T, c, n>), dim3(g.count), dim3(n), 0, s, ex.d_tiles, g.list);                 \
+        break;
+                XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 192) XC_SAT(2, 256) XC_SAT(2, 320) XC_SAT(2, 384)
+                XC_SAT(2, 448) XC_SAT(2, 512) XC_SAT(2, 576) XC_SAT(2, 640) XC_SAT(2, 704) XC_SAT(2, 768)
+                XC_SAT(2, 832) XC_SAT(2, 896) XC_SAT(2, 960) XC_SAT(2, 1024)
+                XC_SAT(4, 576) XC_SAT(4, 640) XC_SAT(4, 704) XC_SAT(4, 768) XC_SAT(4, 832) XC_SAT(4, 896)
+                XC_SAT(4, 960) XC_SAT(4, 1024)
+                XC_SAT(8, 576) XC_SAT(8, 640) XC_SAT(8, 704) XC_SAT(8, 768) XC_SAT(8, 832) XC_SAT(8, 896)
+                XC_SAT(8, 960) XC_SAT(8, 1024)
+#undef XC_SAT
+            default: return set_err(ctx, XC_E_SHAPE, "no table sweep for %d threads x %d columns", g.threads, g.cpt);
+            }
+        }
         XC_HIP(ctx, hipGetLastError());
     }
     for (const Bucket &b : ex.buckets) {

@@ -24,11 +24,24 @@ struct Bucket {
 
 // Device-resident execution description.  All pointers are device pointers
 // into memory owned by the caller (a plan or the per-call arena).
+// Images of one table-sweep launch (same columns-per-thread).
+struct SatGroup {
+    int cpt = 0;
+    int threads = 0;
+    int count = 0;
+    const int *list = nullptr;  // device: tile indices
+};
+
+// Tile indices grouped by sat_cpt(w): fills `groups` (count, cpt) and the
+// concatenated index list `flat` (upload it, then point each group's list
+// into the device copy at its offset).
+void make_sat_groups(const std::vector<int> &widths, std::vector<SatGroup> &groups, std::vector<int> &flat,
+                     std::vector<int> &offsets);
+
 struct Exec {
     int ntiles = 0;
     const TileDev *d_tiles = nullptr;
-    const int2 *d_tp_work = nullptr;  // (tile, strip) items
-    int n_tp_work = 0;
+    std::vector<SatGroup> sat_groups;
     const PairDev *d_pairs = nullptr;
     std::vector<Bucket> buckets;
     bool fp64 = false;
@@ -41,10 +54,9 @@ int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
 
 // Bytes of device scratch/table memory one prepared image needs.
 struct TileSizes {
-    size_t S, Q, Z, part, rowtot, carry, msd;
+    size_t S, Q, Z, part, msd;
 };
 TileSizes tile_sizes(int w, int h);
-int nstrips_of(int w);
 
 // Group pairs into (NH, NY, whiten) buckets; returns the order of pair
 // indices (bucket-major) and the bucket list (first/count filled).

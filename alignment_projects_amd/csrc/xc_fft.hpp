@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace xc {
 
 constexpr int XC_TW_N = 8192;          // twiddle table length (largest Nx)
@@ -92,6 +94,15 @@ template <int DIR, int R, typename T> __device__ __forceinline__ void bfly(cpx<T
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 template <int N> struct Plan {
     static constexpr int L = ilog2c(N);
     static constexpr int E = N >= 8 ? 8 : N;
@@ -115,19 +126,87 @@ __device__ __forceinline__ cpx<T> twiddle(const cpx<T> *__restrict__ tw, int m, 
     return w;
 }
 
-// LDS view: element pos of this transform lives at base[pos * stride].
+// LDS view: element pos of this transform lives at base[(pos + pos/8) *
+// stride].  The one-in-eight padding spreads the radix-8 scatter
+// (pos = 8j + r across lanes j) over all banks instead of two.
 template <typename T> struct LdsView {
     cpx<T> *base;
     int stride;
-    __device__ __forceinline__ cpx<T> &operator[](int pos) const { return base[pos * stride]; }
+    __device__ __forceinline__ cpx<T> &operator[](int pos) const { return base[(pos + (pos >> 3)) * stride]; }
 };
+
+// LDS elements one padded length-n transform occupies.
+constexpr int lds_len(int n) { return n + n / 8 + 1; }
+
+// Per-thread Stockham twiddles of every pass, loaded once from the table
+// before any data arrives (they depend only on the thread index), so the
+// table latency is paid once instead of once per pass.  Forward sign;
+// the inverse transform conjugates on use.
+template <int N> struct TwPlan {
+    using PL = Plan<N>;
+    static constexpr int count(int upto)
+    {
+        int c = 0;
+        for (int p = 1; p < upto && p < PL::NP; ++p) c += (PL::E / PL::radix(p)) * (PL::radix(p) - 1);
+        return c;
+    }
+    static constexpr int TOTAL = count(32) > 0 ? count(32) : 1;
+};
+
+template <int N, typename T> struct TwReg {
+    cpx<T> w[TwPlan<N>::TOTAL];
+};
+
+template <int N, typename T, int P = 1>
+__device__ __forceinline__ void tw_load_pass(TwReg<N, T> &R, int t, const cpx<T> *__restrict__ tw)
+{
+    using PL = Plan<N>;
+    if constexpr (P < PL::NP) {
+        constexpr int E = PL::E, TPF = PL::TPF, Rdx = PL::radix(P), NB = E / Rdx;
+        constexpr int Ns = PL::stride_before(P), OFF = TwPlan<N>::count(P);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int k = (t + TPF * b) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < Rdx; ++r) R.w[OFF + b * (Rdx - 1) + (r - 1)] = twiddle<-1>(tw, r * k, Ns * Rdx);
+        }
+        tw_load_pass<N, T, P + 1>(R, t, tw);
+    }
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void tw_load(TwReg<N, T> &R, int t, const cpx<T> *__restrict__ tw)
+{
+    tw_load_pass<N, T, 1>(R, t, tw);
+}
 
 // One length-N transform in the canonical register layout.  Every thread
 // of the workgroup must call it (it contains __syncthreads), including
 // threads whose data is padding.
-template <int DIR, int N, typename T, int P = 0>
-__device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds,
-                                         const cpx<T> *__restrict__ tw)
+// Twiddle sources: registers preloaded by tw_load (TwReg), or the table
+// read lazily inside each pass (TwLazy).
+template <typename T> struct TwLazy {
+    const cpx<T> *tab;
+};
+
+template <int DIR, int N, typename T, int OFF, int B, int RR, int Rdx, int Ns>
+__device__ __forceinline__ cpx<T> tw_get(const TwReg<N, T> &TW, int)
+{
+    cpx<T> w = TW.w[OFF + B * (Rdx - 1) + (RR - 1)];
+    if constexpr (DIR > 0) w.y = -w.y;
+    return w;
+}
+
+template <int DIR, int N, typename T, int OFF, int B, int RR, int Rdx, int Ns>
+__device__ __forceinline__ cpx<T> tw_get(const TwLazy<T> &TW, int t)
+{
+    constexpr int TPF = Plan<N>::TPF;
+    const int k = (t + TPF * B) & (Ns - 1);
+    return twiddle<DIR>(TW.tab, RR * k, Ns * Rdx);
+}
+
+template <int DIR, int N, typename T, int P = 0, typename TWS>
+__device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds, const TWS &TW)
 {
     using PL = Plan<N>;
     if constexpr (P < PL::NP) {
@@ -135,6 +214,7 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
         constexpr int R = PL::radix(P);
         constexpr int NB = E / R;
         constexpr int Ns = PL::stride_before(P);
+        constexpr int OFF = TwPlan<N>::count(P);
         if constexpr (P > 0) {
             // reload canonical layout written by the previous pass
             __syncthreads();
@@ -142,19 +222,15 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
             for (int m = 0; m < E; ++m) v[m] = lds[t + TPF * m];
         }
         cpx<T> u[NB][R];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int j = t + TPF * b;
-            const int k = j & (Ns - 1);
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
+        static_for<0, NB>([&](auto bI) {
+            constexpr int b = decltype(bI)::value;
+            static_for<0, R>([&](auto rI) {
+                constexpr int r = decltype(rI)::value;
                 u[b][r] = v[b + r * NB];
-                if constexpr (P > 0) {
-                    if (r > 0) u[b][r] = cmul(u[b][r], twiddle<DIR>(tw, r * k, Ns * R));
-                }
-            }
+                if constexpr (P > 0 && r > 0) u[b][r] = cmul(u[b][r], tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t));
+            });
             bfly<DIR, R>(u[b]);
-        }
+        });
         if constexpr (P + 1 == PL::NP) {
 #pragma unroll
             for (int b = 0; b < NB; ++b)
@@ -170,16 +246,24 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
 #pragma unroll
                 for (int r = 0; r < R; ++r) lds[base + r * Ns] = u[b][r];
             }
-            fft_pass<DIR, N, T, P + 1>(v, t, lds, tw);
+            fft_pass<DIR, N, T, P + 1, TWS>(v, t, lds, TW);
         }
     }
 }
 
 template <int DIR, int N, typename T>
+__device__ __forceinline__ void fft(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds, const TwReg<N, T> &TW)
+{
+    fft_pass<DIR, N, T, 0, TwReg<N, T>>(v, t, lds, TW);
+}
+
+// Table form: each pass reads its twiddles from the table when it needs them.
+template <int DIR, int N, typename T>
 __device__ __forceinline__ void fft(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds,
                                     const cpx<T> *__restrict__ tw)
 {
-    fft_pass<DIR, N, T, 0>(v, t, lds, tw);
+    const TwLazy<T> TW{tw};
+    fft_pass<DIR, N, T, 0, TwLazy<T>>(v, t, lds, TW);
 }
 
 }  // namespace xc

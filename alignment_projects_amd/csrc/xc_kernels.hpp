@@ -241,21 +241,24 @@ __device__ __forceinline__ T load_val(const ImgView &v, double mean, double sd, 
 // row totals.
 // ------------------------------------------------------------------
 
-// Partial sums of present raw values: TP_PARTS workgroups per image.
+// Partial sums of present raw values: TP_PARTS workgroups per image, each
+// over a band of rows; threads stride along x (coalesced).
 static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restrict__ tiles)
 {
     __shared__ double ss[256], sq[256], sc[256];
     const TileDev &D = tiles[blockIdx.x / TP_PARTS];
     const int part = blockIdx.x % TP_PARTS, tid = threadIdx.x;
-    const long long n = (long long)D.v.w * D.v.h;
+    const int h = D.v.h, w = D.v.w;
+    const int y0 = (int)((long long)h * part / TP_PARTS), y1 = (int)((long long)h * (part + 1) / TP_PARTS);
     double a = 0.0, b = 0.0, c = 0.0;
-    for (long long i = (long long)part * 256 + tid; i < n; i += (long long)TP_PARTS * 256) {
-        const int y = (int)(i / D.v.w), x = (int)(i - (long long)y * D.v.w);
-        if (present(D.v, x, y)) {
-            const double t = load_raw(D.v, x, y);
-            a += t;
-            b += t * t;
-            c += 1.0;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = tid; x < w; x += 256) {
+            if (present(D.v, x, y)) {
+                const double t = load_raw(D.v, x, y);
+                a += t;
+                b += t * t;
+                c += 1.0;
+            }
         }
     }
     ss[tid] = a;
@@ -302,105 +305,182 @@ static __global__ void k_tp_stats_fin(const TileDev *__restrict__ tiles, int nti
     D.msd[1] = sd;
 }
 
-// Strip row totals of v, v^2, (v != 0): work item = (tile, strip).
-template <typename T>
-__global__ void __launch_bounds__(256) k_tp_rowtot(const TileDev *__restrict__ tiles, const int2 *__restrict__ work)
+// Wavefront inclusive scans with DPP row shifts and row broadcasts (GFX9
+// data-parallel primitives: no LDS round trip, unlike a lane permute).
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp_i(int v)
 {
-    const int2 wi = work[blockIdx.x];
-    const TileDev &D = tiles[wi.x];
-    const int s = wi.y, h = D.v.h, x0 = s * TP_STRIP, x1 = imin(D.v.w, x0 + TP_STRIP);
-    const double mean = D.msd[0], sd = D.msd[1];
-    for (int y = threadIdx.x; y < h; y += blockDim.x) {
-        double a = 0.0, b = 0.0, c = 0.0;
-        for (int x = x0; x < x1; ++x) {
-            const double t = (double)load_val<T>(D.v, mean, sd, x, y);
-            a += t;
-            b += t * t;
-            c += (t != 0.0);
-        }
-        const long long o = (long long)s * h + y, plane = (long long)D.nstrips * h;
-        D.rowtot[o] = a;
-        D.rowtot[plane + o] = b;
-        D.rowtot[2 * plane + o] = c;
-    }
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWMASK, 0xf, true);
 }
 
-// Carries: carry[s][y] = sum_{y' <= y} sum_{s' < s} rowtot[s'][y'].  One
-// workgroup per image, one thread per strip (<= 256 strips).
-static __global__ void __launch_bounds__(256) k_tp_carry(const TileDev *__restrict__ tiles)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double v)
 {
-    __shared__ double sa[256], sb[256], sc[256];
-    const TileDev &D = tiles[blockIdx.x];
-    const int s = threadIdx.x, ns = D.nstrips, h = D.v.h;
-    const long long plane = (long long)ns * h;
-    double aa = 0.0, ab = 0.0, ac = 0.0;
-    for (int y = 0; y < h; ++y) {
-        const long long o = (long long)s * h + y;
-        sa[s] = s < ns ? D.rowtot[o] : 0.0;
-        sb[s] = s < ns ? D.rowtot[plane + o] : 0.0;
-        sc[s] = s < ns ? D.rowtot[2 * plane + o] : 0.0;
-        __syncthreads();
-        double ea = 0.0, eb = 0.0, ec = 0.0;
-        for (int k = 0; k < s && k < ns; ++k) {
-            ea += sa[k];
-            eb += sb[k];
-            ec += sc[k];
-        }
-        aa += ea;
-        ab += eb;
-        ac += ec;
-        if (s < ns) {
-            D.carry[o] = aa;
-            D.carry[plane + o] = ab;
-            D.carry[2 * plane + o] = ac;
-        }
-        __syncthreads();
-    }
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_i<CTRL, ROWMASK>((int)(b & 0xffffffffLL));
+    const int hi = dpp_i<CTRL, ROWMASK>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Strip scan: one wavefront per (tile, strip); lane = column.
-template <typename T>
-__global__ void __launch_bounds__(64) k_tp_scan(const TileDev *__restrict__ tiles, const int2 *__restrict__ work)
+__device__ __forceinline__ int wave_scan_i(int v)
 {
-    const int2 wi = work[blockIdx.x];
-    const TileDev &D = tiles[wi.x];
-    const int s = wi.y, lane = threadIdx.x, w = D.v.w, h = D.v.h;
-    const int x = s * TP_STRIP + lane;
-    const bool in = x < w;
+    v += dpp_i<0x111, 0xf>(v);  // row_shr:1
+    v += dpp_i<0x112, 0xf>(v);  // row_shr:2
+    v += dpp_i<0x114, 0xf>(v);  // row_shr:4
+    v += dpp_i<0x118, 0xf>(v);  // row_shr:8
+    v += dpp_i<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3
+    v += dpp_i<0x143, 0xc>(v);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
+__device__ __forceinline__ double wave_scan_d(double v)
+{
+    v += dpp_d<0x111, 0xf>(v);
+    v += dpp_d<0x112, 0xf>(v);
+    v += dpp_d<0x114, 0xf>(v);
+    v += dpp_d<0x118, 0xf>(v);
+    v += dpp_d<0x142, 0xa>(v);
+    v += dpp_d<0x143, 0xc>(v);
+    return v;
+}
+
+constexpr int SAT_MAX_THREADS = 1024;
+
+// Typed pixel read for the table sweep (no per-pixel dtype switch).
+template <typename InT>
+__device__ __forceinline__ double sweep_val(const ImgView &v, int x, int y)
+{
+    return (double)reinterpret_cast<const InT *>(v.data)[(long long)y * v.stride + x];
+}
+
+template <typename T, int CPT, int NT, typename InT, bool MASK>
+__device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *xq, int *xz)
+{
+    constexpr int NWV = NT / 64;
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = tid * CPT;
     const double mean = D.msd[0], sd = D.msd[1];
-    const long long plane = (long long)D.nstrips * h;
-    double cs = 0.0, cq = 0.0;
-    int cz = 0;
-    for (int y = 0; y < h; ++y) {
-        const double t = in ? (double)load_val<T>(D.v, mean, sd, x, y) : 0.0;
-        cs += t;
-        cq += t * t;
-        cz += (t != 0.0);
-        double ps = cs, pq = cq;
-        int pz = cz;
+    double cs[CPT], cq[CPT];
+    int cz[CPT];
+    double nxt[CPT];
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const double us = __shfl_up(ps, off), uq = __shfl_up(pq, off);
-            const int uz = __shfl_up(pz, off);
-            if (lane >= off) {
-                ps += us;
-                pq += uq;
-                pz += uz;
+    for (int j = 0; j < CPT; ++j) {
+        cs[j] = 0.0;
+        cq[j] = 0.0;
+        cz[j] = 0;
+        const int x = x0 + j;
+        nxt[j] = (x < w && h > 0) ? sweep_val<InT>(D.v, x, 0) : 0.0;
+    }
+    for (int y = 0; y < h; ++y) {
+        double ps[CPT], pq[CPT];
+        int pz[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int x = x0 + j;
+            const bool pres = x < w && (!MASK || D.v.mask[(long long)y * D.v.mstride + x]);
+            // the value the transforms see: (raw - mean) / sd rounded to T; absent pixels are 0
+            const double t = pres ? (double)(T)((nxt[j] - mean) / sd) : 0.0;
+            nxt[j] = (x < w && y + 1 < h) ? sweep_val<InT>(D.v, x, y + 1) : 0.0;
+            cs[j] += t;
+            cq[j] += t * t;
+            cz[j] += (t != 0.0);
+            ps[j] = (j ? ps[j - 1] : 0.0) + cs[j];
+            pq[j] = (j ? pq[j - 1] : 0.0) + cq[j];
+            pz[j] = (j ? pz[j - 1] : 0) + cz[j];
+        }
+        const double ts = ps[CPT - 1], tq = pq[CPT - 1];
+        const int tz = pz[CPT - 1];
+        const double is = wave_scan_d(ts), iq = wave_scan_d(tq);
+        const int iz = wave_scan_i(tz);
+        const int par = (y & 1) * NWV;
+        if (lane == 63) {
+            xs[par + wv] = is;
+            xq[par + wv] = iq;
+            xz[par + wv] = iz;
+        }
+        __syncthreads();
+        double bs = is - ts, bq = iq - tq;
+        int bz = iz - tz;
+#pragma unroll
+        for (int k = 0; k < NWV; ++k)
+            if (k < wv) {
+                bs += xs[par + k];
+                bq += xq[par + k];
+                bz += xz[par + k];
+            }
+        const long long row = (long long)y * w;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            if (x0 + j < w) {
+                D.S[row + x0 + j] = bs + ps[j];
+                D.Q[row + x0 + j] = bq + pq[j];
+                if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
             }
         }
-        if (in) {
-            const long long o = (long long)s * h + y, i = (long long)y * w + x;
-            D.S[i] = D.carry[o] + ps;
-            D.Q[i] = D.carry[plane + o] + pq;
-            D.Z[i] = (int32_t)D.carry[2 * plane + o] + pz;
-        }
     }
+}
+
+// Summed-area tables in one sweep: one workgroup per image walks the rows
+// top to bottom.  Each thread owns CPT consecutive columns and keeps their
+// running column sums in registers; a DPP wavefront scan plus a 4-entry
+// cross-wave exchange (double-buffered by row parity, one barrier per row)
+// turns column sums into the 2-D prefix.  Reads each pixel once, writes
+// S, Q, Z once.
+template <typename T, int CPT, int NT>
+__global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list)
+{
+    __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
+    __shared__ int xz[2 * NT / 64];
+    const TileDev D = tiles[list[blockIdx.x]];  // by value: the table stores cannot alias it
+    const bool m = D.v.mask != nullptr;
+    switch (D.v.dtype) {
+    case XC_U8:
+        if (m) sat_sweep<T, CPT, NT, uint8_t, true>(D, xs, xq, xz);
+        else sat_sweep<T, CPT, NT, uint8_t, false>(D, xs, xq, xz);
+        break;
+    case XC_U16:
+        if (m) sat_sweep<T, CPT, NT, uint16_t, true>(D, xs, xq, xz);
+        else sat_sweep<T, CPT, NT, uint16_t, false>(D, xs, xq, xz);
+        break;
+    case XC_F32:
+        if (m) sat_sweep<T, CPT, NT, float, true>(D, xs, xq, xz);
+        else sat_sweep<T, CPT, NT, float, false>(D, xs, xq, xz);
+        break;
+    default:
+        if (m) sat_sweep<T, CPT, NT, double, true>(D, xs, xq, xz);
+        else sat_sweep<T, CPT, NT, double, false>(D, xs, xq, xz);
+        break;
+    }
+}
+
+// Columns per thread of the sweep for width w: two columns per thread up
+// to 1024 threads, then wider.  The launch uses max(64, w / cpt) threads
+// rounded up to a wavefront multiple.
+__host__ __device__ inline int sat_cpt(int w) { return w <= 2048 ? 2 : w <= 4096 ? 4 : 8; }
+__host__ __device__ inline int sat_threads(int w)
+{
+    const int c = sat_cpt(w);
+    int n = (w + c - 1) / c;
+    n = (n + 63) / 64 * 64;
+    return n < 64 ? 64 : (n > SAT_MAX_THREADS ? SAT_MAX_THREADS : n);
 }
 
 // ------------------------------------------------------------------
 // Row r2c: one Nx-point real transform per image row, via an Nx/2-point
 // complex transform and the split post-twiddle.
 // ------------------------------------------------------------------
+
+#define C2R_ROW(nh) ((nh) + 2 > lds_len(nh) ? (nh) + 2 : lds_len(nh))
+
+// Workgroup-index remap so that consecutive logical work items run on the
+// same XCD: the dispatcher deals workgroups to the 8 XCDs round robin, and
+// neighbouring items share cache lines (column tiles) or table rows.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    const int q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
 
 template <int NH> struct RowCfg {
     static constexpr int TPF = Plan<NH>::TPF;
@@ -433,7 +513,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
         v[m].x = valid ? load_val<T>(V, mean, sd, 2 * p, y) : (T)0;
         v[m].y = valid ? load_val<T>(V, mean, sd, 2 * p + 1, y) : (T)0;
     }
-    LdsView<T> L{smem + g * NH, 1};
+    LdsView<T> L{smem + g * lds_len(NH), 1};
     fft<-1, NH>(v, t, L, tw);
 
     __syncthreads();
@@ -467,7 +547,7 @@ template <typename T, int NY> struct ColCfg {
     static constexpr int TPF = Plan<NY>::TPF;
     static constexpr int CW = sizeof(T) == 8 ? 4 : 8;  // columns per workgroup wanted (64 B rows)
     static constexpr int CT = 1024 / TPF > 0 ? 1024 / TPF : 1;                       // <= 1024 threads
-    static constexpr int CL = 65536 / (NY * (int)sizeof(cpx<T>)) > 0 ? 65536 / (NY * (int)sizeof(cpx<T>)) : 1;  // <= 64 KB LDS
+    static constexpr int CL = 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) > 0 ? 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) : 1;  // <= 64 KB LDS
     static constexpr int C = cmin(CW, cmin(CT, CL));
     static constexpr int THREADS = TPF * C;
 };
@@ -481,7 +561,7 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *smem = reinterpret_cast<cpx<T> *>(smem_raw);
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    const int2 wi = work[blockIdx.x];  // {pair, k0}
+    const int2 wi = work[xcd_remap(blockIdx.x, gridDim.x)];  // {pair, k0}
     const PairDev &P = pairs[wi.x];
     const int H = P.Nx / 2 + 1;
     const int k = wi.y + c;
@@ -490,6 +570,8 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
     const cpx<T> *SB = reinterpret_cast<const cpx<T> *>(P.specB);
     cpx<T> *LG = reinterpret_cast<cpx<T> *>(P.lag);
     LdsView<T> L{smem + c, C};
+    TwReg<NY, T> TW;
+    tw_load<NY, T>(TW, t, tw);
 
     cpx<T> a[E], b[E];
 #pragma unroll
@@ -497,14 +579,14 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
         const int y = t + TPF * m;
         a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
     }
-    fft<-1, NY>(a, t, L, tw);
+    fft<-1, NY>(a, t, L, TW);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int y = t + TPF * m;
         b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
     }
     __syncthreads();
-    fft<-1, NY>(b, t, L, tw);
+    fft<-1, NY>(b, t, L, TW);
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         cpx<T> p;
@@ -531,7 +613,7 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
         a[m] = p;
     }
     __syncthreads();
-    fft<+1, NY>(a, t, L, tw);
+    fft<+1, NY>(a, t, L, TW);
     if (kv) {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
@@ -548,10 +630,56 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
 // the row's best candidate with its two in-row neighbours.
 // ------------------------------------------------------------------
 
-struct RedItem {
-    double v;
-    long long k;
-};
+// Value of raster column ix of lag row y (signed; DEAD_ROW for a P2I dead
+// row) from its raw c2r value, with the mode's scan semantics.
+//   *cand : the lag competes for the peak
+//   *win  : the lag belongs to the centred wR x hR image (R/S modes)
+//   *odd  : GetCorr's exit(44) condition
+__device__ __forceinline__ double lag_value(const PairDev &P, const RowBox &rb, int y, int ix, double raw,
+                                            bool *cand, bool *win, bool *odd)
+{
+    *cand = false;
+    *win = false;
+    *odd = false;
+    bool dead;
+    const int x = lag_x_of_raster(P, ix, &dead);
+    if (P.mode == PM_P2I) {
+        // CorrPatchToImage scan (Correlation.cpp:1299-1327): raw surface, zones, disc
+        if (!dead && y != DEAD_ROW) {
+            const long long dxd = x - P.Ox, dyd = y - P.Oy;
+            *cand = dxd * dxd + dyd * dyd <= (long long)P.radius * P.radius;
+        }
+        return raw;
+    }
+    if (dead) return 0.0;
+    if (P.mode == PM_NCC) {
+        // CorrPatches scan: disc about (Ox, Oy), then the rules (Correlation.cpp:1030-1061)
+        const long long dxd = x - P.Ox, dyd = y - P.Oy;
+        if (dxd * dxd + dyd * dyd > (long long)P.radius * P.radius) return 0.0;
+        bool legal;
+        const double v = eval_lag(P, rb, x, raw, &legal, odd);
+        *cand = legal;
+        return v;
+    }
+    bool legal, o;
+    const double v = eval_lag(P, rb, x, raw, &legal, &o);
+    *win = true;
+    bool c = legal;
+    if (P.Rx > 0 && P.Ry > 0) {
+        // MaskA (Correlation.cpp:2011-2054) in centred coordinates
+        const int ox = P.Ox - P.offx + P.nnegx, oy = P.Oy - P.offy + P.nnegy;
+        const int cx = x + P.nnegx, cy = y + P.nnegy;
+        if (cy < oy - P.Ry || cy > oy + P.Ry || cx < ox - P.Rx || cx > ox + P.Rx) c = false;
+        else {
+            const double rx2 = (double)P.Rx * P.Rx, asq = rx2 / ((double)P.Ry * P.Ry);
+            const int xx = cx - ox, yy = cy - oy;
+            if (xx * xx + asq * yy * yy > rx2) c = false;
+        }
+    }
+    if (P.mincor > 0.0 && !(v > P.mincor)) c = false;
+    *cand = c;
+    return v;
+}
 
 template <typename T, int NH>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
@@ -559,23 +687,24 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 {
     using PL = Plan<NH>;
     constexpr int E = PL::E, TPF = PL::TPF, G = RowCfg<NH>::G;
-    constexpr int H = NH + 1;
+    constexpr int H = NH + 1, NX = 2 * NH;
+    constexpr int NW = TPF >= 64 ? TPF / 64 : 1;  // wavefronts per row
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *smem = reinterpret_cast<cpx<T> *>(smem_raw);
-    // reduction scratch after the G spectra (H + 1 entries each, keeps 16 B alignment)
-    double *red_v = reinterpret_cast<double *>(smem + G * (H + 1));
-    long long *red_k = reinterpret_cast<long long *>(red_v + G * TPF);
-    double *red_s = reinterpret_cast<double *>(red_k + G * TPF);
-    double *red_q = red_s + G * TPF;
-    double *nb = red_q + G * TPF;  // [G][2] neighbour values
+    constexpr int RS = C2R_ROW(NH);  // LDS elements per row
+    double *red_v = reinterpret_cast<double *>(smem + G * RS);
+    long long *red_k = reinterpret_cast<long long *>(red_v + G * NW);
+    double *red_s = reinterpret_cast<double *>(red_k + G * NW);
+    double *red_q = red_s + G * NW;
+    int *red_f = reinterpret_cast<int *>(red_q + G * NW);
 
-    const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
-    const int2 wi = work[blockIdx.x];  // {pair, yl0}
+    const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF, lane = tid & 63;
+    const int2 wi = work[xcd_remap(blockIdx.x, gridDim.x)];  // {pair, yl0}
     const PairDev &P = pairs[wi.x];
     const int yl = wi.y + g;
     const bool valid = yl < P.hR;
     const cpx<T> *LG = reinterpret_cast<const cpx<T> *>(P.lag) + (long long)yl * H;
-    cpx<T> *X = smem + g * (H + 1);
+    cpx<T> *X = smem + g * RS;
 
     for (int i = t; i < H; i += TPF) X[i] = valid ? LG[i] : cpx<T>{(T)0, (T)0};
     __syncthreads();
@@ -592,70 +721,37 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     LdsView<T> L{X, 1};
     fft<+1, NH>(v, t, L, tw);
 
-    // ---- evaluation ----
+    // raw row back to LDS in natural order, then evaluate with unit-stride lanes
+    T *R = reinterpret_cast<T *>(X);
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int p = t + TPF * m;
+        R[2 * p] = v[m].x;
+        R[2 * p + 1] = v[m].y;
+    }
+    __syncthreads();
+
     const int y = lag_y_of_row(P, valid ? yl : 0);
     const RowBox rb = row_box(P, y == DEAD_ROW ? 0 : y);
     const bool centred = P.lag_order == LAG_CENTRED;
-    const int dyd = y - P.Oy;
-    const long long rsqr = (long long)P.radius * P.radius;
-    double vals[2 * E];
     double bestv = -1.0E300;
     long long bestk = 0x7fffffffffffffffLL;
     double ssum = 0.0, ssq = 0.0;
     int flags = 0;
-    // oval (MaskA, Correlation.cpp:2011-2054) in centred coordinates
-    const bool oval = centred && P.Rx > 0 && P.Ry > 0;
-    const int ox = P.Ox - P.offx + P.nnegx, oy = P.Oy - P.offy + P.nnegy;
-    const double rx2 = (double)P.Rx * P.Rx, asq = rx2 / ((double)P.Ry * P.Ry);
-    const int cyy = y + P.nnegy;
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-#pragma unroll
-        for (int hf = 0; hf < 2; ++hf) {
-            const int ix = 2 * (t + TPF * m) + hf;
-            const double raw = hf ? (double)v[m].y : (double)v[m].x;
-            double val = 0.0;
-            bool cand = false;
-            bool dead;
-            const int x = lag_x_of_raster(P, ix, &dead);
-            if (P.mode == PM_P2I) {
-                // CorrPatchToImage scan (Correlation.cpp:1299-1327): raw surface, zones, disc
-                val = raw;
-                if (valid && !dead && y != DEAD_ROW) {
-                    const int dxd = x - P.Ox;
-                    cand = (long long)dxd * dxd + (long long)dyd * dyd <= rsqr;
-                }
-            } else if (valid && !dead) {
-                if (!centred) {
-                    // CorrPatches scan: disc about (Ox, Oy), then the rules
-                    const int dxd = x - P.Ox;
-                    if ((long long)dxd * dxd + (long long)dyd * dyd <= rsqr) {
-                        bool odd;
-                        val = eval_lag(P, rb, x, raw, &cand, &odd);
-                        if (odd) flags |= 2;
-                    }
-                } else {
-                    bool odd;
-                    val = eval_lag(P, rb, x, raw, &cand, &odd);
-                    if (P.mode == PM_S) {
-                        ssum += val;
-                        ssq += val * val;
-                    }
-                    if (oval) {
-                        const int cxx = x + P.nnegx;
-                        if (cyy < oy - P.Ry || cyy > oy + P.Ry || cxx < ox - P.Rx || cxx > ox + P.Rx) cand = false;
-                        else {
-                            const int xx = cxx - ox, yy = cyy - oy;
-                            if (xx * xx + asq * yy * yy > rx2) cand = false;
-                        }
-                    }
-                    if (P.mincor > 0.0 && !(val > P.mincor)) cand = false;
-                }
+    if (valid) {
+        for (int ix = t; ix < NX; ix += TPF) {
+            bool cand, win, odd;
+            const double val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+            if (odd) flags |= 2;
+            if (win) {
+                ssum += val;
+                ssq += val * val;
             }
-            vals[2 * m + hf] = val;
             if (cand) {
                 flags |= 1;
-                const long long key = centred ? (long long)(x + P.nnegx) : (long long)ix;
+                bool dead;
+                const long long key = centred ? (long long)(lag_x_of_raster(P, ix, &dead) + P.nnegx) : (long long)ix;
                 if (better(val, key, bestv, bestk)) {
                     bestv = val;
                     bestk = key;
@@ -663,64 +759,71 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
             }
         }
     }
-    // ---- reduction over the row's TPF threads ----
-    double *rv = red_v + g * TPF;
-    long long *rk = red_k + g * TPF;
-    double *rs = red_s + g * TPF;
-    double *rq = red_q + g * TPF;
-    int *rf = reinterpret_cast<int *>(nb + 2 * G) + g * TPF;
-    __syncthreads();
-    rv[t] = bestv;
-    rk[t] = bestk;
-    rs[t] = ssum;
-    rq[t] = ssq;
-    rf[t] = flags;
-    __syncthreads();
-    for (int off = TPF / 2; off > 0; off >>= 1) {
-        if (t < off) {
-            if (better(rv[t + off], rk[t + off], rv[t], rk[t])) {
-                rv[t] = rv[t + off];
-                rk[t] = rk[t + off];
-            }
-            rs[t] += rs[t + off];
-            rq[t] += rq[t + off];
-            rf[t] |= rf[t + off];
+    // ---- reduction over the row's TPF threads: shuffles, then LDS across wavefronts ----
+    constexpr int SPAN = TPF >= 64 ? 64 : TPF;
+#pragma unroll
+    for (int off = SPAN / 2; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bestv, off);
+        const long long ok = __shfl_xor(bestk, off);
+        ssum += __shfl_xor(ssum, off);
+        ssq += __shfl_xor(ssq, off);
+        flags |= __shfl_xor(flags, off);
+        if (better(ov, ok, bestv, bestk)) {
+            bestv = ov;
+            bestk = ok;
+        }
+    }
+    if constexpr (NW > 1) {
+        if (lane == 0) {
+            const int wv = t / 64;
+            red_v[g * NW + wv] = bestv;
+            red_k[g * NW + wv] = bestk;
+            red_s[g * NW + wv] = ssum;
+            red_q[g * NW + wv] = ssq;
+            red_f[g * NW + wv] = flags;
         }
         __syncthreads();
-    }
-    const long long bk = rk[0];
-    // neighbours key-1, key+1 in the same row (raster ix for NCC, centred x otherwise)
-    if (t == 0) {
-        nb[2 * g] = 0.0;
-        nb[2 * g + 1] = 0.0;
-    }
-    __syncthreads();
-    if (rf[0] & 1) {
-#pragma unroll
-        for (int m = 0; m < E; ++m) {
-#pragma unroll
-            for (int hf = 0; hf < 2; ++hf) {
-                const int ix = 2 * (t + TPF * m) + hf;
-                bool dead;
-                const int x = lag_x_of_raster(P, ix, &dead);
-                long long key;
-                if (!centred) key = ix;
-                else key = dead ? -100 : (long long)(x + P.nnegx);
-                if (key == bk - 1) nb[2 * g] = vals[2 * m + hf];
-                if (key == bk + 1) nb[2 * g + 1] = vals[2 * m + hf];
+        if (t == 0) {
+            for (int k = 1; k < NW; ++k) {
+                const int i = g * NW + k;
+                if (better(red_v[i], red_k[i], bestv, bestk)) {
+                    bestv = red_v[i];
+                    bestk = red_k[i];
+                }
+                ssum += red_s[i];
+                ssq += red_q[i];
+                flags |= red_f[i];
             }
         }
     }
-    __syncthreads();
     if (t == 0 && valid) {
         RowRec r;
-        r.best = rv[0];
-        r.key = (int)(rf[0] & 1 ? bk : -1);
-        r.left = nb[2 * g];
-        r.right = nb[2 * g + 1];
-        r.sum = rs[0];
-        r.sum2 = rq[0];
-        r.flags = rf[0];
+        r.best = bestv;
+        r.key = (int)((flags & 1) ? bestk : -1);
+        r.left = 0.0;
+        r.right = 0.0;
+        if (flags & 1) {
+            // the two in-row neighbours of the best, recomputed from the raw row
+            for (int s = 0; s < 2; ++s) {
+                const long long kk = bestk + (s ? 1 : -1);
+                int ix = -1;
+                if (!centred) ix = (int)kk;
+                else {
+                    const int x = (int)kk - P.nnegx;
+                    if (x >= -P.nnegx && x < P.nposx) ix = x >= 0 ? x : P.Nx + x;
+                }
+                double val = 0.0;
+                if (ix >= 0 && ix < NX) {
+                    bool cand, win, odd;
+                    val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+                }
+                if (s) r.right = val;
+                else r.left = val;
+            }
+        }
+        r.sum = ssum;
+        r.sum2 = ssq;
+        r.flags = flags;
         r.pad[0] = r.pad[1] = 0;
         P.rows[yl] = r;
     }
@@ -730,8 +833,8 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 template <typename T, int NH> constexpr size_t c2r_smem()
 {
     constexpr int G = RowCfg<NH>::G, TPF = Plan<NH>::TPF;
-    return sizeof(cpx<T>) * G * (NH + 2) + G * TPF * (sizeof(double) * 3 + sizeof(long long) + sizeof(int)) +
-           sizeof(double) * 2 * G + 16;
+    constexpr int NW = TPF >= 64 ? TPF / 64 : 1;
+    return sizeof(cpx<T>) * G * C2R_ROW(NH) + G * NW * (sizeof(double) * 3 + sizeof(long long) + sizeof(int)) + 16;
 }
 
 // ------------------------------------------------------------------

@@ -169,7 +169,7 @@ template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDe
     if constexpr (NH > max_nh<T>()) {
         return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
     } else {
-        const size_t smem = sizeof(cpx<T>) * RowCfg<NH>::G * NH;
+        const size_t smem = sizeof(cpx<T>) * RowCfg<NH>::G * lds_len(NH);
         hipLaunchKernelGGL((k_row_r2c<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
                            twiddles<T>(ctx));
         XC_HIP(ctx, hipGetLastError());
@@ -195,7 +195,7 @@ template <typename T, int NY> static int col_n(xc_ctx *ctx, bool whiten, int nb,
         return set_err(ctx, XC_E_SHAPE, "column transform %d too long", NY);
     } else {
         using CC = ColCfg<T, NY>;
-        const size_t smem = sizeof(cpx<T>) * CC::C * NY;
+        const size_t smem = sizeof(cpx<T>) * CC::C * lds_len(NY);
         if (whiten)
             hipLaunchKernelGGL((k_col_fused<T, NY, true>), dim3(nb), dim3(CC::THREADS), smem, ctx->stream, pd, w,
                                twiddles<T>(ctx));

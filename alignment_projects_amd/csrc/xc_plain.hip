@@ -69,15 +69,14 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     const int nr2c = (h1 + G - 1) / G + (h2 + G - 1) / G;
     const int ncol = (H + C - 1) / C;
     const int nc2r = (P.hR + G - 1) / G;
-    const int ns1 = nstrips_of(w1), ns2 = nstrips_of(w2);
     const TileSizes t1 = tile_sizes(w1, h1), t2 = tile_sizes(w2, h2);
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n), i_vals = lay.add(sizeof(double) * n);
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
     size_t i_SA = lay.add(t1.S), i_QA = lay.add(t1.Q), i_ZA = lay.add(t1.Z);
     size_t i_SB = lay.add(t2.S), i_QB = lay.add(t2.Q), i_ZB = lay.add(t2.Z);
-    size_t i_pA = lay.add(t1.part), i_rA = lay.add(t1.rowtot), i_cA = lay.add(t1.carry), i_mA = lay.add(t1.msd);
-    size_t i_pB = lay.add(t2.part), i_rB = lay.add(t2.rowtot), i_cB = lay.add(t2.carry), i_mB = lay.add(t2.msd);
+    size_t i_pA = lay.add(t1.part), i_mA = lay.add(t1.msd);
+    size_t i_pB = lay.add(t2.part), i_mB = lay.add(t2.msd);
     size_t i_spA = lay.add(sizeof(cpx<double>) * h1 * H), i_spB = lay.add(sizeof(cpx<double>) * h2 * H);
     size_t i_lag = lay.add(sizeof(cpx<double>) * (size_t)P.hR * H);
     size_t i_rows = lay.add(sizeof(RowRec) * P.hR);
@@ -85,8 +84,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     // uploaded block 2: pair, tiles, work lists, table
     size_t i_pair = lay.add(sizeof(PairDev));
     size_t i_tiles = lay.add(sizeof(TileDev) * 2);
-    size_t i_tpw = lay.add(sizeof(int2) * (ns1 + ns2));
     size_t i_wr = lay.add(sizeof(int4) * nr2c), i_wc = lay.add(sizeof(int2) * ncol), i_wc2 = lay.add(sizeof(int2) * nc2r);
+    size_t i_slist = lay.add(sizeof(int) * 2);
     size_t i_tbl = lay.add(tbl ? tbl : 1);
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
@@ -109,9 +108,6 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
         D.Q = (double *)at(k ? i_QB : i_QA);
         D.Z = (int32_t *)at(k ? i_ZB : i_ZA);
         D.part = (double *)at(k ? i_pB : i_pA);
-        D.rowtot = (double *)at(k ? i_rB : i_rA);
-        D.carry = (double *)at(k ? i_cB : i_cA);
-        D.nstrips = k ? ns2 : ns1;
         D.normalize = 0;
     }
     P.A = T[0].v;
@@ -146,9 +142,6 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     memcpy(h1p(i_vals) + sizeof(double) * n1, iv2, sizeof(double) * n2);
     memcpy(h2p(i_pair), &P, sizeof P);
     memcpy(h2p(i_tiles), T, sizeof T);
-    int2 *tw = (int2 *)h2p(i_tpw);
-    for (int s = 0; s < ns1; ++s) tw[s] = make_int2(0, s);
-    for (int s = 0; s < ns2; ++s) tw[ns1 + s] = make_int2(1, s);
     int4 *wr = (int4 *)h2p(i_wr);
     int k = 0;
     for (int y = 0; y < h1; y += G) wr[k++] = make_int4(0, 0, y, 0);
@@ -158,6 +151,11 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     int2 *wc2 = (int2 *)h2p(i_wc2);
     for (int i = 0; i < nc2r; ++i) wc2[i] = make_int2(0, i * G);
     if (tbl) memcpy(h2p(i_tbl), rgn->table, tbl);
+    std::vector<SatGroup> sgroups;
+    std::vector<int> sflat, soff;
+    make_sat_groups({w1, w2}, sgroups, sflat, soff);
+    memcpy(h2p(i_slist), sflat.data(), sizeof(int) * sflat.size());
+    for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];
 
     hipStream_t s = ctx->stream;
     XC_HIP(ctx, hipMemcpyAsync(base + s1_lo, hs, s1_hi - s1_lo, hipMemcpyHostToDevice, s));
@@ -182,8 +180,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     ex.fp64 = true;
     ex.ntiles = 2;
     ex.d_tiles = (const TileDev *)at(i_tiles);
-    ex.d_tp_work = (const int2 *)at(i_tpw);
-    ex.n_tp_work = ns1 + ns2;
+    ex.sat_groups = sgroups;
     ex.d_pairs = (const PairDev *)at(i_pair);
     Bucket b;
     b.NH = NH;

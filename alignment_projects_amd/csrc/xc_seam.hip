@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 {
     using PL = Plan<NH>;
     constexpr int E = PL::E, TPF = PL::TPF, G = RowCfg<NH>::G;
-    __shared__ cpx<T> smem[G * NH];
+    __shared__ cpx<T> smem[G * lds_len(NH)];
     const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
     const int y = blockIdx.x * G + g;
     const bool valid = y < rows;
@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
         v[m].x = valid ? src[2 * p] : (T)0;
         v[m].y = valid ? src[2 * p + 1] : (T)0;
     }
-    LdsView<T> L{smem + g * NH, 1};
+    LdsView<T> L{smem + g * lds_len(NH), 1};
     fft<-1, NH>(v, t, L, tw);
     __syncthreads();
 #pragma unroll
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
 {
     using PL = Plan<NY>;
     constexpr int E = PL::E, TPF = PL::TPF, C = ColCfg<T, NY>::C;
-    __shared__ cpx<T> smem[C * NY];
+    __shared__ cpx<T> smem[C * lds_len(NY)];
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
     const int k = blockIdx.x * C + c;
     const bool kv = k < H;
@@ -81,11 +81,11 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 {
     using PL = Plan<NH>;
     constexpr int E = PL::E, TPF = PL::TPF, G = RowCfg<NH>::G;
-    __shared__ cpx<T> smem[G * (NH + 1)];
+    __shared__ cpx<T> smem[G * C2R_ROW(NH)];
     const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
     const int y = blockIdx.x * G + g;
     const bool valid = y < rows;
-    cpx<T> *X = smem + g * (NH + 1);
+    cpx<T> *X = smem + g * C2R_ROW(NH);
     for (int i = t; i < NH + 1; i += TPF) X[i] = valid ? in[(long long)y * (NH + 1) + i] : cpx<T>{(T)0, (T)0};
     __syncthreads();
     cpx<T> v[E];

@@ -60,14 +60,12 @@ struct TileDev {
     double *S, *Q;       // out: FP64 summed-area tables, w x h
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
-    double *rowtot;      // scratch: [3][nstrips][h] strip row totals
-    double *carry;       // scratch: [3][nstrips][h] carries
-    int32_t nstrips;
     int32_t normalize;
+    int32_t pad;
 };
 
-constexpr int TP_PARTS = 16;  // statistics partials per image
-constexpr int TP_STRIP = 64;  // summed-area strip width (one wavefront)
+constexpr int TP_PARTS = 16;     // statistics partials per image
+constexpr int TP_MAX_W = 8192;   // widest image the table sweep handles
 
 // One correlation, as the kernels see it.  Images are compact w x h rasters
 // (row stride w) whose origin is the point list's bounding-box corner; the

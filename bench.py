@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--plain-sample", type=int, default=64, help="pairs run through the plain FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--stage-iters", type=int, default=5)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB first (a known-size kernel for counter scaling)")
     return ap.parse_args()
 
 
@@ -79,6 +81,14 @@ def main():
     rgn = abi.rgn_big_enough(H // 2, W * H // 4)
 
     sync = _device_sync()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(args.calibrate_copy * 2 ** 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
     ctx = xcorr.Context(0)
     tiles = [(ctx.tile_put(A[i]), ctx.tile_put(B[i])) for i in range(NP)]
     pairs = [xcorr.make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
