@@ -46,3 +46,15 @@ def test_no_device_is_an_error_not_a_crash():
     h = ctypes.c_void_p()
     st = xcorr.load().xc_ctx_create(0, ctypes.byref(h))
     assert st == abi.XC_E_DEVICE
+
+
+def test_compat_library_exports_reference_signatures():
+    import subprocess
+    lib = os.path.join(ROOT, "alignment_projects_amd", "lib", "libaln_corr.so")
+    syms = subprocess.run(["nm", "-D", "-C", "--defined-only", lib], capture_output=True, text=True,
+                          check=True).stdout
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "aln_corr_compat.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b([A-Z][A-Za-z0-9_]+)\s*\(", text)) - {"Point"})
+    assert {"CorrPatches", "CorrPatchToImage", "CorrImagesR", "CorrImagesS", "FFT_2D", "IFT_2D"} <= set(names)
+    missing = [n for n in names if f" {n}(" not in syms]
+    assert not missing, missing
