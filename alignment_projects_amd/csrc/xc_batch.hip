@@ -5,6 +5,7 @@
 // (Correlation_fft_fftw.cpp:50-51, CThmScan.h:27-34), made explicit.  A plan
 // is a fixed batch of pair correlations laid out once (descriptors, work
 // lists, spectra, lag rows) and replayed as one HIP graph.
+#include <math.h>
 #include <string.h>
 
 #include <algorithm>
@@ -27,6 +28,7 @@ struct xc_tile {
     TileDev dev;               // host copy, device pointers
     TileDev *d_dev = nullptr;  // device copy
     int64_t bytes = 0;
+    std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
 };
 
 struct xc_plan {
@@ -131,6 +133,30 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
         return set_err(ctx, XC_E_DEVICE, "tile descriptor upload failed");
     }
     t->bytes = (int64_t)t->mem.bytes;
+    // per-row present extents (rotated bounding boxes are computed from them)
+    t->rx0.assign(h, 0);
+    t->rx1.assign(h, w - 1);
+    if (mask) {
+        std::vector<uint8_t> hm((size_t)w * h);
+        if (dev_src) {
+            if (hipMemcpy(hm.data(), mask, hm.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+                xc_tile_release(ctx, t);
+                return set_err(ctx, XC_E_DEVICE, "tile mask download failed");
+            }
+        } else {
+            memcpy(hm.data(), mask, hm.size());
+        }
+        for (int y = 0; y < h; ++y) {
+            int a = -1, b = -1;
+            for (int x = 0; x < w; ++x)
+                if (hm[(size_t)y * w + x]) {
+                    if (a < 0) a = x;
+                    b = x;
+                }
+            t->rx0[y] = a;
+            t->rx1[y] = b;
+        }
+    }
     // prepare: statistics + summed-area tables of the FP32 values
     Exec ex;
     ex.ntiles = 1;
@@ -171,23 +197,105 @@ extern "C" int64_t xc_tile_bytes(const xc_tile *t) { return t ? t->bytes : 0; }
 
 extern "C" int xc_plan_destroy(xc_plan *p);
 
+// Host side of the rotated bounding box (BBoxFromPoints over the rotated
+// present pixels, Geometry.cpp:78-106).  x' and y' are monotone in x along a
+// row, so each row's first and last present pixels carry its extremes.
+#pragma clang fp contract(off)
+static void rotated_bbox(const xc_tile *t, double t0, double t1, double t3, double t4, int &L, int &R, int &B, int &T)
+{
+    double xlo = 1e300, xhi = -1e300, ylo = 1e300, yhi = -1e300;
+    for (int y = 0; y < t->h; ++y) {
+        const int a = t->rx0[y], b = t->rx1[y];
+        if (a < 0) continue;
+        for (int x : {a, b}) {
+            const double xr = (double)x * t0 + (double)y * t1, yr = (double)x * t3 + (double)y * t4;
+            xlo = std::min(xlo, xr);
+            xhi = std::max(xhi, xr);
+            ylo = std::min(ylo, yr);
+            yhi = std::max(yhi, yr);
+        }
+    }
+    L = (int)floor(xlo);
+    R = (int)ceil(xhi);
+    B = (int)floor(ylo);
+    T = (int)ceil(yhi);
+}
+#pragma clang fp contract(on)
+
+// GenDefs.h:16: the reference's deg -> rad constant.
+static const double REF_PI = 3.14159265358;
+
+struct RotKey {
+    const xc_tile *t;
+    double deg;
+    bool operator<(const RotKey &o) const { return t < o.t || (t == o.t && deg < o.deg); }
+};
+
 static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan *p)
 {
-    // ---- pair geometry ----
+    const bool refresh = (flags & XC_PLAN_REFRESH_TILES) != 0;
+    // ---- pair geometry, rotated copies ----
     std::vector<PairDev> P(n);
     std::vector<const xc_tile *> tiles;
     std::map<const xc_tile *, int> tile_idx;
+    std::vector<RotDev> rots;
+    std::map<RotKey, int> rot_idx;
+    std::vector<int> pair_rot(n, -1);
+    std::vector<bool> rot_cnt;  // the rotated copy needs a count table
     for (int i = 0; i < n; ++i) {
         const xc_pair &q = pairs[i];
         const xc_tile *a = q.a, *b = q.b;
         if (!a || !b) return set_err(ctx, XC_E_ARG, "pair %d: null tile", i);
-        if (q.angle_deg != 0.0) return set_err(ctx, XC_E_UNSUPPORTED, "pair %d: rotated pairs go through xc_sweep_angles_f32", i);
         if (q.mode != XC_MODE_NCC && q.mode != XC_MODE_R && q.mode != XC_MODE_S)
             return set_err(ctx, XC_E_ARG, "pair %d: bad mode", i);
+        if (q.rgn.enabled && q.rgn.table)
+            return set_err(ctx, XC_E_UNSUPPORTED, "pair %d: tabulated LegalRgn in batched form", i);
+        for (const xc_tile *t : {a, b})
+            if (!tile_idx.count(t)) {
+                tile_idx[t] = (int)tiles.size();
+                tiles.push_back(t);
+            }
+        int wa = a->w, ha = a->h, offx = 0, offy = 0;
+        if (q.angle_deg != 0.0) {
+            const RotKey key{a, q.angle_deg};
+            auto it = rot_idx.find(key);
+            int ri;
+            if (it == rot_idx.end()) {
+                // RFromAngle: R.NUSetRot(deg * PI / 180) (TAffine.cpp:116-122); Tdfm = Tptwk = identity
+                const double r = q.angle_deg * REF_PI / 180.0, c = cos(r), s = sin(r);
+                RotDev R;
+                memset(&R, 0, sizeof R);
+                R.src = a->dev.v;
+                R.t0 = c;
+                R.t1 = -s;
+                R.t3 = s;
+                R.t4 = c;
+                int L, Rr, B, T;
+                rotated_bbox(a, R.t0, R.t1, R.t3, R.t4, L, Rr, B, T);
+                if (L > Rr) return set_err(ctx, XC_E_ARG, "pair %d: tile A has no present pixel", i);
+                R.L = L;
+                R.B = B;
+                R.w = Rr - L + 1;
+                R.h = T - B + 1;
+                if (R.w > TP_MAX_W) return set_err(ctx, XC_E_SHAPE, "pair %d: rotated A wider than %d", i, TP_MAX_W);
+                ri = (int)rots.size();
+                rots.push_back(R);
+                rot_cnt.push_back(false);
+                rot_idx[key] = ri;
+            } else {
+                ri = it->second;
+            }
+            pair_rot[i] = ri;
+            if (q.cnt.enabled) rot_cnt[ri] = true;
+            wa = rots[ri].w;
+            ha = rots[ri].h;
+            offx = -rots[ri].L;  // B2.L - B1.L with B's corner at 0
+            offy = -rots[ri].B;
+        }
         PairDev &D = P[i];
         memset(&D, 0, sizeof D);
         const int mode = q.mode == XC_MODE_NCC ? PM_NCC : (q.mode == XC_MODE_R ? PM_R : PM_S);
-        setup_pair_geometry(D, a->w, a->h, b->w, b->h, mode);
+        setup_pair_geometry(D, wa, ha, b->w, b->h, mode);
         if (mode != PM_NCC) {
             const int md = 11 / 2 + 1;
             if (D.nnegx < md || D.nposx < md || D.nnegy < md || D.nposy < md)
@@ -195,13 +303,13 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         }
         if (D.Nx / 2 > max_nh<float>() || D.Ny > max_ny<float>())
             return set_err(ctx, XC_E_SHAPE, "pair %d: padded size %d x %d too large", i, D.Nx, D.Ny);
-        if (q.rgn.enabled && q.rgn.table)
-            return set_err(ctx, XC_E_UNSUPPORTED, "pair %d: tabulated LegalRgn in batched form", i);
         D.Ox = q.Ox;
         D.Oy = q.Oy;
         D.radius = q.radius;
         D.Rx = q.Rx;
         D.Ry = q.Ry;
+        D.offx = offx;
+        D.offy = offy;
         D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
         D.need_cnt = D.rule.cnt_enabled;
         D.A = a->dev.v;
@@ -212,59 +320,110 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         D.SB = b->dev.S;
         D.QB = b->dev.Q;
         D.ZB = b->dev.Z;
-        for (const xc_tile *t : {a, b})
-            if (!tile_idx.count(t)) {
-                tile_idx[t] = (int)tiles.size();
-                tiles.push_back(t);
-            }
     }
     std::vector<Bucket> buckets;
     make_buckets(P, 0, p->order, buckets);
 
-    // ---- layout ----
-    const bool refresh = (flags & 1) != 0;
-    Layout lay;
-    std::vector<size_t> i_spA(n), i_spB(n), i_lag(n), i_rows(n);
+    // ---- shared row spectra: one per (image, Nx); image = tile or rotated copy ----
+    struct SpecKey {
+        const void *img;
+        int nx;
+        bool operator<(const SpecKey &o) const { return img < o.img || (img == o.img && nx < o.nx); }
+    };
+    std::map<SpecKey, int> spec_idx;
+    std::vector<size_t> spec_bytes;
+    std::vector<int> specA(n), specB(n);
+    std::vector<bool> ownA(n, false), ownB(n, false);  // this pair's r2c produces the spectrum
     const size_t cs = sizeof(cpx<float>);
     for (int k = 0; k < n; ++k) {
-        const PairDev &D = P[p->order[k]];
+        const int i = p->order[k];
+        const PairDev &D = P[i];
         const size_t H = D.Nx / 2 + 1;
-        i_spA[k] = lay.add(cs * H * D.h1);
-        i_spB[k] = lay.add(cs * H * D.h2);
-        i_lag[k] = lay.add(cs * H * D.hR);
+        const void *ia = pair_rot[i] >= 0 ? (const void *)&rots[pair_rot[i]] : (const void *)pairs[i].a;
+        const void *ib = (const void *)pairs[i].b;
+        for (int side = 0; side < 2; ++side) {
+            const SpecKey key{side ? ib : ia, D.Nx};
+            auto it = spec_idx.find(key);
+            int si;
+            if (it == spec_idx.end()) {
+                si = (int)spec_bytes.size();
+                spec_idx[key] = si;
+                spec_bytes.push_back(cs * H * (side ? D.h2 : D.h1));
+                (side ? ownB : ownA)[i] = true;
+            } else {
+                si = it->second;
+            }
+            (side ? specB : specA)[i] = si;
+        }
+    }
+
+    // ---- layout ----
+    Layout lay;
+    std::vector<size_t> i_spec(spec_bytes.size());
+    for (size_t s = 0; s < spec_bytes.size(); ++s) i_spec[s] = lay.add(spec_bytes[s]);
+    std::vector<size_t> i_lag(n), i_rows(n);
+    for (int k = 0; k < n; ++k) {
+        const PairDev &D = P[p->order[k]];
+        i_lag[k] = lay.add(cs * (size_t)(D.Nx / 2 + 1) * D.hR);
         i_rows[k] = lay.add(sizeof(RowRec) * D.hR);
     }
+    // rotated copies: raster + tables
+    std::vector<size_t> i_rimg(rots.size()), i_rS(rots.size()), i_rQ(rots.size()), i_rZ(rots.size());
+    std::vector<bool> rot_sq(rots.size(), false);
+    for (int i = 0; i < n; ++i)
+        if (pair_rot[i] >= 0 && pairs[i].mode != XC_MODE_S) rot_sq[pair_rot[i]] = true;
+    for (size_t r = 0; r < rots.size(); ++r) {
+        const size_t px = (size_t)rots[r].w * rots[r].h;
+        i_rimg[r] = lay.add(sizeof(float) * px);
+        i_rS[r] = lay.add(rot_sq[r] ? sizeof(double) * px : 8);
+        i_rQ[r] = lay.add(rot_sq[r] ? sizeof(double) * px : 8);
+        i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
+    }
     size_t i_res = lay.add(sizeof(xc_result) * n);
-    // uploaded block: pairs, tiles, work lists
+    // uploaded block: constants, pairs, tiles, rotations, work lists
+    size_t i_msd = lay.add(sizeof(double) * 2);
     size_t i_pairs = lay.add(sizeof(PairDev) * n);
     size_t i_tiles = lay.add(sizeof(TileDev) * (refresh ? tiles.size() : 0) + 16);
-    std::vector<SatGroup> sgroups;
-    std::vector<int> sflat, soff;
+    std::vector<SatGroup> sgroups, dgroups;
+    std::vector<int> sflat, soff, dflat, doff;
     if (refresh) {
         std::vector<int> widths;
         for (const xc_tile *t : tiles) widths.push_back(t->w);
         make_sat_groups(widths, sgroups, sflat, soff);
     }
+    {
+        std::vector<int> widths;
+        for (const RotDev &R : rots) widths.push_back(R.w);
+        make_sat_groups(widths, dgroups, dflat, doff);
+    }
     size_t i_slist = lay.add(sizeof(int) * sflat.size() + 16);
+    size_t i_dlist = lay.add(sizeof(int) * dflat.size() + 16);
+    size_t i_rot = lay.add(sizeof(RotDev) * rots.size() + 16);
+    size_t i_dtiles = lay.add(sizeof(TileDev) * rots.size() + 16);
+    int nrw = 0;
+    for (const RotDev &R : rots) nrw += (int)(((long long)R.w * R.h + 255) / 256);
+    size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
         const int G = row_group<float>(B.NH), C = col_group<float>(B.NY);
         int nr = 0, nc = 0, nc2 = 0;
         for (int k = B.first; k < B.first + B.count; ++k) {
-            const PairDev &D = P[p->order[k]];
-            nr += (D.h1 + G - 1) / G + (D.h2 + G - 1) / G;
+            const int i = p->order[k];
+            const PairDev &D = P[i];
+            if (ownA[i]) nr += (D.h1 + G - 1) / G;
+            if (ownB[i]) nr += (D.h2 + G - 1) / G;
             nc += (D.Nx / 2 + 1 + C - 1) / C;
             nc2 += (D.hR + G - 1) / G;
         }
         B.n_r2c = nr;
         B.n_col = nc;
         B.n_c2r = nc2;
-        i_wr[bi] = lay.add(sizeof(int4) * nr);
+        i_wr[bi] = lay.add(sizeof(int4) * nr + 16);
         i_wc[bi] = lay.add(sizeof(int2) * nc);
         i_wc2[bi] = lay.add(sizeof(int2) * nc2);
     }
-    const size_t up_lo = lay.off[i_pairs];
+    const size_t up_lo = lay.off[i_msd];
     const size_t up_hi = lay.total;
     int st = ensure_dev(ctx, p->ws, lay.total);
     if (st) return st;
@@ -274,22 +433,66 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     // ---- host copy of the uploaded block ----
     std::vector<char> up(up_hi - up_lo);
     auto hat = [&](size_t i) { return up.data() + (lay.off[i] - up_lo); };
+    double *hmsd = (double *)hat(i_msd);
+    hmsd[0] = 0.0;
+    hmsd[1] = 1.0;
+    const double *dmsd = (const double *)at(i_msd);
+    // rotated copies and their table descriptors
+    RotDev *hr = (RotDev *)hat(i_rot);
+    TileDev *hd = (TileDev *)hat(i_dtiles);
+    int2 *hrw = (int2 *)hat(i_rw);
+    int w = 0;
+    for (size_t r = 0; r < rots.size(); ++r) {
+        RotDev R = rots[r];
+        R.out = at(i_rimg[r]);
+        hr[r] = R;
+        for (int b = 0; b < (int)(((long long)R.w * R.h + 255) / 256); ++b) hrw[w++] = make_int2((int)r, b);
+        TileDev T;
+        memset(&T, 0, sizeof T);
+        T.v.data = R.out;
+        T.v.dtype = XC_F32;
+        T.v.w = R.w;
+        T.v.h = R.h;
+        T.v.stride = R.w;
+        T.v.mstride = R.w;
+        T.v.msd = dmsd;
+        T.msd = nullptr;
+        T.S = rot_sq[r] ? (double *)at(i_rS[r]) : nullptr;
+        T.Q = rot_sq[r] ? (double *)at(i_rQ[r]) : nullptr;
+        T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
+        hd[r] = T;
+        rots[r] = R;
+    }
+    if (!dflat.empty()) memcpy(hat(i_dlist), dflat.data(), sizeof(int) * dflat.size());
+    for (size_t gi = 0; gi < dgroups.size(); ++gi) dgroups[gi].list = (const int *)at(i_dlist) + doff[gi];
+
     PairDev *hp = (PairDev *)hat(i_pairs);
     p->d_res = (xc_result *)at(i_res);
     for (int k = 0; k < n; ++k) {
-        PairDev D = P[p->order[k]];
-        D.specA = at(i_spA[k]);
-        D.specB = at(i_spB[k]);
+        const int i = p->order[k];
+        PairDev D = P[i];
+        if (pair_rot[i] >= 0) {
+            const TileDev &T = hd[pair_rot[i]];
+            D.A = T.v;
+            D.SA = T.S;
+            D.QA = T.Q;
+            D.ZA = T.Z;
+        }
+        D.specA = at(i_spec[specA[i]]);
+        D.specB = at(i_spec[specB[i]]);
         D.lag = at(i_lag[k]);
         D.rows = (RowRec *)at(i_rows[k]);
-        D.out = p->d_res + p->order[k];
+        D.out = p->d_res + i;
         hp[k] = D;
     }
     if (refresh) {
         // count tables only for images some pair tests with a LegalCnt rule
         std::map<const xc_tile *, bool> needz;
         for (int i = 0; i < n; ++i)
-            if (pairs[i].cnt.enabled) needz[pairs[i].a] = needz[pairs[i].b] = true;
+            if (pairs[i].cnt.enabled) {
+                needz[pairs[i].b] = true;
+                if (pair_rot[i] < 0) needz[pairs[i].a] = true;
+            }
         TileDev *ht = (TileDev *)hat(i_tiles);
         for (size_t ti = 0; ti < tiles.size(); ++ti) {
             ht[ti] = tiles[ti]->dev;
@@ -306,9 +509,12 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         int2 *wc2 = (int2 *)hat(i_wc2[bi]);
         int a = 0, c = 0, d = 0;
         for (int k = B.first; k < B.first + B.count; ++k) {
+            const int i = p->order[k];
             const PairDev &D = hp[k];
-            for (int y = 0; y < D.h1; y += G) wr[a++] = make_int4(k, 0, y, 0);
-            for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
+            if (ownA[i])
+                for (int y = 0; y < D.h1; y += G) wr[a++] = make_int4(k, 0, y, 0);
+            if (ownB[i])
+                for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
             for (int y = 0; y < D.hR; y += G) wc2[d++] = make_int2(k, y);
         }
@@ -324,25 +530,34 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     p->ex.sat_groups = sgroups;
     p->ex.d_pairs = (const PairDev *)at(i_pairs);
     p->ex.buckets = buckets;
+    p->ex.d_rot = (const RotDev *)at(i_rot);
+    p->ex.d_rot_work = (const int2 *)at(i_rw);
+    p->ex.n_rot_work = nrw;
+    p->ex.d_dtiles = (const TileDev *)at(i_dtiles);
+    p->ex.dsat_groups = dgroups;
 
     // ---- algorithmic bytes ----
     for (int s = 0; s < ST_COUNT; ++s) p->stage_bytes[s] = 0.0;
-    std::map<const xc_tile *, bool> prepped;
+    std::map<const void *, bool> prepped, specced;
     p->model_bytes = 0.0;
     for (int i = 0; i < n; ++i) {
         const xc_pair &q = pairs[i];
+        const PairDev &D = P[i];
+        const bool rotated = pair_rot[i] >= 0;
+        const int ea = rotated ? 4 : elem_size(q.a->dtype), eb = elem_size(q.b->dtype);
         double b[ST_COUNT];
-        const bool pa = refresh && !prepped[q.a];
-        prepped[q.a] = true;
+        const bool pa = (refresh || rotated) && !prepped[rotated ? (const void *)&rots[pair_rot[i]] : q.a];
+        prepped[rotated ? (const void *)&rots[pair_rot[i]] : q.a] = true;
         const bool pb = refresh && !prepped[q.b];
         prepped[q.b] = true;
-        pair_stage_bytes(P[i], (int)cs, elem_size(q.a->dtype), elem_size(q.b->dtype), pa, pb, b);
+        pair_stage_bytes(D, (int)cs, ea, eb, pa, pb, b);
+        // a shared row spectrum is produced (and its input read) once
+        const double H = D.Nx / 2 + 1;
+        if (!ownA[i]) b[ST_R2C] -= ea * (double)D.w1 * D.h1 + cs * H * D.h1;
+        if (!ownB[i]) b[ST_R2C] -= eb * (double)D.w2 * D.h2 + cs * H * D.h2;
         for (int s = 0; s < ST_COUNT; ++s) p->stage_bytes[s] += b[s];
-        // whole-path model: inputs once, row spectra, lag rows, tables written and read once
-        double m[ST_COUNT];
-        pair_stage_bytes(P[i], (int)cs, elem_size(q.a->dtype), elem_size(q.b->dtype), true, true, m);
-        p->model_bytes += m[ST_PREP] - (double)elem_size(q.a->dtype) * P[i].w1 * P[i].h1 -
-                          (double)elem_size(q.b->dtype) * P[i].w2 * P[i].h2 + m[ST_R2C] + m[ST_COL] + m[ST_C2R];
+        p->model_bytes += b[ST_PREP] + b[ST_R2C] + b[ST_COL] + b[ST_C2R] -
+                          (pa ? ea * (double)D.w1 * D.h1 : 0.0) - (pb ? eb * (double)D.w2 * D.h2 : 0.0);
     }
 
     // ---- pinned results + graph ----
@@ -458,6 +673,34 @@ extern "C" int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_re
     if (!results) return XC_E_ARG;
     xc_plan *p = nullptr;
     int st = xc_plan_create(ctx, pairs, n, 0, &p);
+    if (st) return st;
+    st = xc_plan_run(p, results);
+    xc_plan_destroy(p);
+    return st;
+}
+
+extern "C" int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *angles_deg,
+                                   int n_angles, const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, int Ox, int Oy,
+                                   int Rx, int Ry, xc_result *results)
+{
+    if (!ctx || !a || !b || !angles_deg || n_angles <= 0 || !results) return XC_E_ARG;
+    std::vector<xc_pair> pairs(n_angles);
+    for (int i = 0; i < n_angles; ++i) {
+        xc_pair &q = pairs[i];
+        memset(&q, 0, sizeof q);
+        q.a = a;
+        q.b = b;
+        q.mode = XC_MODE_S;
+        q.Ox = Ox;
+        q.Oy = Oy;
+        q.Rx = Rx;
+        q.Ry = Ry;
+        q.angle_deg = angles_deg[i];  // R(0) is the identity: 0 deg needs no rotated copy
+        if (rgn) q.rgn = *rgn;
+        if (cnt) q.cnt = *cnt;
+    }
+    xc_plan *p = nullptr;
+    int st = xc_plan_create(ctx, pairs.data(), n_angles, 0, &p);
     if (st) return st;
     st = xc_plan_run(p, results);
     xc_plan_destroy(p);

@@ -74,6 +74,8 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     bytes[ST_FIN] = sizeof(RowRec) * (double)P.hR;
 }
 
+template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles, const std::vector<SatGroup> &groups);
+
 template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, int s1)
 {
     hipStream_t s = ctx->stream;
@@ -81,12 +83,41 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     if (s0 <= ST_PREP && ST_PREP < s1 && ex.ntiles > 0) {
         hipLaunchKernelGGL(k_tp_stats, dim3(ex.ntiles * TP_PARTS), dim3(256), 0, s, ex.d_tiles);
         hipLaunchKernelGGL(k_tp_stats_fin, dim3((ex.ntiles + 63) / 64), dim3(64), 0, s, ex.d_tiles, ex.ntiles);
-        for (const auto &g : ex.sat_groups) {
+        if ((st = launch_sats<T>(ctx, ex.d_tiles, ex.sat_groups))) return st;
+        XC_HIP(ctx, hipGetLastError());
+    }
+    if (s0 <= ST_PREP && ST_PREP < s1 && ex.n_rot_work > 0) {
+        hipLaunchKernelGGL(k_rot_splat<T>, dim3(ex.n_rot_work), dim3(256), 0, s, ex.d_rot, ex.d_rot_work);
+        if ((st = launch_sats<T>(ctx, ex.d_dtiles, ex.dsat_groups))) return st;
+        XC_HIP(ctx, hipGetLastError());
+    }
+    // stage-major: a row spectrum shared by pairs of several buckets is
+    // produced before any bucket's column pass reads it
+    if (s0 <= ST_R2C && ST_R2C < s1)
+        for (const Bucket &b : ex.buckets)
+            if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, ex.d_pairs, b.d_r2c))) return st;
+    if (s0 <= ST_COL && ST_COL < s1)
+        for (const Bucket &b : ex.buckets)
+            if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.n_col, ex.d_pairs, b.d_col))) return st;
+    if (s0 <= ST_C2R && ST_C2R < s1)
+        for (const Bucket &b : ex.buckets)
+            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r))) return st;
+    if (s0 <= ST_FIN && ST_FIN < s1)
+        for (const Bucket &b : ex.buckets)
+            if ((st = launch_finalize<T>(ctx, b.count, ex.d_pairs + b.first))) return st;
+    return XC_OK;
+}
+
+template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
+{
+    hipStream_t s = ctx->stream;
+    for (const auto &g : groups) {
+        {
             const int key = g.cpt * 10000 + g.threads;
             switch (key) {
 #define XC_SAT(c, n)                                                                                                \
     case c * 10000 + n:                                                                                             \
-        hipLaunchKernelGGL((k_tp_sat<T, c, n>), dim3(g.count), dim3(n), 0, s, ex.d_tiles, g.list);                 \
+        hipLaunchKernelGGL((k_tp_sat<T, c, n>), dim3(g.count), dim3(n), 0, s, d_tiles, g.list);                 \
         break;
                 XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 192) XC_SAT(2, 256) XC_SAT(2, 320) XC_SAT(2, 384)
                 XC_SAT(2, 448) XC_SAT(2, 512) XC_SAT(2, 576) XC_SAT(2, 640) XC_SAT(2, 704) XC_SAT(2, 768)
@@ -99,18 +130,6 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
             default: return set_err(ctx, XC_E_SHAPE, "no table sweep for %d threads x %d columns", g.threads, g.cpt);
             }
         }
-        XC_HIP(ctx, hipGetLastError());
-    }
-    for (const Bucket &b : ex.buckets) {
-        const PairDev *pd = ex.d_pairs;
-        if (s0 <= ST_R2C && ST_R2C < s1)
-            if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, pd, b.d_r2c))) return st;
-        if (s0 <= ST_COL && ST_COL < s1)
-            if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.n_col, pd, b.d_col))) return st;
-        if (s0 <= ST_C2R && ST_C2R < s1)
-            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, pd, b.d_c2r))) return st;
-        if (s0 <= ST_FIN && ST_FIN < s1)
-            if ((st = launch_finalize<T>(ctx, b.count, pd + b.first))) return st;
     }
     return XC_OK;
 }

@@ -8,6 +8,8 @@
 
 namespace xc {
 
+// ST_PREP covers the tiles' statistics and tables and, in sweeps, the
+// rotated copies and their tables.
 enum Stage { ST_PREP = 0, ST_R2C = 1, ST_COL = 2, ST_C2R = 3, ST_FIN = 4, ST_COUNT = 5 };
 
 struct Bucket {
@@ -42,6 +44,12 @@ struct Exec {
     int ntiles = 0;
     const TileDev *d_tiles = nullptr;
     std::vector<SatGroup> sat_groups;
+    // rotated copies (sweeps): splat, then their tables (no statistics)
+    const RotDev *d_rot = nullptr;
+    const int2 *d_rot_work = nullptr;
+    int n_rot_work = 0;
+    const TileDev *d_dtiles = nullptr;
+    std::vector<SatGroup> dsat_groups;
     const PairDev *d_pairs = nullptr;
     std::vector<Bucket> buckets;
     bool fp64 = false;

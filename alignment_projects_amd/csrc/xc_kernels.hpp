@@ -360,7 +360,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *
     const int w = D.v.w, h = D.v.h;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x0 = tid * CPT;
-    const double mean = D.msd[0], sd = D.msd[1];
+    const double mean = D.v.msd[0], sd = D.v.msd[1];
     double cs[CPT], cq[CPT];
     int cz[CPT];
     double nxt[CPT];
@@ -413,8 +413,10 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
             if (x0 + j < w) {
-                D.S[row + x0 + j] = bs + ps[j];
-                D.Q[row + x0 + j] = bq + pq[j];
+                if (D.S) {
+                    D.S[row + x0 + j] = bs + ps[j];
+                    D.Q[row + x0 + j] = bq + pq[j];
+                }
                 if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
             }
         }
@@ -465,6 +467,66 @@ __host__ __device__ inline int sat_threads(int w)
     n = (n + 63) / 64 * 64;
     return n < 64 ? 64 : (n > SAT_MAX_THREADS ? SAT_MAX_THREADS : n);
 }
+
+// ------------------------------------------------------------------
+// Rotated splat, gather form: each output pixel sums the bilinear weights
+// of the grid points that land next to it, in the grid's raster order --
+// the order in which the reference's scatter (DistributePixel, Maths.h:
+// 236-276) adds them -- so the result is bit-identical to the scatter and
+// independent of scheduling.
+// ------------------------------------------------------------------
+
+#pragma clang fp contract(off)
+template <typename T>
+__global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ rots, const int2 *__restrict__ work)
+{
+    const int2 wi = work[blockIdx.x];
+    const RotDev R = rots[wi.x];
+    const long long pix = (long long)wi.y * 256 + threadIdx.x;
+    if (pix >= (long long)R.w * R.h) return;
+    const int py = (int)(pix / R.w), px = (int)(pix - (long long)py * R.w);
+    const double mean = R.src.msd[0], sd = R.src.msd[1];
+    // grid points whose rotated position lies in [px-1, px+1) x [py-1, py+1):
+    // bound them by the inverse rotation of the square's corners, plus a margin
+    double gx0 = 1e300, gx1 = -1e300, gy0 = 1e300, gy1 = -1e300;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double cx = (double)(px - 1 + 2 * (c & 1) + R.L), cy = (double)(py - 1 + 2 * (c >> 1) + R.B);
+        const double gx = cx * R.t0 + cy * R.t3, gy = cx * R.t1 + cy * R.t4;
+        gx0 = fmin(gx0, gx);
+        gx1 = fmax(gx1, gx);
+        gy0 = fmin(gy0, gy);
+        gy1 = fmax(gy1, gy);
+    }
+    const int ix0 = imax(0, (int)floor(gx0) - 1), ix1 = imin(R.src.w - 1, (int)ceil(gx1) + 1);
+    const int iy0 = imax(0, (int)floor(gy0) - 1), iy1 = imin(R.src.h - 1, (int)ceil(gy1) + 1);
+    double acc = 0.0;
+    for (int gy = iy0; gy <= iy1; ++gy) {
+        for (int gx = ix0; gx <= ix1; ++gx) {
+            if (!present(R.src, gx, gy)) continue;
+            const double xr = (double)gx * R.t0 + (double)gy * R.t1;
+            const double yr = (double)gx * R.t3 + (double)gy * R.t4;
+            const double fx = xr - R.L, fy = yr - R.B;
+            const int ix = (int)floor(fx), iy = (int)floor(fy);
+            const double alpha = fx - ix, beta = fy - iy;
+            double wgt;
+            if (ix == px) {
+                if (iy == py) wgt = (1.0 - alpha) * (1.0 - beta);
+                else if (iy + 1 == py) wgt = (1.0 - alpha) * beta;
+                else continue;
+            } else if (ix + 1 == px) {
+                if (iy == py) wgt = alpha * (1.0 - beta);
+                else if (iy + 1 == py) wgt = alpha * beta;
+                else continue;
+            } else {
+                continue;
+            }
+            acc += wgt * ((load_raw(R.src, gx, gy) - mean) / sd);
+        }
+    }
+    reinterpret_cast<T *>(R.out)[pix] = (T)acc;
+}
+#pragma clang fp contract(on)
 
 // ------------------------------------------------------------------
 // Row r2c: one Nx-point real transform per image row, via an Nx/2-point

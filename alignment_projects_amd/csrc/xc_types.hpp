@@ -64,6 +64,18 @@ struct TileDev {
     int32_t pad;
 };
 
+// One rotated copy of an image's pixel grid (CThmScan::RFromAngle,
+// CThmScan.cpp:329-366): points (x, y) -> (x t0 + y t1, x t3 + y t4)
+// (TAffine::Apply_R_Part, TAffine.cpp:407-416), splatted bilinearly at
+// (x' - L, y' - B) into a w x h raster (ImageFromValuesAndPoints).
+struct RotDev {
+    ImgView src;
+    double t0, t1, t3, t4;
+    int32_t L, B;
+    int32_t w, h;
+    void *out;
+};
+
 constexpr int TP_PARTS = 16;     // statistics partials per image
 constexpr int TP_MAX_W = 8192;   // widest image the table sweep handles
 

@@ -146,3 +146,35 @@ def normalize(v):
     avg = s / n
     sd = np.sqrt(max(s2 - n * avg * avg, 0.0) / (n - 1.0))
     return (v - avg) / sd
+
+
+# ----------------------------------------------------------------------------
+# Config 3: thumbnail rotation sweeps.
+# ----------------------------------------------------------------------------
+
+def sweep_angles(lo=-5.0, hi=5.0, step=0.5):
+    """Angle list built as CThmScan builds it: for (a = lo; a <= hi; a += step) (CThmScan.cpp:540)."""
+    out, a = [], lo
+    while a <= hi:
+        out.append(a)
+        a += step
+    return out
+
+
+def rotated_pair(seed, size=96, margin=40, deg=2.0, shift=(3.0, -2.0), mean=0.0, sd=1.0):
+    """A: size x size crop of a band-limited field; B: the field rotated by `deg` about the crop centre and
+    shifted by `shift` (bilinear), same crop.  Values as float64 (thumbnail-like, not quantised)."""
+    rng = np.random.default_rng(seed)
+    n = size + 2 * margin
+    F = bandlimited(rng, n, n, mean, sd)
+    c0 = n / 2.0
+    yy, xx = np.mgrid[0:size, 0:size].astype(np.float64)
+    X, Y = xx + margin, yy + margin
+    th = np.deg2rad(deg)
+    # content at A-position p appears at B-position R(th) (p - c) + c + shift
+    u, v = X - c0 - shift[0], Y - c0 - shift[1]
+    xs = np.cos(th) * u + np.sin(th) * v + c0
+    ys = -np.sin(th) * u + np.cos(th) * v + c0
+    A = F[margin:margin + size, margin:margin + size].copy()
+    B = bilinear(F, xs, ys)
+    return A, B

@@ -212,6 +212,15 @@ class Context:
         self._check(st, "xc_tile_put", ok=(abi.XC_OK,))
         return Tile(self, t, w, h)
 
+    def sweep_angles(self, a, b, angles, rgn=None, cnt=None, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """CThmScan angle sweep: CorrImagesS of A rotated by each angle against B (RFromAngle)."""
+        ang = np.ascontiguousarray(angles, dtype=np.float64)
+        res = (XcResult * ang.size)()
+        st = self.L.xc_sweep_angles_f32(self.h, a.h, b.h, ang.ctypes.data_as(_dp), ang.size, _ref(rgn), _ref(cnt),
+                                        Ox, Oy, Rx, Ry, res)
+        self._check(st, "xc_sweep_angles_f32", ok=(abi.XC_OK,))
+        return list(res)
+
     def plan(self, pairs, refresh_tiles=False):
         return Plan(self, pairs, refresh_tiles)
 
