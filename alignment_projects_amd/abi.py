@@ -125,3 +125,17 @@ def check(status, what="xcorr call", allow=(XC_OK, XC_NO_PEAK, XC_TOO_SMALL), ms
         name = STATUS_NAMES.get(status, str(status))
         raise RuntimeError(f"{what} failed: {name}" + (f" ({msg})" if msg else ""))
     return status
+
+
+class XcServeCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_batch", C.c_int32), ("streams", C.c_int32),
+                ("max_wait_us", C.c_int32), ("_pad", C.c_int32)]
+
+
+class XcServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
+                ("batches", C.c_int64), ("spectra_hits", C.c_int64), ("spectra_misses", C.c_int64),
+                ("arena_bytes", C.c_int64), ("arena_peak", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}

@@ -17,19 +17,6 @@
 
 using namespace xc;
 
-struct xc_tile {
-    xc_ctx *ctx = nullptr;
-    int w = 0, h = 0, dtype = 0, flags = 0;
-    void *raw = nullptr;       // device pixels (owned unless adopted)
-    bool own_raw = false;
-    uint8_t *mask = nullptr;   // device mask (owned unless adopted)
-    bool own_mask = false;
-    DevBuf mem;                // tables + scratch + descriptor
-    TileDev dev;               // host copy, device pointers
-    TileDev *d_dev = nullptr;  // device copy
-    int64_t bytes = 0;
-    std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
-};
 
 struct xc_plan {
     xc_ctx *ctx = nullptr;

@@ -19,17 +19,22 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-// Bump allocator over one grow-only device arena: every call of an entry
-// point lays its buffers out from offset 0 again.
-struct Arena {
-    DevBuf buf;
-    size_t used = 0;
-    std::vector<std::pair<size_t, size_t>> plan;  // (offset, bytes) of each reservation
-};
-
-struct TileRec;
-
 }  // namespace xc
+
+// A device-resident image (see xc_tile_put).
+struct xc_tile {
+    xc_ctx *ctx = nullptr;
+    int w = 0, h = 0, dtype = 0, flags = 0;
+    void *raw = nullptr;       // device pixels (owned unless adopted)
+    bool own_raw = false;
+    uint8_t *mask = nullptr;   // device mask (owned unless adopted)
+    bool own_mask = false;
+    xc::DevBuf mem;                // tables + scratch + descriptor
+    xc::TileDev dev;               // host copy, device pointers
+    xc::TileDev *d_dev = nullptr;  // device copy
+    int64_t bytes = 0;
+    std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
+};
 
 struct xc_ctx {
     int device = 0;

@@ -1,0 +1,561 @@
+// xc_serve.hip -- serving loop: continuous batching of pair-correlation
+// requests over resident tiles, admission by an HBM budget, resident row
+// spectra, batches overlapping on several streams.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "xc_exec.hpp"
+#include "xc_kernels.hpp"
+
+using namespace xc;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double us_since(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
+
+// First-fit allocator over one device allocation (offsets, 256-byte aligned).
+struct ArenaAlloc {
+    char *base = nullptr;
+    size_t cap = 0, used = 0, peak = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+    void init(char *b, size_t c)
+    {
+        base = b;
+        cap = c;
+        free_.clear();
+        free_[0] = c;
+    }
+    static size_t round(size_t n) { return (n + 255) & ~(size_t)255; }
+    bool alloc(size_t n, size_t &off)
+    {
+        n = round(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                used += n;
+                peak = std::max(peak, used);
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = round(n);
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct SpecEntry {
+    size_t off = 0, bytes = 0;
+    int refs = 0;                 // batches in flight that read or write it
+    hipEvent_t ready = nullptr;   // recorded after the producing batch's row pass
+    uint64_t last_use = 0;
+};
+
+struct Req {
+    xc_pair pair;
+    Clock::time_point t_submit;
+    double latency_us = 0.0;
+    xc_result res;
+    bool done = false;
+};
+
+struct Batch {
+    int slot = -1;
+    std::vector<uint64_t> ids;
+    std::vector<std::pair<size_t, size_t>> blocks;  // arena blocks to release
+    std::vector<std::pair<const xc_tile *, int>> specs;  // spectra to deref
+    hipEvent_t done = nullptr;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct xc_server {
+    xc_ctx *ctx = nullptr;  // private context: twiddles, error text, launch stream
+    hipStream_t own_stream = nullptr;  // the private context's stream (slots borrow ctx->stream)
+    xc_serve_cfg cfg;
+    void *arena_mem = nullptr;
+    ArenaAlloc arena;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<uint64_t> pending;
+    std::unordered_map<uint64_t, Req> reqs;
+    std::map<std::pair<const xc_tile *, int>, SpecEntry> specs;
+    std::vector<Slot> slots;
+    std::vector<std::unique_ptr<Batch>> inflight;
+    uint64_t next_id = 1, clock = 0;
+    bool stop = false;
+    std::thread thr;
+    xc_serve_stats st;
+    int err = XC_OK;
+};
+
+namespace {
+
+size_t pair_work_bytes(const PairDev &D)
+{
+    return ArenaAlloc::round(sizeof(cpx<float>) * (size_t)(D.Nx / 2 + 1) * D.hR) + ArenaAlloc::round(sizeof(RowRec) * D.hR);
+}
+
+size_t spec_bytes(const xc_tile *t, int Nx) { return sizeof(cpx<float>) * (size_t)(Nx / 2 + 1) * t->h; }
+
+int make_pair_dev(xc_server *S, const xc_pair &q, PairDev &D)
+{
+    memset(&D, 0, sizeof D);
+    const int mode = q.mode == XC_MODE_NCC ? PM_NCC : (q.mode == XC_MODE_R ? PM_R : PM_S);
+    setup_pair_geometry(D, q.a->w, q.a->h, q.b->w, q.b->h, mode);
+    D.Ox = q.Ox;
+    D.Oy = q.Oy;
+    D.radius = q.radius;
+    D.Rx = q.Rx;
+    D.Ry = q.Ry;
+    D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
+    D.need_cnt = D.rule.cnt_enabled;
+    D.A = q.a->dev.v;
+    D.B = q.b->dev.v;
+    D.SA = q.a->dev.S;
+    D.QA = q.a->dev.Q;
+    D.ZA = q.a->dev.Z;
+    D.SB = q.b->dev.S;
+    D.QB = q.b->dev.Q;
+    D.ZB = q.b->dev.Z;
+    (void)S;
+    return XC_OK;
+}
+
+// Evict least-recently-used idle spectra until `need` bytes can be allocated.
+bool alloc_evicting(xc_server *S, size_t need, size_t &off)
+{
+    while (!S->arena.alloc(need, off)) {
+        auto victim = S->specs.end();
+        for (auto it = S->specs.begin(); it != S->specs.end(); ++it)
+            if (it->second.refs == 0 && (victim == S->specs.end() || it->second.last_use < victim->second.last_use))
+                victim = it;
+        if (victim == S->specs.end()) return false;
+        S->arena.release(victim->second.off, victim->second.bytes);
+        if (victim->second.ready) (void)hipEventDestroy(victim->second.ready);
+        S->specs.erase(victim);
+    }
+    return true;
+}
+
+// Build and launch one batch from the head of the queue (lock held).
+int launch_batch(xc_server *S, int slot_i)
+{
+    Slot &slot = S->slots[slot_i];
+    auto B = std::make_unique<Batch>();
+    B->slot = slot_i;
+    std::vector<PairDev> P;
+    std::vector<std::pair<const xc_tile *, int>> keyA, keyB;
+    std::vector<bool> ownA, ownB;
+    std::vector<hipEvent_t> waits;
+    std::map<std::pair<const xc_tile *, int>, bool> fresh;  // produced by this batch
+    while (!S->pending.empty() && (int)B->ids.size() < S->cfg.max_batch) {
+        const uint64_t id = S->pending.front();
+        Req &r = S->reqs[id];
+        PairDev D;
+        make_pair_dev(S, r.pair, D);
+        // memory: workspace + spectra not yet resident
+        const auto ka = std::make_pair(r.pair.a, D.Nx), kb = std::make_pair(r.pair.b, D.Nx);
+        size_t off_w;
+        if (!alloc_evicting(S, pair_work_bytes(D), off_w)) break;
+        bool ok = true;
+        bool oa = false, ob = false;
+        std::vector<std::pair<const xc_tile *, int>> mine, held;  // spectra this request created / found
+        for (int side = 0; side < 2 && ok; ++side) {
+            const auto key = side ? kb : ka;
+            auto it = S->specs.find(key);
+            if (it == S->specs.end()) {
+                SpecEntry e;
+                e.bytes = spec_bytes(key.first, key.second);
+                if (!alloc_evicting(S, e.bytes, e.off)) {
+                    ok = false;
+                    break;
+                }
+                e.refs = 1;  // held from now on, so later allocations cannot evict it
+                S->specs[key] = e;
+                fresh[key] = true;
+                mine.push_back(key);
+                (side ? ob : oa) = true;
+                S->st.spectra_misses++;
+            } else {
+                it->second.refs++;
+                held.push_back(key);
+                if (!fresh.count(key) && it->second.ready) waits.push_back(it->second.ready);
+                S->st.spectra_hits++;
+            }
+        }
+        if (!ok) {
+            for (auto &key : held) S->specs[key].refs--;
+            for (auto &key : mine) {
+                S->arena.release(S->specs[key].off, S->specs[key].bytes);
+                S->specs.erase(key);
+                fresh.erase(key);
+            }
+            S->arena.release(off_w, pair_work_bytes(D));
+            break;
+        }
+        for (const auto &key : {ka, kb}) {
+            S->specs[key].last_use = ++S->clock;
+            B->specs.push_back(key);
+        }
+        B->blocks.push_back(std::make_pair(off_w, pair_work_bytes(D)));
+        D.lag = S->arena.base + off_w;
+        D.rows = (RowRec *)(S->arena.base + off_w + ArenaAlloc::round(sizeof(cpx<float>) * (size_t)(D.Nx / 2 + 1) * D.hR));
+        D.specA = S->arena.base + S->specs[ka].off;
+        D.specB = S->arena.base + S->specs[kb].off;
+        P.push_back(D);
+        keyA.push_back(ka);
+        keyB.push_back(kb);
+        ownA.push_back(oa);
+        ownB.push_back(ob);
+        B->ids.push_back(id);
+        S->pending.pop_front();
+    }
+    const int n = (int)B->ids.size();
+    if (n == 0) {
+        for (auto &b : B->blocks) S->arena.release(b.first, b.second);
+        return XC_E_NOTREADY;
+    }
+    // ---- descriptors, work lists, results (one arena block + pinned staging) ----
+    std::vector<int> order;
+    std::vector<Bucket> buckets;
+    make_buckets(P, 0, order, buckets);
+    Layout lay;
+    size_t i_pairs = lay.add(sizeof(PairDev) * n), i_res = lay.add(sizeof(xc_result) * n);
+    std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        Bucket &b = buckets[bi];
+        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY);
+        int nr = 0, nc = 0, nc2 = 0;
+        for (int k = b.first; k < b.first + b.count; ++k) {
+            const int i = order[k];
+            const PairDev &D = P[i];
+            if (ownA[i]) nr += (D.h1 + G - 1) / G;
+            if (ownB[i]) nr += (D.h2 + G - 1) / G;
+            nc += (D.Nx / 2 + 1 + C - 1) / C;
+            nc2 += (D.hR + G - 1) / G;
+        }
+        b.n_r2c = nr;
+        b.n_col = nc;
+        b.n_c2r = nc2;
+        i_wr[bi] = lay.add(sizeof(int4) * nr + 16);
+        i_wc[bi] = lay.add(sizeof(int2) * nc);
+        i_wc2[bi] = lay.add(sizeof(int2) * nc2);
+    }
+    size_t off_d;
+    if (!alloc_evicting(S, lay.total, off_d)) {
+        // give the requests back (front of the queue, same order)
+        for (int k = n - 1; k >= 0; --k) S->pending.push_front(B->ids[k]);
+        for (auto &b : B->blocks) S->arena.release(b.first, b.second);
+        for (auto &key : B->specs) S->specs[key].refs--;
+        for (auto &kv : fresh) {  // never computed: drop them
+            S->arena.release(S->specs[kv.first].off, S->specs[kv.first].bytes);
+            S->specs.erase(kv.first);
+        }
+        return XC_E_NOTREADY;
+    }
+    B->blocks.push_back(std::make_pair(off_d, lay.total));
+    char *dbase = S->arena.base + off_d;
+    if (slot.pinned_bytes < lay.total) {
+        if (slot.pinned) (void)hipHostFree(slot.pinned);
+        slot.pinned_bytes = lay.total * 2;
+        if (hipHostMalloc(&slot.pinned, slot.pinned_bytes, hipHostMallocDefault) != hipSuccess) {
+            slot.pinned = nullptr;
+            slot.pinned_bytes = 0;
+            return XC_E_NOMEM;
+        }
+    }
+    char *h = (char *)slot.pinned;
+    PairDev *hp = (PairDev *)(h + lay.off[i_pairs]);
+    xc_result *dres = (xc_result *)(dbase + lay.off[i_res]);
+    for (int k = 0; k < n; ++k) {
+        const int i = order[k];
+        PairDev D = P[i];
+        D.out = dres + i;
+        hp[k] = D;
+    }
+    for (size_t bi = 0; bi < buckets.size(); ++bi) {
+        Bucket &b = buckets[bi];
+        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY);
+        int4 *wr = (int4 *)(h + lay.off[i_wr[bi]]);
+        int2 *wc = (int2 *)(h + lay.off[i_wc[bi]]);
+        int2 *wc2 = (int2 *)(h + lay.off[i_wc2[bi]]);
+        int a = 0, c = 0, d = 0;
+        for (int k = b.first; k < b.first + b.count; ++k) {
+            const int i = order[k];
+            const PairDev &D = hp[k];
+            if (ownA[i])
+                for (int y = 0; y < D.h1; y += G) wr[a++] = make_int4(k, 0, y, 0);
+            if (ownB[i])
+                for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
+            for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
+            for (int y = 0; y < D.hR; y += G) wc2[d++] = make_int2(k, y);
+        }
+        b.d_r2c = (const int4 *)(dbase + lay.off[i_wr[bi]]);
+        b.d_col = (const int2 *)(dbase + lay.off[i_wc[bi]]);
+        b.d_c2r = (const int2 *)(dbase + lay.off[i_wc2[bi]]);
+    }
+    xc_ctx *ctx = S->ctx;
+    ctx->stream = slot.stream;
+    for (hipEvent_t e : waits) XC_HIP(ctx, hipStreamWaitEvent(slot.stream, e, 0));
+    XC_HIP(ctx, hipMemcpyAsync(dbase, h, lay.total, hipMemcpyHostToDevice, slot.stream));
+    Exec ex;
+    ex.fp64 = false;
+    ex.d_pairs = (const PairDev *)(dbase + lay.off[i_pairs]);
+    ex.buckets = buckets;
+    int st = enqueue(ctx, ex, ST_R2C, ST_R2C + 1);
+    if (st) return st;
+    // spectra produced here become readable by later batches after this point
+    for (auto &kv : fresh) {
+        SpecEntry &e = S->specs[kv.first];
+        if (!e.ready) XC_HIP(ctx, hipEventCreateWithFlags(&e.ready, hipEventDisableTiming));
+        XC_HIP(ctx, hipEventRecord(e.ready, slot.stream));
+    }
+    if ((st = enqueue(ctx, ex, ST_COL, ST_COUNT))) return st;
+    // results back into the slot's pinned area (after the uploaded block, reused only when the slot is free)
+    XC_HIP(ctx, hipMemcpyAsync(h + lay.off[i_res], dres, sizeof(xc_result) * n, hipMemcpyDeviceToHost, slot.stream));
+    XC_HIP(ctx, hipEventCreateWithFlags(&B->done, hipEventDisableTiming));
+    XC_HIP(ctx, hipEventRecord(B->done, slot.stream));
+    // the result block's host copy lives at h + lay.off[i_res]; remember it through the Batch
+    B->blocks.push_back(std::make_pair(SIZE_MAX, lay.off[i_res]));
+    slot.busy = true;
+    S->st.batches++;
+    S->inflight.push_back(std::move(B));
+    return XC_OK;
+}
+
+// Deliver finished batches (lock held).  Returns true if any finished.
+bool reap(xc_server *S)
+{
+    bool any = false;
+    for (size_t k = 0; k < S->inflight.size();) {
+        Batch &B = *S->inflight[k];
+        if (hipEventQuery(B.done) != hipSuccess) {
+            ++k;
+            continue;
+        }
+        Slot &slot = S->slots[B.slot];
+        size_t res_off = 0;
+        for (auto &b : B.blocks)
+            if (b.first == SIZE_MAX) res_off = b.second;
+        const xc_result *hr = (const xc_result *)((char *)slot.pinned + res_off);
+        for (size_t i = 0; i < B.ids.size(); ++i) {
+            Req &r = S->reqs[B.ids[i]];
+            r.res = hr[i];
+            r.latency_us = us_since(r.t_submit);
+            r.done = true;
+            S->st.completed++;
+        }
+        for (auto &b : B.blocks)
+            if (b.first != SIZE_MAX) S->arena.release(b.first, b.second);
+        for (auto &key : B.specs) S->specs[key].refs--;
+        (void)hipEventDestroy(B.done);
+        slot.busy = false;
+        S->inflight.erase(S->inflight.begin() + k);
+        any = true;
+    }
+    return any;
+}
+
+void run(xc_server *S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    Clock::time_point oldest_wait = Clock::now();
+    bool waiting = false;
+    for (;;) {
+        bool progress = reap(S);
+        if (progress) S->cv.notify_all();
+        if (!S->pending.empty()) {
+            if (!waiting) {
+                waiting = true;
+                oldest_wait = Clock::now();
+            }
+            const bool full = (int)S->pending.size() >= S->cfg.max_batch;
+            const bool aged = us_since(oldest_wait) >= S->cfg.max_wait_us;
+            if (full || aged || S->stop) {
+                for (size_t si = 0; si < S->slots.size() && !S->pending.empty(); ++si) {
+                    if (S->slots[si].busy) continue;
+                    const int st = launch_batch(S, (int)si);
+                    if (st == XC_OK) {
+                        progress = true;
+                    } else if (st != XC_E_NOTREADY) {
+                        S->err = st;
+                        // fail the queue rather than hang
+                        while (!S->pending.empty()) {
+                            Req &r = S->reqs[S->pending.front()];
+                            r.res.status = st;
+                            r.done = true;
+                            S->pending.pop_front();
+                        }
+                        S->cv.notify_all();
+                    }
+                    break;
+                }
+                if (S->pending.empty()) waiting = false;
+                else oldest_wait = Clock::now();
+            }
+        } else {
+            waiting = false;
+        }
+        if (S->stop && S->pending.empty() && S->inflight.empty()) break;
+        if (!progress) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(S->inflight.empty() ? 50 : 10));
+            lk.lock();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server **out)
+{
+    if (!ctx || !cfg || !out || cfg->hbm_budget <= 0 || cfg->max_batch <= 0 || cfg->streams <= 0) return XC_E_ARG;
+    *out = nullptr;
+    auto S = std::make_unique<xc_server>();
+    S->cfg = *cfg;
+    memset(&S->st, 0, sizeof S->st);
+    int st = xc_ctx_create(ctx->device, &S->ctx);
+    if (st) return st;
+    S->own_stream = S->ctx->stream;
+    if (hipMalloc(&S->arena_mem, (size_t)cfg->hbm_budget) != hipSuccess) {
+        (void)hipGetLastError();
+        xc_ctx_destroy(S->ctx);
+        return set_err(ctx, XC_E_NOMEM, "server arena of %lld bytes", (long long)cfg->hbm_budget);
+    }
+    S->arena.init((char *)S->arena_mem, (size_t)cfg->hbm_budget);
+    S->slots.resize(cfg->streams);
+    for (auto &sl : S->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) return XC_E_DEVICE;
+    }
+    xc_server *p = S.release();
+    p->thr = std::thread(run, p);
+    *out = p;
+    return XC_OK;
+}
+
+extern "C" int xc_server_submit(xc_server *S, const xc_pair *pair, uint64_t *ticket)
+{
+    if (!S || !pair || !ticket || !pair->a || !pair->b) return XC_E_ARG;
+    if (pair->angle_deg != 0.0) return XC_E_UNSUPPORTED;
+    PairDev D;
+    make_pair_dev(S, *pair, D);
+    if (D.Nx / 2 > max_nh<float>() || D.Ny > max_ny<float>()) return XC_E_SHAPE;
+    if (D.mode != PM_NCC && (D.nnegx < 6 || D.nposx < 6 || D.nnegy < 6 || D.nposy < 6)) return XC_E_SHAPE;
+    const size_t need = pair_work_bytes(D) + ArenaAlloc::round(spec_bytes(pair->a, D.Nx)) +
+                        ArenaAlloc::round(spec_bytes(pair->b, D.Nx)) + 65536;
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->st.submitted++;
+    if ((int64_t)need > S->cfg.hbm_budget) {
+        S->st.rejected++;
+        return XC_E_BUDGET;
+    }
+    const uint64_t id = S->next_id++;
+    Req r;
+    r.pair = *pair;
+    r.t_submit = Clock::now();
+    memset(&r.res, 0, sizeof r.res);
+    S->reqs[id] = r;
+    S->pending.push_back(id);
+    *ticket = id;
+    return XC_OK;
+}
+
+extern "C" int xc_server_poll(xc_server *S, uint64_t ticket, xc_result *res)
+{
+    if (!S) return XC_E_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return XC_E_ARG;
+    if (!it->second.done) return XC_E_NOTREADY;
+    if (res) *res = it->second.res;
+    return XC_OK;
+}
+
+extern "C" int xc_server_wait(xc_server *S, uint64_t ticket, xc_result *res, double *latency_us)
+{
+    if (!S) return XC_E_ARG;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->reqs.find(ticket);
+    if (it == S->reqs.end()) return XC_E_ARG;
+    S->cv.wait(lk, [&] { return it->second.done; });
+    if (res) *res = it->second.res;
+    if (latency_us) *latency_us = it->second.latency_us;
+    const int st = it->second.res.status;
+    S->reqs.erase(it);
+    return st < 0 ? st : XC_OK;
+}
+
+extern "C" int xc_server_stats(xc_server *S, xc_serve_stats *out)
+{
+    if (!S || !out) return XC_E_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->st;
+    out->arena_bytes = (int64_t)S->arena.used;
+    out->arena_peak = (int64_t)S->arena.peak;
+    return XC_OK;
+}
+
+extern "C" int xc_server_destroy(xc_server *S)
+{
+    if (!S) return XC_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    if (S->thr.joinable()) S->thr.join();
+    (void)hipSetDevice(S->ctx->device);
+    for (auto &sl : S->slots) {
+        if (sl.stream) {
+            (void)hipStreamSynchronize(sl.stream);
+            (void)hipStreamDestroy(sl.stream);
+        }
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+    }
+    for (auto &kv : S->specs)
+        if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
+    if (S->arena_mem) (void)hipFree(S->arena_mem);
+    xc_ctx *c = S->ctx;
+    c->stream = S->own_stream;
+    delete S;
+    xc_ctx_destroy(c);
+    return XC_OK;
+}

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import XcCntRule, XcPair, XcPoint, XcResult, XcRgnRule
+from .abi import XcCntRule, XcPair, XcPoint, XcResult, XcRgnRule, XcServeCfg, XcServeStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libxcorr.so")
@@ -74,6 +74,12 @@ _OPTIONAL = {
     "xc_plan_stage_times": ([C.c_void_p, C.c_int, _dp, _dp], C.c_int),
     "xc_plan_model_bytes": ([C.c_void_p], C.c_double),
     "xc_plan_destroy": ([C.c_void_p], C.c_int),
+    "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
+    "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
+    "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
+    "xc_server_wait": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult), _dp], C.c_int),
+    "xc_server_stats": ([C.c_void_p, C.POINTER(XcServeStats)], C.c_int),
+    "xc_server_destroy": ([C.c_void_p], C.c_int),
     "xc_batch_corr_f32": ([C.c_void_p, C.POINTER(XcPair), C.c_int, C.POINTER(XcResult)], C.c_int),
     "xc_sweep_angles_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int, C.POINTER(XcRgnRule),
                              C.POINTER(XcCntRule), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(XcResult)],
@@ -297,6 +303,54 @@ class Plan:
     def __del__(self):
         try:
             self.destroy()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop (xc_server): continuous batching of pair requests over resident tiles."""
+
+    def __init__(self, ctx, hbm_budget=8 << 30, max_batch=64, streams=3, max_wait_us=200):
+        self.ctx = ctx
+        cfg = XcServeCfg(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, max_wait_us=max_wait_us)
+        h = C.c_void_p()
+        st = ctx.L.xc_server_create(ctx.h, C.byref(cfg), C.byref(h))
+        ctx._check(st, "xc_server_create", ok=(abi.XC_OK,))
+        self.h = h
+
+    def submit(self, pair):
+        t = C.c_uint64()
+        st = self.ctx.L.xc_server_submit(self.h, C.byref(pair), C.byref(t))
+        if st != abi.XC_OK:
+            raise XcorrError(st, "xc_server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        r = XcResult()
+        lat = C.c_double()
+        st = self.ctx.L.xc_server_wait(self.h, ticket, C.byref(r), C.byref(lat))
+        if st != abi.XC_OK:
+            raise XcorrError(st, "xc_server_wait")
+        return r, lat.value
+
+    def poll(self, ticket):
+        r = XcResult()
+        st = self.ctx.L.xc_server_poll(self.h, ticket, C.byref(r))
+        return (r if st == abi.XC_OK else None)
+
+    def stats(self):
+        s = XcServeStats()
+        self.ctx.L.xc_server_stats(self.h, C.byref(s))
+        return s.as_dict()
+
+    def close(self):
+        if self.h:
+            self.ctx.L.xc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

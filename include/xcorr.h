@@ -291,6 +291,53 @@ int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
                         const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
                         int Ox, int Oy, int Rx, int Ry, xc_result *results);
 
+/* ------------------------------------------------------------------ */
+/* Serving loop: continuous batching of pair requests                 */
+/* ------------------------------------------------------------------ */
+
+/*
+ * A server owns a scheduler thread, `streams` HIP streams and one device
+ * arena of `hbm_budget` bytes.  Requests (xc_pair over resident tiles, no
+ * rotation) are admitted in arrival order while their workspace fits in the
+ * arena; admitted requests are launched as a batch on a free stream as soon
+ * as one is free, so batches overlap on the device and new arrivals join
+ * the next batch while earlier ones run.  Row spectra of tiles are kept
+ * resident in the arena keyed by (tile, Nx) and reused by later requests
+ * (least recently used first out when memory is needed).  A request's
+ * result does not depend on what it was batched with.
+ * This has no single counterpart in the reference, whose pair jobs are
+ * independent processes (make -j on grid-engine nodes); it replaces that
+ * job-level parallelism on one GPU.
+ */
+typedef struct xc_server xc_server;
+
+typedef struct xc_serve_cfg {
+    int64_t hbm_budget;     /* bytes of device memory for workspaces + resident spectra */
+    int32_t max_batch;      /* requests per launched batch */
+    int32_t streams;        /* batches in flight at once */
+    int32_t max_wait_us;    /* a partial batch waits at most this long for company (0: launch at once) */
+    int32_t _pad;
+} xc_serve_cfg;
+
+typedef struct xc_serve_stats {
+    int64_t submitted, completed, rejected, batches;
+    int64_t spectra_hits, spectra_misses;
+    int64_t arena_bytes, arena_peak;
+} xc_serve_stats;
+
+int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server **out);
+/* Submit one request; *ticket identifies it.  XC_E_BUDGET if the request
+ * alone can never fit the budget.  Thread-safe. */
+int xc_server_submit(xc_server *s, const xc_pair *pair, uint64_t *ticket);
+/* XC_OK with *res filled, or XC_E_NOTREADY. */
+int xc_server_poll(xc_server *s, uint64_t ticket, xc_result *res);
+/* Block until the request completes; *latency_us (may be NULL) receives
+ * submit-to-completion time as measured by the server. */
+int xc_server_wait(xc_server *s, uint64_t ticket, xc_result *res, double *latency_us);
+int xc_server_stats(xc_server *s, xc_serve_stats *out);
+/* Drain outstanding requests and stop. */
+int xc_server_destroy(xc_server *s);
+
 #ifdef __cplusplus
 }
 #endif
