@@ -9,6 +9,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <vector>
 
@@ -54,7 +55,9 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     if (stride < w) return set_err(ctx, XC_E_ARG, "stride < width");
     std::lock_guard<std::mutex> lock(ctx->mu);
     XC_HIP(ctx, hipSetDevice(ctx->device));
+    static std::atomic<uint64_t> next_uid{1};
     xc_tile *t = new xc_tile;
+    t->uid = next_uid++;
     t->ctx = ctx;
     t->w = w;
     t->h = h;

@@ -33,6 +33,7 @@ struct xc_tile {
     xc::TileDev dev;               // host copy, device pointers
     xc::TileDev *d_dev = nullptr;  // device copy
     int64_t bytes = 0;
+    uint64_t uid = 0;           // process-unique id (caches key on it, never on the address)
     std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
 };
 

@@ -93,7 +93,7 @@ struct Batch {
     int slot = -1;
     std::vector<uint64_t> ids;
     std::vector<std::pair<size_t, size_t>> blocks;  // arena blocks to release
-    std::vector<std::pair<const xc_tile *, int>> specs;  // spectra to deref
+    std::vector<std::pair<uint64_t, int>> specs;  // spectra to deref
     hipEvent_t done = nullptr;
 };
 
@@ -116,7 +116,7 @@ struct xc_server {
     std::condition_variable cv;
     std::deque<uint64_t> pending;
     std::unordered_map<uint64_t, Req> reqs;
-    std::map<std::pair<const xc_tile *, int>, SpecEntry> specs;
+    std::map<std::pair<uint64_t, int>, SpecEntry> specs;  // (tile uid, Nx) -> row spectrum
     std::vector<Slot> slots;
     std::vector<std::unique_ptr<Batch>> inflight;
     uint64_t next_id = 1, clock = 0;
@@ -182,28 +182,27 @@ int launch_batch(xc_server *S, int slot_i)
     auto B = std::make_unique<Batch>();
     B->slot = slot_i;
     std::vector<PairDev> P;
-    std::vector<std::pair<const xc_tile *, int>> keyA, keyB;
     std::vector<bool> ownA, ownB;
     std::vector<hipEvent_t> waits;
-    std::map<std::pair<const xc_tile *, int>, bool> fresh;  // produced by this batch
+    std::map<std::pair<uint64_t, int>, bool> fresh;  // produced by this batch
     while (!S->pending.empty() && (int)B->ids.size() < S->cfg.max_batch) {
         const uint64_t id = S->pending.front();
         Req &r = S->reqs[id];
         PairDev D;
         make_pair_dev(S, r.pair, D);
         // memory: workspace + spectra not yet resident
-        const auto ka = std::make_pair(r.pair.a, D.Nx), kb = std::make_pair(r.pair.b, D.Nx);
+        const auto ka = std::make_pair(r.pair.a->uid, D.Nx), kb = std::make_pair(r.pair.b->uid, D.Nx);
         size_t off_w;
         if (!alloc_evicting(S, pair_work_bytes(D), off_w)) break;
         bool ok = true;
         bool oa = false, ob = false;
-        std::vector<std::pair<const xc_tile *, int>> mine, held;  // spectra this request created / found
+        std::vector<std::pair<uint64_t, int>> mine, held;  // spectra this request created / found
         for (int side = 0; side < 2 && ok; ++side) {
             const auto key = side ? kb : ka;
             auto it = S->specs.find(key);
             if (it == S->specs.end()) {
                 SpecEntry e;
-                e.bytes = spec_bytes(key.first, key.second);
+                e.bytes = spec_bytes(side ? r.pair.b : r.pair.a, key.second);
                 if (!alloc_evicting(S, e.bytes, e.off)) {
                     ok = false;
                     break;
@@ -241,8 +240,6 @@ int launch_batch(xc_server *S, int slot_i)
         D.specA = S->arena.base + S->specs[ka].off;
         D.specB = S->arena.base + S->specs[kb].off;
         P.push_back(D);
-        keyA.push_back(ka);
-        keyB.push_back(kb);
         ownA.push_back(oa);
         ownB.push_back(ob);
         B->ids.push_back(id);
@@ -416,6 +413,15 @@ void run(xc_server *S)
                     const int st = launch_batch(S, (int)si);
                     if (st == XC_OK) {
                         progress = true;
+                    } else if (st == XC_E_NOTREADY && S->inflight.empty()) {
+                        // nothing in flight and the head request still does not fit: it never will
+                        Req &r = S->reqs[S->pending.front()];
+                        r.res.status = XC_E_BUDGET;
+                        r.done = true;
+                        S->pending.pop_front();
+                        S->st.rejected++;
+                        S->cv.notify_all();
+                        progress = true;
                     } else if (st != XC_E_NOTREADY) {
                         S->err = st;
                         // fail the queue rather than hang
@@ -475,7 +481,8 @@ extern "C" int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server 
 extern "C" int xc_server_submit(xc_server *S, const xc_pair *pair, uint64_t *ticket)
 {
     if (!S || !pair || !ticket || !pair->a || !pair->b) return XC_E_ARG;
-    if (pair->angle_deg != 0.0) return XC_E_UNSUPPORTED;
+    if (pair->angle_deg != 0.0 || (pair->rgn.enabled && pair->rgn.table)) return XC_E_UNSUPPORTED;
+    if (pair->mode != XC_MODE_NCC && pair->mode != XC_MODE_R && pair->mode != XC_MODE_S) return XC_E_ARG;
     PairDev D;
     make_pair_dev(S, *pair, D);
     if (D.Nx / 2 > max_nh<float>() || D.Ny > max_ny<float>()) return XC_E_SHAPE;

@@ -304,7 +304,10 @@ int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
  * the next batch while earlier ones run.  Row spectra of tiles are kept
  * resident in the arena keyed by (tile, Nx) and reused by later requests
  * (least recently used first out when memory is needed).  A request's
- * result does not depend on what it was batched with.
+ * result does not depend on what it was batched with.  Tiles must outlive
+ * the requests that name them; a released tile's cached spectra are never
+ * reused (the cache keys on a per-tile id, not its address).  Tabulated
+ * LegalRgn rules and rotated pairs are XC_E_UNSUPPORTED here.
  * This has no single counterpart in the reference, whose pair jobs are
  * independent processes (make -j on grid-engine nodes); it replaces that
  * job-level parallelism on one GPU.
