@@ -302,6 +302,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         D.offy = offy;
         D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
         D.need_cnt = D.rule.cnt_enabled;
+        narrow_rows(D);
         D.A = a->dev.v;
         D.B = b->dev.v;
         D.SA = a->dev.S;
@@ -371,7 +372,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     }
     size_t i_res = lay.add(sizeof(xc_result) * n);
     // uploaded block: constants, pairs, tiles, rotations, work lists
-    size_t i_msd = lay.add(sizeof(double) * 2);
+    size_t i_msd = lay.add(sizeof(double) * 3);
     size_t i_pairs = lay.add(sizeof(PairDev) * n);
     size_t i_tiles = lay.add(sizeof(TileDev) * (refresh ? tiles.size() : 0) + 16);
     std::vector<SatGroup> sgroups, dgroups;
@@ -394,6 +395,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     for (const RotDev &R : rots) nrw += (int)(((long long)R.w * R.h + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
+    std::vector<int> starts;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
         const int G = row_group<float>(B.NH), C = col_group<float>(B.NY);
@@ -404,7 +406,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             if (ownA[i]) nr += (D.h1 + G - 1) / G;
             if (ownB[i]) nr += (D.h2 + G - 1) / G;
             nc += (D.Nx / 2 + 1 + C - 1) / C;
-            nc2 += (D.hR + G - 1) / G;
+            live_row_groups(D, G, starts);
+            nc2 += (int)starts.size();
         }
         B.n_r2c = nr;
         B.n_col = nc;
@@ -426,6 +429,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     double *hmsd = (double *)hat(i_msd);
     hmsd[0] = 0.0;
     hmsd[1] = 1.0;
+    hmsd[2] = 1.0;
     const double *dmsd = (const double *)at(i_msd);
     // rotated copies and their table descriptors
     RotDev *hr = (RotDev *)hat(i_rot);
@@ -506,7 +510,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             if (ownB[i])
                 for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
-            for (int y = 0; y < D.hR; y += G) wc2[d++] = make_int2(k, y);
+            live_row_groups(D, G, starts);
+            for (int y : starts) wc2[d++] = make_int2(k, y);
         }
         B.d_r2c = (const int4 *)at(i_wr[bi]);
         B.d_col = (const int2 *)at(i_wc[bi]);

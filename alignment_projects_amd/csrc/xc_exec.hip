@@ -36,7 +36,7 @@ TileSizes tile_sizes(int w, int h)
     t.Q = sizeof(double) * n;
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
-    t.msd = sizeof(double) * 2;
+    t.msd = sizeof(double) * 3;
     return t;
 }
 
@@ -69,9 +69,10 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     const double sat = (P.mode == PM_S) ? zb : 16.0 + zb;  // S mode needs only the count table
     bytes[ST_PREP] = (prep_a ? ba * n1 + sat * n1 : 0.0) + (prep_b ? bb * n2 + sat * n2 : 0.0);
     bytes[ST_R2C] = ba * n1 + bb * n2 + s * H * (P.h1 + P.h2);
-    bytes[ST_COL] = s * H * (P.h1 + P.h2) + s * H * P.hR;
-    bytes[ST_C2R] = s * H * P.hR + sat * (n1 + n2);
-    bytes[ST_FIN] = sizeof(RowRec) * (double)P.hR;
+    const double rows = live_rows(P);  // lag rows written by the column pass and read by c2r
+    bytes[ST_COL] = s * H * (P.h1 + P.h2) + s * H * rows;
+    bytes[ST_C2R] = s * H * rows + sat * (n1 + n2);
+    bytes[ST_FIN] = sizeof(RowRec) * rows;
 }
 
 template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles, const std::vector<SatGroup> &groups);

@@ -92,6 +92,15 @@ RuleDev make_rule(const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, const uint8_t 
 // Fill the derived geometry of a PairDev from (w1,h1,w2,h2) and the mode.
 int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode);
 
+// NCC pairs: shrink [ylo, yhi] to the hull of the lag rows where some lag
+// can pass the disc and the rules (call after rule, Ox, Oy, radius are set).
+// Outside it every lag is 0 and no candidate, as the reference's scan leaves
+// it, so those rows are neither written nor evaluated.
+void narrow_rows(PairDev &P);
+// Start rows of the G-row c2r workgroups that cover the live rows.
+void live_row_groups(const PairDev &P, int G, std::vector<int> &starts);
+int live_rows(const PairDev &P);
+
 // Kernel launchers (dispatch on transform length).
 template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work);
 template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nblocks, const PairDev *pd, const int2 *work);

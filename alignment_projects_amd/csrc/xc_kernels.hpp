@@ -28,8 +28,8 @@ namespace xc {
 // Small helpers
 // ------------------------------------------------------------------
 
-__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
-__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+__host__ __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__host__ __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 // "a better than b" in the peak order: larger value, then smaller order key.
 __device__ __forceinline__ bool better(double va, long long ka, double vb, long long kb)
@@ -38,7 +38,7 @@ __device__ __forceinline__ bool better(double va, long long ka, double vb, long 
 }
 
 // Legality of an overlap of olw x olh (the LegalRgn descriptor).
-__device__ __forceinline__ bool rgn_ok(const RuleDev &g, int olw, int olh)
+__host__ __device__ __forceinline__ bool rgn_ok(const RuleDev &g, int olw, int olh)
 {
     if (!g.rgn_enabled) return true;
     if (olw < g.min_w || olh < g.min_h) return false;
@@ -69,7 +69,7 @@ struct RowBox {
     int a_b, a_t, b_b, b_t, olh;
 };
 
-__device__ __forceinline__ RowBox row_box(const PairDev &P, int y)
+__host__ __device__ __forceinline__ RowBox row_box(const PairDev &P, int y)
 {
     RowBox rb;
     int B = imax(0, y), T = imin(P.h2 - 1, y + P.h1 - 1);
@@ -147,7 +147,7 @@ __device__ __forceinline__ int lag_row_of_raster(const PairDev &P, int iy)
 // Signed lag y of stored lag row yl.
 constexpr int DEAD_ROW = -0x40000000;
 
-__device__ __forceinline__ int lag_y_of_row(const PairDev &P, int yl)
+__host__ __device__ __forceinline__ int lag_y_of_row(const PairDev &P, int yl)
 {
     if (P.lag_order == LAG_ALL) {
         if (yl < P.nposy) return yl;
@@ -156,6 +156,13 @@ __device__ __forceinline__ int lag_y_of_row(const PairDev &P, int yl)
     }
     if (P.lag_order == LAG_RASTER) return yl < P.nposy ? yl : yl - P.nposy - P.nnegy;
     return yl - P.nnegy;
+}
+
+// Stored lag row yl is computed (its signed lag lies in the pair's live window).
+__host__ __device__ __forceinline__ bool row_live(const PairDev &P, int yl)
+{
+    const int y = lag_y_of_row(P, yl);
+    return y >= P.ylo && y <= P.yhi;
 }
 
 // Signed lag x of raster column ix, with *dead for the dead zone.
@@ -224,13 +231,32 @@ __device__ __forceinline__ bool present(const ImgView &v, int x, int y)
     return !v.mask || v.mask[(long long)y * v.mstride + x] != 0;
 }
 
-// Value of pixel (x, y) in transform precision: (raw - mean) / sd rounded to
-// T, 0 outside the image or where the mask says absent.
+// Value of pixel (x, y) in transform precision: (raw - mean) * (1/sd)
+// rounded to T, 0 outside the image or where the mask says absent.
 template <typename T>
-__device__ __forceinline__ T load_val(const ImgView &v, double mean, double sd, int x, int y)
+__device__ __forceinline__ T load_val(const ImgView &v, double mean, double inv, int x, int y)
 {
     if (x >= v.w || y >= v.h || !present(v, x, y)) return (T)0;
-    return (T)((load_raw(v, x, y) - mean) / sd);
+    return (T)((load_raw(v, x, y) - mean) * inv);
+}
+
+// Row y of a typed source as E complex pairs (x = 2p, 2p + 1 with
+// p = t + TPF m), the values load_val gives.
+template <typename T, typename InT, int E, int TPF>
+__device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, double mean, double inv, int t, int y,
+                                               bool valid)
+{
+    const InT *row = reinterpret_cast<const InT *>(V.data) + (long long)y * V.stride;
+    const uint8_t *mrow = V.mask ? V.mask + (long long)y * V.mstride : nullptr;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const int x = 2 * (t + TPF * m);
+        T a = (T)0, b = (T)0;
+        if (valid && x < V.w && (!mrow || mrow[x])) a = (T)(((double)row[x] - mean) * inv);
+        if (valid && x + 1 < V.w && (!mrow || mrow[x + 1])) b = (T)(((double)row[x + 1] - mean) * inv);
+        v[m].x = a;
+        v[m].y = b;
+    }
 }
 
 // ------------------------------------------------------------------
@@ -303,6 +329,7 @@ static __global__ void k_tp_stats_fin(const TileDev *__restrict__ tiles, int nti
     }
     D.msd[0] = mean;
     D.msd[1] = sd;
+    D.msd[2] = 1.0 / sd;
 }
 
 // Wavefront inclusive scans with DPP row shifts and row broadcasts (GFX9
@@ -360,7 +387,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *
     const int w = D.v.w, h = D.v.h;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x0 = tid * CPT;
-    const double mean = D.v.msd[0], sd = D.v.msd[1];
+    const double mean = D.v.msd[0], inv = D.v.msd[2];
     double cs[CPT], cq[CPT];
     int cz[CPT];
     double nxt[CPT];
@@ -379,8 +406,8 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *
         for (int j = 0; j < CPT; ++j) {
             const int x = x0 + j;
             const bool pres = x < w && (!MASK || D.v.mask[(long long)y * D.v.mstride + x]);
-            // the value the transforms see: (raw - mean) / sd rounded to T; absent pixels are 0
-            const double t = pres ? (double)(T)((nxt[j] - mean) / sd) : 0.0;
+            // the value the transforms see: (raw - mean) * (1/sd) rounded to T; absent pixels are 0
+            const double t = pres ? (double)(T)((nxt[j] - mean) * inv) : 0.0;
             nxt[j] = (x < w && y + 1 < h) ? sweep_val<InT>(D.v, x, y + 1) : 0.0;
             cs[j] += t;
             cq[j] += t * t;
@@ -566,14 +593,14 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     cpx<T> *dst = reinterpret_cast<cpx<T> *>(img ? P.specB : P.specA);
     const bool valid = y < V.h;
     const int H = NH + 1;
-    const double mean = V.msd[0], sd = V.msd[1];
+    const double mean = V.msd[0], inv = V.msd[2];
 
     cpx<T> v[E];
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const int p = t + TPF * m;
-        v[m].x = valid ? load_val<T>(V, mean, sd, 2 * p, y) : (T)0;
-        v[m].y = valid ? load_val<T>(V, mean, sd, 2 * p + 1, y) : (T)0;
+    switch (V.dtype) {  // uniform per workgroup
+    case XC_U8: load_row_pairs<T, uint8_t, E, TPF>(v, V, mean, inv, t, y, valid); break;
+    case XC_U16: load_row_pairs<T, uint16_t, E, TPF>(v, V, mean, inv, t, y, valid); break;
+    case XC_F32: load_row_pairs<T, float, E, TPF>(v, V, mean, inv, t, y, valid); break;
+    default: load_row_pairs<T, double, E, TPF>(v, V, mean, inv, t, y, valid); break;
     }
     LdsView<T> L{smem + g * lds_len(NH), 1};
     fft<-1, NH>(v, t, L, tw);
@@ -681,7 +708,7 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
         for (int m = 0; m < E; ++m) {
             const int iy = t + TPF * m;
             const int yl = lag_row_of_raster(P, iy);
-            if (yl >= 0) LG[(long long)yl * H + k] = a[m];
+            if (yl >= 0 && row_live(P, yl)) LG[(long long)yl * H + k] = a[m];
         }
     }
 }
@@ -743,6 +770,127 @@ __device__ __forceinline__ double lag_value(const PairDev &P, const RowBox &rb, 
     return v;
 }
 
+// ---- CorrPatches rows (NCC mode): the legal lags of one row form an x
+// interval, so a row is evaluated over that interval only.
+//
+// With olw(x) = min(w1, w2, w2 - x, w1 + x), every scalar LegalRgn term is
+// "olw >= m" for a per-row m (min_w; olw > min_area / olh; the or-term when
+// olh <= or_h_gt), and the disc is |x - Ox| <= s with s = floor(sqrt(r^2 -
+// dy^2)).  exact_w, a tabulated rule and the count predicate stay per lag.
+struct NccRow {
+    int olh;
+    int x0, x1;    // legal-interval bounds (empty when x0 > x1)
+    double invN;   // 1 / (Nx Ny), a power of two: raw * invN == raw / Nxy exactly
+    const double *sAt, *sAb, *qAt, *qAb, *sBt, *sBb, *qBt, *qBb;  // table rows: band top, row below band (or null)
+    const int32_t *zAt, *zAb, *zBt, *zBb;
+};
+
+__host__ __device__ __forceinline__ long long isqrt_floor(long long v)
+{
+    long long s = (long long)sqrt((double)v);
+    while (s > 0 && s * s > v) --s;
+    while ((s + 1) * (s + 1) <= v) ++s;
+    return s;
+}
+
+__host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
+{
+    NccRow q;
+    const RowBox rb = row_box(P, y);
+    const RuleDev &g = P.rule;
+    q.olh = rb.olh;
+    q.invN = 1.0 / ((double)P.Nx * (double)P.Ny);
+    long long lo = -P.nnegx, hi = P.nposx - 1;
+    const long long dy = (long long)y - P.Oy, r2 = (long long)P.radius * P.radius - dy * dy;
+    bool ok = r2 >= 0;
+    if (ok) {
+        const long long s = isqrt_floor(r2);
+        lo = lo > P.Ox - s ? lo : P.Ox - s;
+        hi = hi < P.Ox + s ? hi : P.Ox + s;
+    }
+    long long m = 1;
+    if (g.rgn_enabled) {
+        ok = ok && rb.olh >= g.min_h && (g.exact_h <= 0 || rb.olh == g.exact_h);
+        if (g.min_w > m) m = g.min_w;
+        if (g.min_area >= 0 && g.min_area / rb.olh + 1 > m) m = g.min_area / rb.olh + 1;
+        if ((g.or_w_ge > 0 || g.or_h_gt > 0) && !(rb.olh > g.or_h_gt) && g.or_w_ge > m) m = g.or_w_ge;
+    }
+    // olw(x) >= m  <=>  min(w1, w2) >= m  &&  m - w1 <= x <= w2 - m
+    ok = ok && m <= (P.w1 < P.w2 ? P.w1 : P.w2);
+    if (m - P.w1 > lo) lo = m - P.w1;
+    if (P.w2 - m < hi) hi = P.w2 - m;
+    ok = ok && lo <= hi;
+    q.x0 = ok ? (int)lo : 1;
+    q.x1 = ok ? (int)hi : 0;
+    const long long at = (long long)rb.a_t * P.w1, ab = (long long)(rb.a_b - 1) * P.w1;
+    const long long bt = (long long)rb.b_t * P.w2, bb = (long long)(rb.b_b - 1) * P.w2;
+    q.sAt = P.SA + at;
+    q.qAt = P.QA + at;
+    q.sBt = P.SB + bt;
+    q.qBt = P.QB + bt;
+    q.sAb = rb.a_b > 0 ? P.SA + ab : nullptr;
+    q.qAb = rb.a_b > 0 ? P.QA + ab : nullptr;
+    q.sBb = rb.b_b > 0 ? P.SB + bb : nullptr;
+    q.qBb = rb.b_b > 0 ? P.QB + bb : nullptr;
+    q.zAt = P.ZA ? P.ZA + at : nullptr;
+    q.zBt = P.ZB ? P.ZB + bt : nullptr;
+    q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
+    q.zBb = (P.ZB && rb.b_b > 0) ? P.ZB + bb : nullptr;
+    return q;
+}
+
+// IntegralTable (Correlation.cpp:160-201) with the box's two table rows
+// given: t[R,T] - t[L-1,T] - t[R,B-1] + t[L-1,B-1], in that order.
+template <typename TT>
+__device__ __forceinline__ TT band_sum(const TT *__restrict__ top, const TT *__restrict__ bot, int L, int R)
+{
+    TT v = top[R];
+    if (L > 0) v -= top[L - 1];
+    if (bot) {
+        v -= bot[R];
+        if (L > 0) v += bot[L - 1];
+    }
+    return v;
+}
+
+// NCC value of lag x of a row (CLinCorr::CheckDensity/GetCorr, Correlation.cpp:697-754);
+// the lag is inside the row's legal interval.  *cand: passes exact_w, the table and
+// LegalCnt; *odd: |r| > 1.0001.
+#pragma clang fp contract(off)
+__device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int x, double raw, bool *cand, bool *odd)
+{
+    const int BL = x > 0 ? x : 0;
+    const int BR = (P.w2 - 1 < x + P.w1 - 1) ? P.w2 - 1 : x + P.w1 - 1;
+    const int AL = BL - x, AR = BR - x;
+    const int olw = BR - BL + 1;
+    const RuleDev &g = P.rule;
+    *odd = false;
+    bool c = true;
+    if (g.exact_w > 0 && olw != g.exact_w) c = false;
+    if (g.table && (olw > g.table_w || q.olh > g.table_h || !g.table[(long long)(q.olh - 1) * g.table_w + (olw - 1)]))
+        c = false;
+    if (g.cnt_enabled) {
+        const int c1 = band_sum(q.zAt, q.zAb, AL, AR);
+        const int c2 = band_sum(q.zBt, q.zBb, BL, BR);
+        if (!((long long)c1 > g.k1 && (long long)c2 > g.k2)) c = false;
+    }
+    *cand = c;
+    if (!c) return 0.0;
+    const double n = (double)(olw * q.olh);
+    const double s1 = band_sum(q.sAt, q.sAb, AL, AR);
+    const double q1 = band_sum(q.qAt, q.qAb, AL, AR);
+    const double s2 = band_sum(q.sBt, q.sBb, BL, BR);
+    const double q2 = band_sum(q.qBt, q.qBb, BL, BR);
+    const double num = n * raw * q.invN - s1 * s2;
+    const double d1 = n * q1 - s1 * s1;
+    const double d2 = n * q2 - s2 * s2;
+    const double d = d1 * d2;
+    const double r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
+    *odd = fabs(r) > 1.0001;
+    return r;
+}
+#pragma clang fp contract(on)
+
 template <typename T, int NH>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     k_row_c2r_eval(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw)
@@ -764,7 +912,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     const int2 wi = work[xcd_remap(blockIdx.x, gridDim.x)];  // {pair, yl0}
     const PairDev &P = pairs[wi.x];
     const int yl = wi.y + g;
-    const bool valid = yl < P.hR;
+    const bool valid = yl < P.hR && row_live(P, yl);
     const cpx<T> *LG = reinterpret_cast<const cpx<T> *>(P.lag) + (long long)yl * H;
     cpx<T> *X = smem + g * RS;
 
@@ -801,7 +949,26 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     long long bestk = 0x7fffffffffffffffLL;
     double ssum = 0.0, ssq = 0.0;
     int flags = 0;
-    if (valid) {
+    if (valid && P.mode == PM_NCC) {
+        // raster order: x >= 0 first (ix = x), then x < 0 (ix = Nx + x); each thread's
+        // keys ascend, so a strict > keeps the first of equal values
+        const NccRow q = ncc_row(P, y);
+        const int segs[2][2] = {{q.x0 > 0 ? q.x0 : 0, q.x1}, {q.x0, q.x1 < -1 ? q.x1 : -1}};
+        for (int sgi = 0; sgi < 2; ++sgi) {
+            for (int x = segs[sgi][0] + t; x <= segs[sgi][1]; x += TPF) {
+                const int ix = x >= 0 ? x : NX + x;
+                bool cand, odd;
+                const double val = ncc_lag(P, q, x, (double)R[ix], &cand, &odd);
+                if (cand) {
+                    flags |= odd ? 3 : 1;
+                    if (val > bestv) {
+                        bestv = val;
+                        bestk = ix;
+                    }
+                }
+            }
+        }
+    } else if (valid) {
         for (int ix = t; ix < NX; ix += TPF) {
             bool cand, win, odd;
             const double val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
@@ -943,6 +1110,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
     long long bk = 0x7fffffffffffffffLL;
     int fl = 0;
     for (int yl = tid; yl < P.hR; yl += 256) {
+        if (!row_live(P, yl)) continue;  // no legal lag: all 0, no candidate
         const RowRec r = P.rows[yl];
         fl |= r.flags;
         s1 += r.sum;
@@ -1042,8 +1210,8 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
                 for (int s = 0; s < 2; ++s) {
                     const int iy = y + (s ? 1 : -1);  // raster row (non-negative here)
                     const int ylr = lag_row_of_raster(P, iy);
-                    double val = 0.0;
-                    if (ylr >= 0) {
+                    double val = 0.0;  // rows outside the live window hold 0
+                    if (ylr >= 0 && row_live(P, ylr)) {
                         const int yy = lag_y_of_row(P, ylr);
                         const double raw = point_c2r<T>(P, ylr, x, tw, sv);
                         const int dxd = x - P.Ox, dyd = yy - P.Oy;

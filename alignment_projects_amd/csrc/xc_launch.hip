@@ -1,6 +1,9 @@
 // xc_launch.hip -- context management, workspace and kernel dispatch of
 // libxcorr (everything the C-ABI entry points share).
 #include <math.h>
+
+#include <algorithm>
+#include <vector>
 #include <stdarg.h>
 #include <stdio.h>
 #include <string.h>
@@ -129,7 +132,72 @@ int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode)
     P.nposy = h2;
     P.wR = P.nnegx + P.nposx;
     P.hR = P.nnegy + P.nposy;
+    P.ylo = -0x7fffffff;  // every row (narrow_rows may shrink this)
+    P.yhi = 0x7fffffff;
     return XC_OK;
+}
+
+// The scalar part of the LegalRgn descriptor (the device's rgn_ok without a table).
+static bool rgn_scalar_ok(const RuleDev &g, int olw, int olh)
+{
+    if (olw < g.min_w || olh < g.min_h) return false;
+    if (!((long long)olw * olh > g.min_area)) return false;
+    if (g.exact_w > 0 && olw != g.exact_w) return false;
+    if (g.exact_h > 0 && olh != g.exact_h) return false;
+    if ((g.or_w_ge > 0 || g.or_h_gt > 0) && !(olw >= g.or_w_ge || olh > g.or_h_gt)) return false;
+    return true;
+}
+
+void narrow_rows(PairDev &P)
+{
+    if (P.mode != PM_NCC) return;  // R/S images and the P2I surface need every row
+    const RuleDev &g = P.rule;
+    const int maxw = std::min(P.w1, P.w2);  // widest overlap any x lag reaches
+    int lo = 0x7fffffff, hi = -0x7fffffff;
+    for (int y = -P.nnegy; y < P.nposy; ++y) {
+        const long long dy = (long long)y - P.Oy;
+        if (dy * dy > (long long)P.radius * P.radius) continue;  // the disc excludes the whole row
+        const int B = std::max(0, y);
+        const int T = std::max(B, std::min(P.h2 - 1, y + P.h1 - 1));
+        const int olh = T - B + 1;
+        // every rule term is non-decreasing in olw, so the widest overlap decides (or the one exact width)
+        int olw = maxw;
+        if (g.rgn_enabled && !g.table) {
+            if (g.exact_w > 0) {
+                if (g.exact_w > maxw) continue;
+                olw = g.exact_w;
+            }
+            if (!rgn_scalar_ok(g, olw, olh)) continue;
+        }
+        // present-pixel counts cannot exceed the overlap area
+        if (g.cnt_enabled && !((long long)olw * olh > g.k1 && (long long)olw * olh > g.k2)) continue;
+        lo = std::min(lo, y);
+        hi = std::max(hi, y);
+    }
+    if (lo > hi) {
+        P.ylo = 1;  // no live row: no legal lag anywhere
+        P.yhi = 0;
+    } else {
+        P.ylo = lo;
+        P.yhi = hi;
+    }
+}
+
+void live_row_groups(const PairDev &P, int G, std::vector<int> &starts)
+{
+    starts.clear();
+    for (int y0 = 0; y0 < P.hR; y0 += G) {
+        bool any = false;
+        for (int y = y0; y < y0 + G && y < P.hR && !any; ++y) any = row_live(P, y);
+        if (any) starts.push_back(y0);
+    }
+}
+
+int live_rows(const PairDev &P)
+{
+    int n = 0;
+    for (int y = 0; y < P.hR; ++y) n += row_live(P, y);
+    return n;
 }
 
 template <> const cpx<double> *twiddles<double>(xc_ctx *ctx) { return ctx->tw64; }

@@ -125,6 +125,9 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     P.out = at(i_res);
     P.rule = make_rule(rgn, cnt, (const uint8_t *)at(i_tbl));
     P.need_cnt = P.rule.cnt_enabled;
+    narrow_rows(P);
+    std::vector<int> c2r_starts;
+    live_row_groups(P, G, c2r_starts);
 
     // ---- host staging: points + values, then descriptors ----
     const size_t s1_lo = lay.off[i_pts], s1_hi = lay.off[i_vals] + sizeof(double) * n;
@@ -149,7 +152,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     int2 *wc = (int2 *)h2p(i_wc);
     for (int i = 0; i < ncol; ++i) wc[i] = make_int2(0, i * C);
     int2 *wc2 = (int2 *)h2p(i_wc2);
-    for (int i = 0; i < nc2r; ++i) wc2[i] = make_int2(0, i * G);
+    for (size_t i = 0; i < c2r_starts.size(); ++i) wc2[i] = make_int2(0, c2r_starts[i]);
     if (tbl) memcpy(h2p(i_tbl), rgn->table, tbl);
     std::vector<SatGroup> sgroups;
     std::vector<int> sflat, soff;
@@ -193,7 +196,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     b.d_col = (const int2 *)at(i_wc);
     b.n_col = ncol;
     b.d_c2r = (const int2 *)at(i_wc2);
-    b.n_c2r = nc2r;
+    b.n_c2r = (int)c2r_starts.size();
     ex.buckets.push_back(b);
     if ((st = enqueue(ctx, ex))) return st;
     XC_HIP(ctx, hipMemcpyAsync(hres, at(i_res), sizeof(xc_result), hipMemcpyDeviceToHost, s));

@@ -273,7 +273,7 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n1), i_vals = lay.add(sizeof(double) * n1);
     size_t i_i2 = lay.add(sizeof(double) * n_i2), i_i1 = lay.add(sizeof(double) * n_i2);
-    size_t i_own = lay.add(sizeof(int) * n_i2), i_msd = lay.add(sizeof(double) * 2);
+    size_t i_own = lay.add(sizeof(int) * n_i2), i_msd = lay.add(sizeof(double) * 3);
     size_t i_spA = lay.add(sizeof(cpx<double>) * N * H), i_spB = lay.add(sizeof(cpx<double>) * N * H);
     size_t i_lag = lay.add(sizeof(cpx<double>) * N * H), i_rows = lay.add(sizeof(RowRec) * N);
     size_t i_res = lay.add(sizeof(xc_result)), i_pair = lay.add(sizeof(PairDev));
@@ -315,7 +315,7 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     xc_result *hres = (xc_result *)(((uintptr_t)(hs + (up_hi - up_lo)) + 63) & ~(uintptr_t)63);
 
     hipStream_t s = ctx->stream;
-    const double msd[2] = {0.0, 1.0};
+    const double msd[3] = {0.0, 1.0, 1.0};
     XC_HIP(ctx, hipMemcpyAsync(base + up_lo, hs, up_hi - up_lo, hipMemcpyHostToDevice, s));
     XC_HIP(ctx, hipMemcpyAsync(at(i_msd), msd, sizeof msd, hipMemcpyHostToDevice, s));
     XC_HIP(ctx, hipMemcpyAsync(at(i_pts), ip1, sizeof(xc_point) * n1, hipMemcpyHostToDevice, s));

@@ -147,6 +147,7 @@ int make_pair_dev(xc_server *S, const xc_pair &q, PairDev &D)
     D.Ry = q.Ry;
     D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
     D.need_cnt = D.rule.cnt_enabled;
+    narrow_rows(D);
     D.A = q.a->dev.v;
     D.B = q.b->dev.v;
     D.SA = q.a->dev.S;
@@ -257,6 +258,7 @@ int launch_batch(xc_server *S, int slot_i)
     Layout lay;
     size_t i_pairs = lay.add(sizeof(PairDev) * n), i_res = lay.add(sizeof(xc_result) * n);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
+    std::vector<int> starts;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &b = buckets[bi];
         const int G = row_group<float>(b.NH), C = col_group<float>(b.NY);
@@ -267,7 +269,8 @@ int launch_batch(xc_server *S, int slot_i)
             if (ownA[i]) nr += (D.h1 + G - 1) / G;
             if (ownB[i]) nr += (D.h2 + G - 1) / G;
             nc += (D.Nx / 2 + 1 + C - 1) / C;
-            nc2 += (D.hR + G - 1) / G;
+            live_row_groups(D, G, starts);
+            nc2 += (int)starts.size();
         }
         b.n_r2c = nr;
         b.n_col = nc;
@@ -323,7 +326,8 @@ int launch_batch(xc_server *S, int slot_i)
             if (ownB[i])
                 for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
-            for (int y = 0; y < D.hR; y += G) wc2[d++] = make_int2(k, y);
+            live_row_groups(D, G, starts);
+            for (int y : starts) wc2[d++] = make_int2(k, y);
         }
         b.d_r2c = (const int4 *)(dbase + lay.off[i_wr[bi]]);
         b.d_col = (const int2 *)(dbase + lay.off[i_wc[bi]]);

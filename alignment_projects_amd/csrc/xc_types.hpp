@@ -38,14 +38,15 @@ struct RowRec {
 };
 
 // A source image as the kernels read it: raw pixels of any supported type,
-// an optional presence mask, and a device (mean, sd) pair applied as
-// (v - mean) / sd before rounding to the transform precision (Normalize,
-// Maths.cpp:192-212; (0, 1) for raw values).  Absent pixels read as 0,
+// an optional presence mask, and a device (mean, sd, 1/sd) triple applied
+// as (v - mean) * (1/sd) before rounding to the transform precision
+// (Normalize, Maths.cpp:192-212; (0, 1, 1) for raw values, which leaves
+// them unchanged).  Absent pixels read as 0,
 // exactly as ImageFromValuesAndPoints leaves them.
 struct ImgView {
     const void *data;
     const uint8_t *mask;
-    const double *msd;   // device (mean, sd)
+    const double *msd;   // device (mean, sd, 1/sd)
     int32_t dtype;       // xc_dtype
     int32_t w, h;
     int32_t stride;      // elements per row of data
@@ -56,7 +57,7 @@ struct ImgView {
 // Per-image preparation record (statistics and summed-area tables).
 struct TileDev {
     ImgView v;
-    double *msd;         // out: (mean, sd)
+    double *msd;         // out: (mean, sd, 1/sd)
     double *S, *Q;       // out: FP64 summed-area tables, w x h
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
@@ -94,6 +95,7 @@ struct PairDev {
     int32_t nnegx, nposx, nnegy, nposy, wR, hR;
     int32_t need_cnt;        // count predicate active (nz tables needed)
     int32_t filter;          // PM_P2I: Gaussian low-pass of the cross power (bFilter)
+    int32_t ylo, yhi;        // signed lag rows computed; rows outside hold no legal lag (all 0, no candidate)
     double norm;             // PM_P2I: np1 * N^2
     double mincor;
     RuleDev rule;
