@@ -114,6 +114,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.Q = (double *)(base + lay.off[iQ]);
     D.Z = (int32_t *)(base + lay.off[iZ]);
     D.part = (double *)(base + lay.off[iP]);
+    D.nbands = 1;  // swept unbanded, as a plan's batched refresh sweeps it
     D.normalize = (flags & XC_TILE_NORMALIZE) ? 1 : 0;
     t->d_dev = (TileDev *)(base + lay.off[iD]);
     const int zero = 0;
@@ -454,6 +455,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         T.S = rot_sq[r] ? (double *)at(i_rS[r]) : nullptr;
         T.Q = rot_sq[r] ? (double *)at(i_rQ[r]) : nullptr;
         T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
+        T.nbands = 1;
         hd[r] = T;
         rots[r] = R;
     }

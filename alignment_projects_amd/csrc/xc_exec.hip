@@ -37,6 +37,7 @@ TileSizes tile_sizes(int w, int h)
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
     t.msd = sizeof(double) * 3;
+    t.band = (sizeof(double) * 2 + sizeof(int)) * SAT_NB * (size_t)w;
     return t;
 }
 
@@ -118,7 +119,9 @@ template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles
             switch (key) {
 #define XC_SAT(c, n)                                                                                                \
     case c * 10000 + n:                                                                                             \
-        hipLaunchKernelGGL((k_tp_sat<T, c, n>), dim3(g.count), dim3(n), 0, s, d_tiles, g.list);                 \
+        if (g.nb > 1)                                                                                               \
+            hipLaunchKernelGGL((k_tp_colsum<T, c, n>), dim3(g.count * g.nb), dim3(n), 0, s, d_tiles, g.list, g.nb); \
+        hipLaunchKernelGGL((k_tp_sat<T, c, n>), dim3(g.count * g.nb), dim3(n), 0, s, d_tiles, g.list, g.nb);        \
         break;
                 XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 192) XC_SAT(2, 256) XC_SAT(2, 320) XC_SAT(2, 384)
                 XC_SAT(2, 448) XC_SAT(2, 512) XC_SAT(2, 576) XC_SAT(2, 640) XC_SAT(2, 704) XC_SAT(2, 768)

@@ -31,6 +31,7 @@ struct SatGroup {
     int cpt = 0;
     int threads = 0;
     int count = 0;
+    int nb = 1;  // row bands per image (the largest of the group's images)
     const int *list = nullptr;  // device: tile indices
 };
 
@@ -62,7 +63,7 @@ int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
 
 // Bytes of device scratch/table memory one prepared image needs.
 struct TileSizes {
-    size_t S, Q, Z, part, msd;
+    size_t S, Q, Z, part, msd, band;
 };
 TileSizes tile_sizes(int w, int h);
 

@@ -266,4 +266,16 @@ __device__ __forceinline__ void fft(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> l
     fft_pass<DIR, N, T, 0, TwLazy<T>>(v, t, lds, TW);
 }
 
+// Forward or inverse transform of data whose registers m >= E/2 are zero
+// (the upper half of the input is padding).  The zeros are constants here,
+// so the first pass's butterflies lose their dead terms.
+template <int DIR, int N, typename T, typename TWA>
+__device__ __forceinline__ void fft_upper_zero(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> lds, const TWA &tw)
+{
+    constexpr int E = Plan<N>::E;
+#pragma unroll
+    for (int m = E / 2; m < E; ++m) v[m] = cpx<T>{(T)0, (T)0};
+    fft<DIR, N>(v, t, lds, tw);
+}
+
 }  // namespace xc

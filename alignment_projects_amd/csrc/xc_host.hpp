@@ -101,6 +101,10 @@ void narrow_rows(PairDev &P);
 void live_row_groups(const PairDev &P, int G, std::vector<int> &starts);
 int live_rows(const PairDev &P);
 
+// Power-of-two transform lengths the launchers dispatch on.
+#define XC_POW2_CASES(F) \
+    F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048) F(4096)
+
 // Kernel launchers (dispatch on transform length).
 template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work);
 template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nblocks, const PairDev *pd, const int2 *work);

@@ -240,16 +240,16 @@ __device__ __forceinline__ T load_val(const ImgView &v, double mean, double inv,
     return (T)((load_raw(v, x, y) - mean) * inv);
 }
 
-// Row y of a typed source as E complex pairs (x = 2p, 2p + 1 with
-// p = t + TPF m), the values load_val gives.
-template <typename T, typename InT, int E, int TPF>
+// Row y of a typed source as complex pairs (x = 2p, 2p + 1 with
+// p = t + TPF m) for registers m < ME, the values load_val gives.
+template <typename T, typename InT, int ME, int TPF>
 __device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, double mean, double inv, int t, int y,
                                                bool valid)
 {
     const InT *row = reinterpret_cast<const InT *>(V.data) + (long long)y * V.stride;
     const uint8_t *mrow = V.mask ? V.mask + (long long)y * V.mstride : nullptr;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
+    for (int m = 0; m < ME; ++m) {
         const int x = 2 * (t + TPF * m);
         T a = (T)0, b = (T)0;
         if (valid && x < V.w && (!mrow || mrow[x])) a = (T)(((double)row[x] - mean) * inv);
@@ -267,6 +267,33 @@ __device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, doub
 // row totals.
 // ------------------------------------------------------------------
 
+template <typename InT>
+__device__ __forceinline__ void stats_rows(const ImgView &V, int y0, int y1, int w, int tid, double &a, double &b,
+                                           double &c)
+{
+    for (int y = y0; y < y1; ++y) {
+        const InT *row = reinterpret_cast<const InT *>(V.data) + (long long)y * V.stride;
+        const uint8_t *mrow = V.mask ? V.mask + (long long)y * V.mstride : nullptr;
+        for (int xb = 0; xb < w; xb += 256 * 8) {
+            double v[8];
+            bool p[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {  // eight loads in flight per thread
+                const int x = xb + tid + 256 * k;
+                p[k] = x < w && (!mrow || mrow[x]);
+                v[k] = x < w ? (double)row[x] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (p[k]) {
+                    a += v[k];
+                    b += v[k] * v[k];
+                    c += 1.0;
+                }
+        }
+    }
+}
+
 // Partial sums of present raw values: TP_PARTS workgroups per image, each
 // over a band of rows; threads stride along x (coalesced).
 static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restrict__ tiles)
@@ -277,15 +304,11 @@ static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restri
     const int h = D.v.h, w = D.v.w;
     const int y0 = (int)((long long)h * part / TP_PARTS), y1 = (int)((long long)h * (part + 1) / TP_PARTS);
     double a = 0.0, b = 0.0, c = 0.0;
-    for (int y = y0; y < y1; ++y) {
-        for (int x = tid; x < w; x += 256) {
-            if (present(D.v, x, y)) {
-                const double t = load_raw(D.v, x, y);
-                a += t;
-                b += t * t;
-                c += 1.0;
-            }
-        }
+    switch (D.v.dtype) {  // uniform per workgroup
+    case XC_U8: stats_rows<uint8_t>(D.v, y0, y1, w, tid, a, b, c); break;
+    case XC_U16: stats_rows<uint16_t>(D.v, y0, y1, w, tid, a, b, c); break;
+    case XC_F32: stats_rows<float>(D.v, y0, y1, w, tid, a, b, c); break;
+    default: stats_rows<double>(D.v, y0, y1, w, tid, a, b, c); break;
     }
     ss[tid] = a;
     sq[tid] = b;
@@ -380,35 +403,118 @@ __device__ __forceinline__ double sweep_val(const ImgView &v, int x, int y)
     return (double)reinterpret_cast<const InT *>(v.data)[(long long)y * v.stride + x];
 }
 
-template <typename T, int CPT, int NT, typename InT, bool MASK>
-__device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *xq, int *xz)
+// Row bands of the table sweep: a lone image of h rows (the plain path's
+// two) is swept as up to SAT_NB bands of >= 32 rows, each by its own
+// workgroup; a band starts from the column sums of the bands above it
+// (k_tp_colsum).  Batches of images fill the device one workgroup per image
+// and sweep unbanded (nbands = 1), which moves fewer bytes.
+constexpr int SAT_NB = 8;
+__host__ __device__ inline int sat_bands(int h)
 {
-    constexpr int NWV = NT / 64;
-    const int w = D.v.w, h = D.v.h;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int x0 = tid * CPT;
+    const int nb = (h + 31) / 32;
+    return nb < 1 ? 1 : (nb > SAT_NB ? SAT_NB : nb);
+}
+__host__ __device__ inline int tile_bands(const TileDev &D) { return D.nbands > 1 ? D.nbands : 1; }
+__host__ __device__ inline int tile_band_rows(const TileDev &D) { return (D.v.h + tile_bands(D) - 1) / tile_bands(D); }
+
+// Column-sum scratch of a tile: [SAT_NB][w] S, [SAT_NB][w] Q, then [SAT_NB][w] int Z.
+__device__ __forceinline__ double *bandp_s(const TileDev &D, int b) { return D.bandp + (long long)b * D.v.w; }
+__device__ __forceinline__ double *bandp_q(const TileDev &D, int b) { return D.bandp + (long long)(SAT_NB + b) * D.v.w; }
+__device__ __forceinline__ int *bandp_z(const TileDev &D, int b)
+{
+    return reinterpret_cast<int *>(D.bandp + 2LL * SAT_NB * D.v.w) + (long long)b * D.v.w;
+}
+
+// The value the transforms see: (raw - mean) * (1/sd) rounded to T; absent pixels are 0.
+template <typename T, typename InT, bool MASK>
+__device__ __forceinline__ double sat_val(const TileDev &D, double raw, int x, int y, double mean, double inv)
+{
+    const bool pres = x < D.v.w && (!MASK || D.v.mask[(long long)y * D.v.mstride + x]);
+    return pres ? (double)(T)((raw - mean) * inv) : 0.0;
+}
+
+// Column sums of rows [y0, y1) over the thread's CPT columns.
+template <typename T, int CPT, typename InT, bool MASK>
+__device__ __forceinline__ void band_colsum(const TileDev &D, int band, int y0, int y1)
+{
+    const int w = D.v.w, x0 = threadIdx.x * CPT;
     const double mean = D.v.msd[0], inv = D.v.msd[2];
     double cs[CPT], cq[CPT];
     int cz[CPT];
-    double nxt[CPT];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         cs[j] = 0.0;
         cq[j] = 0.0;
         cz[j] = 0;
-        const int x = x0 + j;
-        nxt[j] = (x < w && h > 0) ? sweep_val<InT>(D.v, x, 0) : 0.0;
     }
-    for (int y = 0; y < h; ++y) {
-        double ps[CPT], pq[CPT];
-        int pz[CPT];
+#pragma unroll 4
+    for (int y = y0; y < y1; ++y) {
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
             const int x = x0 + j;
-            const bool pres = x < w && (!MASK || D.v.mask[(long long)y * D.v.mstride + x]);
-            // the value the transforms see: (raw - mean) * (1/sd) rounded to T; absent pixels are 0
-            const double t = pres ? (double)(T)((nxt[j] - mean) * inv) : 0.0;
-            nxt[j] = (x < w && y + 1 < h) ? sweep_val<InT>(D.v, x, y + 1) : 0.0;
+            const double t = sat_val<T, InT, MASK>(D, x < w ? sweep_val<InT>(D.v, x, y) : 0.0, x, y, mean, inv);
+            cs[j] += t;
+            cq[j] += t * t;
+            cz[j] += (t != 0.0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        const int x = x0 + j;
+        if (x < w) {
+            bandp_s(D, band)[x] = cs[j];
+            bandp_q(D, band)[x] = cq[j];
+            bandp_z(D, band)[x] = cz[j];
+        }
+    }
+}
+
+template <typename T, int CPT, int NT, typename InT, bool MASK>
+__device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs, double *xq, int *xz)
+{
+    constexpr int NWV = NT / 64;
+    const int w = D.v.w, h = D.v.h, bh = tile_band_rows(D);
+    const int y0 = band * bh, y1 = min(h, y0 + bh);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = tid * CPT;
+    const double mean = D.v.msd[0], inv = D.v.msd[2];
+    double cs[CPT], cq[CPT];
+    int cz[CPT];
+    constexpr int LA = 4;  // rows loaded ahead of the sweep
+    double nxt[LA][CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        // carry: the column sums of the bands above, added in band order
+        cs[j] = 0.0;
+        cq[j] = 0.0;
+        cz[j] = 0;
+        const int x = x0 + j;
+        if (x < w)
+            for (int b = 0; b < band; ++b) {
+                cs[j] += bandp_s(D, b)[x];
+                cq[j] += bandp_q(D, b)[x];
+                cz[j] += bandp_z(D, b)[x];
+            }
+#pragma unroll
+        for (int a = 0; a < LA; ++a) nxt[a][j] = (x < w && y0 + a < y1) ? sweep_val<InT>(D.v, x, y0 + a) : 0.0;
+    }
+    const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);  // 16-byte aligned column pairs
+    for (int y = y0; y < y1; ++y) {
+        double ps[CPT], pq[CPT];
+        int pz[CPT];
+        double cur[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            cur[j] = nxt[0][j];
+#pragma unroll
+            for (int a = 0; a + 1 < LA; ++a) nxt[a][j] = nxt[a + 1][j];
+            const int x = x0 + j;
+            nxt[LA - 1][j] = (x < w && y + LA < y1) ? sweep_val<InT>(D.v, x, y + LA) : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int x = x0 + j;
+            const double t = sat_val<T, InT, MASK>(D, cur[j], x, y, mean, inv);
             cs[j] += t;
             cq[j] += t * t;
             cz[j] += (t != 0.0);
@@ -437,55 +543,86 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, double *xs, double *
                 bz += xz[par + k];
             }
         const long long row = (long long)y * w;
+        if (pair_st) {
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            if (x0 + j < w) {
-                if (D.S) {
-                    D.S[row + x0 + j] = bs + ps[j];
-                    D.Q[row + x0 + j] = bq + pq[j];
+            for (int j = 0; j < CPT; j += 2) {
+                if (x0 + j < w) {  // w even: x0 + j + 1 < w too
+                    if (D.S) {
+                        *reinterpret_cast<double2 *>(D.S + row + x0 + j) = make_double2(bs + ps[j], bs + ps[j + 1]);
+                        *reinterpret_cast<double2 *>(D.Q + row + x0 + j) = make_double2(bq + pq[j], bq + pq[j + 1]);
+                    }
+                    if (D.Z) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(bz + pz[j], bz + pz[j + 1]);
                 }
-                if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                if (x0 + j < w) {
+                    if (D.S) {
+                        D.S[row + x0 + j] = bs + ps[j];
+                        D.Q[row + x0 + j] = bq + pq[j];
+                    }
+                    if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
+                }
             }
         }
     }
 }
 
-// Summed-area tables in one sweep: one workgroup per image walks the rows
-// top to bottom.  Each thread owns CPT consecutive columns and keeps their
-// running column sums in registers; a DPP wavefront scan plus a 4-entry
-// cross-wave exchange (double-buffered by row parity, one barrier per row)
-// turns column sums into the 2-D prefix.  Reads each pixel once, writes
-// S, Q, Z once.
+#define XC_SAT_DISPATCH(CALL)                                                   \
+    switch (D.v.dtype) {                                                        \
+    case XC_U8:                                                                 \
+        if (m) CALL(uint8_t, true); else CALL(uint8_t, false);                  \
+        break;                                                                  \
+    case XC_U16:                                                                \
+        if (m) CALL(uint16_t, true); else CALL(uint16_t, false);                \
+        break;                                                                  \
+    case XC_F32:                                                                \
+        if (m) CALL(float, true); else CALL(float, false);                      \
+        break;                                                                  \
+    default:                                                                    \
+        if (m) CALL(double, true); else CALL(double, false);                    \
+        break;                                                                  \
+    }
+
+// Column sums of every band but the last (the carries of the bands below).
 template <typename T, int CPT, int NT>
-__global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list)
+__global__ void __launch_bounds__(NT) k_tp_colsum(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
+{
+    const TileDev D = tiles[list[blockIdx.x / gnb]];
+    const int band = blockIdx.x % gnb, nb = tile_bands(D), bh = tile_band_rows(D);
+    if (band >= nb - 1) return;
+    const bool m = D.v.mask != nullptr;
+    const int y0 = band * bh, y1 = min(D.v.h, y0 + bh);
+#define CS_CALL(InT, MK) band_colsum<T, CPT, InT, MK>(D, band, y0, y1)
+    XC_SAT_DISPATCH(CS_CALL)
+#undef CS_CALL
+}
+
+// Summed-area tables (IntegrateImage, Correlation.cpp:49-105): one
+// workgroup per row band walks its rows top to bottom.  Each thread owns
+// CPT consecutive columns and keeps their running column sums in registers
+// (starting from the bands above, the next rows loaded ahead); a DPP
+// wavefront scan plus a cross-wave exchange through LDS (double-buffered by
+// row parity, one barrier per row) turns column sums into the 2-D prefix.
+// Writes S, Q, Z once.
+template <typename T, int CPT, int NT>
+__global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
     __shared__ int xz[2 * NT / 64];
-    const TileDev D = tiles[list[blockIdx.x]];  // by value: the table stores cannot alias it
+    const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
+    const int band = blockIdx.x % gnb;
+    if (band >= tile_bands(D)) return;
     const bool m = D.v.mask != nullptr;
-    switch (D.v.dtype) {
-    case XC_U8:
-        if (m) sat_sweep<T, CPT, NT, uint8_t, true>(D, xs, xq, xz);
-        else sat_sweep<T, CPT, NT, uint8_t, false>(D, xs, xq, xz);
-        break;
-    case XC_U16:
-        if (m) sat_sweep<T, CPT, NT, uint16_t, true>(D, xs, xq, xz);
-        else sat_sweep<T, CPT, NT, uint16_t, false>(D, xs, xq, xz);
-        break;
-    case XC_F32:
-        if (m) sat_sweep<T, CPT, NT, float, true>(D, xs, xq, xz);
-        else sat_sweep<T, CPT, NT, float, false>(D, xs, xq, xz);
-        break;
-    default:
-        if (m) sat_sweep<T, CPT, NT, double, true>(D, xs, xq, xz);
-        else sat_sweep<T, CPT, NT, double, false>(D, xs, xq, xz);
-        break;
-    }
+#define SW_CALL(InT, MK) sat_sweep<T, CPT, NT, InT, MK>(D, band, xs, xq, xz)
+    XC_SAT_DISPATCH(SW_CALL)
+#undef SW_CALL
 }
 
-// Columns per thread of the sweep for width w: two columns per thread up
-// to 1024 threads, then wider.  The launch uses max(64, w / cpt) threads
-// rounded up to a wavefront multiple.
+// Columns per thread of the sweep for width w: two up to 1024 threads (a
+// wavefront's row stores are then contiguous 1 KB runs), then wider.  The
+// launch uses max(64, w / cpt) threads rounded up to a wavefront multiple.
 __host__ __device__ inline int sat_cpt(int w) { return w <= 2048 ? 2 : w <= 4096 ? 4 : 8; }
 __host__ __device__ inline int sat_threads(int w)
 {
@@ -596,14 +733,23 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     const double mean = V.msd[0], inv = V.msd[2];
 
     cpx<T> v[E];
-    switch (V.dtype) {  // uniform per workgroup
-    case XC_U8: load_row_pairs<T, uint8_t, E, TPF>(v, V, mean, inv, t, y, valid); break;
-    case XC_U16: load_row_pairs<T, uint16_t, E, TPF>(v, V, mean, inv, t, y, valid); break;
-    case XC_F32: load_row_pairs<T, float, E, TPF>(v, V, mean, inv, t, y, valid); break;
-    default: load_row_pairs<T, double, E, TPF>(v, V, mean, inv, t, y, valid); break;
-    }
     LdsView<T> L{smem + g * lds_len(NH), 1};
-    fft<-1, NH>(v, t, L, tw);
+#define XC_LOAD_ROW(ME)                                                                           \
+    switch (V.dtype) { /* uniform per workgroup */                                                 \
+    case XC_U8: load_row_pairs<T, uint8_t, ME, TPF>(v, V, mean, inv, t, y, valid); break;         \
+    case XC_U16: load_row_pairs<T, uint16_t, ME, TPF>(v, V, mean, inv, t, y, valid); break;       \
+    case XC_F32: load_row_pairs<T, float, ME, TPF>(v, V, mean, inv, t, y, valid); break;          \
+    default: load_row_pairs<T, double, ME, TPF>(v, V, mean, inv, t, y, valid); break;             \
+    }
+    if (E == 8 && V.w <= NH) {
+        // x >= NH (registers m >= E/2) is padding for every row of this workgroup
+        XC_LOAD_ROW(E / 2)
+        fft_upper_zero<-1, NH>(v, t, L, tw);
+    } else {
+        XC_LOAD_ROW(E)
+        fft<-1, NH>(v, t, L, tw);
+    }
+#undef XC_LOAD_ROW
 
     __syncthreads();
 #pragma unroll
@@ -663,19 +809,38 @@ __global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
     tw_load<NY, T>(TW, t, tw);
 
     cpx<T> a[E], b[E];
+    // rows y >= NY/2 (registers m >= E/2) are padding when the image is at most NY/2 high
+    if (E == 8 && P.A.h <= NY / 2) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const int y = t + TPF * m;
-        a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
-    }
-    fft<-1, NY>(a, t, L, TW);
+        for (int m = 0; m < E / 2; ++m) {
+            const int y = t + TPF * m;
+            a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        }
+        fft_upper_zero<-1, NY>(a, t, L, TW);
+    } else {
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-        const int y = t + TPF * m;
-        b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        for (int m = 0; m < E; ++m) {
+            const int y = t + TPF * m;
+            a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        }
+        fft<-1, NY>(a, t, L, TW);
     }
     __syncthreads();
-    fft<-1, NY>(b, t, L, TW);
+    if (E == 8 && P.B.h <= NY / 2) {
+#pragma unroll
+        for (int m = 0; m < E / 2; ++m) {
+            const int y = t + TPF * m;
+            b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        }
+        fft_upper_zero<-1, NY>(b, t, L, TW);
+    } else {
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const int y = t + TPF * m;
+            b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+        }
+        fft<-1, NY>(b, t, L, TW);
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         cpx<T> p;

@@ -203,136 +203,6 @@ int live_rows(const PairDev &P)
 template <> const cpx<double> *twiddles<double>(xc_ctx *ctx) { return ctx->tw64; }
 template <> const cpx<float> *twiddles<float>(xc_ctx *ctx) { return ctx->tw32; }
 
-// ---------------------------------------------------------------------------
-// Dispatch on power-of-two transform lengths
-// ---------------------------------------------------------------------------
-
-#define XC_POW2_CASES(F) \
-    F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048) F(4096)
-
-template <typename T> int row_group(int NH)
-{
-    switch (NH) {
-#define F(n) \
-    case n: return RowCfg<n>::G;
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return 1;
-}
-
-template <typename T> int col_group(int NY)
-{
-    switch (NY) {
-#define F(n) \
-    case n: return ColCfg<T, n>::C;
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return 1;
-}
-
-template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
-{
-    if constexpr (NH > max_nh<T>()) {
-        return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
-    } else {
-        const size_t smem = sizeof(cpx<T>) * RowCfg<NH>::G * lds_len(NH);
-        hipLaunchKernelGGL((k_row_r2c<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
-                           twiddles<T>(ctx));
-        XC_HIP(ctx, hipGetLastError());
-        return XC_OK;
-    }
-}
-
-template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int4 *w)
-{
-    if (nb <= 0) return XC_OK;
-    switch (NH) {
-#define F(n) \
-    case n: return r2c_n<T, n>(ctx, nb, pd, w);
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
-}
-
-template <typename T, int NY> static int col_n(xc_ctx *ctx, bool whiten, int nb, const PairDev *pd, const int2 *w)
-{
-    if constexpr (NY > max_ny<T>()) {
-        return set_err(ctx, XC_E_SHAPE, "column transform %d too long", NY);
-    } else {
-        using CC = ColCfg<T, NY>;
-        const size_t smem = sizeof(cpx<T>) * CC::C * lds_len(NY);
-        if (whiten)
-            hipLaunchKernelGGL((k_col_fused<T, NY, true>), dim3(nb), dim3(CC::THREADS), smem, ctx->stream, pd, w,
-                               twiddles<T>(ctx));
-        else
-            hipLaunchKernelGGL((k_col_fused<T, NY, false>), dim3(nb), dim3(CC::THREADS), smem, ctx->stream, pd, w,
-                               twiddles<T>(ctx));
-        XC_HIP(ctx, hipGetLastError());
-        return XC_OK;
-    }
-}
-
-template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nb, const PairDev *pd, const int2 *w)
-{
-    if (nb <= 0) return XC_OK;
-    switch (NY) {
-#define F(n) \
-    case n: return col_n<T, n>(ctx, whiten, nb, pd, w);
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return set_err(ctx, XC_E_SHAPE, "unsupported column length %d", NY);
-}
-
-template <typename T, int NH> static int c2r_n(xc_ctx *ctx, int nb, const PairDev *pd, const int2 *w)
-{
-    if constexpr (NH > max_nh<T>()) {
-        return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
-    } else {
-        const size_t smem = c2r_smem<T, NH>();
-        hipLaunchKernelGGL((k_row_c2r_eval<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
-                           twiddles<T>(ctx));
-        XC_HIP(ctx, hipGetLastError());
-        return XC_OK;
-    }
-}
-
-template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int2 *w)
-{
-    if (nb <= 0) return XC_OK;
-    switch (NH) {
-#define F(n) \
-    case n: return c2r_n<T, n>(ctx, nb, pd, w);
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
-}
-
-template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd)
-{
-    if (npairs <= 0) return XC_OK;
-    hipLaunchKernelGGL((k_finalize<T>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
-    XC_HIP(ctx, hipGetLastError());
-    return XC_OK;
-}
-
-template int launch_row_r2c<double>(xc_ctx *, int, int, const PairDev *, const int4 *);
-template int launch_row_r2c<float>(xc_ctx *, int, int, const PairDev *, const int4 *);
-template int launch_col<double>(xc_ctx *, int, bool, int, const PairDev *, const int2 *);
-template int launch_col<float>(xc_ctx *, int, bool, int, const PairDev *, const int2 *);
-template int launch_row_c2r<double>(xc_ctx *, int, int, const PairDev *, const int2 *);
-template int launch_row_c2r<float>(xc_ctx *, int, int, const PairDev *, const int2 *);
-template int launch_finalize<double>(xc_ctx *, int, const PairDev *);
-template int launch_finalize<float>(xc_ctx *, int, const PairDev *);
-template int row_group<double>(int);
-template int row_group<float>(int);
-template int col_group<double>(int);
-template int col_group<float>(int);
-
 }  // namespace xc
 
 using namespace xc;

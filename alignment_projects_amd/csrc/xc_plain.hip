@@ -75,8 +75,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
     size_t i_SA = lay.add(t1.S), i_QA = lay.add(t1.Q), i_ZA = lay.add(t1.Z);
     size_t i_SB = lay.add(t2.S), i_QB = lay.add(t2.Q), i_ZB = lay.add(t2.Z);
-    size_t i_pA = lay.add(t1.part), i_mA = lay.add(t1.msd);
-    size_t i_pB = lay.add(t2.part), i_mB = lay.add(t2.msd);
+    size_t i_pA = lay.add(t1.part), i_mA = lay.add(t1.msd), i_bA = lay.add(t1.band);
+    size_t i_pB = lay.add(t2.part), i_mB = lay.add(t2.msd), i_bB = lay.add(t2.band);
     size_t i_spA = lay.add(sizeof(cpx<double>) * h1 * H), i_spB = lay.add(sizeof(cpx<double>) * h2 * H);
     size_t i_lag = lay.add(sizeof(cpx<double>) * (size_t)P.hR * H);
     size_t i_rows = lay.add(sizeof(RowRec) * P.hR);
@@ -108,6 +108,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
         D.Q = (double *)at(k ? i_QB : i_QA);
         D.Z = (int32_t *)at(k ? i_ZB : i_ZA);
         D.part = (double *)at(k ? i_pB : i_pA);
+        D.bandp = (double *)at(k ? i_bB : i_bA);
+        D.nbands = sat_bands(D.v.h);  // a lone pair: its two images swept in row bands
         D.normalize = 0;
     }
     P.A = T[0].v;
@@ -157,6 +159,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     std::vector<SatGroup> sgroups;
     std::vector<int> sflat, soff;
     make_sat_groups({w1, w2}, sgroups, sflat, soff);
+    for (SatGroup &g : sgroups) g.nb = std::max(sat_bands(h1), sat_bands(h2));
     memcpy(h2p(i_slist), sflat.data(), sizeof(int) * sflat.size());
     for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];
 

@@ -61,8 +61,9 @@ struct TileDev {
     double *S, *Q;       // out: FP64 summed-area tables, w x h
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
+    double *bandp;       // scratch: column sums of the table sweep's row bands (nbands > 1)
     int32_t normalize;
-    int32_t pad;
+    int32_t nbands;      // row bands of the table sweep (1: one workgroup walks every row)
 };
 
 // One rotated copy of an image's pixel grid (CThmScan::RFromAngle,
