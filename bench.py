@@ -179,6 +179,24 @@ def run_strips(args, sync):
     value = NP * args.steps / elapsed
     ms_step = 1e3 * elapsed / args.steps
     log(f"batched: {value:.1f} pairs/s, {ms_step:.3f} ms/step")
+    pcie = None
+    try:
+        # the batch's pixels crossing PCIe from pinned host memory (not part of the value)
+        import torch
+        ha, hb = torch.from_numpy(A).pin_memory(), torch.from_numpy(B).pin_memory()
+        da, db = torch.empty_like(ha, device="cuda"), torch.empty_like(hb, device="cuda")
+        da.copy_(ha, non_blocking=True)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        da.copy_(ha, non_blocking=True)
+        db.copy_(hb, non_blocking=True)
+        torch.cuda.synchronize()
+        h2d = time.perf_counter() - t1
+        pcie = {"h2d_ms": round(1e3 * h2d, 3), "bytes": int(A.nbytes + B.nbytes),
+                "pairs_s_with_upload": round(NP / (h2d + ms_step * 1e-3), 1)}
+        del ha, hb, da, db
+    except Exception as e:  # pragma: no cover - depends on the box
+        log(f"PCIe measurement skipped: {e}")
     stages, roofline = stage_roofline(plan, args.stage_iters, ms_step)
     log("stages:", json.dumps(stages))
 
@@ -228,6 +246,7 @@ def run_strips(args, sync):
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
+    line["pcie_inclusive"] = pcie
     if cpu:
         line["batched_over_cpu"] = round(value / cpu["value"], 1)
     plan.destroy()

@@ -9,6 +9,9 @@ term, a wrong sign or index, a transposed operand -- fails it:
     CorrImagesR/S, including scan order, zones, disc, rules and parabola quirks;
   * closed forms for the geometry helpers.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -63,7 +66,8 @@ def test_fft2d_shift_theorem():
 def test_time_fft_planted_shift():
     # Retired_Proj/4_Time_FFT/time_fft.cpp:29-62: in2[(i + 2N + 3) % N^2] = in1[i],
     # out = IFT(F(in1) * conj(F(in2))), argmax j -> (j / N, j % N), wrapped above N/2.
-    N = 64
+    g = _golden()["time_fft"]
+    N = g["N"]
     rng = np.random.default_rng(2)
     in1 = rng.integers(0, 2 ** 31, N * N).astype(np.float64) / 1e9
     in2 = np.zeros(N * N)
@@ -75,7 +79,7 @@ def test_time_fft_planted_shift():
     x, y = j // N, j - N * (j // N)
     x = x - N if x > N // 2 else x
     y = y - N if y > N // 2 else y
-    assert (x, y) == (-2, -3)
+    assert [x, y] == g["expect"]["shift"]
 
 
 # ---------------------------------------------------------------------------
@@ -143,15 +147,18 @@ def test_integrate_image_box_sums():
 # Correlators
 # ---------------------------------------------------------------------------
 
+def _golden():
+    with open(os.path.join(os.path.dirname(__file__), "golden", "reference_harnesses.json")) as f:
+        return json.load(f)
+
+
 def test_testnormcorr_kat():
     # Retired_Proj/4_TestNormCorr/testnormcorr.cpp:20-63: "exact match with delta y = 2, delta x = -4"
-    pb = [(0, 3), (2, 3), (3, 2), (3, 3), (3, 4), (4, 3), (5, 4)]
-    vb = [1, 2, 1, 3, 1, 2, 1]
-    pa = [(7, 2), (8, 2), (9, 2), (7, 1), (8, 1)]
-    va = [1, 0, 1, 3, 2]
-    st, r, dx, dy, res = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=abi.rgn_some_area())
-    assert st == abi.XC_OK and abs(r - 1.0) < 1e-12
-    assert (res.lag_x, res.lag_y) == (3, 1) and round(dx) == -4 and dy == 2.0
+    g = _golden()["testnormcorr"]
+    pa, va, pb, vb, e = g["ip1"], g["iv1"], g["ip2"], g["iv2"], g["expect"]
+    st, r, dx, dy, res = oracle.corr_patches(pa, va, pb, vb, g["Ox"], g["Oy"], g["radius"], rgn=abi.rgn_some_area())
+    assert st == abi.XC_OK and abs(r - e["r"]) < 1e-12
+    assert [res.lag_x, res.lag_y] == e["lag"] and round(dx) == e["dx_rounded"] and dy == e["dy"]
     # the same answer through the provider that speaks the FFTW3 API, if one is installed
     try:
         oracle.set_fft_provider(1)
