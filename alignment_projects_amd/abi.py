@@ -127,6 +127,18 @@ def check(status, what="xcorr call", allow=(XC_OK, XC_NO_PEAK, XC_TOO_SMALL), ms
     return status
 
 
+XC_PLAN_REFRESH_TILES = 1
+XC_PLAN_AUTOTUNE = 2
+
+
+class XcBucketInfo(C.Structure):
+    _fields_ = [("nh", C.c_int32), ("ny", C.c_int32), ("whiten", C.c_int32), ("pairs", C.c_int32),
+                ("col_c", C.c_int32), ("col_xcd", C.c_int32), ("col_us", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class XcServeCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_batch", C.c_int32), ("streams", C.c_int32),
                 ("max_wait_us", C.c_int32), ("_pad", C.c_int32)]

@@ -6,6 +6,7 @@
 // is a fixed batch of pair correlations laid out once (descriptors, work
 // lists, spectra, lag rows) and replayed as one HIP graph.
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
@@ -221,6 +222,70 @@ struct RotKey {
     double deg;
     bool operator<(const RotKey &o) const { return t < o.t || (t == o.t && deg < o.deg); }
 };
+
+// Autotuner of the column pass: for each size bucket, time every shape the
+// device accepts (columns per workgroup x XCD order) on the plan's own data
+// and keep the fastest; a context remembers its choices (and XC_TUNE_FILE
+// keeps them across processes).
+static void tune_save(xc_ctx *ctx, const std::tuple<int, int, int> &key, const xc_col_tune &t)
+{
+    if (ctx->tune_file.empty()) return;
+    FILE *f = fopen(ctx->tune_file.c_str(), "a");
+    if (!f) return;
+    fprintf(f, "col %d %d %d %d %d %.3f\n", std::get<0>(key), std::get<1>(key), std::get<2>(key), t.c, t.xcd, t.us);
+    fclose(f);
+}
+
+static int tune_columns(xc_ctx *ctx, xc_plan *p)
+{
+    int st = enqueue(ctx, p->ex, ST_PREP, ST_COL);  // real spectra for the candidates to transform
+    if (st) return st;
+    XC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipEvent_t e0, e1;
+    XC_HIP(ctx, hipEventCreate(&e0));
+    XC_HIP(ctx, hipEventCreate(&e1));
+    for (Bucket &b : p->ex.buckets) {
+        const auto key = std::make_tuple(b.NH, b.NY, b.whiten ? 1 : 0);
+        auto it = ctx->tune.find(key);
+        if (it == ctx->tune.end()) {
+            xc_col_tune best;
+            best.us = 1e300;
+            for (int c : {1, 2, 4, 8, 16}) {
+                if (!col_shape_ok(ctx, b.NY, c)) continue;
+                for (int x : {1, 0}) {
+                    float tmin = 1e30f;
+                    bool ok = true;
+                    for (int rep = 0; rep < 3 && ok; ++rep) {
+                        ok = hipEventRecord(e0, ctx->stream) == hipSuccess &&
+                             launch_col<float>(ctx, b.NY, b.whiten, c, x, b.n_col, p->ex.d_pairs, b.d_col) == XC_OK &&
+                             hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+                        float t = 0.f;
+                        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) tmin = std::min(tmin, t);
+                    }
+                    if (!ok) {  // a shape this device refuses: skip it
+                        (void)hipGetLastError();
+                        continue;
+                    }
+                    if (1e3 * tmin < best.us) {
+                        best.c = c;
+                        best.xcd = x;
+                        best.us = 1e3 * tmin;
+                    }
+                }
+            }
+            if (best.c == 0) best.us = 0.0;  // nothing ran: keep the default shape
+            it = ctx->tune.emplace(key, best).first;
+            tune_save(ctx, key, best);
+        }
+        b.col_c = it->second.c;
+        b.col_xcd = it->second.xcd;
+        b.col_us = it->second.us;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->err.clear();
+    return XC_OK;
+}
 
 static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan *p)
 {
@@ -557,6 +622,11 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
                           (pa ? ea * (double)D.w1 * D.h1 : 0.0) - (pb ? eb * (double)D.w2 * D.h2 : 0.0);
     }
 
+    if (flags & XC_PLAN_AUTOTUNE) {
+        st = tune_columns(ctx, p);
+        if (st) return st;
+    }
+
     // ---- pinned results + graph ----
     XC_HIP(ctx, hipHostMalloc((void **)&p->h_res, sizeof(xc_result) * n, hipHostMallocDefault));
     XC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -589,6 +659,24 @@ extern "C" int xc_plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flag
     }
     *out = p;
     return XC_OK;
+}
+
+extern "C" int xc_plan_buckets(const xc_plan *p, xc_bucket_info *out, int cap)
+{
+    if (!p) return XC_E_ARG;
+    const int nb = (int)p->ex.buckets.size();
+    for (int i = 0; i < nb && i < cap && out; ++i) {
+        const Bucket &b = p->ex.buckets[i];
+        xc_bucket_info &o = out[i];
+        o.nh = b.NH;
+        o.ny = b.NY;
+        o.whiten = b.whiten ? 1 : 0;
+        o.pairs = b.count;
+        o.col_c = b.col_c;
+        o.col_xcd = b.col_xcd;
+        o.col_us = b.col_us;
+    }
+    return nb;
 }
 
 extern "C" int xc_plan_launch(xc_plan *p)

@@ -19,7 +19,10 @@ struct Bucket {
     const int4 *d_r2c = nullptr;
     int n_r2c = 0;
     const int2 *d_col = nullptr;
-    int n_col = 0;
+    int n_col = 0;     // column work items (COL_STEP columns each)
+    int col_c = 0;     // columns per workgroup of the column pass (0: default)
+    int col_xcd = 1;   // XCD-aware workgroup order in the column pass
+    double col_us = 0; // tuned time of the column pass (0: not tuned)
     const int2 *d_c2r = nullptr;
     int n_c2r = 0;
 };

@@ -3,7 +3,9 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -37,6 +39,12 @@ struct xc_tile {
     std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
 };
 
+// A tuned column-pass shape (see XC_PLAN_AUTOTUNE).
+struct xc_col_tune {
+    int c = 0, xcd = 1;
+    double us = 0.0;
+};
+
 struct xc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -48,6 +56,9 @@ struct xc_ctx {
     void *pinned = nullptr; // pinned host staging
     size_t pinned_bytes = 0;
     int num_cu = 0;
+    size_t lds_max = 65536;  // LDS bytes one workgroup may allocate
+    std::map<std::tuple<int, int, int>, xc_col_tune> tune;  // (NH, NY, whiten) -> column-pass shape
+    std::string tune_file;  // XC_TUNE_FILE: tuned shapes persist here across processes
 };
 
 namespace xc {
@@ -107,11 +118,14 @@ int live_rows(const PairDev &P);
 
 // Kernel launchers (dispatch on transform length).
 template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work);
-template <typename T> int launch_col(xc_ctx *ctx, int NY, bool whiten, int nblocks, const PairDev *pd, const int2 *work);
+// cwg: columns per workgroup (0: the default shape); xcd: XCD-aware workgroup order.
+template <typename T>
+int launch_col(xc_ctx *ctx, int NY, bool whiten, int cwg, int xcd, int nitems, const PairDev *pd, const int2 *work);
+bool col_shape_ok(const xc_ctx *ctx, int NY, int cwg);  // an FP32 column pass of cwg columns fits
 template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int2 *work);
 template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd);
 template <typename T> int row_group(int NH);   // rows per workgroup of the row kernels
-template <typename T> int col_group(int NY);   // columns per workgroup of the column kernel
+template <typename T> int col_group(int NY);   // columns per work item of the column kernel
 
 template <typename T> const cpx<T> *twiddles(xc_ctx *ctx);
 

@@ -777,29 +777,42 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 // ------------------------------------------------------------------
 
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
+constexpr int pow2_floor(int n) { return n >= 2 ? 2 * pow2_floor(n / 2) : 1; }
+
+// Column work items cover COL_STEP spectrum columns; a workgroup takes C of
+// them (C divides COL_STEP), so one work list serves every C.
+constexpr int COL_STEP = 16;
 
 template <typename T, int NY> struct ColCfg {
     static constexpr int TPF = Plan<NY>::TPF;
-    static constexpr int CW = sizeof(T) == 8 ? 4 : 8;  // columns per workgroup wanted (64 B rows)
+    static constexpr int CW = sizeof(T) == 8 ? 4 : 8;  // columns per workgroup by default (64 B rows)
     static constexpr int CT = 1024 / TPF > 0 ? 1024 / TPF : 1;                       // <= 1024 threads
     static constexpr int CL = 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) > 0 ? 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) : 1;  // <= 64 KB LDS
-    static constexpr int C = cmin(CW, cmin(CT, CL));
+    static constexpr int C = pow2_floor(cmin(CW, cmin(CT, CL)));  // a power of two, so it divides COL_STEP
     static constexpr int THREADS = TPF * C;
+    // a workgroup of CC columns fits: <= 1024 threads and <= 160 KB of LDS
+    static constexpr bool fits(int CC)
+    {
+        return CC >= 1 && COL_STEP % CC == 0 && TPF * CC <= 1024 &&
+               (long long)CC * lds_len(NY) * (long long)sizeof(cpx<T>) <= 163840;
+    }
 };
 
-template <typename T, int NY, bool WHITEN>
-__global__ void __launch_bounds__((ColCfg<T, NY>::THREADS))
-    k_col_fused(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw)
+template <typename T, int NY, bool WHITEN, int C>
+__global__ void __launch_bounds__(Plan<NY>::TPF * C)
+    k_col_fused(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw,
+                int xcd)
 {
     using PL = Plan<NY>;
-    constexpr int E = PL::E, TPF = PL::TPF, C = ColCfg<T, NY>::C;
+    constexpr int E = PL::E, TPF = PL::TPF, SUB = COL_STEP / C;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *smem = reinterpret_cast<cpx<T> *>(smem_raw);
     const int tid = threadIdx.x, c = tid % C, t = tid / C;
-    const int2 wi = work[xcd_remap(blockIdx.x, gridDim.x)];  // {pair, k0}
+    const int bid = xcd ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int2 wi = work[bid / SUB];  // {pair, k0}
     const PairDev &P = pairs[wi.x];
     const int H = P.Nx / 2 + 1;
-    const int k = wi.y + c;
+    const int k = wi.y + (bid % SUB) * C + c;
     const bool kv = k < H;
     const cpx<T> *SA = reinterpret_cast<const cpx<T> *>(P.specA);
     const cpx<T> *SB = reinterpret_cast<const cpx<T> *>(P.specB);

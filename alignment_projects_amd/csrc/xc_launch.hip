@@ -6,6 +6,7 @@
 #include <vector>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "xc_host.hpp"
@@ -233,7 +234,28 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
         return XC_E_DEVICE;
     }
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cu = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        ctx->num_cu = prop.multiProcessorCount;
+        ctx->lds_max = prop.sharedMemPerBlock;
+    }
+    if (const char *tf = getenv("XC_TUNE_FILE")) {
+        // "col NH NY whiten c xcd us" lines written by earlier tuning runs
+        ctx->tune_file = tf;
+        if (FILE *f = fopen(tf, "r")) {
+            char tag[8];
+            int nh, ny, wh, c, x;
+            double us;
+            while (fscanf(f, "%7s %d %d %d %d %d %lf", tag, &nh, &ny, &wh, &c, &x, &us) == 7)
+                if (!strcmp(tag, "col") && (c == 0 || col_shape_ok(ctx, ny, c))) {
+                    xc_col_tune t;
+                    t.c = c;
+                    t.xcd = x;
+                    t.us = us;
+                    ctx->tune[std::make_tuple(nh, ny, wh)] = t;
+                }
+            fclose(f);
+        }
+    }
     // e^{-2 pi i m / XC_TW_N}, from long-double exact-angle cos/sin
     std::vector<cpx<double>> t64(XC_TW_N);
     std::vector<cpx<float>> t32(XC_TW_N);

@@ -16,16 +16,7 @@ template <typename T> int row_group(int NH)
     return 1;
 }
 
-template <typename T> int col_group(int NY)
-{
-    switch (NY) {
-#define F(n) \
-    case n: return ColCfg<T, n>::C;
-        XC_POW2_CASES(F)
-#undef F
-    }
-    return 1;
-}
+template <typename T> int col_group(int) { return COL_STEP; }
 
 template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
 {

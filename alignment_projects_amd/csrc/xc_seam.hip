@@ -127,7 +127,7 @@ __global__ void k_p2i_assign(const xc_point *__restrict__ pts, const double *__r
     if (x >= 0 && x < N && y >= 0 && y < N && owner[x + (long long)N * y] == i) img[x + (long long)N * y] = v[i];
 }
 
-#define XC_POW2_CASES(F) \
+#define XC_SEAM_CASES(F) \
     F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048)
 
 static int seam_r2c(xc_ctx *ctx, int NH, const double *in, int rows, cpx<double> *out)
@@ -138,7 +138,7 @@ static int seam_r2c(xc_ctx *ctx, int NH, const double *in, int rows, cpx<double>
         hipLaunchKernelGGL((k_seam_r2c<double, n>), dim3((rows + RowCfg<n>::G - 1) / RowCfg<n>::G),          \
                            dim3(RowCfg<n>::THREADS), 0, ctx->stream, in, rows, out, ctx->tw64);             \
         return XC_OK;
-        XC_POW2_CASES(F)
+        XC_SEAM_CASES(F)
 #undef F
     }
     return set_err(ctx, XC_E_SHAPE, "row length %d unsupported", 2 * NH);
@@ -152,7 +152,7 @@ static int seam_c2r(xc_ctx *ctx, int NH, const cpx<double> *in, int rows, double
         hipLaunchKernelGGL((k_seam_c2r<double, n>), dim3((rows + RowCfg<n>::G - 1) / RowCfg<n>::G),          \
                            dim3(RowCfg<n>::THREADS), 0, ctx->stream, in, rows, out, ctx->tw64);             \
         return XC_OK;
-        XC_POW2_CASES(F)
+        XC_SEAM_CASES(F)
 #undef F
     }
     return set_err(ctx, XC_E_SHAPE, "row length %d unsupported", 2 * NH);
@@ -166,7 +166,7 @@ template <int DIR> static int seam_col(xc_ctx *ctx, int NY, cpx<double> *data, i
         hipLaunchKernelGGL((k_seam_col<double, n, DIR>), dim3((H + ColCfg<double, n>::C - 1) / ColCfg<double, n>::C), \
                            dim3(ColCfg<double, n>::THREADS), 0, ctx->stream, data, H, ctx->tw64);                \
         return XC_OK;
-        XC_POW2_CASES(F) F(4096)
+        XC_SEAM_CASES(F) F(4096)
 #undef F
     }
     return set_err(ctx, XC_E_SHAPE, "column length %d unsupported", NY);
@@ -331,7 +331,7 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     XC_HIP(ctx, hipGetLastError());
     const PairDev *dP = (const PairDev *)at(i_pair);
     if ((st = launch_row_r2c<double>(ctx, NH, nr2c, dP, (const int4 *)at(i_wr)))) return st;
-    if ((st = launch_col<double>(ctx, N, false, ncol, dP, (const int2 *)at(i_wc)))) return st;
+    if ((st = launch_col<double>(ctx, N, false, 0, 1, ncol, dP, (const int2 *)at(i_wc)))) return st;
     if ((st = launch_row_c2r<double>(ctx, NH, nc2r, dP, (const int2 *)at(i_wc2)))) return st;
     if ((st = launch_finalize<double>(ctx, 1, dP))) return st;
     XC_HIP(ctx, hipMemcpyAsync(hres, at(i_res), sizeof(xc_result), hipMemcpyDeviceToHost, s));

@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import abi
-from .abi import XcCntRule, XcPair, XcPoint, XcResult, XcRgnRule, XcServeCfg, XcServeStats
+from .abi import XcBucketInfo, XcCntRule, XcPair, XcPoint, XcResult, XcRgnRule, XcServeCfg, XcServeStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libxcorr.so")
@@ -74,6 +74,7 @@ _OPTIONAL = {
     "xc_plan_stage_times": ([C.c_void_p, C.c_int, _dp, _dp], C.c_int),
     "xc_plan_model_bytes": ([C.c_void_p], C.c_double),
     "xc_plan_destroy": ([C.c_void_p], C.c_int),
+    "xc_plan_buckets": ([C.c_void_p, C.POINTER(XcBucketInfo), C.c_int], C.c_int),
     "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
     "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
     "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
@@ -227,8 +228,8 @@ class Context:
         self._check(st, "xc_sweep_angles_f32", ok=(abi.XC_OK,))
         return list(res)
 
-    def plan(self, pairs, refresh_tiles=False):
-        return Plan(self, pairs, refresh_tiles)
+    def plan(self, pairs, refresh_tiles=False, autotune=False):
+        return Plan(self, pairs, refresh_tiles, autotune)
 
     def batch_corr(self, pairs):
         arr = (XcPair * len(pairs))(*pairs)
@@ -261,12 +262,13 @@ def make_pair(a, b, mode=abi.XC_MODE_NCC, Ox=0, Oy=0, radius=4096, Rx=-1, Ry=-1,
 class Plan:
     """A fixed batch of pairs, laid out once and replayed as one HIP graph (xc_plan)."""
 
-    def __init__(self, ctx, pairs, refresh_tiles=False):
+    def __init__(self, ctx, pairs, refresh_tiles=False, autotune=False):
         self.ctx = ctx
         self.n = len(pairs)
         arr = (XcPair * self.n)(*pairs)
         h = C.c_void_p()
-        st = ctx.L.xc_plan_create(ctx.h, arr, self.n, 1 if refresh_tiles else 0, C.byref(h))
+        flags = (abi.XC_PLAN_REFRESH_TILES if refresh_tiles else 0) | (abi.XC_PLAN_AUTOTUNE if autotune else 0)
+        st = ctx.L.xc_plan_create(ctx.h, arr, self.n, flags, C.byref(h))
         ctx._check(st, "xc_plan_create", ok=(abi.XC_OK,))
         self.h = h
         self._res = (XcResult * self.n)()
@@ -294,6 +296,13 @@ class Plan:
     @property
     def model_bytes(self):
         return self.ctx.L.xc_plan_model_bytes(self.h)
+
+    def buckets(self):
+        """Size buckets with their column-pass shapes (tuned ones carry col_us)."""
+        nb = self.ctx.L.xc_plan_buckets(self.h, None, 0)
+        arr = (XcBucketInfo * max(nb, 1))()
+        self.ctx.L.xc_plan_buckets(self.h, arr, nb)
+        return [arr[i].as_dict() for i in range(nb)]
 
     def destroy(self):
         if self.h:

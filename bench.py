@@ -172,7 +172,7 @@ def run_strips(args, sync):
     ctx = xcorr.Context(0)
     tiles = [(ctx.tile_put(A[i]), ctx.tile_put(B[i])) for i in range(NP)]
     pairs = [xcorr.make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
-    plan = ctx.plan(pairs, refresh_tiles=True)
+    plan = ctx.plan(pairs, refresh_tiles=True, autotune=True)
     log("plan ready")
     elapsed, lat = time_plan(plan, args.steps, args.warmup, sync)
     res = plan.run(want=True)
@@ -243,6 +243,7 @@ def run_strips(args, sync):
     line["plain_fp64_gpu"] = {"value": round(plain_rate, 1), "unit": "pairs/s", "pairs": nplain,
                               "batched_over_plain": round(value / plain_rate, 1)}
     line["stages"] = stages
+    line["autotune"] = plan.buckets()
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
@@ -277,7 +278,7 @@ def run_sweep(args, sync):
               ctx.tile_put(B[i].astype(np.float32), normalize=False)) for i in range(NP)]
     pairs = [xcorr.make_pair(ta, tb, mode=abi.XC_MODE_S, rgn=rgn, cnt=cnt, angle=ang)
              for ta, tb in tiles for ang in angles]
-    plan = ctx.plan(pairs)
+    plan = ctx.plan(pairs, autotune=True)
     log(f"plan ready: {len(pairs)} correlations")
     elapsed, lat = time_plan(plan, args.steps, args.warmup, sync)
     res = plan.run(want=True)
@@ -319,6 +320,7 @@ def run_sweep(args, sync):
     line["latency_ms"] = {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["stages"] = stages
+    line["autotune"] = plan.buckets()
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
@@ -355,7 +357,7 @@ def run_uneven(args, sync):
         tiles.append((ta, tb))
         h, w = a.shape
         pairs.append(xcorr.make_pair(ta, tb, cnt=abi.cnt_enough_points(w * h // 4)))
-    plan = ctx.plan(pairs, refresh_tiles=True)
+    plan = ctx.plan(pairs, refresh_tiles=True, autotune=True)
     log("plan ready")
     elapsed, lat = time_plan(plan, args.steps, args.warmup, sync)
     res = plan.run(want=True)
@@ -396,6 +398,7 @@ def run_uneven(args, sync):
     line["latency_ms"] = {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["stages"] = stages
+    line["autotune"] = plan.buckets()
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc

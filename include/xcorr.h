@@ -256,10 +256,23 @@ int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *resul
  * (CThmScan::TCDGet, CThmScan.cpp:77-107), without per-call planning.
  * XC_PLAN_REFRESH_TILES re-runs the tiles' statistics and summed-area tables
  * inside every run (for tiles whose pixels are rewritten between runs).
+ * XC_PLAN_AUTOTUNE times the candidate shapes of the column pass (columns
+ * per workgroup 1..16, XCD-aware workgroup order on/off) on each of the
+ * plan's size buckets and keeps the fastest; choices are cached per context
+ * (and in the file named by XC_TUNE_FILE, if set).  Shapes never change
+ * results, only speed.  (The reference plans each FFT per call with
+ * FFTW_ESTIMATE, Correlation_fft_fftw.cpp:20-26.)
  */
 typedef struct xc_plan xc_plan;
 
-enum xc_plan_flags { XC_PLAN_REFRESH_TILES = 1 };
+enum xc_plan_flags { XC_PLAN_REFRESH_TILES = 1, XC_PLAN_AUTOTUNE = 2 };
+
+/* One size bucket of a plan: transform shape, pairs, column-pass shape. */
+typedef struct xc_bucket_info {
+    int32_t nh, ny, whiten, pairs;
+    int32_t col_c, col_xcd;  /* columns per workgroup (0: default), XCD order */
+    double col_us;           /* tuned column-pass time, microseconds (0: not tuned) */
+} xc_bucket_info;
 
 /* Pipeline stages reported by xc_plan_stage_times. */
 enum xc_stage { XC_STAGE_PREP = 0, XC_STAGE_R2C = 1, XC_STAGE_COL = 2, XC_STAGE_C2R = 3, XC_STAGE_FIN = 4,
@@ -279,6 +292,8 @@ int xc_plan_stage_times(xc_plan *plan, int iters, double *ms, double *bytes);
 /* Algorithmic HBM bytes of one run of the whole plan (inputs once; row
  * spectra, lag rows and FP64 tables written once and read once). */
 double xc_plan_model_bytes(const xc_plan *plan);
+/* Copies up to cap bucket records; returns the plan's bucket count. */
+int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);
 int xc_plan_destroy(xc_plan *plan);
 
 /*

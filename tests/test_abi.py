@@ -29,6 +29,9 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.XcRgnRule) == 56
     assert ctypes.sizeof(abi.XcCntRule) == 24
     assert ctypes.sizeof(abi.XcResult) == 72
+    assert ctypes.sizeof(abi.XcBucketInfo) == 32
+    assert ctypes.sizeof(abi.XcServeCfg) == 24
+    assert ctypes.sizeof(abi.XcServeStats) == 64
     assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
 
 

@@ -116,3 +116,34 @@ def test_alone_vs_mix_bitwise(ctx):
     for ta, tb in tiles:
         ta.release()
         tb.release()
+
+
+def test_autotuned_plan_matches_default(ctx):
+    """Column-pass shapes chosen by the autotuner change speed only: results are bitwise those of the
+    default shapes, for strips (one bucket) and a mix of sizes (several buckets)."""
+    from alignment_projects_amd.xcorr import make_pair
+    data = _strips(6, 512, 64, 20, 900)
+    rng = np.random.default_rng(901)
+    extra = []
+    for i in range(6):
+        w, h = (int(v) for v in rng.integers(40, 300, size=2))
+        A = synth.quantize(synth.bandlimited(rng, h, w, 128, 40), np.uint8)
+        B = np.roll(A, (int(rng.integers(-5, 6)), int(rng.integers(-5, 6))), axis=(0, 1))
+        extra.append((A, B))
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data] + [(ctx.tile_put(a), ctx.tile_put(b)) for a, b in extra]
+    rgn = abi.rgn_min(16, 16)
+    pairs = [make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
+    pairs += [make_pair(ta, tb, mode=abi.XC_MODE_S, rgn=rgn) for ta, tb in tiles[6:]]
+    p0 = ctx.plan(pairs)
+    p1 = ctx.plan(pairs, autotune=True)
+    r0, r1 = p0.run(), p1.run()
+    b1 = p1.buckets()
+    assert len(b1) >= 3 and all(b["col_us"] > 0 and b["col_c"] in (1, 2, 4, 8, 16) for b in b1), b1
+    fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak")
+    for a, b in zip(r0, r1):
+        assert all(getattr(a, f) == getattr(b, f) for f in fields), (a.as_dict(), b.as_dict())
+    p0.destroy()
+    p1.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
