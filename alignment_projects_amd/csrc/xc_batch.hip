@@ -71,7 +71,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
         return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
     }
     Layout lay;
-    size_t iS = lay.add(ts.S), iQ = lay.add(ts.Q), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
+    size_t iS = lay.add(ts.SQ), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
     size_t iM = lay.add(ts.msd), iD = lay.add(sizeof(TileDev)), iL = lay.add(sizeof(int));
     size_t iRaw = dev_src ? 0 : lay.add((size_t)es * stride * h);
     size_t iMask = (!dev_src && mask) ? lay.add((size_t)w * h) : 0;
@@ -111,8 +111,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.v.mstride = w;
     D.msd = (double *)(base + lay.off[iM]);
     D.v.msd = D.msd;
-    D.S = (double *)(base + lay.off[iS]);
-    D.Q = (double *)(base + lay.off[iQ]);
+    D.SQ = (double2 *)(base + lay.off[iS]);
     D.Z = (int32_t *)(base + lay.off[iZ]);
     D.part = (double *)(base + lay.off[iP]);
     D.nbands = 1;  // swept unbanded, as a plan's batched refresh sweeps it
@@ -371,11 +370,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         narrow_rows(D);
         D.A = a->dev.v;
         D.B = b->dev.v;
-        D.SA = a->dev.S;
-        D.QA = a->dev.Q;
+        D.SQA = a->dev.SQ;
         D.ZA = a->dev.Z;
-        D.SB = b->dev.S;
-        D.QB = b->dev.Q;
+        D.SQB = b->dev.SQ;
         D.ZB = b->dev.Z;
     }
     std::vector<Bucket> buckets;
@@ -425,15 +422,14 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         i_rows[k] = lay.add(sizeof(RowRec) * D.hR);
     }
     // rotated copies: raster + tables
-    std::vector<size_t> i_rimg(rots.size()), i_rS(rots.size()), i_rQ(rots.size()), i_rZ(rots.size());
+    std::vector<size_t> i_rimg(rots.size()), i_rS(rots.size()), i_rZ(rots.size());
     std::vector<bool> rot_sq(rots.size(), false);
     for (int i = 0; i < n; ++i)
         if (pair_rot[i] >= 0 && pairs[i].mode != XC_MODE_S) rot_sq[pair_rot[i]] = true;
     for (size_t r = 0; r < rots.size(); ++r) {
         const size_t px = (size_t)rots[r].w * rots[r].h;
         i_rimg[r] = lay.add(sizeof(float) * px);
-        i_rS[r] = lay.add(rot_sq[r] ? sizeof(double) * px : 8);
-        i_rQ[r] = lay.add(rot_sq[r] ? sizeof(double) * px : 8);
+        i_rS[r] = lay.add(rot_sq[r] ? sizeof(double2) * px : 16);
         i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
     }
     size_t i_res = lay.add(sizeof(xc_result) * n);
@@ -517,8 +513,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         T.v.mstride = R.w;
         T.v.msd = dmsd;
         T.msd = nullptr;
-        T.S = rot_sq[r] ? (double *)at(i_rS[r]) : nullptr;
-        T.Q = rot_sq[r] ? (double *)at(i_rQ[r]) : nullptr;
+        T.SQ = rot_sq[r] ? (double2 *)at(i_rS[r]) : nullptr;
         T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
         T.nbands = 1;
         hd[r] = T;
@@ -535,8 +530,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         if (pair_rot[i] >= 0) {
             const TileDev &T = hd[pair_rot[i]];
             D.A = T.v;
-            D.SA = T.S;
-            D.QA = T.Q;
+            D.SQA = T.SQ;
             D.ZA = T.Z;
         }
         D.specA = at(i_spec[specA[i]]);

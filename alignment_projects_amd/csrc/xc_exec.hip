@@ -32,8 +32,7 @@ TileSizes tile_sizes(int w, int h)
 {
     TileSizes t;
     const size_t n = (size_t)w * h;
-    t.S = sizeof(double) * n;
-    t.Q = sizeof(double) * n;
+    t.SQ = sizeof(double2) * n;
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
     t.msd = sizeof(double) * 3;

@@ -66,7 +66,7 @@ int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
 
 // Bytes of device scratch/table memory one prepared image needs.
 struct TileSizes {
-    size_t S, Q, Z, part, msd, band;
+    size_t SQ, Z, part, msd, band;
 };
 TileSizes tile_sizes(int w, int h);
 

@@ -64,6 +64,29 @@ __device__ __forceinline__ TT box_sum(const TT *__restrict__ t, int w, int L, in
     return r;
 }
 
+// The same lookups on the interleaved (S, S^2) table: one 16-byte load per
+// corner, each component summed as box_sum sums it.
+__host__ __device__ __forceinline__ double2 box_sum2(const double2 *__restrict__ t, int w, int L, int R, int B, int T)
+{
+    double2 r = t[R + (long long)w * T];
+    if (L > 0) {
+        const double2 u = t[L - 1 + (long long)w * T];
+        r.x -= u.x;
+        r.y -= u.y;
+    }
+    if (B > 0) {
+        const double2 u = t[R + (long long)w * (B - 1)];
+        r.x -= u.x;
+        r.y -= u.y;
+    }
+    if (L > 0 && B > 0) {
+        const double2 u = t[(L - 1) + (long long)w * (B - 1)];
+        r.x += u.x;
+        r.y += u.y;
+    }
+    return r;
+}
+
 // Row part of BoxesFromShifts (Geometry.cpp:150-176) for signed lag y.
 struct RowBox {
     int a_b, a_t, b_b, b_t, olh;
@@ -115,10 +138,9 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
     *legal = ok;
     if (P.mode == PM_S) return raw / Nxy;
     double n = (double)(olw * rb.olh);
-    double s1 = box_sum(P.SA, P.w1, AL, AR, rb.a_b, rb.a_t);
-    double q1 = box_sum(P.QA, P.w1, AL, AR, rb.a_b, rb.a_t);
-    double s2 = box_sum(P.SB, P.w2, BL, BR, rb.b_b, rb.b_t);
-    double q2 = box_sum(P.QB, P.w2, BL, BR, rb.b_b, rb.b_t);
+    const double2 sqa = box_sum2(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
+    const double2 sqb = box_sum2(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
+    double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
     double num = n * raw / Nxy - s1 * s2;
     double d1 = n * q1 - s1 * s1;
     double d2 = n * q2 - s2 * s2;
@@ -547,9 +569,9 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 #pragma unroll
             for (int j = 0; j < CPT; j += 2) {
                 if (x0 + j < w) {  // w even: x0 + j + 1 < w too
-                    if (D.S) {
-                        *reinterpret_cast<double2 *>(D.S + row + x0 + j) = make_double2(bs + ps[j], bs + ps[j + 1]);
-                        *reinterpret_cast<double2 *>(D.Q + row + x0 + j) = make_double2(bq + pq[j], bq + pq[j + 1]);
+                    if (D.SQ) {
+                        D.SQ[row + x0 + j] = make_double2(bs + ps[j], bq + pq[j]);
+                        D.SQ[row + x0 + j + 1] = make_double2(bs + ps[j + 1], bq + pq[j + 1]);
                     }
                     if (D.Z) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(bz + pz[j], bz + pz[j + 1]);
                 }
@@ -558,10 +580,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 if (x0 + j < w) {
-                    if (D.S) {
-                        D.S[row + x0 + j] = bs + ps[j];
-                        D.Q[row + x0 + j] = bq + pq[j];
-                    }
+                    if (D.SQ) D.SQ[row + x0 + j] = make_double2(bs + ps[j], bq + pq[j]);
                     if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
                 }
             }
@@ -959,7 +978,7 @@ struct NccRow {
     int olh;
     int x0, x1;    // legal-interval bounds (empty when x0 > x1)
     double invN;   // 1 / (Nx Ny), a power of two: raw * invN == raw / Nxy exactly
-    const double *sAt, *sAb, *qAt, *qAb, *sBt, *sBb, *qBt, *qBb;  // table rows: band top, row below band (or null)
+    const double2 *sqAt, *sqAb, *sqBt, *sqBb;  // (S, S^2) table rows: band top, row below the band (or null)
     const int32_t *zAt, *zAb, *zBt, *zBb;
 };
 
@@ -1002,14 +1021,10 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
     q.x1 = ok ? (int)hi : 0;
     const long long at = (long long)rb.a_t * P.w1, ab = (long long)(rb.a_b - 1) * P.w1;
     const long long bt = (long long)rb.b_t * P.w2, bb = (long long)(rb.b_b - 1) * P.w2;
-    q.sAt = P.SA + at;
-    q.qAt = P.QA + at;
-    q.sBt = P.SB + bt;
-    q.qBt = P.QB + bt;
-    q.sAb = rb.a_b > 0 ? P.SA + ab : nullptr;
-    q.qAb = rb.a_b > 0 ? P.QA + ab : nullptr;
-    q.sBb = rb.b_b > 0 ? P.SB + bb : nullptr;
-    q.qBb = rb.b_b > 0 ? P.QB + bb : nullptr;
+    q.sqAt = P.SQA + at;
+    q.sqBt = P.SQB + bt;
+    q.sqAb = rb.a_b > 0 ? P.SQA + ab : nullptr;
+    q.sqBb = rb.b_b > 0 ? P.SQB + bb : nullptr;
     q.zAt = P.ZA ? P.ZA + at : nullptr;
     q.zBt = P.ZB ? P.ZB + bt : nullptr;
     q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
@@ -1031,10 +1046,36 @@ __device__ __forceinline__ TT band_sum(const TT *__restrict__ top, const TT *__r
     return v;
 }
 
+// band_sum on the interleaved (S, S^2) table.
+__device__ __forceinline__ double2 band_sum2(const double2 *__restrict__ top, const double2 *__restrict__ bot, int L, int R)
+{
+    double2 v = top[R];
+    if (L > 0) {
+        const double2 u = top[L - 1];
+        v.x -= u.x;
+        v.y -= u.y;
+    }
+    if (bot) {
+        const double2 u = bot[R];
+        v.x -= u.x;
+        v.y -= u.y;
+        if (L > 0) {
+            const double2 z = bot[L - 1];
+            v.x += z.x;
+            v.y += z.y;
+        }
+    }
+    return v;
+}
+
 // NCC value of lag x of a row (CLinCorr::CheckDensity/GetCorr, Correlation.cpp:697-754);
 // the lag is inside the row's legal interval.  *cand: passes exact_w, the table and
 // LegalCnt; *odd: |r| > 1.0001.
 #pragma clang fp contract(off)
+// FP64 (plain form): GetCorr's division and square root in FP64.  FP32
+// (batched form): the sums and both subtractions stay FP64, the final
+// num / sqrt(d) is taken in FP32 (the batched form's precision).
+template <typename T>
 __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int x, double raw, bool *cand, bool *odd)
 {
     const int BL = x > 0 ? x : 0;
@@ -1042,7 +1083,9 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     const int AL = BL - x, AR = BR - x;
     const int olw = BR - BL + 1;
     const RuleDev &g = P.rule;
-    *odd = false;
+    const double2 sqa = band_sum2(q.sqAt, q.sqAb, AL, AR);
+    const double2 sqb = band_sum2(q.sqBt, q.sqBb, BL, BR);
+    const double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
     bool c = true;
     if (g.exact_w > 0 && olw != g.exact_w) c = false;
     if (g.table && (olw > g.table_w || q.olh > g.table_h || !g.table[(long long)(q.olh - 1) * g.table_w + (olw - 1)]))
@@ -1053,21 +1096,30 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
         if (!((long long)c1 > g.k1 && (long long)c2 > g.k2)) c = false;
     }
     *cand = c;
-    if (!c) return 0.0;
     const double n = (double)(olw * q.olh);
-    const double s1 = band_sum(q.sAt, q.sAb, AL, AR);
-    const double q1 = band_sum(q.qAt, q.qAb, AL, AR);
-    const double s2 = band_sum(q.sBt, q.sBb, BL, BR);
-    const double q2 = band_sum(q.qBt, q.qBb, BL, BR);
     const double num = n * raw * q.invN - s1 * s2;
     const double d1 = n * q1 - s1 * s1;
     const double d2 = n * q2 - s2 * s2;
     const double d = d1 * d2;
-    const double r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
-    *odd = fabs(r) > 1.0001;
-    return r;
+    double r;
+    if constexpr (sizeof(T) == 8) r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
+    else r = (d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d)));
+    *odd = c && fabs(r) > 1.0001;
+    return c ? r : 0.0;
 }
 #pragma clang fp contract(on)
+
+// CorrPatches' R-image value at lag (x, y): the NCC of a candidate lag, 0
+// for any other (outside the row's legal interval or failing a per-lag
+// rule), as the scan leaves it.
+template <typename T>
+__device__ __forceinline__ double ncc_value(const PairDev &P, int x, int y, double raw)
+{
+    const NccRow q = ncc_row(P, y);
+    if (x < q.x0 || x > q.x1) return 0.0;
+    bool cand, odd;
+    return ncc_lag<T>(P, q, x, raw, &cand, &odd);
+}
 
 template <typename T, int NH>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
@@ -1136,7 +1188,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
             for (int x = segs[sgi][0] + t; x <= segs[sgi][1]; x += TPF) {
                 const int ix = x >= 0 ? x : NX + x;
                 bool cand, odd;
-                const double val = ncc_lag(P, q, x, (double)R[ix], &cand, &odd);
+                const double val = ncc_lag<T>(P, q, x, (double)R[ix], &cand, &odd);
                 if (cand) {
                     flags |= odd ? 3 : 1;
                     if (val > bestv) {
@@ -1221,8 +1273,14 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
                 }
                 double val = 0.0;
                 if (ix >= 0 && ix < NX) {
-                    bool cand, win, odd;
-                    val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+                    if (P.mode == PM_NCC) {
+                        bool dead;
+                        const int x = lag_x_of_raster(P, ix, &dead);
+                        if (!dead) val = ncc_value<T>(P, x, y, (double)R[ix]);
+                    } else {
+                        bool cand, win, odd;
+                        val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+                    }
                 }
                 if (s) r.right = val;
                 else r.left = val;
@@ -1392,13 +1450,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
                     if (ylr >= 0 && row_live(P, ylr)) {
                         const int yy = lag_y_of_row(P, ylr);
                         const double raw = point_c2r<T>(P, ylr, x, tw, sv);
-                        const int dxd = x - P.Ox, dyd = yy - P.Oy;
-                        if ((long long)dxd * dxd + (long long)dyd * dyd <= (long long)P.radius * P.radius) {
-                            bool legal, odd;
-                            const RowBox rb = row_box(P, yy);
-                            const double r = eval_lag(P, rb, x, raw, &legal, &odd);
-                            val = legal ? r : 0.0;
-                        }
+                        val = ncc_value<T>(P, x, yy, raw);  // 0 outside the disc and the rules
                     }
                     zz[s] = val;
                 }

@@ -73,8 +73,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n), i_vals = lay.add(sizeof(double) * n);
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
-    size_t i_SA = lay.add(t1.S), i_QA = lay.add(t1.Q), i_ZA = lay.add(t1.Z);
-    size_t i_SB = lay.add(t2.S), i_QB = lay.add(t2.Q), i_ZB = lay.add(t2.Z);
+    size_t i_SA = lay.add(t1.SQ), i_ZA = lay.add(t1.Z);
+    size_t i_SB = lay.add(t2.SQ), i_ZB = lay.add(t2.Z);
     size_t i_pA = lay.add(t1.part), i_mA = lay.add(t1.msd), i_bA = lay.add(t1.band);
     size_t i_pB = lay.add(t2.part), i_mB = lay.add(t2.msd), i_bB = lay.add(t2.band);
     size_t i_spA = lay.add(sizeof(cpx<double>) * h1 * H), i_spB = lay.add(sizeof(cpx<double>) * h2 * H);
@@ -104,8 +104,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
         D.v.mstride = D.v.w;
         D.msd = (double *)at(k ? i_mB : i_mA);
         D.v.msd = D.msd;
-        D.S = (double *)at(k ? i_SB : i_SA);
-        D.Q = (double *)at(k ? i_QB : i_QA);
+        D.SQ = (double2 *)at(k ? i_SB : i_SA);
         D.Z = (int32_t *)at(k ? i_ZB : i_ZA);
         D.part = (double *)at(k ? i_pB : i_pA);
         D.bandp = (double *)at(k ? i_bB : i_bA);
@@ -114,11 +113,9 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     }
     P.A = T[0].v;
     P.B = T[1].v;
-    P.SA = T[0].S;
-    P.QA = T[0].Q;
+    P.SQA = T[0].SQ;
     P.ZA = T[0].Z;
-    P.SB = T[1].S;
-    P.QB = T[1].Q;
+    P.SQB = T[1].SQ;
     P.ZB = T[1].Z;
     P.specA = at(i_spA);
     P.specB = at(i_spB);

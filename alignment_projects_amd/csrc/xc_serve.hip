@@ -150,11 +150,9 @@ int make_pair_dev(xc_server *S, const xc_pair &q, PairDev &D)
     narrow_rows(D);
     D.A = q.a->dev.v;
     D.B = q.b->dev.v;
-    D.SA = q.a->dev.S;
-    D.QA = q.a->dev.Q;
+    D.SQA = q.a->dev.SQ;
     D.ZA = q.a->dev.Z;
-    D.SB = q.b->dev.S;
-    D.QB = q.b->dev.Q;
+    D.SQB = q.b->dev.SQ;
     D.ZB = q.b->dev.Z;
     (void)S;
     return XC_OK;

@@ -15,7 +15,7 @@ from . import abi
 from .abi import XcBucketInfo, XcCntRule, XcPair, XcPoint, XcResult, XcRgnRule, XcServeCfg, XcServeStats
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libxcorr.so")
+LIB_PATH = os.environ.get("XCORR_LIB") or os.path.join(_HERE, "lib", "libxcorr.so")  # XCORR_LIB: another build
 
 _dp = C.POINTER(C.c_double)
 _pp = C.POINTER(XcPoint)
