@@ -120,6 +120,20 @@ def stage_roofline(plan, iters, ms_step):
     return stages, roof
 
 
+def measured_traffic(name, kernel):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 counters on this workload (the summary
+    committed under profiles/, which gives the command and the scaling), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
+            for ln in f:
+                p = ln.split()
+                if len(p) == 3 and p[0] == "traffic_bytes_per_launch" and p[1] == kernel:
+                    return float(p[2])
+    except OSError:
+        pass
+    return None
+
+
 def accuracy(res, ref):
     """res: XcResult list; ref: list of (status, r, dx, dy, XcResult) for the same correlations."""
     agree, sub, rerr = 0, [], []
@@ -199,6 +213,8 @@ def run_strips(args, sync):
         log(f"PCIe measurement skipped: {e}")
     stages, roofline = stage_roofline(plan, args.stage_iters, ms_step)
     log("stages:", json.dumps(stages))
+    roofline["traffic"] = measured_traffic("traffic_strips_2048x256_256pairs.md", roofline["kernel"]) \
+        if (W, H, NP) == (2048, 256, 256) else None
 
     # ---- plain FP64 GPU path on the same pairs (one pair per call) ----
     nplain = min(args.plain_sample, NP)
