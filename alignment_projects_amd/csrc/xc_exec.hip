@@ -75,22 +75,38 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     bytes[ST_FIN] = sizeof(RowRec) * rows;
 }
 
-template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles, const std::vector<SatGroup> &groups);
+template <typename T>
+static int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups);
 
 template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, int s1)
 {
     hipStream_t s = ctx->stream;
     int st;
-    if (s0 <= ST_PREP && ST_PREP < s1 && ex.ntiles > 0) {
+    const bool prep = s0 <= ST_PREP && ST_PREP < s1;
+    // Only the c2r evaluation and the finalize read the tables, so when the
+    // whole pipeline is enqueued the sweeps (barrier-bound, one row at a time)
+    // run on the side stream while the row and column transforms run here.
+    const bool fork = prep && ST_C2R < s1 && ctx->side && (!ex.sat_groups.empty() || !ex.dsat_groups.empty());
+    if (prep && ex.ntiles > 0) {
         hipLaunchKernelGGL(k_tp_stats, dim3(ex.ntiles * TP_PARTS), dim3(256), 0, s, ex.d_tiles);
         hipLaunchKernelGGL(k_tp_stats_fin, dim3((ex.ntiles + 63) / 64), dim3(64), 0, s, ex.d_tiles, ex.ntiles);
-        if ((st = launch_sats<T>(ctx, ex.d_tiles, ex.sat_groups))) return st;
         XC_HIP(ctx, hipGetLastError());
     }
-    if (s0 <= ST_PREP && ST_PREP < s1 && ex.n_rot_work > 0) {
+    if (prep && ex.n_rot_work > 0) {
         hipLaunchKernelGGL(k_rot_splat<T>, dim3(ex.n_rot_work), dim3(256), 0, s, ex.d_rot, ex.d_rot_work);
-        if ((st = launch_sats<T>(ctx, ex.d_dtiles, ex.dsat_groups))) return st;
         XC_HIP(ctx, hipGetLastError());
+    }
+    if (prep) {
+        hipStream_t ss = s;
+        if (fork) {
+            XC_HIP(ctx, hipEventRecord(ctx->fork_ev, s));
+            XC_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+            ss = ctx->side;
+        }
+        if (ex.ntiles > 0 && (st = launch_sats<T>(ctx, ss, ex.d_tiles, ex.sat_groups))) return st;
+        if (ex.n_rot_work > 0 && (st = launch_sats<T>(ctx, ss, ex.d_dtiles, ex.dsat_groups))) return st;
+        XC_HIP(ctx, hipGetLastError());
+        if (fork) XC_HIP(ctx, hipEventRecord(ctx->join_ev, ss));
     }
     // stage-major: a row spectrum shared by pairs of several buckets is
     // produced before any bucket's column pass reads it
@@ -100,6 +116,7 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     if (s0 <= ST_COL && ST_COL < s1)
         for (const Bucket &b : ex.buckets)
             if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.col_c, b.col_xcd, b.n_col, ex.d_pairs, b.d_col))) return st;
+    if (fork) XC_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
     if (s0 <= ST_C2R && ST_C2R < s1)
         for (const Bucket &b : ex.buckets)
             if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r))) return st;
@@ -109,9 +126,9 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     return XC_OK;
 }
 
-template <typename T> static int launch_sats(xc_ctx *ctx, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
+template <typename T>
+static int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
 {
-    hipStream_t s = ctx->stream;
     for (const auto &g : groups) {
         {
             const int key = g.cpt * 10000 + g.threads;

@@ -48,6 +48,8 @@ struct xc_col_tune {
 struct xc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;  // second stream: table sweeps run here beside the transforms
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::mutex mu;
     std::string err;
     xc::cpx<double> *tw64 = nullptr;

@@ -233,6 +233,13 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
         delete ctx;
         return XC_E_DEVICE;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        xc_ctx_destroy(ctx);
+        return XC_E_DEVICE;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         ctx->num_cu = prop.multiProcessorCount;
@@ -287,6 +294,10 @@ extern "C" int xc_ctx_destroy(xc_ctx *ctx)
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->tw64) (void)hipFree(ctx->tw64);
     if (ctx->tw32) (void)hipFree(ctx->tw32);
+    if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+    if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return XC_OK;
