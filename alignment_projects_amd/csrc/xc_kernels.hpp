@@ -456,6 +456,29 @@ __device__ __forceinline__ double wave_scan_d(double v)
 
 constexpr int SAT_MAX_THREADS = 1024;
 
+// Sum over lanes 0..15 of the wavefront, returned to every lane (DPP
+// inclusive scan within the first 16-lane row, then a read of lane 15).
+__device__ __forceinline__ double row16_sum_d(double v)
+{
+    v += dpp_d<0x111, 0xf>(v);
+    v += dpp_d<0x112, 0xf>(v);
+    v += dpp_d<0x114, 0xf>(v);
+    v += dpp_d<0x118, 0xf>(v);
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 15);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 15);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ int row16_sum_i(int v)
+{
+    v += dpp_i<0x111, 0xf>(v);
+    v += dpp_i<0x112, 0xf>(v);
+    v += dpp_i<0x114, 0xf>(v);
+    v += dpp_i<0x118, 0xf>(v);
+    return __builtin_amdgcn_readlane(v, 15);
+}
+
 // Typed pixel read for the table sweep (no per-pixel dtype switch).
 template <typename InT>
 __device__ __forceinline__ double sweep_val(const ImgView &v, int x, int y)
@@ -559,6 +582,8 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
         for (int a = 0; a < LA; ++a) nxt[a][j] = (x < w && y0 + a < y1) ? sweep_val<InT>(D.v, x, y0 + a) : 0.0;
     }
     const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);  // 16-byte aligned column pairs
+    const bool wz = D.Z != nullptr;                        // count table wanted (uniform)
+    static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
     for (int y = y0; y < y1; ++y) {
         double ps[CPT], pq[CPT];
         int pz[CPT];
@@ -585,7 +610,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
         const double ts = ps[CPT - 1], tq = pq[CPT - 1];
         const int tz = pz[CPT - 1];
         const double is = wave_scan_d(ts), iq = wave_scan_d(tq);
-        const int iz = wave_scan_i(tz);
+        const int iz = wz ? wave_scan_i(tz) : 0;
         const int par = (y & 1) * NWV;
         if (lane == 63) {
             xs[par + wv] = is;
@@ -595,13 +620,14 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
         __syncthreads();
         double bs = is - ts, bq = iq - tq;
         int bz = iz - tz;
-#pragma unroll
-        for (int k = 0; k < NWV; ++k)
-            if (k < wv) {
-                bs += xs[par + k];
-                bq += xq[par + k];
-                bz += xz[par + k];
-            }
+        if constexpr (NWV > 1) {
+            // totals of the waves to the left: lane k < wv holds wave k's, summed over the first 16 lanes
+            const bool left = lane < wv;  // wv < NWV, so k stays inside the row's NWV slots
+            const int k = par + (left ? lane : 0);
+            bs += row16_sum_d(left ? xs[k] : 0.0);
+            bq += row16_sum_d(left ? xq[k] : 0.0);
+            if (wz) bz += row16_sum_i(left ? xz[k] : 0);
+        }
         const long long row = (long long)y * w;
         if (pair_st) {
 #pragma unroll
