@@ -1132,6 +1132,24 @@ __device__ __forceinline__ double2 band_sum2(const double2 *__restrict__ top, co
     return v;
 }
 
+#pragma clang fp contract(off)
+// GetCorr's value from the overlap size, the raw correlation sum and the two
+// images' band sums (S, S^2): FP64, except that the FP32 (batched) form takes
+// the final num / sqrt(d) in FP32.
+template <typename T>
+__device__ __forceinline__ double ncc_r(int olw, int olh, double invN, double raw, double s1, double q1, double s2,
+                                        double q2)
+{
+    const double n = (double)(olw * olh);
+    const double num = n * raw * invN - s1 * s2;
+    const double d1 = n * q1 - s1 * s1;
+    const double d2 = n * q2 - s2 * s2;
+    const double d = d1 * d2;
+    if constexpr (sizeof(T) == 8) return d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d);
+    else return d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d));
+}
+#pragma clang fp contract(on)
+
 // NCC value of lag x of a row (CLinCorr::CheckDensity/GetCorr, Correlation.cpp:697-754);
 // the lag is inside the row's legal interval.  *cand: passes exact_w, the table and
 // LegalCnt; *odd: |r| > 1.0001.
@@ -1160,18 +1178,108 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
         if (!((long long)c1 > g.k1 && (long long)c2 > g.k2)) c = false;
     }
     *cand = c;
-    const double n = (double)(olw * q.olh);
-    const double num = n * raw * q.invN - s1 * s2;
-    const double d1 = n * q1 - s1 * s1;
-    const double d2 = n * q2 - s2 * s2;
-    const double d = d1 * d2;
-    double r;
-    if constexpr (sizeof(T) == 8) r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
-    else r = (d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d)));
+    const double r = ncc_r<T>(olw, q.olh, q.invN, raw, s1, q1, s2, q2);
     *odd = c && fabs(r) > 1.0001;
     return c ? r : 0.0;
 }
+
+// The NCC scan of one lag row when every lag of the legal interval is a
+// candidate (no exact width, table or count rule): ncc_lag's arithmetic
+// with the row's table rows held in registers, the edge cases of band_sum2
+// resolved per segment, and no per-lag rule tests.  AB / BB: the band has a
+// row below it in A / B's table.
+// Table element i read as a global-memory load (the table rows arrive as
+// generic pointers).
+typedef double xc_dv2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ld_sq(const double2 *p, int i)
+{
+    typedef __attribute__((address_space(1))) const xc_dv2 gdv2;
+    const xc_dv2 v = ((gdv2 *)(const void *)p)[i];
+    return make_double2(v.x, v.y);
+}
+
+template <typename T, int TPF, bool AB, bool BB>
+__device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
+                                              long long &bestk, int &flags)
+{
+    const double2 *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
+    const int w1 = P.w1, w2 = P.w2, olh = q.olh;
+    const double invN = q.invN;
+    // x >= 0: A's band starts at column 0, B's at x
+    for (int x = (q.x0 > 0 ? q.x0 : 0) + t; x <= q.x1; x += TPF) {
+        const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x;
+        const int BL1 = x > 0 ? x - 1 : 0;  // L - 1 of B's band (unused at x = 0)
+        double2 a = ld_sq(At, AR);
+        if constexpr (AB) {
+            const double2 u = ld_sq(Ab, AR);
+            a.x -= u.x;
+            a.y -= u.y;
+        }
+        // x = 0 has no left term: subtract / add +0.0 instead (exact), so that
+        // every load is unconditional and all of them are in flight together
+        const bool l = x > 0;
+        double2 b = ld_sq(Bt, BR);
+        const double2 bl = ld_sq(Bt, BL1);
+        b.x -= l ? bl.x : 0.0;
+        b.y -= l ? bl.y : 0.0;
+        if constexpr (BB) {
+            const double2 u = ld_sq(Bb, BR), z = ld_sq(Bb, BL1);
+            b.x -= u.x;
+            b.y -= u.y;
+            b.x += l ? z.x : 0.0;
+            b.y += l ? z.y : 0.0;
+        }
+        const double r = ncc_r<T>(BR - x + 1, olh, invN, (double)R[x], a.x, a.y, b.x, b.y);
+        flags |= fabs(r) > 1.0001 ? 3 : 1;
+        if (r > bestv) {
+            bestv = r;
+            bestk = x;
+        }
+    }
+    // x < 0: A's band starts at column -x, B's at 0
+    const int xe = q.x1 < -1 ? q.x1 : -1;
+    for (int x = q.x0 + t; x <= xe; x += TPF) {
+        const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x, AL1 = -x - 1;
+        double2 a = ld_sq(At, AR);
+        const double2 al = ld_sq(At, AL1);
+        a.x -= al.x;
+        a.y -= al.y;
+        if constexpr (AB) {
+            const double2 u = ld_sq(Ab, AR), z = ld_sq(Ab, AL1);
+            a.x -= u.x;
+            a.y -= u.y;
+            a.x += z.x;
+            a.y += z.y;
+        }
+        double2 b = ld_sq(Bt, BR);
+        if constexpr (BB) {
+            const double2 u = ld_sq(Bb, BR);
+            b.x -= u.x;
+            b.y -= u.y;
+        }
+        const int ix = NX + x;
+        const double r = ncc_r<T>(BR + 1, olh, invN, (double)R[ix], a.x, a.y, b.x, b.y);
+        flags |= fabs(r) > 1.0001 ? 3 : 1;
+        if (r > bestv) {
+            bestv = r;
+            bestk = ix;
+        }
+    }
+}
 #pragma clang fp contract(on)
+
+template <typename T, int TPF>
+__device__ __forceinline__ void ncc_scan_fast(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
+                                              long long &bestk, int &flags)
+{
+    if (q.sqAb) {
+        if (q.sqBb) ncc_scan_rows<T, TPF, true, true>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TPF, true, false>(P, q, R, t, NX, bestv, bestk, flags);
+    } else {
+        if (q.sqBb) ncc_scan_rows<T, TPF, false, true>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TPF, false, false>(P, q, R, t, NX, bestv, bestk, flags);
+    }
+}
 
 // CorrPatches' R-image value at lag (x, y): the NCC of a candidate lag, 0
 // for any other (outside the row's legal interval or failing a per-lag
@@ -1248,7 +1356,9 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
         // keys ascend, so a strict > keeps the first of equal values
         const NccRow q = ncc_row(P, y);
         const int segs[2][2] = {{q.x0 > 0 ? q.x0 : 0, q.x1}, {q.x0, q.x1 < -1 ? q.x1 : -1}};
-        for (int sgi = 0; sgi < 2; ++sgi) {
+        const bool every = P.rule.exact_w <= 0 && !P.rule.table && !P.rule.cnt_enabled;
+        if (every) ncc_scan_fast<T, TPF>(P, q, R, t, NX, bestv, bestk, flags);
+        else for (int sgi = 0; sgi < 2; ++sgi) {
             for (int x = segs[sgi][0] + t; x <= segs[sgi][1]; x += TPF) {
                 const int ix = x >= 0 ? x : NX + x;
                 bool cand, odd;
