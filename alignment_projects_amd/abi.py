@@ -31,6 +31,7 @@ STATUS_NAMES = {
 XC_MODE_NCC = 0
 XC_MODE_R = 1
 XC_MODE_S = 2
+XC_MODE_F = 3
 
 XC_U8, XC_U16, XC_F32, XC_F64 = 0, 1, 2, 3
 XC_TILE_NORMALIZE = 1

@@ -280,6 +280,15 @@ static double corr_images(int mode, FILE *flog, int verbose, double &dx, double 
     return r;
 }
 
+double CorrImagesF(FILE *flog, int verbose, double &dx, double &dy, const vector<Point> &ip1,
+                   const vector<double> &iv1, const vector<Point> &ip2, const vector<double> &iv2, EvalType LegalRgn,
+                   void *arglr, EvalType LegalCnt, void *arglc, double mincor, double nbmaxht, int Ox, int Oy, int Rx,
+                   int Ry, vector<CD> &fft2)
+{
+    return corr_images(XC_MODE_F, flog, verbose, dx, dy, ip1, iv1, ip2, iv2, LegalRgn, arglr, LegalCnt, arglc, mincor,
+                       nbmaxht, Ox, Oy, Rx, Ry, fft2);
+}
+
 double CorrImagesR(FILE *flog, int verbose, double &dx, double &dy, const vector<Point> &ip1,
                    const vector<double> &iv1, const vector<Point> &ip2, const vector<double> &iv2, EvalType LegalRgn,
                    void *arglr, EvalType LegalCnt, void *arglc, double mincor, double nbmaxht, int Ox, int Oy, int Rx,

@@ -1380,6 +1380,14 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
             if (win) {
                 ssum += val;
                 ssq += val * val;
+                if (P.rimg) {
+                    // the centred R image and its candidate flags (CorrImagesF)
+                    bool dead;
+                    const long long ir = (long long)(lag_x_of_raster(P, ix, &dead) + P.nnegx) +
+                                         (long long)P.wR * (y + P.nnegy);
+                    P.rimg[ir] = val;
+                    P.cimg[ir] = cand ? 1 : 0;
+                }
             }
             if (cand) {
                 flags |= 1;

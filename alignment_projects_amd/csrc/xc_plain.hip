@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "xc_exec.hpp"
+#include "xc_fmode.hpp"
 #include "xc_host.hpp"
 #include "xc_kernels.hpp"
 
@@ -15,9 +16,12 @@ using namespace xc;
 namespace {
 
 // Run one FP64 correlation of point lists (A = ip1/iv1, B = ip2/iv2).
+// fmode (with mode PM_R): CorrImagesF -- the R image is kept and the peak
+// is chosen by MakeF / OrderF / FPeak with nbmaxht.
 int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int n1, const xc_point *ip2,
               const double *iv2, int n2, int Ox, int Oy, int radius, int Rx, int Ry, double mincor,
-              const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out)
+              const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out, bool fmode = false,
+              double nbmaxht = 0.0)
 {
     if (!ctx) return XC_E_ARG;
     if (!ip1 || !iv1 || !ip2 || !iv2 || n1 <= 0 || n2 <= 0)
@@ -87,6 +91,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     size_t i_wr = lay.add(sizeof(int4) * nr2c), i_wc = lay.add(sizeof(int2) * ncol), i_wc2 = lay.add(sizeof(int2) * nc2r);
     size_t i_slist = lay.add(sizeof(int) * 2);
     size_t i_tbl = lay.add(tbl ? tbl : 1);
+    const long long nR = (long long)P.wR * P.hR;
+    size_t i_f = lay.add(fmode ? fmode_bytes(nR) : 1);
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
     char *base = (char *)ctx->arena.p;
@@ -124,6 +130,13 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     P.out = at(i_res);
     P.rule = make_rule(rgn, cnt, (const uint8_t *)at(i_tbl));
     P.need_cnt = P.rule.cnt_enabled;
+    FWork fw;
+    memset(&fw, 0, sizeof fw);
+    if (fmode) {
+        fmode_carve(at(i_f), nR, fw);
+        P.rimg = fw.R;
+        P.cimg = fw.cand;
+    }
     narrow_rows(P);
     std::vector<int> c2r_starts;
     live_row_groups(P, G, c2r_starts);
@@ -199,6 +212,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     b.n_c2r = (int)c2r_starts.size();
     ex.buckets.push_back(b);
     if ((st = enqueue(ctx, ex))) return st;
+    if (fmode && (st = fmode_select(ctx, s, ex.d_pairs, P.wR, P.hR, nbmaxht, fw))) return st;
     XC_HIP(ctx, hipMemcpyAsync(hres, at(i_res), sizeof(xc_result), hipMemcpyDeviceToHost, s));
     XC_HIP(ctx, hipStreamSynchronize(s));
     *out = *hres;
@@ -231,12 +245,12 @@ extern "C" int xc_corr_images_f64(xc_ctx *ctx, int mode, const xc_point *ip1, co
                                   const xc_cnt_rule *cnt, double mincor, double nbmaxht, int Ox, int Oy, int Rx,
                                   int Ry, double *dx, double *dy, double *r, xc_result *res)
 {
-    (void)nbmaxht;
-    if (mode != XC_MODE_R && mode != XC_MODE_S) return set_err(ctx, XC_E_ARG, "mode must be XC_MODE_R or XC_MODE_S");
+    if (mode != XC_MODE_R && mode != XC_MODE_S && mode != XC_MODE_F)
+        return set_err(ctx, XC_E_ARG, "mode must be XC_MODE_R, XC_MODE_S or XC_MODE_F");
     xc_result R;
     memset(&R, 0, sizeof R);
-    int st = run_plain(ctx, mode == XC_MODE_R ? PM_R : PM_S, ip1, iv1, n1, ip2, iv2, n2, Ox, Oy, 0, Rx, Ry, mincor,
-                       rgn, cnt, &R);
+    int st = run_plain(ctx, mode == XC_MODE_S ? PM_S : PM_R, ip1, iv1, n1, ip2, iv2, n2, Ox, Oy, 0, Rx, Ry, mincor,
+                       rgn, cnt, &R, mode == XC_MODE_F, nbmaxht);
     if (st == XC_OK) {
         if (dx) *dx = R.dx;
         if (dy) *dy = R.dy;

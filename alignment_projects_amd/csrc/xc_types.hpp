@@ -107,6 +107,8 @@ struct PairDev {
     void *lag;                        // cpx<T> [hR][Nx/2+1] lag rows
     RowRec *rows;                     // [hR]
     void *out;                        // xc_result (device)
+    double *rimg;                     // R mode: centred wR x hR R image written by the c2r (or null)
+    uint8_t *cimg;                    //         and its candidate flags (A, oval, mincor)
 };
 
 }  // namespace xc

@@ -147,7 +147,8 @@ class Context:
 
     def corr_images(self, mode, ip1, iv1, ip2, iv2, rgn=None, cnt=None, mincor=0.0, nbmaxht=0.0,
                     Ox=0, Oy=0, Rx=-1, Ry=-1):
-        """CorrImagesR (mode=XC_MODE_R) / CorrImagesS (mode=XC_MODE_S). Returns (status, r, dx, dy, XcResult)."""
+        """CorrImagesR / CorrImagesS / CorrImagesF (mode XC_MODE_R / XC_MODE_S / XC_MODE_F; nbmaxht is F mode's
+        neighbour height limit). Returns (status, r, dx, dy, XcResult)."""
         p1, pp1 = _pts(ip1)
         v1, pv1 = _vals(iv1)
         p2, pp2 = _pts(ip2)

@@ -67,7 +67,11 @@ double CorrPatches(FILE *flog, int verbose, double &dx, double &dy, const vector
 double CorrPatchToImage(double &dx, double &dy, const vector<Point> &ip1, const vector<double> &iv1,
                         const vector<double> &i2, int Ox, int Oy, int radius, bool bFilter);
 
-/* CorrImagesR / CorrImagesS (Correlation.cpp:3144-3290). */
+/* CorrImagesF / CorrImagesR / CorrImagesS (Correlation.cpp:3066-3122, 3144-3290). */
+double CorrImagesF(FILE *flog, int verbose, double &dx, double &dy, const vector<Point> &ip1,
+                   const vector<double> &iv1, const vector<Point> &ip2, const vector<double> &iv2, EvalType LegalRgn,
+                   void *arglr, EvalType LegalCnt, void *arglc, double mincor, double nbmaxht, int Ox, int Oy, int Rx,
+                   int Ry, vector<CD> &fft2);
 double CorrImagesR(FILE *flog, int verbose, double &dx, double &dy, const vector<Point> &ip1,
                    const vector<double> &iv1, const vector<Point> &ip2, const vector<double> &iv2, EvalType LegalRgn,
                    void *arglr, EvalType LegalCnt, void *arglc, double mincor, double nbmaxht, int Ox, int Oy, int Rx,

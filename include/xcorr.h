@@ -118,7 +118,8 @@ typedef struct xc_result {
 enum xc_mode {
     XC_MODE_NCC = 0,    /* CorrPatches: normalised cross-correlation, whole-grid wrapped scan */
     XC_MODE_R   = 1,    /* CorrImagesR: centred R image, A-mask, straight max */
-    XC_MODE_S   = 2     /* CorrImagesS: whitened cross-power S image, score 10(r-mn)/(sd sqrt(n)) */
+    XC_MODE_S   = 2,    /* CorrImagesS: whitened cross-power S image, score 10(r-mn)/(sd sqrt(n)) */
+    XC_MODE_F   = 3     /* CorrImagesF: R image, LoG+I filtered F image, best well-isolated F peak */
 };
 
 /* ------------------------------------------------------------------ */
@@ -183,10 +184,12 @@ int xc_corr_patch_to_image_f64(xc_ctx *ctx,
                                double *dx, double *dy, double *r, xc_result *res);
 
 /*
- * Replaces CorrImagesR / CorrImagesS (Correlation.cpp:3144-3290) with
- * mode XC_MODE_R / XC_MODE_S.  mincor and nbmaxht keep their reference
- * meaning (nbmaxht is a dummy slot in R/S modes).  Rx<=0 or Ry<=0 disables
- * the oval search mask.  On XC_TOO_SMALL / XC_NO_PEAK, dx=dy=0 and r=0.
+ * Replaces CorrImagesF / CorrImagesR / CorrImagesS (Correlation.cpp:
+ * 3066-3122, 3144-3290) with mode XC_MODE_F / XC_MODE_R / XC_MODE_S.
+ * mincor and nbmaxht keep their reference meaning (nbmaxht is used by
+ * F mode only: a candidate F peak is rejected when its guard band holds a
+ * pixel above nbmaxht times its height).  Rx<=0 or Ry<=0 disables the oval
+ * search mask.  On XC_TOO_SMALL / XC_NO_PEAK, dx=dy=0 and r=0.
  */
 int xc_corr_images_f64(xc_ctx *ctx, int mode,
                        const xc_point *ip1, const double *iv1, int n1,

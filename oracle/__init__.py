@@ -65,6 +65,8 @@ def lib():
         L.xo_set_fft_provider.argtypes = [C.c_int, C.c_char_p]
         L.xo_fft_provider_name.restype = C.c_char_p
         L.xo_rgn_ok.argtypes = [C.POINTER(XcRgnRule), C.c_int, C.c_int]
+        L.xo_convolve.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_fpeak.argtypes = [_dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_double, _ip, _ip]
         _lib = L
     return _lib
 
@@ -111,7 +113,8 @@ def corr_patches(ip1, iv1, ip2, iv2, Ox=0, Oy=0, radius=4096, rgn=None, cnt=None
 
 def corr_images(mode, ip1, iv1, ip2, iv2, rgn=None, cnt=None, mincor=0.0, nbmaxht=0.0,
                 Ox=0, Oy=0, Rx=-1, Ry=-1, want_images=False):
-    """CorrImagesR/S (0_GEN/Correlation.cpp:3144-3290). Returns (status, r, dx, dy, XcResult[, R, S, A])."""
+    """CorrImagesR/S/F (0_GEN/Correlation.cpp:3066-3290). Returns (status, r, dx, dy, XcResult[, R, S, A]);
+    in F mode the S slot holds the F image and A has MakeF's guard band cleared."""
     p1, pp1 = _pts(ip1)
     v1, pv1 = _vals(iv1)
     p2, pp2 = _pts(ip2)
@@ -136,6 +139,31 @@ def corr_images(mode, ip1, iv1, ip2, iv2, rgn=None, cnt=None, mincor=0.0, nbmaxh
     if want_images:
         out = out + (R.reshape(hR, wR), S.reshape(hR, wR), A.reshape(hR, wR))
     return out
+
+
+def convolve(src, K, sym=True, prenorm=True):
+    """Convolve (0_GEN/Correlation.cpp:371-457) of src (h x w) with the odd-sized kernel K (hk x wk)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    hs, ws = src.shape
+    hk, wk = K.shape
+    dst = np.zeros_like(src)
+    lib().xo_convolve(dst.ctypes.data_as(_dp), src.ctypes.data_as(_dp), ws, hs, K.ctypes.data_as(_dp), wk, hk,
+                      int(bool(sym)), int(bool(prenorm)))
+    return dst
+
+
+def fpeak(F, R, order, nbmaxht):
+    """CCorImg::FPeak (0_GEN/Correlation.cpp:2580-2767) over the candidate indices `order`.
+    Returns (rx, ry) or None."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    hR, wR = F.shape
+    o = np.ascontiguousarray(order, dtype=np.int64)
+    rx, ry = C.c_int(), C.c_int()
+    ok = lib().xo_fpeak(F.ctypes.data_as(_dp), R.ctypes.data_as(_dp), wR, hR, o.ctypes.data_as(C.POINTER(C.c_long)),
+                        o.size, nbmaxht, C.byref(rx), C.byref(ry))
+    return (rx.value, ry.value) if ok else None
 
 
 def corr_patch_to_image(ip1, iv1, i2, Ox=0, Oy=0, radius=4096, bfilter=False):

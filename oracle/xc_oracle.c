@@ -734,13 +734,121 @@ static void parab_peak(double *xpk, double *ypk, int d, const double *I, int w, 
     }
 }
 
-/* CCorImg::SetDims / MakeRandA / MakeSandRandA / MaskA / OrderF /
- * SimpleMax / ReturnR and CorrImagesR / CorrImagesS
- * (Correlation.cpp:1949-2370, 2509-2569, 2939-3043, 3144-3290).
- * mode: XC_MODE_R or XC_MODE_S.  Optional outputs (each nR = wR*hR):
- * R_out, S_out (S mode), A_out.  On exact ties in OrderF's sort the oracle
- * takes the lowest centred-image index (std::sort's order on exact ties is
- * unspecified). */
+/* Convolve (Correlation.cpp:371-457): dst (ws x hs) = src convolved with
+ * the wk x hk kernel K through 2-D transforms of size FFTSize(ws, wk/(1+sym))
+ * x FFTSize(hs, hk/(1+sym)), the kernel centred on the origin with wrap,
+ * divided by its sum when prenorm (a zero sum leaves it as is), product of
+ * the two r2c transforms, unnormalised c2r, then / Nxy. */
+int xo_convolve(double *dst, const double *src, int ws, int hs, const double *K, int wk, int hk, int sym, int prenorm)
+{
+    int Nx = xo_fft_size(ws, wk / (1 + sym)), Ny = xo_fft_size(hs, hk / (1 + sym));
+    long Nxy = (long)Nx * Ny, M = (long)Ny * (Nx / 2 + 1);
+    double *KK = (double *)calloc(Nxy, sizeof(double));
+    double *SS = (double *)calloc(Nxy, sizeof(double));
+    cd *kf = (cd *)malloc(sizeof(cd) * M), *sf = (cd *)malloc(sizeof(cd) * M);
+    double nrm = 1.0;
+    if (prenorm) {
+        nrm = 0.0;
+        for (int i = 0; i < wk * hk; ++i) nrm += K[i];
+        if (!nrm) nrm = 1.0;
+    }
+    int w2 = wk / 2, h2 = hk / 2;
+    for (int i = -h2; i <= h2; ++i) {
+        int y = (i >= 0 ? i : Ny + i);
+        for (int j = -w2; j <= w2; ++j) {
+            int x = (j >= 0 ? j : Nx + j);
+            KK[x + (long)Nx * y] = K[j + w2 + wk * (i + h2)] / nrm;
+        }
+    }
+    xo_fft2d_r2c(KK, Nx, Ny, (double *)kf);
+    for (int y = 0; y < hs; ++y) memcpy(SS + (long)Nx * y, src + (long)ws * y, sizeof(double) * ws);
+    xo_fft2d_r2c(SS, Nx, Ny, (double *)sf);
+    for (long i = 0; i < M; ++i) {  /* std::complex<double> operator*= */
+        double re = sf[i].re * kf[i].re - sf[i].im * kf[i].im;
+        double im = sf[i].re * kf[i].im + sf[i].im * kf[i].re;
+        sf[i].re = re;
+        sf[i].im = im;
+    }
+    xo_ift2d_c2r((const double *)sf, Nx, Ny, SS);
+    for (int y = 0; y < hs; ++y)
+        for (int x = 0; x < ws; ++x) dst[x + (long)ws * y] = SS[x + (long)Nx * y] / Nxy;
+    free(KK); free(SS); free(kf); free(sf);
+    return 0;
+}
+
+/* CCorImg::FPeak (Correlation.cpp:2580-2767): walk the candidates `order`
+ * (highest F first); skip one inside the core of an earlier candidate;
+ * inner radius ri = the least of the four axis distances to a pixel with
+ * F <= fpk/2 (none in some direction: reject); reject ri <= 1 with
+ * R < 0.1 and an outer band ro = 3 ri that leaves the image; mask the core
+ * (half-width max(ri/3, 1)); accept when no pixel of the ro box outside
+ * the ri box exceeds nbmaxht * fpk.  Returns 1 with (rx, ry), else 0. */
+int xo_fpeak(const double *F, const double *R, int wR, int hR, const long *order, long n, double nbmaxht, int *rxo,
+             int *ryo)
+{
+    uint8_t *mask = (uint8_t *)calloc((size_t)wR * hR, 1);
+    int ok = 0;
+    for (long i = 0; i < n && !ok; ++i) {
+        long k = order[i];
+        if (mask[k]) continue;
+        double fpk = F[k], tol = nbmaxht * fpk;
+        int ry = (int)(k / wR), rx = (int)(k - (long)wR * ry);
+        int ri = 0x7fffffff, hit, y, x;
+        hit = 0;
+        for (y = ry - 1; y >= 0; --y) if (F[rx + (long)wR * y] <= 0.5 * fpk) { if (ry - y < ri) ri = ry - y; hit = 1; break; }
+        if (!hit) continue;
+        hit = 0;
+        for (y = ry + 1; y < hR; ++y) if (F[rx + (long)wR * y] <= 0.5 * fpk) { if (y - ry < ri) ri = y - ry; hit = 1; break; }
+        if (!hit) continue;
+        hit = 0;
+        for (x = rx - 1; x >= 0; --x) if (F[x + (long)wR * ry] <= 0.5 * fpk) { if (rx - x < ri) ri = rx - x; hit = 1; break; }
+        if (!hit) continue;
+        hit = 0;
+        for (x = rx + 1; x < wR; ++x) if (F[x + (long)wR * ry] <= 0.5 * fpk) { if (x - rx < ri) ri = x - rx; hit = 1; break; }
+        if (!hit) continue;
+        if (ri <= 1 && R[k] < 0.1) continue;                /* Micropk */
+        int ro = 3 * ri;
+        if (ry - ro < 0 || ry + ro >= hR || rx - ro < 0 || rx + ro >= wR) continue;  /* Ro_edge */
+        int rm = ri / 3;
+        if (rm < 1) rm = 1;
+        for (y = ry - rm; y <= ry + rm; ++y)
+            for (x = rx - rm; x <= rx + rm; ++x) mask[x + (long)wR * y] = 1;
+        int clean = 1;
+        for (y = ry - ro; y <= ry + ro && clean; ++y)
+            for (x = rx - ro; x <= rx + ro; ++x) {
+                if (y >= ry - ri && y <= ry + ri && x >= rx - ri && x <= rx + ri) continue;
+                if (F[x + (long)wR * y] > tol) { clean = 0; break; }  /* Hi_neib */
+            }
+        if (clean) {
+            *rxo = rx;
+            *ryo = ry;
+            ok = 1;
+        }
+    }
+    free(mask);
+    return ok;
+}
+
+typedef struct { double f; long i; } fkey;
+
+/* OrderF's sort (Correlation.cpp:2542-2546): decreasing F; exact ties by
+ * increasing index. */
+static int fkey_cmp(const void *a, const void *b)
+{
+    const fkey *p = (const fkey *)a, *q = (const fkey *)b;
+    if (p->f > q->f) return -1;
+    if (p->f < q->f) return 1;
+    return p->i < q->i ? -1 : (p->i > q->i);
+}
+
+/* CCorImg::SetDims / MakeRandA / MakeSandRandA / MaskA / MakeF / OrderF /
+ * FPeak / SimpleMax / ReturnR and CorrImagesF / CorrImagesR / CorrImagesS
+ * (Correlation.cpp:1949-2370, 2412-2767, 2939-3122, 3144-3290).
+ * mode: XC_MODE_R, XC_MODE_S or XC_MODE_F.  Optional outputs (each nR =
+ * wR*hR): R_out, S_out (the S image in S mode, the F image in F mode),
+ * A_out (F mode: with MakeF's guard band cleared).  On exact ties in
+ * OrderF's sort the oracle takes the lowest centred-image index
+ * (std::sort's order on exact ties is unspecified). */
 int xo_corr_images(int mode,
                    const xc_point *ip1, const double *iv1, int n1,
                    const xc_point *ip2, const double *iv2, int n2,
@@ -750,9 +858,8 @@ int xo_corr_images(int mode,
                    double *dx, double *dy, double *r_out, xc_result *res,
                    double *R_out, double *S_out, uint8_t *A_out, int cap)
 {
-    (void)nbmaxht;
     if (n1 <= 0 || n2 <= 0) return XC_E_ARG;
-    if (mode != XC_MODE_R && mode != XC_MODE_S) return XC_E_ARG;
+    if (mode != XC_MODE_R && mode != XC_MODE_S && mode != XC_MODE_F) return XC_E_ARG;
     ibox B1, B2;
     bbox_from_points(&B1, ip1, n1);
     bbox_from_points(&B2, ip2, n2);
@@ -841,13 +948,55 @@ int xo_corr_images(int mode,
         }
     }
 
-    /* OrderF (:2509-2569) + SimpleMax (:2939-2948) */
     const double *F = (mode == XC_MODE_S) ? S : R;
     long best = -1;
-    for (long i = 0; i < nR; ++i) {
-        if (!A[i]) continue;
-        if (mincor > 0.0 && !(R[i] > mincor)) continue;
-        if (best < 0 || F[i] > F[best]) best = i;
+    double *Fimg = NULL;
+    if (mode == XC_MODE_F) {
+        /* MakeF (:2412-2492): LoG+I filter of R, guard band of 2 cleared in A
+         * and F, then F / max(F, 1e-7) where F > 0, else 0 */
+        static const double K[9] = {-1, -1, -1, -1, 9, -1, -1, -1, -1};
+        const int grd = 2;
+        Fimg = (double *)malloc(sizeof(double) * nR);
+        xo_convolve(Fimg, R, wR, hR, K, 3, 3, 1, 1);
+        for (long i = 0; i < (long)grd * wR; ++i) {
+            A[i] = 0; A[nR - 1 - i] = 0;
+            Fimg[i] = 0.0; Fimg[nR - 1 - i] = 0.0;
+        }
+        for (int y = grd; y < hR - grd; ++y)
+            for (int x = 0; x < grd; ++x) {
+                long i = x + (long)wR * y;
+                A[i] = 0; Fimg[i] = 0.0;
+                i = wR - 1 - x + (long)wR * y;
+                A[i] = 0; Fimg[i] = 0.0;
+            }
+        double vmax = 1e-7;
+        for (long i = 0; i < nR; ++i) if (Fimg[i] > vmax) vmax = Fimg[i];
+        for (long i = 0; i < nR; ++i) Fimg[i] = Fimg[i] > 0.0 ? Fimg[i] / vmax : 0.0;
+        /* OrderF (:2509-2569): A pixels (with R > mincor when mincor > 0) by decreasing F */
+        fkey *keys = (fkey *)malloc(sizeof(fkey) * nR);
+        long nc = 0;
+        for (long i = 0; i < nR; ++i) {
+            if (!A[i]) continue;
+            if (mincor > 0.0 && !(R[i] > mincor)) continue;
+            keys[nc].f = Fimg[i];
+            keys[nc].i = i;
+            ++nc;
+        }
+        qsort(keys, nc, sizeof(fkey), fkey_cmp);
+        long *order = (long *)malloc(sizeof(long) * (nc ? nc : 1));
+        for (long i = 0; i < nc; ++i) order[i] = keys[i].i;
+        int rx, ry;
+        if (nc > 0 && xo_fpeak(Fimg, R, wR, hR, order, nc, nbmaxht, &rx, &ry)) best = rx + (long)wR * ry;
+        free(keys);
+        free(order);
+        F = R;  /* ReturnR( dx, dy, rx, ry, R, R ) */
+    } else {
+        /* OrderF (:2509-2569) + SimpleMax (:2939-2948) */
+        for (long i = 0; i < nR; ++i) {
+            if (!A[i]) continue;
+            if (mincor > 0.0 && !(R[i] > mincor)) continue;
+            if (best < 0 || F[i] > F[best]) best = i;
+        }
     }
 
     int status = XC_OK;
@@ -879,11 +1028,12 @@ int xo_corr_images(int mode,
     if (cap >= nR) {
         if (R_out) memcpy(R_out, R, sizeof(double) * nR);
         if (S_out && S) memcpy(S_out, S, sizeof(double) * nR);
+        if (S_out && Fimg) memcpy(S_out, Fimg, sizeof(double) * nR);
         if (A_out) memcpy(A_out, A, nR);
     }
     free(i1); free(i2); free(rslt); free(wrs);
     free(s1); free(q1); free(z1); free(s2); free(q2); free(z2);
-    free(R); free(S); free(A);
+    free(R); free(S); free(A); free(Fimg);
     return status;
 }
 

@@ -184,3 +184,120 @@ def corr_images(mode, A, B, offx, offy, rgn=None, cnt=None, Ox=0, Oy=0, Rx=-1, R
         sd = F.std(ddof=1)
         score = 10.0 * (z1 - F.mean()) / (sd * np.sqrt(n))
     return abi.XC_OK, score, px + offx - nnegx, py + offy - nnegy, (rx - nnegx, ry - nnegy)
+
+
+def r_image(A, B, offx, offy, rgn=None, cnt=None, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """MakeRandA + MaskA on rasters: the centred R image (CalcR's clamp) and the A mask."""
+    h1, w1 = A.shape
+    h2, w2 = B.shape
+    nnegx, nposx, nnegy, nposy = w1 - 1, w2, h1 - 1, h2
+    wR, hR = nnegx + nposx, nnegy + nposy
+    R = np.zeros((hR, wR))
+    Am = np.zeros((hR, wR), bool)
+    for y in range(-nnegy, nposy):
+        for x in range(-nnegx, nposx):
+            sa, sb = overlap(w1, h1, w2, h2, x, y)
+            a, b = A[sa], B[sb]
+            Am[y + nnegy, x + nnegx] = rgn_ok(rgn, a.shape[1], a.shape[0]) and cnt_ok(
+                cnt, int((a != 0).sum()), int((b != 0).sum()))
+            r = ncc_direct(a, b)
+            R[y + nnegy, x + nnegx] = r if -1.0 < r < 1.0 else 0.0
+    if Rx > 0 and Ry > 0:
+        ox, oy = Ox - offx + nnegx, Oy - offy + nnegy
+        yy, xx = np.mgrid[0:hR, 0:wR]
+        inside = (np.abs(yy - oy) <= Ry) & (np.abs(xx - ox) <= Rx)
+        inside &= (xx - ox) ** 2 + (Rx * Rx / (Ry * Ry)) * (yy - oy) ** 2 <= Rx * Rx
+        Am &= inside
+    return R, Am
+
+
+def make_f(R, Am):
+    """MakeF (Correlation.cpp:2412-2492) with the 3x3 filter applied by array shifts (no FFT):
+    9 R - (sum of the 8 neighbours) inside, 0 and A cleared in the 2-pixel guard band, then
+    F / max(F, 1e-7) for F > 0, else 0."""
+    hR, wR = R.shape
+    F = np.zeros_like(R)
+    c = R[2:hR - 2, 2:wR - 2]
+    nb = sum(R[2 + i:hR - 2 + i, 2 + j:wR - 2 + j] for i in (-1, 0, 1) for j in (-1, 0, 1) if i or j)
+    F[2:hR - 2, 2:wR - 2] = 9.0 * c - nb
+    A2 = np.zeros_like(Am)
+    A2[2:hR - 2, 2:wR - 2] = Am[2:hR - 2, 2:wR - 2]
+    vmax = max(1e-7, float(F.max()))
+    return np.where(F > 0, F / vmax, 0.0), A2
+
+
+def fpeak(F, R, order, nbmaxht):
+    """FPeak (Correlation.cpp:2580-2767) with the axis scans and the band test as array
+    searches.  Returns (rx, ry) or None."""
+    hR, wR = F.shape
+    mask = np.zeros(F.shape, bool)
+    for k in order:
+        ry, rx = divmod(int(k), wR)
+        if mask[ry, rx]:
+            continue
+        fpk = F[ry, rx]
+        col, row = F[:, rx], F[ry, :]
+        hits = [np.nonzero(col[:ry][::-1] <= 0.5 * fpk)[0], np.nonzero(col[ry + 1:] <= 0.5 * fpk)[0],
+                np.nonzero(row[:rx][::-1] <= 0.5 * fpk)[0], np.nonzero(row[rx + 1:] <= 0.5 * fpk)[0]]
+        if any(h.size == 0 for h in hits):
+            continue
+        ri = 1 + min(int(h[0]) for h in hits)
+        if ri <= 1 and R[ry, rx] < 0.1:
+            continue
+        ro = 3 * ri
+        if ry - ro < 0 or ry + ro >= hR or rx - ro < 0 or rx + ro >= wR:
+            continue
+        rm = max(ri // 3, 1)
+        mask[ry - rm:ry + rm + 1, rx - rm:rx + rm + 1] = True
+        band = F[ry - ro:ry + ro + 1, rx - ro:rx + ro + 1].copy()
+        band[ro - ri:ro + ri + 1, ro - ri:ro + ri + 1] = -np.inf
+        if (band > nbmaxht * fpk).any():
+            continue
+        return rx, ry
+    return None
+
+
+def corr_images_f(A, B, offx, offy, rgn=None, cnt=None, Ox=0, Oy=0, Rx=-1, Ry=-1, mincor=0.0, nbmaxht=0.5):
+    """CorrImagesF (Correlation.cpp:3066-3122) on rasters.  Returns (status, r, dx, dy, lag, F, order)."""
+    h1, w1 = A.shape
+    h2, w2 = B.shape
+    nnegx, nnegy = w1 - 1, h1 - 1
+    if min(nnegx, w2, nnegy, h2) < 6:
+        return abi.XC_TOO_SMALL, 0.0, 0.0, 0.0, None, None, None
+    R, Am = r_image(A, B, offx, offy, rgn, cnt, Ox, Oy, Rx, Ry)
+    F, A2 = make_f(R, Am)
+    sel = A2 & (R > mincor) if mincor > 0 else A2
+    idx = np.flatnonzero(sel)
+    order = idx[np.lexsort((idx, -F.ravel()[idx]))]
+    pk = fpeak(F, R, order, nbmaxht) if order.size else None
+    if pk is None:
+        return abi.XC_NO_PEAK, 0.0, 0.0, 0.0, None, F, order
+    rx, ry = pk
+    hR, wR = R.shape
+    px, py, z1 = float(rx), float(ry), R[ry, rx]
+    if rx - 1 >= 0 and rx + 1 < wR and R[ry, rx - 1] > 0 and R[ry, rx + 1] > 0:
+        z0, z2 = R[ry, rx - 1], R[ry, rx + 1]
+        d = (z0 - z2) / (2 * (z0 + z2 - z1 - z1))
+        if abs(d) <= 8:
+            px += d
+    if ry - 1 >= 0 and ry + 1 < hR and R[ry - 1, rx] > 0 and R[ry + 1, rx] > 0:
+        z0, z2 = R[ry - 1, rx], R[ry + 1, rx]
+        d = (z0 - z2) / (2 * (z0 + z2 - z1 - z1))
+        if abs(d) <= 8:
+            py += d
+    return abi.XC_OK, z1, px + offx - nnegx, py + offy - nnegy, (rx - nnegx, ry - nnegy), F, order
+
+
+def f_pair(seed, lo=12, hi=20, field=40):
+    """Test pair cut from one smooth field at a known shift (every third seed: unrelated noise)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(300 + seed)
+    w1, h1, w2, h2 = (int(v) for v in rng.integers(lo, hi, size=4))
+    base = gaussian_filter(rng.standard_normal((field, field)), 1.0)
+    sx, sy = (int(v) for v in rng.integers(-3, 4, size=2))
+    m = (field - hi) // 2
+    A = base[m:m + h1, m:m + w1]
+    B = base[m + sy:m + sy + h2, m + sx:m + sx + w2] + 0.05 * rng.standard_normal((h2, w2))
+    if seed % 3 == 2:
+        B = rng.standard_normal((h2, w2))
+    return A, B, (sx, sy)

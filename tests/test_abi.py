@@ -58,6 +58,6 @@ def test_compat_library_exports_reference_signatures():
                           check=True).stdout
     text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "aln_corr_compat.h")).read(), flags=re.S)
     names = sorted(set(re.findall(r"\b([A-Z][A-Za-z0-9_]+)\s*\(", text)) - {"Point"})
-    assert {"CorrPatches", "CorrPatchToImage", "CorrImagesR", "CorrImagesS", "FFT_2D", "IFT_2D"} <= set(names)
+    assert {"CorrPatches", "CorrPatchToImage", "CorrImagesF", "CorrImagesR", "CorrImagesS", "FFT_2D", "IFT_2D"} <= set(names)
     missing = [n for n in names if f" {n}(" not in syms]
     assert not missing, missing

@@ -144,3 +144,39 @@ def test_corr_patch_to_image(ctx, seed, bfilter):
     assert (g[4].lag_x, g[4].lag_y) == (o[4].lag_x, o[4].lag_y)
     assert abs(g[1] - o[1]) <= 1e-9 * max(1.0, abs(o[1]))
     assert abs(g[2] - o[2]) <= 1e-9 and abs(g[3] - o[3]) <= 1e-9
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_corr_images_f(ctx, seed):
+    """CorrImagesF: R image, MakeF, OrderF, FPeak, ReturnR on the device against the oracle."""
+    from tests import bruteforce as bf
+    A, B, _ = bf.f_pair(seed, lo=12, hi=64, field=96) if seed >= 6 else bf.f_pair(seed)
+    pa, va = synth.raster_points(A, x0=2)
+    pb, vb = synth.raster_points(B, y0=-3)
+    h1, w1 = A.shape
+    h2, w2 = B.shape
+    req = int(0.2 * min(w1 * h1, w2 * h2))
+    rgn = abi.rgn_big_enough(3, req)
+    cnt = abi.cnt_enough_points(req) if seed % 3 == 0 else None
+    mincor = 0.0 if seed % 2 else 0.2
+    Rx, Ry = (9, 6) if seed % 4 == 1 else (-1, -1)
+    kw = dict(rgn=rgn, cnt=cnt, mincor=mincor, nbmaxht=0.5, Ox=1, Oy=0, Rx=Rx, Ry=Ry)
+    g = ctx.corr_images(abi.XC_MODE_F, pa, va, pb, vb, **kw)
+    o = oracle.corr_images(abi.XC_MODE_F, pa, va, pb, vb, **kw)
+    _same(g, o, f"F mode seed {seed}")
+
+
+def test_corr_images_f_thumbnails(ctx):
+    """CorrImagesF on a 128 x 96 pair with a known shift (the R image is 255 x 191)."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(77)
+    base = gaussian_filter(rng.standard_normal((160, 200)), 1.5)
+    A = base[20:116, 30:158]
+    B = base[25:121, 22:150] + 0.05 * rng.standard_normal((96, 128))
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    rgn = abi.rgn_big_enough(32, 64 * 48)
+    g = ctx.corr_images(abi.XC_MODE_F, pa, va, pb, vb, rgn=rgn, nbmaxht=0.5)
+    o = oracle.corr_images(abi.XC_MODE_F, pa, va, pb, vb, rgn=rgn, nbmaxht=0.5)
+    _same(g, o, "F thumbnails")
+    assert g[0] == abi.XC_OK and (g[4].lag_x, g[4].lag_y) == (8, -5)

@@ -286,3 +286,92 @@ def test_rfromangle_rotation_matches_closed_form():
     pb, vb = synth.raster_points(Bimg, x0=b[0], y0=b[2])
     st, R, X, Y, res = oracle.rfromangle(deg, pa, va, pb, vb, 8, 200)
     assert st == abi.XC_OK and abs(X) < 0.5 and abs(Y) < 0.5
+
+
+# ---- CorrImagesF: Convolve, MakeF, OrderF, FPeak ----
+
+_K3 = np.array([[-1.0, -1.0, -1.0], [-1.0, 9.0, -1.0], [-1.0, -1.0, -1.0]])
+
+
+@pytest.mark.parametrize("shape", [(13, 21), (16, 32), (37, 50), (8, 8)])
+def test_convolve_equals_circular_stencil(shape):
+    """Convolve through transforms equals the 9-point sum on the power-of-two canvas with wrap
+    (zero padding beyond the image, wrap onto it when the size is already a power of two)."""
+    h, w = shape
+    src = np.random.default_rng(h * 100 + w).standard_normal((h, w))
+    got = oracle.convolve(src, _K3)
+    Nx, Ny = oracle.fft_size(w, 1), oracle.fft_size(h, 1)
+    pad = np.zeros((Ny, Nx))
+    pad[:h, :w] = src
+    ref = sum(_K3[i + 1, j + 1] * np.roll(np.roll(pad, i, 0), j, 1) for i in (-1, 0, 1) for j in (-1, 0, 1))
+    assert np.abs(got - ref[:h, :w]).max() < 1e-12
+
+
+def test_convolve_prenorm_divides_by_kernel_sum():
+    src = np.random.default_rng(5).standard_normal((9, 11))
+    K = np.arange(9.0).reshape(3, 3) + 1.0  # sum 45
+    assert np.abs(oracle.convolve(src, K) - oracle.convolve(src, K / 45.0, prenorm=False)).max() < 1e-13
+
+
+def _gauss(h, w, cx, cy, s, a=1.0):
+    yy, xx = np.mgrid[0:h, 0:w]
+    return a * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2 * s * s))
+
+
+def _order(F):
+    idx = np.arange(F.size)
+    return idx[np.lexsort((idx, -F.ravel()))]
+
+
+def test_fpeak_closed_form_cases():
+    h, w = 40, 50
+    R = np.full((h, w), 0.5)
+    yy, _ = np.mgrid[0:h, 0:w]
+    # one isolated Gaussian: its centre
+    F = _gauss(h, w, 20, 15, 1.5)
+    assert oracle.fpeak(F, R, _order(F), 0.5) == (20, 15)
+    # the tallest peak has a 0.7 spike 5 px away, inside its outer band (ri = 2, ro = 6): rejected,
+    # its 3x3 core masked; the spike is a micro-peak (ri = 1, R < 0.1) -> the isolated 0.6 peak wins
+    F = _gauss(h, w, 20, 15, 1.5) + _gauss(h, w, 38, 28, 1.5, 0.6)
+    F[15, 25] += 0.7
+    R2 = R.copy()
+    R2[15, 25] = 0.05
+    assert oracle.fpeak(F, R2, _order(F), 0.5) == (38, 28)
+    # same with R = 0.5 at the spike: not a micro-peak, but the big peak's flank lies in its band
+    assert oracle.fpeak(F, R, _order(F), 0.5) == (38, 28)
+    # without the isolation requirement the tallest peak is taken
+    assert oracle.fpeak(F, R, _order(F), 1.0) == (20, 15)
+    # a peak 3 px from the border: its outer band (ro = 6) leaves the image -> the next peak
+    F = _gauss(h, w, 3, 20, 1.5) + _gauss(h, w, 30, 20, 1.5, 0.9)
+    assert oracle.fpeak(F, R, _order(F), 0.5) == (30, 20)
+    # a horizontal ridge never falls to half height along x: every ridge pixel is rejected
+    F = np.exp(-(yy - 20.0) ** 2 / (2 * 1.5 ** 2)) + _gauss(h, w, 25, 30, 1.5, 0.3)
+    assert oracle.fpeak(F, R, _order(F), 0.5) == (25, 30)
+    F = np.exp(-(yy - 20.0) ** 2 / (2 * 1.5 ** 2))
+    assert oracle.fpeak(F, R, _order(F), 0.5) is None
+    assert oracle.fpeak(F, R, np.array([], np.int64), 0.5) is None
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_corr_images_f_vs_bruteforce(seed):
+    A, B, _ = bf.f_pair(seed)
+    pa, va = synth.raster_points(A, x0=2)
+    pb, vb = synth.raster_points(B, y0=-3)
+    h1, w1 = A.shape
+    h2, w2 = B.shape
+    req = int(0.2 * min(w1 * h1, w2 * h2))
+    rgn = abi.rgn_big_enough(3, req)
+    mincor = 0.0 if seed % 2 else 0.2
+    Rx, Ry = (9, 6) if seed % 4 == 1 else (-1, -1)
+    st, r, dx, dy, res, _, F, Am = oracle.corr_images(abi.XC_MODE_F, pa, va, pb, vb, rgn=rgn, mincor=mincor,
+                                                      nbmaxht=0.5, Ox=1, Oy=0, Rx=Rx, Ry=Ry, want_images=True)
+    bst, br, bdx, bdy, blag, bF, _ = bf.corr_images_f(A, B, 0 - 2, -3 - 0, rgn, None, 1, 0, Rx, Ry, mincor, 0.5)
+    assert np.abs(F - bF).max() < 1e-12
+    assert not Am[:2].any() and not Am[-2:].any() and not Am[:, :2].any() and not Am[:, -2:].any()
+    assert st == bst
+    if st == abi.XC_OK:
+        assert (res.lag_x, res.lag_y) == blag
+        assert abs(r - br) <= 1e-9 * max(1, abs(br))
+        assert abs(dx - bdx) <= 1e-9 and abs(dy - bdy) <= 1e-9
+    else:
+        assert st == abi.XC_NO_PEAK and (r, dx, dy) == (0.0, 0.0, 0.0)
