@@ -652,6 +652,91 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
     }
 }
 
+// Unmasked u8 tiles without a count table: the rows are scanned in exact
+// integer arithmetic (a row's prefix of v and v^2 fits in 32 bits for
+// w <= TP_MAX_W), the raw sums accumulate down the columns in FP64 (exact
+// integers), and each stored entry is normalised from them:
+//   S = (Sraw - n mean) / sd,  Q = (Qraw - 2 mean Sraw + n mean^2) / sd^2
+// with n = (x+1)(y+1).  This is the exact sum of the normalised values; the
+// general sweep sums the values already rounded to T, which differs by the
+// accumulated rounding of T (relative 1e-8 for FP32), far below the
+// transforms' own error.  Integer DPP scans are one instruction per step,
+// against three for an FP64 scan.
+template <int CPT, int NT>
+__device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
+{
+    constexpr int NWV = NT / 64;
+    static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = tid * CPT;
+    const double mean = D.v.msd[0], inv = D.v.msd[2];
+    const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
+    const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
+    double as[CPT], aq[CPT];
+    constexpr int LA = 4;  // rows loaded ahead of the sweep
+    int nxt[LA][CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        as[j] = 0.0;
+        aq[j] = 0.0;
+        const int x = x0 + j;
+#pragma unroll
+        for (int a = 0; a < LA; ++a) nxt[a][j] = (x < w && a < h) ? src[(long long)a * D.v.stride + x] : 0;
+    }
+    const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);
+    for (int y = 0; y < h; ++y) {
+        int ps[CPT], pq[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int v = nxt[0][j];
+#pragma unroll
+            for (int a = 0; a + 1 < LA; ++a) nxt[a][j] = nxt[a + 1][j];
+            const int x = x0 + j;
+            nxt[LA - 1][j] = (x < w && y + LA < h) ? src[(long long)(y + LA) * D.v.stride + x] : 0;
+            ps[j] = (j ? ps[j - 1] : 0) + v;
+            pq[j] = (j ? pq[j - 1] : 0) + v * v;
+        }
+        const int ts = ps[CPT - 1], tq = pq[CPT - 1];
+        const int is = wave_scan_i(ts), iq = wave_scan_i(tq);
+        const int par = (y & 1) * 2 * NWV;
+        if (lane == 63) {
+            xi[par + wv] = is;
+            xi[par + NWV + wv] = iq;
+        }
+        __syncthreads();
+        int bs = is - ts, bq = iq - tq;
+        if constexpr (NWV > 1) {
+            const bool left = lane < wv;
+            const int k = left ? lane : 0;
+            bs += row16_sum_i(left ? xi[par + k] : 0);
+            bq += row16_sum_i(left ? xi[par + NWV + k] : 0);
+        }
+        const long long row = (long long)y * w;
+        double sv[CPT], qv[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            as[j] += (double)(bs + ps[j]);
+            aq[j] += (double)(bq + pq[j]);
+            const double n = (double)(x0 + j + 1) * (double)(y + 1);
+            sv[j] = (as[j] - n * mean) * inv;
+            qv[j] = ((aq[j] - m2 * as[j]) + n * msq) * inv2;
+        }
+        if (pair_st) {
+#pragma unroll
+            for (int j = 0; j < CPT; j += 2)
+                if (x0 + j < w) {
+                    D.SQ[row + x0 + j] = make_double2(sv[j], qv[j]);
+                    D.SQ[row + x0 + j + 1] = make_double2(sv[j + 1], qv[j + 1]);
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+                if (x0 + j < w) D.SQ[row + x0 + j] = make_double2(sv[j], qv[j]);
+        }
+    }
+}
+
 #define XC_SAT_DISPATCH(CALL)                                                   \
     switch (D.v.dtype) {                                                        \
     case XC_U8:                                                                 \
@@ -693,11 +778,15 @@ template <typename T, int CPT, int NT>
 __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
-    __shared__ int xz[2 * NT / 64];
+    __shared__ int xz[2 * NT / 64], xi[4 * NT / 64];
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
     const bool m = D.v.mask != nullptr;
+    if (D.v.dtype == XC_U8 && !m && !D.Z && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
+        sat_sweep_u8<CPT, NT>(D, xi);
+        return;
+    }
 #define SW_CALL(InT, MK) sat_sweep<T, CPT, NT, InT, MK>(D, band, xs, xq, xz)
     XC_SAT_DISPATCH(SW_CALL)
 #undef SW_CALL
