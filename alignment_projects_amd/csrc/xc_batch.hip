@@ -65,7 +65,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     t->dtype = dtype;
     t->flags = flags;
     const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
-    const TileSizes ts = tile_sizes(w, h);
+    const TileSizes ts = tile_sizes(w, h, false);
     if (w > TP_MAX_W) {
         delete t;
         return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
@@ -111,7 +111,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.v.mstride = w;
     D.msd = (double *)(base + lay.off[iM]);
     D.v.msd = D.msd;
-    D.SQ = (double2 *)(base + lay.off[iS]);
+    D.SQ = base + lay.off[iS];
     D.Z = (int32_t *)(base + lay.off[iZ]);
     D.part = (double *)(base + lay.off[iP]);
     D.nbands = 1;  // swept unbanded, as a plan's batched refresh sweeps it
@@ -429,7 +429,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     for (size_t r = 0; r < rots.size(); ++r) {
         const size_t px = (size_t)rots[r].w * rots[r].h;
         i_rimg[r] = lay.add(sizeof(float) * px);
-        i_rS[r] = lay.add(rot_sq[r] ? sizeof(double2) * px : 16);
+        i_rS[r] = lay.add(rot_sq[r] ? sizeof(float2) * px : 16);
         i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
     }
     size_t i_res = lay.add(sizeof(xc_result) * n);
@@ -513,7 +513,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         T.v.mstride = R.w;
         T.v.msd = dmsd;
         T.msd = nullptr;
-        T.SQ = rot_sq[r] ? (double2 *)at(i_rS[r]) : nullptr;
+        T.SQ = rot_sq[r] ? (void *)at(i_rS[r]) : nullptr;
         T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
         T.nbands = 1;
         hd[r] = T;

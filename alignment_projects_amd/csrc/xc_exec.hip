@@ -28,11 +28,11 @@ void make_sat_groups(const std::vector<int> &widths, std::vector<SatGroup> &grou
     }
 }
 
-TileSizes tile_sizes(int w, int h)
+TileSizes tile_sizes(int w, int h, bool fp64)
 {
     TileSizes t;
     const size_t n = (size_t)w * h;
-    t.SQ = sizeof(double2) * n;
+    t.SQ = (fp64 ? sizeof(double2) : sizeof(float2)) * n;
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
     t.msd = sizeof(double) * 3;
@@ -66,7 +66,7 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     const double H = P.Nx / 2 + 1;
     const double n1 = (double)P.w1 * P.h1, n2 = (double)P.w2 * P.h2;
     const double zb = P.need_cnt ? 4.0 : 0.0;
-    const double sat = (P.mode == PM_S) ? zb : 16.0 + zb;  // S mode needs only the count table
+    const double sat = (P.mode == PM_S) ? zb : s + zb;  // (S, S^2) pair of the pair's precision (= s bytes); S mode: count table only
     bytes[ST_PREP] = (prep_a ? ba * n1 + sat * n1 : 0.0) + (prep_b ? bb * n2 + sat * n2 : 0.0);
     bytes[ST_R2C] = ba * n1 + bb * n2 + s * H * (P.h1 + P.h2);
     const double rows = live_rows(P);  // lag rows written by the column pass and read by c2r

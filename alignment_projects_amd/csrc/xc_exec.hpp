@@ -68,7 +68,7 @@ int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
 struct TileSizes {
     size_t SQ, Z, part, msd, band;
 };
-TileSizes tile_sizes(int w, int h);
+TileSizes tile_sizes(int w, int h, bool fp64);  // fp64: the plain form's FP64 tables, else FP32
 
 // Group pairs into (NH, NY, whiten) buckets; returns the order of pair
 // indices (bucket-major) and the bucket list (first/count filled).

@@ -64,23 +64,44 @@ __device__ __forceinline__ TT box_sum(const TT *__restrict__ t, int w, int L, in
     return r;
 }
 
-// The same lookups on the interleaved (S, S^2) table: one 16-byte load per
-// corner, each component summed as box_sum sums it.
-__host__ __device__ __forceinline__ double2 box_sum2(const double2 *__restrict__ t, int w, int L, int R, int B, int T)
+// Interleaved (S, S^2) table entries: FP64 pairs for the FP64 form, FP32
+// pairs for the FP32 (batched) form -- each entry is the FP64 sum rounded
+// once; lookups widen it back to FP64 and all box arithmetic is FP64.
+template <typename FT> struct TabE {
+    typedef double2 type;
+    static __host__ __device__ __forceinline__ type make(double a, double b) { return make_double2(a, b); }
+};
+template <> struct TabE<float> {
+    typedef float2 type;
+    static __host__ __device__ __forceinline__ type make(double a, double b) { return make_float2((float)a, (float)b); }
+};
+template <typename FT> using tab_t = typename TabE<FT>::type;
+
+template <typename FT>
+__host__ __device__ __forceinline__ double2 tab_at(const void *t, long long i)
 {
-    double2 r = t[R + (long long)w * T];
+    const tab_t<FT> v = reinterpret_cast<const tab_t<FT> *>(t)[i];
+    return make_double2((double)v.x, (double)v.y);
+}
+
+// The same lookups on the interleaved (S, S^2) table: one load per corner,
+// each component summed as box_sum sums it.
+template <typename FT>
+__host__ __device__ __forceinline__ double2 box_sum2(const void *__restrict__ t, int w, int L, int R, int B, int T)
+{
+    double2 r = tab_at<FT>(t, R + (long long)w * T);
     if (L > 0) {
-        const double2 u = t[L - 1 + (long long)w * T];
+        const double2 u = tab_at<FT>(t, L - 1 + (long long)w * T);
         r.x -= u.x;
         r.y -= u.y;
     }
     if (B > 0) {
-        const double2 u = t[R + (long long)w * (B - 1)];
+        const double2 u = tab_at<FT>(t, R + (long long)w * (B - 1));
         r.x -= u.x;
         r.y -= u.y;
     }
     if (L > 0 && B > 0) {
-        const double2 u = t[(L - 1) + (long long)w * (B - 1)];
+        const double2 u = tab_at<FT>(t, (L - 1) + (long long)w * (B - 1));
         r.x += u.x;
         r.y += u.y;
     }
@@ -113,6 +134,7 @@ __host__ __device__ __forceinline__ RowBox row_box(const PairDev &P, int y)
 // *legal receives the candidate flag (LegalRgn and LegalCnt), *odd the
 // exit(44) condition of GetCorr.
 #pragma clang fp contract(off)
+template <typename FT>
 __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, int x, double raw,
                                            bool *legal, bool *odd)
 {
@@ -138,8 +160,8 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
     *legal = ok;
     if (P.mode == PM_S) return raw / Nxy;
     double n = (double)(olw * rb.olh);
-    const double2 sqa = box_sum2(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
-    const double2 sqb = box_sum2(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
+    const double2 sqa = box_sum2<FT>(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
+    const double2 sqb = box_sum2<FT>(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
     double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
     double num = n * raw / Nxy - s1 * s2;
     double d1 = n * q1 - s1 * s1;
@@ -634,8 +656,9 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
             for (int j = 0; j < CPT; j += 2) {
                 if (x0 + j < w) {  // w even: x0 + j + 1 < w too
                     if (D.SQ) {
-                        D.SQ[row + x0 + j] = make_double2(bs + ps[j], bq + pq[j]);
-                        D.SQ[row + x0 + j + 1] = make_double2(bs + ps[j + 1], bq + pq[j + 1]);
+                        tab_t<T> *sq = reinterpret_cast<tab_t<T> *>(D.SQ);
+                        sq[row + x0 + j] = TabE<T>::make(bs + ps[j], bq + pq[j]);
+                        sq[row + x0 + j + 1] = TabE<T>::make(bs + ps[j + 1], bq + pq[j + 1]);
                     }
                     if (D.Z) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(bz + pz[j], bz + pz[j + 1]);
                 }
@@ -644,7 +667,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 if (x0 + j < w) {
-                    if (D.SQ) D.SQ[row + x0 + j] = make_double2(bs + ps[j], bq + pq[j]);
+                    if (D.SQ) reinterpret_cast<tab_t<T> *>(D.SQ)[row + x0 + j] = TabE<T>::make(bs + ps[j], bq + pq[j]);
                     if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
                 }
             }
@@ -662,7 +685,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 // accumulated rounding of T (relative 1e-8 for FP32), far below the
 // transforms' own error.  Integer DPP scans are one instruction per step,
 // against three for an FP64 scan.
-template <int CPT, int NT>
+template <typename T, int CPT, int NT>
 __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 {
     constexpr int NWV = NT / 64;
@@ -673,6 +696,7 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
     const double mean = D.v.msd[0], inv = D.v.msd[2];
     const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
     const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
+    tab_t<T> *sq = reinterpret_cast<tab_t<T> *>(D.SQ);
     double as[CPT], aq[CPT];
     constexpr int LA = 4;  // rows loaded ahead of the sweep
     int nxt[LA][CPT];
@@ -726,13 +750,13 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 #pragma unroll
             for (int j = 0; j < CPT; j += 2)
                 if (x0 + j < w) {
-                    D.SQ[row + x0 + j] = make_double2(sv[j], qv[j]);
-                    D.SQ[row + x0 + j + 1] = make_double2(sv[j + 1], qv[j + 1]);
+                    sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
+                    sq[row + x0 + j + 1] = TabE<T>::make(sv[j + 1], qv[j + 1]);
                 }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j)
-                if (x0 + j < w) D.SQ[row + x0 + j] = make_double2(sv[j], qv[j]);
+                if (x0 + j < w) sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
         }
     }
 }
@@ -784,7 +808,7 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
     if (band >= tile_bands(D)) return;
     const bool m = D.v.mask != nullptr;
     if (D.v.dtype == XC_U8 && !m && !D.Z && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
-        sat_sweep_u8<CPT, NT>(D, xi);
+        sat_sweep_u8<T, CPT, NT>(D, xi);
         return;
     }
 #define SW_CALL(InT, MK) sat_sweep<T, CPT, NT, InT, MK>(D, band, xs, xq, xz)
@@ -1074,6 +1098,7 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 //   *cand : the lag competes for the peak
 //   *win  : the lag belongs to the centred wR x hR image (R/S modes)
 //   *odd  : GetCorr's exit(44) condition
+template <typename FT>
 __device__ __forceinline__ double lag_value(const PairDev &P, const RowBox &rb, int y, int ix, double raw,
                                             bool *cand, bool *win, bool *odd)
 {
@@ -1096,12 +1121,12 @@ __device__ __forceinline__ double lag_value(const PairDev &P, const RowBox &rb, 
         const long long dxd = x - P.Ox, dyd = y - P.Oy;
         if (dxd * dxd + dyd * dyd > (long long)P.radius * P.radius) return 0.0;
         bool legal;
-        const double v = eval_lag(P, rb, x, raw, &legal, odd);
+        const double v = eval_lag<FT>(P, rb, x, raw, &legal, odd);
         *cand = legal;
         return v;
     }
     bool legal, o;
-    const double v = eval_lag(P, rb, x, raw, &legal, &o);
+    const double v = eval_lag<FT>(P, rb, x, raw, &legal, &o);
     *win = true;
     bool c = legal;
     if (P.Rx > 0 && P.Ry > 0) {
@@ -1131,7 +1156,7 @@ struct NccRow {
     int olh;
     int x0, x1;    // legal-interval bounds (empty when x0 > x1)
     double invN;   // 1 / (Nx Ny), a power of two: raw * invN == raw / Nxy exactly
-    const double2 *sqAt, *sqAb, *sqBt, *sqBb;  // (S, S^2) table rows: band top, row below the band (or null)
+    const void *sqAt, *sqAb, *sqBt, *sqBb;  // (S, S^2) table rows: band top, row below the band (or null)
     const int32_t *zAt, *zAb, *zBt, *zBb;
 };
 
@@ -1143,6 +1168,7 @@ __host__ __device__ __forceinline__ long long isqrt_floor(long long v)
     return s;
 }
 
+template <typename FT>
 __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
 {
     NccRow q;
@@ -1174,10 +1200,11 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
     q.x1 = ok ? (int)hi : 0;
     const long long at = (long long)rb.a_t * P.w1, ab = (long long)(rb.a_b - 1) * P.w1;
     const long long bt = (long long)rb.b_t * P.w2, bb = (long long)(rb.b_b - 1) * P.w2;
-    q.sqAt = P.SQA + at;
-    q.sqBt = P.SQB + bt;
-    q.sqAb = rb.a_b > 0 ? P.SQA + ab : nullptr;
-    q.sqBb = rb.b_b > 0 ? P.SQB + bb : nullptr;
+    const tab_t<FT> *SA = reinterpret_cast<const tab_t<FT> *>(P.SQA), *SB = reinterpret_cast<const tab_t<FT> *>(P.SQB);
+    q.sqAt = SA + at;
+    q.sqBt = SB + bt;
+    q.sqAb = rb.a_b > 0 ? SA + ab : nullptr;
+    q.sqBb = rb.b_b > 0 ? SB + bb : nullptr;
     q.zAt = P.ZA ? P.ZA + at : nullptr;
     q.zBt = P.ZB ? P.ZB + bt : nullptr;
     q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
@@ -1200,20 +1227,21 @@ __device__ __forceinline__ TT band_sum(const TT *__restrict__ top, const TT *__r
 }
 
 // band_sum on the interleaved (S, S^2) table.
-__device__ __forceinline__ double2 band_sum2(const double2 *__restrict__ top, const double2 *__restrict__ bot, int L, int R)
+template <typename FT>
+__device__ __forceinline__ double2 band_sum2(const void *__restrict__ top, const void *__restrict__ bot, int L, int R)
 {
-    double2 v = top[R];
+    double2 v = tab_at<FT>(top, R);
     if (L > 0) {
-        const double2 u = top[L - 1];
+        const double2 u = tab_at<FT>(top, L - 1);
         v.x -= u.x;
         v.y -= u.y;
     }
     if (bot) {
-        const double2 u = bot[R];
+        const double2 u = tab_at<FT>(bot, R);
         v.x -= u.x;
         v.y -= u.y;
         if (L > 0) {
-            const double2 z = bot[L - 1];
+            const double2 z = tab_at<FT>(bot, L - 1);
             v.x += z.x;
             v.y += z.y;
         }
@@ -1254,8 +1282,8 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     const int AL = BL - x, AR = BR - x;
     const int olw = BR - BL + 1;
     const RuleDev &g = P.rule;
-    const double2 sqa = band_sum2(q.sqAt, q.sqAb, AL, AR);
-    const double2 sqb = band_sum2(q.sqBt, q.sqBb, BL, BR);
+    const double2 sqa = band_sum2<T>(q.sqAt, q.sqAb, AL, AR);
+    const double2 sqb = band_sum2<T>(q.sqBt, q.sqBb, BL, BR);
     const double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
     bool c = true;
     if (g.exact_w > 0 && olw != g.exact_w) c = false;
@@ -1278,41 +1306,45 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
 // resolved per segment, and no per-lag rule tests.  AB / BB: the band has a
 // row below it in A / B's table.
 // Table element i read as a global-memory load (the table rows arrive as
-// generic pointers).
-typedef double xc_dv2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ double2 ld_sq(const double2 *p, int i)
+// generic pointers), widened to FP64.
+template <typename FT> struct TabV;
+template <> struct TabV<double> { typedef double type __attribute__((ext_vector_type(2))); };
+template <> struct TabV<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <typename FT>
+__device__ __forceinline__ double2 ld_sq(const void *p, int i)
 {
-    typedef __attribute__((address_space(1))) const xc_dv2 gdv2;
-    const xc_dv2 v = ((gdv2 *)(const void *)p)[i];
-    return make_double2(v.x, v.y);
+    typedef typename TabV<FT>::type V;
+    typedef __attribute__((address_space(1))) const V gv;
+    const V v = ((gv *)p)[i];
+    return make_double2((double)v.x, (double)v.y);
 }
 
 template <typename T, int TPF, bool AB, bool BB>
 __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
                                               long long &bestk, int &flags)
 {
-    const double2 *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
+    const void *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
     const int w1 = P.w1, w2 = P.w2, olh = q.olh;
     const double invN = q.invN;
     // x >= 0: A's band starts at column 0, B's at x
     for (int x = (q.x0 > 0 ? q.x0 : 0) + t; x <= q.x1; x += TPF) {
         const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x;
         const int BL1 = x > 0 ? x - 1 : 0;  // L - 1 of B's band (unused at x = 0)
-        double2 a = ld_sq(At, AR);
+        double2 a = ld_sq<T>(At, AR);
         if constexpr (AB) {
-            const double2 u = ld_sq(Ab, AR);
+            const double2 u = ld_sq<T>(Ab, AR);
             a.x -= u.x;
             a.y -= u.y;
         }
         // x = 0 has no left term: subtract / add +0.0 instead (exact), so that
         // every load is unconditional and all of them are in flight together
         const bool l = x > 0;
-        double2 b = ld_sq(Bt, BR);
-        const double2 bl = ld_sq(Bt, BL1);
+        double2 b = ld_sq<T>(Bt, BR);
+        const double2 bl = ld_sq<T>(Bt, BL1);
         b.x -= l ? bl.x : 0.0;
         b.y -= l ? bl.y : 0.0;
         if constexpr (BB) {
-            const double2 u = ld_sq(Bb, BR), z = ld_sq(Bb, BL1);
+            const double2 u = ld_sq<T>(Bb, BR), z = ld_sq<T>(Bb, BL1);
             b.x -= u.x;
             b.y -= u.y;
             b.x += l ? z.x : 0.0;
@@ -1329,20 +1361,20 @@ __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q,
     const int xe = q.x1 < -1 ? q.x1 : -1;
     for (int x = q.x0 + t; x <= xe; x += TPF) {
         const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x, AL1 = -x - 1;
-        double2 a = ld_sq(At, AR);
-        const double2 al = ld_sq(At, AL1);
+        double2 a = ld_sq<T>(At, AR);
+        const double2 al = ld_sq<T>(At, AL1);
         a.x -= al.x;
         a.y -= al.y;
         if constexpr (AB) {
-            const double2 u = ld_sq(Ab, AR), z = ld_sq(Ab, AL1);
+            const double2 u = ld_sq<T>(Ab, AR), z = ld_sq<T>(Ab, AL1);
             a.x -= u.x;
             a.y -= u.y;
             a.x += z.x;
             a.y += z.y;
         }
-        double2 b = ld_sq(Bt, BR);
+        double2 b = ld_sq<T>(Bt, BR);
         if constexpr (BB) {
-            const double2 u = ld_sq(Bb, BR);
+            const double2 u = ld_sq<T>(Bb, BR);
             b.x -= u.x;
             b.y -= u.y;
         }
@@ -1376,7 +1408,7 @@ __device__ __forceinline__ void ncc_scan_fast(const PairDev &P, const NccRow &q,
 template <typename T>
 __device__ __forceinline__ double ncc_value(const PairDev &P, int x, int y, double raw)
 {
-    const NccRow q = ncc_row(P, y);
+    const NccRow q = ncc_row<T>(P, y);
     if (x < q.x0 || x > q.x1) return 0.0;
     bool cand, odd;
     return ncc_lag<T>(P, q, x, raw, &cand, &odd);
@@ -1443,7 +1475,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     if (valid && P.mode == PM_NCC) {
         // raster order: x >= 0 first (ix = x), then x < 0 (ix = Nx + x); each thread's
         // keys ascend, so a strict > keeps the first of equal values
-        const NccRow q = ncc_row(P, y);
+        const NccRow q = ncc_row<T>(P, y);
         const int segs[2][2] = {{q.x0 > 0 ? q.x0 : 0, q.x1}, {q.x0, q.x1 < -1 ? q.x1 : -1}};
         const bool every = P.rule.exact_w <= 0 && !P.rule.table && !P.rule.cnt_enabled;
         if (every) ncc_scan_fast<T, TPF>(P, q, R, t, NX, bestv, bestk, flags);
@@ -1464,7 +1496,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     } else if (valid) {
         for (int ix = t; ix < NX; ix += TPF) {
             bool cand, win, odd;
-            const double val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+            const double val = lag_value<T>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
             if (odd) flags |= 2;
             if (win) {
                 ssum += val;
@@ -1550,7 +1582,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
                         if (!dead) val = ncc_value<T>(P, x, y, (double)R[ix]);
                     } else {
                         bool cand, win, odd;
-                        val = lag_value(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+                        val = lag_value<T>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
                     }
                 }
                 if (s) r.right = val;
@@ -1751,7 +1783,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
                     const double raw = point_c2r<T>(P, ylr, ix, tw, sv);
                     bool legal, odd;
                     const RowBox rb = row_box(P, yy);
-                    zz[s] = eval_lag(P, rb, x, raw, &legal, &odd);
+                    zz[s] = eval_lag<T>(P, rb, x, raw, &legal, &odd);
                 }
                 if (zz[0] > 0.0 && zz[1] > 0.0) {
                     const double d = 1.0 * (zz[0] - zz[1]) / (2 * (zz[0] + zz[1] - z1 - z1));

@@ -73,7 +73,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     const int nr2c = (h1 + G - 1) / G + (h2 + G - 1) / G;
     const int ncol = (H + C - 1) / C;
     const int nc2r = (P.hR + G - 1) / G;
-    const TileSizes t1 = tile_sizes(w1, h1), t2 = tile_sizes(w2, h2);
+    const TileSizes t1 = tile_sizes(w1, h1, true), t2 = tile_sizes(w2, h2, true);
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n), i_vals = lay.add(sizeof(double) * n);
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
@@ -110,7 +110,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
         D.v.mstride = D.v.w;
         D.msd = (double *)at(k ? i_mB : i_mA);
         D.v.msd = D.msd;
-        D.SQ = (double2 *)at(k ? i_SB : i_SA);
+        D.SQ = at(k ? i_SB : i_SA);
         D.Z = (int32_t *)at(k ? i_ZB : i_ZA);
         D.part = (double *)at(k ? i_pB : i_pA);
         D.bandp = (double *)at(k ? i_bB : i_bA);

@@ -58,7 +58,8 @@ struct ImgView {
 struct TileDev {
     ImgView v;
     double *msd;         // out: (mean, sd, 1/sd)
-    double2 *SQ;         // out: FP64 summed-area tables of x and x^2, interleaved per pixel, w x h
+    void *SQ;            // out: summed-area tables of x and x^2, interleaved per pixel, w x h (FP64 pairs
+                         //      for the FP64 form, FP32 pairs for the batched FP32 form)
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
     double *bandp;       // scratch: column sums of the table sweep's row bands (nbands > 1)
@@ -101,7 +102,7 @@ struct PairDev {
     double mincor;
     RuleDev rule;
     ImgView A, B;                     // source images
-    const double2 *SQA, *SQB;          // FP64 summed-area tables (S, S^2) interleaved
+    const void *SQA, *SQB;             // summed-area tables (S, S^2) interleaved, entries of the pair's precision
     const int32_t *ZA, *ZB;           // non-zero count tables
     void *specA, *specB;              // cpx<T> [h][Nx/2+1] row spectra
     void *lag;                        // cpx<T> [hR][Nx/2+1] lag rows
