@@ -292,6 +292,30 @@ __device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, doub
 {
     const InT *row = reinterpret_cast<const InT *>(V.data) + (long long)y * V.stride;
     const uint8_t *mrow = V.mask ? V.mask + (long long)y * V.mstride : nullptr;
+    if constexpr (sizeof(T) == 4 && sizeof(InT) == 1) {
+        // FP32 form, unmasked u8 row at an even address: one 16-bit load per pixel
+        // pair and an FP32 fma for the normalisation (within an FP32 rounding of
+        // the FP64 value)
+        if (!mrow && (reinterpret_cast<uintptr_t>(row) & 1) == 0) {
+            const uint16_t *r2 = reinterpret_cast<const uint16_t *>(row);
+            const float sc = (float)inv, of = (float)(-mean * inv);
+#pragma unroll
+            for (int m = 0; m < ME; ++m) {
+                const int p = t + TPF * m, x = 2 * p;
+                T a = (T)0, b = (T)0;
+                if (valid && x + 1 < V.w) {
+                    const unsigned int u = r2[p];
+                    a = fmaf((float)(u & 255u), sc, of);
+                    b = fmaf((float)(u >> 8), sc, of);
+                } else if (valid && x < V.w) {
+                    a = fmaf((float)row[x], sc, of);
+                }
+                v[m].x = a;
+                v[m].y = b;
+            }
+            return;
+        }
+    }
 #pragma unroll
     for (int m = 0; m < ME; ++m) {
         const int x = 2 * (t + TPF * m);
