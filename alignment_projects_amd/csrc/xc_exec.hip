@@ -56,7 +56,10 @@ void make_buckets(const std::vector<PairDev> &pairs, int, std::vector<int> &orde
         b.whiten = std::get<2>(kv.first) != 0;
         b.first = (int)order.size();
         b.count = (int)kv.second.size();
-        for (int i : kv.second) order.push_back(i);
+        for (int i : kv.second) {
+            order.push_back(i);
+            b.cnt = b.cnt || (pairs[i].mode == PM_NCC && pairs[i].rule.cnt_enabled);
+        }
         buckets.push_back(b);
     }
 }
@@ -119,7 +122,7 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     if (fork) XC_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
     if (s0 <= ST_C2R && ST_C2R < s1)
         for (const Bucket &b : ex.buckets)
-            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r))) return st;
+            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r, b.cnt))) return st;
     if (s0 <= ST_FIN && ST_FIN < s1)
         for (const Bucket &b : ex.buckets)
             if ((st = launch_finalize<T>(ctx, b.count, ex.d_pairs + b.first))) return st;

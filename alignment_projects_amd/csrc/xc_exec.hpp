@@ -25,6 +25,7 @@ struct Bucket {
     double col_us = 0; // tuned time of the column pass (0: not tuned)
     const int2 *d_c2r = nullptr;
     int n_c2r = 0;
+    bool cnt = false;  // some NCC pair of the bucket has a count rule (c2r variant with the count-table scan)
 };
 
 // Device-resident execution description.  All pointers are device pointers

@@ -1324,11 +1324,12 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     return c ? r : 0.0;
 }
 
-// The NCC scan of one lag row when every lag of the legal interval is a
-// candidate (no exact width, table or count rule): ncc_lag's arithmetic
-// with the row's table rows held in registers, the edge cases of band_sum2
-// resolved per segment, and no per-lag rule tests.  AB / BB: the band has a
-// row below it in A / B's table.
+// The NCC scan of one lag row when no per-lag rule but the count rule
+// applies (no exact width, no table): ncc_lag's arithmetic with the row's
+// table rows held in registers, the edge cases of band_sum2 resolved per
+// segment.  AB / BB: the band has a row below it in A / B's tables.  CNT:
+// LegalCnt's threshold test on the count tables (a lag that fails it is no
+// candidate).
 // Table element i read as a global-memory load (the table rows arrive as
 // generic pointers), widened to FP64.
 template <typename FT> struct TabV;
@@ -1343,26 +1344,46 @@ __device__ __forceinline__ double2 ld_sq(const void *p, int i)
     return make_double2((double)v.x, (double)v.y);
 }
 
-template <typename T, int TPF, bool AB, bool BB>
+__device__ __forceinline__ int ld_z(const int32_t *p, int i)
+{
+    typedef __attribute__((address_space(1))) const int gi;
+    return ((gi *)(const void *)p)[i];
+}
+
+template <typename T, int TPF, bool AB, bool BB, bool CNT>
 __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
                                               long long &bestk, int &flags)
 {
     const void *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
     const int w1 = P.w1, w2 = P.w2, olh = q.olh;
     const double invN = q.invN;
+    const long long k1 = P.rule.k1, k2 = P.rule.k2;
     // x >= 0: A's band starts at column 0, B's at x
     for (int x = (q.x0 > 0 ? q.x0 : 0) + t; x <= q.x1; x += TPF) {
         const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x;
         const int BL1 = x > 0 ? x - 1 : 0;  // L - 1 of B's band (unused at x = 0)
+        // x = 0 has no left term: subtract / add 0 instead (exact), so that
+        // every load is unconditional and all of them are in flight together
+        const bool l = x > 0;
+        bool cand = true;
+        if constexpr (CNT) {
+            int c1 = ld_z(q.zAt, AR), c2 = ld_z(q.zBt, BR);
+            const int bl = ld_z(q.zBt, BL1);
+            if constexpr (AB) c1 -= ld_z(q.zAb, AR);
+            c2 -= l ? bl : 0;
+            if constexpr (BB) {
+                const int u = ld_z(q.zBb, BR), z = ld_z(q.zBb, BL1);
+                c2 -= u;
+                c2 += l ? z : 0;
+            }
+            cand = (long long)c1 > k1 && (long long)c2 > k2;
+        }
         double2 a = ld_sq<T>(At, AR);
         if constexpr (AB) {
             const double2 u = ld_sq<T>(Ab, AR);
             a.x -= u.x;
             a.y -= u.y;
         }
-        // x = 0 has no left term: subtract / add +0.0 instead (exact), so that
-        // every load is unconditional and all of them are in flight together
-        const bool l = x > 0;
         double2 b = ld_sq<T>(Bt, BR);
         const double2 bl = ld_sq<T>(Bt, BL1);
         b.x -= l ? bl.x : 0.0;
@@ -1375,16 +1396,28 @@ __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q,
             b.y += l ? z.y : 0.0;
         }
         const double r = ncc_r<T>(BR - x + 1, olh, invN, (double)R[x], a.x, a.y, b.x, b.y);
-        flags |= fabs(r) > 1.0001 ? 3 : 1;
-        if (r > bestv) {
-            bestv = r;
-            bestk = x;
+        if (cand) {
+            flags |= fabs(r) > 1.0001 ? 3 : 1;
+            if (r > bestv) {
+                bestv = r;
+                bestk = x;
+            }
         }
     }
     // x < 0: A's band starts at column -x, B's at 0
     const int xe = q.x1 < -1 ? q.x1 : -1;
     for (int x = q.x0 + t; x <= xe; x += TPF) {
         const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x, AL1 = -x - 1;
+        bool cand = true;
+        if constexpr (CNT) {
+            int c1 = ld_z(q.zAt, AR) - ld_z(q.zAt, AL1), c2 = ld_z(q.zBt, BR);
+            if constexpr (AB) {
+                c1 -= ld_z(q.zAb, AR);
+                c1 += ld_z(q.zAb, AL1);
+            }
+            if constexpr (BB) c2 -= ld_z(q.zBb, BR);
+            cand = (long long)c1 > k1 && (long long)c2 > k2;
+        }
         double2 a = ld_sq<T>(At, AR);
         const double2 al = ld_sq<T>(At, AL1);
         a.x -= al.x;
@@ -1404,25 +1437,27 @@ __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q,
         }
         const int ix = NX + x;
         const double r = ncc_r<T>(BR + 1, olh, invN, (double)R[ix], a.x, a.y, b.x, b.y);
-        flags |= fabs(r) > 1.0001 ? 3 : 1;
-        if (r > bestv) {
-            bestv = r;
-            bestk = ix;
+        if (cand) {
+            flags |= fabs(r) > 1.0001 ? 3 : 1;
+            if (r > bestv) {
+                bestv = r;
+                bestk = ix;
+            }
         }
     }
 }
 #pragma clang fp contract(on)
 
-template <typename T, int TPF>
+template <typename T, int TPF, bool CNT>
 __device__ __forceinline__ void ncc_scan_fast(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
                                               long long &bestk, int &flags)
 {
     if (q.sqAb) {
-        if (q.sqBb) ncc_scan_rows<T, TPF, true, true>(P, q, R, t, NX, bestv, bestk, flags);
-        else ncc_scan_rows<T, TPF, true, false>(P, q, R, t, NX, bestv, bestk, flags);
+        if (q.sqBb) ncc_scan_rows<T, TPF, true, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TPF, true, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
     } else {
-        if (q.sqBb) ncc_scan_rows<T, TPF, false, true>(P, q, R, t, NX, bestv, bestk, flags);
-        else ncc_scan_rows<T, TPF, false, false>(P, q, R, t, NX, bestv, bestk, flags);
+        if (q.sqBb) ncc_scan_rows<T, TPF, false, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TPF, false, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
     }
 }
 
@@ -1438,7 +1473,9 @@ __device__ __forceinline__ double ncc_value(const PairDev &P, int x, int y, doub
     return ncc_lag<T>(P, q, x, raw, &cand, &odd);
 }
 
-template <typename T, int NH>
+// CNT: the variant for launches with count-rule NCC pairs (scans the count
+// tables in the fast NCC path); without it such rows take the general path.
+template <typename T, int NH, bool CNT>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     k_row_c2r_eval(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw)
 {
@@ -1501,8 +1538,9 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
         // keys ascend, so a strict > keeps the first of equal values
         const NccRow q = ncc_row<T>(P, y);
         const int segs[2][2] = {{q.x0 > 0 ? q.x0 : 0, q.x1}, {q.x0, q.x1 < -1 ? q.x1 : -1}};
-        const bool every = P.rule.exact_w <= 0 && !P.rule.table && !P.rule.cnt_enabled;
-        if (every) ncc_scan_fast<T, TPF>(P, q, R, t, NX, bestv, bestk, flags);
+        const bool fast = P.rule.exact_w <= 0 && !P.rule.table;
+        if (fast && !P.rule.cnt_enabled) ncc_scan_fast<T, TPF, false>(P, q, R, t, NX, bestv, bestk, flags);
+        else if (CNT && fast && q.zAt && q.zBt) ncc_scan_fast<T, TPF, CNT>(P, q, R, t, NX, bestv, bestk, flags);
         else for (int sgi = 0; sgi < 2; ++sgi) {
             for (int x = segs[sgi][0] + t; x <= segs[sgi][1]; x += TPF) {
                 const int ix = x >= 0 ? x : NX + x;

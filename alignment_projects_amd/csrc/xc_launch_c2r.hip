@@ -5,32 +5,41 @@
 
 namespace xc {
 
-template <typename T, int NH> static int c2r_n(xc_ctx *ctx, int nb, const PairDev *pd, const int2 *w)
+template <typename T, int NH> static int c2r_n(xc_ctx *ctx, int nb, const PairDev *pd, const int2 *w, bool cnt)
 {
     if constexpr (NH > max_nh<T>()) {
         return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
     } else {
         const size_t smem = c2r_smem<T, NH>();
-        hipLaunchKernelGGL((k_row_c2r_eval<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
-                           twiddles<T>(ctx));
+        // the count-table scan holds more registers: a separate variant keeps the others' occupancy
+        if constexpr (sizeof(T) == 4) {
+            if (cnt) {
+                hipLaunchKernelGGL((k_row_c2r_eval<T, NH, true>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem,
+                                   ctx->stream, pd, w, twiddles<T>(ctx));
+                XC_HIP(ctx, hipGetLastError());
+                return XC_OK;
+            }
+        }
+        hipLaunchKernelGGL((k_row_c2r_eval<T, NH, false>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd,
+                           w, twiddles<T>(ctx));
         XC_HIP(ctx, hipGetLastError());
         return XC_OK;
     }
 }
 
-template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int2 *w)
+template <typename T> int launch_row_c2r(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int2 *w, bool cnt)
 {
     if (nb <= 0) return XC_OK;
     switch (NH) {
 #define F(n) \
-    case n: return c2r_n<T, n>(ctx, nb, pd, w);
+    case n: return c2r_n<T, n>(ctx, nb, pd, w, cnt);
         XC_POW2_CASES(F)
 #undef F
     }
     return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
 }
 
-template int launch_row_c2r<double>(xc_ctx *, int, int, const PairDev *, const int2 *);
-template int launch_row_c2r<float>(xc_ctx *, int, int, const PairDev *, const int2 *);
+template int launch_row_c2r<double>(xc_ctx *, int, int, const PairDev *, const int2 *, bool);
+template int launch_row_c2r<float>(xc_ctx *, int, int, const PairDev *, const int2 *, bool);
 
 }  // namespace xc
