@@ -871,7 +871,9 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
     const int py = (int)(pix / R.w), px = (int)(pix - (long long)py * R.w);
     const double mean = R.src.msd[0], sd = R.src.msd[1];
     // grid points whose rotated position lies in [px-1, px+1) x [py-1, py+1):
-    // bound them by the inverse rotation of the square's corners, plus a margin
+    // they lie in the inverse rotation of that square (the transpose of the
+    // rotation), so in the integer points of its bounding box, taken with a
+    // slack of 1e-7 px for the rounding of the corners (~1e-13 px here)
     double gx0 = 1e300, gx1 = -1e300, gy0 = 1e300, gy1 = -1e300;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -882,8 +884,8 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
         gy0 = fmin(gy0, gy);
         gy1 = fmax(gy1, gy);
     }
-    const int ix0 = imax(0, (int)floor(gx0) - 1), ix1 = imin(R.src.w - 1, (int)ceil(gx1) + 1);
-    const int iy0 = imax(0, (int)floor(gy0) - 1), iy1 = imin(R.src.h - 1, (int)ceil(gy1) + 1);
+    const int ix0 = imax(0, (int)ceil(gx0 - 1e-7)), ix1 = imin(R.src.w - 1, (int)floor(gx1 + 1e-7));
+    const int iy0 = imax(0, (int)ceil(gy0 - 1e-7)), iy1 = imin(R.src.h - 1, (int)floor(gy1 + 1e-7));
     double acc = 0.0;
     for (int gy = iy0; gy <= iy1; ++gy) {
         for (int gx = ix0; gx <= ix1; ++gx) {
