@@ -72,6 +72,15 @@ class XcResult(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("_")}
 
 
+class XcAngleRec(C.Structure):
+    """CorRec without its affine (CThmScan.h:36-44): angle, score, displacement."""
+    _fields_ = [("A", C.c_double), ("R", C.c_double), ("X", C.c_double), ("Y", C.c_double)]
+
+
+XC_SCAN_MAXR = 0
+XC_SCAN_CONSTXY = 1
+
+
 class XcPair(C.Structure):
     _fields_ = [
         ("a", C.c_void_p), ("b", C.c_void_p),

@@ -1,0 +1,270 @@
+// xc_anglescan.hip -- CThmScan's angle search around the rotation sweep:
+// RFromAngle on point lists (FP64), AngleScanMaxR / AngleScanConstXY and
+// PeakHunt, evaluated either with the FP64 plain correlator (the
+// reference's arithmetic, one angle per call) or with the batched FP32
+// sweep over resident tiles (every angle of a scan in one launch).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <functional>
+#include <vector>
+
+#include "xc_host.hpp"
+
+using namespace xc;
+
+namespace {
+
+constexpr double REF_PI = 3.14159265358;  // the reference's truncated pi (GenDefs.h)
+
+struct Rules {
+    xc_rgn_rule rgn;
+    xc_cnt_rule cnt;
+};
+
+// BigEnough(olap1D, reqArea) and EnoughPoints(reqArea), as RFromAngle passes them
+// (CThmScan.cpp:28-33, 360-365): both sides >= olap1D, area > reqArea; both counts > reqArea.
+Rules scan_rules(int olap1D, int64_t req)
+{
+    Rules r;
+    memset(&r, 0, sizeof r);
+    r.rgn.enabled = 1;
+    r.rgn.min_w = r.rgn.min_h = olap1D;
+    r.rgn.min_area = req;
+    r.cnt.enabled = 1;
+    r.cnt.k1 = r.cnt.k2 = req;
+    return r;
+}
+
+// LineFit's correlation coefficient (Maths.cpp:457-507); NaN or Inf for a zero variance.
+#pragma clang fp contract(off)
+double lincor(const double *x, const double *y, int n)
+{
+    double nn = 0.0, sx = 0.0, sy = 0.0, sxx = 0.0, syy = 0.0, sxy = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double X = x[i], Y = y[i];
+        sx += X;
+        sy += Y;
+        sxx += X * X;
+        syy += Y * Y;
+        sxy += X * Y;
+        ++nn;
+    }
+    const double r = nn * sxy - sx * sy, vx = nn * sxx - sx * sx, vy = nn * syy - sy * sy;
+    return r / sqrt(vx * vy);
+}
+#pragma clang fp contract(on)
+
+// ROUND (Maths.h:19)
+double round_ref(double x) { return (double)(x >= 0.0 ? (int)(x + 0.5) : (int)(x - 0.5)); }
+
+// The angle lists of AngleScanMaxR (CThmScan.cpp:444-470) and AngleScanConstXY (:543-544).
+std::vector<double> scan_angles(int mode, double center, double hlfwid, double step, bool favor)
+{
+    std::vector<double> a;
+    if (mode == XC_SCAN_MAXR) {
+        a.push_back(center);
+        if (favor) {
+            const double cwide = 2.0 * step, cstep = step / 3.0;
+            for (double v = center - cwide; v <= center + cwide; v += cstep) a.push_back(v);
+            if (hlfwid >= 180) {
+                const double opc = center + 180;
+                for (double v = opc - cwide; v <= opc + cwide; v += cstep) a.push_back(v);
+            }
+        }
+    }
+    for (double v = center - hlfwid; v <= center + hlfwid; v += step) a.push_back(v);
+    return a;
+}
+
+// The selections of AngleScanMaxR (:476-487) and AngleScanConstXY (:554-613).
+xc_angle_rec scan_select(int mode, const std::vector<xc_angle_rec> &vC)
+{
+    xc_angle_rec best;
+    memset(&best, 0, sizeof best);
+    const int nC = (int)vC.size();
+    if (mode == XC_SCAN_MAXR) {
+        if (!nC) return best;
+        best = vC[0];
+        for (int ic = 1; ic < nC; ++ic)
+            if (vC[ic].R > best.R) best = vC[ic];
+        return best;
+    }
+    std::vector<int> order(nC);
+    for (int i = 0; i < nC; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return vC[a].R > vC[b].R; });
+    const double rthresh = 0.7;
+    const int m = 3;
+    for (int ic : order) {
+        if (ic < m || nC - 1 - ic < m) continue;
+        double A[2 * m + 1], X[2 * m + 1], Y[2 * m + 1];
+        int n = 0;
+        for (int j = ic - m; j <= ic + m; ++j, ++n) {
+            A[n] = vC[j].A;
+            X[n] = round_ref(vC[j].X);
+            Y[n] = round_ref(vC[j].Y);
+        }
+        double lc = lincor(A, X, n);
+        if (std::isfinite(lc) && fabs(lc) < rthresh) continue;
+        lc = lincor(A, Y, n);
+        if (std::isfinite(lc) && fabs(lc) < rthresh) continue;
+        return vC[ic];
+    }
+    return best;
+}
+
+// RFromAngle on point lists (CThmScan.cpp:329-366, identity deformation and tweak).
+int rfa64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
+          double deg, const Rules &ru, int Ox, int Oy, int Rx, int Ry, std::vector<xc_point> &buf, xc_angle_rec *rec)
+{
+#pragma clang fp contract(off)
+    const double c = cos(deg * REF_PI / 180.0), s = sin(deg * REF_PI / 180.0);
+    const double t0 = c, t1 = -s, t3 = s, t4 = c;
+    buf.resize(na);
+    for (int i = 0; i < na; ++i) {
+        buf[i].x = ap[i].x * t0 + ap[i].y * t1;
+        buf[i].y = ap[i].x * t3 + ap[i].y * t4;
+    }
+    double X = 0.0, Y = 0.0, R = 0.0;
+    const int st = xc_corr_images_f64(ctx, XC_MODE_S, buf.data(), av, na, bp, bv, nb, &ru.rgn, &ru.cnt, 0.0, 0.0, Ox,
+                                      Oy, Rx, Ry, &X, &Y, &R, nullptr);
+    if (st < 0) return st;
+    rec->A = deg;
+    rec->R = R;
+    rec->X = X;
+    rec->Y = Y;
+    return XC_OK;
+}
+
+// A batch of angles through the FP32 sweep of tiles.
+int rfa32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const std::vector<double> &angles, const Rules &ru, int Ox,
+          int Oy, int Rx, int Ry, std::vector<xc_angle_rec> &out)
+{
+    const int n = (int)angles.size();
+    out.resize(n);
+    if (!n) return XC_OK;
+    std::vector<xc_result> res(n);
+    const int st = xc_sweep_angles_f32(ctx, a, b, angles.data(), n, &ru.rgn, &ru.cnt, Ox, Oy, Rx, Ry, res.data());
+    if (st < 0) return st;
+    for (int i = 0; i < n; ++i) {
+        const bool ok = res[i].status == XC_OK;
+        out[i].A = angles[i];
+        out[i].R = ok ? res[i].r : 0.0;  // CorrImagesS: 0 and (0, 0) without a peak
+        out[i].X = ok ? res[i].dx : 0.0;
+        out[i].Y = ok ? res[i].dy : 0.0;
+    }
+    return XC_OK;
+}
+
+// PeakHunt (CThmScan.cpp:676-732) over an evaluator of one angle.
+int peak_hunt(double hlfwid, xc_angle_rec *best, int *n_evals, const std::function<int(double, xc_angle_rec *)> &eval)
+{
+    xc_angle_rec C, B0 = *best;
+    double L = best->A - hlfwid, R = best->A + hlfwid, M;
+    int k = 1, st;
+    if ((st = eval(M = (L + R) / 2.0, best))) return st;
+    while (R - L > 0.0001) {
+        double a;
+        ++k;
+        if ((st = eval(a = (L + M) / 2.0, &C))) return st;
+        if (C.R >= best->R) {
+            R = M;
+            *best = C;
+            M = a;
+        } else {
+            L = a;
+        }
+        if ((st = eval(a = (M + R) / 2.0, &C))) return st;
+        if (C.R >= best->R) {
+            L = M;
+            *best = C;
+            M = a;
+        } else {
+            R = a;
+        }
+    }
+    if (B0.R > best->R) *best = B0;
+    if (n_evals) *n_evals = k;
+    return XC_OK;
+}
+
+int finish_scan(int mode, const std::vector<xc_angle_rec> &vC, xc_angle_rec *best, xc_angle_rec *recs, int cap,
+                int *n_recs)
+{
+    *best = scan_select(mode, vC);
+    if (recs && cap > 0) memcpy(recs, vC.data(), sizeof(xc_angle_rec) * std::min<size_t>(cap, vC.size()));
+    if (n_recs) *n_recs = (int)vC.size();
+    return XC_OK;
+}
+
+bool bad_scan(int mode, double step, const xc_angle_rec *best)
+{
+    return !best || (mode != XC_SCAN_MAXR && mode != XC_SCAN_CONSTXY) || !(step > 0.0);
+}
+
+}  // namespace
+
+extern "C" int xc_rfromangle_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                 const double *bv, int nb, double deg, int olap1D, int64_t req_area, int Ox, int Oy,
+                                 int Rx, int Ry, xc_angle_rec *rec)
+{
+    if (!ctx || !rec) return XC_E_ARG;
+    std::vector<xc_point> buf;
+    return rfa64(ctx, ap, av, na, bp, bv, nb, deg, scan_rules(olap1D, req_area), Ox, Oy, Rx, Ry, buf, rec);
+}
+
+extern "C" int xc_angle_scan_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                 const double *bv, int nb, int mode, double center, double hlfwid, double step,
+                                 int favor_center, int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                                 xc_angle_rec *best, xc_angle_rec *recs, int cap, int *n_recs)
+{
+    if (!ctx || bad_scan(mode, step, best)) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    const std::vector<double> ang = scan_angles(mode, center, hlfwid, step, favor_center != 0);
+    std::vector<xc_angle_rec> vC(ang.size());
+    std::vector<xc_point> buf;
+    for (size_t i = 0; i < ang.size(); ++i) {
+        const int st = rfa64(ctx, ap, av, na, bp, bv, nb, ang[i], ru, Ox, Oy, Rx, Ry, buf, &vC[i]);
+        if (st) return st;
+    }
+    return finish_scan(mode, vC, best, recs, cap, n_recs);
+}
+
+extern "C" int xc_angle_scan_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, int mode, double center,
+                                 double hlfwid, double step, int favor_center, int olap1D, int64_t req_area, int Ox,
+                                 int Oy, int Rx, int Ry, xc_angle_rec *best, xc_angle_rec *recs, int cap, int *n_recs)
+{
+    if (!ctx || !a || !b || bad_scan(mode, step, best)) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    const std::vector<double> ang = scan_angles(mode, center, hlfwid, step, favor_center != 0);
+    std::vector<xc_angle_rec> vC;
+    const int st = rfa32(ctx, a, b, ang, ru, Ox, Oy, Rx, Ry, vC);
+    if (st) return st;
+    return finish_scan(mode, vC, best, recs, cap, n_recs);
+}
+
+extern "C" int xc_peak_hunt_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                const double *bv, int nb, double hlfwid, int olap1D, int64_t req_area, int Ox, int Oy,
+                                int Rx, int Ry, xc_angle_rec *best, int *n_evals)
+{
+    if (!ctx || !best) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    std::vector<xc_point> buf;
+    return peak_hunt(hlfwid, best, n_evals, [&](double deg, xc_angle_rec *rec) {
+        return rfa64(ctx, ap, av, na, bp, bv, nb, deg, ru, Ox, Oy, Rx, Ry, buf, rec);
+    });
+}
+
+extern "C" int xc_peak_hunt_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double hlfwid, int olap1D,
+                                int64_t req_area, int Ox, int Oy, int Rx, int Ry, xc_angle_rec *best, int *n_evals)
+{
+    if (!ctx || !a || !b || !best) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    std::vector<xc_angle_rec> one;
+    return peak_hunt(hlfwid, best, n_evals, [&](double deg, xc_angle_rec *rec) {
+        const int st = rfa32(ctx, a, b, std::vector<double>(1, deg), ru, Ox, Oy, Rx, Ry, one);
+        if (!st) *rec = one[0];
+        return st;
+    });
+}

@@ -85,6 +85,21 @@ _OPTIONAL = {
     "xc_sweep_angles_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int, C.POINTER(XcRgnRule),
                              C.POINTER(XcCntRule), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(XcResult)],
                             C.c_int),
+    "xc_rfromangle_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_int64,
+                           C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_angle_scan_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_int, C.c_double, C.c_double,
+                           C.c_double, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                           C.POINTER(abi.XcAngleRec), C.POINTER(abi.XcAngleRec), C.c_int, C.POINTER(C.c_int)],
+                          C.c_int),
+    "xc_angle_scan_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                           C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int,
+                           C.POINTER(abi.XcAngleRec), C.POINTER(abi.XcAngleRec), C.c_int, C.POINTER(C.c_int)],
+                          C.c_int),
+    "xc_peak_hunt_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_int64,
+                          C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)],
+                         C.c_int),
+    "xc_peak_hunt_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int,
+                          C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
 }
 
 
@@ -228,6 +243,67 @@ class Context:
                                         Ox, Oy, Rx, Ry, res)
         self._check(st, "xc_sweep_angles_f32", ok=(abi.XC_OK,))
         return list(res)
+
+    # ---- CThmScan angle search (RFromAngle, AngleScanMaxR / AngleScanConstXY, PeakHunt) ----
+
+    def rfromangle(self, deg, ap, av, bp, bv, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """RFromAngle in FP64 on point lists. Returns XcAngleRec (A, R, X, Y)."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        rec = abi.XcAngleRec()
+        st = self.L.xc_rfromangle_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), deg, olap1d, req_area,
+                                      Ox, Oy, Rx, Ry, C.byref(rec))
+        self._check(st, "xc_rfromangle_f64", ok=(abi.XC_OK,))
+        return rec
+
+    def angle_scan(self, mode, ap, av, bp, bv, center, hlfwid, step, favor_center, olap1d, req_area,
+                   Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """AngleScanMaxR / AngleScanConstXY in FP64 on point lists. Returns (best, [recs])."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        best, n = abi.XcAngleRec(), C.c_int()
+        cap = 4096
+        recs = (abi.XcAngleRec * cap)()
+        st = self.L.xc_angle_scan_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), mode, center, hlfwid, step,
+                                      int(bool(favor_center)), olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(best),
+                                      recs, cap, C.byref(n))
+        self._check(st, "xc_angle_scan_f64", ok=(abi.XC_OK,))
+        return best, list(recs[:min(n.value, cap)])
+
+    def angle_scan_tiles(self, mode, a, b, center, hlfwid, step, favor_center, olap1d, req_area,
+                         Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """AngleScanMaxR / AngleScanConstXY over resident tiles, all angles in one FP32 sweep."""
+        best, n = abi.XcAngleRec(), C.c_int()
+        cap = 4096
+        recs = (abi.XcAngleRec * cap)()
+        st = self.L.xc_angle_scan_f32(self.h, a.h, b.h, mode, center, hlfwid, step, int(bool(favor_center)), olap1d,
+                                      req_area, Ox, Oy, Rx, Ry, C.byref(best), recs, cap, C.byref(n))
+        self._check(st, "xc_angle_scan_f32", ok=(abi.XC_OK,))
+        return best, list(recs[:min(n.value, cap)])
+
+    def peak_hunt(self, best, ap, av, bp, bv, hlfwid, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """PeakHunt in FP64 on point lists from `best` (XcAngleRec). Returns (best, k)."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        b, k = abi.XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+        st = self.L.xc_peak_hunt_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), hlfwid, olap1d, req_area,
+                                     Ox, Oy, Rx, Ry, C.byref(b), C.byref(k))
+        self._check(st, "xc_peak_hunt_f64", ok=(abi.XC_OK,))
+        return b, k.value
+
+    def peak_hunt_tiles(self, best, a, b, hlfwid, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """PeakHunt over resident tiles, each probe an FP32 sweep of one angle. Returns (best, k)."""
+        bb, k = abi.XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+        st = self.L.xc_peak_hunt_f32(self.h, a.h, b.h, hlfwid, olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(bb),
+                                     C.byref(k))
+        self._check(st, "xc_peak_hunt_f32", ok=(abi.XC_OK,))
+        return bb, k.value
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

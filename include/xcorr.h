@@ -310,6 +310,70 @@ int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
                         int Ox, int Oy, int Rx, int Ry, xc_result *results);
 
 /* ------------------------------------------------------------------ */
+/* Angle search around the rotation sweep (CThmScan)                   */
+/* ------------------------------------------------------------------ */
+
+/* One evaluated angle: CorRec (CThmScan.h:36-44) without its affine:
+ * A = angle (deg), R = CorrImagesS' score, X, Y = its displacement. */
+typedef struct xc_angle_rec {
+    double A, R, X, Y;
+} xc_angle_rec;
+
+enum xc_scan_mode {
+    XC_SCAN_MAXR    = 0,  /* AngleScanMaxR (CThmScan.cpp:431-505): largest R, the centre first */
+    XC_SCAN_CONSTXY = 1   /* AngleScanConstXY (:514-621): by decreasing R, the first angle whose
+                           * +-3 neighbours' rounded X and Y both fit a line with |lincor| >= 0.7 */
+};
+
+/*
+ * CThmScan::RFromAngle (CThmScan.cpp:329-366) in FP64: A's points rotated
+ * by deg (TAffine::NUSetRot with PI = 3.14159265358, Apply_R_Part), then
+ * CorrImagesS against B with BigEnough(olap1D, req_area) and
+ * EnoughPoints(req_area), oval (Ox, Oy, Rx, Ry).  rec->R = 0 and X = Y = 0
+ * when CorrImagesS finds no peak.  Returns XC_OK or an error.
+ */
+int xc_rfromangle_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                      const xc_point *bp, const double *bv, int nb, double deg,
+                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                      xc_angle_rec *rec);
+
+/*
+ * AngleScanMaxR / AngleScanConstXY (mode): RFromAngle over
+ * a = center - hlfwid, += step, while a <= center + hlfwid; MAXR with
+ * favor_center first evaluates the centre and a band of step/3 within
+ * 2 step of it (and of centre + 180 when hlfwid >= 180).  *best receives
+ * the selection (CONSTXY: R = A = X = Y = 0 when no angle qualifies; exact
+ * ties of R keep the earlier angle).  recs (may be NULL) receives up to cap
+ * evaluated angles in evaluation order, *n_recs their count.
+ * The f64 form evaluates each angle with xc_rfromangle_f64 (the
+ * reference's arithmetic); the f32 form evaluates all angles in one
+ * batched sweep of tiles (xc_sweep_angles_f32).
+ */
+int xc_angle_scan_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                      const xc_point *bp, const double *bv, int nb, int mode,
+                      double center, double hlfwid, double step, int favor_center,
+                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                      xc_angle_rec *best, xc_angle_rec *recs, int cap, int *n_recs);
+int xc_angle_scan_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, int mode,
+                      double center, double hlfwid, double step, int favor_center,
+                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                      xc_angle_rec *best, xc_angle_rec *recs, int cap, int *n_recs);
+
+/*
+ * CThmScan::PeakHunt (CThmScan.cpp:676-732): bracket search for the best
+ * angle within best->A +- hlfwid, probing the left then the right half
+ * until the bracket is narrower than 1e-4 deg; the entry *best is kept if
+ * nothing beats it.  *n_evals receives the reference's probe counter k.
+ */
+int xc_peak_hunt_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                     const xc_point *bp, const double *bv, int nb, double hlfwid,
+                     int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                     xc_angle_rec *best, int *n_evals);
+int xc_peak_hunt_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double hlfwid,
+                     int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                     xc_angle_rec *best, int *n_evals);
+
+/* ------------------------------------------------------------------ */
 /* Serving loop: continuous batching of pair requests                 */
 /* ------------------------------------------------------------------ */
 

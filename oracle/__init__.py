@@ -18,7 +18,7 @@ import subprocess
 
 import numpy as np
 
-from alignment_projects_amd.abi import XcCntRule, XcPoint, XcResult, XcRgnRule
+from alignment_projects_amd.abi import XcAngleRec, XcCntRule, XcPoint, XcResult, XcRgnRule
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -65,6 +65,15 @@ def lib():
         L.xo_set_fft_provider.argtypes = [C.c_int, C.c_char_p]
         L.xo_fft_provider_name.restype = C.c_char_p
         L.xo_rgn_ok.argtypes = [C.POINTER(XcRgnRule), C.c_int, C.c_int]
+        L.xo_linefit_lincor.argtypes = [_dp, _dp, C.c_int]
+        L.xo_linefit_lincor.restype = C.c_double
+        L.xo_scan_angles.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp, C.c_int]
+        L.xo_scan_select.argtypes = [C.c_int, C.POINTER(XcAngleRec), C.c_int, C.POINTER(XcAngleRec)]
+        L.xo_angle_scan.argtypes = [C.c_int, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_int, C.c_int, C.c_long, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(XcAngleRec), C.POINTER(XcAngleRec), C.c_int, C.POINTER(C.c_int)]
+        L.xo_peak_hunt.argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_long,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(XcAngleRec), C.POINTER(C.c_int)]
         L.xo_convolve.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_fpeak.argtypes = [_dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_double, _ip, _ip]
         _lib = L
@@ -296,3 +305,54 @@ def set_fft_provider(which):
 
 def provider_name():
     return lib().xo_fft_provider_name().decode()
+
+
+# ---- CThmScan angle search ----
+
+def linefit_lincor(x, y):
+    """LineFit's correlation coefficient (0_GEN/Maths.cpp:457-507)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    return lib().xo_linefit_lincor(x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), x.size)
+
+
+def scan_angles(mode, center, hlfwid, step, favor):
+    """The angle list of AngleScanMaxR / AngleScanConstXY (0_GEN/CThmScan.cpp:444-470, 543-544)."""
+    n = lib().xo_scan_angles(mode, center, hlfwid, step, int(bool(favor)), None, 0)
+    a = np.zeros(max(n, 1))
+    lib().xo_scan_angles(mode, center, hlfwid, step, int(bool(favor)), a.ctypes.data_as(_dp), n)
+    return a[:n]
+
+
+def scan_select(mode, recs):
+    """The selection of AngleScanMaxR / AngleScanConstXY over (A, R, X, Y) records."""
+    arr = (XcAngleRec * len(recs))(*[XcAngleRec(*r) for r in recs])
+    best = XcAngleRec()
+    lib().xo_scan_select(mode, arr, len(recs), C.byref(best))
+    return best
+
+
+def angle_scan(mode, ap, av, bp, bv, center, hlfwid, step, favor, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """AngleScanMaxR / AngleScanConstXY (0_GEN/CThmScan.cpp:431-621) over RFromAngle. Returns (best, recs)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    cap = 4096
+    recs = (XcAngleRec * cap)()
+    best, n = XcAngleRec(), C.c_int()
+    lib().xo_angle_scan(mode, pp1, pv1, len(v1), pp2, pv2, len(v2), center, hlfwid, step, int(bool(favor)), olap1d,
+                        req_area, Ox, Oy, Rx, Ry, C.byref(best), recs, cap, C.byref(n))
+    return best, list(recs[:min(n.value, cap)])
+
+
+def peak_hunt(best, ap, av, bp, bv, hlfwid, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """PeakHunt (0_GEN/CThmScan.cpp:676-732). Returns (best, k)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    b, k = XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+    lib().xo_peak_hunt(pp1, pv1, len(v1), pp2, pv2, len(v2), hlfwid, olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(b),
+                       C.byref(k))
+    return b, k.value

@@ -1105,3 +1105,153 @@ int xo_decimate_vector(xc_point *p, double *v, int np, int w, int h, int lbin)
     free(cr);
     return k;
 }
+
+/* ------------------------------------------------------------------ */
+/* CThmScan angle search                                               */
+/* ------------------------------------------------------------------ */
+
+/* LineFit (Maths.cpp:457-507): lincor = r / sqrt(varX varY) of the least-
+ * squares line through (x[i], y[i]), i < n (NaN or Inf for a zero variance). */
+double xo_linefit_lincor(const double *x, const double *y, int n)
+{
+    double r, varX, varY, nn = 0.0, sumX = 0.0, sumY = 0.0, sumXX = 0.0, sumYY = 0.0, sumXY = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double X = x[i], Y = y[i];
+        sumX += X;
+        sumY += Y;
+        sumXX += X * X;
+        sumYY += Y * Y;
+        sumXY += X * Y;
+        ++nn;
+    }
+    r = nn * sumXY - sumX * sumY;
+    varX = nn * sumXX - sumX * sumX;
+    varY = nn * sumYY - sumY * sumY;
+    return r / sqrt(varX * varY);
+}
+
+/* ROUND (Maths.h:19). */
+static double round_ref(double x) { return (double)(x >= 0.0 ? (int)(x + 0.5) : (int)(x - 0.5)); }
+
+/* The angle lists of AngleScanMaxR / AngleScanConstXY (CThmScan.cpp:444-470,
+ * 543-544).  Returns the count; a (may be NULL) receives the angles. */
+int xo_scan_angles(int mode, double center, double hlfwid, double step, int favor, double *a_out, int cap)
+{
+    int n = 0;
+#define PUSH(v) do { if (a_out && n < cap) a_out[n] = (v); ++n; } while (0)
+    if (mode == XC_SCAN_MAXR) {
+        PUSH(center);
+        if (favor) {
+            double cwide = 2.0 * step, cstep = step / 3.0;
+            for (double a = center - cwide; a <= center + cwide; a += cstep) PUSH(a);
+            if (hlfwid >= 180) {
+                double opc = center + 180;
+                for (double a = opc - cwide; a <= opc + cwide; a += cstep) PUSH(a);
+            }
+        }
+    }
+    for (double a = center - hlfwid; a <= center + hlfwid; a += step) PUSH(a);
+#undef PUSH
+    return n;
+}
+
+/* The selections of AngleScanMaxR (:476-487) and AngleScanConstXY
+ * (:554-613) over evaluated records vC[nC].  ConstXY's sort by decreasing R
+ * keeps exact ties in index order (std::sort leaves them unspecified). */
+void xo_scan_select(int mode, const xc_angle_rec *vC, int nC, xc_angle_rec *best)
+{
+    if (mode == XC_SCAN_MAXR) {
+        *best = vC[0];
+        for (int ic = 1; ic < nC; ++ic)
+            if (vC[ic].R > best->R) *best = vC[ic];
+        return;
+    }
+    best->X = best->Y = best->A = best->R = 0.0;
+    int *order = (int *)malloc(sizeof(int) * (nC ? nC : 1));
+    for (int i = 0; i < nC; ++i) order[i] = i;
+    for (int i = 1; i < nC; ++i) {  /* stable insertion sort by decreasing R */
+        int v = order[i], j = i - 1;
+        while (j >= 0 && vC[order[j]].R < vC[v].R) { order[j + 1] = order[j]; --j; }
+        order[j + 1] = v;
+    }
+    const double rthresh = 0.7;
+    const int m = 3, M = 2 * m + 1;
+    for (int i = 0; i < nC; ++i) {
+        int ic = order[i];
+        if (ic < m) continue;
+        if (nC - 1 - ic < m) continue;
+        double A[7], X[7], Y[7], lincor;
+        int n = 0;
+        for (int j = ic - m; j <= ic + m; ++j) {
+            A[n] = vC[j].A;
+            X[n] = round_ref(vC[j].X);
+            Y[n] = round_ref(vC[j].Y);
+            ++n;
+        }
+        (void)M;
+        lincor = xo_linefit_lincor(A, X, n);
+        if (isfinite(lincor) && fabs(lincor) < rthresh) continue;
+        lincor = xo_linefit_lincor(A, Y, n);
+        if (isfinite(lincor) && fabs(lincor) < rthresh) continue;
+        *best = vC[ic];
+        break;
+    }
+    free(order);
+}
+
+static void rfa_rec(double deg, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                    int nb, int olap1D, long reqArea, int Ox, int Oy, int Rx, int Ry, xc_angle_rec *C)
+{
+    C->A = deg;
+    xo_rfromangle(deg, ap, av, na, bp, bv, nb, olap1D, reqArea, Ox, Oy, Rx, Ry, &C->X, &C->Y, &C->R, NULL);
+}
+
+/* AngleScanMaxR / AngleScanConstXY (CThmScan.cpp:431-621) over RFromAngle. */
+int xo_angle_scan(int mode, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                  int nb, double center, double hlfwid, double step, int favor, int olap1D, long reqArea, int Ox,
+                  int Oy, int Rx, int Ry, xc_angle_rec *best, xc_angle_rec *recs, int cap, int *nrec)
+{
+    int nC = xo_scan_angles(mode, center, hlfwid, step, favor, NULL, 0);
+    double *a = (double *)malloc(sizeof(double) * (nC ? nC : 1));
+    xc_angle_rec *vC = (xc_angle_rec *)malloc(sizeof(xc_angle_rec) * (nC ? nC : 1));
+    xo_scan_angles(mode, center, hlfwid, step, favor, a, nC);
+    for (int i = 0; i < nC; ++i) rfa_rec(a[i], ap, av, na, bp, bv, nb, olap1D, reqArea, Ox, Oy, Rx, Ry, &vC[i]);
+    if (nC > 0) xo_scan_select(mode, vC, nC, best);
+    else best->X = best->Y = best->A = best->R = 0.0;
+    if (recs) memcpy(recs, vC, sizeof(xc_angle_rec) * (nC < cap ? nC : cap));
+    if (nrec) *nrec = nC;
+    free(a);
+    free(vC);
+    return XC_OK;
+}
+
+/* PeakHunt (CThmScan.cpp:676-732). */
+int xo_peak_hunt(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
+                 double hlfwid, int olap1D, long reqArea, int Ox, int Oy, int Rx, int Ry, xc_angle_rec *best, int *kout)
+{
+    xc_angle_rec C, B0 = *best;
+    double L = best->A - hlfwid, R = best->A + hlfwid, M;
+    int k = 1;
+    rfa_rec(M = (L + R) / 2.0, ap, av, na, bp, bv, nb, olap1D, reqArea, Ox, Oy, Rx, Ry, best);
+    while (R - L > 0.0001) {
+        double a;
+        ++k;
+        rfa_rec(a = (L + M) / 2.0, ap, av, na, bp, bv, nb, olap1D, reqArea, Ox, Oy, Rx, Ry, &C);
+        if (C.R >= best->R) {
+            R = M;
+            *best = C;
+            M = a;
+        } else
+            L = a;
+        rfa_rec(a = (M + R) / 2.0, ap, av, na, bp, bv, nb, olap1D, reqArea, Ox, Oy, Rx, Ry, &C);
+        if (C.R >= best->R) {
+            L = M;
+            *best = C;
+            M = a;
+        } else
+            R = a;
+    }
+    if (B0.R > best->R) *best = B0;
+    if (kout) *kout = k;
+    return XC_OK;
+}

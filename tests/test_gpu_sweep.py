@@ -53,3 +53,80 @@ def test_sweep_rotated_splat_is_deterministic(ctx):
         assert (a.r, a.dx, a.dy, a.lag_x, a.lag_y) == (b.r, b.dx, b.dy, b.lag_x, b.lag_y)
     ta.release()
     tb.release()
+
+
+def _scan_pair(seed, deg, size=96):
+    A, B = synth.rotated_pair(seed, size=size, deg=deg, shift=(2.0, -1.0))
+    A = synth.normalize(A.ravel()).reshape(A.shape)
+    B = synth.normalize(B.ravel()).reshape(B.shape)
+    return A, B, size // 4, size * size // 4
+
+
+@pytest.mark.parametrize("mode", [abi.XC_SCAN_CONSTXY, abi.XC_SCAN_MAXR])
+def test_angle_scan_f64_matches_oracle(ctx, mode):
+    """AngleScan on point lists in FP64 (RFromAngle per angle on the device) against the oracle:
+    every evaluated angle's R within 1e-9, X and Y within 1e-9 px, the same selection."""
+    A, B, olap, req = _scan_pair(21, 2.0)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    g, grecs = ctx.angle_scan(mode, pa, va, pb, vb, 0.0, 5.0, 0.5, True, olap, req)
+    o, orecs = oracle.angle_scan(mode, pa, va, pb, vb, 0.0, 5.0, 0.5, True, olap, req)
+    assert len(grecs) == len(orecs)
+    for a, b in zip(grecs, orecs):
+        assert a.A == b.A
+        assert abs(a.R - b.R) <= 1e-9 * max(1.0, abs(b.R)), (a.A, a.R, b.R)
+        assert abs(a.X - b.X) <= 1e-9 and abs(a.Y - b.Y) <= 1e-9
+    assert g.A == o.A and abs(g.R - o.R) <= 1e-9 * max(1.0, abs(o.R))
+
+
+def test_peak_hunt_f64_matches_oracle(ctx):
+    """PeakHunt in FP64: the same probes and decisions as the oracle (R agree to 1e-9)."""
+    A, B, olap, req = _scan_pair(22, -1.5)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    start = abi.XcAngleRec(-1.0, 0.0, 0.0, 0.0)
+    g, gk = ctx.peak_hunt(start, pa, va, pb, vb, 0.5, olap, req)
+    o, ok = oracle.peak_hunt(start, pa, va, pb, vb, 0.5, olap, req)
+    assert gk == ok
+    assert abs(g.A - o.A) <= 1e-12 and abs(g.R - o.R) <= 1e-9 * max(1.0, abs(o.R))
+    assert abs(g.X - o.X) <= 1e-9 and abs(g.Y - o.Y) <= 1e-9
+
+
+@pytest.mark.parametrize("mode", [abi.XC_SCAN_CONSTXY, abi.XC_SCAN_MAXR])
+def test_angle_scan_tiles_f32(ctx, mode):
+    """AngleScan over resident tiles (all angles in one FP32 sweep) against the FP64 oracle:
+    R within 1e-3 relative, X and Y within 0.05 px, the same selected angle."""
+    A, B, olap, req = _scan_pair(23, 2.5)
+    ta = ctx.tile_put(A, normalize=False)
+    tb = ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    g, grecs = ctx.angle_scan_tiles(mode, ta, tb, 0.0, 5.0, 0.5, True, olap, req)
+    o, orecs = oracle.angle_scan(mode, pa, va, pb, vb, 0.0, 5.0, 0.5, True, olap, req)
+    assert len(grecs) == len(orecs)
+    for a, b in zip(grecs, orecs):
+        assert a.A == b.A
+        assert abs(a.R - b.R) <= 1e-3 * max(1.0, abs(b.R)), (a.A, a.R, b.R)
+        assert abs(a.X - b.X) <= 0.05 and abs(a.Y - b.Y) <= 0.05
+    assert g.A == o.A
+    ta.release()
+    tb.release()
+
+
+def test_peak_hunt_tiles_f32(ctx):
+    """PeakHunt over tiles, each probe an FP32 sweep of one angle: near the optimum the probes' R
+    differ by less than FP32 resolves, so the bracket may close elsewhere in the flat top; the result
+    must be within 0.05 deg of the FP64 oracle's and as good to 1e-4."""
+    A, B, olap, req = _scan_pair(24, 1.0)
+    ta = ctx.tile_put(A, normalize=False)
+    tb = ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    start = abi.XcAngleRec(1.5, 0.0, 0.0, 0.0)
+    g, gk = ctx.peak_hunt_tiles(start, ta, tb, 0.5, olap, req)
+    o, ok = oracle.peak_hunt(start, pa, va, pb, vb, 0.5, olap, req)
+    assert abs(g.A - o.A) <= 0.05, (g.A, o.A)
+    assert abs(g.R - o.R) <= 1e-4 * max(1.0, abs(o.R)), (g.R, o.R)
+    assert 10 <= gk <= 16
+    ta.release()
+    tb.release()

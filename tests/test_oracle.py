@@ -375,3 +375,76 @@ def test_corr_images_f_vs_bruteforce(seed):
         assert abs(dx - bdx) <= 1e-9 and abs(dy - bdy) <= 1e-9
     else:
         assert st == abi.XC_NO_PEAK and (r, dx, dy) == (0.0, 0.0, 0.0)
+
+
+# ---- CThmScan angle search: LineFit, angle lists, AngleScan selections, PeakHunt ----
+
+def test_linefit_lincor_closed_form():
+    x = np.arange(7.0)
+    assert abs(oracle.linefit_lincor(x, 3 * x - 2) - 1.0) < 1e-15
+    assert abs(oracle.linefit_lincor(x, -0.5 * x + 9) + 1.0) < 1e-15
+    assert np.isnan(oracle.linefit_lincor(x, np.full(7, 4.0)))  # 0 / 0: a stationary coordinate
+    rng = np.random.default_rng(3)
+    for _ in range(10):
+        a, b = rng.standard_normal((2, 7))
+        assert abs(oracle.linefit_lincor(a, b) - np.corrcoef(a, b)[0, 1]) < 1e-12
+
+
+def test_scan_angle_lists():
+    # for (a = center - hlfwid; a <= center + hlfwid; a += step): 21 angles from -5 by 0.5
+    a = oracle.scan_angles(abi.XC_SCAN_CONSTXY, 0.0, 5.0, 0.5, False)
+    assert a.size == 21 and a[0] == -5.0 and abs(a[-1] - 5.0) < 1e-12
+    assert np.array_equal(a, np.array(synth.sweep_angles(-5.0, 5.0, 0.5)))
+    # MaxR: the centre first, then (favorCenter) 13 angles within 2 steps by step/3, then the sweep
+    b = oracle.scan_angles(abi.XC_SCAN_MAXR, 10.0, 5.0, 0.5, True)
+    assert b[0] == 10.0 and b.size == 1 + 13 + 21 and abs(b[1] - 9.0) < 1e-12 and abs(b[13] - 11.0) < 1e-9
+    # a full circle adds the opposite band about centre + 180
+    c = oracle.scan_angles(abi.XC_SCAN_MAXR, 0.0, 180.0, 30.0, True)
+    assert c.size == 1 + 13 + 13 + 13 and abs(c[14] - 120.0) < 1e-9
+
+
+def test_scan_select_cases():
+    # MaxR: largest R, exact ties keep the earlier record, the centre (first) included
+    recs = [(0.0, 0.5, 0, 0), (1.0, 0.7, 1, 1), (2.0, 0.7, 2, 2), (3.0, 0.6, 3, 3)]
+    b = oracle.scan_select(abi.XC_SCAN_MAXR, recs)
+    assert (b.A, b.R) == (1.0, 0.7)
+    # ConstXY: 11 angles; X, Y drift linearly with the angle except where noted
+    A = np.arange(11.0)
+    R = np.array([.1, .2, .3, .4, .5, .6, .95, .9, .8, .2, .99])
+    X = 10 + 2 * A
+    Y = -3 + A
+    recs = [(A[i], R[i], X[i], Y[i]) for i in range(11)]
+    # the top R (index 10) has no 3 neighbours to its right: skipped; index 6 qualifies
+    b = oracle.scan_select(abi.XC_SCAN_CONSTXY, recs)
+    assert (b.A, b.R) == (6.0, 0.95)
+    # jitter X around index 6 (alternating +-20 px): |lincor| < 0.7 -> next best (index 7) is tried;
+    # its window [4, 10] holds the same jitter, so it fails too; index 8's window [5, 11) runs off
+    # the end; index 5's window [2, 8] is jittered as well -> nothing qualifies
+    Xj = X.copy()
+    Xj[3:10] += np.array([20, -20, 20, -20, 20, -20, 20])
+    recs = [(A[i], R[i], Xj[i], Y[i]) for i in range(11)]
+    b = oracle.scan_select(abi.XC_SCAN_CONSTXY, recs)
+    assert (b.A, b.R, b.X, b.Y) == (0.0, 0.0, 0.0, 0.0)
+    # a stationary X (all equal after rounding: lincor NaN) is accepted, as the reference notes
+    recs = [(A[i], R[i], 5.2, Y[i]) for i in range(11)]
+    b = oracle.scan_select(abi.XC_SCAN_CONSTXY, recs)
+    assert (b.A, b.R) == (6.0, 0.95)
+
+
+def test_angle_scan_and_peak_hunt_recover_rotation():
+    """ConstXY and MaxR scans pick an angle next to the true rotation; PeakHunt refines it
+    (the true rotation is what the pair was made with)."""
+    deg = 2.0
+    A, B = synth.rotated_pair(11, size=96, deg=deg, shift=(2.0, -1.0))
+    A = synth.normalize(A.ravel()).reshape(A.shape)
+    B = synth.normalize(B.ravel()).reshape(B.shape)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    for mode in (abi.XC_SCAN_CONSTXY, abi.XC_SCAN_MAXR):
+        best, recs = oracle.angle_scan(mode, pa, va, pb, vb, 0.0, 5.0, 0.5, True, olap, req)
+        assert abs(best.A - deg) <= 0.5 and best.R > 0
+        assert best.R == max(r.R for r in recs) or mode == abi.XC_SCAN_CONSTXY
+    hunted, k = oracle.peak_hunt(best, pa, va, pb, vb, 0.5, olap, req)
+    # the bracket halves twice per round from 1 deg to < 1e-4 deg; the result is never worse than the start
+    assert abs(hunted.A - deg) <= 0.5 and hunted.R >= best.R and 10 <= k <= 16
