@@ -460,13 +460,13 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::vector<int> starts;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
-        const int G = row_group<float>(B.NH), C = col_group<float>(B.NY);
+        const int G = row_group<float>(B.NH), C = col_group<float>(B.NY), GR = r2c_rows<float>(B.NH);
         int nr = 0, nc = 0, nc2 = 0;
         for (int k = B.first; k < B.first + B.count; ++k) {
             const int i = p->order[k];
             const PairDev &D = P[i];
-            if (ownA[i]) nr += (D.h1 + G - 1) / G;
-            if (ownB[i]) nr += (D.h2 + G - 1) / G;
+            if (ownA[i]) nr += (D.h1 + GR - 1) / GR;
+            if (ownB[i]) nr += (D.h2 + GR - 1) / GR;
             nc += (D.Nx / 2 + 1 + C - 1) / C;
             live_row_groups(D, G, starts);
             nc2 += (int)starts.size();
@@ -558,7 +558,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     }
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
-        const int G = row_group<float>(B.NH), C = col_group<float>(B.NY);
+        const int G = row_group<float>(B.NH), C = col_group<float>(B.NY), GR = r2c_rows<float>(B.NH);
         int4 *wr = (int4 *)hat(i_wr[bi]);
         int2 *wc = (int2 *)hat(i_wc[bi]);
         int2 *wc2 = (int2 *)hat(i_wc2[bi]);
@@ -567,9 +567,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             const int i = p->order[k];
             const PairDev &D = hp[k];
             if (ownA[i])
-                for (int y = 0; y < D.h1; y += G) wr[a++] = make_int4(k, 0, y, 0);
+                for (int y = 0; y < D.h1; y += GR) wr[a++] = make_int4(k, 0, y, 0);
             if (ownB[i])
-                for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
+                for (int y = 0; y < D.h2; y += GR) wr[a++] = make_int4(k, 1, y, 0);
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
             live_row_groups(D, G, starts);
             for (int y : starts) wc2[d++] = make_int2(k, y);

@@ -266,6 +266,85 @@ __device__ __forceinline__ void fft(cpx<T> (&v)[Plan<N>::E], int t, LdsView<T> l
     fft_pass<DIR, N, T, 0, TwLazy<T>>(v, t, lds, TW);
 }
 
+// RR independent transforms per thread, interleaved pass by pass: the passes'
+// barriers and twiddles are shared, and the RR butterflies of a pass give the
+// scheduler independent work.  Each transform has its own LDS view.
+template <int DIR, int N, typename T, int RR, int P = 0, typename TWS>
+__device__ __forceinline__ void fft_pass_rows(cpx<T> (&v)[RR][Plan<N>::E], int t, const LdsView<T> (&lds)[RR],
+                                              const TWS &TW)
+{
+    using PL = Plan<N>;
+    if constexpr (P < PL::NP) {
+        constexpr int E = PL::E, TPF = PL::TPF;
+        constexpr int R = PL::radix(P);
+        constexpr int NB = E / R;
+        constexpr int Ns = PL::stride_before(P);
+        constexpr int OFF = TwPlan<N>::count(P);
+        if constexpr (P > 0) {
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < RR; ++q)
+#pragma unroll
+                for (int m = 0; m < E; ++m) v[q][m] = lds[q][t + TPF * m];
+        }
+        cpx<T> u[RR][NB][R];
+        static_for<0, NB>([&](auto bI) {
+            constexpr int b = decltype(bI)::value;
+            static_for<0, R>([&](auto rI) {
+                constexpr int r = decltype(rI)::value;
+                if constexpr (P > 0 && r > 0) {
+                    const cpx<T> w = tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t);
+#pragma unroll
+                    for (int q = 0; q < RR; ++q) u[q][b][r] = cmul(v[q][b + r * NB], w);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < RR; ++q) u[q][b][r] = v[q][b + r * NB];
+                }
+            });
+#pragma unroll
+            for (int q = 0; q < RR; ++q) bfly<DIR, R>(u[q][b]);
+        });
+        if constexpr (P + 1 == PL::NP) {
+#pragma unroll
+            for (int q = 0; q < RR; ++q)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) v[q][b + r * NB] = u[q][b][r];
+        } else {
+            __syncthreads();  // everyone finished reading the previous contents
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int j = t + TPF * b;
+                const int k = j & (Ns - 1);
+                const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+                for (int q = 0; q < RR; ++q)
+#pragma unroll
+                    for (int r = 0; r < R; ++r) lds[q][base + r * Ns] = u[q][b][r];
+            }
+            fft_pass_rows<DIR, N, T, RR, P + 1, TWS>(v, t, lds, TW);
+        }
+    }
+}
+
+// RR table-twiddle transforms; `upper_zero`: registers m >= E/2 are zero padding
+// (constants, so the first pass drops their terms).
+template <int DIR, int N, typename T, int RR, bool UPPER_ZERO>
+__device__ __forceinline__ void fft_rows(cpx<T> (&v)[RR][Plan<N>::E], int t, const LdsView<T> (&lds)[RR],
+                                         const cpx<T> *__restrict__ tw)
+{
+    if constexpr (UPPER_ZERO) {
+        constexpr int E = Plan<N>::E;
+#pragma unroll
+        for (int q = 0; q < RR; ++q)
+#pragma unroll
+            for (int m = E / 2; m < E; ++m) v[q][m] = cpx<T>{(T)0, (T)0};
+    }
+    const TwLazy<T> TW{tw};
+    fft_pass_rows<DIR, N, T, RR, 0, TwLazy<T>>(v, t, lds, TW);
+}
+
 // Forward or inverse transform of data whose registers m >= E/2 are zero
 // (the upper half of the input is padding).  The zeros are constants here,
 // so the first pass's butterflies lose their dead terms.

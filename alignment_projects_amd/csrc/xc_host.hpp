@@ -130,6 +130,7 @@ template <typename T>
 int launch_row_c2r(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int2 *work, bool cnt = false);
 template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd);
 template <typename T> int row_group(int NH);   // rows per workgroup of the row kernels
+template <typename T> int r2c_rows(int NH);    // rows per work item of the row r2c
 template <typename T> int col_group(int NY);   // columns per work item of the column kernel
 
 template <typename T> const cpx<T> *twiddles(xc_ctx *ctx);

@@ -936,59 +936,77 @@ template <int NH> struct RowCfg {
     static constexpr int THREADS = TPF * G;
 };
 
+// Row r2c shape: RowCfg<NH>'s thread groups, each transforming RPT rows at
+// once (FP32 rows of >= 2048 points: two, sharing barriers and twiddles).
+template <typename T, int NH> struct R2cCfg {
+    static constexpr int RPT = (sizeof(T) == 4 && Plan<NH>::TPF >= 256) ? 2 : 1;
+    static constexpr int ROWS = RowCfg<NH>::G * RPT;  // rows per workgroup (per work item)
+};
+
 template <typename T, int NH>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     k_row_r2c(const PairDev *__restrict__ pairs, const int4 *__restrict__ work, const cpx<T> *__restrict__ tw)
 {
     using PL = Plan<NH>;
-    constexpr int E = PL::E, TPF = PL::TPF;
+    constexpr int E = PL::E, TPF = PL::TPF, RPT = R2cCfg<T, NH>::RPT;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *smem = reinterpret_cast<cpx<T> *>(smem_raw);
     const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
     const int4 wi = work[blockIdx.x];  // {pair, image, y0, -}
     const PairDev &P = pairs[wi.x];
-    const int img = wi.y, y = wi.z + g;
+    const int img = wi.y, y0 = wi.z + g * RPT;
     const ImgView &V = img ? P.B : P.A;
     cpx<T> *dst = reinterpret_cast<cpx<T> *>(img ? P.specB : P.specA);
-    const bool valid = y < V.h;
     const int H = NH + 1;
     const double mean = V.msd[0], inv = V.msd[2];
 
-    cpx<T> v[E];
-    LdsView<T> L{smem + g * lds_len(NH), 1};
-#define XC_LOAD_ROW(ME)                                                                           \
-    switch (V.dtype) { /* uniform per workgroup */                                                 \
-    case XC_U8: load_row_pairs<T, uint8_t, ME, TPF>(v, V, mean, inv, t, y, valid); break;         \
-    case XC_U16: load_row_pairs<T, uint16_t, ME, TPF>(v, V, mean, inv, t, y, valid); break;       \
-    case XC_F32: load_row_pairs<T, float, ME, TPF>(v, V, mean, inv, t, y, valid); break;          \
-    default: load_row_pairs<T, double, ME, TPF>(v, V, mean, inv, t, y, valid); break;             \
+    cpx<T> v[RPT][E];
+    LdsView<T> L[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) L[q] = LdsView<T>{smem + (g * RPT + q) * lds_len(NH), 1};
+#define XC_LOAD_ROW(ME)                                                                                    \
+    _Pragma("unroll") for (int q = 0; q < RPT; ++q) {                                                       \
+        const int y = y0 + q;                                                                               \
+        const bool valid = y < V.h;                                                                         \
+        switch (V.dtype) { /* uniform per workgroup */                                                      \
+        case XC_U8: load_row_pairs<T, uint8_t, ME, TPF>(v[q], V, mean, inv, t, y, valid); break;            \
+        case XC_U16: load_row_pairs<T, uint16_t, ME, TPF>(v[q], V, mean, inv, t, y, valid); break;          \
+        case XC_F32: load_row_pairs<T, float, ME, TPF>(v[q], V, mean, inv, t, y, valid); break;             \
+        default: load_row_pairs<T, double, ME, TPF>(v[q], V, mean, inv, t, y, valid); break;                \
+        }                                                                                                   \
     }
     if (E == 8 && V.w <= NH) {
         // x >= NH (registers m >= E/2) is padding for every row of this workgroup
         XC_LOAD_ROW(E / 2)
-        fft_upper_zero<-1, NH>(v, t, L, tw);
+        fft_rows<-1, NH, T, RPT, true>(v, t, L, tw);
     } else {
         XC_LOAD_ROW(E)
-        fft<-1, NH>(v, t, L, tw);
+        fft_rows<-1, NH, T, RPT, false>(v, t, L, tw);
     }
 #undef XC_LOAD_ROW
 
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < E; ++m) L[t + TPF * m] = v[m];
-    __syncthreads();
-    if (valid) {
-        cpx<T> *row = dst + (long long)y * H;
+    for (int q = 0; q < RPT; ++q)
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const int k = t + TPF * m;
-            const cpx<T> zk = v[m];
-            const cpx<T> zn = cconj(L[(NH - k) & (NH - 1)]);
-            const cpx<T> a = cscale(zk + zn, (T)0.5);
-            const cpx<T> b = rot90<-1>(cscale(zk - zn, (T)0.5));
-            row[k] = a + cmul(twiddle<-1>(tw, k, 2 * NH), b);
+        for (int m = 0; m < E; ++m) L[q][t + TPF * m] = v[q][m];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int y = y0 + q;
+        if (y < V.h) {
+            cpx<T> *row = dst + (long long)y * H;
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int k = t + TPF * m;
+                const cpx<T> zk = v[q][m];
+                const cpx<T> zn = cconj(L[q][(NH - k) & (NH - 1)]);
+                const cpx<T> a = cscale(zk + zn, (T)0.5);
+                const cpx<T> b = rot90<-1>(cscale(zk - zn, (T)0.5));
+                row[k] = a + cmul(twiddle<-1>(tw, k, 2 * NH), b);
+            }
+            if (t == 0) row[NH] = cpx<T>{v[q][0].x - v[q][0].y, (T)0};
         }
-        if (t == 0) row[NH] = cpx<T>{v[0].x - v[0].y, (T)0};
     }
 }
 

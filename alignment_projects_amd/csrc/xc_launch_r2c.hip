@@ -16,6 +16,17 @@ template <typename T> int row_group(int NH)
     return 1;
 }
 
+template <typename T> int r2c_rows(int NH)
+{
+    switch (NH) {
+#define F(n) \
+    case n: return R2cCfg<T, n>::ROWS;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return 1;
+}
+
 template <typename T> int col_group(int) { return COL_STEP; }
 
 template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
@@ -23,7 +34,7 @@ template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDe
     if constexpr (NH > max_nh<T>()) {
         return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
     } else {
-        const size_t smem = sizeof(cpx<T>) * RowCfg<NH>::G * lds_len(NH);
+        const size_t smem = sizeof(cpx<T>) * R2cCfg<T, NH>::ROWS * lds_len(NH);
         hipLaunchKernelGGL((k_row_r2c<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
                            twiddles<T>(ctx));
         XC_HIP(ctx, hipGetLastError());
@@ -57,6 +68,8 @@ template int launch_finalize<double>(xc_ctx *, int, const PairDev *);
 template int launch_finalize<float>(xc_ctx *, int, const PairDev *);
 template int row_group<double>(int);
 template int row_group<float>(int);
+template int r2c_rows<double>(int);
+template int r2c_rows<float>(int);
 template int col_group<double>(int);
 template int col_group<float>(int);
 

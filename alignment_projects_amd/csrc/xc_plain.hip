@@ -69,8 +69,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     // ---- workspace layout ----
     const size_t n = (size_t)n1 + n2;
     const size_t tbl = (rgn && rgn->enabled && rgn->table) ? (size_t)rgn->table_w * rgn->table_h : 0;
-    const int G = row_group<double>(NH), C = col_group<double>(NY);
-    const int nr2c = (h1 + G - 1) / G + (h2 + G - 1) / G;
+    const int G = row_group<double>(NH), C = col_group<double>(NY), GR = r2c_rows<double>(NH);
+    const int nr2c = (h1 + GR - 1) / GR + (h2 + GR - 1) / GR;
     const int ncol = (H + C - 1) / C;
     const int nc2r = (P.hR + G - 1) / G;
     const TileSizes t1 = tile_sizes(w1, h1, true), t2 = tile_sizes(w2, h2, true);
@@ -159,8 +159,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     memcpy(h2p(i_tiles), T, sizeof T);
     int4 *wr = (int4 *)h2p(i_wr);
     int k = 0;
-    for (int y = 0; y < h1; y += G) wr[k++] = make_int4(0, 0, y, 0);
-    for (int y = 0; y < h2; y += G) wr[k++] = make_int4(0, 1, y, 0);
+    for (int y = 0; y < h1; y += GR) wr[k++] = make_int4(0, 0, y, 0);
+    for (int y = 0; y < h2; y += GR) wr[k++] = make_int4(0, 1, y, 0);
     int2 *wc = (int2 *)h2p(i_wc);
     for (int i = 0; i < ncol; ++i) wc[i] = make_int2(0, i * C);
     int2 *wc2 = (int2 *)h2p(i_wc2);

@@ -267,8 +267,8 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     P.radius = radius;
     P.filter = bFilter ? 1 : 0;
     P.norm = (double)n1 * n_i2;
-    const int NH = N / 2, H = NH + 1, G = row_group<double>(NH), C = col_group<double>(N);
-    const int nr2c = 2 * ((N + G - 1) / G), ncol = (H + C - 1) / C, nc2r = (N + G - 1) / G;
+    const int NH = N / 2, H = NH + 1, G = row_group<double>(NH), C = col_group<double>(N), GR = r2c_rows<double>(NH);
+    const int nr2c = 2 * ((N + GR - 1) / GR), ncol = (H + C - 1) / C, nc2r = (N + G - 1) / G;
 
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n1), i_vals = lay.add(sizeof(double) * n1);
@@ -307,7 +307,7 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     int4 *wr = (int4 *)hp(i_wr);
     int k = 0;
     for (int img = 0; img < 2; ++img)
-        for (int y = 0; y < N; y += G) wr[k++] = make_int4(0, img, y, 0);
+        for (int y = 0; y < N; y += GR) wr[k++] = make_int4(0, img, y, 0);
     int2 *wc = (int2 *)hp(i_wc);
     for (int i = 0; i < ncol; ++i) wc[i] = make_int2(0, i * C);
     int2 *wc2 = (int2 *)hp(i_wc2);

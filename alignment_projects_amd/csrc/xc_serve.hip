@@ -259,13 +259,13 @@ int launch_batch(xc_server *S, int slot_i)
     std::vector<int> starts;
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &b = buckets[bi];
-        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY);
+        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY), GR = r2c_rows<float>(b.NH);
         int nr = 0, nc = 0, nc2 = 0;
         for (int k = b.first; k < b.first + b.count; ++k) {
             const int i = order[k];
             const PairDev &D = P[i];
-            if (ownA[i]) nr += (D.h1 + G - 1) / G;
-            if (ownB[i]) nr += (D.h2 + G - 1) / G;
+            if (ownA[i]) nr += (D.h1 + GR - 1) / GR;
+            if (ownB[i]) nr += (D.h2 + GR - 1) / GR;
             nc += (D.Nx / 2 + 1 + C - 1) / C;
             live_row_groups(D, G, starts);
             nc2 += (int)starts.size();
@@ -311,7 +311,7 @@ int launch_batch(xc_server *S, int slot_i)
     }
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &b = buckets[bi];
-        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY);
+        const int G = row_group<float>(b.NH), C = col_group<float>(b.NY), GR = r2c_rows<float>(b.NH);
         int4 *wr = (int4 *)(h + lay.off[i_wr[bi]]);
         int2 *wc = (int2 *)(h + lay.off[i_wc[bi]]);
         int2 *wc2 = (int2 *)(h + lay.off[i_wc2[bi]]);
@@ -320,9 +320,9 @@ int launch_batch(xc_server *S, int slot_i)
             const int i = order[k];
             const PairDev &D = hp[k];
             if (ownA[i])
-                for (int y = 0; y < D.h1; y += G) wr[a++] = make_int4(k, 0, y, 0);
+                for (int y = 0; y < D.h1; y += GR) wr[a++] = make_int4(k, 0, y, 0);
             if (ownB[i])
-                for (int y = 0; y < D.h2; y += G) wr[a++] = make_int4(k, 1, y, 0);
+                for (int y = 0; y < D.h2; y += GR) wr[a++] = make_int4(k, 1, y, 0);
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
             live_row_groups(D, G, starts);
             for (int y : starts) wc2[d++] = make_int2(k, y);
