@@ -328,11 +328,12 @@ __device__ __forceinline__ void fft_pass_rows(cpx<T> (&v)[RR][Plan<N>::E], int t
     }
 }
 
-// RR table-twiddle transforms; `upper_zero`: registers m >= E/2 are zero padding
-// (constants, so the first pass drops their terms).
-template <int DIR, int N, typename T, int RR, bool UPPER_ZERO>
-__device__ __forceinline__ void fft_rows(cpx<T> (&v)[RR][Plan<N>::E], int t, const LdsView<T> (&lds)[RR],
-                                         const cpx<T> *__restrict__ tw)
+// RR transforms with the twiddle source TW (TwLazy or TwReg); UPPER_ZERO:
+// registers m >= E/2 are zero padding (constants, so the first pass drops
+// their terms).
+template <int DIR, int N, typename T, int RR, bool UPPER_ZERO, typename TWS>
+__device__ __forceinline__ void fft_rows_tw(cpx<T> (&v)[RR][Plan<N>::E], int t, const LdsView<T> (&lds)[RR],
+                                            const TWS &TW)
 {
     if constexpr (UPPER_ZERO) {
         constexpr int E = Plan<N>::E;
@@ -341,8 +342,16 @@ __device__ __forceinline__ void fft_rows(cpx<T> (&v)[RR][Plan<N>::E], int t, con
 #pragma unroll
             for (int m = E / 2; m < E; ++m) v[q][m] = cpx<T>{(T)0, (T)0};
     }
+    fft_pass_rows<DIR, N, T, RR, 0, TWS>(v, t, lds, TW);
+}
+
+// RR transforms with twiddles read from the table in each pass.
+template <int DIR, int N, typename T, int RR, bool UPPER_ZERO>
+__device__ __forceinline__ void fft_rows(cpx<T> (&v)[RR][Plan<N>::E], int t, const LdsView<T> (&lds)[RR],
+                                         const cpx<T> *__restrict__ tw)
+{
     const TwLazy<T> TW{tw};
-    fft_pass_rows<DIR, N, T, RR, 0, TwLazy<T>>(v, t, lds, TW);
+    fft_rows_tw<DIR, N, T, RR, UPPER_ZERO, TwLazy<T>>(v, t, lds, TW);
 }
 
 // Forward or inverse transform of data whose registers m >= E/2 are zero

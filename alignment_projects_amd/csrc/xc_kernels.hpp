@@ -1027,14 +1027,19 @@ template <typename T, int NY> struct ColCfg {
     static constexpr int TPF = Plan<NY>::TPF;
     static constexpr int CW = sizeof(T) == 8 ? 4 : 8;  // columns per workgroup by default (64 B rows)
     static constexpr int CT = 1024 / TPF > 0 ? 1024 / TPF : 1;                       // <= 1024 threads
-    static constexpr int CL = 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) > 0 ? 65536 / (lds_len(NY) * (int)sizeof(cpx<T>)) : 1;  // <= 64 KB LDS
+    // FP32: the A and B forward transforms run interleaved (sharing barriers and
+    // twiddles), each column then holds two LDS regions
+    static constexpr int AB = sizeof(T) == 4 ? 2 : 1;
+    static constexpr int CL = 65536 / (AB * lds_len(NY) * (int)sizeof(cpx<T>)) > 0
+                                  ? 65536 / (AB * lds_len(NY) * (int)sizeof(cpx<T>))
+                                  : 1;  // <= 64 KB LDS
     static constexpr int C = pow2_floor(cmin(CW, cmin(CT, CL)));  // a power of two, so it divides COL_STEP
     static constexpr int THREADS = TPF * C;
+    static constexpr long long lds_bytes(int CC) { return (long long)CC * AB * lds_len(NY) * (long long)sizeof(cpx<T>); }
     // a workgroup of CC columns fits: <= 1024 threads and <= 160 KB of LDS
     static constexpr bool fits(int CC)
     {
-        return CC >= 1 && COL_STEP % CC == 0 && TPF * CC <= 1024 &&
-               (long long)CC * lds_len(NY) * (long long)sizeof(cpx<T>) <= 163840;
+        return CC >= 1 && COL_STEP % CC == 0 && TPF * CC <= 1024 && lds_bytes(CC) <= 163840;
     }
 };
 
@@ -1062,6 +1067,34 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
     tw_load<NY, T>(TW, t, tw);
 
     cpx<T> a[E], b[E];
+    if constexpr (ColCfg<T, NY>::AB == 2) {
+        // both forward transforms at once, B's in the second LDS region
+        cpx<T> ab[2][E];
+        const LdsView<T> L2[2] = {L, LdsView<T>{smem + C * lds_len(NY) + c, C}};
+        if (E == 8 && P.A.h <= NY / 2 && P.B.h <= NY / 2) {
+            // rows y >= NY/2 (registers m >= E/2) are padding of both images
+#pragma unroll
+            for (int m = 0; m < E / 2; ++m) {
+                const int y = t + TPF * m;
+                ab[0][m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+                ab[1][m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            }
+            fft_rows_tw<-1, NY, T, 2, true>(ab, t, L2, TW);
+        } else {
+#pragma unroll
+            for (int m = 0; m < E; ++m) {
+                const int y = t + TPF * m;
+                ab[0][m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+                ab[1][m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            }
+            fft_rows_tw<-1, NY, T, 2, false>(ab, t, L2, TW);
+        }
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            a[m] = ab[0][m];
+            b[m] = ab[1][m];
+        }
+    } else {
     // rows y >= NY/2 (registers m >= E/2) are padding when the image is at most NY/2 high
     if (E == 8 && P.A.h <= NY / 2) {
 #pragma unroll
@@ -1093,6 +1126,7 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
             b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
         }
         fft<-1, NY>(b, t, L, TW);
+    }
     }
 #pragma unroll
     for (int m = 0; m < E; ++m) {

@@ -11,7 +11,7 @@ static int col_nc(xc_ctx *ctx, bool whiten, int xcd, int nitems, const PairDev *
     if constexpr (NY > max_ny<T>() || !ColCfg<T, NY>::fits(C)) {
         return set_err(ctx, XC_E_SHAPE, "no column pass of length %d with %d columns per workgroup", NY, C);
     } else {
-        const size_t smem = sizeof(cpx<T>) * C * lds_len(NY);
+        const size_t smem = (size_t)ColCfg<T, NY>::lds_bytes(C);
         if (smem > ctx->lds_max) return set_err(ctx, XC_E_SHAPE, "column pass needs %zu bytes of LDS", smem);
         const dim3 grid(nitems * (COL_STEP / C)), block(Plan<NY>::TPF * C);
         if (whiten) {
@@ -68,7 +68,7 @@ int launch_col(xc_ctx *ctx, int NY, bool whiten, int cwg, int xcd, int nitems, c
 bool col_shape_ok(const xc_ctx *ctx, int NY, int cwg)
 {
     const int TPF = NY >= 8 ? NY / 8 : 1;
-    const long long lds = (long long)cwg * lds_len(NY) * (long long)sizeof(cpx<float>);
+    const long long lds = (long long)cwg * ColCfg<float, 8>::AB * lds_len(NY) * (long long)sizeof(cpx<float>);
     return cwg >= 1 && COL_STEP % cwg == 0 && TPF * cwg <= 1024 && lds <= (long long)ctx->lds_max && lds <= 163840;
 }
 
