@@ -77,6 +77,13 @@ class XcAngleRec(C.Structure):
     _fields_ = [("A", C.c_double), ("R", C.c_double), ("X", C.c_double), ("Y", C.c_double)]
 
 
+class XcThmPair(C.Structure):
+    """One ThmPair_za^zb.txt entry (ThmPair, 0_GEN/PipeFiles.h:145-152)."""
+    _fields_ = [("atl", C.c_int32), ("acr", C.c_int32), ("btl", C.c_int32), ("bcr", C.c_int32),
+                ("err", C.c_int32), ("_pad", C.c_int32), ("A", C.c_double), ("R", C.c_double),
+                ("T", C.c_double * 6)]
+
+
 XC_SCAN_MAXR = 0
 XC_SCAN_CONSTXY = 1
 

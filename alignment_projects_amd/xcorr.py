@@ -98,6 +98,11 @@ _OPTIONAL = {
     "xc_peak_hunt_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_int64,
                           C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)],
                          C.c_int),
+    "xc_thmpair_create": ([C.c_char_p], C.c_int),
+    "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
+    "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
+    "xc_thmpair_find": ([C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcThmPair)], C.c_int),
+    "xc_thmpair_prior_angle": ([C.POINTER(abi.XcThmPair), C.c_int, _dp], C.c_int),
     "xc_peak_hunt_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int,
                           C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
 }
@@ -447,3 +452,50 @@ def fft_size(n1, n2):
 
 def version():
     return load().xc_version().decode()
+
+
+# ---- ThmPair tables (host only: no device needed) ----
+
+def thmpair_create(path):
+    """WriteThmPairHdr: create or truncate a ThmPair table with its header line."""
+    st = load().xc_thmpair_create(os.fsencode(path))
+    if st != abi.XC_OK:
+        raise OSError(f"xc_thmpair_create({path}): status {st}")
+
+
+def thmpair_append(path, atl, acr, btl, bcr, err, A, R, T):
+    """WriteThmPair: append one entry (skipped when atl < 0 or btl < 0)."""
+    e = abi.XcThmPair(atl, acr, btl, bcr, err, 0, A, R, (C.c_double * 6)(*T))
+    st = load().xc_thmpair_append(os.fsencode(path), C.byref(e))
+    if st != abi.XC_OK:
+        raise OSError(f"xc_thmpair_append({path}): status {st}")
+
+
+def thmpair_read_all(path):
+    """ReadAllThmPair: every entry after the header (XcThmPair list)."""
+    L = load()
+    n = L.xc_thmpair_read_all(os.fsencode(path), None, 0)
+    if n < 0:
+        raise OSError(f"xc_thmpair_read_all({path}): status {n}")
+    out = (abi.XcThmPair * max(n, 1))()
+    L.xc_thmpair_read_all(os.fsencode(path), out, n)
+    return list(out[:n])
+
+
+def thmpair_find(path, atl, acr, btl, bcr):
+    """ReadThmPair: the first entry of the given tiles, or None."""
+    e = abi.XcThmPair()
+    st = load().xc_thmpair_find(os.fsencode(path), atl, acr, btl, bcr, C.byref(e))
+    if st == abi.XC_OK:
+        return e
+    if st == abi.XC_NO_PEAK:
+        return None
+    raise OSError(f"xc_thmpair_find({path}): status {st}")
+
+
+def thmpair_prior_angle(entries):
+    """Mode-Y starting angle: (count of error-free entries used, median angle) or (0, None)."""
+    arr = (abi.XcThmPair * max(len(entries), 1))(*entries)
+    a = C.c_double(0.0)
+    n = load().xc_thmpair_prior_angle(arr, len(entries), C.byref(a))
+    return (n, a.value) if n > 0 else (0, None)

@@ -374,6 +374,38 @@ int xc_peak_hunt_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double hlf
                      xc_angle_rec *best, int *n_evals);
 
 /* ------------------------------------------------------------------ */
+/* ThmPair tables (the thumbnail matcher's result file)                */
+/* ------------------------------------------------------------------ */
+
+/* One entry of a ThmPair_za^zb.txt table (ThmPair, PipeFiles.h:145-152):
+ * tiles, error code, angle (deg), R, and the affine T[6] (T0 T1 X T3 T4 Y). */
+typedef struct xc_thmpair {
+    int32_t atl, acr, btl, bcr, err, _pad;
+    double A, R;
+    double T[6];
+} xc_thmpair;
+
+/* WriteThmPairHdr (PipeFiles.cpp:1284-1289): create or truncate `path`
+ * with the header line.  No device needed. */
+int xc_thmpair_create(const char *path);
+/* WriteThmPair (PipeFiles.cpp:1319-1355): append one entry in the
+ * reference's format ("%d\t" x 5, then "%f\t" x 8 with six decimals) in
+ * one write; an entry with atl < 0 or btl < 0 is skipped, as there. */
+int xc_thmpair_append(const char *path, const xc_thmpair *e);
+/* ReadAllThmPair (PipeFiles.cpp:1200-1262): every entry after the header.
+ * Returns the entry count (entries beyond cap are counted, not stored) or
+ * XC_E_ARG when the file cannot be opened or has no header line. */
+int xc_thmpair_read_all(const char *path, xc_thmpair *out, int cap);
+/* ReadThmPair (PipeFiles.cpp:1132-1195): the first entry for (atl, acr,
+ * btl, bcr): XC_OK, XC_NO_PEAK when there is none, XC_E_ARG as above. */
+int xc_thmpair_find(const char *path, int atl, int acr, int btl, int bcr, xc_thmpair *out);
+/* The mode-Y starting angle of CThmUtil::SetStartingAngle (1_DMesh/
+ * CThmUtil.cpp:80-133): the median (MedianVal, Maths.cpp:436-449) of the
+ * error-free entries' angles when there are at least 4, with |ang0| < 0.001
+ * forced to 0.  Returns that count, or 0 (no prior; *ang0 untouched). */
+int xc_thmpair_prior_angle(const xc_thmpair *e, int n, double *ang0);
+
+/* ------------------------------------------------------------------ */
 /* Serving loop: continuous batching of pair requests                 */
 /* ------------------------------------------------------------------ */
 

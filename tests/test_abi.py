@@ -32,6 +32,8 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.XcBucketInfo) == 32
     assert ctypes.sizeof(abi.XcServeCfg) == 24
     assert ctypes.sizeof(abi.XcServeStats) == 64
+    assert ctypes.sizeof(abi.XcAngleRec) == 32
+    assert ctypes.sizeof(abi.XcThmPair) == 6 * 4 + 2 * 8 + 6 * 8
     assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
 
 

@@ -1,0 +1,88 @@
+"""ThmPair_za^zb.txt tables (0_GEN/PipeFiles.cpp:1132-1360) through the C-ABI, and the mode-Y
+starting angle (1_DMesh/CThmUtil.cpp:80-133).  Host code only: runs without a device.
+
+The reference ships no ThmPair file; the fixture is its format strings: the header of
+WriteThmPairHdr and WriteThmPair's printf line, reproduced here with Python's %-formatting
+(correctly rounded, as glibc's printf is)."""
+import numpy as np
+
+from alignment_projects_amd import xcorr
+
+HDR = "Atl\tAcr\tBtl\tBcr\tErr\tDeg\tR\tT0\tT1\tX\tT3\tT4\tY\n"
+FMT = "%d\t%d\t%d\t%d\t%d\t%f\t%f\t%f\t%f\t%f\t%f\t%f\t%f\n"
+
+
+def _entries():
+    rng = np.random.default_rng(4)
+    out = []
+    for i in range(6):
+        T = [float(v) for v in rng.normal(0, 50, 6)]
+        out.append((i, 2 * i + 1, 100 + i, 7, i % 3 == 2, float(rng.uniform(-5, 5)), float(rng.uniform(0, 1)), T))
+    return out
+
+
+def test_write_format_and_round_trip(tmp_path):
+    p = tmp_path / "ThmPair_3^2.txt"
+    xcorr.thmpair_create(p)
+    assert p.read_text() == HDR
+    ents = _entries()
+    for atl, acr, btl, bcr, err, A, R, T in ents:
+        xcorr.thmpair_append(p, atl, acr, btl, bcr, int(err), A, R, T)
+    xcorr.thmpair_append(p, -1, 0, 5, 0, 0, 1.0, 0.5, [0] * 6)  # invalid tile id: not written
+    xcorr.thmpair_append(p, 5, 0, -1, 0, 0, 1.0, 0.5, [0] * 6)
+    expect = HDR + "".join(FMT % (atl, acr, btl, bcr, int(err), A, R, *T) for atl, acr, btl, bcr, err, A, R, T in ents)
+    assert p.read_text() == expect
+    got = xcorr.thmpair_read_all(p)
+    assert len(got) == len(ents)
+    for g, (atl, acr, btl, bcr, err, A, R, T) in zip(got, ents):
+        assert (g.atl, g.acr, g.btl, g.bcr, g.err) == (atl, acr, btl, bcr, int(err))
+        assert abs(g.A - A) <= 5e-7 and abs(g.R - R) <= 5e-7
+        assert all(abs(g.T[i] - T[i]) <= 5e-7 for i in range(6))
+
+
+def test_find_and_missing(tmp_path):
+    p = tmp_path / "ThmPair_1^0.txt"
+    xcorr.thmpair_create(p)
+    ents = _entries()
+    for atl, acr, btl, bcr, err, A, R, T in ents:
+        xcorr.thmpair_append(p, atl, acr, btl, bcr, int(err), A, R, T)
+    atl, acr, btl, bcr, err, A, R, T = ents[3]
+    e = xcorr.thmpair_find(p, atl, acr, btl, bcr)
+    assert e is not None and abs(e.A - A) <= 5e-7
+    assert xcorr.thmpair_find(p, atl, acr + 1, btl, bcr) is None
+    try:
+        xcorr.thmpair_find(tmp_path / "nope.txt", 0, 0, 0, 0)
+        raise AssertionError("missing file accepted")
+    except OSError:
+        pass
+    (tmp_path / "empty.txt").write_text("")
+    try:
+        xcorr.thmpair_read_all(tmp_path / "empty.txt")
+        raise AssertionError("empty file accepted")
+    except OSError:
+        pass
+
+
+def test_reader_takes_hand_written_lines(tmp_path):
+    # sscanf semantics: fields separated by tabs or spaces, trailing extras ignored
+    p = tmp_path / "t.txt"
+    p.write_text(HDR + "1 2 3 4 0 1.5 0.75 1 0 10 0 1 -20 extra\n7\t8\t9\t10\t1\t-2.25\t0.5\t0\t0\t0\t0\t0\t0\n")
+    got = xcorr.thmpair_read_all(p)
+    assert [(g.atl, g.acr, g.btl, g.bcr, g.err, g.A, g.R) for g in got] == [(1, 2, 3, 4, 0, 1.5, 0.75),
+                                                                            (7, 8, 9, 10, 1, -2.25, 0.5)]
+    assert list(got[0].T) == [1.0, 0.0, 10.0, 0.0, 1.0, -20.0]
+
+
+def test_prior_angle_median():
+    from alignment_projects_amd import abi
+
+    def E(A, err=0):
+        return abi.XcThmPair(0, 0, 0, 0, err, 0, A, 0.0)
+    # fewer than 4 error-free entries: no prior
+    assert xcorr.thmpair_prior_angle([E(1.0), E(2.0), E(3.0), E(9.0, 1)]) == (0, None)
+    # odd count: the middle value; entries with an error code are left out
+    assert xcorr.thmpair_prior_angle([E(3.0), E(-1.0), E(2.0), E(10.0), E(0.5), E(99.0, 2)]) == (5, 2.0)
+    # even count: the mean of the two middle values
+    assert xcorr.thmpair_prior_angle([E(4.0), E(1.0), E(2.0), E(3.0)]) == (4, 2.5)
+    # |median| < 0.001 is forced to 0
+    assert xcorr.thmpair_prior_angle([E(0.0005), E(-0.0002), E(0.0004), E(-0.0001)]) == (4, 0.0)
