@@ -699,20 +699,23 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
     }
 }
 
-// Unmasked u8 tiles without a count table: the rows are scanned in exact
-// integer arithmetic (a row's prefix of v and v^2 fits in 32 bits for
-// w <= TP_MAX_W), the raw sums accumulate down the columns in FP64 (exact
-// integers), and each stored entry is normalised from them:
+// u8 tiles: the rows are scanned in exact integer arithmetic (a row's prefix
+// of v and v^2 fits in 32 bits for w <= TP_MAX_W), the raw sums accumulate
+// down the columns in FP64 (exact integers), and each stored entry is
+// normalised from them over the n present pixels of the box:
 //   S = (Sraw - n mean) / sd,  Q = (Qraw - 2 mean Sraw + n mean^2) / sd^2
-// with n = (x+1)(y+1).  This is the exact sum of the normalised values; the
-// general sweep sums the values already rounded to T, which differs by the
-// accumulated rounding of T (relative 1e-8 for FP32), far below the
-// transforms' own error.  Integer DPP scans are one instruction per step,
-// against three for an FP64 scan.
-template <typename T, int CPT, int NT>
+// (unmasked: n = (x+1)(y+1); masked: a third integer scan counts the present
+// pixels).  This is the exact sum of the normalised values; the general sweep
+// sums the values already rounded to T, which differs by the accumulated
+// rounding of T (relative 1e-8 for FP32), far below the transforms' own
+// error.  WZ: the count table (pixels whose value rounded to T is non-zero,
+// as the general sweep counts them) is scanned and written too.  Integer DPP
+// scans are one instruction per step, against three for an FP64 scan.
+template <typename T, int CPT, int NT, bool MASK, bool WZ>
 __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 {
     constexpr int NWV = NT / 64;
+    constexpr int NQ = 2 + (MASK ? 1 : 0) + (WZ ? 1 : 0);  // S, Q [, present] [, non-zero]
     static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
     const int w = D.v.w, h = D.v.h;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -721,52 +724,77 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
     const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
     const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
     tab_t<T> *sq = reinterpret_cast<tab_t<T> *>(D.SQ);
-    double as[CPT], aq[CPT];
+    double as[CPT], aq[CPT], an[CPT];
+    int az[CPT];
     constexpr int LA = 4;  // rows loaded ahead of the sweep
-    int nxt[LA][CPT];
+    int nxt[LA][CPT];      // pixel, or -1 for an absent one
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         as[j] = 0.0;
         aq[j] = 0.0;
+        an[j] = 0.0;
+        az[j] = 0;
         const int x = x0 + j;
 #pragma unroll
-        for (int a = 0; a < LA; ++a) nxt[a][j] = (x < w && a < h) ? src[(long long)a * D.v.stride + x] : 0;
+        for (int a = 0; a < LA; ++a) {
+            int v = (x < w && a < h) ? src[(long long)a * D.v.stride + x] : 0;
+            if constexpr (MASK) v = (x < w && a < h && D.v.mask[(long long)a * D.v.mstride + x]) ? v : -1;
+            nxt[a][j] = v;
+        }
     }
     const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);
     for (int y = 0; y < h; ++y) {
-        int ps[CPT], pq[CPT];
+        int pf[NQ][CPT];  // per-thread row prefixes
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
-            const int v = nxt[0][j];
+            const int raw = nxt[0][j];
 #pragma unroll
             for (int a = 0; a + 1 < LA; ++a) nxt[a][j] = nxt[a + 1][j];
             const int x = x0 + j;
-            nxt[LA - 1][j] = (x < w && y + LA < h) ? src[(long long)(y + LA) * D.v.stride + x] : 0;
-            ps[j] = (j ? ps[j - 1] : 0) + v;
-            pq[j] = (j ? pq[j - 1] : 0) + v * v;
+            int nv = (x < w && y + LA < h) ? src[(long long)(y + LA) * D.v.stride + x] : 0;
+            if constexpr (MASK)
+                nv = (x < w && y + LA < h && D.v.mask[(long long)(y + LA) * D.v.mstride + x]) ? nv : -1;
+            nxt[LA - 1][j] = nv;
+            const bool pres = raw >= 0 && x < w;
+            const int v = pres ? raw : 0;
+            pf[0][j] = (j ? pf[0][j - 1] : 0) + v;
+            pf[1][j] = (j ? pf[1][j - 1] : 0) + v * v;
+            if constexpr (MASK) pf[2][j] = (j ? pf[2][j - 1] : 0) + (pres ? 1 : 0);
+            if constexpr (WZ) {
+                const bool nz = pres && (T)(((double)raw - mean) * inv) != (T)0;
+                pf[NQ - 1][j] = (j ? pf[NQ - 1][j - 1] : 0) + (nz ? 1 : 0);
+            }
         }
-        const int ts = ps[CPT - 1], tq = pq[CPT - 1];
-        const int is = wave_scan_i(ts), iq = wave_scan_i(tq);
-        const int par = (y & 1) * 2 * NWV;
-        if (lane == 63) {
-            xi[par + wv] = is;
-            xi[par + NWV + wv] = iq;
+        int base[NQ];
+        const int par = (y & 1) * NQ * NWV;
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+            const int tot = pf[qi][CPT - 1];
+            const int inc = wave_scan_i(tot);
+            if (lane == 63) xi[par + qi * NWV + wv] = inc;
+            base[qi] = inc - tot;
         }
         __syncthreads();
-        int bs = is - ts, bq = iq - tq;
         if constexpr (NWV > 1) {
             const bool left = lane < wv;
             const int k = left ? lane : 0;
-            bs += row16_sum_i(left ? xi[par + k] : 0);
-            bq += row16_sum_i(left ? xi[par + NWV + k] : 0);
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi) base[qi] += row16_sum_i(left ? xi[par + qi * NWV + k] : 0);
         }
         const long long row = (long long)y * w;
         double sv[CPT], qv[CPT];
 #pragma unroll
         for (int j = 0; j < CPT; ++j) {
-            as[j] += (double)(bs + ps[j]);
-            aq[j] += (double)(bq + pq[j]);
-            const double n = (double)(x0 + j + 1) * (double)(y + 1);
+            as[j] += (double)(base[0] + pf[0][j]);
+            aq[j] += (double)(base[1] + pf[1][j]);
+            double n;
+            if constexpr (MASK) {
+                an[j] += (double)(base[2] + pf[2][j]);
+                n = an[j];
+            } else {
+                n = (double)(x0 + j + 1) * (double)(y + 1);
+            }
+            if constexpr (WZ) az[j] += base[NQ - 1] + pf[NQ - 1][j];
             sv[j] = (as[j] - n * mean) * inv;
             qv[j] = ((aq[j] - m2 * as[j]) + n * msq) * inv2;
         }
@@ -776,11 +804,15 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
                 if (x0 + j < w) {
                     sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
                     sq[row + x0 + j + 1] = TabE<T>::make(sv[j + 1], qv[j + 1]);
+                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
                 }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j)
-                if (x0 + j < w) sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
+                if (x0 + j < w) {
+                    sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
+                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                }
         }
     }
 }
@@ -826,13 +858,19 @@ template <typename T, int CPT, int NT>
 __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
-    __shared__ int xz[2 * NT / 64], xi[4 * NT / 64];
+    __shared__ int xz[2 * NT / 64], xi[8 * NT / 64];
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
     const bool m = D.v.mask != nullptr;
-    if (D.v.dtype == XC_U8 && !m && !D.Z && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
-        sat_sweep_u8<T, CPT, NT>(D, xi);
+    if (D.v.dtype == XC_U8 && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
+        if (m) {
+            if (D.Z) sat_sweep_u8<T, CPT, NT, true, true>(D, xi);
+            else sat_sweep_u8<T, CPT, NT, true, false>(D, xi);
+        } else {
+            if (D.Z) sat_sweep_u8<T, CPT, NT, false, true>(D, xi);
+            else sat_sweep_u8<T, CPT, NT, false, false>(D, xi);
+        }
         return;
     }
 #define SW_CALL(InT, MK) sat_sweep<T, CPT, NT, InT, MK>(D, band, xs, xq, xz)
