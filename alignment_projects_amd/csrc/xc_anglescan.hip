@@ -114,6 +114,68 @@ xc_angle_rec scan_select(int mode, const std::vector<xc_angle_rec> &vC)
     return best;
 }
 
+// TAffine (t0 t1 X t3 t4 Y) and its product (TAffine.cpp:220-232).
+struct Aff {
+    double t[6];
+};
+constexpr Aff AFF_ID = {{1, 0, 0, 0, 1, 0}};
+
+Aff aff_mul(const Aff &L, const Aff &R)
+{
+#pragma clang fp contract(off)
+    Aff r;
+    r.t[0] = L.t[0] * R.t[0] + L.t[1] * R.t[3];
+    r.t[1] = L.t[0] * R.t[1] + L.t[1] * R.t[4];
+    r.t[2] = L.t[0] * R.t[2] + L.t[1] * R.t[5] + L.t[2];
+    r.t[3] = L.t[3] * R.t[0] + L.t[4] * R.t[3];
+    r.t[4] = L.t[3] * R.t[1] + L.t[4] * R.t[4];
+    r.t[5] = L.t[3] * R.t[2] + L.t[4] * R.t[5] + L.t[5];
+    return r;
+}
+
+Aff aff_of(const double *t) { return t ? Aff{{t[0], t[1], t[2], t[3], t[4], t[5]}} : AFF_ID; }
+
+// TAffine::NUSelect (TAffine.cpp:76-135): 0 scale, 1 x-scale, 2 y-scale, 3 x-skew, 4 y-skew, 5 rotation.
+Aff nu_select(int sel, double a)
+{
+    switch (sel) {
+    case 1: return Aff{{a, 0, 0, 0, 1, 0}};
+    case 2: return Aff{{1, 0, 0, 0, a, 0}};
+    case 3: return Aff{{1, a, 0, 0, 1, 0}};
+    case 4: return Aff{{1, 0, 0, a, 1, 0}};
+    case 5: {
+        const double c = cos(a), s = sin(a);
+        return Aff{{c, -s, 0, s, c, 0}};
+    }
+    default: return Aff{{a, 0, 0, 0, a, 0}};
+    }
+}
+
+// RFromAngle on point lists (CThmScan.cpp:329-366): A's points by the 2x2 part of
+// R(deg) * (Tdfm * Ttw) (identity for both when null), then CorrImagesS.
+int rfa64_aff(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
+              double deg, const Aff *Tdfm, const Aff *Ttw, const Rules &ru, int Ox, int Oy, int Rx, int Ry,
+              std::vector<xc_point> &buf, xc_angle_rec *rec)
+{
+#pragma clang fp contract(off)
+    const Aff T = aff_mul(nu_select(5, deg * REF_PI / 180.0), aff_mul(Tdfm ? *Tdfm : AFF_ID, Ttw ? *Ttw : AFF_ID));
+    const double t0 = T.t[0], t1 = T.t[1], t3 = T.t[3], t4 = T.t[4];
+    buf.resize(na);
+    for (int i = 0; i < na; ++i) {
+        buf[i].x = ap[i].x * t0 + ap[i].y * t1;
+        buf[i].y = ap[i].x * t3 + ap[i].y * t4;
+    }
+    double X = 0.0, Y = 0.0, R = 0.0;
+    const int st = xc_corr_images_f64(ctx, XC_MODE_S, buf.data(), av, na, bp, bv, nb, &ru.rgn, &ru.cnt, 0.0, 0.0, Ox,
+                                      Oy, Rx, Ry, &X, &Y, &R, nullptr);
+    if (st < 0) return st;
+    rec->A = deg;
+    rec->R = R;
+    rec->X = X;
+    rec->Y = Y;
+    return XC_OK;
+}
+
 // RFromAngle on point lists (CThmScan.cpp:329-366, identity deformation and tweak).
 int rfa64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
           double deg, const Rules &ru, int Ox, int Oy, int Rx, int Ry, std::vector<xc_point> &buf, xc_angle_rec *rec)
@@ -267,4 +329,96 @@ extern "C" int xc_peak_hunt_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
         if (!st) *rec = one[0];
         return st;
     });
+}
+
+extern "C" int xc_rfromangle_affine_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                        const double *bv, int nb, double deg, const double *Tdfm, const double *Ttw,
+                                        int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                                        xc_angle_rec *rec)
+{
+    if (!ctx || !rec) return XC_E_ARG;
+    std::vector<xc_point> buf;
+    const Aff D = aff_of(Tdfm), W = aff_of(Ttw);
+    return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, scan_rules(olap1D, req_area), Ox, Oy, Rx, Ry, buf, rec);
+}
+
+extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                const double *bv, int nb, double deg, const double *Tdfm, double *Tptwk,
+                                double *bestR, int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                                int *changed, int *n_evals)
+{
+    if (!ctx || !Tptwk || !bestR) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    const Aff D = aff_of(Tdfm);
+    Aff tw = aff_of(Tptwk);
+    std::vector<xc_point> buf;
+    int evals = 0, st;
+    auto eval = [&](int sel, double a, xc_angle_rec *C) {
+        const Aff W = aff_mul(nu_select(sel, a), tw);  // M * Tptwk (RFromAngle's iaux branch)
+        ++evals;
+        return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, Ox, Oy, Rx, Ry, buf, C);
+    };
+    const double astep = 0.02;
+    const int nstep = 5;
+    bool any = false;
+    int vused[5] = {0, 0, 0, 0, 0};
+    double bR = *bestR;
+    for (int itype = 0; itype < 5; ++itype) {
+        // sweep every unused type's magnitude about its identity value
+        std::vector<int> vsel;
+        std::vector<double> va;
+        for (int sel = 0; sel < 5; ++sel) {
+            if (vused[sel]) continue;
+            const double abase = (sel >= 0 && sel <= 2) ? 1.0 : 0.0;
+            for (int j = -nstep; j <= nstep; ++j) {
+                vsel.push_back(sel);
+                va.push_back(abase + j * astep);
+            }
+        }
+        double rbest = 0;
+        int ibest = -1;
+        for (size_t ic = 0; ic < vsel.size(); ++ic) {
+            xc_angle_rec C;
+            if ((st = eval(vsel[ic], va[ic], &C))) return st;
+            if (C.R > rbest) {
+                rbest = C.R;
+                ibest = (int)ic;
+            }
+        }
+        if (ibest == -1) break;
+        // PTWInterp (CThmScan.cpp:119-160): parabola through the neighbours at +-astep
+        const int sel = vsel[ibest];
+        const double x1 = va[ibest], y1 = rbest, d = astep;
+        xc_angle_rec C;
+        if ((st = eval(sel, x1 - d, &C))) return st;
+        const double y0 = C.R;
+        if ((st = eval(sel, x1 + d, &C))) return st;
+        const double y2 = C.R;
+        double xnew = x1;
+        {
+#pragma clang fp contract(off)
+            const double Dp = d * (y0 - y2) / (2 * (y0 + y2 - y1 - y1));  // NewXFromParabola (:48-61)
+            if (fabs(Dp) <= 2 * d) xnew += Dp;
+        }
+        if ((st = eval(sel, xnew, &C))) return st;
+        double ynew = C.R;
+        if (ynew < y1) {
+            xnew = x1;
+            ynew = y1;
+        }
+        rbest = ynew;
+        if (rbest > bR) {
+            tw = aff_mul(nu_select(sel, xnew), tw);
+            vused[sel] = 1;
+            bR = rbest;
+            any = true;
+        } else {
+            break;
+        }
+    }
+    memcpy(Tptwk, tw.t, sizeof tw.t);
+    *bestR = bR;
+    if (changed) *changed = any ? 1 : 0;
+    if (n_evals) *n_evals = evals;
+    return XC_OK;
 }

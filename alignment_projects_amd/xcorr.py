@@ -98,6 +98,11 @@ _OPTIONAL = {
     "xc_peak_hunt_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_int64,
                           C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)],
                          C.c_int),
+    "xc_rfromangle_affine_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, _dp, _dp, C.c_int,
+                                  C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_pretweaks_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, C.c_int,
+                          C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+                         C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -309,6 +314,37 @@ class Context:
                                      C.byref(k))
         self._check(st, "xc_peak_hunt_f32", ok=(abi.XC_OK,))
         return bb, k.value
+
+    def rfromangle_affine(self, deg, ap, av, bp, bv, Tdfm, Ttw, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """RFromAngle with deformation and pretweak affines (t[6] each, None = identity)."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+        W = None if Ttw is None else np.ascontiguousarray(Ttw, dtype=np.float64)
+        rec = abi.XcAngleRec()
+        st = self.L.xc_rfromangle_affine_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), deg,
+                                             None if D is None else D.ctypes.data_as(_dp),
+                                             None if W is None else W.ctypes.data_as(_dp), olap1d, req_area,
+                                             Ox, Oy, Rx, Ry, C.byref(rec))
+        self._check(st, "xc_rfromangle_affine_f64", ok=(abi.XC_OK,))
+        return rec
+
+    def pretweaks(self, ap, av, bp, bv, deg, Tdfm, Tptwk, best_r, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """Pretweaks in FP64. Returns (Tptwk[6], best_r, changed, n_evals)."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+        W = np.array(Tptwk if Tptwk is not None else [1, 0, 0, 0, 1, 0], dtype=np.float64)
+        r, ch, ne = C.c_double(best_r), C.c_int(), C.c_int()
+        st = self.L.xc_pretweaks_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), deg,
+                                     None if D is None else D.ctypes.data_as(_dp), W.ctypes.data_as(_dp),
+                                     C.byref(r), olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(ch), C.byref(ne))
+        self._check(st, "xc_pretweaks_f64", ok=(abi.XC_OK,))
+        return W, r.value, ch.value, ne.value
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

@@ -373,6 +373,33 @@ int xc_peak_hunt_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double hlf
                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
                      xc_angle_rec *best, int *n_evals);
 
+/*
+ * RFromAngle with a deformation and a pretweak (CThmScan.cpp:329-366):
+ * A's points mapped by the 2x2 part of R(deg) * (Tdfm * Ttw), with the
+ * affines as TAffine's t[6] (t0 t1 X t3 t4 Y) and products as
+ * TAffine::operator* (TAffine.cpp:220-232).  Tdfm or Ttw NULL = identity.
+ */
+int xc_rfromangle_affine_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                             const xc_point *bp, const double *bv, int nb, double deg,
+                             const double *Tdfm, const double *Ttw,
+                             int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                             xc_angle_rec *rec);
+
+/*
+ * CThmScan::Pretweaks (CThmScan.cpp:205-323) in FP64: up to five rounds; in
+ * each, sweep every unused near-unity transform type (scale, x-scale,
+ * y-scale, x-skew, y-skew; TAffine::NUSelect, TAffine.cpp:76-135) over
+ * base +- 5 x 0.02 at angle deg, refine the round's best magnitude with
+ * PTWInterp's parabola (:119-160), and compose it into Tptwk (in/out, t[6])
+ * when it beats *bestR (in/out).  *changed = 1 when Tptwk changed;
+ * *n_evals receives the number of RFromAngle evaluations.
+ */
+int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                     const xc_point *bp, const double *bv, int nb, double deg,
+                     const double *Tdfm, double *Tptwk, double *bestR,
+                     int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                     int *changed, int *n_evals);
+
 /* ------------------------------------------------------------------ */
 /* ThmPair tables (the thumbnail matcher's result file)                */
 /* ------------------------------------------------------------------ */

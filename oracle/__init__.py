@@ -74,6 +74,10 @@ def lib():
                                     C.POINTER(XcAngleRec), C.POINTER(XcAngleRec), C.c_int, C.POINTER(C.c_int)]
         L.xo_peak_hunt.argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, C.c_int, C.c_long,
                                    C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(XcAngleRec), C.POINTER(C.c_int)]
+        L.xo_rfromangle_affine.argtypes = [C.c_double, _pp, _dp, C.c_int, _pp, _dp, C.c_int, _dp, _dp, C.c_int,
+                                            C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.xo_pretweaks.argtypes = [C.c_double, _pp, _dp, C.c_int, _pp, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_long,
+                                   C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.xo_convolve.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_fpeak.argtypes = [_dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_double, _ip, _ip]
         _lib = L
@@ -356,3 +360,33 @@ def peak_hunt(best, ap, av, bp, bv, hlfwid, olap1d, req_area, Ox=0, Oy=0, Rx=-1,
     lib().xo_peak_hunt(pp1, pv1, len(v1), pp2, pv2, len(v2), hlfwid, olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(b),
                        C.byref(k))
     return b, k.value
+
+
+def pretweaks(ap, av, bp, bv, deg, Tdfm, Tptwk, best_r, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """CThmScan::Pretweaks (0_GEN/CThmScan.cpp:205-323). Returns (Tptwk[6], best_r, changed, n_evals)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+    W = np.array(Tptwk if Tptwk is not None else [1, 0, 0, 0, 1, 0], dtype=np.float64)
+    r, ch, ne = C.c_double(best_r), C.c_int(), C.c_int()
+    lib().xo_pretweaks(deg, pp1, pv1, len(v1), pp2, pv2, len(v2), None if D is None else D.ctypes.data_as(_dp),
+                       W.ctypes.data_as(_dp), C.byref(r), olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(ch), C.byref(ne))
+    return W, r.value, ch.value, ne.value
+
+
+def rfromangle_affine(deg, ap, av, bp, bv, Tdfm, Ttw, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+    """RFromAngle with deformation and pretweak affines. Returns (status, R, X, Y)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+    W = None if Ttw is None else np.ascontiguousarray(Ttw, dtype=np.float64)
+    X, Y, R = C.c_double(), C.c_double(), C.c_double()
+    st = lib().xo_rfromangle_affine(deg, pp1, pv1, len(v1), pp2, pv2, len(v2),
+                                    None if D is None else D.ctypes.data_as(_dp),
+                                    None if W is None else W.ctypes.data_as(_dp), olap1d, req_area, Ox, Oy, Rx, Ry,
+                                    C.byref(X), C.byref(Y), C.byref(R))
+    return st, R.value, X.value, Y.value

@@ -1255,3 +1255,143 @@ int xo_peak_hunt(const xc_point *ap, const double *av, int na, const xc_point *b
     if (kout) *kout = k;
     return XC_OK;
 }
+
+/* ---- Pretweaks ---------------------------------------------------- */
+
+typedef struct { double t[6]; } xo_aff;
+
+/* TAffine::operator* (TAffine.cpp:220-232). */
+static xo_aff aff_mul(xo_aff L, xo_aff R)
+{
+    xo_aff r;
+    r.t[0] = L.t[0] * R.t[0] + L.t[1] * R.t[3];
+    r.t[1] = L.t[0] * R.t[1] + L.t[1] * R.t[4];
+    r.t[2] = L.t[0] * R.t[2] + L.t[1] * R.t[5] + L.t[2];
+    r.t[3] = L.t[3] * R.t[0] + L.t[4] * R.t[3];
+    r.t[4] = L.t[3] * R.t[1] + L.t[4] * R.t[4];
+    r.t[5] = L.t[3] * R.t[2] + L.t[4] * R.t[5] + L.t[5];
+    return r;
+}
+
+/* TAffine::NUSelect (TAffine.cpp:76-135). */
+static xo_aff nu_select(int sel, double a)
+{
+    xo_aff r = {{1, 0, 0, 0, 1, 0}};
+    switch (sel) {
+    case 1: r.t[0] = a; break;
+    case 2: r.t[4] = a; break;
+    case 3: r.t[1] = a; break;
+    case 4: r.t[3] = a; break;
+    case 5: { double c = cos(a), s = sin(a); r.t[0] = c; r.t[1] = -s; r.t[3] = s; r.t[4] = c; break; }
+    default: r.t[0] = a; r.t[4] = a; break;
+    }
+    return r;
+}
+
+static xo_aff aff_of(const double *t)
+{
+    xo_aff r = {{1, 0, 0, 0, 1, 0}};
+    if (t) memcpy(r.t, t, sizeof r.t);
+    return r;
+}
+
+/* RFromAngle (CThmScan.cpp:329-366) with C.T = R * (Tdfm * T) applied to A's points
+ * (Apply_R_Part, TAffine.cpp:407-423), then CorrImagesS with BigEnough/EnoughPoints. */
+int xo_rfromangle_affine(double deg, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                         const double *bv, int nb, const double *Tdfm, const double *Ttw, int olap1D, long reqArea,
+                         int Ox, int Oy, int Rx, int Ry, double *X, double *Y, double *Rv)
+{
+    xo_aff T = aff_mul(nu_select(5, deg * REF_PI / 180.0), aff_mul(aff_of(Tdfm), aff_of(Ttw)));
+    xc_point *pts = (xc_point *)malloc(sizeof(xc_point) * na);
+    for (int i = 0; i < na; ++i) {
+        double x = ap[i].x * T.t[0] + ap[i].y * T.t[1];
+        double y = ap[i].x * T.t[3] + ap[i].y * T.t[4];
+        pts[i].x = x;
+        pts[i].y = y;
+    }
+    xc_rgn_rule g;
+    memset(&g, 0, sizeof g);
+    g.enabled = 1;
+    g.min_w = g.min_h = olap1D;
+    g.min_area = reqArea;
+    xc_cnt_rule k;
+    memset(&k, 0, sizeof k);
+    k.enabled = 1;
+    k.k1 = k.k2 = reqArea;
+    int st = xo_corr_images(XC_MODE_S, pts, av, na, bp, bv, nb, &g, &k, 0.0, 0.0, Ox, Oy, Rx, Ry, X, Y, Rv, NULL,
+                            NULL, NULL, NULL, 0);
+    free(pts);
+    return st;
+}
+
+/* CThmScan::Pretweaks (CThmScan.cpp:205-323) with PTWInterp (:119-160) and
+ * NewXFromParabola (:48-61). */
+int xo_pretweaks(double deg, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                 int nb, const double *Tdfm, double *Tptwk, double *bestR, int olap1D, long reqArea, int Ox, int Oy,
+                 int Rx, int Ry, int *changed, int *n_evals)
+{
+    xo_aff tw = aff_of(Tptwk);
+    int vused[5] = {0, 0, 0, 0, 0}, any = 0, evals = 0;
+    const double astep = 0.02;
+    const int nstep = 5;
+    double bR = *bestR, X, Y, R;
+#define EVAL(sel, a, out)                                                                                     \
+    do {                                                                                                      \
+        xo_aff W = aff_mul(nu_select((sel), (a)), tw);                                                        \
+        ++evals;                                                                                              \
+        xo_rfromangle_affine(deg, ap, av, na, bp, bv, nb, Tdfm, W.t, olap1D, reqArea, Ox, Oy, Rx, Ry, &X, &Y, \
+                             &R);                                                                             \
+        (out) = R;                                                                                            \
+    } while (0)
+    for (int itype = 0; itype < 5; ++itype) {
+        int vsel[5 * 11];  /* 5 types x (2 nstep + 1) magnitudes */
+        double va[5 * 11];
+        int n = 0;
+        for (int sel = 0; sel < 5; ++sel) {
+            if (vused[sel]) continue;
+            double abase = (sel >= 0 && sel <= 2) ? 1.0 : 0.0;
+            for (int j = -nstep; j <= nstep; ++j) {
+                vsel[n] = sel;
+                va[n] = abase + j * astep;
+                ++n;
+            }
+        }
+        double rbest = 0;
+        int ibest = -1;
+        for (int ic = 0; ic < n; ++ic) {
+            double r;
+            EVAL(vsel[ic], va[ic], r);
+            if (r > rbest) {
+                rbest = r;
+                ibest = ic;
+            }
+        }
+        if (ibest == -1) break;
+        int sel = vsel[ibest];
+        double x1 = va[ibest], y1 = rbest, d = astep, y0, y2, ynew, xnew;
+        EVAL(sel, x1 - d, y0);
+        EVAL(sel, x1 + d, y2);
+        double Dp = d * (y0 - y2) / (2 * (y0 + y2 - y1 - y1));
+        xnew = x1;
+        if (fabs(Dp) <= 2 * d) xnew += Dp;
+        EVAL(sel, xnew, ynew);
+        if (ynew < y1) {
+            xnew = x1;
+            ynew = y1;
+        }
+        rbest = ynew;
+        if (rbest > bR) {
+            tw = aff_mul(nu_select(sel, xnew), tw);
+            vused[sel] = 1;
+            bR = rbest;
+            any = 1;
+        } else
+            break;
+    }
+#undef EVAL
+    memcpy(Tptwk, tw.t, sizeof tw.t);
+    *bestR = bR;
+    if (changed) *changed = any;
+    if (n_evals) *n_evals = evals;
+    return XC_OK;
+}

@@ -301,3 +301,19 @@ def f_pair(seed, lo=12, hi=20, field=40):
     if seed % 3 == 2:
         B = rng.standard_normal((h2, w2))
     return A, B, (sx, sy)
+
+
+def xscaled_pair(seed=12, size=96, s=1.04):
+    """A: size x size crop of a band-limited field; B: the same field sampled at (x s, y) (bilinear),
+    so A's points mapped by diag(1/s, 1) land on B's content.  Both normalised."""
+    from scipy.ndimage import map_coordinates
+
+    from alignment_projects_amd import synth
+    rng = np.random.default_rng(seed)
+    Fd = synth.bandlimited(rng, size + 64, size + 64, 0.0, 1.0)
+    yy, xx = np.mgrid[0:size, 0:size].astype(float)
+    A = Fd[30:30 + size, 30:30 + size]
+    B = map_coordinates(Fd, [yy + 30, xx * s + 30], order=1)
+    A = synth.normalize(A.ravel()).reshape(A.shape)
+    B = synth.normalize(B.ravel()).reshape(B.shape)
+    return A, B

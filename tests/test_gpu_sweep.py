@@ -130,3 +130,23 @@ def test_peak_hunt_tiles_f32(ctx):
     assert 10 <= gk <= 16
     ta.release()
     tb.release()
+
+
+def test_rfromangle_affine_and_pretweaks_f64_match_oracle(ctx):
+    """RFromAngle with affines and Pretweaks in FP64 against the oracle: the same R values to 1e-9,
+    hence the same choices: identical composed tweak, R, change flag and evaluation count."""
+    from tests import bruteforce as bf
+    A, B = bf.xscaled_pair(14)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    Tdfm, Ttw = [1.01, 0.02, 3.0, -0.01, 0.99, -2.0], [0.98, 0.0, 0.0, 0.01, 1.0, 0.0]
+    g = ctx.rfromangle_affine(1.25, pa, va, pb, vb, Tdfm, Ttw, olap, req)
+    st, R, X, Y = oracle.rfromangle_affine(1.25, pa, va, pb, vb, Tdfm, Ttw, olap, req)
+    assert abs(g.R - R) <= 1e-9 * max(1.0, abs(R)) and abs(g.X - X) <= 1e-9 and abs(g.Y - Y) <= 1e-9
+    r0 = ctx.rfromangle(0.0, pa, va, pb, vb, olap, req).R
+    gW, gr, gch, gn = ctx.pretweaks(pa, va, pb, vb, 0.0, None, None, r0, olap, req)
+    oW, orr, och, on = oracle.pretweaks(pa, va, pb, vb, 0.0, None, None, r0, olap, req)
+    assert (gch, gn) == (och, on)
+    assert np.abs(gW - oW).max() <= 1e-12 and abs(gr - orr) <= 1e-9 * max(1.0, abs(orr))
+    assert gch == 1 and abs(gW[0] - 1 / 1.04) <= 0.005

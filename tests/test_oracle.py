@@ -448,3 +448,29 @@ def test_angle_scan_and_peak_hunt_recover_rotation():
     hunted, k = oracle.peak_hunt(best, pa, va, pb, vb, 0.5, olap, req)
     # the bracket halves twice per round from 1 deg to < 1e-4 deg; the result is never worse than the start
     assert abs(hunted.A - deg) <= 0.5 and hunted.R >= best.R and 10 <= k <= 16
+
+
+def test_rfromangle_affine_identity_is_rfromangle():
+    A, B = bf.xscaled_pair(13, size=48)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    for deg in (0.0, 1.5, -2.0):
+        st, R, X, Y, _ = oracle.rfromangle(deg, pa, va, pb, vb, 12, 48 * 48 // 4)
+        st2, R2, X2, Y2 = oracle.rfromangle_affine(deg, pa, va, pb, vb, None, [1, 0, 0, 0, 1, 0], 12, 48 * 48 // 4)
+        assert (st, R, X, Y) == (st2, R2, X2, Y2)
+
+
+def test_pretweaks_recovers_x_scale():
+    """B is A's field stretched by 1.04 along x: Pretweaks picks the x-scale type and, after the
+    parabola, a magnitude within 0.005 of 1/1.04; R improves; the other terms stay at identity."""
+    A, B = bf.xscaled_pair(12)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    st, R0, _, _ = oracle.rfromangle_affine(0.0, pa, va, pb, vb, None, None, olap, req)
+    W, r, changed, n = oracle.pretweaks(pa, va, pb, vb, 0.0, None, None, R0, olap, req)
+    assert changed == 1 and r > R0
+    assert abs(W[0] - 1 / 1.04) <= 0.005 and abs(W[1]) <= 0.01 and abs(W[3]) <= 0.01 and abs(W[4] - 1) <= 0.01
+    # each round sweeps 11 magnitudes per unused type and spends 3 more on the parabola
+    assert n in (55 + 3, 55 + 3 + 44 + 3, 55 + 3 + 44 + 3 + 33 + 3, 55 + 3 + 44 + 3 + 33 + 3 + 22 + 3,
+                 55 + 3 + 44 + 3 + 33 + 3 + 22 + 3 + 11 + 3)
