@@ -77,6 +77,24 @@ class XcAngleRec(C.Structure):
     _fields_ = [("A", C.c_double), ("R", C.c_double), ("X", C.c_double), ("Y", C.c_double)]
 
 
+class XcThmParams(C.Structure):
+    """CThmScan's matcher state for one pair (CThmScan.h:85-97) as the thm entry points read and update it."""
+    _fields_ = [("rthresh", C.c_double), ("const_xy", C.c_int32), ("pretweak", C.c_int32), ("olap1D", C.c_int32),
+                ("Ox", C.c_int32), ("Oy", C.c_int32), ("Rx", C.c_int32), ("Ry", C.c_int32), ("n_evals", C.c_int32),
+                ("req_area", C.c_int64), ("Tdfm", C.c_double * 6), ("Tptwk", C.c_double * 6)]
+
+    @classmethod
+    def default(cls, olap1D=0, req_area=0):
+        """CThmScan's constructor values (CThmScan.cpp:160-177)."""
+        p = cls()
+        p.rthresh, p.const_xy, p.pretweak, p.olap1D, p.req_area = 0.30, 1, 1, olap1D, req_area
+        p.Ox, p.Oy, p.Rx, p.Ry = 0, 0, -1, -1
+        for i, v in enumerate((1, 0, 0, 0, 1, 0)):
+            p.Tdfm[i] = v
+            p.Tptwk[i] = v
+        return p
+
+
 class XcThmPair(C.Structure):
     """One ThmPair_za^zb.txt entry (ThmPair, 0_GEN/PipeFiles.h:145-152)."""
     _fields_ = [("atl", C.c_int32), ("acr", C.c_int32), ("btl", C.c_int32), ("bcr", C.c_int32),

@@ -342,16 +342,14 @@ extern "C" int xc_rfromangle_affine_f64(xc_ctx *ctx, const xc_point *ap, const d
     return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, scan_rules(olap1D, req_area), Ox, Oy, Rx, Ry, buf, rec);
 }
 
-extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
-                                const double *bv, int nb, double deg, const double *Tdfm, double *Tptwk,
-                                double *bestR, int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
-                                int *changed, int *n_evals)
+namespace {
+
+// Pretweaks (CThmScan.cpp:205-323) with PTWInterp (:119-160) and NewXFromParabola (:48-61)
+// at angle deg: tw (the pretweak) and bR (the best R) in/out.
+int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                  int nb, double deg, const Aff &D, Aff &tw, double &bR, const Rules &ru, int Ox, int Oy, int Rx,
+                  int Ry, std::vector<xc_point> &buf, bool *changed, int *n_evals)
 {
-    if (!ctx || !Tptwk || !bestR) return XC_E_ARG;
-    const Rules ru = scan_rules(olap1D, req_area);
-    const Aff D = aff_of(Tdfm);
-    Aff tw = aff_of(Tptwk);
-    std::vector<xc_point> buf;
     int evals = 0, st;
     auto eval = [&](int sel, double a, xc_angle_rec *C) {
         const Aff W = aff_mul(nu_select(sel, a), tw);  // M * Tptwk (RFromAngle's iaux branch)
@@ -362,7 +360,6 @@ extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *a
     const int nstep = 5;
     bool any = false;
     int vused[5] = {0, 0, 0, 0, 0};
-    double bR = *bestR;
     for (int itype = 0; itype < 5; ++itype) {
         // sweep every unused type's magnitude about its identity value
         std::vector<int> vsel;
@@ -386,7 +383,7 @@ extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *a
             }
         }
         if (ibest == -1) break;
-        // PTWInterp (CThmScan.cpp:119-160): parabola through the neighbours at +-astep
+        // PTWInterp: parabola through the neighbours at +-astep
         const int sel = vsel[ibest];
         const double x1 = va[ibest], y1 = rbest, d = astep;
         xc_angle_rec C;
@@ -397,7 +394,7 @@ extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *a
         double xnew = x1;
         {
 #pragma clang fp contract(off)
-            const double Dp = d * (y0 - y2) / (2 * (y0 + y2 - y1 - y1));  // NewXFromParabola (:48-61)
+            const double Dp = d * (y0 - y2) / (2 * (y0 + y2 - y1 - y1));
             if (fabs(Dp) <= 2 * d) xnew += Dp;
         }
         if ((st = eval(sel, xnew, &C))) return st;
@@ -416,9 +413,192 @@ extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *a
             break;
         }
     }
+    if (changed) *changed = any;
+    if (n_evals) *n_evals = evals;
+    return XC_OK;
+}
+
+// CThmScan's matching state for one pair of point lists: the parameter block (thresholds,
+// sweep switches, disc, Tdfm and the running Tptwk) and the evaluation count.
+struct Thm {
+    xc_ctx *ctx;
+    const xc_point *ap;
+    const double *av;
+    int na;
+    const xc_point *bp;
+    const double *bv;
+    int nb;
+    xc_thm_params *p;
+    Rules ru;
+    std::vector<xc_point> buf;
+
+    // RFromAngle (CThmScan.cpp:329-366) with the current Tdfm, Tptwk and disc
+    int rfa(double deg, xc_angle_rec *C)
+    {
+        const Aff D = aff_of(p->Tdfm), W = aff_of(p->Tptwk);
+        ++p->n_evals;
+        return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, p->Ox, p->Oy, p->Rx, p->Ry, buf, C);
+    }
+    // AngleScanSel (CThmScan.cpp:627-641)
+    int scan_sel(xc_angle_rec *best, double center, double hlfwid, double step, bool favor)
+    {
+        const int mode = p->const_xy ? XC_SCAN_CONSTXY : XC_SCAN_MAXR;
+        const std::vector<double> ang = scan_angles(mode, center, hlfwid, step, favor);
+        std::vector<xc_angle_rec> vC(ang.size());
+        for (size_t i = 0; i < ang.size(); ++i) {
+            const int st = rfa(ang[i], &vC[i]);
+            if (st) return st;
+        }
+        *best = scan_select(mode, vC);
+        return XC_OK;
+    }
+    // Pretweaks (CThmScan.cpp:205-323) folding into p->Tptwk
+    int pretweaks(double bestR, double deg, bool *changed)
+    {
+        Aff tw = aff_of(p->Tptwk);
+        double bR = bestR;
+        int ev = 0;
+        const int st = pretweaks_run(ctx, ap, av, na, bp, bv, nb, deg, aff_of(p->Tdfm), tw, bR, ru, p->Ox, p->Oy,
+                                     p->Rx, p->Ry, buf, changed, &ev);
+        p->n_evals += ev;
+        if (!st) memcpy(p->Tptwk, tw.t, sizeof tw.t);
+        return st;
+    }
+    // AngleScanWithTweaks (CThmScan.cpp:647-672)
+    int with_tweaks(xc_angle_rec *best, double center, double hlfwid, double step)
+    {
+        int st = scan_sel(best, center, hlfwid, step, true);
+        if (st || !(best->R < p->rthresh) || !p->pretweak) return st;
+        bool changed = false;
+        if ((st = pretweaks(best->R, best->R > 0.0 ? best->A : center, &changed))) return st;
+        return changed ? scan_sel(best, center, hlfwid, step, true) : XC_OK;
+    }
+    // PeakHunt (CThmScan.cpp:676-732)
+    int hunt(xc_angle_rec *best, double hlfwid)
+    {
+        return peak_hunt(hlfwid, best, nullptr, [&](double deg, xc_angle_rec *rec) { return rfa(deg, rec); });
+    }
+};
+
+bool bad_thm(xc_ctx *ctx, const xc_thm_params *p, const xc_angle_rec *best)
+{
+    return !ctx || !p || !best;
+}
+
+}  // namespace
+
+extern "C" int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp,
+                                const double *bv, int nb, double deg, const double *Tdfm, double *Tptwk,
+                                double *bestR, int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
+                                int *changed, int *n_evals)
+{
+    if (!ctx || !Tptwk || !bestR) return XC_E_ARG;
+    std::vector<xc_point> buf;
+    Aff tw = aff_of(Tptwk);
+    double bR = *bestR;
+    bool any = false;
+    const int st = pretweaks_run(ctx, ap, av, na, bp, bv, nb, deg, aff_of(Tdfm), tw, bR,
+                                 scan_rules(olap1D, req_area), Ox, Oy, Rx, Ry, buf, &any, n_evals);
+    if (st) return st;
     memcpy(Tptwk, tw.t, sizeof tw.t);
     *bestR = bR;
     if (changed) *changed = any ? 1 : 0;
-    if (n_evals) *n_evals = evals;
     return XC_OK;
+}
+
+extern "C" int xc_finish_at_full_res_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                         const xc_point *bp, const double *bv, int nb, const double *Tdfm,
+                                         const double *Tptwk, int olap1D, int64_t req_area, xc_angle_rec *best,
+                                         int *used)
+{
+    if (!ctx || !best) return XC_E_ARG;
+    const xc_angle_rec b0 = *best;
+    const Aff D = aff_of(Tdfm), W = aff_of(Tptwk);
+    std::vector<xc_point> buf;
+    xc_angle_rec C;
+    // SetDisc( (int)best.X, (int)best.Y, 80, 80 ), then RFromAngle( best, best.A )
+    const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, b0.A, &D, &W, scan_rules(olap1D, req_area), (int)b0.X,
+                             (int)b0.Y, 80, 80, buf, &C);
+    if (st) return st;
+    const bool ok = fabs(C.X - b0.X) <= 20 && fabs(C.Y - b0.Y) <= 20;
+    if (ok) {
+        *best = C;
+        best->R = b0.R;  // the thumbnail R, comparable across the table
+    } else {
+        *best = b0;
+    }
+    if (used) *used = ok ? 1 : 0;
+    return XC_OK;
+}
+
+extern "C" void xc_thm_params_default(xc_thm_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof *p);
+    p->rthresh = 0.30;
+    p->const_xy = 1;
+    p->pretweak = 1;
+    p->Rx = p->Ry = -1;
+    p->Tdfm[0] = p->Tdfm[4] = 1.0;
+    p->Tptwk[0] = p->Tptwk[4] = 1.0;
+}
+
+extern "C" int xc_thm_scan_with_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                           const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                           double center, double hlfwid, double step, xc_angle_rec *best)
+{
+    if (bad_thm(ctx, p, best) || !(step > 0.0)) return XC_E_ARG;
+    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
+    return t.with_tweaks(best, center, hlfwid, step);
+}
+
+extern "C" int xc_thm_use_prior_angles_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                           const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                           double ang0, double hfangpr, xc_angle_rec *best, int *ok)
+{
+    if (bad_thm(ctx, p, best)) return XC_E_ARG;
+    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
+    int st = t.with_tweaks(best, ang0, hfangpr, 0.1);
+    bool good = !st && !(best->R < p->rthresh);
+    if (good) {
+        st = t.hunt(best, 0.1 * 1.5);
+        good = !st && !(best->R < p->rthresh);
+    }
+    if (ok) *ok = good ? 1 : 0;
+    return st;
+}
+
+extern "C" int xc_thm_denovo_best_angle_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                            const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                            double ang0, double hfangdn, double step, xc_angle_rec *best, int *ok)
+{
+    if (bad_thm(ctx, p, best) || !(step > 0.0)) return XC_E_ARG;
+    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
+    int st = t.with_tweaks(best, ang0, hfangdn, step);
+    bool good = !st && !(best->R < p->rthresh);
+    if (good) {
+        st = t.scan_sel(best, best->A, step * 2.0, step * 0.05, false);
+        good = !st && !(best->R < p->rthresh);
+    }
+    if (good) {
+        st = t.hunt(best, step * 0.05 * 1.5);
+        good = !st && !(best->R < p->rthresh);
+    }
+    if (ok) *ok = good ? 1 : 0;
+    return st;
+}
+
+extern "C" int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                      const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                      xc_angle_rec *best)
+{
+    if (bad_thm(ctx, p, best)) return XC_E_ARG;
+    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
+    p->Ox = (int)best->X;  // SetDisc( (int)best.X, (int)best.Y, 40, 40 )
+    p->Oy = (int)best->Y;
+    p->Rx = p->Ry = 40;
+    bool changed = false;
+    const int st = t.pretweaks(best->R, best->A, &changed);
+    if (st) return st;
+    return t.rfa(best->A, best);
 }

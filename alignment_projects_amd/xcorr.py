@@ -103,6 +103,18 @@ _OPTIONAL = {
     "xc_pretweaks_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_double, _dp, _dp, _dp, C.c_int,
                           C.c_int64, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
                          C.c_int),
+    "xc_finish_at_full_res_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_int64,
+                                   C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
+    "xc_thm_params_default": ([C.POINTER(abi.XcThmParams)], None),
+    "xc_thm_scan_with_tweaks_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(abi.XcThmParams),
+                                     C.c_double, C.c_double, C.c_double, C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_thm_use_prior_angles_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(abi.XcThmParams),
+                                     C.c_double, C.c_double, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
+    "xc_thm_denovo_best_angle_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(abi.XcThmParams),
+                                      C.c_double, C.c_double, C.c_double, C.POINTER(abi.XcAngleRec),
+                                      C.POINTER(C.c_int)], C.c_int),
+    "xc_thm_post_tweaks_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(abi.XcThmParams),
+                                C.POINTER(abi.XcAngleRec)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -345,6 +357,59 @@ class Context:
                                      C.byref(r), olap1d, req_area, Ox, Oy, Rx, Ry, C.byref(ch), C.byref(ne))
         self._check(st, "xc_pretweaks_f64", ok=(abi.XC_OK,))
         return W, r.value, ch.value, ne.value
+
+    def finish_at_full_res(self, best, ap, av, bp, bv, Tdfm, Tptwk, olap1d, req_area):
+        """FinishAtFullRes in FP64 on the full-resolution point lists. Returns (best, used)."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+        W = None if Tptwk is None else np.ascontiguousarray(Tptwk, dtype=np.float64)
+        b, u = abi.XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+        st = self.L.xc_finish_at_full_res_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2),
+                                              None if D is None else D.ctypes.data_as(_dp),
+                                              None if W is None else W.ctypes.data_as(_dp), olap1d, req_area,
+                                              C.byref(b), C.byref(u))
+        self._check(st, "xc_finish_at_full_res_f64", ok=(abi.XC_OK,))
+        return b, u.value
+
+    def _thm(self, name, ap, av, bp, bv, params, *args, ok=False):
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        best = abi.XcAngleRec()
+        extra = [C.byref(best)]
+        okv = C.c_int()
+        if ok:
+            extra.append(C.byref(okv))
+        st = getattr(self.L, name)(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), C.byref(params), *args, *extra)
+        self._check(st, name, ok=(abi.XC_OK,))
+        return (best, okv.value) if ok else best
+
+    def thm_scan_with_tweaks(self, ap, av, bp, bv, params, center, hlfwid, step):
+        """AngleScanWithTweaks in FP64; params (an XcThmParams) is updated in place. Returns best."""
+        return self._thm("xc_thm_scan_with_tweaks_f64", ap, av, bp, bv, params, center, hlfwid, step)
+
+    def thm_use_prior_angles(self, ap, av, bp, bv, params, ang0, hfangpr):
+        """UsePriorAngles in FP64. Returns (best, ok)."""
+        return self._thm("xc_thm_use_prior_angles_f64", ap, av, bp, bv, params, ang0, hfangpr, ok=True)
+
+    def thm_denovo_best_angle(self, ap, av, bp, bv, params, ang0, hfangdn, step):
+        """DenovoBestAngle in FP64. Returns (best, ok)."""
+        return self._thm("xc_thm_denovo_best_angle_f64", ap, av, bp, bv, params, ang0, hfangdn, step, ok=True)
+
+    def thm_post_tweaks(self, ap, av, bp, bv, params, best):
+        """PostTweaks in FP64 from the record best. Returns the new record."""
+        p1, pp1 = _pts(ap)
+        v1, pv1 = _vals(av)
+        p2, pp2 = _pts(bp)
+        v2, pv2 = _vals(bv)
+        b = abi.XcAngleRec(best.A, best.R, best.X, best.Y)
+        st = self.L.xc_thm_post_tweaks_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), C.byref(params), C.byref(b))
+        self._check(st, "xc_thm_post_tweaks_f64", ok=(abi.XC_OK,))
+        return b
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

@@ -400,6 +400,66 @@ int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
                      int *changed, int *n_evals);
 
+/*
+ * CThmScan::FinishAtFullRes (CThmScan.cpp:810-842): RFromAngle at best->A
+ * on the full-resolution point lists with the oval ((int)X, (int)Y, 80, 80)
+ * about the thumbnail offset; when the new X and Y both lie within 20 of
+ * it they replace it (R stays the thumbnail R), else *best is kept.
+ * *used receives 1 or 0.
+ */
+int xc_finish_at_full_res_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                              const xc_point *bp, const double *bv, int nb,
+                              const double *Tdfm, const double *Tptwk,
+                              int olap1D, int64_t req_area, xc_angle_rec *best, int *used);
+
+/*
+ * CThmScan's matcher state for one pair (CThmScan.h:85-97): thresholds,
+ * sweep switches, the disc and the two affines. The entry points below
+ * read it and update it as the reference's methods update their object:
+ * Pretweaks folds into Tptwk, PostTweaks sets the disc. n_evals counts
+ * the RFromAngle evaluations made (the caller sets it; each call adds).
+ * xc_thm_params_default gives the constructor's values (CThmScan.cpp:
+ * 160-177): rthresh 0.30, ConstXY and Pretweak sweeps on, no disc
+ * (0, 0, -1, -1), identity affines.
+ */
+typedef struct xc_thm_params {
+    double  rthresh;
+    int32_t const_xy;     /* swpConstXY: AngleScanConstXY, else AngleScanMaxR */
+    int32_t pretweak;     /* swpPretweak */
+    int32_t olap1D;
+    int32_t Ox, Oy, Rx, Ry;
+    int32_t n_evals;
+    int64_t req_area;
+    double  Tdfm[6];
+    double  Tptwk[6];
+} xc_thm_params;
+
+void xc_thm_params_default(xc_thm_params *p);
+
+/* AngleScanWithTweaks (CThmScan.cpp:647-672): AngleScanSel(favorCenter);
+ * below rthresh and with the Pretweak sweep on, Pretweaks at best->A (or
+ * center when best->R <= 0) and, when that changed Tptwk, the scan again. */
+int xc_thm_scan_with_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                double center, double hlfwid, double step, xc_angle_rec *best);
+/* UsePriorAngles (CThmScan.cpp:734-758): the tweaked scan about ang0 at
+ * step 0.1, then PeakHunt(0.15); *ok = 0 when either R < rthresh. */
+int xc_thm_use_prior_angles_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                double ang0, double hfangpr, xc_angle_rec *best, int *ok);
+/* DenovoBestAngle (CThmScan.cpp:764-790): the tweaked scan, AngleScanSel
+ * about best->A over +-2 step at step/20, PeakHunt(0.075 step); *ok = 0
+ * at the first R < rthresh (the later stages are then skipped). */
+int xc_thm_denovo_best_angle_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                                 const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                                 double ang0, double hfangdn, double step, xc_angle_rec *best,
+                                 int *ok);
+/* PostTweaks (CThmScan.cpp:796-802): disc ((int)X, (int)Y, 40, 40),
+ * Pretweaks at best->A, RFromAngle at best->A. */
+int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
+                           const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
+                           xc_angle_rec *best);
+
 /* ------------------------------------------------------------------ */
 /* ThmPair tables (the thumbnail matcher's result file)                */
 /* ------------------------------------------------------------------ */

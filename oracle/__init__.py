@@ -18,7 +18,7 @@ import subprocess
 
 import numpy as np
 
-from alignment_projects_amd.abi import XcAngleRec, XcCntRule, XcPoint, XcResult, XcRgnRule
+from alignment_projects_amd.abi import XcAngleRec, XcCntRule, XcPoint, XcResult, XcRgnRule, XcThmParams
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -78,6 +78,15 @@ def lib():
                                             C.c_long, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.xo_pretweaks.argtypes = [C.c_double, _pp, _dp, C.c_int, _pp, _dp, C.c_int, _dp, _dp, _dp, C.c_int, C.c_long,
                                    C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.xo_finish_at_full_res.argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, _dp, _dp, C.c_int, C.c_long,
+                                             C.POINTER(XcAngleRec), C.POINTER(C.c_int)]
+        for nm, extra in (("xo_thm_scan_with_tweaks", [C.c_double, C.c_double, C.c_double, C.POINTER(XcAngleRec)]),
+                          ("xo_thm_use_prior_angles", [C.c_double, C.c_double, C.POINTER(XcAngleRec),
+                                                       C.POINTER(C.c_int)]),
+                          ("xo_thm_denovo_best_angle", [C.c_double, C.c_double, C.c_double, C.POINTER(XcAngleRec),
+                                                        C.POINTER(C.c_int)]),
+                          ("xo_thm_post_tweaks", [C.POINTER(XcAngleRec)])):
+            getattr(L, nm).argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(XcThmParams)] + extra
         L.xo_convolve.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_fpeak.argtypes = [_dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_double, _ip, _ip]
         _lib = L
@@ -390,3 +399,49 @@ def rfromangle_affine(deg, ap, av, bp, bv, Tdfm, Ttw, olap1d, req_area, Ox=0, Oy
                                     None if W is None else W.ctypes.data_as(_dp), olap1d, req_area, Ox, Oy, Rx, Ry,
                                     C.byref(X), C.byref(Y), C.byref(R))
     return st, R.value, X.value, Y.value
+
+
+def finish_at_full_res(best, ap, av, bp, bv, Tdfm, Tptwk, olap1d, req_area):
+    """CThmScan::FinishAtFullRes (0_GEN/CThmScan.cpp:810-842). Returns (best, used)."""
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+    W = None if Tptwk is None else np.ascontiguousarray(Tptwk, dtype=np.float64)
+    b, u = XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+    lib().xo_finish_at_full_res(pp1, pv1, len(v1), pp2, pv2, len(v2), None if D is None else D.ctypes.data_as(_dp),
+                                None if W is None else W.ctypes.data_as(_dp), olap1d, req_area, C.byref(b), C.byref(u))
+    return b, u.value
+
+
+def _thm(name, ap, av, bp, bv, params, *args, ok=False, best=None):
+    p1, pp1 = _pts(ap)
+    v1, pv1 = _vals(av)
+    p2, pp2 = _pts(bp)
+    v2, pv2 = _vals(bv)
+    b = XcAngleRec() if best is None else XcAngleRec(best.A, best.R, best.X, best.Y)
+    okv = C.c_int()
+    extra = [C.byref(b)] + ([C.byref(okv)] if ok else [])
+    getattr(lib(), name)(pp1, pv1, len(v1), pp2, pv2, len(v2), C.byref(params), *args, *extra)
+    return (b, okv.value) if ok else b
+
+
+def thm_scan_with_tweaks(ap, av, bp, bv, params, center, hlfwid, step):
+    """AngleScanWithTweaks (0_GEN/CThmScan.cpp:647-672); params updated in place."""
+    return _thm("xo_thm_scan_with_tweaks", ap, av, bp, bv, params, center, hlfwid, step)
+
+
+def thm_use_prior_angles(ap, av, bp, bv, params, ang0, hfangpr):
+    """UsePriorAngles (0_GEN/CThmScan.cpp:734-758). Returns (best, ok)."""
+    return _thm("xo_thm_use_prior_angles", ap, av, bp, bv, params, ang0, hfangpr, ok=True)
+
+
+def thm_denovo_best_angle(ap, av, bp, bv, params, ang0, hfangdn, step):
+    """DenovoBestAngle (0_GEN/CThmScan.cpp:764-790). Returns (best, ok)."""
+    return _thm("xo_thm_denovo_best_angle", ap, av, bp, bv, params, ang0, hfangdn, step, ok=True)
+
+
+def thm_post_tweaks(ap, av, bp, bv, params, best):
+    """PostTweaks (0_GEN/CThmScan.cpp:796-802) from the record best."""
+    return _thm("xo_thm_post_tweaks", ap, av, bp, bv, params, best=best)

@@ -1395,3 +1395,147 @@ int xo_pretweaks(double deg, const xc_point *ap, const double *av, int na, const
     if (n_evals) *n_evals = evals;
     return XC_OK;
 }
+
+/* CThmScan::FinishAtFullRes (CThmScan.cpp:810-842). */
+int xo_finish_at_full_res(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
+                          const double *Tdfm, const double *Tptwk, int olap1D, long reqArea, xc_angle_rec *best,
+                          int *used)
+{
+    xc_angle_rec b0 = *best, C;
+    C.A = b0.A;
+    xo_rfromangle_affine(b0.A, ap, av, na, bp, bv, nb, Tdfm, Tptwk, olap1D, reqArea, (int)b0.X, (int)b0.Y, 80, 80,
+                         &C.X, &C.Y, &C.R);
+    int ok = (fabs(C.X - b0.X) <= 20) && (fabs(C.Y - b0.Y) <= 20);
+    if (ok) {
+        *best = C;
+        best->R = b0.R;
+    } else
+        *best = b0;
+    if (used) *used = ok;
+    return XC_OK;
+}
+
+/* ---- CThmScan's matching sequence (CThmScan.cpp:627-802) over an xc_thm_params block ---- */
+
+typedef struct {
+    const xc_point *ap, *bp;
+    const double *av, *bv;
+    int na, nb;
+    xc_thm_params *p;
+} xo_thm;
+
+/* RFromAngle (:329-366) with the block's Tdfm, Tptwk and disc. */
+static void thm_rfa(xo_thm *t, double deg, xc_angle_rec *C)
+{
+    xc_thm_params *p = t->p;
+    C->A = deg;
+    ++p->n_evals;
+    xo_rfromangle_affine(deg, t->ap, t->av, t->na, t->bp, t->bv, t->nb, p->Tdfm, p->Tptwk, p->olap1D, p->req_area,
+                         p->Ox, p->Oy, p->Rx, p->Ry, &C->X, &C->Y, &C->R);
+}
+
+/* AngleScanSel (:627-641). */
+static double thm_scan_sel(xo_thm *t, xc_angle_rec *best, double center, double hlfwid, double step, int favor)
+{
+    int mode = t->p->const_xy ? XC_SCAN_CONSTXY : XC_SCAN_MAXR;
+    int nC = xo_scan_angles(mode, center, hlfwid, step, favor, NULL, 0);
+    double *a = (double *)malloc(sizeof(double) * (nC ? nC : 1));
+    xc_angle_rec *vC = (xc_angle_rec *)malloc(sizeof(xc_angle_rec) * (nC ? nC : 1));
+    xo_scan_angles(mode, center, hlfwid, step, favor, a, nC);
+    for (int i = 0; i < nC; ++i) thm_rfa(t, a[i], &vC[i]);
+    if (nC > 0) xo_scan_select(mode, vC, nC, best);
+    else best->X = best->Y = best->A = best->R = 0.0;
+    free(a);
+    free(vC);
+    return best->R;
+}
+
+/* Pretweaks (:205-323) folding into the block's Tptwk. */
+static int thm_pretweaks(xo_thm *t, double bestR, double deg)
+{
+    xc_thm_params *p = t->p;
+    int changed = 0, ev = 0;
+    xo_pretweaks(deg, t->ap, t->av, t->na, t->bp, t->bv, t->nb, p->Tdfm, p->Tptwk, &bestR, p->olap1D, p->req_area,
+                 p->Ox, p->Oy, p->Rx, p->Ry, &changed, &ev);
+    p->n_evals += ev;
+    return changed;
+}
+
+/* AngleScanWithTweaks (:647-672). */
+static double thm_with_tweaks(xo_thm *t, xc_angle_rec *best, double center, double hlfwid, double step)
+{
+    if (thm_scan_sel(t, best, center, hlfwid, step, 1) < t->p->rthresh && t->p->pretweak) {
+        if (thm_pretweaks(t, best->R, best->R > 0.0 ? best->A : center))
+            thm_scan_sel(t, best, center, hlfwid, step, 1);
+    }
+    return best->R;
+}
+
+/* PeakHunt (:676-732). */
+static double thm_hunt(xo_thm *t, xc_angle_rec *best, double hlfwid)
+{
+    xc_angle_rec C, B0 = *best;
+    double L = best->A - hlfwid, R = best->A + hlfwid, M;
+    thm_rfa(t, M = (L + R) / 2.0, best);
+    while (R - L > 0.0001) {
+        double a;
+        thm_rfa(t, a = (L + M) / 2.0, &C);
+        if (C.R >= best->R) {
+            R = M;
+            *best = C;
+            M = a;
+        } else
+            L = a;
+        thm_rfa(t, a = (M + R) / 2.0, &C);
+        if (C.R >= best->R) {
+            L = M;
+            *best = C;
+            M = a;
+        } else
+            R = a;
+    }
+    if (B0.R > best->R) *best = B0;
+    return best->R;
+}
+
+int xo_thm_scan_with_tweaks(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                            int nb, xc_thm_params *p, double center, double hlfwid, double step, xc_angle_rec *best)
+{
+    xo_thm t = {ap, bp, av, bv, na, nb, p};
+    thm_with_tweaks(&t, best, center, hlfwid, step);
+    return XC_OK;
+}
+
+/* UsePriorAngles (:734-758). */
+int xo_thm_use_prior_angles(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                            int nb, xc_thm_params *p, double ang0, double hfangpr, xc_angle_rec *best, int *ok)
+{
+    xo_thm t = {ap, bp, av, bv, na, nb, p};
+    *ok = !(thm_with_tweaks(&t, best, ang0, hfangpr, 0.1) < p->rthresh || thm_hunt(&t, best, 0.1 * 1.5) < p->rthresh);
+    return XC_OK;
+}
+
+/* DenovoBestAngle (:764-790). */
+int xo_thm_denovo_best_angle(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                             int nb, xc_thm_params *p, double ang0, double hfangdn, double step, xc_angle_rec *best,
+                             int *ok)
+{
+    xo_thm t = {ap, bp, av, bv, na, nb, p};
+    *ok = !(thm_with_tweaks(&t, best, ang0, hfangdn, step) < p->rthresh ||
+            thm_scan_sel(&t, best, best->A, step * 2.0, step * 0.05, 0) < p->rthresh ||
+            thm_hunt(&t, best, step * 0.05 * 1.5) < p->rthresh);
+    return XC_OK;
+}
+
+/* PostTweaks (:796-802). */
+int xo_thm_post_tweaks(const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv, int nb,
+                       xc_thm_params *p, xc_angle_rec *best)
+{
+    xo_thm t = {ap, bp, av, bv, na, nb, p};
+    p->Ox = (int)best->X;
+    p->Oy = (int)best->Y;
+    p->Rx = p->Ry = 40;
+    thm_pretweaks(&t, best->R, best->A);
+    thm_rfa(&t, best->A, best);
+    return XC_OK;
+}

@@ -34,6 +34,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.XcServeStats) == 64
     assert ctypes.sizeof(abi.XcAngleRec) == 32
     assert ctypes.sizeof(abi.XcThmPair) == 6 * 4 + 2 * 8 + 6 * 8
+    assert ctypes.sizeof(abi.XcThmParams) == 8 + 8 * 4 + 8 + 12 * 8
     assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
 
 
@@ -63,3 +64,14 @@ def test_compat_library_exports_reference_signatures():
     assert {"CorrPatches", "CorrPatchToImage", "CorrImagesF", "CorrImagesR", "CorrImagesS", "FFT_2D", "IFT_2D"} <= set(names)
     missing = [n for n in names if f" {n}(" not in syms]
     assert not missing, missing
+
+
+def test_thm_params_default_matches_the_library():
+    """xc_thm_params_default (host only) writes the same block, field by field, as the Python default."""
+    L = xcorr.load()
+    a, b = abi.XcThmParams(), abi.XcThmParams.default()
+    a.n_evals = 7
+    L.xc_thm_params_default(ctypes.byref(a))
+    for name, _ in abi.XcThmParams._fields_:
+        va, vb = getattr(a, name), getattr(b, name)
+        assert (list(va) == list(vb)) if hasattr(va, "__len__") else va == vb, name

@@ -150,3 +150,73 @@ def test_rfromangle_affine_and_pretweaks_f64_match_oracle(ctx):
     assert (gch, gn) == (och, on)
     assert np.abs(gW - oW).max() <= 1e-12 and abs(gr - orr) <= 1e-9 * max(1.0, abs(orr))
     assert gch == 1 and abs(gW[0] - 1 / 1.04) <= 0.005
+
+
+def test_finish_at_full_res_f64_matches_oracle(ctx):
+    """FinishAtFullRes in FP64 against the oracle, with a deformation and a tweak: the same use flag and
+    the same record (X and Y within 1e-9 px), on a near and a far thumbnail offset."""
+    A, B = synth.rotated_pair(11, size=96, deg=0.0, shift=(2.0, -1.0))
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    Tdfm, Ttw = [1.0, 0.01, 0.0, -0.01, 1.0, 0.0], [0.995, 0.0, 0.0, 0.0, 1.0, 0.0]
+    for x, y in ((7.4, -4.2), (32.0, -1.0)):
+        best = abi.XcAngleRec(0.25, 0.55, x, y)
+        g, gu = ctx.finish_at_full_res(best, pa, va, pb, vb, Tdfm, Ttw, olap, req)
+        o, ou = oracle.finish_at_full_res(best, pa, va, pb, vb, Tdfm, Ttw, olap, req)
+        assert gu == ou and g.A == o.A and g.R == o.R
+        assert abs(g.X - o.X) <= 1e-9 and abs(g.Y - o.Y) <= 1e-9
+
+
+def _thm_inputs(deg=2.0):
+    A, B, olap, req = _scan_pair(11, deg)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    return pa, va, pb, vb, olap, req
+
+
+def _same_thm(g, o, gp, op):
+    assert g.A == o.A and abs(g.R - o.R) <= 1e-9 * max(1.0, abs(o.R))
+    assert abs(g.X - o.X) <= 1e-9 and abs(g.Y - o.Y) <= 1e-9
+    assert gp.n_evals == op.n_evals and (gp.Ox, gp.Oy, gp.Rx, gp.Ry) == (op.Ox, op.Oy, op.Rx, op.Ry)
+    assert np.abs(np.array(gp.Tptwk) - np.array(op.Tptwk)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("const_xy", [1, 0])
+def test_thm_denovo_and_prior_f64_match_oracle(ctx, const_xy):
+    """DenovoBestAngle and UsePriorAngles in FP64 against the oracle, under both scan selections: the
+    same success flag, evaluation count and record."""
+    pa, va, pb, vb, olap, req = _thm_inputs()
+    for run in ("denovo", "prior"):
+        gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+        gp.const_xy = op.const_xy = const_xy
+        if run == "denovo":
+            g, gok = ctx.thm_denovo_best_angle(pa, va, pb, vb, gp, 0.0, 5.0, 0.5)
+            o, ook = oracle.thm_denovo_best_angle(pa, va, pb, vb, op, 0.0, 5.0, 0.5)
+        else:
+            g, gok = ctx.thm_use_prior_angles(pa, va, pb, vb, gp, 1.8, 0.5)
+            o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 1.8, 0.5)
+        assert gok == ook == 1
+        _same_thm(g, o, gp, op)
+
+
+def test_thm_tweaked_paths_f64_match_oracle(ctx):
+    """The Pretweaks branches in FP64 against the oracle: a failing tweaked scan (rthresh above every R,
+    so Pretweaks runs and the scan may repeat) and PostTweaks on an x-stretched pair."""
+    from tests import bruteforce as bf
+    pa, va, pb, vb, olap, req = _thm_inputs()
+    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+    gp.rthresh = op.rthresh = 1e9
+    g, gok = ctx.thm_use_prior_angles(pa, va, pb, vb, gp, 1.8, 0.5)
+    o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 1.8, 0.5)
+    assert gok == ook == 0
+    _same_thm(g, o, gp, op)
+    A, B = bf.xscaled_pair(12)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    gp, op = abi.XcThmParams.default(24, 96 * 96 // 4), abi.XcThmParams.default(24, 96 * 96 // 4)
+    start = abi.XcAngleRec(0.0, 1.0, 0.4, -0.3)
+    g = ctx.thm_post_tweaks(pa, va, pb, vb, gp, start)
+    o = oracle.thm_post_tweaks(pa, va, pb, vb, op, start)
+    _same_thm(g, o, gp, op)
+    assert abs(gp.Tptwk[0] - 1 / 1.04) <= 0.005

@@ -474,3 +474,85 @@ def test_pretweaks_recovers_x_scale():
     # each round sweeps 11 magnitudes per unused type and spends 3 more on the parabola
     assert n in (55 + 3, 55 + 3 + 44 + 3, 55 + 3 + 44 + 3 + 33 + 3, 55 + 3 + 44 + 3 + 33 + 3 + 22 + 3,
                  55 + 3 + 44 + 3 + 33 + 3 + 22 + 3 + 11 + 3)
+
+
+def test_finish_at_full_res_accepts_near_and_rejects_far():
+    """FinishAtFullRes: B is A shifted by (2, -1). From a thumbnail offset a few pixels off, the
+    80-pixel oval holds the global peak, so the full-resolution result is RFromAngle's without an oval
+    (within a pixel of the shift); it is used and keeps the thumbnail R. From an offset 30 pixels off
+    the result lands more than 20 away, so the thumbnail record is returned unchanged."""
+    A, B = synth.rotated_pair(11, size=96, deg=0.0, shift=(2.0, -1.0))
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    st, _, X0, Y0, _ = oracle.rfromangle(0.0, pa, va, pb, vb, olap, req)
+    assert st == 0 and abs(X0 - 2.0) <= 1.0 and abs(Y0 + 1.0) <= 1.0
+    near = oracle.XcAngleRec(0.0, 0.55, 2.0 + 5.4, -1.0 - 3.2)
+    b, used = oracle.finish_at_full_res(near, pa, va, pb, vb, None, None, olap, req)
+    assert used == 1 and b.R == 0.55 and b.A == 0.0
+    assert abs(b.X - X0) <= 1e-12 and abs(b.Y - Y0) <= 1e-12
+    far = oracle.XcAngleRec(0.0, 0.55, 2.0 + 30.0, -1.0)
+    b, used = oracle.finish_at_full_res(far, pa, va, pb, vb, None, None, olap, req)
+    assert used == 0 and (b.A, b.R, b.X, b.Y) == (0.0, 0.55, 32.0, -1.0)
+
+
+def _thm_pair(deg=2.0):
+    A, B = synth.rotated_pair(11, size=96, deg=deg, shift=(2.0, -1.0))
+    A = synth.normalize(A.ravel()).reshape(A.shape)
+    B = synth.normalize(B.ravel()).reshape(B.shape)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    return pa, va, pb, vb, abi.XcThmParams.default(24, 96 * 96 // 4)
+
+
+def _hunt_evals(best, pa, va, pb, vb, hlfwid):
+    """PeakHunt's evaluations from best: the midpoint, then two per bracket round."""
+    _, k = oracle.peak_hunt(best, pa, va, pb, vb, hlfwid, 24, 96 * 96 // 4)
+    return 1 + 2 * (k - 1)
+
+
+def test_thm_denovo_best_angle_recovers_rotation():
+    """DenovoBestAngle on a pair rotated by 2 deg: succeeds near 2 deg; above rthresh no Pretweaks run, so
+    the evaluations are exactly the coarse scan, the fine ConstXY scan about its pick and PeakHunt's."""
+    pa, va, pb, vb, p = _thm_pair()
+    best, ok = oracle.thm_denovo_best_angle(pa, va, pb, vb, p, 0.0, 5.0, 0.5)
+    assert ok == 1 and abs(best.A - 2.0) <= 0.15 and best.R >= p.rthresh
+    assert list(p.Tptwk) == [1, 0, 0, 0, 1, 0]
+    coarse, _ = oracle.angle_scan(abi.XC_SCAN_CONSTXY, pa, va, pb, vb, 0.0, 5.0, 0.5, True, 24, 96 * 96 // 4)
+    fine, recs = oracle.angle_scan(abi.XC_SCAN_CONSTXY, pa, va, pb, vb, coarse.A, 1.0, 0.025, False, 24, 96 * 96 // 4)
+    n_coarse = len(oracle.scan_angles(abi.XC_SCAN_CONSTXY, 0.0, 5.0, 0.5, True))
+    assert p.n_evals == n_coarse + len(recs) + _hunt_evals(fine, pa, va, pb, vb, 0.5 * 0.05 * 1.5)
+
+
+def test_thm_use_prior_angles_and_failure_short_circuit():
+    """UsePriorAngles about a prior of 1.8 deg lands near 2 deg. With rthresh above every R the tweaked
+    scan fails: Pretweaks runs once (the sweep switch on), PeakHunt never does, ok is 0; with the switch
+    off only the scan is evaluated."""
+    pa, va, pb, vb, p = _thm_pair()
+    best, ok = oracle.thm_use_prior_angles(pa, va, pb, vb, p, 1.8, 0.5)
+    scan, recs = oracle.angle_scan(abi.XC_SCAN_CONSTXY, pa, va, pb, vb, 1.8, 0.5, 0.1, True, 24, 96 * 96 // 4)
+    n_scan = len(recs)
+    assert ok == 1 and abs(best.A - 2.0) <= 0.15 and p.n_evals == n_scan + _hunt_evals(scan, pa, va, pb, vb, 0.15)
+    pa, va, pb, vb, p = _thm_pair()
+    p.rthresh, p.pretweak = 1e9, 0
+    best, ok = oracle.thm_use_prior_angles(pa, va, pb, vb, p, 1.8, 0.5)
+    assert ok == 0 and p.n_evals == n_scan
+    pa, va, pb, vb, p = _thm_pair()
+    p.rthresh = 1e9
+    scan = oracle.thm_scan_with_tweaks(pa, va, pb, vb, abi.XcThmParams.default(24, 96 * 96 // 4), 1.8, 0.5, 0.1)
+    _, _, changed, n_tw = oracle.pretweaks(pa, va, pb, vb, scan.A, None, None, scan.R, 24, 96 * 96 // 4)
+    best, ok = oracle.thm_use_prior_angles(pa, va, pb, vb, p, 1.8, 0.5)
+    assert ok == 0 and p.n_evals == n_scan + n_tw + (n_scan if changed else 0)
+
+
+def test_thm_post_tweaks_finds_x_scale():
+    """PostTweaks on a pair stretched 1.04 in x: the disc becomes ((int)X, (int)Y, 40, 40), the composed
+    tweak's x scale lands within 0.005 of 1/1.04 and R improves on the untweaked record."""
+    A, B = bf.xscaled_pair(12)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    p = abi.XcThmParams.default(24, 96 * 96 // 4)
+    st, R0, X0, Y0 = oracle.rfromangle_affine(0.0, pa, va, pb, vb, None, None, 24, 96 * 96 // 4)
+    b = oracle.thm_post_tweaks(pa, va, pb, vb, p, oracle.XcAngleRec(0.0, R0, X0, Y0))
+    assert (p.Ox, p.Oy, p.Rx, p.Ry) == (int(X0), int(Y0), 40, 40)
+    assert abs(p.Tptwk[0] - 1 / 1.04) <= 0.005 and b.R > R0 and b.A == 0.0
