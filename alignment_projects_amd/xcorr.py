@@ -115,6 +115,11 @@ _OPTIONAL = {
                                       C.POINTER(C.c_int)], C.c_int),
     "xc_thm_post_tweaks_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(abi.XcThmParams),
                                 C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_pixpair_prep_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
+                             _dp, _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "xc_pixpair_prep_dev_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -410,6 +415,26 @@ class Context:
         st = self.L.xc_thm_post_tweaks_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), C.byref(params), C.byref(b))
         self._check(st, "xc_thm_post_tweaks_f64", ok=(abi.XC_OK,))
         return b
+
+    def pixpair_prep(self, ras, order=2, dog=False, r1=3, r2=6):
+        """PixPair::Load's raster preparation of one 8-bit picture (host arrays). Returns a dict with vf,
+        vfflt (with dog), vs and vsflt (when downsampled), ws, hs and scl."""
+        ras = np.ascontiguousarray(ras, dtype=np.uint8)
+        h, w = ras.shape
+        ws, hs, scl = C.c_int(), C.c_int(), C.c_int()
+        probe = w, h
+        while probe[0] > 2048 or probe[1] > 2048:
+            probe = probe[0] // 2, probe[1] // 2
+        vf = np.empty((h, w))
+        vfflt = np.empty((h, w)) if dog else None
+        down = probe != (w, h)
+        vs = np.empty((probe[1], probe[0])) if down else None
+        vsflt = np.empty((probe[1], probe[0])) if down and dog else None
+        ptr = lambda a: None if a is None else a.ctypes.data_as(_dp)
+        st = self.L.xc_pixpair_prep_f64(self.h, ras.ctypes.data, w, h, order, int(bool(dog)), r1, r2, ptr(vf),
+                                        ptr(vfflt), ptr(vs), ptr(vsflt), C.byref(ws), C.byref(hs), C.byref(scl))
+        self._check(st, "xc_pixpair_prep_f64", ok=(abi.XC_OK,))
+        return {"vf": vf, "vfflt": vfflt, "vs": vs, "vsflt": vsflt, "ws": ws.value, "hs": hs.value, "scl": scl.value}
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

@@ -461,6 +461,33 @@ int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, in
                            xc_angle_rec *best);
 
 /* ------------------------------------------------------------------ */
+/* PixPair::Load's raster preparation (the pair's pre-path)            */
+/* ------------------------------------------------------------------ */
+
+/*
+ * One picture of PixPair::Load (CPixPair.cpp:276-460) from its 8-bit
+ * raster (w x h, row-major): LegPolyFlatten (Maths.cpp:1006-1086) of the
+ * given order (0..6) and Normalize -> vf; with dog, Convolve by
+ * MakeDoGKernel(r1, r2) (CPixPair.cpp:36-80; symmetric, pre-normalised by
+ * its sum) and Normalize -> vfflt; while w or h exceeds 2048 both halve
+ * (scl doubles), and vs / vsflt receive the scl x scl box means of vf /
+ * vfflt (PixPair::Downsample, CPixPair.cpp:90-113) -- written only when
+ * scl > 1 (the reference then uses vf / vfflt as they are).  *ws, *hs, *scl
+ * receive the downsampled geometry.  XC_E_ARG: order outside 0..6, r1 < 1,
+ * r2 outside 1..18, or w, h not multiples of scl (the reference's
+ * "Dimensions not multiple of scale" failure).
+ * The _f64 form takes and returns host arrays (any output may be NULL);
+ * the _dev_f64 form takes device pointers (vf required, vfflt required
+ * with dog) and synchronises the context stream before returning.
+ */
+int xc_pixpair_prep_f64(xc_ctx *ctx, const uint8_t *ras, int w, int h, int order,
+                        int dog, int r1, int r2, double *vf, double *vfflt,
+                        double *vs, double *vsflt, int *ws, int *hs, int *scl);
+int xc_pixpair_prep_dev_f64(xc_ctx *ctx, const uint8_t *ras, int w, int h, int order,
+                            int dog, int r1, int r2, double *vf, double *vfflt,
+                            double *vs, double *vsflt, int *ws, int *hs, int *scl);
+
+/* ------------------------------------------------------------------ */
 /* ThmPair tables (the thumbnail matcher's result file)                */
 /* ------------------------------------------------------------------ */
 

@@ -87,6 +87,14 @@ def lib():
                                                         C.POINTER(C.c_int)]),
                           ("xo_thm_post_tweaks", [C.POINTER(XcAngleRec)])):
             getattr(L, nm).argtypes = [_pp, _dp, C.c_int, _pp, _dp, C.c_int, C.POINTER(XcThmParams)] + extra
+        L.xo_leg_poly_create.argtypes = [_dp, C.c_int, C.c_int]
+        L.xo_leg_poly_create.restype = None
+        L.xo_leg_poly_flatten.argtypes = [_dp, C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.xo_make_dog_kernel.argtypes = [_dp, C.c_int, C.c_int]
+        L.xo_downsample.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.xo_downsample.restype = None
+        L.xo_pixpair_prep.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp,
+                                      _dp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.xo_convolve.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.xo_fpeak.argtypes = [_dp, _dp, C.c_int, C.c_int, C.POINTER(C.c_long), C.c_long, C.c_double, _ip, _ip]
         _lib = L
@@ -445,3 +453,59 @@ def thm_denovo_best_angle(ap, av, bp, bv, params, ang0, hfangdn, step):
 def thm_post_tweaks(ap, av, bp, bv, params, best):
     """PostTweaks (0_GEN/CThmScan.cpp:796-802) from the record best."""
     return _thm("xo_thm_post_tweaks", ap, av, bp, bv, params, best=best)
+
+
+def leg_poly_create(order, n):
+    """LegPolyCreate (0_GEN/Maths.cpp:962-997): (order+1) x n."""
+    L = np.zeros((order + 1) * n)
+    lib().xo_leg_poly_create(L.ctypes.data_as(_dp), order, n)
+    return L.reshape(order + 1, n)
+
+
+def leg_poly_flatten(ras, order):
+    """LegPolyFlatten for 8-bit rasters (0_GEN/Maths.cpp:1006-1086), then Normalize."""
+    ras = np.ascontiguousarray(ras, dtype=np.uint8)
+    out = np.zeros(ras.shape)
+    st = lib().xo_leg_poly_flatten(out.ctypes.data_as(_dp), ras.ctypes.data, ras.shape[1], ras.shape[0], order)
+    assert st == 0
+    return out
+
+
+def make_dog_kernel(r1, r2):
+    """MakeDoGKernel (0_GEN/CPixPair.cpp:36-80): D x D."""
+    D = lib().xo_make_dog_kernel(None, r1, r2)
+    K = np.zeros(D * D)
+    lib().xo_make_dog_kernel(K.ctypes.data_as(_dp), r1, r2)
+    return K.reshape(D, D)
+
+
+def pixpair_prep(ras, order=2, dog=False, r1=3, r2=6):
+    """PixPair::Load's raster stage for one picture (0_GEN/CPixPair.cpp:276-460). Returns the same dict as
+    Context.pixpair_prep, or None for the reference's failures."""
+    ras = np.ascontiguousarray(ras, dtype=np.uint8)
+    h, w = ras.shape
+    ws, hs, scl = C.c_int(), C.c_int(), C.c_int()
+    probe = w, h
+    while probe[0] > 2048 or probe[1] > 2048:
+        probe = probe[0] // 2, probe[1] // 2
+    vf = np.empty((h, w))
+    vfflt = np.empty((h, w)) if dog else None
+    down = probe != (w, h)
+    vs = np.empty((probe[1], probe[0])) if down else None
+    vsflt = np.empty((probe[1], probe[0])) if down and dog else None
+    ptr = lambda a: None if a is None else a.ctypes.data_as(_dp)
+    st = lib().xo_pixpair_prep(ras.ctypes.data, w, h, order, int(bool(dog)), r1, r2, ptr(vf), ptr(vfflt), ptr(vs),
+                               ptr(vsflt), C.byref(ws), C.byref(hs), C.byref(scl))
+    if st:
+        return None
+    return {"vf": vf, "vfflt": vfflt, "vs": vs, "vsflt": vsflt, "ws": ws.value, "hs": hs.value, "scl": scl.value}
+
+
+def downsample(src, scl):
+    """PixPair::Downsample (0_GEN/CPixPair.cpp:90-113)."""
+    src = np.ascontiguousarray(src, dtype=np.float64)
+    hf, wf = src.shape
+    ws, hs = wf // scl, hf // scl
+    out = np.zeros((hs, ws))
+    lib().xo_downsample(out.ctypes.data_as(_dp), src.ctypes.data_as(_dp), wf, scl, ws, hs)
+    return out

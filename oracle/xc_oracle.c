@@ -1539,3 +1539,140 @@ int xo_thm_post_tweaks(const xc_point *ap, const double *av, int na, const xc_po
     thm_rfa(&t, best->A, best);
     return XC_OK;
 }
+
+/* ---- PixPair::Load's raster preparation (CPixPair.cpp:276-460) ---- */
+
+/* LegPolyCreate (Maths.cpp:962-997): L[o * n + s], o = 0..ord (ord <= 6). */
+void xo_leg_poly_create(double *L, int ord, int n)
+{
+    double dn = n, n_1 = dn - 1;
+    for (int s = 0; s < n; ++s) {
+        double x = (2.0 * s - n_1) / dn;
+        double v[7];
+        v[0] = 1.0;
+        v[1] = x;
+        v[2] = 1.5 * x * x - 0.5;
+        v[3] = 2.5 * x * x * x - 1.5 * x;
+        v[4] = 0.125 * (35.0 * x * x * x * x - 30.0 * x * x + 3.0);
+        v[5] = 0.125 * (63.0 * x * x * x * x * x - 70.0 * x * x * x + 15.0 * x);
+        v[6] = 0.0625 * (231.0 * x * x * x * x * x * x - 315.0 * x * x * x * x + 105.0 * x * x - 5.0);
+        for (int o = 0; o <= ord; ++o) L[(long)o * n + s] = v[o];
+    }
+}
+
+/* LegPolyFlatten for 8-bit rasters (Maths.cpp:1006-1086): remove, term by term
+ * in (xo, yo) order, the projection onto each Legendre product but (0,0), then
+ * Normalize.  order 0..6. */
+int xo_leg_poly_flatten(double *vals, const uint8_t *src, int w, int h, int order)
+{
+    long n = (long)w * h;
+    if (order < 0 || order > 6) return XC_E_ARG;
+    for (long i = 0; i < n; ++i) vals[i] = src[i];
+    if (order > 0) {
+        double *xp = (double *)malloc(sizeof(double) * (order + 1) * w);
+        double *yp = (double *)malloc(sizeof(double) * (order + 1) * h);
+        xo_leg_poly_create(xp, order, w);
+        xo_leg_poly_create(yp, order, h);
+        for (int xo = 0; xo <= order; ++xo) {
+            const double *XP = xp + (long)xo * w;
+            for (int yo = 0; yo <= order; ++yo) {
+                if (xo + yo == 0) continue;
+                const double *YP = yp + (long)yo * h;
+                double coef = 0.0, norm = 0.0;
+                for (long i = 0; i < n; ++i) {
+                    int y = (int)(i / w), x = (int)(i - (long)w * y);
+                    double F = XP[x] * YP[y];
+                    coef += F * vals[i];
+                    norm += F * F;
+                }
+                coef /= norm;
+                for (long i = 0; i < n; ++i) {
+                    int y = (int)(i / w), x = (int)(i - (long)w * y);
+                    vals[i] -= coef * XP[x] * YP[y];
+                }
+            }
+        }
+        free(xp);
+        free(yp);
+    }
+    xo_normalize(vals, (int)n);
+    return XC_OK;
+}
+
+/* MakeDoGKernel (CPixPair.cpp:36-80): D x D, D = 6 r2 + 1; returns D. */
+int xo_make_dog_kernel(double *DoG, int r1, int r2)
+{
+    int d = 3 * r2, D = 2 * d + 1;
+    double sum1 = 0.0, sum2 = 0.0;
+    for (int y = -d; y <= d; ++y)
+        for (int x = -d; x <= d; ++x) {
+            double R = x * x + y * y;
+            sum1 += exp(-(R / (2 * r1 * r1)));
+            sum2 += exp(-(R / (2 * r2 * r2)));
+        }
+    if (DoG)
+        for (int y = -d; y <= d; ++y)
+            for (int x = -d; x <= d; ++x) {
+                double R = x * x + y * y;
+                double v1 = exp(-(R / (2 * r1 * r1)));
+                double v2 = exp(-(R / (2 * r2 * r2)));
+                DoG[x + d + D * (y + d)] = v1 / sum1 - v2 / sum2;
+            }
+    return D;
+}
+
+/* PixPair::Downsample (CPixPair.cpp:90-113): scl x scl box means. */
+void xo_downsample(double *dst, const double *src, int wf, int scl, int ws, int hs)
+{
+    int n = scl * scl;
+    for (int iy = 0; iy < hs; ++iy)
+        for (int ix = 0; ix < ws; ++ix) {
+            double sum = 0.0;
+            for (int dy = 0; dy < scl; ++dy)
+                for (int dx = 0; dx < scl; ++dx) sum += src[ix * scl + dx + (long)wf * (iy * scl + dy)];
+            dst[ix + (long)ws * iy] = sum / n;
+        }
+}
+
+/* PixPair::Load's raster stage for one picture (CPixPair.cpp:276-298, 371-389,
+ * 427-460): flatten; with dog, Convolve by the DoG kernel (symmetric,
+ * pre-normalised) and Normalize; halve while w or h > 2048 and box-average.
+ * Outputs may be NULL; the downsampled ones are written only when scl > 1.
+ * XC_E_ARG for order outside 0..6, r1/r2 < 1, or sizes not multiples of scl. */
+int xo_pixpair_prep(const uint8_t *ras, int w, int h, int order, int dog, int r1, int r2, double *vf, double *vfflt,
+                    double *vs, double *vsflt, int *ws_out, int *hs_out, int *scl_out)
+{
+    int ws = w, hs = h, scl = 1;
+    long n = (long)w * h;
+    if (w < 1 || h < 1 || order < 0 || order > 6 || (dog && (r1 < 1 || r2 < 1))) return XC_E_ARG;
+    while (ws > 2048 || hs > 2048) {
+        ws /= 2;
+        hs /= 2;
+        scl *= 2;
+    }
+    if (ws * scl != w || hs * scl != h) return XC_E_ARG;
+    double *f = (double *)malloc(sizeof(double) * n);
+    xo_leg_poly_flatten(f, ras, w, h, order);
+    double *g = NULL;
+    if (dog) {
+        int D = xo_make_dog_kernel(NULL, r1, r2);
+        double *K = (double *)malloc(sizeof(double) * D * D);
+        xo_make_dog_kernel(K, r1, r2);
+        g = (double *)malloc(sizeof(double) * n);
+        xo_convolve(g, f, w, h, K, D, D, 1, 1);
+        xo_normalize(g, (int)n);
+        free(K);
+    }
+    if (vf) memcpy(vf, f, sizeof(double) * n);
+    if (vfflt && g) memcpy(vfflt, g, sizeof(double) * n);
+    if (scl > 1) {
+        if (vs) xo_downsample(vs, f, w, scl, ws, hs);
+        if (vsflt && g) xo_downsample(vsflt, g, w, scl, ws, hs);
+    }
+    free(f);
+    free(g);
+    if (ws_out) *ws_out = ws;
+    if (hs_out) *hs_out = hs;
+    if (scl_out) *scl_out = scl;
+    return XC_OK;
+}

@@ -1,0 +1,155 @@
+"""PixPair::Load's raster preparation (0_GEN/CPixPair.cpp:276-460): Legendre flattening, the DoG filter,
+Normalize and the <= 2048-pixel downsample.
+
+The oracle is pinned by checks that do not restate it: numpy's Legendre series for LegPolyCreate; at order 1,
+where every removed term is odd in x or y (so the sequential projections are the least-squares fit), numpy's
+lstsq residual; at higher orders, the moment form of the sequential removal (its residual orthogonal to the
+last term removed); the DoG against scipy's direct
+convolution, with the sign the kernel's rounding-noise sum gives it; the downsample against a reshape-mean.
+The device path is then held to the oracle on the same rasters."""
+import numpy as np
+import pytest
+from numpy.polynomial import legendre
+from scipy import signal
+
+import oracle
+from alignment_projects_amd import synth
+
+
+def raster(seed, h, w):
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    f = 100 + 40 * (xx / w) ** 2 + 30 * (yy / h) - 25 * (xx / w) * (yy / h) + synth.bandlimited(rng, h, w, 0, 20)
+    return np.clip(f, 0, 255).astype(np.uint8)
+
+
+def test_leg_poly_create_is_legendre():
+    for n in (7, 64, 301):
+        L = oracle.leg_poly_create(6, n)
+        x = (2.0 * np.arange(n) - (n - 1)) / n
+        for o in range(7):
+            c = np.zeros(o + 1)
+            c[o] = 1
+            assert np.abs(L[o] - legendre.legval(x, c)).max() <= 1e-14
+
+
+def test_flatten_order1_is_the_least_squares_residual():
+    ras = raster(3, 120, 160)
+    out = oracle.leg_poly_flatten(ras, 1)
+    X, Y = oracle.leg_poly_create(1, 160), oracle.leg_poly_create(1, 120)
+    A = np.stack([np.outer(Y[b], X[a]).ravel() for a in range(2) for b in range(2)], 1)
+    v = ras.astype(float).ravel()
+    c, *_ = np.linalg.lstsq(A, v, rcond=None)
+    assert np.abs(out.ravel() - synth.normalize(v - A @ c)).max() <= 1e-10
+    assert abs(out.mean()) <= 1e-12 and abs(out.std(ddof=1) - 1) <= 1e-12
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_flatten_higher_orders(order):
+    """Removing F_k = X_a Y_b in turn from the current residual equals, in moments, c_k = (<F_k, v> -
+    sum_{j<k} c_j <F_k, F_j>) / <F_k, F_k>; that recurrence's residual is orthogonal to the last term removed
+    and, normalised, is the oracle's pixel-by-pixel result.  It stays within the (small) non-orthogonality of
+    the sampled polynomials of the least-squares residual."""
+    h, w = 96, 130
+    ras = raster(4, h, w)
+    out = oracle.leg_poly_flatten(ras, order)
+    X, Y = oracle.leg_poly_create(order, w), oracle.leg_poly_create(order, h)
+    v = ras.astype(float)
+    terms = [(a, b) for a in range(order + 1) for b in range(order + 1) if a + b]
+    F = {t: np.outer(Y[t[1]], X[t[0]]) for t in terms}
+    c = []
+    for k, t in enumerate(terms):
+        p = (F[t] * v).sum() - sum(c[j] * (F[t] * F[terms[j]]).sum() for j in range(k))
+        c.append(p / (F[t] * F[t]).sum())
+    resid = v - sum(ck * F[t] for ck, t in zip(c, terms))
+    last = F[terms[-1]]
+    assert abs((resid * last).sum()) / (np.linalg.norm(resid) * np.linalg.norm(last)) <= 1e-12
+    assert np.abs(out - synth.normalize(resid.ravel()).reshape(h, w)).max() <= 1e-10
+    A = np.stack([np.outer(Y[b], X[a]).ravel() for a in range(order + 1) for b in range(order + 1)], 1)
+    cl, *_ = np.linalg.lstsq(A, v.ravel(), rcond=None)
+    assert np.abs(out.ravel() - synth.normalize(v.ravel() - A @ cl)).max() <= 1e-2
+
+
+def test_dog_kernel_shape_and_normalisation():
+    for r1, r2 in ((3, 6), (1, 2)):
+        K = oracle.make_dog_kernel(r1, r2)
+        assert K.shape == (6 * r2 + 1, 6 * r2 + 1)
+        assert np.array_equal(K, K.T) and np.array_equal(K, K[::-1, ::-1])
+        assert abs(K.sum()) <= 1e-14  # two unit-sum Gaussians: the sum is rounding noise
+
+
+@pytest.mark.parametrize("r1,r2", [(3, 6), (1, 2)])
+def test_dog_is_the_direct_convolution_with_the_noise_sign(r1, r2):
+    """Convolve pre-normalises by the kernel's sum, rounding noise of either sign: the filtered picture is
+    Normalize(K * v) times the sign of that sum (here (3, 6) sums positive, (1, 2) negative)."""
+    ras = raster(5, 100, 140)
+    r = oracle.pixpair_prep(ras, order=1, dog=True, r1=r1, r2=r2)
+    K = oracle.make_dog_kernel(r1, r2)
+    nrm = 0.0
+    for k in K.ravel():
+        nrm += k
+    assert (nrm > 0) == ((r1, r2) == (3, 6))
+    direct = signal.convolve2d(r["vf"], K, mode="same")
+    want = np.sign(nrm) * synth.normalize(direct.ravel()).reshape(direct.shape)
+    assert np.abs(r["vfflt"] - want).max() <= 1e-9
+
+
+def test_downsample_and_geometry():
+    src = np.random.default_rng(6).normal(size=(24, 40))
+    d = oracle.downsample(src, 4)
+    assert np.abs(d - src.reshape(6, 4, 10, 4).mean(axis=(1, 3))).max() <= 1e-14
+    ints = np.arange(64.0).reshape(8, 8)
+    assert np.array_equal(oracle.downsample(ints, 2), ints.reshape(4, 2, 4, 2).mean(axis=(1, 3)))
+    r = oracle.pixpair_prep(raster(7, 32, 4100 // 4 * 4), order=1)
+    assert (r["ws"], r["hs"], r["scl"]) == (1025, 8, 4)
+    assert oracle.pixpair_prep(raster(7, 32, 4098), order=1) is None  # not a multiple of the scale
+
+
+# ---------------------------------------------------------------------------------------------- device
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [0, 1, 2, 3, 6])
+def test_flatten_matches_oracle(ctx, order):
+    """Moments, the Gram recurrence and the residual passes against the oracle's pixel-by-pixel removal."""
+    ras = raster(10 + order, 389, 517)
+    g = ctx.pixpair_prep(ras, order=order)
+    o = oracle.pixpair_prep(ras, order=order)
+    assert (g["ws"], g["hs"], g["scl"]) == (517, 389, 1) and g["vs"] is None
+    assert np.abs(g["vf"] - o["vf"]).max() <= 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("r1,r2,h,w", [(3, 6, 200, 300), (1, 2, 211, 333), (3, 6, 239, 495)])
+def test_dog_matches_oracle(ctx, r1, r2, h, w):
+    """The direct sum with Convolve's circular geometry against the oracle's FFT Convolve; 239 x 495 with
+    r2 = 6 fills the transform exactly (495 + 17 = 512, 239 + 17 = 256), so the far edge wraps."""
+    ras = raster(20 + r2, h, w)
+    g = ctx.pixpair_prep(ras, order=2, dog=True, r1=r1, r2=r2)
+    o = oracle.pixpair_prep(ras, order=2, dog=True, r1=r1, r2=r2)
+    assert np.abs(g["vf"] - o["vf"]).max() <= 1e-9
+    assert np.abs(g["vfflt"] - o["vfflt"]).max() <= 1e-8
+
+
+@pytest.mark.gpu
+def test_downsampled_picture_matches_oracle(ctx):
+    """2600 x 1300 halves to 1300 x 650: the box means are those of the device's own vf, bit for bit, and
+    within 1e-9 of the oracle's."""
+    ras = raster(30, 1300, 2600)
+    g = ctx.pixpair_prep(ras, order=2, dog=True, r1=1, r2=2)
+    o = oracle.pixpair_prep(ras, order=2, dog=True, r1=1, r2=2)
+    assert (g["ws"], g["hs"], g["scl"]) == (1300, 650, 2)
+    assert np.array_equal(g["vs"], oracle.downsample(g["vf"], 2))
+    assert np.array_equal(g["vsflt"], oracle.downsample(g["vfflt"], 2))
+    assert np.abs(g["vs"] - o["vs"]).max() <= 1e-9 and np.abs(g["vsflt"] - o["vsflt"]).max() <= 1e-8
+
+
+@pytest.mark.gpu
+def test_prep_rejects_what_the_reference_fails(ctx):
+    from alignment_projects_amd.xcorr import XcorrError
+    with pytest.raises(XcorrError):
+        ctx.pixpair_prep(raster(1, 32, 4098), order=1)
+    with pytest.raises(XcorrError):
+        ctx.pixpair_prep(raster(1, 32, 32), order=7)
+    with pytest.raises(XcorrError):
+        ctx.pixpair_prep(raster(1, 32, 32), order=1, dog=True, r1=3, r2=19)
