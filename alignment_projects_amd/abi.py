@@ -95,6 +95,13 @@ class XcThmParams(C.Structure):
         return p
 
 
+class XcFourierQa(C.Structure):
+    """FourierMatch's metric and the figures the reference logs beside it (0_GEN/Metrics.cpp:517-658)."""
+    _fields_ = [("dot", C.c_double), ("energy", C.c_double), ("msd", C.c_double), ("tot_power", C.c_double),
+                ("half_frac", C.c_double), ("half_index", C.c_int32), ("maxf", C.c_int32), ("deg", C.c_int32),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("_pad", C.c_int32)]
+
+
 class XcThmPair(C.Structure):
     """One ThmPair_za^zb.txt entry (ThmPair, 0_GEN/PipeFiles.h:145-152)."""
     _fields_ = [("atl", C.c_int32), ("acr", C.c_int32), ("btl", C.c_int32), ("bcr", C.c_int32),

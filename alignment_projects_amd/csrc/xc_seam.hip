@@ -345,3 +345,303 @@ extern "C" int xc_corr_patch_to_image_f64(xc_ctx *ctx, const xc_point *ip1, cons
     if (res) *res = R;
     return R.status;
 }
+
+// ---------------------------------------------------------------------------
+// FourierMatch (Metrics.cpp:517-658): the smallest-footprint rotation and the
+// image box on the host (TightestBBox's 90 outline rotations), the splats,
+// the three transforms and the spectrum reductions on the device.
+// ---------------------------------------------------------------------------
+
+namespace xc {
+
+constexpr int FM_NT = 256;
+constexpr int FM_PARTS = 256;   // partial sums per reduction
+constexpr int FM_ROWS = 32;     // spectrum rows per workgroup
+
+__device__ __forceinline__ void fm_block_sum2(double &a, double &b, double *sa, double *sb)
+{
+    const int t = threadIdx.x;
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int off = FM_NT / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            sa[t] += sa[t + off];
+            sb[t] += sb[t + off];
+        }
+        __syncthreads();
+    }
+    a = sa[0];
+    b = sb[0];
+}
+
+// MeanSqrDiff's sum, partial per workgroup.
+__global__ void __launch_bounds__(FM_NT) k_fm_msd(const double *__restrict__ a, const double *__restrict__ b, int n,
+                                                  double *__restrict__ part)
+{
+    __shared__ double sa[FM_NT], sb[FM_NT];
+    double s = 0.0, z = 0.0;
+    for (int i = blockIdx.x * FM_NT + threadIdx.x; i < n; i += FM_PARTS * FM_NT) {
+        const double d = a[i] - b[i];
+        s += d * d;
+    }
+    fm_block_sum2(s, z, sa, sb);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = s, part[2 * blockIdx.x + 1] = 0.0;
+}
+
+__global__ void k_fm_diff(const double *__restrict__ i1, const double *__restrict__ i2, long long n,
+                          double *__restrict__ d)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        d[i] = i1[i] - i2[i];
+}
+
+// FFT_r2c_lookup (Metrics.cpp:420-435): negative kx as the conjugate at (Nx - kx, ky).
+__device__ __forceinline__ cpx<double> fm_lookup(const cpx<double> *__restrict__ c, int Nx, int Ny, int x, int y)
+{
+    const int M = Nx / 2 + 1;
+    if (x < 0) x += Nx;
+    if (y < 0) y += Ny;
+    if (x > Nx / 2) {
+        cpx<double> v = c[(Nx - x) + (long long)M * y];
+        v.y = -v.y;
+        return v;
+    }
+    return c[x + (long long)M * y];
+}
+
+// The low-frequency (dot, total) over |kx| <= xlim, |ky| <= ylim, partial per workgroup.
+__global__ void __launch_bounds__(FM_NT)
+    k_fm_lowfreq(const cpx<double> *__restrict__ f1, const cpx<double> *__restrict__ f2, int Nx, int Ny, int xlim,
+                 int ylim, double *__restrict__ part)
+{
+    __shared__ double sa[FM_NT], sb[FM_NT];
+    const int ny = 2 * ylim + 1;
+    const long long cnt = (long long)(2 * xlim + 1) * ny;
+    double dot = 0.0, tot = 0.0;
+    for (long long k = (long long)blockIdx.x * FM_NT + threadIdx.x; k < cnt; k += (long long)FM_PARTS * FM_NT) {
+        const int x = (int)(k / ny) - xlim, y = (int)(k % ny) - ylim;
+        const cpx<double> v1 = fm_lookup(f1, Nx, Ny, x, y), v2 = fm_lookup(f2, Nx, Ny, x, y);
+        tot += sqrt((v1.x * v1.x + v1.y * v1.y) * (v2.x * v2.x + v2.y * v2.y));
+        dot += v1.x * v2.x + v1.y * v2.y;
+    }
+    fm_block_sum2(dot, tot, sa, sb);
+    if (threadIdx.x == 0) part[2 * blockIdx.x] = dot, part[2 * blockIdx.x + 1] = tot;
+}
+
+__global__ void __launch_bounds__(FM_NT) k_fm_sum_parts(const double *__restrict__ part, int n, double *__restrict__ out)
+{
+    __shared__ double sa[FM_NT], sb[FM_NT];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += FM_NT) a += part[2 * i], b += part[2 * i + 1];
+    fm_block_sum2(a, b, sa, sb);
+    if (threadIdx.x == 0) out[0] = a, out[1] = b;
+}
+
+// The difference power by integer wavelength (Metrics.cpp:603-628): bins in
+// LDS per workgroup of FM_ROWS rows (ky = 1..Ny-1), then added to ps.
+__global__ void __launch_bounds__(FM_NT)
+    k_fm_spectrum(const cpx<double> *__restrict__ fd, int Nx, int Ny, int maxf, double *__restrict__ ps)
+{
+    extern __shared__ double hist[];
+    const int M = Nx / 2 + 1;
+    for (int i = threadIdx.x; i < maxf; i += FM_NT) hist[i] = 0.0;
+    __syncthreads();
+    const int y0 = 1 + blockIdx.x * FM_ROWS, y1 = min(Ny, y0 + FM_ROWS);
+    for (int y = y0; y < y1; ++y) {
+        const double wavey = (double)Ny / (y > Ny / 2 ? Ny - y : y);
+        for (int x = 1 + threadIdx.x; x < M; x += FM_NT) {
+            const double wavex = (double)Nx / x;
+            int iwave = (int)sqrt(wavex * wavex + wavey * wavey);
+            if (iwave > maxf - 1) iwave = maxf - 1;
+            const cpx<double> v = fd[x + (long long)M * y];
+            atomicAdd(&hist[iwave], v.x * v.x + v.y * v.y);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < maxf; i += FM_NT)
+        if (hist[i] != 0.0) atomicAdd(&ps[i], hist[i]);
+}
+
+namespace {
+
+constexpr double FM_PI = 3.14159265358;  // the reference's truncated pi (GenDefs.h)
+
+struct DBoxH {
+    double L, R, B, T;
+};
+
+// BBoxFromPoints, DBox form (Geometry.cpp:109-137).
+void dbox_from_points(DBoxH &B, const xc_point *p, int n)
+{
+    B.L = B.R = p[0].x;
+    B.B = B.T = p[0].y;
+    for (int i = 1; i < n; ++i) {
+        double t;
+        if ((t = p[i].x) < B.L) B.L = t;
+        else if (t > B.R) B.R = t;
+        if ((t = p[i].y) < B.B) B.B = t;
+        else if (t > B.T) B.T = t;
+    }
+    B.L = floor(B.L);
+    B.R = ceil(B.R);
+    B.B = floor(B.B);
+    B.T = ceil(B.T);
+}
+
+#pragma clang fp contract(off)
+// NUSetRot then Apply_R_Part (TAffine.cpp:116-122, 407-414).
+void rotate_points(std::vector<xc_point> &q, const xc_point *p, int n, double rad)
+{
+    const double c = cos(rad), s = sin(rad), t1 = -s;
+    q.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const double x = p[i].x * c + p[i].y * t1;
+        const double y = p[i].x * s + p[i].y * c;
+        q[i].x = x;
+        q[i].y = y;
+    }
+}
+
+// TightestBBox (Geometry.cpp:187-267): the region's row outline (or the points
+// when that is no smaller) rotated by whole degrees -45..45; the strictly
+// least box area wins, 0 degrees first.
+int tightest_bbox_deg(const xc_point *pts, int np)
+{
+    DBoxH B;
+    dbox_from_points(B, pts, np);
+    const int ny = (int)(B.T - B.B) + 1;
+    std::vector<double> minx(ny, B.R + 1), maxx(ny, B.L - 1);
+    for (int i = 0; i < np; ++i) {
+        const int iy = (int)floor(pts[i].y - B.B);
+        minx[iy] = fmin(minx[iy], pts[i].x);
+        maxx[iy] = fmax(maxx[iy], pts[i].x);
+    }
+    std::vector<xc_point> outline;
+    for (int iy = 0; iy < ny; ++iy)
+        if (minx[iy] <= maxx[iy]) {
+            outline.push_back({minx[iy], iy + B.B});
+            outline.push_back({maxx[iy], iy + B.B});
+        }
+    const xc_point *set = outline.data();
+    int ns = (int)outline.size();
+    if (ns >= np) {
+        set = pts;
+        ns = np;
+    }
+    double best_area = (B.R - B.L) * (B.T - B.B);
+    int best = 0;
+    std::vector<xc_point> P;
+    for (int angle = -45; angle <= 45; ++angle) {
+        if (!angle) continue;
+        rotate_points(P, set, ns, angle * FM_PI / 180);
+        DBoxH box;
+        dbox_from_points(box, P.data(), ns);
+        const double area = (box.R - box.L) * (box.T - box.B);
+        if (area < best_area) {
+            best_area = area;
+            best = angle;
+        }
+    }
+    return best;
+}
+#pragma clang fp contract(on)
+
+}  // namespace
+}  // namespace xc
+
+extern "C" int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const double *av, const double *bv, int n,
+                                    int wvlen, double *dot, xc_fourier_qa *qa)
+{
+    if (!ctx || !dot) return XC_E_ARG;
+    if (n < 0 || (n > 0 && (!pts || !av || !bv))) return set_err(ctx, XC_E_ARG, "FourierMatch: null or negative input");
+    if (wvlen < 2) return set_err(ctx, XC_E_ARG, "FourierMatch: wvlen %d < 2", wvlen);
+    xc_fourier_qa q;
+    memset(&q, 0, sizeof q);
+    *dot = 0.0;
+    if (!n) {  // MeanSqrDiff's "No data?"
+        if (qa) *qa = q;
+        return XC_OK;
+    }
+    // SmallestFootprint and MakeMetricImagesFFT's box (host geometry)
+    q.deg = tightest_bbox_deg(pts, n);
+    std::vector<xc_point> rot;
+    const xc_point *P = pts;
+    if (q.deg) {
+        rotate_points(rot, pts, n, q.deg * FM_PI / 180);
+        P = rot.data();
+    }
+    IBox B;
+    bbox_from_points(B, P, n);
+    const int Nx = ceil_pow2(B.R - B.L + 1), Ny = ceil_pow2(B.T - B.B + 1);
+    if (Nx < 2 || Nx / 2 > max_nh<double>() || Ny > max_ny<double>())
+        return set_err(ctx, XC_E_SHAPE, "FourierMatch: metric image %d x %d (need 2 <= Nx <= %d, Ny <= %d)", Nx, Ny,
+                       2 * max_nh<double>(), max_ny<double>());
+    const int NH = Nx / 2, H = NH + 1;
+    const long long N2 = (long long)Nx * Ny, MS = (long long)H * Ny;
+    const int maxf = (int)(sqrt((double)(Nx * Nx + Ny * Ny)) / 2.0) + 1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    Layout lay;
+    const size_t i_p = lay.add(sizeof(xc_point) * n), i_a = lay.add(sizeof(double) * n), i_b = lay.add(sizeof(double) * n);
+    const size_t i_i1 = lay.add(sizeof(double) * N2), i_i2 = lay.add(sizeof(double) * N2), i_d = lay.add(sizeof(double) * N2);
+    const size_t i_f1 = lay.add(sizeof(cpx<double>) * MS), i_f2 = lay.add(sizeof(cpx<double>) * MS);
+    const size_t i_fd = lay.add(sizeof(cpx<double>) * MS), i_part = lay.add(sizeof(double) * 4 * FM_PARTS);
+    const size_t i_ps = lay.add(sizeof(double) * maxf), i_out = lay.add(sizeof(double) * 4);
+    int st = ensure_dev(ctx, ctx->arena, lay.total);
+    if (st) return st;
+    char *base = (char *)ctx->arena.p;
+    auto at = [&](size_t i) { return base + lay.off[i]; };
+    double *i1 = (double *)at(i_i1), *i2 = (double *)at(i_i2), *df = (double *)at(i_d);
+    cpx<double> *f1 = (cpx<double> *)at(i_f1), *f2 = (cpx<double> *)at(i_f2), *fd = (cpx<double> *)at(i_fd);
+    double *part = (double *)at(i_part), *ps = (double *)at(i_ps), *out = (double *)at(i_out);
+    hipStream_t s = ctx->stream;
+    XC_HIP(ctx, hipMemcpyAsync(at(i_p), P, sizeof(xc_point) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_a), av, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_b), bv, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemsetAsync(i1, 0, sizeof(double) * N2, s));
+    XC_HIP(ctx, hipMemsetAsync(i2, 0, sizeof(double) * N2, s));
+    XC_HIP(ctx, hipMemsetAsync(ps, 0, sizeof(double) * maxf, s));
+    const int nb = (n + 255) / 256;
+    k_fm_msd<<<FM_PARTS, FM_NT, 0, s>>>((const double *)at(i_a), (const double *)at(i_b), n, part);
+    k_fm_sum_parts<<<1, FM_NT, 0, s>>>(part, FM_PARTS, out + 2);
+    k_splat_points<<<nb, 256, 0, s>>>((const xc_point *)at(i_p), (const double *)at(i_a), n, (double)B.L, (double)B.B,
+                                      i1, Nx, Ny);
+    k_splat_points<<<nb, 256, 0, s>>>((const xc_point *)at(i_p), (const double *)at(i_b), n, (double)B.L, (double)B.B,
+                                      i2, Nx, Ny);
+    k_fm_diff<<<(int)std::min<long long>((N2 + 255) / 256, 8192), 256, 0, s>>>(i1, i2, N2, df);
+    XC_HIP(ctx, hipGetLastError());
+    if ((st = seam_r2c(ctx, NH, i1, Ny, f1))) return st;
+    if ((st = seam_col<-1>(ctx, Ny, f1, H))) return st;
+    if ((st = seam_r2c(ctx, NH, i2, Ny, f2))) return st;
+    if ((st = seam_col<-1>(ctx, Ny, f2, H))) return st;
+    if ((st = seam_r2c(ctx, NH, df, Ny, fd))) return st;
+    if ((st = seam_col<-1>(ctx, Ny, fd, H))) return st;
+    k_fm_lowfreq<<<FM_PARTS, FM_NT, 0, s>>>(f1, f2, Nx, Ny, Nx / wvlen, Ny / wvlen, part);
+    k_fm_sum_parts<<<1, FM_NT, 0, s>>>(part, FM_PARTS, out);
+    if (Ny > 1)
+        k_fm_spectrum<<<(Ny - 1 + FM_ROWS - 1) / FM_ROWS, FM_NT, sizeof(double) * maxf, s>>>(fd, Nx, Ny, maxf, ps);
+    XC_HIP(ctx, hipGetLastError());
+    std::vector<double> hps(maxf);
+    double ho[4];
+    XC_HIP(ctx, hipMemcpyAsync(ho, out, sizeof ho, hipMemcpyDeviceToHost, s));
+    XC_HIP(ctx, hipMemcpyAsync(hps.data(), ps, sizeof(double) * maxf, hipMemcpyDeviceToHost, s));
+    XC_HIP(ctx, hipStreamSynchronize(s));
+    // the half-power bin (Metrics.cpp:634-650)
+    double tot = 0.0, cum = 0.0;
+    int i;
+    for (i = 0; i < maxf; ++i) tot += hps[i];
+    for (i = 0; cum < tot * 0.50 && i < maxf; ++i) cum += hps[i];
+    q.dot = ho[0] / ho[1];
+    q.energy = ho[1];
+    q.msd = ho[2] / n;
+    q.tot_power = tot;
+    q.half_frac = cum / tot;
+    q.half_index = i;
+    q.maxf = maxf;
+    q.nx = Nx;
+    q.ny = Ny;
+    *dot = q.dot;
+    if (qa) *qa = q;
+    return XC_OK;
+}

@@ -120,6 +120,8 @@ _OPTIONAL = {
     "xc_pixpair_prep_dev_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
+    "xc_fourier_match_f64": ([C.c_void_p, _pp, _dp, _dp, C.c_int, C.c_int, _dp, C.POINTER(abi.XcFourierQa)],
+                             C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -435,6 +437,17 @@ class Context:
                                         ptr(vfflt), ptr(vs), ptr(vsflt), C.byref(ws), C.byref(hs), C.byref(scl))
         self._check(st, "xc_pixpair_prep_f64", ok=(abi.XC_OK,))
         return {"vf": vf, "vfflt": vfflt, "vs": vs, "vsflt": vsflt, "ws": ws.value, "hs": hs.value, "scl": scl.value}
+
+    def fourier_match(self, pts, av, bv, wvlen=25):
+        """FourierMatch in FP64 (point list and the two value lists). Returns an XcFourierQa (qa.dot is the
+        metric)."""
+        p, pp = _pts(pts)
+        a, pa = _vals(av)
+        b, pb = _vals(bv)
+        d, qa = C.c_double(), abi.XcFourierQa()
+        st = self.L.xc_fourier_match_f64(self.h, pp, pa, pb, len(a), wvlen, C.byref(d), C.byref(qa))
+        self._check(st, "xc_fourier_match_f64", ok=(abi.XC_OK,))
+        return qa
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

@@ -488,6 +488,42 @@ int xc_pixpair_prep_dev_f64(xc_ctx *ctx, const uint8_t *ras, int w, int h, int o
                             double *vs, double *vsflt, int *ws, int *hs, int *scl);
 
 /* ------------------------------------------------------------------ */
+/* Fourier QA metric                                                   */
+/* ------------------------------------------------------------------ */
+
+typedef struct xc_fourier_qa {
+    double  dot;         /* the metric: norm-dot of the low-frequency coefficients */
+    double  energy;      /* its normaliser, sum sqrt(|v1|^2 |v2|^2) */
+    double  msd;         /* MeanSqrDiff: mean of (av - bv)^2 */
+    double  tot_power;   /* difference spectrum: total power */
+    double  half_frac;   /* cumulative fraction at the half-power bin */
+    int32_t half_index;  /* first bin count whose cumulative power reaches half */
+    int32_t maxf;        /* wavelength bins */
+    int32_t deg;         /* SmallestFootprint's rotation (TightestBBox) */
+    int32_t nx, ny;      /* the metric images' size */
+    int32_t _pad;
+} xc_fourier_qa;
+
+/*
+ * FourierMatch (Metrics.cpp:517-658): rotate the region to its smallest
+ * footprint (TightestBBox, Geometry.cpp:187-267: whole degrees -45..45),
+ * splat av and bv into CeilPow2 x CeilPow2 images (MakeMetricImagesFFT,
+ * Metrics.cpp:102-193), and return in *dot the normalised dot product of
+ * their Fourier coefficients with |kx| <= Nx/wvlen and |ky| <= Ny/wvlen,
+ * looked up as FFT_r2c_lookup does (Metrics.cpp:420-435: negative kx
+ * read as the conjugate at (-kx, ky)).  qa (may be NULL) also receives the
+ * figures the reference logs: the mean square difference, and the
+ * difference image's power by integer wavelength, with the bin where the
+ * cumulative power first reaches half.  n == 0 gives *dot = 0 (the
+ * reference's "No data?").  Host arrays; wvlen >= 2 (1 reads past the
+ * spectrum in the reference); images up to 4096 x 4096.
+ * The wavelength bins are summed with device atomics: their last bits
+ * depend on the order the additions land in.
+ */
+int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const double *av, const double *bv,
+                         int n, int wvlen, double *dot, xc_fourier_qa *qa);
+
+/* ------------------------------------------------------------------ */
 /* ThmPair tables (the thumbnail matcher's result file)                */
 /* ------------------------------------------------------------------ */
 

@@ -1676,3 +1676,185 @@ int xo_pixpair_prep(const uint8_t *ras, int w, int h, int order, int dog, int r1
     if (scl_out) *scl_out = scl;
     return XC_OK;
 }
+
+/* ---- FourierMatch (Metrics.cpp:517-658) ---- */
+
+typedef struct { double L, R, B, T; } xo_dbox;
+
+/* BBoxFromPoints, DBox form (Geometry.cpp:109-137). */
+static void dbox_from_points(xo_dbox *B, const xc_point *p, int n)
+{
+    B->L = B->R = p[0].x;
+    B->B = B->T = p[0].y;
+    for (int i = 1; i < n; ++i) {
+        double t;
+        if ((t = p[i].x) < B->L) B->L = t;
+        else if (t > B->R) B->R = t;
+        if ((t = p[i].y) < B->B) B->B = t;
+        else if (t > B->T) B->T = t;
+    }
+    B->L = floor(B->L);
+    B->R = ceil(B->R);
+    B->B = floor(B->B);
+    B->T = ceil(B->T);
+}
+
+/* NUSetRot + Apply_R_Part (TAffine.cpp): x' = c x - s y, y' = s x + c y. */
+static void rotate_points(xc_point *q, const xc_point *p, int n, double rad)
+{
+    double c = cos(rad), s = sin(rad);
+    for (int i = 0; i < n; ++i) {
+        double x = p[i].x, y = p[i].y;
+        q[i].x = c * x + -s * y;
+        q[i].y = s * x + c * y;
+    }
+}
+
+/* TightestBBox (Geometry.cpp:187-267): the whole-degree angle in -45..45 whose
+ * rotation of the region's row outline has the least bounding-box area. */
+int xo_tightest_bbox(const xc_point *pts, int np)
+{
+    xo_dbox B;
+    dbox_from_points(&B, pts, np);
+    int ny = (int)(B.T - B.B) + 1;
+    double *minx = (double *)malloc(sizeof(double) * ny), *maxx = (double *)malloc(sizeof(double) * ny);
+    for (int i = 0; i < ny; ++i) {
+        minx[i] = B.R + 1;
+        maxx[i] = B.L - 1;
+    }
+    for (int i = 0; i < np; ++i) {
+        int iy = (int)floor(pts[i].y - B.B);
+        minx[iy] = fmin(minx[iy], pts[i].x);
+        maxx[iy] = fmax(maxx[iy], pts[i].x);
+    }
+    xc_point *outline = (xc_point *)malloc(sizeof(xc_point) * 2 * ny);
+    int no = 0;
+    for (int iy = 0; iy < ny; ++iy)
+        if (minx[iy] <= maxx[iy]) {
+            outline[no].x = minx[iy];
+            outline[no++].y = iy + B.B;
+            outline[no].x = maxx[iy];
+            outline[no++].y = iy + B.B;
+        }
+    const xc_point *set = outline;
+    int ns = no;
+    if (no >= np) {
+        set = pts;
+        ns = np;
+    }
+    double best_area = (B.R - B.L) * (B.T - B.B);
+    int best_angle = 0;
+    xc_point *P = (xc_point *)malloc(sizeof(xc_point) * (ns ? ns : 1));
+    for (int angle = -45; angle <= 45; ++angle) {
+        if (angle == 0) continue;
+        xo_dbox box;
+        rotate_points(P, set, ns, angle * REF_PI / 180);
+        dbox_from_points(&box, P, ns);
+        double area = (box.R - box.L) * (box.T - box.B);
+        if (area < best_area) {
+            best_area = area;
+            best_angle = angle;
+        }
+    }
+    free(P);
+    free(outline);
+    free(minx);
+    free(maxx);
+    return best_angle;
+}
+
+/* FFT_r2c_lookup (Metrics.cpp:420-435), the reference's form. */
+static void r2c_lookup(const cd *c, int Nx, int Ny, int x, int y, double *re, double *im)
+{
+    int M = Nx / 2 + 1;
+    if (x < 0) x += Nx;
+    if (y < 0) y += Ny;
+    if (x > Nx / 2) {
+        *re = c[(Nx - x) + (long)M * y].re;
+        *im = -c[(Nx - x) + (long)M * y].im;
+        return;
+    }
+    *re = c[x + (long)M * y].re;
+    *im = c[x + (long)M * y].im;
+}
+
+/* FourierMatch (Metrics.cpp:517-658) with MeanSqrDiff (:30-52),
+ * SmallestFootprint (:68-89) and MakeMetricImagesFFT (:102-193). */
+int xo_fourier_match(const xc_point *pts, const double *av, const double *bv, int n, int wvlen, double *dot_out,
+                     xc_fourier_qa *qa)
+{
+    xc_fourier_qa q;
+    memset(&q, 0, sizeof q);
+    *dot_out = 0.0;
+    if (!n) {
+        if (qa) *qa = q;
+        return XC_OK;
+    }
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double d = av[i] - bv[i];
+        sum += d * d;
+    }
+    q.msd = sum / n;
+    q.deg = xo_tightest_bbox(pts, n);
+    xc_point *P = (xc_point *)malloc(sizeof(xc_point) * n);
+    if (q.deg) rotate_points(P, pts, n, q.deg * REF_PI / 180);
+    else memcpy(P, pts, sizeof(xc_point) * n);
+    ibox B;
+    bbox_from_points(&B, P, n);
+    int Nx = xo_ceil_pow2(B.R - B.L + 1), Ny = xo_ceil_pow2(B.T - B.B + 1);
+    long N2 = (long)Nx * Ny, M = Nx / 2 + 1;
+    double *i1 = (double *)calloc(N2, sizeof(double)), *i2 = (double *)calloc(N2, sizeof(double));
+    double *df = (double *)malloc(sizeof(double) * N2);
+    for (int i = 0; i < n; ++i) {
+        double x = P[i].x - B.L, y = P[i].y - B.B;
+        distribute_pixel(x, y, av[i], i1, Nx, Ny);
+        distribute_pixel(x, y, bv[i], i2, Nx, Ny);
+    }
+    for (long i = 0; i < N2; ++i) df[i] = i1[i] - i2[i];
+    cd *f1 = (cd *)malloc(sizeof(cd) * M * Ny), *f2 = (cd *)malloc(sizeof(cd) * M * Ny);
+    cd *fd = (cd *)malloc(sizeof(cd) * M * Ny);
+    xo_fft2d_r2c(i1, Nx, Ny, (double *)f1);
+    xo_fft2d_r2c(i2, Nx, Ny, (double *)f2);
+    xo_fft2d_r2c(df, Nx, Ny, (double *)fd);
+    double total = 0.0, dot = 0.0;
+    int xlim = Nx / wvlen, ylim = Ny / wvlen;
+    for (int x = -xlim; x <= xlim; ++x)
+        for (int y = -ylim; y <= ylim; ++y) {
+            double a, b, c, d;
+            r2c_lookup(f1, Nx, Ny, x, y, &a, &b);
+            r2c_lookup(f2, Nx, Ny, x, y, &c, &d);
+            total += sqrt((a * a + b * b) * (c * c + d * d));
+            dot += a * c + b * d;
+        }
+    dot /= total;
+    int maxf = (int)(sqrt((double)(Nx * Nx + Ny * Ny)) / 2.0) + 1;
+    double *ps = (double *)calloc(maxf, sizeof(double));
+    for (int x = 1; x < M; ++x) {
+        double wavex = (double)Nx / x;
+        for (int y = 1; y < Ny; ++y) {
+            double wavey = (double)Ny / (y > Ny / 2 ? Ny - y : y);
+            double wave = sqrt(wavex * wavex + wavey * wavey);
+            int iwave = (int)wave;
+            if (iwave > maxf - 1) iwave = maxf - 1;
+            cd v = fd[x + M * y];
+            ps[iwave] += v.re * v.re + v.im * v.im;
+        }
+    }
+    double tot = 0.0, cum = 0.0;
+    int i;
+    for (i = 0; i < maxf; ++i) tot += ps[i];
+    for (i = 0; cum < tot * 0.50 && i < maxf; ++i) cum += ps[i];
+    q.dot = dot;
+    q.energy = total;
+    q.tot_power = tot;
+    q.half_frac = cum / tot;
+    q.half_index = i;
+    q.maxf = maxf;
+    q.nx = Nx;
+    q.ny = Ny;
+    *dot_out = dot;
+    if (qa) *qa = q;
+    free(P); free(i1); free(i2); free(df); free(f1); free(f2); free(fd); free(ps);
+    return XC_OK;
+}

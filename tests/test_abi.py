@@ -35,6 +35,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.XcAngleRec) == 32
     assert ctypes.sizeof(abi.XcThmPair) == 6 * 4 + 2 * 8 + 6 * 8
     assert ctypes.sizeof(abi.XcThmParams) == 8 + 8 * 4 + 8 + 12 * 8
+    assert ctypes.sizeof(abi.XcFourierQa) == 5 * 8 + 6 * 4
     assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
 
 
@@ -46,11 +47,13 @@ def test_version_and_sizes_without_device():
 
 
 def test_no_device_is_an_error_not_a_crash():
-    import torch
-    if torch.cuda.is_available():
-        return
+    import pytest
+    L = xcorr.load()
     h = ctypes.c_void_p()
-    st = xcorr.load().xc_ctx_create(0, ctypes.byref(h))
+    st = L.xc_ctx_create(0, ctypes.byref(h))
+    if st == abi.XC_OK:
+        L.xc_ctx_destroy(h)
+        pytest.skip("a device is present")
     assert st == abi.XC_E_DEVICE
 
 
