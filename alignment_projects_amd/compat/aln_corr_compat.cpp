@@ -306,3 +306,31 @@ double CorrImagesS(FILE *flog, int verbose, double &dx, double &dy, const vector
     return corr_images(XC_MODE_S, flog, verbose, dx, dy, ip1, iv1, ip2, iv2, LegalRgn, arglr, LegalCnt, arglc, mincor,
                        nbmaxht, Ox, Oy, Rx, Ry, fft2);
 }
+
+void LegPolyFlatten(vector<double> &vals, const unsigned char *src, int w, int h, int maxOrder)
+{
+    vals.resize((size_t)w * h);
+    const int st = xc_leg_poly_flatten_f64(ctx(), src, w, h, maxOrder, vals.data());
+    if (st != XC_OK) die(stderr, "LegPolyFlatten", st);
+}
+
+double FourierMatch(const vector<Point> &pts, const vector<double> &av, const vector<double> &bv, int wvlen,
+                    bool write_images, const char *msg, FILE *flog)
+{
+    double dot = 0.0;
+    xc_fourier_qa qa;
+    if (pts.empty()) {
+        fprintf(flog, "Metrics: No data?\n");
+        return 0.0;
+    }
+    const int st = xc_fourier_match_f64(ctx(), reinterpret_cast<const xc_point *>(pts.data()), av.data(), bv.data(),
+                                        (int)pts.size(), wvlen, &dot, &qa);
+    if (st != XC_OK) die(flog, "FourierMatch", st);
+    if (write_images) fprintf(flog, "FourierMatch: metric images are not written by this build.\n");
+    fprintf(flog, "Metrics: %s: Mean square difference %f\n", msg, qa.msd);
+    fprintf(flog, "Metrics: %s: Smallest footprint deg=%d\n", msg, qa.deg);
+    fprintf(flog, "FFT: norm-dot %f, energy %f\n", qa.dot, qa.energy);
+    fprintf(flog, "FFT: %s: Cum power exceeds %2d%% of %f at index %d/%d (frac %f).\n", msg, 50, qa.tot_power,
+            qa.half_index, qa.maxf, qa.half_frac);
+    return dot;
+}

@@ -1,7 +1,8 @@
 // compat_kat.cpp -- runs the reference's retired harness testnormcorr
 // (Retired_Proj/4_TestNormCorr/testnormcorr.cpp:20-63) through the drop-in
-// signatures, plus a CorrImagesS call with EnoughPoints-style callbacks, and
-// prints one machine-readable line per case.
+// signatures, plus a CorrImagesS call with EnoughPoints-style callbacks,
+// FourierMatch and LegPolyFlatten, and prints one machine-readable line per
+// case.
 #include <stdio.h>
 
 #include "../../include/aln_corr_compat.h"
@@ -52,5 +53,16 @@ int main()
     double r = CorrImagesS(stdout, false, dx, dy, a, va, b, vbb, BigEnough16, (void *)req, EnoughPoints, (void *)req,
                            0.0, 0.0, 0, 0, -1, -1, ftc2);
     printf("KAT imagesS %.17g %.17g %.17g %zu\n", r, dx, dy, ftc2.size());
+
+    // FourierMatch of the same region's two value lists, wavelength 25
+    double fm = FourierMatch(a, va, vbb, 25, false, "KAT", stdout);
+    printf("KAT fourier %.17g\n", fm);
+
+    // LegPolyFlatten of the 80 x 64 field as 8-bit pixels, order 2
+    vector<unsigned char> ras(f.size());
+    for (size_t i = 0; i < f.size(); ++i) ras[i] = (unsigned char)((f[i] + 0.5) * 255.0);
+    vector<double> flat;
+    LegPolyFlatten(flat, ras.data(), 80, 64, 2);
+    printf("KAT legflat %zu %.17g %.17g %.17g\n", flat.size(), flat[0], flat[1234], flat.back());
     return 0;
 }

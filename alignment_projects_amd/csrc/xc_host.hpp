@@ -102,6 +102,13 @@ int ceil_pow2(int n);                                      // Maths.cpp:97-105
 
 RuleDev make_rule(const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, const uint8_t *dev_table);
 
+// ImageFromValuesAndPoints into a zeroed w x h raster, in the reference's order of
+// additions (xc_splat.hip).  Off-grid lists need splat_scratch_bytes of device scratch.
+bool points_off_grid(const xc_point *p, int n);
+size_t splat_scratch_bytes(int n, long long npix);
+int splat_points(xc_ctx *ctx, hipStream_t s, const xc_point *pts, const double *val, int n, double xmin, double ymin,
+                 double *img, int w, int h, bool off_grid, void *scratch, size_t scratch_bytes);
+
 // Fill the derived geometry of a PairDev from (w1,h1,w2,h2) and the mode.
 int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode);
 

@@ -503,3 +503,32 @@ extern "C" int xc_pixpair_prep_f64(xc_ctx *ctx, const uint8_t *ras, int w, int h
     if (scl_out) *scl_out = scl;
     return done(XC_OK);
 }
+
+extern "C" int xc_leg_poly_flatten_f64(xc_ctx *ctx, const uint8_t *src, int w, int h, int order, double *vals)
+{
+    if (!ctx) return XC_E_ARG;
+    if (!src || !vals || w < 1 || h < 1) return set_err(ctx, XC_E_ARG, "LegPolyFlatten: null or empty raster");
+    if (order < 0 || order > PP_MAXORD) return set_err(ctx, XC_E_ARG, "flatten order %d outside 0..6", order);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    const size_t n = (size_t)w * h;
+    DevBuf buf;
+    Layout lay;
+    const size_t i_r = lay.add(n), i_f = lay.add(sizeof(double) * n);
+    XC_HIP(ctx, hipMalloc(&buf.p, lay.total));
+    char *base = (char *)buf.p;
+    auto done = [&](int code) {
+        (void)hipFree(buf.p);
+        return code;
+    };
+    hipStream_t s = ctx->stream;
+    if (hipMemcpyAsync(base + lay.off[i_r], src, n, hipMemcpyHostToDevice, s) != hipSuccess)
+        return done(set_err(ctx, XC_E_DEVICE, "raster upload failed"));
+    int st = pp_run(ctx, (const uint8_t *)(base + lay.off[i_r]), w, h, order, 0, 0, 0, (double *)(base + lay.off[i_f]),
+                    nullptr, nullptr, nullptr, w, h, 1);
+    if (st) return done(st);
+    if (hipMemcpyAsync(vals, base + lay.off[i_f], sizeof(double) * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return done(set_err(ctx, XC_E_DEVICE, "result download failed"));
+    return done(XC_OK);
+}

@@ -93,6 +93,10 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     size_t i_tbl = lay.add(tbl ? tbl : 1);
     const long long nR = (long long)P.wR * P.hR;
     size_t i_f = lay.add(fmode ? fmode_bytes(nR) : 1);
+    const bool og1 = points_off_grid(ip1, n1), og2 = points_off_grid(ip2, n2);
+    const size_t spl = std::max(og1 ? splat_scratch_bytes(n1, (long long)w1 * h1) : 0,
+                                og2 ? splat_scratch_bytes(n2, (long long)w2 * h2) : 0);
+    size_t i_spl = lay.add(spl ? spl : 1);
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
     char *base = (char *)ctx->arena.p;
@@ -185,11 +189,12 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
         hipLaunchKernelGGL(k_fill<double>, dim3(fb), dim3(256), 0, s, (double *)at(i_iB), nB, 0.0);
         xc_point *dp = (xc_point *)at(i_pts);
         double *dv = (double *)at(i_vals);
-        hipLaunchKernelGGL(k_splat_points, dim3((n1 + 255) / 256), dim3(256), 0, s, dp, dv, n1, (double)B1.L,
-                           (double)B1.B, (double *)at(i_iA), w1, h1);
-        hipLaunchKernelGGL(k_splat_points, dim3((n2 + 255) / 256), dim3(256), 0, s, dp + n1, dv + n1, n2,
-                           (double)B2.L, (double)B2.B, (double *)at(i_iB), w2, h2);
-        XC_HIP(ctx, hipGetLastError());
+        if ((st = splat_points(ctx, s, dp, dv, n1, (double)B1.L, (double)B1.B, (double *)at(i_iA), w1, h1, og1,
+                               at(i_spl), spl)))
+            return st;
+        if ((st = splat_points(ctx, s, dp + n1, dv + n1, n2, (double)B2.L, (double)B2.B, (double *)at(i_iB), w2, h2,
+                               og2, at(i_spl), spl)))
+            return st;
     }
     // ---- tables, transforms, evaluation ----
     Exec ex;

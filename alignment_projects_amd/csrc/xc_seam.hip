@@ -588,6 +588,9 @@ extern "C" int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const doub
     const size_t i_f1 = lay.add(sizeof(cpx<double>) * MS), i_f2 = lay.add(sizeof(cpx<double>) * MS);
     const size_t i_fd = lay.add(sizeof(cpx<double>) * MS), i_part = lay.add(sizeof(double) * 4 * FM_PARTS);
     const size_t i_ps = lay.add(sizeof(double) * maxf), i_out = lay.add(sizeof(double) * 4);
+    const bool og = points_off_grid(P, n);
+    const size_t spl = og ? splat_scratch_bytes(n, N2) : 0;
+    const size_t i_spl = lay.add(spl ? spl : 1);
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
     char *base = (char *)ctx->arena.p;
@@ -602,13 +605,14 @@ extern "C" int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const doub
     XC_HIP(ctx, hipMemsetAsync(i1, 0, sizeof(double) * N2, s));
     XC_HIP(ctx, hipMemsetAsync(i2, 0, sizeof(double) * N2, s));
     XC_HIP(ctx, hipMemsetAsync(ps, 0, sizeof(double) * maxf, s));
-    const int nb = (n + 255) / 256;
     k_fm_msd<<<FM_PARTS, FM_NT, 0, s>>>((const double *)at(i_a), (const double *)at(i_b), n, part);
     k_fm_sum_parts<<<1, FM_NT, 0, s>>>(part, FM_PARTS, out + 2);
-    k_splat_points<<<nb, 256, 0, s>>>((const xc_point *)at(i_p), (const double *)at(i_a), n, (double)B.L, (double)B.B,
-                                      i1, Nx, Ny);
-    k_splat_points<<<nb, 256, 0, s>>>((const xc_point *)at(i_p), (const double *)at(i_b), n, (double)B.L, (double)B.B,
-                                      i2, Nx, Ny);
+    if ((st = splat_points(ctx, s, (const xc_point *)at(i_p), (const double *)at(i_a), n, (double)B.L, (double)B.B, i1,
+                           Nx, Ny, og, at(i_spl), spl)))
+        return st;
+    if ((st = splat_points(ctx, s, (const xc_point *)at(i_p), (const double *)at(i_b), n, (double)B.L, (double)B.B, i2,
+                           Nx, Ny, og, at(i_spl), spl)))
+        return st;
     k_fm_diff<<<(int)std::min<long long>((N2 + 255) / 256, 8192), 256, 0, s>>>(i1, i2, N2, df);
     XC_HIP(ctx, hipGetLastError());
     if ((st = seam_r2c(ctx, NH, i1, Ny, f1))) return st;

@@ -81,4 +81,12 @@ double CorrImagesS(FILE *flog, int verbose, double &dx, double &dy, const vector
                    void *arglr, EvalType LegalCnt, void *arglc, double mincor, double nbmaxht, int Ox, int Oy, int Rx,
                    int Ry, vector<CD> &fft2);
 
+/* LegPolyFlatten, 8-bit form (Maths.cpp:1006-1086, Maths.h:153-158), on the device in FP64. */
+void LegPolyFlatten(vector<double> &vals, const unsigned char *src, int w, int h, int maxOrder);
+
+/* FourierMatch (Metrics.cpp:517-658, Metrics.h:23-30), on the device in FP64.  flog receives the
+ * mean-square-difference, norm-dot and half-power lines; write_images is not supported (noted on flog). */
+double FourierMatch(const vector<Point> &pts, const vector<double> &av, const vector<double> &bv, int wvlen,
+                    bool write_images, const char *msg, FILE *flog);
+
 #endif /* ALN_CORR_COMPAT_H */

@@ -487,6 +487,10 @@ int xc_pixpair_prep_dev_f64(xc_ctx *ctx, const uint8_t *ras, int w, int h, int o
                             int dog, int r1, int r2, double *vf, double *vfflt,
                             double *vs, double *vsflt, int *ws, int *hs, int *scl);
 
+/* LegPolyFlatten for 8-bit rasters alone (Maths.cpp:1006-1086, with its
+ * final Normalize): host arrays, any w x h, order 0..6. */
+int xc_leg_poly_flatten_f64(xc_ctx *ctx, const uint8_t *src, int w, int h, int order, double *vals);
+
 /* ------------------------------------------------------------------ */
 /* Fourier QA metric                                                   */
 /* ------------------------------------------------------------------ */

@@ -48,3 +48,10 @@ def test_compat_kat_program():
     assert o[0] == abi.XC_OK
     assert abs(r - o[1]) < 1e-9 * max(1, abs(o[1])) and abs(dx - o[2]) < 1e-9 and abs(dy - o[3]) < 1e-9
     assert (round(dx), round(dy)) == (5, -3)
+    # FourierMatch(pts, av, bv, 25, ...) and LegPolyFlatten(vals, uint8*, 80, 64, 2)
+    (fm,) = lines["fourier"]
+    assert abs(fm - oracle.fourier_match(pa, va, vb, 25).dot) <= 1e-9
+    n, v0, v1234, vlast = lines["legflat"]
+    ras = ((f + 0.5) * 255.0).astype(np.uint8)
+    o = oracle.leg_poly_flatten(ras, 2).ravel()
+    assert n == 80 * 64 and max(abs(v0 - o[0]), abs(v1234 - o[1234]), abs(vlast - o[-1])) <= 1e-9

@@ -201,22 +201,41 @@ def test_thm_denovo_and_prior_f64_match_oracle(ctx, const_xy):
 
 
 def test_thm_tweaked_paths_f64_match_oracle(ctx):
-    """The Pretweaks branches in FP64 against the oracle: a failing tweaked scan (rthresh above every R,
-    so Pretweaks runs and the scan may repeat) and PostTweaks on an x-stretched pair."""
+    """The Pretweaks branches in FP64 against the oracle on a pair stretched 1.04 in x, where every Pretweaks
+    decision has a clear margin: a failing tweaked scan (rthresh above every R, so Pretweaks runs and the scan
+    repeats), then PostTweaks.  (On a pair no tweak helps, Pretweaks' accept tests compare R values equal up
+    to rounding, and the device's and the oracle's rounding may decide them differently.)"""
     from tests import bruteforce as bf
-    pa, va, pb, vb, olap, req = _thm_inputs()
-    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
-    gp.rthresh = op.rthresh = 1e9
-    g, gok = ctx.thm_use_prior_angles(pa, va, pb, vb, gp, 1.8, 0.5)
-    o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 1.8, 0.5)
-    assert gok == ook == 0
-    _same_thm(g, o, gp, op)
     A, B = bf.xscaled_pair(12)
     pa, va = synth.raster_points(A)
     pb, vb = synth.raster_points(B)
+    gp, op = abi.XcThmParams.default(24, 96 * 96 // 4), abi.XcThmParams.default(24, 96 * 96 // 4)
+    gp.rthresh = op.rthresh = 1e9
+    g, gok = ctx.thm_use_prior_angles(pa, va, pb, vb, gp, 0.2, 0.5)
+    o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 0.2, 0.5)
+    assert gok == ook == 0
+    _same_thm(g, o, gp, op)
+    assert abs(gp.Tptwk[0] - 1 / 1.04) <= 0.005
     gp, op = abi.XcThmParams.default(24, 96 * 96 // 4), abi.XcThmParams.default(24, 96 * 96 // 4)
     start = abi.XcAngleRec(0.0, 1.0, 0.4, -0.3)
     g = ctx.thm_post_tweaks(pa, va, pb, vb, gp, start)
     o = oracle.thm_post_tweaks(pa, va, pb, vb, op, start)
     _same_thm(g, o, gp, op)
     assert abs(gp.Tptwk[0] - 1 / 1.04) <= 0.005
+
+
+def test_off_grid_evaluations_repeat_bit_for_bit(ctx):
+    """Rotated and tweaked point lists splat off the grid; the device sums each pixel's contributions in the
+    reference's point order, so re-evaluating one transform gives the same bits (Pretweaks compares such
+    re-evaluations with its best R), and a pair no tweak helps runs the same Pretweaks rounds as the oracle."""
+    pa, va, pb, vb, olap, req = _thm_inputs()
+    Ttw = [0.99, 0.01, 0.0, -0.02, 1.01, 0.0]
+    r1 = ctx.rfromangle_affine(1.7, pa, va, pb, vb, None, Ttw, olap, req)
+    r2 = ctx.rfromangle_affine(1.7, pa, va, pb, vb, None, Ttw, olap, req)
+    assert (r1.R, r1.X, r1.Y) == (r2.R, r2.X, r2.Y)
+    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+    gp.rthresh = op.rthresh = 1e9
+    g, gok = ctx.thm_use_prior_angles(pa, va, pb, vb, gp, 1.8, 0.5)
+    o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 1.8, 0.5)
+    assert gok == ook == 0
+    _same_thm(g, o, gp, op)
