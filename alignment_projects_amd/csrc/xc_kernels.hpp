@@ -1442,93 +1442,92 @@ __device__ __forceinline__ int ld_z(const int32_t *p, int i)
     return ((gi *)(const void *)p)[i];
 }
 
-template <typename T, int TPF, bool AB, bool BB, bool CNT>
-__device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
-                                              long long &bestk, int &flags)
+// One x range of a row's legal interval.  NEG: x < 0 (A's band starts at column -x,
+// B's at 0), else x >= 0 (A's at 0, B's at x).  BC: B's right edge is the constant
+// w2 - 1 over the range (x >= w2 - w1), else A's is the constant w1 - 1; the
+// constant corner's table entries are loaded once.  The sums are formed in the
+// same order either way.
+template <typename T, int TPF, bool AB, bool BB, bool CNT, bool NEG, bool BC>
+__device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q, const T *R, int t, int NX, int xlo,
+                                               int xhi, double &bestv, long long &bestk, int &flags)
 {
     const void *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
     const int w1 = P.w1, w2 = P.w2, olh = q.olh;
     const double invN = q.invN;
     const long long k1 = P.rule.k1, k2 = P.rule.k2;
-    // x >= 0: A's band starts at column 0, B's at x
-    for (int x = (q.x0 > 0 ? q.x0 : 0) + t; x <= q.x1; x += TPF) {
-        const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x;
-        const int BL1 = x > 0 ? x - 1 : 0;  // L - 1 of B's band (unused at x = 0)
-        // x = 0 has no left term: subtract / add 0 instead (exact), so that
-        // every load is unconditional and all of them are in flight together
-        const bool l = x > 0;
+    const int KC = BC ? w2 - 1 : w1 - 1;  // the constant right edge
+    const double2 ct = ld_sq<T>(BC ? Bt : At, KC);
+    const double2 cb = (BC ? BB : AB) ? ld_sq<T>(BC ? Bb : Ab, KC) : make_double2(0.0, 0.0);
+    for (int x = xlo + t; x <= xhi; x += TPF) {
+        const int BR = BC ? w2 - 1 : x + w1 - 1, AR = BR - x;
         bool cand = true;
-        if constexpr (CNT) {
-            int c1 = ld_z(q.zAt, AR), c2 = ld_z(q.zBt, BR);
-            const int bl = ld_z(q.zBt, BL1);
-            if constexpr (AB) c1 -= ld_z(q.zAb, AR);
-            c2 -= l ? bl : 0;
-            if constexpr (BB) {
-                const int u = ld_z(q.zBb, BR), z = ld_z(q.zBb, BL1);
-                c2 -= u;
-                c2 += l ? z : 0;
+        double2 a, b;
+        if constexpr (!NEG) {
+            // x >= 0: A's band starts at column 0, B's at x; x = 0 has no left term:
+            // subtract / add 0 instead (exact), so that every load is unconditional
+            const int BL1 = x > 0 ? x - 1 : 0;
+            const bool l = x > 0;
+            if constexpr (CNT) {
+                int c1 = ld_z(q.zAt, AR), c2 = ld_z(q.zBt, BR);
+                const int bl = ld_z(q.zBt, BL1);
+                if constexpr (AB) c1 -= ld_z(q.zAb, AR);
+                c2 -= l ? bl : 0;
+                if constexpr (BB) {
+                    const int u = ld_z(q.zBb, BR), z = ld_z(q.zBb, BL1);
+                    c2 -= u;
+                    c2 += l ? z : 0;
+                }
+                cand = (long long)c1 > k1 && (long long)c2 > k2;
             }
-            cand = (long long)c1 > k1 && (long long)c2 > k2;
-        }
-        double2 a = ld_sq<T>(At, AR);
-        if constexpr (AB) {
-            const double2 u = ld_sq<T>(Ab, AR);
-            a.x -= u.x;
-            a.y -= u.y;
-        }
-        double2 b = ld_sq<T>(Bt, BR);
-        const double2 bl = ld_sq<T>(Bt, BL1);
-        b.x -= l ? bl.x : 0.0;
-        b.y -= l ? bl.y : 0.0;
-        if constexpr (BB) {
-            const double2 u = ld_sq<T>(Bb, BR), z = ld_sq<T>(Bb, BL1);
-            b.x -= u.x;
-            b.y -= u.y;
-            b.x += l ? z.x : 0.0;
-            b.y += l ? z.y : 0.0;
-        }
-        const double r = ncc_r<T>(BR - x + 1, olh, invN, (double)R[x], a.x, a.y, b.x, b.y);
-        if (cand) {
-            flags |= fabs(r) > 1.0001 ? 3 : 1;
-            if (r > bestv) {
-                bestv = r;
-                bestk = x;
-            }
-        }
-    }
-    // x < 0: A's band starts at column -x, B's at 0
-    const int xe = q.x1 < -1 ? q.x1 : -1;
-    for (int x = q.x0 + t; x <= xe; x += TPF) {
-        const int BR = (w2 - 1 < x + w1 - 1) ? w2 - 1 : x + w1 - 1, AR = BR - x, AL1 = -x - 1;
-        bool cand = true;
-        if constexpr (CNT) {
-            int c1 = ld_z(q.zAt, AR) - ld_z(q.zAt, AL1), c2 = ld_z(q.zBt, BR);
+            a = BC ? ld_sq<T>(At, AR) : ct;
             if constexpr (AB) {
-                c1 -= ld_z(q.zAb, AR);
-                c1 += ld_z(q.zAb, AL1);
+                const double2 u = BC ? ld_sq<T>(Ab, AR) : cb;
+                a.x -= u.x;
+                a.y -= u.y;
             }
-            if constexpr (BB) c2 -= ld_z(q.zBb, BR);
-            cand = (long long)c1 > k1 && (long long)c2 > k2;
+            b = BC ? ct : ld_sq<T>(Bt, BR);
+            const double2 bl = ld_sq<T>(Bt, BL1);
+            b.x -= l ? bl.x : 0.0;
+            b.y -= l ? bl.y : 0.0;
+            if constexpr (BB) {
+                const double2 u = BC ? cb : ld_sq<T>(Bb, BR), z = ld_sq<T>(Bb, BL1);
+                b.x -= u.x;
+                b.y -= u.y;
+                b.x += l ? z.x : 0.0;
+                b.y += l ? z.y : 0.0;
+            }
+        } else {
+            // x < 0: A's band starts at column -x, B's at 0
+            const int AL1 = -x - 1;
+            if constexpr (CNT) {
+                int c1 = ld_z(q.zAt, AR) - ld_z(q.zAt, AL1), c2 = ld_z(q.zBt, BR);
+                if constexpr (AB) {
+                    c1 -= ld_z(q.zAb, AR);
+                    c1 += ld_z(q.zAb, AL1);
+                }
+                if constexpr (BB) c2 -= ld_z(q.zBb, BR);
+                cand = (long long)c1 > k1 && (long long)c2 > k2;
+            }
+            a = BC ? ld_sq<T>(At, AR) : ct;
+            const double2 al = ld_sq<T>(At, AL1);
+            a.x -= al.x;
+            a.y -= al.y;
+            if constexpr (AB) {
+                const double2 u = BC ? ld_sq<T>(Ab, AR) : cb, z = ld_sq<T>(Ab, AL1);
+                a.x -= u.x;
+                a.y -= u.y;
+                a.x += z.x;
+                a.y += z.y;
+            }
+            b = BC ? ct : ld_sq<T>(Bt, BR);
+            if constexpr (BB) {
+                const double2 u = BC ? cb : ld_sq<T>(Bb, BR);
+                b.x -= u.x;
+                b.y -= u.y;
+            }
         }
-        double2 a = ld_sq<T>(At, AR);
-        const double2 al = ld_sq<T>(At, AL1);
-        a.x -= al.x;
-        a.y -= al.y;
-        if constexpr (AB) {
-            const double2 u = ld_sq<T>(Ab, AR), z = ld_sq<T>(Ab, AL1);
-            a.x -= u.x;
-            a.y -= u.y;
-            a.x += z.x;
-            a.y += z.y;
-        }
-        double2 b = ld_sq<T>(Bt, BR);
-        if constexpr (BB) {
-            const double2 u = ld_sq<T>(Bb, BR);
-            b.x -= u.x;
-            b.y -= u.y;
-        }
-        const int ix = NX + x;
-        const double r = ncc_r<T>(BR + 1, olh, invN, (double)R[ix], a.x, a.y, b.x, b.y);
+        const int ix = NEG ? NX + x : x;
+        const double r = ncc_r<T>(NEG ? BR + 1 : BR - x + 1, olh, invN, (double)R[ix], a.x, a.y, b.x, b.y);
         if (cand) {
             flags |= fabs(r) > 1.0001 ? 3 : 1;
             if (r > bestv) {
@@ -1537,6 +1536,24 @@ __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q,
             }
         }
     }
+}
+
+// A row's legal interval in raster order: x >= 0 first, then x < 0; each thread's
+// keys ascend, so a strict > keeps the first of equal values.  Each sign splits at
+// x = w2 - w1, where the constant right edge passes from A to B.
+template <typename T, int TPF, bool AB, bool BB, bool CNT>
+__device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
+                                              long long &bestk, int &flags)
+{
+    const int sw = P.w2 - P.w1;
+    const int p0 = q.x0 > 0 ? q.x0 : 0, p1 = q.x1;        // x >= 0
+    const int n0 = q.x0, n1 = q.x1 < -1 ? q.x1 : -1;     // x < 0
+    ncc_scan_range<T, TPF, AB, BB, CNT, false, false>(P, q, R, t, NX, p0, p1 < sw - 1 ? p1 : sw - 1, bestv, bestk,
+                                                      flags);
+    ncc_scan_range<T, TPF, AB, BB, CNT, false, true>(P, q, R, t, NX, p0 > sw ? p0 : sw, p1, bestv, bestk, flags);
+    ncc_scan_range<T, TPF, AB, BB, CNT, true, false>(P, q, R, t, NX, n0, n1 < sw ? n1 : sw, bestv, bestk, flags);
+    ncc_scan_range<T, TPF, AB, BB, CNT, true, true>(P, q, R, t, NX, n0 > sw + 1 ? n0 : sw + 1, n1, bestv, bestk,
+                                                    flags);
 }
 #pragma clang fp contract(on)
 
