@@ -344,22 +344,21 @@ extern "C" int xc_rfromangle_affine_f64(xc_ctx *ctx, const xc_point *ap, const d
 
 namespace {
 
-// Pretweaks (CThmScan.cpp:205-323) with PTWInterp (:119-160) and NewXFromParabola (:48-61)
-// at angle deg: tw (the pretweak) and bR (the best R) in/out.
-int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
-                  int nb, double deg, const Aff &D, Aff &tw, double &bR, const Rules &ru, int Ox, int Oy, int Rx,
-                  int Ry, std::vector<xc_point> &buf, bool *changed, int *n_evals)
+// R of each candidate pretweak M(sel[i], mag[i]) * tw at the round's angle.
+using PtwEval = std::function<int(const std::vector<int> &sel, const std::vector<double> &mag, const Aff &tw,
+                                  std::vector<double> &R)>;
+
+// Pretweaks (CThmScan.cpp:205-323) with PTWInterp (:119-160) and NewXFromParabola (:48-61):
+// tw (the pretweak) and bR (the best R) in/out.  Each round's magnitude sweep is one
+// evaluation batch, then the parabola's two neighbours, then its vertex.
+int pretweaks_core(const PtwEval &eval, Aff &tw, double &bR, bool *changed, int *n_evals)
 {
     int evals = 0, st;
-    auto eval = [&](int sel, double a, xc_angle_rec *C) {
-        const Aff W = aff_mul(nu_select(sel, a), tw);  // M * Tptwk (RFromAngle's iaux branch)
-        ++evals;
-        return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, Ox, Oy, Rx, Ry, buf, C);
-    };
     const double astep = 0.02;
     const int nstep = 5;
     bool any = false;
     int vused[5] = {0, 0, 0, 0, 0};
+    std::vector<double> Rv;
     for (int itype = 0; itype < 5; ++itype) {
         // sweep every unused type's magnitude about its identity value
         std::vector<int> vsel;
@@ -372,33 +371,31 @@ int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, con
                 va.push_back(abase + j * astep);
             }
         }
+        if ((st = eval(vsel, va, tw, Rv))) return st;
+        evals += (int)vsel.size();
         double rbest = 0;
         int ibest = -1;
-        for (size_t ic = 0; ic < vsel.size(); ++ic) {
-            xc_angle_rec C;
-            if ((st = eval(vsel[ic], va[ic], &C))) return st;
-            if (C.R > rbest) {
-                rbest = C.R;
+        for (size_t ic = 0; ic < vsel.size(); ++ic)
+            if (Rv[ic] > rbest) {
+                rbest = Rv[ic];
                 ibest = (int)ic;
             }
-        }
         if (ibest == -1) break;
         // PTWInterp: parabola through the neighbours at +-astep
         const int sel = vsel[ibest];
         const double x1 = va[ibest], y1 = rbest, d = astep;
-        xc_angle_rec C;
-        if ((st = eval(sel, x1 - d, &C))) return st;
-        const double y0 = C.R;
-        if ((st = eval(sel, x1 + d, &C))) return st;
-        const double y2 = C.R;
+        if ((st = eval({sel, sel}, {x1 - d, x1 + d}, tw, Rv))) return st;
+        evals += 2;
+        const double y0 = Rv[0], y2 = Rv[1];
         double xnew = x1;
         {
 #pragma clang fp contract(off)
             const double Dp = d * (y0 - y2) / (2 * (y0 + y2 - y1 - y1));
             if (fabs(Dp) <= 2 * d) xnew += Dp;
         }
-        if ((st = eval(sel, xnew, &C))) return st;
-        double ynew = C.R;
+        if ((st = eval({sel}, {xnew}, tw, Rv))) return st;
+        evals += 1;
+        double ynew = Rv[0];
         if (ynew < y1) {
             xnew = x1;
             ynew = y1;
@@ -416,6 +413,26 @@ int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, con
     if (changed) *changed = any;
     if (n_evals) *n_evals = evals;
     return XC_OK;
+}
+
+// Pretweaks on point lists in FP64, one device correlation per candidate.
+int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, const xc_point *bp, const double *bv,
+                  int nb, double deg, const Aff &D, Aff &tw, double &bR, const Rules &ru, int Ox, int Oy, int Rx,
+                  int Ry, std::vector<xc_point> &buf, bool *changed, int *n_evals)
+{
+    const PtwEval eval = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
+                             std::vector<double> &R) {
+        R.resize(sel.size());
+        for (size_t i = 0; i < sel.size(); ++i) {
+            const Aff W = aff_mul(nu_select(sel[i], mag[i]), t);  // M * Tptwk (RFromAngle's iaux branch)
+            xc_angle_rec C;
+            const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, Ox, Oy, Rx, Ry, buf, &C);
+            if (st) return st;
+            R[i] = C.R;
+        }
+        return (int)XC_OK;
+    };
+    return pretweaks_core(eval, tw, bR, changed, n_evals);
 }
 
 // CThmScan's matching state for one pair of point lists: the parameter block (thresholds,
@@ -601,4 +618,44 @@ extern "C" int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const dou
     const int st = t.pretweaks(best->R, best->A, &changed);
     if (st) return st;
     return t.rfa(best->A, best);
+}
+
+extern "C" int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double deg, const double *Tdfm,
+                                double *Tptwk, double *bestR, int olap1D, int64_t req_area, int Ox, int Oy, int Rx,
+                                int Ry, int *changed, int *n_evals)
+{
+    if (!ctx || !a || !b || !Tptwk || !bestR) return XC_E_ARG;
+    const Rules ru = scan_rules(olap1D, req_area);
+    const Aff D = aff_of(Tdfm), Rot = nu_select(5, deg * REF_PI / 180.0);
+    Aff tw = aff_of(Tptwk);
+    double bR = *bestR;
+    bool any = false;
+    std::vector<double> maps;
+    std::vector<xc_result> res;
+    const PtwEval eval = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
+                             std::vector<double> &R) {
+        const size_t n = sel.size();
+        maps.resize(4 * n);
+        res.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            // RFromAngle's C.T = R * (Tdfm * (M * Tptwk)), its 2 x 2 part
+            const Aff T = aff_mul(Rot, aff_mul(D, aff_mul(nu_select(sel[i], mag[i]), t)));
+            maps[4 * i + 0] = T.t[0];
+            maps[4 * i + 1] = T.t[1];
+            maps[4 * i + 2] = T.t[3];
+            maps[4 * i + 3] = T.t[4];
+        }
+        const int st = xc_sweep_affine_f32(ctx, a, b, maps.data(), (int)n, &ru.rgn, &ru.cnt, Ox, Oy, Rx, Ry,
+                                           res.data());
+        if (st < 0) return st;
+        R.resize(n);
+        for (size_t i = 0; i < n; ++i) R[i] = res[i].status == XC_OK ? res[i].r : 0.0;  // CorrImagesS: 0 without a peak
+        return (int)XC_OK;
+    };
+    const int st = pretweaks_core(eval, tw, bR, &any, n_evals);
+    if (st) return st;
+    memcpy(Tptwk, tw.t, sizeof tw.t);
+    *bestR = bR;
+    if (changed) *changed = any ? 1 : 0;
+    return XC_OK;
 }

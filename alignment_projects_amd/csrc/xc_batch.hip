@@ -216,10 +216,16 @@ static void rotated_bbox(const xc_tile *t, double t0, double t1, double t3, doub
 // GenDefs.h:16: the reference's deg -> rad constant.
 static const double REF_PI = 3.14159265358;
 
-struct RotKey {
+struct RotKey {  // a tile and the 2 x 2 map of its copy
     const xc_tile *t;
-    double deg;
-    bool operator<(const RotKey &o) const { return t < o.t || (t == o.t && deg < o.deg); }
+    double m[4];
+    bool operator<(const RotKey &o) const
+    {
+        if (t != o.t) return t < o.t;
+        for (int k = 0; k < 4; ++k)
+            if (m[k] != o.m[k]) return m[k] < o.m[k];
+        return false;
+    }
 };
 
 // Autotuner of the column pass: for each size bucket, time every shape the
@@ -286,7 +292,9 @@ static int tune_columns(xc_ctx *ctx, xc_plan *p)
     return XC_OK;
 }
 
-static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan *p)
+// aff4 (may be null): per pair the 2 x 2 map {t0, t1, t3, t4} applied to A's points in place
+// of the pair's angle (an identity map means no copy, as 0 degrees does).
+static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan *p, const double *aff4)
 {
     const bool refresh = (flags & XC_PLAN_REFRESH_TILES) != 0;
     // ---- pair geometry, rotated copies ----
@@ -311,20 +319,43 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
                 tiles.push_back(t);
             }
         int wa = a->w, ha = a->h, offx = 0, offy = 0;
-        if (q.angle_deg != 0.0) {
-            const RotKey key{a, q.angle_deg};
+        RotKey key{a, {1.0, 0.0, 0.0, 1.0}};
+        bool inv_t = true;  // a rotation: its inverse is its transpose
+        if (aff4) {
+            for (int k = 0; k < 4; ++k) key.m[k] = aff4[4 * i + k];
+            inv_t = false;
+        } else if (q.angle_deg != 0.0) {
+            // RFromAngle: R.NUSetRot(deg * PI / 180) (TAffine.cpp:116-122); Tdfm = Tptwk = identity
+            const double r = q.angle_deg * REF_PI / 180.0, c = cos(r), s = sin(r);
+            key.m[0] = c;
+            key.m[1] = -s;
+            key.m[2] = s;
+            key.m[3] = c;
+        }
+        if (!(key.m[0] == 1.0 && key.m[1] == 0.0 && key.m[2] == 0.0 && key.m[3] == 1.0)) {
             auto it = rot_idx.find(key);
             int ri;
             if (it == rot_idx.end()) {
-                // RFromAngle: R.NUSetRot(deg * PI / 180) (TAffine.cpp:116-122); Tdfm = Tptwk = identity
-                const double r = q.angle_deg * REF_PI / 180.0, c = cos(r), s = sin(r);
                 RotDev R;
                 memset(&R, 0, sizeof R);
                 R.src = a->dev.v;
-                R.t0 = c;
-                R.t1 = -s;
-                R.t3 = s;
-                R.t4 = c;
+                R.t0 = key.m[0];
+                R.t1 = key.m[1];
+                R.t3 = key.m[2];
+                R.t4 = key.m[3];
+                if (inv_t) {
+                    R.u0 = R.t0;
+                    R.u1 = R.t3;
+                    R.u3 = R.t1;
+                    R.u4 = R.t4;
+                } else {
+                    const double det = R.t0 * R.t4 - R.t1 * R.t3;
+                    if (!(fabs(det) > 1e-12)) return set_err(ctx, XC_E_ARG, "pair %d: singular map", i);
+                    R.u0 = R.t4 / det;
+                    R.u1 = -R.t1 / det;
+                    R.u3 = -R.t3 / det;
+                    R.u4 = R.t0 / det;
+                }
                 int L, Rr, B, T;
                 rotated_bbox(a, R.t0, R.t1, R.t3, R.t4, L, Rr, B, T);
                 if (L > Rr) return set_err(ctx, XC_E_ARG, "pair %d: tile A has no present pixel", i);
@@ -637,7 +668,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     return XC_OK;
 }
 
-extern "C" int xc_plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan **out)
+static int plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, const double *aff4, xc_plan **out)
 {
     if (!ctx || !pairs || n <= 0 || !out) return XC_E_ARG;
     *out = nullptr;
@@ -646,13 +677,18 @@ extern "C" int xc_plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flag
     xc_plan *p = new xc_plan;
     p->ctx = ctx;
     p->n = n;
-    int st = plan_build(ctx, pairs, n, flags, p);
+    int st = plan_build(ctx, pairs, n, flags, p, aff4);
     if (st) {
         xc_plan_destroy(p);
         return st;
     }
     *out = p;
     return XC_OK;
+}
+
+extern "C" int xc_plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_plan **out)
+{
+    return plan_create(ctx, pairs, n, flags, nullptr, out);
 }
 
 extern "C" int xc_plan_buckets(const xc_plan *p, xc_bucket_info *out, int cap)
@@ -780,6 +816,33 @@ extern "C" int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile 
     }
     xc_plan *p = nullptr;
     int st = xc_plan_create(ctx, pairs.data(), n_angles, 0, &p);
+    if (st) return st;
+    st = xc_plan_run(p, results);
+    xc_plan_destroy(p);
+    return st;
+}
+
+extern "C" int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps, int n_maps,
+                                   const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, int Ox, int Oy, int Rx, int Ry,
+                                   xc_result *results)
+{
+    if (!ctx || !a || !b || !maps || n_maps <= 0 || !results) return XC_E_ARG;
+    std::vector<xc_pair> pairs(n_maps);
+    for (int i = 0; i < n_maps; ++i) {
+        xc_pair &q = pairs[i];
+        memset(&q, 0, sizeof q);
+        q.a = a;
+        q.b = b;
+        q.mode = XC_MODE_S;
+        q.Ox = Ox;
+        q.Oy = Oy;
+        q.Rx = Rx;
+        q.Ry = Ry;
+        if (rgn) q.rgn = *rgn;
+        if (cnt) q.cnt = *cnt;
+    }
+    xc_plan *p = nullptr;
+    int st = plan_create(ctx, pairs.data(), n_maps, 0, maps, &p);
     if (st) return st;
     st = xc_plan_run(p, results);
     xc_plan_destroy(p);

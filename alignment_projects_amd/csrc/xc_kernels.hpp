@@ -908,15 +908,15 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
     if (pix >= (long long)R.w * R.h) return;
     const int py = (int)(pix / R.w), px = (int)(pix - (long long)py * R.w);
     const double mean = R.src.msd[0], sd = R.src.msd[1];
-    // grid points whose rotated position lies in [px-1, px+1) x [py-1, py+1):
-    // they lie in the inverse rotation of that square (the transpose of the
-    // rotation), so in the integer points of its bounding box, taken with a
+    // grid points whose mapped position lies in [px-1, px+1) x [py-1, py+1):
+    // they lie in the inverse image of that square (for a rotation, its
+    // transpose), so in the integer points of its bounding box, taken with a
     // slack of 1e-7 px for the rounding of the corners (~1e-13 px here)
     double gx0 = 1e300, gx1 = -1e300, gy0 = 1e300, gy1 = -1e300;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const double cx = (double)(px - 1 + 2 * (c & 1) + R.L), cy = (double)(py - 1 + 2 * (c >> 1) + R.B);
-        const double gx = cx * R.t0 + cy * R.t3, gy = cx * R.t1 + cy * R.t4;
+        const double gx = cx * R.u0 + cy * R.u1, gy = cx * R.u3 + cy * R.u4;
         gx0 = fmin(gx0, gx);
         gx1 = fmax(gx1, gx);
         gy0 = fmin(gy0, gy);

@@ -73,7 +73,8 @@ struct TileDev {
 // (x' - L, y' - B) into a w x h raster (ImageFromValuesAndPoints).
 struct RotDev {
     ImgView src;
-    double t0, t1, t3, t4;
+    double t0, t1, t3, t4;  // the 2 x 2 map applied to A's points
+    double u0, u1, u3, u4;  // its inverse (the transpose for a rotation)
     int32_t L, B;
     int32_t w, h;
     void *out;

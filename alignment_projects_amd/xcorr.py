@@ -129,6 +129,11 @@ _OPTIONAL = {
     "xc_thmpair_prior_angle": ([C.POINTER(abi.XcThmPair), C.c_int, _dp], C.c_int),
     "xc_peak_hunt_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_int64, C.c_int, C.c_int,
                           C.c_int, C.c_int, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
+    "xc_sweep_affine_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int, C.POINTER(abi.XcRgnRule),
+                             C.POINTER(abi.XcCntRule), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(abi.XcResult)],
+                            C.c_int),
+    "xc_pretweaks_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, _dp, _dp, _dp, C.c_int, C.c_int64, C.c_int,
+                          C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
 }
 
 
@@ -333,6 +338,27 @@ class Context:
                                      C.byref(k))
         self._check(st, "xc_peak_hunt_f32", ok=(abi.XC_OK,))
         return bb, k.value
+
+    def sweep_affine_tiles(self, a, b, maps, rgn=None, cnt=None, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """CorrImagesS of A mapped by each 2 x 2 map {t0, t1, t3, t4} against B, one FP32 sweep."""
+        m = np.ascontiguousarray(maps, dtype=np.float64).reshape(-1, 4)
+        res = (abi.XcResult * len(m))()
+        st = self.L.xc_sweep_affine_f32(self.h, a.h, b.h, m.ctypes.data_as(_dp), len(m),
+                                        None if rgn is None else C.byref(rgn), None if cnt is None else C.byref(cnt),
+                                        Ox, Oy, Rx, Ry, res)
+        self._check(st, "xc_sweep_affine_f32", ok=(abi.XC_OK,))
+        return list(res)
+
+    def pretweaks_tiles(self, a, b, deg, Tdfm, Tptwk, best_r, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
+        """Pretweaks over resident tiles, each round one FP32 sweep. Returns (Tptwk[6], best_r, changed, n_evals)."""
+        D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+        W = np.array(Tptwk if Tptwk is not None else [1, 0, 0, 0, 1, 0], dtype=np.float64)
+        r, ch, ne = C.c_double(best_r), C.c_int(), C.c_int()
+        st = self.L.xc_pretweaks_f32(self.h, a.h, b.h, deg, None if D is None else D.ctypes.data_as(_dp),
+                                     W.ctypes.data_as(_dp), C.byref(r), olap1d, req_area, Ox, Oy, Rx, Ry,
+                                     C.byref(ch), C.byref(ne))
+        self._check(st, "xc_pretweaks_f32", ok=(abi.XC_OK,))
+        return W, r.value, ch.value, ne.value
 
     def rfromangle_affine(self, deg, ap, av, bp, bv, Tdfm, Ttw, olap1d, req_area, Ox=0, Oy=0, Rx=-1, Ry=-1):
         """RFromAngle with deformation and pretweak affines (t[6] each, None = identity)."""

@@ -360,6 +360,17 @@ int xc_angle_scan_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, int mode,
                       xc_angle_rec *best, xc_angle_rec *recs, int cap, int *n_recs);
 
 /*
+ * The same sweep over general 2 x 2 maps: maps[4 i .. 4 i + 3] = {t0, t1, t3, t4}
+ * is the linear part applied to A's points (RFromAngle's C.T = R * (Tdfm * T),
+ * CThmScan.cpp:348-349, with a deformation or pretweak), splatted by the
+ * inverse-map gather; an identity map uses A as it is.  XC_E_ARG for a
+ * singular map.
+ */
+int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps,
+                        int n_maps, const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                        int Ox, int Oy, int Rx, int Ry, xc_result *results);
+
+/*
  * CThmScan::PeakHunt (CThmScan.cpp:676-732): bracket search for the best
  * angle within best->A +- hlfwid, probing the left then the right half
  * until the bracket is narrower than 1e-4 deg; the entry *best is kept if
@@ -399,6 +410,14 @@ int xc_pretweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
                      const double *Tdfm, double *Tptwk, double *bestR,
                      int olap1D, int64_t req_area, int Ox, int Oy, int Rx, int Ry,
                      int *changed, int *n_evals);
+/* The same over resident tiles: each round's candidate maps (5 x 11, then
+ * fewer) in one batched FP32 sweep (xc_sweep_affine_f32), then the
+ * parabola's two neighbours in one and its vertex; the decisions are the
+ * FP64 form's, taken on the FP32 R values. */
+int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double deg,
+                     const double *Tdfm, double *Tptwk, double *bestR, int olap1D,
+                     int64_t req_area, int Ox, int Oy, int Rx, int Ry, int *changed,
+                     int *n_evals);
 
 /*
  * CThmScan::FinishAtFullRes (CThmScan.cpp:810-842): RFromAngle at best->A

@@ -239,3 +239,55 @@ def test_off_grid_evaluations_repeat_bit_for_bit(ctx):
     o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 1.8, 0.5)
     assert gok == ook == 0
     _same_thm(g, o, gp, op)
+
+
+def _aff6(m):
+    return [m[0], m[1], 0.0, m[2], m[3], 0.0]
+
+
+def test_sweep_affine_tiles_f32(ctx):
+    """General 2 x 2 maps through the batched FP32 sweep (inverse-map gather splat) against the FP64
+    oracle's RFromAngle with the same map as its tweak: R within 1e-3 relative, X and Y within 0.05 px.
+    A pure rotation given as a map matches the angle sweep."""
+    A, B, olap, req = _scan_pair(25, 1.5)
+    ta = ctx.tile_put(A, normalize=False)
+    tb = ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    t = 1.5 * 3.14159265358 / 180  # RFromAngle's radians (the reference's truncated pi)
+    c, s = np.cos(t), np.sin(t)
+    maps = [[c, -s, s, c], [1.02, 0.0, 0.0, 1.02], [0.98, 0.0, 0.0, 1.0], [1.0, 0.03, 0.0, 1.0],
+            [1.0, 0.0, -0.02, 1.0], [c * 1.01, -s, s, c * 0.99], [1.0, 0.0, 0.0, 1.0]]
+    rgn, cnt = abi.rgn_big_enough(olap, req), abi.cnt_enough_points(req)
+    res = ctx.sweep_affine_tiles(ta, tb, maps, rgn, cnt)
+    for m, r in zip(maps, res):
+        st, R, X, Y = oracle.rfromangle_affine(0.0, pa, va, pb, vb, None, _aff6(m), olap, req)
+        assert r.status == st == 0
+        assert abs(r.r - R) <= 1e-3 * max(1.0, abs(R)), (m, r.r, R)
+        assert abs(r.dx - X) <= 0.05 and abs(r.dy - Y) <= 0.05, (m, r.dx, X, r.dy, Y)
+    ra = ctx.sweep_angles(ta, tb, [1.5], rgn, cnt)[0]
+    assert abs(ra.r - res[0].r) <= 1e-6 * max(1.0, abs(ra.r))
+    assert abs(ra.dx - res[0].dx) <= 1e-6 and abs(ra.dy - res[0].dy) <= 1e-6
+    ta.release()
+    tb.release()
+
+
+def test_pretweaks_tiles_f32(ctx):
+    """Pretweaks over tiles (each round one FP32 sweep) on the pair stretched 1.04 in x: it changes the
+    tweak, picks the x scale within 0.005 of 1/1.04 as the FP64 oracle does, improves R, and its
+    tweak is within 5e-3 of the oracle's entry by entry."""
+    from tests import bruteforce as bf
+    A, B = bf.xscaled_pair(12)
+    ta = ctx.tile_put(A, normalize=False)
+    tb = ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    r0 = oracle.rfromangle_affine(0.0, pa, va, pb, vb, None, None, olap, req)[1]
+    gW, gr, gch, gn = ctx.pretweaks_tiles(ta, tb, 0.0, None, None, r0, olap, req)
+    oW, orr, och, on = oracle.pretweaks(pa, va, pb, vb, 0.0, None, None, r0, olap, req)
+    assert gch == och == 1 and gr > r0
+    assert abs(gW[0] - 1 / 1.04) <= 0.005 and np.abs(gW - oW).max() <= 5e-3
+    assert abs(gr - orr) <= 1e-3 * orr
+    ta.release()
+    tb.release()
