@@ -435,9 +435,15 @@ int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, con
     return pretweaks_core(eval, tw, bR, changed, n_evals);
 }
 
-// CThmScan's matching state for one pair of point lists: the parameter block (thresholds,
-// sweep switches, disc, Tdfm and the running Tptwk) and the evaluation count.
-struct Thm {
+// RFromAngle at each angle with the deformation D, the pretweak W and the disc (Ox, Oy, Rx, Ry).
+using AngEval = std::function<int(const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx,
+                                  int Ry, std::vector<xc_angle_rec> &out)>;
+// R of each candidate pretweak M(sel[i], mag[i]) * tw at angle deg, with D and the disc.
+using TwkEval = std::function<int(double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+                                  const std::vector<double> &mag, const Aff &tw, std::vector<double> &R)>;
+
+// The two evaluators on point lists in FP64: one device correlation per probe.
+struct PointEvals {
     xc_ctx *ctx;
     const xc_point *ap;
     const double *av;
@@ -445,38 +451,139 @@ struct Thm {
     const xc_point *bp;
     const double *bv;
     int nb;
-    xc_thm_params *p;
     Rules ru;
     std::vector<xc_point> buf;
+    AngEval ang()
+    {
+        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx, int Ry,
+                      std::vector<xc_angle_rec> &out) {
+            out.resize(deg.size());
+            for (size_t i = 0; i < deg.size(); ++i) {
+                const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg[i], &D, &W, ru, Ox, Oy, Rx, Ry, buf, &out[i]);
+                if (st) return st;
+            }
+            return (int)XC_OK;
+        };
+    }
+    TwkEval twk()
+    {
+        return [this](double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+                      const std::vector<double> &mag, const Aff &tw, std::vector<double> &R) {
+            R.resize(sel.size());
+            for (size_t i = 0; i < sel.size(); ++i) {
+                const Aff W = aff_mul(nu_select(sel[i], mag[i]), tw);  // M * Tptwk (RFromAngle's iaux branch)
+                xc_angle_rec C;
+                const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, Ox, Oy, Rx, Ry, buf, &C);
+                if (st) return st;
+                R[i] = C.R;
+            }
+            return (int)XC_OK;
+        };
+    }
+};
 
-    // RFromAngle (CThmScan.cpp:329-366) with the current Tdfm, Tptwk and disc
+// The two evaluators over resident tiles in FP32: every probe list is one batched sweep of
+// the maps R(deg) * (D * W) (xc_sweep_affine_f32); no peak gives R = 0 at (0, 0) as CorrImagesS does.
+struct TileEvals {
+    xc_ctx *ctx;
+    const xc_tile *a, *b;
+    Rules ru;
+    std::vector<double> maps;
+    std::vector<xc_result> res;
+    int sweep(const std::vector<Aff> &T, int Ox, int Oy, int Rx, int Ry)
+    {
+        const size_t n = T.size();
+        maps.resize(4 * n);
+        res.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            maps[4 * i + 0] = T[i].t[0];
+            maps[4 * i + 1] = T[i].t[1];
+            maps[4 * i + 2] = T[i].t[3];
+            maps[4 * i + 3] = T[i].t[4];
+        }
+        const int st = xc_sweep_affine_f32(ctx, a, b, maps.data(), (int)n, &ru.rgn, &ru.cnt, Ox, Oy, Rx, Ry,
+                                           res.data());
+        return st < 0 ? st : (int)XC_OK;
+    }
+    AngEval ang()
+    {
+        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx, int Ry,
+                      std::vector<xc_angle_rec> &out) {
+            std::vector<Aff> T(deg.size());
+            const Aff DW = aff_mul(D, W);
+            for (size_t i = 0; i < deg.size(); ++i) T[i] = aff_mul(nu_select(5, deg[i] * REF_PI / 180.0), DW);
+            const int st = sweep(T, Ox, Oy, Rx, Ry);
+            if (st) return st;
+            out.resize(deg.size());
+            for (size_t i = 0; i < deg.size(); ++i) {
+                const bool ok = res[i].status == XC_OK;
+                out[i].A = deg[i];
+                out[i].R = ok ? res[i].r : 0.0;
+                out[i].X = ok ? res[i].dx : 0.0;
+                out[i].Y = ok ? res[i].dy : 0.0;
+            }
+            return (int)XC_OK;
+        };
+    }
+    TwkEval twk()
+    {
+        return [this](double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+                      const std::vector<double> &mag, const Aff &tw, std::vector<double> &R) {
+            const Aff Rot = nu_select(5, deg * REF_PI / 180.0);
+            std::vector<Aff> T(sel.size());
+            for (size_t i = 0; i < sel.size(); ++i) T[i] = aff_mul(Rot, aff_mul(D, aff_mul(nu_select(sel[i], mag[i]), tw)));
+            const int st = sweep(T, Ox, Oy, Rx, Ry);
+            if (st) return st;
+            R.resize(sel.size());
+            for (size_t i = 0; i < sel.size(); ++i) R[i] = res[i].status == XC_OK ? res[i].r : 0.0;
+            return (int)XC_OK;
+        };
+    }
+};
+
+// CThmScan's matching state for one pair: the parameter block (thresholds, sweep switches,
+// disc, Tdfm and the running Tptwk) and the evaluation count, over either evaluator pair.
+struct Thm {
+    xc_thm_params *p;
+    AngEval eval_ang;
+    TwkEval eval_twk;
+
+    // RFromAngle (CThmScan.cpp:329-366) with the current Tdfm, Tptwk and disc, over a list of angles
+    int rfa_list(const std::vector<double> &deg, std::vector<xc_angle_rec> &out)
+    {
+        p->n_evals += (int)deg.size();
+        return eval_ang(deg, aff_of(p->Tdfm), aff_of(p->Tptwk), p->Ox, p->Oy, p->Rx, p->Ry, out);
+    }
     int rfa(double deg, xc_angle_rec *C)
     {
-        const Aff D = aff_of(p->Tdfm), W = aff_of(p->Tptwk);
-        ++p->n_evals;
-        return rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, p->Ox, p->Oy, p->Rx, p->Ry, buf, C);
+        std::vector<xc_angle_rec> o;
+        const int st = rfa_list(std::vector<double>(1, deg), o);
+        if (!st) *C = o[0];
+        return st;
     }
-    // AngleScanSel (CThmScan.cpp:627-641)
+    // AngleScanSel (CThmScan.cpp:627-641): one list of angles
     int scan_sel(xc_angle_rec *best, double center, double hlfwid, double step, bool favor)
     {
         const int mode = p->const_xy ? XC_SCAN_CONSTXY : XC_SCAN_MAXR;
         const std::vector<double> ang = scan_angles(mode, center, hlfwid, step, favor);
-        std::vector<xc_angle_rec> vC(ang.size());
-        for (size_t i = 0; i < ang.size(); ++i) {
-            const int st = rfa(ang[i], &vC[i]);
-            if (st) return st;
-        }
+        std::vector<xc_angle_rec> vC;
+        const int st = rfa_list(ang, vC);
+        if (st) return st;
         *best = scan_select(mode, vC);
         return XC_OK;
     }
-    // Pretweaks (CThmScan.cpp:205-323) folding into p->Tptwk
+    // Pretweaks (CThmScan.cpp:205-323) folding into p->Tptwk; each round's candidates one list
     int pretweaks(double bestR, double deg, bool *changed)
     {
         Aff tw = aff_of(p->Tptwk);
         double bR = bestR;
         int ev = 0;
-        const int st = pretweaks_run(ctx, ap, av, na, bp, bv, nb, deg, aff_of(p->Tdfm), tw, bR, ru, p->Ox, p->Oy,
-                                     p->Rx, p->Ry, buf, changed, &ev);
+        const Aff D = aff_of(p->Tdfm);
+        const PtwEval e = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
+                              std::vector<double> &R) {
+            return eval_twk(deg, D, p->Ox, p->Oy, p->Rx, p->Ry, sel, mag, t, R);
+        };
+        const int st = pretweaks_core(e, tw, bR, changed, &ev);
         p->n_evals += ev;
         if (!st) memcpy(p->Tptwk, tw.t, sizeof tw.t);
         return st;
@@ -490,10 +597,49 @@ struct Thm {
         if ((st = pretweaks(best->R, best->R > 0.0 ? best->A : center, &changed))) return st;
         return changed ? scan_sel(best, center, hlfwid, step, true) : XC_OK;
     }
-    // PeakHunt (CThmScan.cpp:676-732)
+    // PeakHunt (CThmScan.cpp:676-732): each probe depends on the last decision
     int hunt(xc_angle_rec *best, double hlfwid)
     {
         return peak_hunt(hlfwid, best, nullptr, [&](double deg, xc_angle_rec *rec) { return rfa(deg, rec); });
+    }
+    // UsePriorAngles (CThmScan.cpp:734-758)
+    int use_prior(double ang0, double hfangpr, xc_angle_rec *best, int *ok)
+    {
+        int st = with_tweaks(best, ang0, hfangpr, 0.1);
+        bool good = !st && !(best->R < p->rthresh);
+        if (good) {
+            st = hunt(best, 0.1 * 1.5);
+            good = !st && !(best->R < p->rthresh);
+        }
+        if (ok) *ok = good ? 1 : 0;
+        return st;
+    }
+    // DenovoBestAngle (CThmScan.cpp:764-790)
+    int denovo(double ang0, double hfangdn, double step, xc_angle_rec *best, int *ok)
+    {
+        int st = with_tweaks(best, ang0, hfangdn, step);
+        bool good = !st && !(best->R < p->rthresh);
+        if (good) {
+            st = scan_sel(best, best->A, step * 2.0, step * 0.05, false);
+            good = !st && !(best->R < p->rthresh);
+        }
+        if (good) {
+            st = hunt(best, step * 0.05 * 1.5);
+            good = !st && !(best->R < p->rthresh);
+        }
+        if (ok) *ok = good ? 1 : 0;
+        return st;
+    }
+    // PostTweaks (CThmScan.cpp:796-802)
+    int post_tweaks(xc_angle_rec *best)
+    {
+        p->Ox = (int)best->X;  // SetDisc( (int)best.X, (int)best.Y, 40, 40 )
+        p->Oy = (int)best->Y;
+        p->Rx = p->Ry = 40;
+        bool changed = false;
+        const int st = pretweaks(best->R, best->A, &changed);
+        if (st) return st;
+        return rfa(best->A, best);
     }
 };
 
@@ -565,7 +711,8 @@ extern "C" int xc_thm_scan_with_tweaks_f64(xc_ctx *ctx, const xc_point *ap, cons
                                            double center, double hlfwid, double step, xc_angle_rec *best)
 {
     if (bad_thm(ctx, p, best) || !(step > 0.0)) return XC_E_ARG;
-    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
+    PointEvals E{ctx, ap, av, na, bp, bv, nb, scan_rules(p->olap1D, p->req_area), {}};
+    Thm t{p, E.ang(), E.twk()};
     return t.with_tweaks(best, center, hlfwid, step);
 }
 
@@ -574,15 +721,9 @@ extern "C" int xc_thm_use_prior_angles_f64(xc_ctx *ctx, const xc_point *ap, cons
                                            double ang0, double hfangpr, xc_angle_rec *best, int *ok)
 {
     if (bad_thm(ctx, p, best)) return XC_E_ARG;
-    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
-    int st = t.with_tweaks(best, ang0, hfangpr, 0.1);
-    bool good = !st && !(best->R < p->rthresh);
-    if (good) {
-        st = t.hunt(best, 0.1 * 1.5);
-        good = !st && !(best->R < p->rthresh);
-    }
-    if (ok) *ok = good ? 1 : 0;
-    return st;
+    PointEvals E{ctx, ap, av, na, bp, bv, nb, scan_rules(p->olap1D, p->req_area), {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.use_prior(ang0, hfangpr, best, ok);
 }
 
 extern "C" int xc_thm_denovo_best_angle_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
@@ -590,19 +731,9 @@ extern "C" int xc_thm_denovo_best_angle_f64(xc_ctx *ctx, const xc_point *ap, con
                                             double ang0, double hfangdn, double step, xc_angle_rec *best, int *ok)
 {
     if (bad_thm(ctx, p, best) || !(step > 0.0)) return XC_E_ARG;
-    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
-    int st = t.with_tweaks(best, ang0, hfangdn, step);
-    bool good = !st && !(best->R < p->rthresh);
-    if (good) {
-        st = t.scan_sel(best, best->A, step * 2.0, step * 0.05, false);
-        good = !st && !(best->R < p->rthresh);
-    }
-    if (good) {
-        st = t.hunt(best, step * 0.05 * 1.5);
-        good = !st && !(best->R < p->rthresh);
-    }
-    if (ok) *ok = good ? 1 : 0;
-    return st;
+    PointEvals E{ctx, ap, av, na, bp, bv, nb, scan_rules(p->olap1D, p->req_area), {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.denovo(ang0, hfangdn, step, best, ok);
 }
 
 extern "C" int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, int na,
@@ -610,14 +741,45 @@ extern "C" int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const dou
                                       xc_angle_rec *best)
 {
     if (bad_thm(ctx, p, best)) return XC_E_ARG;
-    Thm t{ctx, ap, av, na, bp, bv, nb, p, scan_rules(p->olap1D, p->req_area), {}};
-    p->Ox = (int)best->X;  // SetDisc( (int)best.X, (int)best.Y, 40, 40 )
-    p->Oy = (int)best->Y;
-    p->Rx = p->Ry = 40;
-    bool changed = false;
-    const int st = t.pretweaks(best->R, best->A, &changed);
-    if (st) return st;
-    return t.rfa(best->A, best);
+    PointEvals E{ctx, ap, av, na, bp, bv, nb, scan_rules(p->olap1D, p->req_area), {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.post_tweaks(best);
+}
+
+extern "C" int xc_thm_scan_with_tweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                           double center, double hlfwid, double step, xc_angle_rec *best)
+{
+    if (bad_thm(ctx, p, best) || !a || !b || !(step > 0.0)) return XC_E_ARG;
+    TileEvals E{ctx, a, b, scan_rules(p->olap1D, p->req_area), {}, {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.with_tweaks(best, center, hlfwid, step);
+}
+
+extern "C" int xc_thm_use_prior_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                           double ang0, double hfangpr, xc_angle_rec *best, int *ok)
+{
+    if (bad_thm(ctx, p, best) || !a || !b) return XC_E_ARG;
+    TileEvals E{ctx, a, b, scan_rules(p->olap1D, p->req_area), {}, {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.use_prior(ang0, hfangpr, best, ok);
+}
+
+extern "C" int xc_thm_denovo_best_angle_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                            double ang0, double hfangdn, double step, xc_angle_rec *best, int *ok)
+{
+    if (bad_thm(ctx, p, best) || !a || !b || !(step > 0.0)) return XC_E_ARG;
+    TileEvals E{ctx, a, b, scan_rules(p->olap1D, p->req_area), {}, {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.denovo(ang0, hfangdn, step, best, ok);
+}
+
+extern "C" int xc_thm_post_tweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                      xc_angle_rec *best)
+{
+    if (bad_thm(ctx, p, best) || !a || !b) return XC_E_ARG;
+    TileEvals E{ctx, a, b, scan_rules(p->olap1D, p->req_area), {}, {}};
+    Thm t{p, E.ang(), E.twk()};
+    return t.post_tweaks(best);
 }
 
 extern "C" int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, double deg, const double *Tdfm,
@@ -625,34 +787,15 @@ extern "C" int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
                                 int Ry, int *changed, int *n_evals)
 {
     if (!ctx || !a || !b || !Tptwk || !bestR) return XC_E_ARG;
-    const Rules ru = scan_rules(olap1D, req_area);
-    const Aff D = aff_of(Tdfm), Rot = nu_select(5, deg * REF_PI / 180.0);
+    TileEvals E{ctx, a, b, scan_rules(olap1D, req_area), {}, {}};
+    const TwkEval twk = E.twk();
+    const Aff D = aff_of(Tdfm);
     Aff tw = aff_of(Tptwk);
     double bR = *bestR;
     bool any = false;
-    std::vector<double> maps;
-    std::vector<xc_result> res;
-    const PtwEval eval = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
-                             std::vector<double> &R) {
-        const size_t n = sel.size();
-        maps.resize(4 * n);
-        res.resize(n);
-        for (size_t i = 0; i < n; ++i) {
-            // RFromAngle's C.T = R * (Tdfm * (M * Tptwk)), its 2 x 2 part
-            const Aff T = aff_mul(Rot, aff_mul(D, aff_mul(nu_select(sel[i], mag[i]), t)));
-            maps[4 * i + 0] = T.t[0];
-            maps[4 * i + 1] = T.t[1];
-            maps[4 * i + 2] = T.t[3];
-            maps[4 * i + 3] = T.t[4];
-        }
-        const int st = xc_sweep_affine_f32(ctx, a, b, maps.data(), (int)n, &ru.rgn, &ru.cnt, Ox, Oy, Rx, Ry,
-                                           res.data());
-        if (st < 0) return st;
-        R.resize(n);
-        for (size_t i = 0; i < n; ++i) R[i] = res[i].status == XC_OK ? res[i].r : 0.0;  // CorrImagesS: 0 without a peak
-        return (int)XC_OK;
-    };
-    const int st = pretweaks_core(eval, tw, bR, &any, n_evals);
+    const PtwEval e = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
+                          std::vector<double> &R) { return twk(deg, D, Ox, Oy, Rx, Ry, sel, mag, t, R); };
+    const int st = pretweaks_core(e, tw, bR, &any, n_evals);
     if (st) return st;
     memcpy(Tptwk, tw.t, sizeof tw.t);
     *bestR = bR;

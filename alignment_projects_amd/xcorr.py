@@ -122,6 +122,15 @@ _OPTIONAL = {
                                  C.POINTER(C.c_int), C.POINTER(C.c_int)], C.c_int),
     "xc_fourier_match_f64": ([C.c_void_p, _pp, _dp, _dp, C.c_int, C.c_int, _dp, C.POINTER(abi.XcFourierQa)],
                              C.c_int),
+    "xc_thm_scan_with_tweaks_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.XcThmParams), C.c_double,
+                                     C.c_double, C.c_double, C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_thm_use_prior_angles_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.XcThmParams), C.c_double,
+                                     C.c_double, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
+    "xc_thm_denovo_best_angle_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.XcThmParams), C.c_double,
+                                      C.c_double, C.c_double, C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)],
+                                     C.c_int),
+    "xc_thm_post_tweaks_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.XcThmParams),
+                                C.POINTER(abi.XcAngleRec)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -474,6 +483,19 @@ class Context:
         st = self.L.xc_fourier_match_f64(self.h, pp, pa, pb, len(a), wvlen, C.byref(d), C.byref(qa))
         self._check(st, "xc_fourier_match_f64", ok=(abi.XC_OK,))
         return qa
+
+    def thm_tiles(self, which, a, b, params, *args, best=None):
+        """The CThmScan sequence over resident tiles in FP32: which is "scan_with_tweaks" (center, hlfwid, step),
+        "use_prior_angles" (ang0, hfangpr), "denovo_best_angle" (ang0, hfangdn, step) or "post_tweaks" (from
+        best).  Returns best, or (best, ok) for the two that report success."""
+        name = f"xc_thm_{which}_f32"
+        b0 = abi.XcAngleRec() if best is None else abi.XcAngleRec(best.A, best.R, best.X, best.Y)
+        okv = C.c_int()
+        with_ok = which in ("use_prior_angles", "denovo_best_angle")
+        extra = [C.byref(b0)] + ([C.byref(okv)] if with_ok else [])
+        st = getattr(self.L, name)(self.h, a.h, b.h, C.byref(params), *args, *extra)
+        self._check(st, name, ok=(abi.XC_OK,))
+        return (b0, okv.value) if with_ok else b0
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

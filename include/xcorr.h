@@ -479,6 +479,19 @@ int xc_thm_post_tweaks_f64(xc_ctx *ctx, const xc_point *ap, const double *av, in
                            const xc_point *bp, const double *bv, int nb, xc_thm_params *p,
                            xc_angle_rec *best);
 
+/* The same sequence over resident tiles in FP32: every scan's angles (with
+ * the current Tdfm and Tptwk) and every Pretweaks round are one batched
+ * sweep (xc_sweep_affine_f32); PeakHunt's probes stay one at a time. */
+int xc_thm_scan_with_tweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                double center, double hlfwid, double step, xc_angle_rec *best);
+int xc_thm_use_prior_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                double ang0, double hfangpr, xc_angle_rec *best, int *ok);
+int xc_thm_denovo_best_angle_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                                 double ang0, double hfangdn, double step, xc_angle_rec *best,
+                                 int *ok);
+int xc_thm_post_tweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_thm_params *p,
+                           xc_angle_rec *best);
+
 /* ------------------------------------------------------------------ */
 /* PixPair::Load's raster preparation (the pair's pre-path)            */
 /* ------------------------------------------------------------------ */

@@ -291,3 +291,40 @@ def test_pretweaks_tiles_f32(ctx):
     assert abs(gr - orr) <= 1e-3 * orr
     ta.release()
     tb.release()
+
+
+def test_thm_sequence_tiles_f32_matches_oracle(ctx):
+    """The CThmScan sequence over resident tiles in FP32 (scans and Pretweaks rounds as batched sweeps)
+    against the FP64 oracle: the same success flags, angles within 0.05 deg, R within 1e-4 relative (PeakHunt
+    may close elsewhere in the flat top, as FP32 resolves it), tweaks within 5e-3."""
+    from tests import bruteforce as bf
+    A, B, olap, req = _scan_pair(11, 2.0)
+    ta, tb = ctx.tile_put(A, normalize=False), ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    for which, args in (("denovo_best_angle", (0.0, 5.0, 0.5)), ("use_prior_angles", (1.8, 0.5))):
+        gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+        g, gok = ctx.thm_tiles(which, ta, tb, gp, *args)
+        o, ook = getattr(oracle, "thm_" + which)(pa, va, pb, vb, op, *args)
+        assert gok == ook == 1, which
+        assert abs(g.A - o.A) <= 0.05 and abs(g.R - o.R) <= 1e-4 * abs(o.R), (which, g.A, o.A, g.R, o.R)
+    ta.release()
+    tb.release()
+    A, B = bf.xscaled_pair(12)
+    ta, tb = ctx.tile_put(A, normalize=False), ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+    gp.rthresh = op.rthresh = 1e9
+    g, gok = ctx.thm_tiles("use_prior_angles", ta, tb, gp, 0.2, 0.5)
+    o, ook = oracle.thm_use_prior_angles(pa, va, pb, vb, op, 0.2, 0.5)
+    assert gok == ook == 0 and np.abs(np.array(gp.Tptwk) - np.array(op.Tptwk)).max() <= 5e-3
+    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+    start = abi.XcAngleRec(0.0, 1.0, 0.4, -0.3)
+    g = ctx.thm_tiles("post_tweaks", ta, tb, gp, best=start)
+    o = oracle.thm_post_tweaks(pa, va, pb, vb, op, start)
+    assert (gp.Ox, gp.Oy, gp.Rx, gp.Ry) == (op.Ox, op.Oy, op.Rx, op.Ry)
+    assert np.abs(np.array(gp.Tptwk) - np.array(op.Tptwk)).max() <= 5e-3 and abs(g.R - o.R) <= 1e-3 * abs(o.R)
+    ta.release()
+    tb.release()
