@@ -93,9 +93,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     size_t i_tbl = lay.add(tbl ? tbl : 1);
     const long long nR = (long long)P.wR * P.hR;
     size_t i_f = lay.add(fmode ? fmode_bytes(nR) : 1);
-    const bool og1 = points_off_grid(ip1, n1), og2 = points_off_grid(ip2, n2);
-    const size_t spl = std::max(og1 ? splat_scratch_bytes(n1, (long long)w1 * h1) : 0,
-                                og2 ? splat_scratch_bytes(n2, (long long)w2 * h2) : 0);
+    // scratch for an ordered splat, in case a list is off the grid (found while staging it)
+    const size_t spl = std::max(splat_scratch_bytes(n1, (long long)w1 * h1), splat_scratch_bytes(n2, (long long)w2 * h2));
     size_t i_spl = lay.add(spl ? spl : 1);
     int st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
@@ -155,8 +154,8 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     xc_result *hres = (xc_result *)(((uintptr_t)(hs2 + (s2_hi - s2_lo)) + 63) & ~(uintptr_t)63);
     auto h1p = [&](size_t i) { return hs + (lay.off[i] - s1_lo); };
     auto h2p = [&](size_t i) { return hs2 + (lay.off[i] - s2_lo); };
-    memcpy(h1p(i_pts), ip1, sizeof(xc_point) * n1);
-    memcpy(h1p(i_pts) + sizeof(xc_point) * n1, ip2, sizeof(xc_point) * n2);
+    const bool og1 = copy_points_off_grid((xc_point *)h1p(i_pts), ip1, n1);
+    const bool og2 = copy_points_off_grid((xc_point *)(h1p(i_pts) + sizeof(xc_point) * n1), ip2, n2);
     memcpy(h1p(i_vals), iv1, sizeof(double) * n1);
     memcpy(h1p(i_vals) + sizeof(double) * n1, iv2, sizeof(double) * n2);
     memcpy(h2p(i_pair), &P, sizeof P);

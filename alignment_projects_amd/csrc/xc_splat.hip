@@ -14,6 +14,9 @@
 // order: the reference's value, bit for bit, and the same every run.
 #include <math.h>
 
+#include <map>
+#include <mutex>
+
 #include <hipcub/hipcub.hpp>
 
 #include "xc_host.hpp"
@@ -68,21 +71,47 @@ static int key_bits(uint32_t npix)
     return b;
 }
 
-bool points_off_grid(const xc_point *p, int n)
+// Copies n points and reports whether any coordinate is not an integer (blocks of 256
+// points without branches, so the rounding test vectorises; coordinates beyond +-2^51
+// count as off the grid).  dst may be null (check only).
+bool copy_points_off_grid(xc_point *dst, const xc_point *p, int n)
 {
-    for (int i = 0; i < n; ++i)
-        if (p[i].x != floor(p[i].x) || p[i].y != floor(p[i].y)) return true;
-    return false;
+    const double C = 6755399441055744.0;  // 1.5 * 2^52: (v + C) - C rounds v to an integer (|v| < 2^51)
+    bool bad = false;
+    for (int i0 = 0; i0 < n; i0 += 256) {
+        const int i1 = n - i0 < 256 ? n : i0 + 256;
+        for (int i = i0; i < i1; ++i) {
+            const double x = p[i].x, y = p[i].y;
+            if (dst) dst[i] = p[i];
+            bad |= ((x + C) - C != x) | ((y + C) - C != y) | !(fabs(x) < 2251799813685248.0) |
+                   !(fabs(y) < 2251799813685248.0);
+        }
+        if (bad && !dst) return true;
+    }
+    return bad;
 }
+
+bool points_off_grid(const xc_point *p, int n) { return copy_points_off_grid(nullptr, p, n); }
 
 size_t splat_scratch_bytes(int n, long long npix)
 {
     if (n <= 0) return 0;
-    const int m = 4 * n;
+    const int m = 4 * n, bits = key_bits((uint32_t)npix);
+    // the sort's temporary storage, asked once per (items, key bits)
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, size_t> known;
     size_t cub = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, cub, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (const double *)nullptr, (double *)nullptr, m, 0,
-                                             key_bits((uint32_t)npix));
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = known.find({m, bits});
+        if (it != known.end()) {
+            cub = it->second;
+        } else {
+            (void)hipcub::DeviceRadixSort::SortPairs(nullptr, cub, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                                     (const double *)nullptr, (double *)nullptr, m, 0, bits);
+            known[{m, bits}] = cub;
+        }
+    }
     return 4 * 256 + (size_t)m * (2 * sizeof(uint32_t) + 2 * sizeof(double)) + cub;
 }
 
