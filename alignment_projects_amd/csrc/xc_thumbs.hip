@@ -1,0 +1,340 @@
+// xc_thumbs.hip -- thumbnail preparation on the device: DecimateVector
+// (Maths.cpp:1315-1364), Normalize (Maths.cpp:192-212) and
+// CThmUtil::MakeThumbs (1_DMesh/CThmUtil.cpp:470-523).
+//
+// DecimateVector sums each point's value into its lbin x lbin cell in point
+// order and counts the points in an 8-bit counter (a cell of 256 points wraps
+// to 0 and is dropped, as in the reference), then lists the cells with a
+// non-zero count in cell order with their means.  Point-list form: every
+// point emits (cell, value); a stable radix sort by cell keeps point order
+// inside a cell, so each run sums exactly as the reference does; a scan over
+// the occupied cells places the outputs.  Raster form (a device picture, every
+// pixel a point in raster order): each cell sums its rows in order, which is
+// the same point order; cells of at most 15 x 15 pixels (no counter wrap).
+#include <math.h>
+#include <string.h>
+
+#include <vector>
+
+#include <hipcub/hipcub.hpp>
+
+#include "xc_host.hpp"
+
+namespace xc {
+
+constexpr int TH_NT = 256;
+
+#pragma clang fp contract(off)
+
+__global__ void k_dec_emit(const xc_point *__restrict__ p, const double *__restrict__ v, int n, int lbin, int wd,
+                           int hd, uint32_t *__restrict__ key, double *__restrict__ val, int *__restrict__ bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int x = (int)p[i].x / lbin, y = (int)p[i].y / lbin;
+    const bool in = p[i].x >= 0.0 && p[i].y >= 0.0 && x < wd && y < hd;
+    key[i] = in ? (uint32_t)x + (uint32_t)wd * (uint32_t)y : (uint32_t)wd * (uint32_t)hd;
+    val[i] = v[i];
+    if (!in) *bad = 1;
+}
+
+// Run heads: the cell's in-order sum and its 8-bit count.
+__global__ void k_dec_runs(const uint32_t *__restrict__ key, const double *__restrict__ val, int m, uint32_t nr,
+                           double *__restrict__ csum, int *__restrict__ ccnt)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t k = key[j];
+    if (k >= nr || (j > 0 && key[j - 1] == k)) return;
+    double s = 0.0;
+    int c = 0;
+    for (int t = j; t < m && key[t] == k; ++t) {
+        s += val[t];
+        ++c;
+    }
+    csum[k] = s;
+    ccnt[k] = c & 255;  // the reference's uint8 counter
+}
+
+__global__ void k_dec_flags(const int *__restrict__ ccnt, int nr, int *__restrict__ flag)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < nr) flag[c] = ccnt[c] != 0;
+}
+
+__global__ void k_dec_out(const double *__restrict__ csum, const int *__restrict__ ccnt, const int *__restrict__ idx,
+                          int nr, int wd, xc_point *__restrict__ op, double *__restrict__ ov, int *__restrict__ kout)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nr) return;
+    const int cnt = ccnt[c];
+    if (c == nr - 1) *kout = idx[c] + (cnt != 0);
+    if (!cnt) return;
+    const int y = c / wd, x = c - wd * y;
+    op[idx[c]] = xc_point{(double)x, (double)y};
+    ov[idx[c]] = csum[c] / cnt;
+}
+
+// Raster form: cell (cx, cy) = mean of its pixels, rows in order.
+template <typename P>
+__global__ void k_dec_raster(const P *__restrict__ ras, int w, int h, int stride, int lbin, int wd, int hd,
+                             double *__restrict__ out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= wd * hd) return;
+    const int cy = c / wd, cx = c - wd * cy;
+    const int x0 = cx * lbin, y0 = cy * lbin;
+    const int x1 = min(w, x0 + lbin), y1 = min(h, y0 + lbin);
+    double s = 0.0;
+    for (int y = y0; y < y1; ++y)
+        for (int x = x0; x < x1; ++x) s += (double)ras[(long long)y * stride + x];
+    out[c] = s / ((x1 - x0) * (y1 - y0));
+}
+
+// Sums (v, v^2) of n values, partial per workgroup; then Normalize in place.
+__global__ void __launch_bounds__(TH_NT) k_th_stats(const double *__restrict__ v, int n, double *__restrict__ part)
+{
+    __shared__ double sa[TH_NT], sb[TH_NT];
+    double a = 0.0, b = 0.0;
+    for (int i = blockIdx.x * TH_NT + threadIdx.x; i < n; i += gridDim.x * TH_NT) {
+        a += v[i];
+        b += v[i] * v[i];
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int off = TH_NT / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            sa[threadIdx.x] += sa[threadIdx.x + off];
+            sb[threadIdx.x] += sb[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa[0];
+        part[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+// MeanStd::Stats (Maths.cpp:111-124), n-1 deviation; msd = (avg, sd).
+__global__ void k_th_fin(const double *__restrict__ part, int np, int n, double *__restrict__ msd)
+{
+    double s = 0.0, q = 0.0;
+    for (int i = 0; i < np; ++i) {
+        s += part[2 * i];
+        q += part[2 * i + 1];
+    }
+    double avg = s, sd = 0.0;
+    if (n > 1) {
+        avg /= n;
+        sd = sqrt(fmax(q - n * avg * avg, 0.0) / (n - 1.0));
+    }
+    msd[0] = avg;
+    msd[1] = sd;
+}
+
+__global__ void k_th_norm(double *__restrict__ v, int n, const double *__restrict__ msd)
+{
+    const double avg = msd[0], sd = msd[1];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = (v[i] - avg) / sd;
+}
+
+#pragma clang fp contract(on)
+
+namespace {
+
+constexpr int TH_PARTS = 64;
+
+// Normalize the device array v[n] in place; *sd (host) receives the natural sd.
+int normalize_dev(xc_ctx *ctx, hipStream_t s, double *v, int n, double *part, double *msd, double *sd)
+{
+    if (n <= 0) {
+        if (sd) *sd = 0.0;
+        return XC_OK;
+    }
+    k_th_stats<<<TH_PARTS, TH_NT, 0, s>>>(v, n, part);
+    k_th_fin<<<1, 1, 0, s>>>(part, TH_PARTS, n, msd);
+    k_th_norm<<<std::min((n + TH_NT - 1) / TH_NT, 4096), TH_NT, 0, s>>>(v, n, msd);
+    XC_HIP(ctx, hipGetLastError());
+    if (sd) {
+        double h[2];
+        XC_HIP(ctx, hipMemcpyAsync(h, msd, sizeof h, hipMemcpyDeviceToHost, s));
+        XC_HIP(ctx, hipStreamSynchronize(s));
+        *sd = h[1];
+    }
+    return XC_OK;
+}
+
+// DecimateVector of n host points into the device arrays op / ov; *k = cells kept.
+int decimate_points(xc_ctx *ctx, const xc_point *p, const double *v, int n, int w, int h, int lbin, DevBuf &buf,
+                    xc_point **dop, double **dov, int *k)
+{
+    hipStream_t s = ctx->stream;
+    const int wd = (w + lbin - 1) / lbin, hd = (h + lbin - 1) / lbin;
+    const long long nrl = (long long)wd * hd;
+    if (nrl >= 0x7fffffffLL) return set_err(ctx, XC_E_SHAPE, "DecimateVector: %lld cells", nrl);
+    const int nr = (int)nrl;
+    int bits = 1;
+    while (bits < 32 && ((uint32_t)nr >> bits) != 0) ++bits;
+    size_t sort_tmp = 0, scan_tmp = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (const double *)nullptr, (double *)nullptr, n, 0, bits);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, (const int *)nullptr, (int *)nullptr, nr);
+    Layout lay;
+    const size_t i_p = lay.add(sizeof(xc_point) * n), i_v = lay.add(sizeof(double) * n);
+    const size_t i_k0 = lay.add(sizeof(uint32_t) * n), i_k1 = lay.add(sizeof(uint32_t) * n);
+    const size_t i_v0 = lay.add(sizeof(double) * n), i_v1 = lay.add(sizeof(double) * n);
+    const size_t i_cs = lay.add(sizeof(double) * nr), i_cc = lay.add(sizeof(int) * nr);
+    const size_t i_fl = lay.add(sizeof(int) * nr), i_ix = lay.add(sizeof(int) * nr);
+    const size_t i_op = lay.add(sizeof(xc_point) * nr), i_ov = lay.add(sizeof(double) * nr);
+    const size_t i_tmp = lay.add(std::max(sort_tmp, scan_tmp) + 16), i_sc = lay.add(sizeof(int) * 4);
+    int st = ensure_dev(ctx, buf, lay.total);
+    if (st) return st;
+    char *base = (char *)buf.p;
+    auto at = [&](size_t i) { return base + lay.off[i]; };
+    XC_HIP(ctx, hipMemcpyAsync(at(i_p), p, sizeof(xc_point) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_v), v, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemsetAsync(at(i_cc), 0, sizeof(int) * nr, s));
+    XC_HIP(ctx, hipMemsetAsync(at(i_sc), 0, sizeof(int) * 4, s));
+    int *sc = (int *)at(i_sc);  // [0] out-of-range flag, [1] cells kept
+    const int nb = (n + TH_NT - 1) / TH_NT, cb = (nr + TH_NT - 1) / TH_NT;
+    k_dec_emit<<<nb, TH_NT, 0, s>>>((const xc_point *)at(i_p), (const double *)at(i_v), n, lbin, wd, hd,
+                                    (uint32_t *)at(i_k0), (double *)at(i_v0), sc);
+    XC_HIP(ctx, hipGetLastError());
+    size_t tb = std::max(sort_tmp, scan_tmp);
+    XC_HIP(ctx, hipcub::DeviceRadixSort::SortPairs((void *)at(i_tmp), tb, (const uint32_t *)at(i_k0),
+                                                   (uint32_t *)at(i_k1), (const double *)at(i_v0), (double *)at(i_v1),
+                                                   n, 0, bits, s));
+    k_dec_runs<<<nb, TH_NT, 0, s>>>((const uint32_t *)at(i_k1), (const double *)at(i_v1), n, (uint32_t)nr,
+                                    (double *)at(i_cs), (int *)at(i_cc));
+    k_dec_flags<<<cb, TH_NT, 0, s>>>((const int *)at(i_cc), nr, (int *)at(i_fl));
+    XC_HIP(ctx, hipGetLastError());
+    tb = std::max(sort_tmp, scan_tmp);
+    XC_HIP(ctx, hipcub::DeviceScan::ExclusiveSum((void *)at(i_tmp), tb, (const int *)at(i_fl), (int *)at(i_ix), nr, s));
+    k_dec_out<<<cb, TH_NT, 0, s>>>((const double *)at(i_cs), (const int *)at(i_cc), (const int *)at(i_ix), nr, wd,
+                                   (xc_point *)at(i_op), (double *)at(i_ov), sc + 1);
+    XC_HIP(ctx, hipGetLastError());
+    int h2[2];
+    XC_HIP(ctx, hipMemcpyAsync(h2, sc, sizeof h2, hipMemcpyDeviceToHost, s));
+    XC_HIP(ctx, hipStreamSynchronize(s));
+    if (h2[0]) return set_err(ctx, XC_E_ARG, "DecimateVector: a point lies outside the %d x %d box", w, h);
+    *dop = (xc_point *)at(i_op);
+    *dov = (double *)at(i_ov);
+    *k = h2[1];
+    return XC_OK;
+}
+
+}  // namespace
+}  // namespace xc
+
+using namespace xc;
+
+extern "C" int xc_decimate_vector_f64(xc_ctx *ctx, xc_point *p, double *v, int n, int w, int h, int lbin, int *n_out)
+{
+    if (!ctx) return XC_E_ARG;
+    if (n < 0 || (n > 0 && (!p || !v)) || w < 1 || h < 1 || lbin < 1)
+        return set_err(ctx, XC_E_ARG, "DecimateVector: bad arguments");
+    if (n == 0) {
+        if (n_out) *n_out = 0;
+        return XC_OK;
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    DevBuf buf;
+    xc_point *dop;
+    double *dov;
+    int k = 0;
+    int st = decimate_points(ctx, p, v, n, w, h, lbin, buf, &dop, &dov, &k);
+    if (!st && k > 0) {
+        if (hipMemcpy(p, dop, sizeof(xc_point) * k, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(v, dov, sizeof(double) * k, hipMemcpyDeviceToHost) != hipSuccess)
+            st = set_err(ctx, XC_E_DEVICE, "DecimateVector: download failed");
+    }
+    (void)hipFree(buf.p);
+    if (!st && n_out) *n_out = k;
+    return st;
+}
+
+extern "C" int xc_make_thumbs_f64(xc_ctx *ctx, xc_point *ap, double *av, int na, int aw, int ah, xc_point *bp,
+                                  double *bv, int nb, int bw, int bh, int decfactor, int *olap1D, int64_t *req_area,
+                                  int *na_out, int *nb_out)
+{
+    if (!ctx || !olap1D || !req_area || !na_out || !nb_out) return XC_E_ARG;
+    if (na < 1 || nb < 1 || !ap || !av || !bp || !bv || decfactor < 1)
+        return set_err(ctx, XC_E_ARG, "MakeThumbs: bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    int k[2] = {na, nb};
+    double sd[2] = {0.0, 0.0};
+    int st = XC_OK;
+    for (int side = 0; side < 2 && !st; ++side) {
+        xc_point *p = side ? bp : ap;
+        double *v = side ? bv : av;
+        const int n = side ? nb : na, w = side ? bw : aw, h = side ? bh : ah;
+        DevBuf buf, nbuf;
+        double *dv = nullptr;
+        if (decfactor > 1) {
+            xc_point *dp;
+            st = decimate_points(ctx, p, v, n, w, h, decfactor, buf, &dp, &dv, &k[side]);
+            if (!st && k[side] > 0 && hipMemcpyAsync(p, dp, sizeof(xc_point) * k[side], hipMemcpyDeviceToHost, s))
+                st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: download failed");
+        } else if (!(st = ensure_dev(ctx, buf, sizeof(double) * n))) {
+            dv = (double *)buf.p;
+            if (hipMemcpyAsync(dv, v, sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+                st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: upload failed");
+        }
+        if (!st && !(st = ensure_dev(ctx, nbuf, sizeof(double) * (2 * TH_PARTS + 2))))
+            st = normalize_dev(ctx, s, dv, k[side], (double *)nbuf.p, (double *)nbuf.p + 2 * TH_PARTS, &sd[side]);
+        if (!st && k[side] > 0 &&
+            (hipMemcpyAsync(v, dv, sizeof(double) * k[side], hipMemcpyDeviceToHost, s) != hipSuccess ||
+             hipStreamSynchronize(s) != hipSuccess))
+            st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: download failed");
+        (void)hipFree(buf.p);
+        (void)hipFree(nbuf.p);
+    }
+    if (st) return st;
+    int64_t req = *req_area;
+    int olap = *olap1D;
+    if (decfactor > 1) {
+        req /= (int64_t)decfactor * decfactor;
+        olap /= decfactor;
+    }
+    *req_area = req;
+    *olap1D = olap;
+    *na_out = k[0];
+    *nb_out = k[1];
+    if (k[0] < req) return set_err(ctx, XC_TOO_SMALL, "MakeThumbs: small intersection %d (required %lld)", k[0],
+                                   (long long)req);
+    if (!(sd[0] != 0.0)) return set_err(ctx, XC_NO_PEAK, "MakeThumbs: image A intersection region: stdev = 0");
+    if (!(sd[1] != 0.0)) return set_err(ctx, XC_NO_PEAK, "MakeThumbs: image B intersection region: stdev = 0");
+    return XC_OK;
+}
+
+extern "C" int xc_make_thumb_raster_dev_f64(xc_ctx *ctx, const void *ras, int dtype, int w, int h, int stride,
+                                            int lbin, double *out, double *sd)
+{
+    if (!ctx) return XC_E_ARG;
+    if (!ras || !out || w < 1 || h < 1 || lbin < 1 || lbin > 15 || stride < w)
+        return set_err(ctx, XC_E_ARG, "thumbnail raster: bad arguments (bins of 1..15)");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const int wd = (w + lbin - 1) / lbin, hd = (h + lbin - 1) / lbin, nc = wd * hd, cb = (nc + TH_NT - 1) / TH_NT;
+    switch (dtype) {
+    case XC_U8: k_dec_raster<uint8_t><<<cb, TH_NT, 0, s>>>((const uint8_t *)ras, w, h, stride, lbin, wd, hd, out); break;
+    case XC_U16:
+        k_dec_raster<uint16_t><<<cb, TH_NT, 0, s>>>((const uint16_t *)ras, w, h, stride, lbin, wd, hd, out);
+        break;
+    case XC_F32: k_dec_raster<float><<<cb, TH_NT, 0, s>>>((const float *)ras, w, h, stride, lbin, wd, hd, out); break;
+    case XC_F64: k_dec_raster<double><<<cb, TH_NT, 0, s>>>((const double *)ras, w, h, stride, lbin, wd, hd, out); break;
+    default: return set_err(ctx, XC_E_ARG, "thumbnail raster: unknown pixel type %d", dtype);
+    }
+    XC_HIP(ctx, hipGetLastError());
+    DevBuf nbuf;
+    int st = ensure_dev(ctx, nbuf, sizeof(double) * (2 * TH_PARTS + 2));
+    if (!st) st = normalize_dev(ctx, s, out, nc, (double *)nbuf.p, (double *)nbuf.p + 2 * TH_PARTS, sd);
+    if (!st && !sd && hipStreamSynchronize(s) != hipSuccess) st = set_err(ctx, XC_E_DEVICE, "synchronise failed");
+    (void)hipFree(nbuf.p);
+    return st;
+}

@@ -80,11 +80,14 @@ def _holes(gen, h, w, max_holes=3, rmin=10.0, rmax=80.0):
 # ----------------------------------------------------------------------------
 # Cross-layer thumbnail rotation sweep (CThmScan): 2048 x 2048 uint16 pairs,
 # B rotated by U[-2.5, 2.5] deg and shifted by U[-60, 60] px, both decimated
-# 8x by box mean (DecimateVector) to 256 x 256 and normalised.
+# 8x (DecimateVector's box means) to 256 x 256 and normalised by the library's
+# thumbnail kernel (MakeThumbs on the device).
 # ----------------------------------------------------------------------------
 
-def thumb_pairs(n, seed0=3000, size=2048, dec=8, mean=30000.0, sd=6000.0, max_deg=2.5, max_shift=60.0):
-    """Returns (A list, B list) of float64 numpy thumbnails and the true (deg, sx, sy) per pair."""
+def thumb_pairs(ctx, n, seed0=3000, size=2048, dec=8, mean=30000.0, sd=6000.0, max_deg=2.5, max_shift=60.0):
+    """Returns (A list, B list) of float64 numpy thumbnails and the true (deg, sx, sy) per pair.  ctx (an
+    xcorr.Context) makes the thumbnails from the 16-bit pictures on the device."""
+    from . import abi
     A, B, truth = [], [], []
     mg = 160
     n_f = size + 2 * mg
@@ -106,8 +109,10 @@ def thumb_pairs(n, seed0=3000, size=2048, dec=8, mean=30000.0, sd=6000.0, max_de
         b16 = quantize(_bilinear(Fd, xs, ys), 65535)
         out = []
         for im in (a16, b16):
-            t = F.avg_pool2d(im.to(torch.float64)[None, None], dec)[0, 0]
-            t = (t - t.mean()) / t.std()  # Normalize: mean 0, sd 1 with the n-1 deviation
+            pic = im.to(torch.float32).contiguous()  # 16-bit values, exact in FP32
+            t = torch.empty((size // dec, size // dec), dtype=torch.float64, device=_DEV)
+            torch.cuda.synchronize()  # the library works on its own stream: pic must be written first
+            ctx.thumb_raster_dev(pic.data_ptr(), abi.XC_F32, size, size, size, dec, t.data_ptr())
             out.append(t.cpu().numpy())
         A.append(out[0])
         B.append(out[1])

@@ -131,6 +131,13 @@ _OPTIONAL = {
                                      C.c_int),
     "xc_thm_post_tweaks_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(abi.XcThmParams),
                                 C.POINTER(abi.XcAngleRec)], C.c_int),
+    "xc_decimate_vector_f64": ([C.c_void_p, _pp, _dp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)],
+                               C.c_int),
+    "xc_make_thumbs_f64": ([C.c_void_p, _pp, _dp, C.c_int, C.c_int, C.c_int, _pp, _dp, C.c_int, C.c_int, C.c_int,
+                           C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                           C.POINTER(C.c_int)], C.c_int),
+    "xc_make_thumb_raster_dev_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_void_p, C.POINTER(C.c_double)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -496,6 +503,36 @@ class Context:
         st = getattr(self.L, name)(self.h, a.h, b.h, C.byref(params), *args, *extra)
         self._check(st, name, ok=(abi.XC_OK,))
         return (b0, okv.value) if with_ok else b0
+
+    def decimate_vector(self, pts, vals, w, h, lbin):
+        """DecimateVector on host lists. Returns (points (k, 2), values (k,))."""
+        p = np.array(pts, dtype=np.float64).reshape(-1, 2).copy()
+        v = np.array(vals, dtype=np.float64).copy()
+        k = C.c_int()
+        st = self.L.xc_decimate_vector_f64(self.h, p.ctypes.data_as(_pp), v.ctypes.data_as(_dp), len(v), w, h, lbin,
+                                           C.byref(k))
+        self._check(st, "xc_decimate_vector_f64", ok=(abi.XC_OK,))
+        return p[:k.value], v[:k.value]
+
+    def make_thumbs(self, ap, av, aw, ah, bp, bv, bw, bh, decfactor, olap1d, req_area):
+        """MakeThumbs on both sides. Returns (status, ap, av, bp, bv, olap1d, req_area)."""
+        a_p = np.array(ap, dtype=np.float64).reshape(-1, 2).copy()
+        a_v = np.array(av, dtype=np.float64).copy()
+        b_p = np.array(bp, dtype=np.float64).reshape(-1, 2).copy()
+        b_v = np.array(bv, dtype=np.float64).copy()
+        ol, rq, ka, kb = C.c_int(olap1d), C.c_int64(req_area), C.c_int(), C.c_int()
+        st = self.L.xc_make_thumbs_f64(self.h, a_p.ctypes.data_as(_pp), a_v.ctypes.data_as(_dp), len(a_v), aw, ah,
+                                       b_p.ctypes.data_as(_pp), b_v.ctypes.data_as(_dp), len(b_v), bw, bh, decfactor,
+                                       C.byref(ol), C.byref(rq), C.byref(ka), C.byref(kb))
+        self._check(st, "xc_make_thumbs_f64", ok=(abi.XC_OK, abi.XC_TOO_SMALL, abi.XC_NO_PEAK))
+        return st, a_p[:ka.value], a_v[:ka.value], b_p[:kb.value], b_v[:kb.value], ol.value, rq.value
+
+    def thumb_raster_dev(self, ras, dtype, w, h, stride, lbin, out_ptr):
+        """Thumbnail of a device picture (pointer ras) into the device array out_ptr; returns the natural sd."""
+        sd = C.c_double()
+        st = self.L.xc_make_thumb_raster_dev_f64(self.h, ras, dtype, w, h, stride, lbin, out_ptr, C.byref(sd))
+        self._check(st, "xc_make_thumb_raster_dev_f64", ok=(abi.XC_OK,))
+        return sd.value
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

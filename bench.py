@@ -283,13 +283,13 @@ def run_sweep(args, sync):
 
     NP = args.pairs or 64
     t0 = time.time()
-    A, B, truth = workloads.thumb_pairs(NP)
+    ctx = xcorr.Context(0)
+    A, B, truth = workloads.thumb_pairs(ctx, NP)
     log(f"generated {NP} thumbnail pairs in {time.time() - t0:.1f}s")
     angles = synth.sweep_angles()
     h, w = A[0].shape
     req, olap = int(0.25 * w * h), 64
     rgn, cnt = abi.rgn_big_enough(olap, req), abi.cnt_enough_points(req)
-    ctx = xcorr.Context(0)
     tiles = [(ctx.tile_put(A[i].astype(np.float32), normalize=False),
               ctx.tile_put(B[i].astype(np.float32), normalize=False)) for i in range(NP)]
     pairs = [xcorr.make_pair(ta, tb, mode=abi.XC_MODE_S, rgn=rgn, cnt=cnt, angle=ang)

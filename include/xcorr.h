@@ -493,6 +493,44 @@ int xc_thm_post_tweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, xc_t
                            xc_angle_rec *best);
 
 /* ------------------------------------------------------------------ */
+/* Thumbnails                                                          */
+/* ------------------------------------------------------------------ */
+
+/*
+ * DecimateVector (Maths.cpp:1315-1364), in place on host arrays: the n
+ * points (bounding-box corner at (0, 0), inside w x h) summed in point order
+ * into lbin x lbin cells and counted in an 8-bit counter (a cell of 256
+ * points wraps to 0 and is dropped, as in the reference); the cells with a
+ * non-zero count, in cell order, replace the list as (cell x, cell y) and
+ * their mean.  *n_out receives the new length.  XC_E_ARG for a point outside
+ * the box.
+ */
+int xc_decimate_vector_f64(xc_ctx *ctx, xc_point *p, double *v, int n, int w, int h, int lbin,
+                           int *n_out);
+
+/*
+ * CThmUtil::MakeThumbs (1_DMesh/CThmUtil.cpp:470-523) on both sides, in place:
+ * DecimateVector by decfactor (when > 1; *olap1D and *req_area are divided as
+ * the reference divides them), then Normalize each value list.  *na_out and
+ * *nb_out receive the lengths.  XC_TOO_SMALL when A keeps fewer than
+ * *req_area points, XC_NO_PEAK when a side's sd is 0 (the reference's two
+ * failures).
+ */
+int xc_make_thumbs_f64(xc_ctx *ctx, xc_point *ap, double *av, int na, int aw, int ah,
+                       xc_point *bp, double *bv, int nb, int bw, int bh, int decfactor,
+                       int *olap1D, int64_t *req_area, int *na_out, int *nb_out);
+
+/*
+ * The thumbnail of a whole device picture (every pixel a point, raster order):
+ * lbin x lbin box means (DecimateVector's values, cells of 1..15 pixels so the
+ * count never wraps) into out[ceil(h/lbin)][ceil(w/lbin)] (device, FP64), then
+ * Normalize; *sd (may be NULL) receives the natural sd.  dtype: XC_U8, XC_U16,
+ * XC_F32, XC_F64; stride in pixels.
+ */
+int xc_make_thumb_raster_dev_f64(xc_ctx *ctx, const void *ras, int dtype, int w, int h, int stride,
+                                 int lbin, double *out, double *sd);
+
+/* ------------------------------------------------------------------ */
 /* PixPair::Load's raster preparation (the pair's pre-path)            */
 /* ------------------------------------------------------------------ */
 

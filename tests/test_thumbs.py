@@ -1,0 +1,110 @@
+"""Thumbnail preparation: DecimateVector (0_GEN/Maths.cpp:1315-1364), Normalize and CThmUtil::MakeThumbs
+(1_DMesh/CThmUtil.cpp:470-523).
+
+The oracle's DecimateVector is pinned by block means (numpy reshape-means on a full raster, the partial
+cells at the right and bottom edges by hand) and by the reference's 8-bit counter: a full 16 x 16 cell holds
+256 points, wraps to 0 and is dropped.  The device forms are held to the oracle: the point-list form bit for
+bit, also on a shuffled list with holes (point order inside a cell is kept), the raster form and MakeThumbs
+after Normalize within 1e-10 (the one-pass variance's conditioning)."""
+import numpy as np
+import pytest
+
+import oracle
+from alignment_projects_amd import abi, synth
+
+
+def _raster(seed, h, w):
+    return np.random.default_rng(seed).normal(100.0, 20.0, size=(h, w))
+
+
+def test_decimate_is_block_means_with_edge_cells():
+    img = _raster(1, 20, 30)
+    pts, vals = synth.raster_points(img)
+    p, v = oracle.decimate_vector(pts, vals, 30, 20, 8)
+    assert len(v) == 4 * 3  # ceil(30 / 8) x ceil(20 / 8) cells, all occupied
+    full = img[:16, :24].reshape(2, 8, 3, 8).mean(axis=(1, 3))
+    got = v.reshape(3, 4)
+    assert np.abs(got[:2, :3] - full).max() <= 1e-12
+    assert abs(got[2, 3] - img[16:, 24:].mean()) <= 1e-12 and abs(got[0, 3] - img[:8, 24:].mean()) <= 1e-12
+    assert np.array_equal(p[:4], [[0, 0], [1, 0], [2, 0], [3, 0]])
+
+
+def test_decimate_counter_wraps_like_the_reference():
+    """A full 16 x 16 cell counts 256 points in the reference's uint8 counter, which wraps to 0: the cell is
+    dropped.  Edge cells with fewer points stay."""
+    img = _raster(2, 40, 40)
+    pts, vals = synth.raster_points(img)
+    p, v = oracle.decimate_vector(pts, vals, 40, 40, 16)
+    kept = {(int(x), int(y)) for x, y in p}
+    assert kept == {(2, 0), (2, 1), (0, 2), (1, 2), (2, 2)}  # the 8-wide and 8-high edge cells only
+    p15, _ = oracle.decimate_vector(pts, vals, 40, 40, 15)
+    assert len(p15) == 9
+
+
+# ---------------------------------------------------------------------------------------------- device
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lbin", [2, 8, 16])
+def test_decimate_points_matches_oracle_bitwise(ctx, lbin):
+    img = _raster(3, 203, 301)
+    mask = np.ones(img.shape, bool)
+    mask[50:90, 100:170] = False  # a hole
+    pts, vals = synth.raster_points(img, mask)
+    perm = np.random.default_rng(4).permutation(len(vals))
+    for order in (np.arange(len(vals)), perm):
+        g_p, g_v = ctx.decimate_vector(pts[order], vals[order], 301, 203, lbin)
+        o_p, o_v = oracle.decimate_vector(pts[order], vals[order], 301, 203, lbin)
+        assert np.array_equal(g_p, o_p) and np.array_equal(g_v, o_v)
+
+
+@pytest.mark.gpu
+def test_make_thumbs_matches_oracle(ctx):
+    A, B = _raster(5, 160, 240), _raster(6, 150, 260)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    st, ga_p, ga_v, gb_p, gb_v, olap, req = ctx.make_thumbs(pa, va, 240, 160, pb, vb, 260, 150, 4, 64, 8000)
+    assert st == abi.XC_OK and (olap, req) == (16, 500)
+    for (gp, gv), (p, v, w, h) in (((ga_p, ga_v), (pa, va, 240, 160)), ((gb_p, gb_v), (pb, vb, 260, 150))):
+        o_p, o_v = oracle.decimate_vector(p, v, w, h, 4)
+        # Normalize's one-pass variance (sum of squares minus n mean^2) turns the different summation
+        # order of the device's tree into ~1e-12 here (mean 100, sd 20): 1e-10
+        assert np.array_equal(gp, o_p) and np.abs(gv - oracle.normalize(o_v)).max() <= 1e-10
+    st = ctx.make_thumbs(pa, va, 240, 160, pb, vb, 260, 150, 4, 64, 8000 * 16)[0]
+    assert st == abi.XC_TOO_SMALL
+
+
+def _hip():
+    """The HIP runtime libxcorr.so already loaded (device buffers without another framework)."""
+    import ctypes as C
+    from alignment_projects_amd import xcorr
+    xcorr.load()
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    return hip
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", ["u16", "f32"])
+def test_thumb_raster_matches_oracle(ctx, dtype):
+    import ctypes as C
+    hip = _hip()
+    img = np.clip(_raster(7, 2048, 2048) * 300, 0, 65535).round()
+    host = np.ascontiguousarray(img.astype(np.float32 if dtype == "f32" else np.uint16))
+    d_in, d_out = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_in), host.nbytes) == 0 and hip.hipMalloc(C.byref(d_out), 256 * 256 * 8) == 0
+    try:
+        assert hip.hipMemcpy(d_in, host.ctypes.data, host.nbytes, 1) == 0  # host to device
+        code = abi.XC_F32 if dtype == "f32" else abi.XC_U16
+        sd = ctx.thumb_raster_dev(d_in, code, 2048, 2048, 2048, 8, d_out)
+        out = np.empty(256 * 256)
+        assert hip.hipMemcpy(out.ctypes.data, d_out, out.nbytes, 2) == 0  # device to host
+    finally:
+        hip.hipFree(d_in)
+        hip.hipFree(d_out)
+    pts, vals = synth.raster_points(img)
+    _, o_v = oracle.decimate_vector(pts, vals, 2048, 2048, 8)
+    assert abs(sd - np.std(o_v, ddof=1)) <= 1e-9 * sd
+    assert np.abs(out - oracle.normalize(o_v)).max() <= 1e-10
