@@ -144,7 +144,8 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
     int olw = BR - BL + 1;
     bool ok = rgn_ok(P.rule, olw, rb.olh);
     *odd = false;
-    double Nxy = (double)P.Nx * (double)P.Ny;
+    // Nx Ny is a power of two: multiplying by its reciprocal is the division, exactly
+    const double invNxy = 1.0 / ((double)P.Nx * (double)P.Ny);
     if (P.mode == PM_NCC) {
         if (!ok) { *legal = false; return 0.0; }
         if (P.rule.cnt_enabled) {
@@ -152,18 +153,18 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
             int c2 = box_sum(P.ZB, P.w2, BL, BR, rb.b_b, rb.b_t);
             if (!((long long)c1 > P.rule.k1 && (long long)c2 > P.rule.k2)) { *legal = false; return 0.0; }
         }
-    } else if (P.rule.cnt_enabled) {
+    } else if (ok && P.rule.cnt_enabled) {  // a lag the region rule rejects needs no counts
         int c1 = box_sum(P.ZA, P.w1, AL, AR, rb.a_b, rb.a_t);
         int c2 = box_sum(P.ZB, P.w2, BL, BR, rb.b_b, rb.b_t);
         if (!((long long)c1 > P.rule.k1 && (long long)c2 > P.rule.k2)) ok = false;
     }
     *legal = ok;
-    if (P.mode == PM_S) return raw / Nxy;
+    if (P.mode == PM_S) return raw * invNxy;
     double n = (double)(olw * rb.olh);
     const double2 sqa = box_sum2<FT>(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
     const double2 sqb = box_sum2<FT>(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
     double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
-    double num = n * raw / Nxy - s1 * s2;
+    double num = n * raw * invNxy - s1 * s2;
     double d1 = n * q1 - s1 * s1;
     double d2 = n * q2 - s2 * s2;
     double d = d1 * d2;
