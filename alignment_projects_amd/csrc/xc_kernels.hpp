@@ -1183,11 +1183,23 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
             p.y = p.y * filt;
         }
         if constexpr (WHITEN) {
-            T mag = hypot(p.x, p.y);
-            if (mag > (T)1e-10) {
-                T s = sqrt(mag);
-                p.x /= s;
-                p.y /= s;
+            if constexpr (sizeof(T) == 8) {
+                // the reference's arithmetic (Correlation.cpp:2311-2317): |P|, then P / sqrt(|P|)
+                T mag = hypot(p.x, p.y);
+                if (mag > (T)1e-10) {
+                    T s = sqrt(mag);
+                    p.x /= s;
+                    p.y /= s;
+                }
+            } else {
+                // FP32: P * |P|^(-1/2) from |P|^2 (|P| > 1e-10 is |P|^2 > 1e-20): one square root and
+                // one reciprocal square root; |P|^2 beyond the FP32 range (un-normalised pixels) takes hypot
+                const float m2 = p.x * p.x + p.y * p.y;
+                if (m2 > 1e-20f) {
+                    const float s = m2 < 1e38f ? __frsqrt_rn(__fsqrt_rn(m2)) : 1.0f / sqrtf(hypotf(p.x, p.y));
+                    p.x *= s;
+                    p.y *= s;
+                }
             }
         }
         a[m] = p;
