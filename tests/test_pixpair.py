@@ -153,3 +153,12 @@ def test_prep_rejects_what_the_reference_fails(ctx):
         ctx.pixpair_prep(raster(1, 32, 32), order=7)
     with pytest.raises(XcorrError):
         ctx.pixpair_prep(raster(1, 32, 32), order=1, dog=True, r1=3, r2=19)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("h,w,order", [(3, 5, 1), (1, 64, 0), (17, 1, 2)])
+def test_tiny_rasters_match_oracle(ctx, h, w, order):
+    ras = raster(40 + h, h, w)
+    g = ctx.pixpair_prep(ras, order=order)
+    o = oracle.pixpair_prep(ras, order=order)
+    assert np.allclose(g["vf"], o["vf"], atol=1e-9, equal_nan=True)

@@ -108,3 +108,21 @@ def test_thumb_raster_matches_oracle(ctx, dtype):
     _, o_v = oracle.decimate_vector(pts, vals, 2048, 2048, 8)
     assert abs(sd - np.std(o_v, ddof=1)) <= 1e-9 * sd
     assert np.abs(out - oracle.normalize(o_v)).max() <= 1e-10
+
+
+@pytest.mark.gpu
+def test_thumbs_edges(ctx):
+    """Empty list; MakeThumbs without decimation (Normalize only); a raster bin that would wrap the counter
+    is refused; the stdev = 0 failure."""
+    from alignment_projects_amd.xcorr import XcorrError
+    p, v = ctx.decimate_vector(np.zeros((0, 2)), np.zeros(0), 10, 10, 4)
+    assert len(p) == 0 and len(v) == 0
+    A = _raster(8, 30, 40)
+    pa, va = synth.raster_points(A)
+    st, ga_p, ga_v, gb_p, gb_v, olap, req = ctx.make_thumbs(pa, va, 40, 30, pa, va, 40, 30, 1, 10, 100)
+    assert st == abi.XC_OK and (olap, req) == (10, 100) and np.array_equal(ga_p, pa)
+    assert np.abs(ga_v - oracle.normalize(va)).max() <= 1e-10
+    st = ctx.make_thumbs(pa, np.full(len(va), 7.0), 40, 30, pa, va, 40, 30, 1, 10, 100)[0]
+    assert st == abi.XC_NO_PEAK
+    with pytest.raises(XcorrError):
+        ctx.thumb_raster_dev(1, abi.XC_F32, 64, 64, 64, 16, 1)  # refused before any device access
