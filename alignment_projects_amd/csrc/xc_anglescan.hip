@@ -802,3 +802,27 @@ extern "C" int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
     if (changed) *changed = any ? 1 : 0;
     return XC_OK;
 }
+
+extern "C" int xc_finish_at_full_res_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *Tdfm,
+                                         const double *Tptwk, int olap1D, int64_t req_area, xc_angle_rec *best,
+                                         int *used)
+{
+    if (!ctx || !a || !b || !best) return XC_E_ARG;
+    TileEvals E{ctx, a, b, scan_rules(olap1D, req_area), {}, {}};
+    const AngEval ang = E.ang();
+    const xc_angle_rec b0 = *best;
+    std::vector<xc_angle_rec> out;
+    // SetDisc( (int)best.X, (int)best.Y, 80, 80 ), then RFromAngle( best, best.A )
+    const int st = ang(std::vector<double>(1, b0.A), aff_of(Tdfm), aff_of(Tptwk), (int)b0.X, (int)b0.Y, 80, 80, out);
+    if (st) return st;
+    const xc_angle_rec &C = out[0];
+    const bool ok = fabs(C.X - b0.X) <= 20 && fabs(C.Y - b0.Y) <= 20;
+    if (ok) {
+        *best = C;
+        best->R = b0.R;  // the thumbnail R, comparable across the table
+    } else {
+        *best = b0;
+    }
+    if (used) *used = ok ? 1 : 0;
+    return XC_OK;
+}

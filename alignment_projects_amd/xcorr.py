@@ -138,6 +138,8 @@ _OPTIONAL = {
                            C.POINTER(C.c_int)], C.c_int),
     "xc_make_thumb_raster_dev_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "xc_finish_at_full_res_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_int64,
+                                   C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
     "xc_thmpair_append": ([C.c_char_p, C.POINTER(abi.XcThmPair)], C.c_int),
     "xc_thmpair_read_all": ([C.c_char_p, C.POINTER(abi.XcThmPair), C.c_int], C.c_int),
@@ -533,6 +535,17 @@ class Context:
         st = self.L.xc_make_thumb_raster_dev_f64(self.h, ras, dtype, w, h, stride, lbin, out_ptr, C.byref(sd))
         self._check(st, "xc_make_thumb_raster_dev_f64", ok=(abi.XC_OK,))
         return sd.value
+
+    def finish_at_full_res_tiles(self, best, a, b, Tdfm, Tptwk, olap1d, req_area):
+        """FinishAtFullRes over full-resolution resident tiles in FP32. Returns (best, used)."""
+        D = None if Tdfm is None else np.ascontiguousarray(Tdfm, dtype=np.float64)
+        W = None if Tptwk is None else np.ascontiguousarray(Tptwk, dtype=np.float64)
+        bb, u = abi.XcAngleRec(best.A, best.R, best.X, best.Y), C.c_int()
+        st = self.L.xc_finish_at_full_res_f32(self.h, a.h, b.h, None if D is None else D.ctypes.data_as(_dp),
+                                              None if W is None else W.ctypes.data_as(_dp), olap1d, req_area,
+                                              C.byref(bb), C.byref(u))
+        self._check(st, "xc_finish_at_full_res_f32", ok=(abi.XC_OK,))
+        return bb, u.value
 
     def plan(self, pairs, refresh_tiles=False, autotune=False):
         return Plan(self, pairs, refresh_tiles, autotune)

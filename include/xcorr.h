@@ -430,6 +430,10 @@ int xc_finish_at_full_res_f64(xc_ctx *ctx, const xc_point *ap, const double *av,
                               const xc_point *bp, const double *bv, int nb,
                               const double *Tdfm, const double *Tptwk,
                               int olap1D, int64_t req_area, xc_angle_rec *best, int *used);
+/* The same over full-resolution resident tiles, one FP32 sweep of one map. */
+int xc_finish_at_full_res_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *Tdfm,
+                              const double *Tptwk, int olap1D, int64_t req_area, xc_angle_rec *best,
+                              int *used);
 
 /*
  * CThmScan's matcher state for one pair (CThmScan.h:85-97): thresholds,

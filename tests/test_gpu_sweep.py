@@ -328,3 +328,22 @@ def test_thm_sequence_tiles_f32_matches_oracle(ctx):
     assert np.abs(np.array(gp.Tptwk) - np.array(op.Tptwk)).max() <= 5e-3 and abs(g.R - o.R) <= 1e-3 * abs(o.R)
     ta.release()
     tb.release()
+
+
+def test_finish_at_full_res_tiles_f32(ctx):
+    """FinishAtFullRes over tiles in FP32 against the FP64 oracle: the same use flag, X and Y within
+    0.05 px when used, the record unchanged when not."""
+    A, B = synth.rotated_pair(11, size=96, deg=0.0, shift=(2.0, -1.0))
+    ta, tb = ctx.tile_put(A, normalize=False), ctx.tile_put(B, normalize=False)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    olap, req = 24, 96 * 96 // 4
+    Tdfm, Ttw = [1.0, 0.01, 0.0, -0.01, 1.0, 0.0], [0.995, 0.0, 0.0, 0.0, 1.0, 0.0]
+    for x, y in ((7.4, -4.2), (32.0, -1.0)):
+        best = abi.XcAngleRec(0.25, 0.55, x, y)
+        g, gu = ctx.finish_at_full_res_tiles(best, ta, tb, Tdfm, Ttw, olap, req)
+        o, ou = oracle.finish_at_full_res(best, pa, va, pb, vb, Tdfm, Ttw, olap, req)
+        assert gu == ou and g.A == o.A and g.R == o.R
+        assert abs(g.X - o.X) <= 0.05 and abs(g.Y - o.Y) <= 0.05
+    ta.release()
+    tb.release()
