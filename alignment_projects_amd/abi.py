@@ -102,6 +102,11 @@ class XcFourierQa(C.Structure):
                 ("nx", C.c_int32), ("ny", C.c_int32), ("_pad", C.c_int32)]
 
 
+class XcSubimage(C.Structure):
+    """One side of CThmUtil::Crop's overlap (SubI, 1_DMesh/CThmUtil.h:14-20): corner, size, point count."""
+    _fields_ = [("Ox", C.c_double), ("Oy", C.c_double), ("w", C.c_int32), ("h", C.c_int32), ("n", C.c_int64)]
+
+
 class XcThmPair(C.Structure):
     """One ThmPair_za^zb.txt entry (ThmPair, 0_GEN/PipeFiles.h:145-152)."""
     _fields_ = [("atl", C.c_int32), ("acr", C.c_int32), ("btl", C.c_int32), ("bcr", C.c_int32),

@@ -11,9 +11,17 @@
 // the occupied cells places the outputs.  Raster form (a device picture, every
 // pixel a point in raster order): each cell sums its rows in order, which is
 // the same point order; cells of at most 15 x 15 pixels (no counter wrap).
+//
+// Before them, CThmUtil::Crop / Crop_NoCR (1_DMesh/CThmUtil.cpp:352-464) cut
+// the likely overlap out of the two resident pictures: the box arithmetic and
+// the selection of a connected region's points run on the host (the points
+// are host data), the values are gathered from the device pictures (one
+// 4-byte index up, one 8-byte value down per point); without connected
+// regions the box is one strided copy.
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include <hipcub/hipcub.hpp>
@@ -139,6 +147,14 @@ __global__ void k_th_norm(double *__restrict__ v, int n, const double *__restric
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v[i] = (v[i] - avg) / sd;
 }
 
+// SubI_ThesePoints' value lookup (CThmUtil.cpp:179): out[i] = v[idx[i]].
+__global__ void k_crop_gather(const double *__restrict__ v, const uint32_t *__restrict__ idx, int n,
+                              double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v[idx[i]];
+}
+
 #pragma clang fp contract(on)
 
 namespace {
@@ -224,10 +240,151 @@ int decimate_points(xc_ctx *ctx, const xc_point *p, const double *v, int n, int 
     return XC_OK;
 }
 
+// GetOlapBoxes (1_DMesh/CThmUtil.cpp:352-373): a's origin in b's system scaled by xyconf / scl and
+// truncated, then BoxesFromShifts (Geometry.cpp:150-176) of two ws x hs pictures.  Returns 1 when Crop uses
+// the whole pictures instead (xyconf 0, or a box below max(olap1d, 8): CThmUtil.cpp:393-411), -1 for an
+// offset that is not a finite number below 2^30.
+int crop_boxes(IBox &Ba, IBox &Bb, int ws, int hs, double tx, double ty, double xyconf, double scl, int olap1d)
+{
+    if (xyconf == 0.0) return 1;
+    const double fx = xyconf * tx / scl, fy = xyconf * ty / scl;
+    if (!(fabs(fx) < 1073741824.0) || !(fabs(fy) < 1073741824.0)) return -1;
+    const int x = (int)fx, y = (int)fy;
+    Bb.L = std::max(0, x);
+    Bb.R = std::max(Bb.L, std::min(ws - 1, x + ws - 1));
+    Bb.B = std::max(0, y);
+    Bb.T = std::max(Bb.B, std::min(hs - 1, y + hs - 1));
+    Ba = IBox{Bb.L - x, Bb.R - x, Bb.B - y, Bb.T - y};
+    const int min1d = std::max(olap1d, 8);
+    return (Ba.R - Ba.L + 1 < min1d || Ba.T - Ba.B + 1 < min1d) ? 1 : 0;
+}
+
+// One side of Crop: SubI_ThisBox (CThmUtil.cpp:195-235) when box is set, else SubI_ThesePoints
+// (:163-189).  The kept points (list order) go to op relative to their bounding box's corner, their values
+// are gathered from the device picture dv into ov.  idx / dout: device room for n indexes and values; the
+// indexes and values pass through the context's pinned staging (room for both, values at voff).
+int crop_side(xc_ctx *ctx, const double *dv, int ws, int hs, const xc_point *p, int n, const IBox *box,
+              int64_t olap2d, xc_point *op, double *ov, xc_subimage *S, uint32_t *idx, double *dout, size_t voff)
+{
+    uint32_t *hidx = (uint32_t *)ctx->pinned;
+    double *hval = (double *)((char *)ctx->pinned + voff);
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        const double x = p[i].x, y = p[i].y;
+        if (box && !(x >= box->L && x <= box->R && y >= box->B && y <= box->T)) continue;
+        // (int) truncates toward zero, as the reference's int(P.x): -0.5 is column 0
+        const int ix = fabs(x) < 2147483647.0 ? (int)x : -1, iy = fabs(y) < 2147483647.0 ? (int)y : -1;
+        if (ix < 0 || ix >= ws || iy < 0 || iy >= hs)
+            return set_err(ctx, XC_E_ARG, "Crop: point %d (%g, %g) lies outside the %d x %d picture", i, x, y, ws, hs);
+        op[k] = p[i];
+        hidx[k++] = (uint32_t)ix + (uint32_t)ws * (uint32_t)iy;
+    }
+    if (box && (k <= olap2d || k == 0))
+        return set_err(ctx, XC_TOO_SMALL, "Crop: small intersection %d (required %lld)", k, (long long)olap2d);
+    hipStream_t s = ctx->stream;
+    XC_HIP(ctx, hipMemcpyAsync(idx, hidx, sizeof(uint32_t) * k, hipMemcpyHostToDevice, s));
+    k_crop_gather<<<(k + TH_NT - 1) / TH_NT, TH_NT, 0, s>>>(dv, idx, k, dout);
+    XC_HIP(ctx, hipGetLastError());
+    XC_HIP(ctx, hipMemcpyAsync(hval, dout, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    IBox B;  // while the gather runs
+    bbox_from_points(B, op, k);
+    for (int i = 0; i < k; ++i) {
+        op[i].x -= B.L;
+        op[i].y -= B.B;
+    }
+    S->Ox = B.L;
+    S->Oy = B.B;
+    S->w = B.R - B.L + 1;
+    S->h = B.T - B.B + 1;
+    S->n = k;
+    XC_HIP(ctx, hipStreamSynchronize(s));
+    memcpy(ov, hval, sizeof(double) * k);
+    return XC_OK;
+}
+
 }  // namespace
 }  // namespace xc
 
 using namespace xc;
+
+extern "C" int xc_crop_f64(xc_ctx *ctx, const double *av, const double *bv, int ws, int hs, const xc_point *apts,
+                           int na, const xc_point *bpts, int nb, double tx, double ty, double xyconf, double scl,
+                           int olap1d, int64_t olap2d, xc_point *ap_out, double *av_out, xc_subimage *a,
+                           xc_point *bp_out, double *bv_out, xc_subimage *b, int *whole)
+{
+    if (!ctx) return XC_E_ARG;
+    if (!av || !bv || ws < 1 || hs < 1 || (long long)ws * hs >= 0xffffffffLL || !apts || !bpts || na < 1 ||
+        nb < 1 || !ap_out || !av_out || !a || !bp_out || !bv_out || !b || !whole)
+        return set_err(ctx, XC_E_ARG, "Crop: bad arguments");
+    IBox Ba{}, Bb{};
+    const int wh = crop_boxes(Ba, Bb, ws, hs, tx, ty, xyconf, scl, olap1d);
+    if (wh < 0) return set_err(ctx, XC_E_ARG, "Crop: offset (%g, %g) x %g / %g out of range", tx, ty, xyconf, scl);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    const int n = std::max(na, nb);
+    DevBuf buf;
+    const size_t off = ((sizeof(uint32_t) * n + 255) / 256) * 256;
+    int st = ensure_dev(ctx, buf, off + sizeof(double) * n);
+    if (!st) st = ensure_pinned(ctx, off + sizeof(double) * n);
+    if (st) {
+        (void)hipFree(buf.p);
+        return st;
+    }
+    uint32_t *idx = (uint32_t *)buf.p;
+    double *dout = (double *)((char *)buf.p + off);
+    *whole = wh;
+    st = crop_side(ctx, av, ws, hs, apts, na, wh ? nullptr : &Ba, olap2d, ap_out, av_out, a, idx, dout, off);
+    if (!st)
+        st = crop_side(ctx, bv, ws, hs, bpts, nb, wh ? nullptr : &Bb, olap2d, bp_out, bv_out, b, idx, dout, off);
+    (void)hipFree(buf.p);
+    return st;
+}
+
+extern "C" int xc_crop_nocr_f64(xc_ctx *ctx, const double *av, const double *bv, int ws, int hs, double tx,
+                                double ty, double xyconf, double scl, int olap1d, double *av_out, double *bv_out,
+                                xc_point *p_out, xc_subimage *a, xc_subimage *b, int *whole)
+{
+    if (!ctx) return XC_E_ARG;
+    if (!av || !bv || ws < 1 || hs < 1 || (long long)ws * hs >= 0x7fffffffLL || !av_out || !bv_out || !a || !b ||
+        !whole)
+        return set_err(ctx, XC_E_ARG, "Crop_NoCR: bad arguments");
+    IBox Ba{}, Bb{};
+    const int wh = crop_boxes(Ba, Bb, ws, hs, tx, ty, xyconf, scl, olap1d);
+    if (wh < 0) return set_err(ctx, XC_E_ARG, "Crop_NoCR: offset (%g, %g) x %g / %g out of range", tx, ty, xyconf, scl);
+    if (wh) Ba = Bb = IBox{0, ws - 1, 0, hs - 1};
+    const int ow = Ba.R - Ba.L + 1, oh = Ba.T - Ba.B + 1;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    // the two boxes packed on the device, one contiguous copy into pinned staging, then into the outputs
+    const size_t row = sizeof(double) * ws, wb = sizeof(double) * ow, box = wb * oh;
+    DevBuf buf;
+    int st = ensure_dev(ctx, buf, 2 * box);
+    if (!st) st = ensure_pinned(ctx, 2 * box);
+    if (!st && (hipMemcpy2DAsync(buf.p, wb, av + Ba.L + (size_t)ws * Ba.B, row, wb, oh, hipMemcpyDeviceToDevice,
+                                 s) != hipSuccess ||
+                hipMemcpy2DAsync((char *)buf.p + box, wb, bv + Bb.L + (size_t)ws * Bb.B, row, wb, oh,
+                                 hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipMemcpyAsync(ctx->pinned, buf.p, 2 * box, hipMemcpyDeviceToHost, s) != hipSuccess))
+        st = set_err(ctx, XC_E_DEVICE, "Crop_NoCR: copy failed");
+    if (!st && p_out)  // MakeZeroBasedPoints (CPoint.cpp:82-95) while the copies run
+        for (int y = 0; y < oh; ++y)
+            for (int x = 0; x < ow; ++x) p_out[(size_t)ow * y + x] = xc_point{(double)x, (double)y};
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = set_err(ctx, XC_E_DEVICE, "Crop_NoCR: copy failed");
+    (void)hipFree(buf.p);
+    if (st) return st;
+    memcpy(av_out, ctx->pinned, box);
+    memcpy(bv_out, (char *)ctx->pinned + box, box);
+    a->Ox = Ba.L;
+    a->Oy = Ba.B;
+    b->Ox = Bb.L;
+    b->Oy = Bb.B;
+    a->w = b->w = ow;
+    a->h = b->h = oh;
+    a->n = b->n = (int64_t)ow * oh;
+    *whole = wh;
+    return XC_OK;
+}
 
 extern "C" int xc_decimate_vector_f64(xc_ctx *ctx, xc_point *p, double *v, int n, int w, int h, int lbin, int *n_out)
 {

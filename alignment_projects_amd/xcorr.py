@@ -138,6 +138,12 @@ _OPTIONAL = {
                            C.POINTER(C.c_int)], C.c_int),
     "xc_make_thumb_raster_dev_f64": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p, C.POINTER(C.c_double)], C.c_int),
+    "xc_crop_f64": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, _pp, C.c_int, _pp, C.c_int, C.c_double,
+                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_int64, _pp, _dp, C.POINTER(abi.XcSubimage),
+                     _pp, _dp, C.POINTER(abi.XcSubimage), C.POINTER(C.c_int)], C.c_int),
+    "xc_crop_nocr_f64": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_int, _dp, _dp, _pp, C.POINTER(abi.XcSubimage), C.POINTER(abi.XcSubimage),
+                          C.POINTER(C.c_int)], C.c_int),
     "xc_finish_at_full_res_f32": ([C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_int64,
                                    C.POINTER(abi.XcAngleRec), C.POINTER(C.c_int)], C.c_int),
     "xc_thmpair_create": ([C.c_char_p], C.c_int),
@@ -535,6 +541,34 @@ class Context:
         st = self.L.xc_make_thumb_raster_dev_f64(self.h, ras, dtype, w, h, stride, lbin, out_ptr, C.byref(sd))
         self._check(st, "xc_make_thumb_raster_dev_f64", ok=(abi.XC_OK,))
         return sd.value
+
+    def crop(self, av_ptr, bv_ptr, ws, hs, apts, bpts, tx, ty, xyconf, scl, olap1d, olap2d):
+        """CThmUtil::Crop of two ws x hs FP64 device pictures (pointers) with host point lists.
+        Returns (status, whole, (ap, av, a), (bp, bv, b)); a, b are XcSubimage records."""
+        apts = np.ascontiguousarray(apts, dtype=np.float64).reshape(-1, 2)
+        bpts = np.ascontiguousarray(bpts, dtype=np.float64).reshape(-1, 2)
+        ap, bp = np.zeros_like(apts), np.zeros_like(bpts)
+        avo, bvo = np.zeros(len(apts)), np.zeros(len(bpts))
+        a, b, whole = abi.XcSubimage(), abi.XcSubimage(), C.c_int(-1)
+        st = self.L.xc_crop_f64(self.h, av_ptr, bv_ptr, ws, hs, apts.ctypes.data_as(_pp), len(apts),
+                                bpts.ctypes.data_as(_pp), len(bpts), tx, ty, xyconf, scl, olap1d, olap2d,
+                                ap.ctypes.data_as(_pp), avo.ctypes.data_as(_dp), C.byref(a), bp.ctypes.data_as(_pp),
+                                bvo.ctypes.data_as(_dp), C.byref(b), C.byref(whole))
+        self._check(st, "xc_crop_f64", ok=(abi.XC_OK, abi.XC_TOO_SMALL))
+        return st, whole.value, (ap[:a.n], avo[:a.n], a), (bp[:b.n], bvo[:b.n], b)
+
+    def crop_nocr(self, av_ptr, bv_ptr, ws, hs, tx, ty, xyconf, scl, olap1d, points=True):
+        """CThmUtil::Crop_NoCR of two ws x hs FP64 device pictures. Returns (whole, points or None, (av, a),
+        (bv, b))."""
+        avo, bvo = np.zeros(ws * hs), np.zeros(ws * hs)
+        p = np.zeros((ws * hs, 2)) if points else None
+        a, b, whole = abi.XcSubimage(), abi.XcSubimage(), C.c_int(-1)
+        st = self.L.xc_crop_nocr_f64(self.h, av_ptr, bv_ptr, ws, hs, tx, ty, xyconf, scl, olap1d,
+                                     avo.ctypes.data_as(_dp), bvo.ctypes.data_as(_dp),
+                                     p.ctypes.data_as(_pp) if points else None, C.byref(a), C.byref(b),
+                                     C.byref(whole))
+        self._check(st, "xc_crop_nocr_f64", ok=(abi.XC_OK,))
+        return whole.value, (p[:a.n] if points else None), (avo[:a.n], a), (bvo[:b.n], b)
 
     def finish_at_full_res_tiles(self, best, a, b, Tdfm, Tptwk, olap1d, req_area):
         """FinishAtFullRes over full-resolution resident tiles in FP32. Returns (best, used)."""

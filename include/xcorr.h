@@ -534,6 +534,49 @@ int xc_make_thumbs_f64(xc_ctx *ctx, xc_point *ap, double *av, int na, int aw, in
 int xc_make_thumb_raster_dev_f64(xc_ctx *ctx, const void *ras, int dtype, int w, int h, int stride,
                                  int lbin, double *out, double *sd);
 
+/* One side of the overlap CThmUtil::Crop hands to MakeThumbs (SubI, 1_DMesh/CThmUtil.h:14-20). */
+typedef struct xc_subimage {
+    double  Ox, Oy;  /* S.O: the subimage's corner in the picture */
+    int32_t w, h;    /* S.w, S.h */
+    int64_t n;       /* points (and values) written */
+} xc_subimage;       /* 32 bytes */
+
+/*
+ * CThmUtil::Crop (1_DMesh/CThmUtil.cpp:385-430) with GetOlapBoxes (:352-373),
+ * SubI_ThisBox (:195-235), SubI_ThesePoints (:163-189) and Olap_WholeImage
+ * (:243-258).  av, bv: the two ws x hs FP64 pictures in device memory (the
+ * scaled pictures of xc_pixpair_prep_dev_f64); apts / bpts: the connected
+ * regions' points (host).  (tx, ty): a's origin in b's system (the
+ * translation of Tab); the box offset is int(xyconf tx / scl), int(xyconf
+ * ty / scl).  With xyconf = 0, or an overlap box narrower or lower than
+ * max(olap1d, 8), every point is kept (*whole = 1); otherwise the points
+ * inside each overlap box (*whole = 0).  The kept points, relative to their
+ * bounding box's corner, go to ap_out / bp_out and their picture values,
+ * gathered on the device, to av_out / bv_out (room for na and nb).
+ * XC_TOO_SMALL when a box keeps olap2d points or fewer (the reference's
+ * false; A is tried first and B is then not written); XC_E_ARG for an empty
+ * list or a point whose truncated (x, y) lies outside the picture.
+ */
+int xc_crop_f64(xc_ctx *ctx, const double *av, const double *bv, int ws, int hs,
+                const xc_point *apts, int na, const xc_point *bpts, int nb,
+                double tx, double ty, double xyconf, double scl, int olap1d, int64_t olap2d,
+                xc_point *ap_out, double *av_out, xc_subimage *a,
+                xc_point *bp_out, double *bv_out, xc_subimage *b, int *whole);
+
+/*
+ * CThmUtil::Crop_NoCR (1_DMesh/CThmUtil.cpp:437-464) with
+ * Olap_TheseBoxes_NoCR (:262-312) and Olap_WholeImage_NoCR (:318-346): the
+ * same boxes without connected regions; the values of the overlap box (or of
+ * the whole pictures), row by row, are copied from the device pictures to
+ * av_out / bv_out (room for ws * hs each).  The points are the box's
+ * MakeZeroBasedPoints (CPoint.cpp:82-95), written to p_out when it is not
+ * NULL (room for ws * hs; the same list for both sides).
+ */
+int xc_crop_nocr_f64(xc_ctx *ctx, const double *av, const double *bv, int ws, int hs,
+                     double tx, double ty, double xyconf, double scl, int olap1d,
+                     double *av_out, double *bv_out, xc_point *p_out,
+                     xc_subimage *a, xc_subimage *b, int *whole);
+
 /* ------------------------------------------------------------------ */
 /* PixPair::Load's raster preparation (the pair's pre-path)            */
 /* ------------------------------------------------------------------ */

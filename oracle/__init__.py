@@ -19,7 +19,7 @@ import subprocess
 import numpy as np
 
 from alignment_projects_amd.abi import (XcAngleRec, XcCntRule, XcFourierQa, XcPoint, XcResult, XcRgnRule,
-                                        XcThmParams)
+                                        XcSubimage, XcThmParams)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
@@ -63,6 +63,12 @@ def lib():
         L.xo_boxes_from_shifts.argtypes = [C.c_int] * 6 + [_ip, _ip]
         L.xo_parab_peak_fft.argtypes = [_dp, _dp, C.c_int, _dp, C.c_int, C.c_int]
         L.xo_decimate_vector.argtypes = [_pp, _dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        _sp = C.POINTER(XcSubimage)
+        L.xo_crop.argtypes = [_dp, _dp, C.c_int, C.c_int, _pp, C.c_int, _pp, C.c_int, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_int, C.c_long, _pp, _dp, _sp, _pp, _dp, _sp,
+                              C.POINTER(C.c_int)]
+        L.xo_crop_nocr.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_int, _dp, _dp, _pp, _sp, _sp, C.POINTER(C.c_int)]
         L.xo_set_fft_provider.argtypes = [C.c_int, C.c_char_p]
         L.xo_fft_provider_name.restype = C.c_char_p
         L.xo_rgn_ok.argtypes = [C.POINTER(XcRgnRule), C.c_int, C.c_int]
@@ -306,6 +312,37 @@ def decimate_vector(p, v, w, h, lbin):
     v = np.array(v, dtype=np.float64)
     k = lib().xo_decimate_vector(p.ctypes.data_as(_pp), v.ctypes.data_as(_dp), len(v), w, h, lbin)
     return p[:k].copy(), v[:k].copy()
+
+
+def crop(av, bv, apts, bpts, tx, ty, xyconf, scl, olap1d, olap2d):
+    """CThmUtil::Crop; av, bv: hs x ws pictures.  Returns (status, whole, (ap, av, a), (bp, bv, b))."""
+    av = np.ascontiguousarray(av, dtype=np.float64)
+    bv = np.ascontiguousarray(bv, dtype=np.float64)
+    hs, ws = av.shape
+    apts = np.ascontiguousarray(apts, dtype=np.float64).reshape(-1, 2)
+    bpts = np.ascontiguousarray(bpts, dtype=np.float64).reshape(-1, 2)
+    ap, bp = np.zeros_like(apts), np.zeros_like(bpts)
+    avo, bvo = np.zeros(len(apts)), np.zeros(len(bpts))
+    a, b, whole = XcSubimage(), XcSubimage(), C.c_int(-1)
+    st = lib().xo_crop(av.ctypes.data_as(_dp), bv.ctypes.data_as(_dp), ws, hs, apts.ctypes.data_as(_pp), len(apts),
+                       bpts.ctypes.data_as(_pp), len(bpts), tx, ty, xyconf, scl, olap1d, olap2d,
+                       ap.ctypes.data_as(_pp), avo.ctypes.data_as(_dp), C.byref(a), bp.ctypes.data_as(_pp),
+                       bvo.ctypes.data_as(_dp), C.byref(b), C.byref(whole))
+    return st, whole.value, (ap[:a.n], avo[:a.n], a), (bp[:b.n], bvo[:b.n], b)
+
+
+def crop_nocr(av, bv, tx, ty, xyconf, scl, olap1d):
+    """CThmUtil::Crop_NoCR.  Returns (status, whole, points, (av, a), (bv, b))."""
+    av = np.ascontiguousarray(av, dtype=np.float64)
+    bv = np.ascontiguousarray(bv, dtype=np.float64)
+    hs, ws = av.shape
+    p = np.zeros((ws * hs, 2))
+    avo, bvo = np.zeros(ws * hs), np.zeros(ws * hs)
+    a, b, whole = XcSubimage(), XcSubimage(), C.c_int(-1)
+    st = lib().xo_crop_nocr(av.ctypes.data_as(_dp), bv.ctypes.data_as(_dp), ws, hs, tx, ty, xyconf, scl, olap1d,
+                            avo.ctypes.data_as(_dp), bvo.ctypes.data_as(_dp), p.ctypes.data_as(_pp), C.byref(a),
+                            C.byref(b), C.byref(whole))
+    return st, whole.value, p[:a.n], (avo[:a.n], a), (bvo[:b.n], b)
 
 
 def rgn_ok(rule, olw, olh):

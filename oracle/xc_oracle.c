@@ -1106,6 +1106,108 @@ int xo_decimate_vector(xc_point *p, double *v, int np, int w, int h, int lbin)
     return k;
 }
 
+/* CThmUtil::GetOlapBoxes (1_DMesh/CThmUtil.cpp:352-373): a's origin (tx, ty)
+ * in b's system, scaled by xyconf / scl and truncated; BoxesFromShifts of two
+ * ws x hs pictures.  Returns 1 when the boxes are too thin for Crop (:404-411). */
+static int olap_boxes(ibox *Ba, ibox *Bb, int ws, int hs, double tx, double ty, double xyconf, double scl,
+                      int olap1d)
+{
+    int dx = (int)(xyconf * tx / scl), dy = (int)(xyconf * ty / scl);
+    boxes_from_shifts(Ba, Bb, ws, hs, ws, hs, dx, dy);
+    int ow = Ba->R - Ba->L + 1, oh = Ba->T - Ba->B + 1, min1d = olap1d > 8 ? olap1d : 8;
+    return ow < min1d || oh < min1d;
+}
+
+/* SubI_ThesePoints (CThmUtil.cpp:163-189): values v[int(x) + ws int(y)],
+ * points relative to their bounding box's corner. */
+static void subi_these_points(const double *v, int ws, const xc_point *p, int n, xc_point *op, double *ov,
+                              xc_subimage *S)
+{
+    ibox B;
+    bbox_from_points(&B, p, n);
+    for (int i = 0; i < n; ++i) {
+        ov[i] = v[(int)p[i].x + ws * (int)p[i].y];
+        op[i].x = p[i].x - B.L;
+        op[i].y = p[i].y - B.B;
+    }
+    S->Ox = B.L;
+    S->Oy = B.B;
+    S->w = B.R - B.L + 1;
+    S->h = B.T - B.B + 1;
+    S->n = n;
+}
+
+/* SubI_ThisBox (CThmUtil.cpp:195-235): the points inside the box (in list
+ * order); 0 (the reference's false) when olap2d or fewer remain. */
+static int subi_this_box(const double *v, int ws, const xc_point *p, int n, const ibox *Bx, long olap2d,
+                         xc_point *op, double *ov, xc_subimage *S)
+{
+    xc_point *kept = (xc_point *)malloc(sizeof(xc_point) * (n > 0 ? n : 1));
+    int k = 0;
+    for (int i = 0; i < n; ++i)
+        if (p[i].x >= Bx->L && p[i].x <= Bx->R && p[i].y >= Bx->B && p[i].y <= Bx->T) kept[k++] = p[i];
+    int ok = k > olap2d;
+    if (ok) subi_these_points(v, ws, kept, k, op, ov, S);
+    free(kept);
+    return ok;
+}
+
+/* CThmUtil::Crop (CThmUtil.cpp:385-430) with Olap_WholeImage (:243-258).
+ * Returns XC_OK (*whole = 1 for whole images) or XC_TOO_SMALL (false). */
+int xo_crop(const double *av, const double *bv, int ws, int hs, const xc_point *apts, int na, const xc_point *bpts,
+            int nb, double tx, double ty, double xyconf, double scl, int olap1d, long olap2d, xc_point *ap,
+            double *avo, xc_subimage *a, xc_point *bp, double *bvo, xc_subimage *b, int *whole)
+{
+    ibox Ba, Bb;
+    if (xyconf == 0.0 || olap_boxes(&Ba, &Bb, ws, hs, tx, ty, xyconf, scl, olap1d)) {
+        *whole = 1;
+        subi_these_points(av, ws, apts, na, ap, avo, a);
+        subi_these_points(bv, ws, bpts, nb, bp, bvo, b);
+        return XC_OK;
+    }
+    *whole = 0;
+    if (!subi_this_box(av, ws, apts, na, &Ba, olap2d, ap, avo, a) ||
+        !subi_this_box(bv, ws, bpts, nb, &Bb, olap2d, bp, bvo, b))
+        return XC_TOO_SMALL;
+    return XC_OK;
+}
+
+/* CThmUtil::Crop_NoCR (CThmUtil.cpp:437-464) with Olap_TheseBoxes_NoCR
+ * (:262-312), Olap_WholeImage_NoCR (:318-346) and MakeZeroBasedPoints
+ * (CPoint.cpp:82-95). */
+int xo_crop_nocr(const double *av, const double *bv, int ws, int hs, double tx, double ty, double xyconf, double scl,
+                 int olap1d, double *avo, double *bvo, xc_point *p, xc_subimage *a, xc_subimage *b, int *whole)
+{
+    ibox Ba, Bb;
+    if (xyconf == 0.0 || olap_boxes(&Ba, &Bb, ws, hs, tx, ty, xyconf, scl, olap1d)) {
+        Ba.L = Bb.L = 0;
+        Ba.R = Bb.R = ws - 1;
+        Ba.B = Bb.B = 0;
+        Ba.T = Bb.T = hs - 1;
+        *whole = 1;
+    } else {
+        *whole = 0;
+    }
+    int ow = Ba.R - Ba.L + 1, oh = Ba.T - Ba.B + 1, np = ow * oh;
+    for (int i = 0; i < np; ++i) {
+        int y = i / ow, x = i - ow * y;
+        avo[i] = av[Ba.L + x + ws * (Ba.B + y)];
+        bvo[i] = bv[Bb.L + x + ws * (Bb.B + y)];
+        if (p) {
+            p[i].x = x;
+            p[i].y = y;
+        }
+    }
+    a->Ox = Ba.L;
+    a->Oy = Ba.B;
+    b->Ox = Bb.L;
+    b->Oy = Bb.B;
+    a->w = b->w = ow;
+    a->h = b->h = oh;
+    a->n = b->n = np;
+    return XC_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* CThmScan angle search                                               */
 /* ------------------------------------------------------------------ */

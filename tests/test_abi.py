@@ -36,6 +36,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(abi.XcThmPair) == 6 * 4 + 2 * 8 + 6 * 8
     assert ctypes.sizeof(abi.XcThmParams) == 8 + 8 * 4 + 8 + 12 * 8
     assert ctypes.sizeof(abi.XcFourierQa) == 5 * 8 + 6 * 4
+    assert ctypes.sizeof(abi.XcSubimage) == 32
     assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
 
 
