@@ -253,7 +253,7 @@ def run_strips(args, sync):
 
     line = line_base("tile-pair correlations/sec (batched FP32, 2048x256 strip pairs)", value, "pairs/s", args,
                      ms_step, {"workload": "batched same-layer strip overlaps", "pairs": NP, "tile": f"{W}x{H} u8",
-                               "fft": f"{2 * W}x{2 * H}", "tables": "f64", "rule": f"olw >= {W // 2}, olh >= {H // 2}"})
+                               "fft": f"{2 * W}x{2 * H}", "tables": "f32 (S, S^2) pairs", "rule": f"olw >= {W // 2}, olh >= {H // 2}"})
     line["latency_ms"] = {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["plain_fp64_gpu"] = {"value": round(plain_rate, 1), "unit": "pairs/s", "pairs": nplain,
