@@ -539,8 +539,8 @@ def run_serve(args, sync):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); "
-              "pairs shard independently across GPUs, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); pairs are independent, so a batch "
+              "shards across GPUs by pair with no exchange", file=sys.stderr)
         sys.exit(2)
     sync = _device_sync()
     run = {"strips": run_strips, "sweep": run_sweep, "uneven": run_uneven, "serve": run_serve}[args.workload]

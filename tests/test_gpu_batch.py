@@ -1,6 +1,6 @@
 """Batched FP32 form (tiles + plans) on the GPU against the FP64 oracle.
 
-Bar (BASELINE targets for the batched form): integer peak equal to the FP64 oracle's, subpixel error
+Bar for the batched form: integer peak equal to the FP64 oracle's, subpixel error
 <= 0.05 px, correlation value within 1e-4 absolute; results bitwise identical alone and in a mix.
 """
 import numpy as np
