@@ -47,7 +47,12 @@ void stage_lists(xc_point *dp, double *dv, const xc_point *ip1, const double *iv
     const long long step = (total + nt - 1) / nt;
     for (int t = 1; t < nt; ++t) {
         const long long lo = std::min(total, t * step), hi = std::min(total, lo + step);
-        if (lo < hi) th.emplace_back([&part, &o, t, lo, hi] { part(lo, hi, &o[t]); });
+        if (lo >= hi) continue;
+        try {
+            th.emplace_back([&part, &o, t, lo, hi] { part(lo, hi, &o[t]); });
+        } catch (...) {  // no thread to be had: that part on this one
+            part(lo, hi, &o[t]);
+        }
     }
     part(0, std::min(total, step), &o[0]);
     for (auto &x : th) x.join();
