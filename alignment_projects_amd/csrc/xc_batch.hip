@@ -485,7 +485,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     size_t i_rot = lay.add(sizeof(RotDev) * rots.size() + 16);
     size_t i_dtiles = lay.add(sizeof(TileDev) * rots.size() + 16);
     int nrw = 0;
-    for (const RotDev &R : rots) nrw += (int)(((long long)R.w * R.h + 255) / 256);
+    for (const RotDev &R : rots) nrw += (int)((rot_splat_units(R.w, R.h) + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
     std::vector<int> starts;
@@ -533,7 +533,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         RotDev R = rots[r];
         R.out = at(i_rimg[r]);
         hr[r] = R;
-        for (int b = 0; b < (int)(((long long)R.w * R.h + 255) / 256); ++b) hrw[w++] = make_int2((int)r, b);
+        for (int b = 0; b < (int)((rot_splat_units(R.w, R.h) + 255) / 256); ++b) hrw[w++] = make_int2((int)r, b);
         TileDev T;
         memset(&T, 0, sizeof T);
         T.v.data = R.out;

@@ -899,24 +899,50 @@ __host__ __device__ inline int sat_threads(int w)
 // independent of scheduling.
 // ------------------------------------------------------------------
 
+// Each thread makes two neighbouring pixels of a row, (px, py) and (px+1, py),
+// walking the grid points of both pixels' squares once; each pixel still sums
+// its own contributions in the grid's raster order.
+__host__ __device__ inline long long rot_splat_units(int w, int h) { return (long long)((w + 1) / 2) * h; }
+
+// Bilinear weight of a point whose mapped cell is (ix, iy) with fractions
+// (alpha, beta) on pixel (q, py), DistributePixel's four products; false when
+// the point does not reach the pixel.
+__device__ __forceinline__ bool rot_weight(int ix, int iy, double alpha, double beta, int q, int py, double *w)
+{
+    if (ix == q) {
+        if (iy == py) *w = (1.0 - alpha) * (1.0 - beta);
+        else if (iy + 1 == py) *w = (1.0 - alpha) * beta;
+        else return false;
+    } else if (ix + 1 == q) {
+        if (iy == py) *w = alpha * (1.0 - beta);
+        else if (iy + 1 == py) *w = alpha * beta;
+        else return false;
+    } else {
+        return false;
+    }
+    return true;
+}
+
 #pragma clang fp contract(off)
 template <typename T>
 __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ rots, const int2 *__restrict__ work)
 {
     const int2 wi = work[blockIdx.x];
     const RotDev R = rots[wi.x];
-    const long long pix = (long long)wi.y * 256 + threadIdx.x;
-    if (pix >= (long long)R.w * R.h) return;
-    const int py = (int)(pix / R.w), px = (int)(pix - (long long)py * R.w);
+    const long long unit = (long long)wi.y * 256 + threadIdx.x;
+    if (unit >= rot_splat_units(R.w, R.h)) return;
+    const int wp = (R.w + 1) / 2;
+    const int py = (int)(unit / wp), px = 2 * (int)(unit - (long long)py * wp);
+    const bool two = px + 1 < R.w;
     const double mean = R.src.msd[0], sd = R.src.msd[1];
-    // grid points whose mapped position lies in [px-1, px+1) x [py-1, py+1):
-    // they lie in the inverse image of that square (for a rotation, its
+    // grid points whose mapped position lies in [px-1, px+1+two) x [py-1, py+1):
+    // they lie in the inverse image of that rectangle (for a rotation, its
     // transpose), so in the integer points of its bounding box, taken with a
     // slack of 1e-7 px for the rounding of the corners (~1e-13 px here)
     double gx0 = 1e300, gx1 = -1e300, gy0 = 1e300, gy1 = -1e300;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-        const double cx = (double)(px - 1 + 2 * (c & 1) + R.L), cy = (double)(py - 1 + 2 * (c >> 1) + R.B);
+        const double cx = (double)(px - 1 + (c & 1) * (two ? 3 : 2) + R.L), cy = (double)(py - 1 + 2 * (c >> 1) + R.B);
         const double gx = cx * R.u0 + cy * R.u1, gy = cx * R.u3 + cy * R.u4;
         gx0 = fmin(gx0, gx);
         gx1 = fmax(gx1, gx);
@@ -925,7 +951,7 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
     }
     const int ix0 = imax(0, (int)ceil(gx0 - 1e-7)), ix1 = imin(R.src.w - 1, (int)floor(gx1 + 1e-7));
     const int iy0 = imax(0, (int)ceil(gy0 - 1e-7)), iy1 = imin(R.src.h - 1, (int)floor(gy1 + 1e-7));
-    double acc = 0.0;
+    double acc0 = 0.0, acc1 = 0.0;
     for (int gy = iy0; gy <= iy1; ++gy) {
         for (int gx = ix0; gx <= ix1; ++gx) {
             if (!present(R.src, gx, gy)) continue;
@@ -934,22 +960,18 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
             const double fx = xr - R.L, fy = yr - R.B;
             const int ix = (int)floor(fx), iy = (int)floor(fy);
             const double alpha = fx - ix, beta = fy - iy;
-            double wgt;
-            if (ix == px) {
-                if (iy == py) wgt = (1.0 - alpha) * (1.0 - beta);
-                else if (iy + 1 == py) wgt = (1.0 - alpha) * beta;
-                else continue;
-            } else if (ix + 1 == px) {
-                if (iy == py) wgt = alpha * (1.0 - beta);
-                else if (iy + 1 == py) wgt = alpha * beta;
-                else continue;
-            } else {
-                continue;
-            }
-            acc += wgt * ((load_raw(R.src, gx, gy) - mean) / sd);
+            double w0, w1;
+            const bool h0 = rot_weight(ix, iy, alpha, beta, px, py, &w0);
+            const bool h1 = two && rot_weight(ix, iy, alpha, beta, px + 1, py, &w1);
+            if (!h0 && !h1) continue;
+            const double v = (load_raw(R.src, gx, gy) - mean) / sd;
+            if (h0) acc0 += w0 * v;
+            if (h1) acc1 += w1 * v;
         }
     }
-    reinterpret_cast<T *>(R.out)[pix] = (T)acc;
+    T *out = reinterpret_cast<T *>(R.out) + (long long)py * R.w + px;
+    out[0] = (T)acc0;
+    if (two) out[1] = (T)acc1;
 }
 #pragma clang fp contract(on)
 
