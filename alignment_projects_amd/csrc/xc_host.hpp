@@ -106,6 +106,9 @@ RuleDev make_rule(const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, const uint8_t 
 // additions (xc_splat.hip).  Off-grid lists need splat_scratch_bytes of device scratch.
 bool points_off_grid(const xc_point *p, int n);
 bool copy_points_off_grid(xc_point *dst, const xc_point *p, int n);  // the copy and the test in one pass
+// The staging copy of n points (with the test) and of up to two value lists (da / db may be null), split over a
+// few host threads for large lists; returns whether a point is off the grid.
+bool stage_points(xc_point *dp, const xc_point *p, double *da, const double *a, double *db, const double *b, int n);
 size_t splat_scratch_bytes(int n, long long npix);
 int splat_points(xc_ctx *ctx, hipStream_t s, const xc_point *pts, const double *val, int n, double xmin, double ymin,
                  double *img, int w, int h, bool off_grid, void *scratch, size_t scratch_bytes);

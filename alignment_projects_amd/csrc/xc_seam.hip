@@ -582,16 +582,22 @@ extern "C" int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const doub
     const int maxf = (int)(sqrt((double)(Nx * Nx + Ny * Ny)) / 2.0) + 1;
     std::lock_guard<std::mutex> lock(ctx->mu);
     XC_HIP(ctx, hipSetDevice(ctx->device));
+    // the three lists staged in pinned memory (with the on-grid test of the splat), then uploaded
+    const size_t sp = (sizeof(xc_point) * n + 255) / 256 * 256, sa = (sizeof(double) * n + 255) / 256 * 256;
+    int st = ensure_pinned(ctx, sp + 2 * sa);
+    if (st) return st;
+    xc_point *hp = (xc_point *)ctx->pinned;
+    double *ha = (double *)((char *)ctx->pinned + sp), *hb = (double *)((char *)ctx->pinned + sp + sa);
+    const bool og = stage_points(hp, P, ha, av, hb, bv, n);
     Layout lay;
     const size_t i_p = lay.add(sizeof(xc_point) * n), i_a = lay.add(sizeof(double) * n), i_b = lay.add(sizeof(double) * n);
     const size_t i_i1 = lay.add(sizeof(double) * N2), i_i2 = lay.add(sizeof(double) * N2), i_d = lay.add(sizeof(double) * N2);
     const size_t i_f1 = lay.add(sizeof(cpx<double>) * MS), i_f2 = lay.add(sizeof(cpx<double>) * MS);
     const size_t i_fd = lay.add(sizeof(cpx<double>) * MS), i_part = lay.add(sizeof(double) * 4 * FM_PARTS);
     const size_t i_ps = lay.add(sizeof(double) * maxf), i_out = lay.add(sizeof(double) * 4);
-    const bool og = points_off_grid(P, n);
     const size_t spl = og ? splat_scratch_bytes(n, N2) : 0;
     const size_t i_spl = lay.add(spl ? spl : 1);
-    int st = ensure_dev(ctx, ctx->arena, lay.total);
+    st = ensure_dev(ctx, ctx->arena, lay.total);
     if (st) return st;
     char *base = (char *)ctx->arena.p;
     auto at = [&](size_t i) { return base + lay.off[i]; };
@@ -599,9 +605,9 @@ extern "C" int xc_fourier_match_f64(xc_ctx *ctx, const xc_point *pts, const doub
     cpx<double> *f1 = (cpx<double> *)at(i_f1), *f2 = (cpx<double> *)at(i_f2), *fd = (cpx<double> *)at(i_fd);
     double *part = (double *)at(i_part), *ps = (double *)at(i_ps), *out = (double *)at(i_out);
     hipStream_t s = ctx->stream;
-    XC_HIP(ctx, hipMemcpyAsync(at(i_p), P, sizeof(xc_point) * n, hipMemcpyHostToDevice, s));
-    XC_HIP(ctx, hipMemcpyAsync(at(i_a), av, sizeof(double) * n, hipMemcpyHostToDevice, s));
-    XC_HIP(ctx, hipMemcpyAsync(at(i_b), bv, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_p), hp, sizeof(xc_point) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_a), ha, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    XC_HIP(ctx, hipMemcpyAsync(at(i_b), hb, sizeof(double) * n, hipMemcpyHostToDevice, s));
     XC_HIP(ctx, hipMemsetAsync(i1, 0, sizeof(double) * N2, s));
     XC_HIP(ctx, hipMemsetAsync(i2, 0, sizeof(double) * N2, s));
     XC_HIP(ctx, hipMemsetAsync(ps, 0, sizeof(double) * maxf, s));

@@ -14,8 +14,11 @@
 // order: the reference's value, bit for bit, and the same every run.
 #include <math.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
+#include <thread>
+#include <vector>
 
 #include <hipcub/hipcub.hpp>
 
@@ -92,6 +95,33 @@ bool copy_points_off_grid(xc_point *dst, const xc_point *p, int n)
 }
 
 bool points_off_grid(const xc_point *p, int n) { return copy_points_off_grid(nullptr, p, n); }
+
+bool stage_points(xc_point *dp, const xc_point *p, double *da, const double *a, double *db, const double *b, int n)
+{
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = n < (1 << 17) ? 1 : std::max(1, std::min(8, hw));
+    std::vector<char> og(nt, 0);
+    auto part = [&](int t) {
+        const long long lo = (long long)n * t / nt, hi = (long long)n * (t + 1) / nt;
+        const int c = (int)(hi - lo);
+        og[t] = copy_points_off_grid(dp + lo, p + lo, c);
+        if (da) memcpy(da + lo, a + lo, sizeof(double) * c);
+        if (db) memcpy(db + lo, b + lo, sizeof(double) * c);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) {
+        try {
+            th.emplace_back(part, t);
+        } catch (...) {  // no thread to be had: that part on this one
+            part(t);
+        }
+    }
+    part(0);
+    for (auto &x : th) x.join();
+    bool any = false;
+    for (char c : og) any = any || c;
+    return any;
+}
 
 size_t splat_scratch_bytes(int n, long long npix)
 {
