@@ -1,7 +1,7 @@
 // xc_batch.hip -- device-resident tiles and batched FP32 plans.
 //
-// A tile is one image kept in HBM with its FP64 summed-area tables and its
-// (mean, sd); it plays the role of the reference's fft2 / ThmRec.ftc cache
+// A tile is one image kept in HBM with its summed-area tables (FP64 entries
+// unless the tile asks for FP32 ones) and its (mean, sd); it plays the role of the reference's fft2 / ThmRec.ftc cache
 // (Correlation_fft_fftw.cpp:50-51, CThmScan.h:27-34), made explicit.  A plan
 // is a fixed batch of pair correlations laid out once (descriptors, work
 // lists, spectra, lag rows) and replayed as one HIP graph.
@@ -65,7 +65,8 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     t->dtype = dtype;
     t->flags = flags;
     const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
-    const TileSizes ts = tile_sizes(w, h, false);
+    const bool tab32 = (flags & XC_TILE_F32_TABLES) != 0;
+    const TileSizes ts = tile_sizes(w, h, tab32);
     if (w > TP_MAX_W) {
         delete t;
         return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
@@ -116,6 +117,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.part = (double *)(base + lay.off[iP]);
     D.nbands = 1;  // swept unbanded, as a plan's batched refresh sweeps it
     D.normalize = (flags & XC_TILE_NORMALIZE) ? 1 : 0;
+    D.tab32 = tab32 ? 1 : 0;
     t->d_dev = (TileDev *)(base + lay.off[iD]);
     const int zero = 0;
     if (hipMemcpy(t->d_dev, &D, sizeof D, hipMemcpyHostToDevice) != hipSuccess ||
@@ -156,6 +158,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     sg.cpt = sat_cpt(w);
     sg.threads = sat_threads(w);
     sg.count = 1;
+    sg.tab32 = tab32;
     sg.list = (const int *)(base + lay.off[iL]);
     ex.sat_groups.push_back(sg);
     ex.fp64 = false;
@@ -313,6 +316,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             return set_err(ctx, XC_E_ARG, "pair %d: bad mode", i);
         if (q.rgn.enabled && q.rgn.table)
             return set_err(ctx, XC_E_UNSUPPORTED, "pair %d: tabulated LegalRgn in batched form", i);
+        if (a->dev.tab32 != b->dev.tab32)
+            return set_err(ctx, XC_E_ARG, "pair %d: tiles A and B hold tables of different precision", i);
         for (const xc_tile *t : {a, b})
             if (!tile_idx.count(t)) {
                 tile_idx[t] = (int)tiles.size();
@@ -398,6 +403,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         D.offy = offy;
         D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
         D.need_cnt = D.rule.cnt_enabled;
+        D.tab32 = a->dev.tab32;
         narrow_rows(D);
         D.A = a->dev.v;
         D.B = b->dev.v;
@@ -455,12 +461,16 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     // rotated copies: raster + tables
     std::vector<size_t> i_rimg(rots.size()), i_rS(rots.size()), i_rZ(rots.size());
     std::vector<bool> rot_sq(rots.size(), false);
+    std::vector<char> rots_tab32(rots.size(), 0);  // a rotated copy's tables have its source tile's precision
     for (int i = 0; i < n; ++i)
-        if (pair_rot[i] >= 0 && pairs[i].mode != XC_MODE_S) rot_sq[pair_rot[i]] = true;
+        if (pair_rot[i] >= 0) {
+            if (pairs[i].mode != XC_MODE_S) rot_sq[pair_rot[i]] = true;
+            rots_tab32[pair_rot[i]] = (char)pairs[i].a->dev.tab32;
+        }
     for (size_t r = 0; r < rots.size(); ++r) {
         const size_t px = (size_t)rots[r].w * rots[r].h;
         i_rimg[r] = lay.add(sizeof(float) * px);
-        i_rS[r] = lay.add(rot_sq[r] ? sizeof(float2) * px : 16);
+        i_rS[r] = lay.add(rot_sq[r] ? (rots_tab32[r] ? sizeof(float2) : sizeof(double2)) * px : 16);
         i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
     }
     size_t i_res = lay.add(sizeof(xc_result) * n);
@@ -472,13 +482,17 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::vector<int> sflat, soff, dflat, doff;
     if (refresh) {
         std::vector<int> widths;
-        for (const xc_tile *t : tiles) widths.push_back(t->w);
-        make_sat_groups(widths, sgroups, sflat, soff);
+        std::vector<char> t32;
+        for (const xc_tile *t : tiles) {
+            widths.push_back(t->w);
+            t32.push_back((char)t->dev.tab32);
+        }
+        make_sat_groups(widths, t32, sgroups, sflat, soff);
     }
     {
         std::vector<int> widths;
         for (const RotDev &R : rots) widths.push_back(R.w);
-        make_sat_groups(widths, dgroups, dflat, doff);
+        make_sat_groups(widths, rots_tab32, dgroups, dflat, doff);
     }
     size_t i_slist = lay.add(sizeof(int) * sflat.size() + 16);
     size_t i_dlist = lay.add(sizeof(int) * dflat.size() + 16);
@@ -547,6 +561,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         T.SQ = rot_sq[r] ? (void *)at(i_rS[r]) : nullptr;
         T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
         T.nbands = 1;
+        T.tab32 = rots_tab32[r];
         hd[r] = T;
         rots[r] = R;
     }

@@ -4,7 +4,7 @@
 // same kernels serve the plain one-pair FP64 form and the batched FP32 form):
 //
 //   pack / splat  -> compact A, B rasters (w x h, T)
-//   sat_rows/cols -> FP64 summed-area tables S, S^2 and int32 nz
+//   sat_rows/cols -> summed-area tables S, S^2 (FP64 entries; FP32 on request) and int32 nz
 //   row_r2c       -> row spectra [h][Nx/2+1] of A and B (zero pad implicit)
 //   col_fused     -> per spectrum column: Ny-point forward FFT of A and B,
 //                    P = F_B * conj(F_A) (whitened in S mode), Ny-point
@@ -64,9 +64,10 @@ __device__ __forceinline__ TT box_sum(const TT *__restrict__ t, int w, int L, in
     return r;
 }
 
-// Interleaved (S, S^2) table entries: FP64 pairs for the FP64 form, FP32
-// pairs for the FP32 (batched) form -- each entry is the FP64 sum rounded
-// once; lookups widen it back to FP64 and all box arithmetic is FP64.
+// Interleaved (S, S^2) table entries, by table type TB: FP64 pairs (every
+// form by default, as IntegrateImage keeps double tables), or FP32 pairs for
+// tiles that ask for them -- each entry the FP64 sum rounded once; lookups
+// widen it back to FP64 and all box arithmetic is FP64.
 template <typename FT> struct TabE {
     typedef double2 type;
     static __host__ __device__ __forceinline__ type make(double a, double b) { return make_double2(a, b); }
@@ -599,7 +600,7 @@ __device__ __forceinline__ void band_colsum(const TileDev &D, int band, int y0, 
     }
 }
 
-template <typename T, int CPT, int NT, typename InT, bool MASK>
+template <typename T, typename TB, int CPT, int NT, typename InT, bool MASK>
 __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs, double *xq, int *xz)
 {
     constexpr int NWV = NT / 64;
@@ -681,9 +682,9 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
             for (int j = 0; j < CPT; j += 2) {
                 if (x0 + j < w) {  // w even: x0 + j + 1 < w too
                     if (D.SQ) {
-                        tab_t<T> *sq = reinterpret_cast<tab_t<T> *>(D.SQ);
-                        sq[row + x0 + j] = TabE<T>::make(bs + ps[j], bq + pq[j]);
-                        sq[row + x0 + j + 1] = TabE<T>::make(bs + ps[j + 1], bq + pq[j + 1]);
+                        tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
+                        sq[row + x0 + j] = TabE<TB>::make(bs + ps[j], bq + pq[j]);
+                        sq[row + x0 + j + 1] = TabE<TB>::make(bs + ps[j + 1], bq + pq[j + 1]);
                     }
                     if (D.Z) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(bz + pz[j], bz + pz[j + 1]);
                 }
@@ -692,7 +693,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 if (x0 + j < w) {
-                    if (D.SQ) reinterpret_cast<tab_t<T> *>(D.SQ)[row + x0 + j] = TabE<T>::make(bs + ps[j], bq + pq[j]);
+                    if (D.SQ) reinterpret_cast<tab_t<TB> *>(D.SQ)[row + x0 + j] = TabE<TB>::make(bs + ps[j], bq + pq[j]);
                     if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
                 }
             }
@@ -712,7 +713,7 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
 // error.  WZ: the count table (pixels whose value rounded to T is non-zero,
 // as the general sweep counts them) is scanned and written too.  Integer DPP
 // scans are one instruction per step, against three for an FP64 scan.
-template <typename T, int CPT, int NT, bool MASK, bool WZ>
+template <typename T, typename TB, int CPT, int NT, bool MASK, bool WZ>
 __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 {
     constexpr int NWV = NT / 64;
@@ -724,7 +725,7 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
     const double mean = D.v.msd[0], inv = D.v.msd[2];
     const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
     const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
-    tab_t<T> *sq = reinterpret_cast<tab_t<T> *>(D.SQ);
+    tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
     double as[CPT], aq[CPT], an[CPT];
     int az[CPT];
     constexpr int LA = 4;  // rows loaded ahead of the sweep
@@ -803,15 +804,15 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 #pragma unroll
             for (int j = 0; j < CPT; j += 2)
                 if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
-                    sq[row + x0 + j + 1] = TabE<T>::make(sv[j + 1], qv[j + 1]);
+                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
+                    sq[row + x0 + j + 1] = TabE<TB>::make(sv[j + 1], qv[j + 1]);
                     if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
                 }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j)
                 if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<T>::make(sv[j], qv[j]);
+                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
                     if constexpr (WZ) D.Z[row + x0 + j] = az[j];
                 }
         }
@@ -855,7 +856,7 @@ __global__ void __launch_bounds__(NT) k_tp_colsum(const TileDev *__restrict__ ti
 // wavefront scan plus a cross-wave exchange through LDS (double-buffered by
 // row parity, one barrier per row) turns column sums into the 2-D prefix.
 // Writes S, Q, Z once.
-template <typename T, int CPT, int NT>
+template <typename T, typename TB, int CPT, int NT>
 __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
@@ -866,15 +867,15 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
     const bool m = D.v.mask != nullptr;
     if (D.v.dtype == XC_U8 && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
         if (m) {
-            if (D.Z) sat_sweep_u8<T, CPT, NT, true, true>(D, xi);
-            else sat_sweep_u8<T, CPT, NT, true, false>(D, xi);
+            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, true, true>(D, xi);
+            else sat_sweep_u8<T, TB, CPT, NT, true, false>(D, xi);
         } else {
-            if (D.Z) sat_sweep_u8<T, CPT, NT, false, true>(D, xi);
-            else sat_sweep_u8<T, CPT, NT, false, false>(D, xi);
+            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, false, true>(D, xi);
+            else sat_sweep_u8<T, TB, CPT, NT, false, false>(D, xi);
         }
         return;
     }
-#define SW_CALL(InT, MK) sat_sweep<T, CPT, NT, InT, MK>(D, band, xs, xq, xz)
+#define SW_CALL(InT, MK) sat_sweep<T, TB, CPT, NT, InT, MK>(D, band, xs, xq, xz)
     XC_SAT_DISPATCH(SW_CALL)
 #undef SW_CALL
 }
@@ -1425,7 +1426,7 @@ __device__ __forceinline__ double ncc_r(int olw, int olh, double invN, double ra
 // FP64 (plain form): GetCorr's division and square root in FP64.  FP32
 // (batched form): the sums and both subtractions stay FP64, the final
 // num / sqrt(d) is taken in FP32 (the batched form's precision).
-template <typename T>
+template <typename T, typename TB>
 __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int x, double raw, bool *cand, bool *odd)
 {
     const int BL = x > 0 ? x : 0;
@@ -1433,8 +1434,8 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     const int AL = BL - x, AR = BR - x;
     const int olw = BR - BL + 1;
     const RuleDev &g = P.rule;
-    const double2 sqa = band_sum2<T>(q.sqAt, q.sqAb, AL, AR);
-    const double2 sqb = band_sum2<T>(q.sqBt, q.sqBb, BL, BR);
+    const double2 sqa = band_sum2<TB>(q.sqAt, q.sqAb, AL, AR);
+    const double2 sqb = band_sum2<TB>(q.sqBt, q.sqBb, BL, BR);
     const double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
     bool c = true;
     if (g.exact_w > 0 && olw != g.exact_w) c = false;
@@ -1482,7 +1483,7 @@ __device__ __forceinline__ int ld_z(const int32_t *p, int i)
 // w2 - 1 over the range (x >= w2 - w1), else A's is the constant w1 - 1; the
 // constant corner's table entries are loaded once.  The sums are formed in the
 // same order either way.
-template <typename T, int TPF, bool AB, bool BB, bool CNT, bool NEG, bool BC>
+template <typename T, typename TB, int TPF, bool AB, bool BB, bool CNT, bool NEG, bool BC>
 __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q, const T *R, int t, int NX, int xlo,
                                                int xhi, double &bestv, long long &bestk, int &flags)
 {
@@ -1491,8 +1492,8 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
     const double invN = q.invN;
     const long long k1 = P.rule.k1, k2 = P.rule.k2;
     const int KC = BC ? w2 - 1 : w1 - 1;  // the constant right edge
-    const double2 ct = ld_sq<T>(BC ? Bt : At, KC);
-    const double2 cb = (BC ? BB : AB) ? ld_sq<T>(BC ? Bb : Ab, KC) : make_double2(0.0, 0.0);
+    const double2 ct = ld_sq<TB>(BC ? Bt : At, KC);
+    const double2 cb = (BC ? BB : AB) ? ld_sq<TB>(BC ? Bb : Ab, KC) : make_double2(0.0, 0.0);
     for (int x = xlo + t; x <= xhi; x += TPF) {
         const int BR = BC ? w2 - 1 : x + w1 - 1, AR = BR - x;
         bool cand = true;
@@ -1514,18 +1515,18 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
                 }
                 cand = (long long)c1 > k1 && (long long)c2 > k2;
             }
-            a = BC ? ld_sq<T>(At, AR) : ct;
+            a = BC ? ld_sq<TB>(At, AR) : ct;
             if constexpr (AB) {
-                const double2 u = BC ? ld_sq<T>(Ab, AR) : cb;
+                const double2 u = BC ? ld_sq<TB>(Ab, AR) : cb;
                 a.x -= u.x;
                 a.y -= u.y;
             }
-            b = BC ? ct : ld_sq<T>(Bt, BR);
-            const double2 bl = ld_sq<T>(Bt, BL1);
+            b = BC ? ct : ld_sq<TB>(Bt, BR);
+            const double2 bl = ld_sq<TB>(Bt, BL1);
             b.x -= l ? bl.x : 0.0;
             b.y -= l ? bl.y : 0.0;
             if constexpr (BB) {
-                const double2 u = BC ? cb : ld_sq<T>(Bb, BR), z = ld_sq<T>(Bb, BL1);
+                const double2 u = BC ? cb : ld_sq<TB>(Bb, BR), z = ld_sq<TB>(Bb, BL1);
                 b.x -= u.x;
                 b.y -= u.y;
                 b.x += l ? z.x : 0.0;
@@ -1543,20 +1544,20 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
                 if constexpr (BB) c2 -= ld_z(q.zBb, BR);
                 cand = (long long)c1 > k1 && (long long)c2 > k2;
             }
-            a = BC ? ld_sq<T>(At, AR) : ct;
-            const double2 al = ld_sq<T>(At, AL1);
+            a = BC ? ld_sq<TB>(At, AR) : ct;
+            const double2 al = ld_sq<TB>(At, AL1);
             a.x -= al.x;
             a.y -= al.y;
             if constexpr (AB) {
-                const double2 u = BC ? ld_sq<T>(Ab, AR) : cb, z = ld_sq<T>(Ab, AL1);
+                const double2 u = BC ? ld_sq<TB>(Ab, AR) : cb, z = ld_sq<TB>(Ab, AL1);
                 a.x -= u.x;
                 a.y -= u.y;
                 a.x += z.x;
                 a.y += z.y;
             }
-            b = BC ? ct : ld_sq<T>(Bt, BR);
+            b = BC ? ct : ld_sq<TB>(Bt, BR);
             if constexpr (BB) {
-                const double2 u = BC ? cb : ld_sq<T>(Bb, BR);
+                const double2 u = BC ? cb : ld_sq<TB>(Bb, BR);
                 b.x -= u.x;
                 b.y -= u.y;
             }
@@ -1576,50 +1577,50 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
 // A row's legal interval in raster order: x >= 0 first, then x < 0; each thread's
 // keys ascend, so a strict > keeps the first of equal values.  Each sign splits at
 // x = w2 - w1, where the constant right edge passes from A to B.
-template <typename T, int TPF, bool AB, bool BB, bool CNT>
+template <typename T, typename TB, int TPF, bool AB, bool BB, bool CNT>
 __device__ __forceinline__ void ncc_scan_rows(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
                                               long long &bestk, int &flags)
 {
     const int sw = P.w2 - P.w1;
     const int p0 = q.x0 > 0 ? q.x0 : 0, p1 = q.x1;        // x >= 0
     const int n0 = q.x0, n1 = q.x1 < -1 ? q.x1 : -1;     // x < 0
-    ncc_scan_range<T, TPF, AB, BB, CNT, false, false>(P, q, R, t, NX, p0, p1 < sw - 1 ? p1 : sw - 1, bestv, bestk,
+    ncc_scan_range<T, TB, TPF, AB, BB, CNT, false, false>(P, q, R, t, NX, p0, p1 < sw - 1 ? p1 : sw - 1, bestv, bestk,
                                                       flags);
-    ncc_scan_range<T, TPF, AB, BB, CNT, false, true>(P, q, R, t, NX, p0 > sw ? p0 : sw, p1, bestv, bestk, flags);
-    ncc_scan_range<T, TPF, AB, BB, CNT, true, false>(P, q, R, t, NX, n0, n1 < sw ? n1 : sw, bestv, bestk, flags);
-    ncc_scan_range<T, TPF, AB, BB, CNT, true, true>(P, q, R, t, NX, n0 > sw + 1 ? n0 : sw + 1, n1, bestv, bestk,
+    ncc_scan_range<T, TB, TPF, AB, BB, CNT, false, true>(P, q, R, t, NX, p0 > sw ? p0 : sw, p1, bestv, bestk, flags);
+    ncc_scan_range<T, TB, TPF, AB, BB, CNT, true, false>(P, q, R, t, NX, n0, n1 < sw ? n1 : sw, bestv, bestk, flags);
+    ncc_scan_range<T, TB, TPF, AB, BB, CNT, true, true>(P, q, R, t, NX, n0 > sw + 1 ? n0 : sw + 1, n1, bestv, bestk,
                                                     flags);
 }
 #pragma clang fp contract(on)
 
-template <typename T, int TPF, bool CNT>
+template <typename T, typename TB, int TPF, bool CNT>
 __device__ __forceinline__ void ncc_scan_fast(const PairDev &P, const NccRow &q, const T *R, int t, int NX, double &bestv,
                                               long long &bestk, int &flags)
 {
     if (q.sqAb) {
-        if (q.sqBb) ncc_scan_rows<T, TPF, true, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
-        else ncc_scan_rows<T, TPF, true, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        if (q.sqBb) ncc_scan_rows<T, TB, TPF, true, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TB, TPF, true, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
     } else {
-        if (q.sqBb) ncc_scan_rows<T, TPF, false, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
-        else ncc_scan_rows<T, TPF, false, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        if (q.sqBb) ncc_scan_rows<T, TB, TPF, false, true, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        else ncc_scan_rows<T, TB, TPF, false, false, CNT>(P, q, R, t, NX, bestv, bestk, flags);
     }
 }
 
 // CorrPatches' R-image value at lag (x, y): the NCC of a candidate lag, 0
 // for any other (outside the row's legal interval or failing a per-lag
 // rule), as the scan leaves it.
-template <typename T>
+template <typename T, typename TB>
 __device__ __forceinline__ double ncc_value(const PairDev &P, int x, int y, double raw)
 {
-    const NccRow q = ncc_row<T>(P, y);
+    const NccRow q = ncc_row<TB>(P, y);
     if (x < q.x0 || x > q.x1) return 0.0;
     bool cand, odd;
-    return ncc_lag<T>(P, q, x, raw, &cand, &odd);
+    return ncc_lag<T, TB>(P, q, x, raw, &cand, &odd);
 }
 
 // CNT: the variant for launches with count-rule NCC pairs (scans the count
 // tables in the fast NCC path); without it such rows take the general path.
-template <typename T, int NH, bool CNT>
+template <typename T, int NH, bool CNT, typename TB>
 __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     k_row_c2r_eval(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw)
 {
@@ -1680,16 +1681,16 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     if (valid && P.mode == PM_NCC) {
         // raster order: x >= 0 first (ix = x), then x < 0 (ix = Nx + x); each thread's
         // keys ascend, so a strict > keeps the first of equal values
-        const NccRow q = ncc_row<T>(P, y);
+        const NccRow q = ncc_row<TB>(P, y);
         const int segs[2][2] = {{q.x0 > 0 ? q.x0 : 0, q.x1}, {q.x0, q.x1 < -1 ? q.x1 : -1}};
         const bool fast = P.rule.exact_w <= 0 && !P.rule.table;
-        if (fast && !P.rule.cnt_enabled) ncc_scan_fast<T, TPF, false>(P, q, R, t, NX, bestv, bestk, flags);
-        else if (CNT && fast && q.zAt && q.zBt) ncc_scan_fast<T, TPF, CNT>(P, q, R, t, NX, bestv, bestk, flags);
+        if (fast && !P.rule.cnt_enabled) ncc_scan_fast<T, TB, TPF, false>(P, q, R, t, NX, bestv, bestk, flags);
+        else if (CNT && fast && q.zAt && q.zBt) ncc_scan_fast<T, TB, TPF, CNT>(P, q, R, t, NX, bestv, bestk, flags);
         else for (int sgi = 0; sgi < 2; ++sgi) {
             for (int x = segs[sgi][0] + t; x <= segs[sgi][1]; x += TPF) {
                 const int ix = x >= 0 ? x : NX + x;
                 bool cand, odd;
-                const double val = ncc_lag<T>(P, q, x, (double)R[ix], &cand, &odd);
+                const double val = ncc_lag<T, TB>(P, q, x, (double)R[ix], &cand, &odd);
                 if (cand) {
                     flags |= odd ? 3 : 1;
                     if (val > bestv) {
@@ -1702,7 +1703,7 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     } else if (valid) {
         for (int ix = t; ix < NX; ix += TPF) {
             bool cand, win, odd;
-            const double val = lag_value<T>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+            const double val = lag_value<TB>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
             if (odd) flags |= 2;
             if (win) {
                 ssum += val;
@@ -1785,10 +1786,10 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
                     if (P.mode == PM_NCC) {
                         bool dead;
                         const int x = lag_x_of_raster(P, ix, &dead);
-                        if (!dead) val = ncc_value<T>(P, x, y, (double)R[ix]);
+                        if (!dead) val = ncc_value<T, TB>(P, x, y, (double)R[ix]);
                     } else {
                         bool cand, win, odd;
-                        val = lag_value<T>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
+                        val = lag_value<TB>(P, rb, y, ix, (double)R[ix], &cand, &win, &odd);
                     }
                 }
                 if (s) r.right = val;
@@ -1842,7 +1843,7 @@ __device__ double point_c2r(const PairDev &P, int yl, int ix, const cpx<T> *__re
     return r;
 }
 
-template <typename T>
+template <typename T, typename TB>
 __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pairs, const cpx<T> *__restrict__ tw)
 {
     __shared__ double sv[256], ss[256], sq[256];
@@ -1959,7 +1960,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
                     if (ylr >= 0 && row_live(P, ylr)) {
                         const int yy = lag_y_of_row(P, ylr);
                         const double raw = point_c2r<T>(P, ylr, x, tw, sv);
-                        val = ncc_value<T>(P, x, yy, raw);  // 0 outside the disc and the rules
+                        val = ncc_value<T, TB>(P, x, yy, raw);  // 0 outside the disc and the rules
                     }
                     zz[s] = val;
                 }
@@ -1989,7 +1990,7 @@ __global__ void __launch_bounds__(256) k_finalize(const PairDev *__restrict__ pa
                     const double raw = point_c2r<T>(P, ylr, ix, tw, sv);
                     bool legal, odd;
                     const RowBox rb = row_box(P, yy);
-                    zz[s] = eval_lag<T>(P, rb, x, raw, &legal, &odd);
+                    zz[s] = eval_lag<TB>(P, rb, x, raw, &legal, &odd);
                 }
                 if (zz[0] > 0.0 && zz[1] > 0.0) {
                     const double d = 1.0 * (zz[0] - zz[1]) / (2 * (zz[0] + zz[1] - z1 - z1));

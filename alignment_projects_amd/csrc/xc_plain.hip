@@ -120,7 +120,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     const int nr2c = (h1 + GR - 1) / GR + (h2 + GR - 1) / GR;
     const int ncol = (H + C - 1) / C;
     const int nc2r = (P.hR + G - 1) / G;
-    const TileSizes t1 = tile_sizes(w1, h1, true), t2 = tile_sizes(w2, h2, true);
+    const TileSizes t1 = tile_sizes(w1, h1, false), t2 = tile_sizes(w2, h2, false);
     Layout lay;
     size_t i_pts = lay.add(sizeof(xc_point) * n), i_vals = lay.add(sizeof(double) * n);
     size_t i_iA = lay.add(sizeof(double) * w1 * h1), i_iB = lay.add(sizeof(double) * w2 * h2);
@@ -216,7 +216,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     if (tbl) memcpy(h2p(i_tbl), rgn->table, tbl);
     std::vector<SatGroup> sgroups;
     std::vector<int> sflat, soff;
-    make_sat_groups({w1, w2}, sgroups, sflat, soff);
+    make_sat_groups({w1, w2}, {}, sgroups, sflat, soff);
     for (SatGroup &g : sgroups) g.nb = std::max(sat_bands(h1), sat_bands(h2));
     memcpy(h2p(i_slist), sflat.data(), sizeof(int) * sflat.size());
     for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];

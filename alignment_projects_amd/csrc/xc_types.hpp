@@ -58,13 +58,15 @@ struct ImgView {
 struct TileDev {
     ImgView v;
     double *msd;         // out: (mean, sd, 1/sd)
-    void *SQ;            // out: summed-area tables of x and x^2, interleaved per pixel, w x h (FP64 pairs
-                         //      for the FP64 form, FP32 pairs for the batched FP32 form)
+    void *SQ;            // out: summed-area tables of x and x^2, interleaved per pixel, w x h (FP64 pairs,
+                         //      or FP32 pairs when tab32 is set)
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
     double *bandp;       // scratch: column sums of the table sweep's row bands (nbands > 1)
     int32_t normalize;
     int32_t nbands;      // row bands of the table sweep (1: one workgroup walks every row)
+    int32_t tab32;       // SQ holds FP32 (S, S^2) pairs (else FP64 pairs)
+    int32_t pad;
 };
 
 // One rotated copy of an image's pixel grid (CThmScan::RFromAngle,
@@ -99,11 +101,12 @@ struct PairDev {
     int32_t need_cnt;        // count predicate active (nz tables needed)
     int32_t filter;          // PM_P2I: Gaussian low-pass of the cross power (bFilter)
     int32_t ylo, yhi;        // signed lag rows computed; rows outside hold no legal lag (all 0, no candidate)
+    int32_t tab32;           // SQA / SQB hold FP32 (S, S^2) pairs (else FP64 pairs)
     double norm;             // PM_P2I: np1 * N^2
     double mincor;
     RuleDev rule;
     ImgView A, B;                     // source images
-    const void *SQA, *SQB;             // summed-area tables (S, S^2) interleaved, entries of the pair's precision
+    const void *SQA, *SQB;             // summed-area tables (S, S^2) interleaved, FP64 pairs (FP32 pairs if tab32)
     const int32_t *ZA, *ZB;           // non-zero count tables
     void *specA, *specB;              // cpx<T> [h][Nx/2+1] row spectra
     void *lag;                        // cpx<T> [hR][Nx/2+1] lag rows

@@ -268,9 +268,13 @@ class Context:
 
     # ---- tiles and batched plans ----
 
-    def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0):
-        """Upload (or adopt, with device_ptr) one image as a device-resident tile."""
+    def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0, tables32=False):
+        """Upload (or adopt, with device_ptr) one image as a device-resident tile.
+
+        tables32: keep the summed-area tables as FP32 (S, S^2) entries instead of FP64 ones."""
         flags = abi.XC_TILE_NORMALIZE if normalize else 0
+        if tables32:
+            flags |= abi.XC_TILE_F32_TABLES
         if device_ptr is not None:
             flags |= abi.XC_TILE_DEVICE_SRC
             h, w = img

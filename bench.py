@@ -5,7 +5,8 @@ Workloads (--workload):
           per-pair seeds, fractional shifts by bilinear resampling, N(0,10)
           noise), CorrPatches semantics (normalised cross-correlation, FP32
           FFTs at 4096 x 512, FP64 summed-area tables and variance terms,
-          legal overlap olw >= W/2, olh >= H/2).  One step = one pass of the
+          legal overlap olw >= W/2, olh >= H/2).  The same batch with FP32
+          table entries is timed beside it and reported as a variant.  One step = one pass of the
           whole path over the batch, starting from the raw uint8 strips
           resident in HBM: statistics, summed-area tables, row r2c, fused
           column pass, row c2r + NCC + peak search, subpixel fit.
@@ -77,6 +78,7 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=192, help="pairs timed on the CPU baseline")
     ap.add_argument("--plain-sample", type=int, default=192, help="pairs run through the plain FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-variant", action="store_true", help="skip the FP32-table variant of the strips")
     ap.add_argument("--stage-iters", type=int, default=5)
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="duration of each offered load")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -184,6 +186,20 @@ def run_strips(args, sync):
         log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
         del src, dst
     ctx = xcorr.Context(0)
+    variant = None
+    if not args.no_variant:
+        # FP32 (S, S^2) table entries: half the table bytes; reported beside the headline, never as it
+        vt = [(ctx.tile_put(A[i], tables32=True), ctx.tile_put(B[i], tables32=True)) for i in range(NP)]
+        vp = ctx.plan([xcorr.make_pair(ta, tb, rgn=rgn) for ta, tb in vt], refresh_tiles=True, autotune=True)
+        ve, _ = time_plan(vp, args.steps, args.warmup, sync)
+        vres = vp.run(want=True)
+        variant = {"tables": "f32 (S, S^2) pairs", "value": round(NP * args.steps / ve, 1), "unit": "pairs/s",
+                   "ms_per_step": round(1e3 * ve / args.steps, 3)}
+        vp.destroy()
+        for ta, tb in vt:
+            ta.release()
+            tb.release()
+        log(f"variant with FP32 tables: {variant['value']} pairs/s")
     tiles = [(ctx.tile_put(A[i]), ctx.tile_put(B[i])) for i in range(NP)]
     pairs = [xcorr.make_pair(ta, tb, rgn=rgn) for ta, tb in tiles]
     plan = ctx.plan(pairs, refresh_tiles=True, autotune=True)
@@ -253,7 +269,8 @@ def run_strips(args, sync):
 
     line = line_base("tile-pair correlations/sec (batched FP32, 2048x256 strip pairs)", value, "pairs/s", args,
                      ms_step, {"workload": "batched same-layer strip overlaps", "pairs": NP, "tile": f"{W}x{H} u8",
-                               "fft": f"{2 * W}x{2 * H}", "tables": "f32 (S, S^2) pairs", "rule": f"olw >= {W // 2}, olh >= {H // 2}"})
+                               "fft": f"{2 * W}x{2 * H}", "tables": "f64 (S, S^2) pairs",
+                               "rule": f"olw >= {W // 2}, olh >= {H // 2}"})
     line["latency_ms"] = {"p50": round(1e3 * float(np.percentile(lat, 50)), 3),
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["plain_fp64_gpu"] = {"value": round(plain_rate, 1), "unit": "pairs/s", "pairs": nplain,
@@ -264,6 +281,9 @@ def run_strips(args, sync):
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
     line["pcie_inclusive"] = pcie
+    if variant:
+        variant["accuracy_vs_plain_fp64_gpu"] = accuracy(vres[:nplain], plain)
+        line["variant_f32_tables"] = variant
     if cpu:
         line["batched_over_cpu"] = round(value / cpu["value"], 1)
     plan.destroy()

@@ -208,15 +208,21 @@ typedef struct xc_tile xc_tile;
 enum xc_dtype { XC_U8 = 0, XC_U16 = 1, XC_F32 = 2, XC_F64 = 3 };
 
 enum xc_tile_flags {
-    XC_TILE_NORMALIZE = 1,  /* Normalize() the valid pixels: mean 0, sd 1 (n-1), Maths.cpp:192-212 */
-    XC_TILE_DEVICE_SRC = 2  /* `data`/`mask` are device pointers (else host) */
+    XC_TILE_NORMALIZE = 1,   /* Normalize() the valid pixels: mean 0, sd 1 (n-1), Maths.cpp:192-212 */
+    XC_TILE_DEVICE_SRC = 2,  /* `data`/`mask` are device pointers (else host) */
+    XC_TILE_F32_TABLES = 4   /* keep the summed-area tables as FP32 (S, S^2) entries (each the FP64 sum
+                                rounded once) instead of FP64 ones: half the table bytes, for callers that
+                                accept the rounding; every pair of a batch reads tables of one precision */
 };
 
 /*
  * Upload (or adopt from device memory) one w x h image with optional
  * validity mask (non-zero = pixel present in the point list; absent pixels
  * are 0 in the raster, as ImageFromValuesAndPoints leaves them).  The tile
- * keeps FP32 values and its FP64 summed-area tables resident in HBM.
+ * keeps its pixels, (mean, sd) and its summed-area tables resident in HBM:
+ * S and S^2 as interleaved FP64 pairs (IntegrateImage's double tables,
+ * Correlation.cpp:49-105), or FP32 pairs with XC_TILE_F32_TABLES, plus the
+ * int32 non-zero count table.
  */
 int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int h, int stride,
                 const uint8_t *mask, int flags, xc_tile **out);
@@ -248,7 +254,9 @@ typedef struct xc_pair {
 
 /*
  * Correlate n pairs in FP32 (FFTs, multiply) with FP64 summed-area tables
- * and FP64 variance terms.  Blocks until results[0..n) are written.
+ * (FP32 ones for tiles made with XC_TILE_F32_TABLES) and FP64 variance terms;
+ * the final num / sqrt(d) of GetCorr is taken in FP32.  Blocks until
+ * results[0..n) are written.
  */
 int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *results);
 
@@ -293,7 +301,8 @@ int xc_plan_run(xc_plan *plan, xc_result *results);
  * stage, bytes[XC_STAGE_COUNT] its algorithmic HBM bytes per run. */
 int xc_plan_stage_times(xc_plan *plan, int iters, double *ms, double *bytes);
 /* Algorithmic HBM bytes of one run of the whole plan (inputs once; row
- * spectra, lag rows and FP64 tables written once and read once). */
+ * spectra, lag rows and the tables, 16 B per FP64 (S, S^2) entry or 8 B per
+ * FP32 one, written once and read once). */
 double xc_plan_model_bytes(const xc_plan *plan);
 /* Copies up to cap bucket records; returns the plan's bucket count. */
 int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);

@@ -1,0 +1,114 @@
+"""The batched FP32 form at the strip workload's full size: 2048 x 256 u8 strips, FFT 4096 x 512
+(two 2048-point rows per thread in the row r2c, 512-point column pass), FP64 summed-area tables.
+
+Bar (the batched form's): integer peak equal to the reference path's, subpixel error <= 0.05 px,
+correlation value within 1e-4 absolute.  The reference path is the FP64 oracle on a sample and the
+plain FP64 GPU path (itself bit-exact in its integer peak and within 1e-9 in r against the oracle)
+on the whole 256-pair batch.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from alignment_projects_amd import abi, synth
+from alignment_projects_amd.xcorr import make_pair
+
+pytestmark = pytest.mark.gpu
+
+SUBPX = 0.05
+RTOL32 = 1e-4
+W, H = 2048, 256
+
+
+def _norm_points(img):
+    p, v = synth.raster_points(img)
+    return p, synth.normalize(v)
+
+
+def _check(r, o, what):
+    assert r.status == o[0] == abi.XC_OK, (what, r.status, o[0])
+    assert (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y), (what, r.as_dict(), o[4].as_dict())
+    assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX, (what, r.dx, r.dy, o[2], o[3])
+    assert abs(r.r - o[1]) <= RTOL32, (what, r.r, o[1])
+
+
+@pytest.fixture(scope="module")
+def strips():
+    print("generating 256 strip pairs 2048 x 256 (seeds 1000..1255)", flush=True)
+    return synth.strip_batch(256, seed0=1000, w=W, h=H)
+
+
+def test_strips_full_size_vs_oracle(ctx, strips):
+    """16 full strips (the bench's first 16) through a tuned plan against the FP64 oracle."""
+    A, B, S = strips
+    n = 16
+    rgn = abi.rgn_min(W // 2, H // 2)
+    tiles = [(ctx.tile_put(A[i]), ctx.tile_put(B[i])) for i in range(n)]
+    plan = ctx.plan([make_pair(ta, tb, rgn=rgn) for ta, tb in tiles], refresh_tiles=True, autotune=True)
+    assert [(b["nh"], b["ny"]) for b in plan.buckets()] == [(2048, 512)]
+    res = plan.run()
+    res2 = plan.run()
+    for i in range(n):
+        pa, va = _norm_points(A[i])
+        pb, vb = _norm_points(B[i])
+        o = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+        _check(res[i], o, i)
+        assert res2[i].r == res[i].r and res2[i].dx == res[i].dx and res2[i].dy == res[i].dy
+        # the found shift is the planted one (0.3 px fractional parts, applied by bilinear resampling)
+        assert abs(res[i].lag_x - S[i][0]) <= 1 and abs(res[i].lag_y - S[i][1]) <= 1, (i, res[i].as_dict(), S[i])
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
+
+
+def test_strips_full_batch_vs_plain_fp64(ctx, strips):
+    """The whole 256-pair batch against the plain FP64 GPU path, pair by pair; the FP32-table variant
+    is run on the same batch and must find the same integer peaks."""
+    A, B, _ = strips
+    n = len(A)
+    rgn = abi.rgn_min(W // 2, H // 2)
+    out = {}
+    for t32 in (False, True):
+        tiles = [(ctx.tile_put(A[i], tables32=t32), ctx.tile_put(B[i], tables32=t32)) for i in range(n)]
+        plan = ctx.plan([make_pair(ta, tb, rgn=rgn) for ta, tb in tiles], refresh_tiles=True)
+        out[t32] = plan.run()
+        plan.destroy()
+        for ta, tb in tiles:
+            ta.release()
+            tb.release()
+    rerr = []
+    for i in range(n):
+        pa, va = _norm_points(A[i])
+        pb, vb = _norm_points(B[i])
+        g = ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+        _check(out[False][i], g, i)
+        assert (out[True][i].lag_x, out[True][i].lag_y) == (g[4].lag_x, g[4].lag_y), i
+        rerr.append(abs(out[False][i].r - g[1]))
+    print(f"256 strips vs plain FP64: r error max {max(rerr):.3g}", flush=True)
+
+
+def test_strips_far_corner_small_overlaps(ctx, strips):
+    """Peaks confined by a disc to small overlaps at the far corner of B's tables (|lag| near the strip
+    size, boxes of ~40 x 40 px whose table entries are the largest sums of the tile): the FP64 tables
+    keep r within the batched bar there."""
+    A, B, _ = strips
+    rgn = abi.rgn_min(16, 16)
+    cases = [(W - 40, H - 40, 6), (-(W - 40), -(H - 40), 6), (W - 60, -(H - 50), 8), (-(W - 30), H - 30, 5)]
+    for k, (ox, oy, rad) in enumerate(cases):
+        i = 20 + k
+        ta, tb = ctx.tile_put(A[i]), ctx.tile_put(B[i])
+        r = ctx.batch_corr([make_pair(ta, tb, rgn=rgn, Ox=ox, Oy=oy, radius=rad)])[0]
+        pa, va = _norm_points(A[i])
+        pb, vb = _norm_points(B[i])
+        o = oracle.corr_patches(pa, va, pb, vb, ox, oy, rad, rgn=rgn)
+        assert r.status == o[0] == abi.XC_OK, (k, r.status, o[0])
+        if (r.lag_x, r.lag_y) != (o[4].lag_x, o[4].lag_y):
+            # a near tie decided by the FP32 transforms' rounding: the batched lag's FP64 value is the oracle's best
+            g = ctx.corr_patches(pa, va, pb, vb, r.lag_x, r.lag_y, 0, rgn=rgn)
+            assert abs(g[1] - o[1]) <= 2 * RTOL32, (k, r.as_dict(), o[4].as_dict(), g[1])
+        else:
+            assert abs(r.r - o[1]) <= RTOL32, (k, r.r, o[1])
+            assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX, (k, r.dx, r.dy, o[2], o[3])
+        ta.release()
+        tb.release()
