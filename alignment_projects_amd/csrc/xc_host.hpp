@@ -113,6 +113,24 @@ size_t splat_scratch_bytes(int n, long long npix);
 int splat_points(xc_ctx *ctx, hipStream_t s, const xc_point *pts, const double *val, int n, double xmin, double ymin,
                  double *img, int w, int h, bool off_grid, void *scratch, size_t scratch_bytes);
 
+// A device workspace a call must fit in (the fixed scratch of the
+// allocation-free entry point).
+struct FixedWs {
+    void *dev = nullptr;
+    size_t bytes = 0;
+};
+
+// One plain FP64 correlation of point lists (xc_plain.hip); mode PM_NCC / PM_R / PM_S, fmode: CorrImagesF's
+// peak selection; fixed: run in that workspace and allocate nothing (null: the context's arena and staging).
+int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int n1, const xc_point *ip2,
+              const double *iv2, int n2, int Ox, int Oy, int radius, int Rx, int Ry, double mincor,
+              const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out, bool fmode, double nbmaxht,
+              const FixedWs *fixed);
+
+// Looks up the device address of the allocation-free entry point's fixed
+// scratch on ctx->device (current), once per device (xc_noalloc.hip).
+int noalloc_init(xc_ctx *ctx);
+
 // Fill the derived geometry of a PairDev from (w1,h1,w2,h2) and the mode.
 int setup_pair_geometry(PairDev &P, int w1, int h1, int w2, int h2, int mode);
 

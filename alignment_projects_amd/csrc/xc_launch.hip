@@ -281,6 +281,11 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
         xc_ctx_destroy(ctx);
         return XC_E_DEVICE;
     }
+    // the allocation-free entry point's fixed scratch on this device (looked up once per device)
+    if (noalloc_init(ctx) != XC_OK) {
+        xc_ctx_destroy(ctx);
+        return XC_E_DEVICE;
+    }
     *out = ctx;
     return XC_OK;
 }

@@ -62,13 +62,20 @@ void stage_lists(xc_point *dp, double *dv, const xc_point *ip1, const double *iv
     }
 }
 
+}  // namespace
+
+namespace xc {
+
 // Run one FP64 correlation of point lists (A = ip1/iv1, B = ip2/iv2).
 // fmode (with mode PM_R): CorrImagesF -- the R image is kept and the peak
 // is chosen by MakeF / OrderF / FPeak with nbmaxht.
+// fixed (may be null): a device workspace the call must fit in; then the call
+// allocates nothing (no device arena, no pinned staging: the lists are copied
+// from the caller's memory, the descriptors from host memory of this call).
 int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int n1, const xc_point *ip2,
               const double *iv2, int n2, int Ox, int Oy, int radius, int Rx, int Ry, double mincor,
-              const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out, bool fmode = false,
-              double nbmaxht = 0.0)
+              const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out, bool fmode, double nbmaxht,
+              const FixedWs *fixed)
 {
     if (!ctx) return XC_E_ARG;
     if (!ip1 || !iv1 || !ip2 || !iv2 || n1 <= 0 || n2 <= 0)
@@ -140,12 +147,27 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     size_t i_tbl = lay.add(tbl ? tbl : 1);
     const long long nR = (long long)P.wR * P.hR;
     size_t i_f = lay.add(fmode ? fmode_bytes(nR) : 1);
-    // scratch for an ordered splat, in case a list is off the grid (found while staging it)
-    const size_t spl = std::max(splat_scratch_bytes(n1, (long long)w1 * h1), splat_scratch_bytes(n2, (long long)w2 * h2));
+    // scratch for an ordered splat, in case a list is off the grid (found while staging it; with a fixed
+    // workspace the lists are tested first and the scratch is reserved only for one that is off it)
+    bool og1 = false, og2 = false;
+    if (fixed) {
+        og1 = points_off_grid(ip1, n1);
+        og2 = points_off_grid(ip2, n2);
+    }
+    const size_t spl = std::max(fixed && !og1 ? 0 : splat_scratch_bytes(n1, (long long)w1 * h1),
+                                fixed && !og2 ? 0 : splat_scratch_bytes(n2, (long long)w2 * h2));
     size_t i_spl = lay.add(spl ? spl : 1);
-    int st = ensure_dev(ctx, ctx->arena, lay.total);
-    if (st) return st;
-    char *base = (char *)ctx->arena.p;
+    int st;
+    char *base;
+    if (fixed) {
+        if (lay.total > fixed->bytes)
+            return set_err(ctx, XC_E_SHAPE, "the pair needs %zu bytes of device workspace; the fixed scratch holds %zu",
+                           lay.total, fixed->bytes);
+        base = (char *)fixed->dev;
+    } else {
+        if ((st = ensure_dev(ctx, ctx->arena, lay.total))) return st;
+        base = (char *)ctx->arena.p;
+    }
     auto at = [&](size_t i) { return base + lay.off[i]; };
 
     TileDev T[2];
@@ -194,15 +216,25 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     // ---- host staging: points + values, then descriptors ----
     const size_t s1_lo = lay.off[i_pts], s1_hi = lay.off[i_vals] + sizeof(double) * n;
     const size_t s2_lo = lay.off[i_pair], s2_hi = lay.off[i_tbl] + (tbl ? tbl : 1);
-    st = ensure_pinned(ctx, (s1_hi - s1_lo) + (s2_hi - s2_lo) + sizeof(xc_result) + 64);
-    if (st) return st;
-    char *hs = (char *)ctx->pinned;
-    char *hs2 = hs + (s1_hi - s1_lo);
-    xc_result *hres = (xc_result *)(((uintptr_t)(hs2 + (s2_hi - s2_lo)) + 63) & ~(uintptr_t)63);
-    auto h1p = [&](size_t i) { return hs + (lay.off[i] - s1_lo); };
+    char *hs = nullptr, *hs2;
+    xc_result *hres;
+    std::vector<char> hdesc;  // fixed workspace: the descriptor block in this call's host memory
+    xc_result hres_local;
+    if (fixed) {
+        hdesc.resize(s2_hi - s2_lo);
+        hs2 = hdesc.data();
+        hres = &hres_local;
+    } else {
+        if ((st = ensure_pinned(ctx, (s1_hi - s1_lo) + (s2_hi - s2_lo) + sizeof(xc_result) + 64))) return st;
+        hs = (char *)ctx->pinned;
+        hs2 = hs + (s1_hi - s1_lo);
+        hres = (xc_result *)(((uintptr_t)(hs2 + (s2_hi - s2_lo)) + 63) & ~(uintptr_t)63);
+    }
     auto h2p = [&](size_t i) { return hs2 + (lay.off[i] - s2_lo); };
-    bool og1 = false, og2 = false;
-    stage_lists((xc_point *)h1p(i_pts), (double *)h1p(i_vals), ip1, iv1, n1, ip2, iv2, n2, &og1, &og2);
+    if (!fixed) {
+        auto h1p = [&](size_t i) { return hs + (lay.off[i] - s1_lo); };
+        stage_lists((xc_point *)h1p(i_pts), (double *)h1p(i_vals), ip1, iv1, n1, ip2, iv2, n2, &og1, &og2);
+    }
     memcpy(h2p(i_pair), &P, sizeof P);
     memcpy(h2p(i_tiles), T, sizeof T);
     int4 *wr = (int4 *)h2p(i_wr);
@@ -222,7 +254,16 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];
 
     hipStream_t s = ctx->stream;
-    XC_HIP(ctx, hipMemcpyAsync(base + s1_lo, hs, s1_hi - s1_lo, hipMemcpyHostToDevice, s));
+    if (fixed) {
+        // straight from the caller's lists (the host memory stays valid until the call has synchronised)
+        char *dp = base + lay.off[i_pts], *dv = base + lay.off[i_vals];
+        XC_HIP(ctx, hipMemcpyAsync(dp, ip1, sizeof(xc_point) * n1, hipMemcpyHostToDevice, s));
+        XC_HIP(ctx, hipMemcpyAsync(dp + sizeof(xc_point) * n1, ip2, sizeof(xc_point) * n2, hipMemcpyHostToDevice, s));
+        XC_HIP(ctx, hipMemcpyAsync(dv, iv1, sizeof(double) * n1, hipMemcpyHostToDevice, s));
+        XC_HIP(ctx, hipMemcpyAsync(dv + sizeof(double) * n1, iv2, sizeof(double) * n2, hipMemcpyHostToDevice, s));
+    } else {
+        XC_HIP(ctx, hipMemcpyAsync(base + s1_lo, hs, s1_hi - s1_lo, hipMemcpyHostToDevice, s));
+    }
     XC_HIP(ctx, hipMemcpyAsync(base + s2_lo, hs2, s2_hi - s2_lo, hipMemcpyHostToDevice, s));
 
     // ---- rasters: zero then splat (ImageFromValuesAndPoints) ----
@@ -268,7 +309,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     return out->status;
 }
 
-}  // namespace
+}  // namespace xc
 
 extern "C" int xc_corr_patches_f64(xc_ctx *ctx, const xc_point *ip1, const double *iv1, int n1, const xc_point *ip2,
                                    const double *iv2, int n2, int Ox, int Oy, int radius, const xc_rgn_rule *rgn,
@@ -276,7 +317,8 @@ extern "C" int xc_corr_patches_f64(xc_ctx *ctx, const xc_point *ip1, const doubl
 {
     xc_result R;
     memset(&R, 0, sizeof R);
-    int st = run_plain(ctx, PM_NCC, ip1, iv1, n1, ip2, iv2, n2, Ox, Oy, radius, 0, 0, 0.0, rgn, cnt, &R);
+    int st = run_plain(ctx, PM_NCC, ip1, iv1, n1, ip2, iv2, n2, Ox, Oy, radius, 0, 0, 0.0, rgn, cnt, &R, false, 0.0,
+                       nullptr);
     if (st == XC_OK) {
         if (dx) *dx = R.dx;
         if (dy) *dy = R.dy;
@@ -299,7 +341,7 @@ extern "C" int xc_corr_images_f64(xc_ctx *ctx, int mode, const xc_point *ip1, co
     xc_result R;
     memset(&R, 0, sizeof R);
     int st = run_plain(ctx, mode == XC_MODE_S ? PM_S : PM_R, ip1, iv1, n1, ip2, iv2, n2, Ox, Oy, 0, Rx, Ry, mincor,
-                       rgn, cnt, &R, mode == XC_MODE_F, nbmaxht);
+                       rgn, cnt, &R, mode == XC_MODE_F, nbmaxht, nullptr);
     if (st == XC_OK) {
         if (dx) *dx = R.dx;
         if (dy) *dy = R.dy;

@@ -61,6 +61,9 @@ def load():
 _OPTIONAL = {
     "xc_corr_patch_to_image_f64": ([C.c_void_p, _pp, _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_int, _dp, _dp, _dp, C.POINTER(XcResult)], C.c_int),
+    "xc_corr_patches_noalloc_f64": ([C.c_void_p, _pp, _dp, C.c_int, _pp, _dp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(XcRgnRule), C.POINTER(XcCntRule), _dp, _dp, _dp, C.POINTER(XcResult)],
+                                    C.c_int),
     "xc_fft2d_r2c_f64": ([C.c_void_p, _dp, C.c_int, C.c_int, _dp, C.POINTER(C.c_int)], C.c_int),
     "xc_ift2d_c2r_f64": ([C.c_void_p, _dp, C.c_int, C.c_int, _dp], C.c_int),
     "xc_tile_put": ([C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
@@ -216,6 +219,22 @@ class Context:
         st = self.L.xc_corr_patches_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), Ox, Oy, radius,
                                         _ref(rgn), _ref(cnt), C.byref(dx), C.byref(dy), C.byref(r), C.byref(res))
         self._check(st, "xc_corr_patches_f64", ok=(abi.XC_OK, abi.XC_NO_PEAK, abi.XC_E_ODD_R))
+        return st, r.value, dx.value, dy.value, res
+
+    def corr_patches_noalloc(self, ip1, iv1, ip2, iv2, Ox=0, Oy=0, radius=4096, rgn=None, cnt=None, dx0=0.0,
+                             dy0=0.0):
+        """CorrPatches through the entry point that allocates nothing (its workspace is the library's fixed
+        per-GPU scratch).  Same returns as corr_patches."""
+        p1, pp1 = _pts(ip1)
+        v1, pv1 = _vals(iv1)
+        p2, pp2 = _pts(ip2)
+        v2, pv2 = _vals(iv2)
+        dx, dy, r = C.c_double(dx0), C.c_double(dy0), C.c_double(0)
+        res = XcResult()
+        st = self.L.xc_corr_patches_noalloc_f64(self.h, pp1, pv1, len(v1), pp2, pv2, len(v2), Ox, Oy, radius,
+                                                _ref(rgn), _ref(cnt), C.byref(dx), C.byref(dy), C.byref(r),
+                                                C.byref(res))
+        self._check(st, "xc_corr_patches_noalloc_f64", ok=(abi.XC_OK, abi.XC_NO_PEAK, abi.XC_E_ODD_R))
         return st, r.value, dx.value, dy.value, res
 
     def corr_images(self, mode, ip1, iv1, ip2, iv2, rgn=None, cnt=None, mincor=0.0, nbmaxht=0.0,

@@ -173,6 +173,28 @@ int xc_corr_patches_f64(xc_ctx *ctx,
                         double *dx, double *dy, double *r, xc_result *res);
 
 /*
+ * No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  xc_corr_patches_f64 with the same arguments, kernels and
+ * results, but it allocates nothing: no device or pinned memory, no library
+ * routine that allocates.  Its device workspace is a fixed array of
+ * XC_NOALLOC_SCRATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends; the lists are copied from the caller's memory.
+ * A pair whose workspace (lists, rasters, FP64 tables, spectra, lag rows)
+ * exceeds it is refused with XC_E_SHAPE before any device work.  Every call
+ * on one GPU shares that GPU's array and the array keeps the last call's
+ * contents (each call clears what must start at zero); concurrent calls on
+ * one GPU would race for it, so the library runs them one at a time.
+ */
+#define XC_NOALLOC_SCRATCH_BYTES (128ULL << 20)
+int xc_corr_patches_noalloc_f64(xc_ctx *ctx,
+                                const xc_point *ip1, const double *iv1, int n1,
+                                const xc_point *ip2, const double *iv2, int n2,
+                                int Ox, int Oy, int radius,
+                                const xc_rgn_rule *rgn, const xc_cnt_rule *cnt,
+                                double *dx, double *dy, double *r, xc_result *res);
+
+/*
  * Replaces CorrPatchToImage (Correlation.cpp:1186-1355): patch1 (integer
  * points) against the caller-embedded square N x N image i2 (N = sqrt(n_i2)).
  * bFilter applies the reference's Gaussian low-pass of the cross power.
