@@ -189,13 +189,14 @@ class XcBucketInfo(C.Structure):
 
 class XcServeCfg(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_batch", C.c_int32), ("streams", C.c_int32),
-                ("max_wait_us", C.c_int32), ("_pad", C.c_int32)]
+                ("max_wait_us", C.c_int32), ("deadline_us", C.c_int32)]
 
 
 class XcServeStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("rejected", C.c_int64),
                 ("batches", C.c_int64), ("spectra_hits", C.c_int64), ("spectra_misses", C.c_int64),
-                ("arena_bytes", C.c_int64), ("arena_peak", C.c_int64)]
+                ("arena_bytes", C.c_int64), ("arena_peak", C.c_int64), ("timed_out", C.c_int64),
+                ("cancelled", C.c_int64), ("failed", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -1,7 +1,9 @@
 // xc_serve.hip -- serving loop: continuous batching of pair-correlation
 // requests over resident tiles, admission by an HBM budget, resident row
-// spectra, batches overlapping on several streams.
+// spectra, batches overlapping on several streams, request deadlines and
+// cancellation.
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -16,77 +18,21 @@
 
 #include "xc_exec.hpp"
 #include "xc_kernels.hpp"
+#include "xc_sched.hpp"
 
 using namespace xc;
+using xc::sched::ArenaAlloc;
+using xc::sched::Req;
 
 namespace {
 
-using Clock = std::chrono::steady_clock;
-
-double us_since(Clock::time_point t0) { return std::chrono::duration<double, std::micro>(Clock::now() - t0).count(); }
-
-// First-fit allocator over one device allocation (offsets, 256-byte aligned).
-struct ArenaAlloc {
-    char *base = nullptr;
-    size_t cap = 0, used = 0, peak = 0;
-    std::map<size_t, size_t> free_;  // offset -> bytes
-    void init(char *b, size_t c)
-    {
-        base = b;
-        cap = c;
-        free_.clear();
-        free_[0] = c;
-    }
-    static size_t round(size_t n) { return (n + 255) & ~(size_t)255; }
-    bool alloc(size_t n, size_t &off)
-    {
-        n = round(n);
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second >= n) {
-                off = it->first;
-                const size_t rest = it->second - n;
-                free_.erase(it);
-                if (rest) free_[off + n] = rest;
-                used += n;
-                peak = std::max(peak, used);
-                return true;
-            }
-        }
-        return false;
-    }
-    void release(size_t off, size_t n)
-    {
-        n = round(n);
-        used -= n;
-        auto it = free_.emplace(off, n).first;
-        auto nx = std::next(it);
-        if (nx != free_.end() && it->first + it->second == nx->first) {
-            it->second += nx->second;
-            free_.erase(nx);
-        }
-        if (it != free_.begin()) {
-            auto pv = std::prev(it);
-            if (pv->first + pv->second == it->first) {
-                pv->second += it->second;
-                free_.erase(it);
-            }
-        }
-    }
-};
+using Clock = xc::sched::Clock;
 
 struct SpecEntry {
     size_t off = 0, bytes = 0;
     int refs = 0;                 // batches in flight that read or write it
     hipEvent_t ready = nullptr;   // recorded after the producing batch's row pass
     uint64_t last_use = 0;
-};
-
-struct Req {
-    xc_pair pair;
-    Clock::time_point t_submit;
-    double latency_us = 0.0;
-    xc_result res;
-    bool done = false;
 };
 
 struct Batch {
@@ -114,16 +60,16 @@ struct xc_server {
     ArenaAlloc arena;
     std::mutex mu;
     std::condition_variable cv;
-    std::deque<uint64_t> pending;
-    std::unordered_map<uint64_t, Req> reqs;
+    xc::sched::Queue q;  // requests by ticket; those not yet launched in arrival order
     std::map<std::pair<uint64_t, int>, SpecEntry> specs;  // (tile uid, Nx) -> row spectrum
     std::vector<Slot> slots;
     std::vector<std::unique_ptr<Batch>> inflight;
-    uint64_t next_id = 1, clock = 0;
+    uint64_t clock = 0;
     bool stop = false;
     std::thread thr;
     xc_serve_stats st;
     int err = XC_OK;
+    int fail_pinned = 0;  // fault injection (XC_SERVE_FAIL_PINNED): pinned allocations still to refuse
 };
 
 namespace {
@@ -175,82 +121,43 @@ bool alloc_evicting(xc_server *S, size_t need, size_t &off)
     return true;
 }
 
-// Build and launch one batch from the head of the queue (lock held).
-int launch_batch(xc_server *S, int slot_i)
+// A batch whose requests have left the queue failed before it was complete
+// on the device: wait for whatever it enqueued, complete its requests with
+// the error, give back its arena blocks and spectrum references, and drop the
+// spectra it was to produce (never computed, so never to be read).  Lock held.
+void fail_batch(xc_server *S, Batch &B, const std::map<std::pair<uint64_t, int>, bool> &fresh, int st)
+{
+    Slot &slot = S->slots[B.slot];
+    (void)hipStreamSynchronize(slot.stream);
+    (void)hipGetLastError();
+    const auto now = Clock::now();
+    for (uint64_t id : B.ids) S->q.complete(id, st, now);
+    S->st.failed += (int64_t)B.ids.size();
+    for (auto &b : B.blocks)
+        if (b.first != SIZE_MAX) S->arena.release(b.first, b.second);
+    for (auto &key : B.specs) S->specs[key].refs--;
+    for (auto &kv : fresh) {
+        auto it = S->specs.find(kv.first);
+        if (it == S->specs.end()) continue;
+        S->arena.release(it->second.off, it->second.bytes);
+        if (it->second.ready) (void)hipEventDestroy(it->second.ready);
+        S->specs.erase(it);
+    }
+    if (B.done) (void)hipEventDestroy(B.done);
+    slot.busy = false;
+    S->cv.notify_all();
+}
+
+// Descriptors, work lists and launches of a batch whose requests are taken
+// (lock held).  XC_E_NOTREADY: its descriptor block does not fit yet (the
+// requests are given back to the front of the queue); any other error leaves
+// the cleanup to the caller.
+int enqueue_batch(xc_server *S, int slot_i, std::unique_ptr<Batch> &B, std::vector<PairDev> &P,
+                  const std::vector<bool> &ownA, const std::vector<bool> &ownB, const std::vector<hipEvent_t> &waits,
+                  std::map<std::pair<uint64_t, int>, bool> &fresh)
 {
     Slot &slot = S->slots[slot_i];
-    auto B = std::make_unique<Batch>();
-    B->slot = slot_i;
-    std::vector<PairDev> P;
-    std::vector<bool> ownA, ownB;
-    std::vector<hipEvent_t> waits;
-    std::map<std::pair<uint64_t, int>, bool> fresh;  // produced by this batch
-    while (!S->pending.empty() && (int)B->ids.size() < S->cfg.max_batch) {
-        const uint64_t id = S->pending.front();
-        Req &r = S->reqs[id];
-        PairDev D;
-        make_pair_dev(S, r.pair, D);
-        // memory: workspace + spectra not yet resident
-        const auto ka = std::make_pair(r.pair.a->uid, D.Nx), kb = std::make_pair(r.pair.b->uid, D.Nx);
-        size_t off_w;
-        if (!alloc_evicting(S, pair_work_bytes(D), off_w)) break;
-        bool ok = true;
-        bool oa = false, ob = false;
-        std::vector<std::pair<uint64_t, int>> mine, held;  // spectra this request created / found
-        for (int side = 0; side < 2 && ok; ++side) {
-            const auto key = side ? kb : ka;
-            auto it = S->specs.find(key);
-            if (it == S->specs.end()) {
-                SpecEntry e;
-                e.bytes = spec_bytes(side ? r.pair.b : r.pair.a, key.second);
-                if (!alloc_evicting(S, e.bytes, e.off)) {
-                    ok = false;
-                    break;
-                }
-                e.refs = 1;  // held from now on, so later allocations cannot evict it
-                S->specs[key] = e;
-                fresh[key] = true;
-                mine.push_back(key);
-                (side ? ob : oa) = true;
-                S->st.spectra_misses++;
-            } else {
-                it->second.refs++;
-                held.push_back(key);
-                if (!fresh.count(key) && it->second.ready) waits.push_back(it->second.ready);
-                S->st.spectra_hits++;
-            }
-        }
-        if (!ok) {
-            for (auto &key : held) S->specs[key].refs--;
-            for (auto &key : mine) {
-                S->arena.release(S->specs[key].off, S->specs[key].bytes);
-                S->specs.erase(key);
-                fresh.erase(key);
-            }
-            S->arena.release(off_w, pair_work_bytes(D));
-            break;
-        }
-        for (const auto &key : {ka, kb}) {
-            S->specs[key].last_use = ++S->clock;
-            B->specs.push_back(key);
-        }
-        B->blocks.push_back(std::make_pair(off_w, pair_work_bytes(D)));
-        D.lag = S->arena.base + off_w;
-        D.rows = (RowRec *)(S->arena.base + off_w + ArenaAlloc::round(sizeof(cpx<float>) * (size_t)(D.Nx / 2 + 1) * D.hR));
-        D.specA = S->arena.base + S->specs[ka].off;
-        D.specB = S->arena.base + S->specs[kb].off;
-        P.push_back(D);
-        ownA.push_back(oa);
-        ownB.push_back(ob);
-        B->ids.push_back(id);
-        S->pending.pop_front();
-    }
     const int n = (int)B->ids.size();
-    if (n == 0) {
-        for (auto &b : B->blocks) S->arena.release(b.first, b.second);
-        return XC_E_NOTREADY;
-    }
-    // ---- descriptors, work lists, results (one arena block + pinned staging) ----
     std::vector<int> order;
     std::vector<Bucket> buckets;
     make_buckets(P, 0, order, buckets);
@@ -281,25 +188,38 @@ int launch_batch(xc_server *S, int slot_i)
     size_t off_d;
     if (!alloc_evicting(S, lay.total, off_d)) {
         // give the requests back (front of the queue, same order)
-        for (int k = n - 1; k >= 0; --k) S->pending.push_front(B->ids[k]);
+        for (int k = n - 1; k >= 0; --k) {
+            S->q.reqs[B->ids[k]].launched = false;
+            S->q.pending.push_front(B->ids[k]);
+        }
         for (auto &b : B->blocks) S->arena.release(b.first, b.second);
         for (auto &key : B->specs) S->specs[key].refs--;
         for (auto &kv : fresh) {  // never computed: drop them
             S->arena.release(S->specs[kv.first].off, S->specs[kv.first].bytes);
             S->specs.erase(kv.first);
         }
+        B->ids.clear();
+        B->blocks.clear();
+        B->specs.clear();
+        fresh.clear();
         return XC_E_NOTREADY;
     }
     B->blocks.push_back(std::make_pair(off_d, lay.total));
     char *dbase = S->arena.base + off_d;
     if (slot.pinned_bytes < lay.total) {
         if (slot.pinned) (void)hipHostFree(slot.pinned);
-        slot.pinned_bytes = lay.total * 2;
-        if (hipHostMalloc(&slot.pinned, slot.pinned_bytes, hipHostMallocDefault) != hipSuccess) {
-            slot.pinned = nullptr;
-            slot.pinned_bytes = 0;
-            return XC_E_NOMEM;
+        slot.pinned = nullptr;
+        slot.pinned_bytes = 0;
+        if (S->fail_pinned > 0) {
+            --S->fail_pinned;
+            return set_err(S->ctx, XC_E_NOMEM, "pinned staging refused (fault injection)");
         }
+        if (hipHostMalloc(&slot.pinned, lay.total * 2, hipHostMallocDefault) != hipSuccess) {
+            slot.pinned = nullptr;
+            (void)hipGetLastError();
+            return set_err(S->ctx, XC_E_NOMEM, "pinned staging of %zu bytes", lay.total * 2);
+        }
+        slot.pinned_bytes = lay.total * 2;
     }
     char *h = (char *)slot.pinned;
     PairDev *hp = (PairDev *)(h + lay.off[i_pairs]);
@@ -355,6 +275,92 @@ int launch_batch(xc_server *S, int slot_i)
     XC_HIP(ctx, hipEventRecord(B->done, slot.stream));
     // the result block's host copy lives at h + lay.off[i_res]; remember it through the Batch
     B->blocks.push_back(std::make_pair(SIZE_MAX, lay.off[i_res]));
+    return XC_OK;
+}
+
+// Build and launch one batch from the head of the queue (lock held).
+// XC_OK: a batch was launched, or its requests completed with an error;
+// XC_E_NOTREADY: nothing fits yet.
+int launch_batch(xc_server *S, int slot_i)
+{
+    Slot &slot = S->slots[slot_i];
+    auto B = std::make_unique<Batch>();
+    B->slot = slot_i;
+    std::vector<PairDev> P;
+    std::vector<bool> ownA, ownB;
+    std::vector<hipEvent_t> waits;
+    std::map<std::pair<uint64_t, int>, bool> fresh;  // produced by this batch
+    while (!S->q.pending.empty() && (int)B->ids.size() < S->cfg.max_batch) {
+        const uint64_t id = S->q.pending.front();
+        Req &r = S->q.reqs[id];
+        PairDev D;
+        make_pair_dev(S, r.pair, D);
+        // memory: workspace + spectra not yet resident
+        const auto ka = std::make_pair(r.pair.a->uid, D.Nx), kb = std::make_pair(r.pair.b->uid, D.Nx);
+        size_t off_w;
+        if (!alloc_evicting(S, pair_work_bytes(D), off_w)) break;
+        bool ok = true;
+        bool oa = false, ob = false;
+        std::vector<std::pair<uint64_t, int>> mine, held;  // spectra this request created / found
+        for (int side = 0; side < 2 && ok; ++side) {
+            const auto key = side ? kb : ka;
+            auto it = S->specs.find(key);
+            if (it == S->specs.end()) {
+                SpecEntry e;
+                e.bytes = spec_bytes(side ? r.pair.b : r.pair.a, key.second);
+                if (!alloc_evicting(S, e.bytes, e.off)) {
+                    ok = false;
+                    break;
+                }
+                e.refs = 1;  // held from now on, so later allocations cannot evict it
+                S->specs[key] = e;
+                fresh[key] = true;
+                mine.push_back(key);
+                (side ? ob : oa) = true;
+                S->st.spectra_misses++;
+            } else {
+                it->second.refs++;
+                held.push_back(key);
+                if (!fresh.count(key) && it->second.ready) waits.push_back(it->second.ready);
+                S->st.spectra_hits++;
+            }
+        }
+        if (!ok) {
+            for (auto &key : held) S->specs[key].refs--;
+            for (auto &key : mine) {
+                S->arena.release(S->specs[key].off, S->specs[key].bytes);
+                S->specs.erase(key);
+                fresh.erase(key);
+            }
+            S->arena.release(off_w, pair_work_bytes(D));
+            break;
+        }
+        for (const auto &key : {ka, kb}) {
+            S->specs[key].last_use = ++S->clock;
+            B->specs.push_back(key);
+        }
+        B->blocks.push_back(std::make_pair(off_w, pair_work_bytes(D)));
+        D.lag = S->arena.base + off_w;
+        D.rows = (RowRec *)(S->arena.base + off_w + ArenaAlloc::round(sizeof(cpx<float>) * (size_t)(D.Nx / 2 + 1) * D.hR));
+        D.specA = S->arena.base + S->specs[ka].off;
+        D.specB = S->arena.base + S->specs[kb].off;
+        P.push_back(D);
+        ownA.push_back(oa);
+        ownB.push_back(ob);
+        B->ids.push_back(id);
+        r.launched = true;
+        S->q.pending.pop_front();
+    }
+    if (B->ids.empty()) {
+        for (auto &b : B->blocks) S->arena.release(b.first, b.second);
+        return XC_E_NOTREADY;
+    }
+    const int st = enqueue_batch(S, slot_i, B, P, ownA, ownB, waits, fresh);
+    if (st == XC_E_NOTREADY) return st;
+    if (st != XC_OK) {
+        fail_batch(S, *B, fresh, st);
+        return XC_OK;
+    }
     slot.busy = true;
     S->st.batches++;
     S->inflight.push_back(std::move(B));
@@ -367,7 +373,8 @@ bool reap(xc_server *S)
     bool any = false;
     for (size_t k = 0; k < S->inflight.size();) {
         Batch &B = *S->inflight[k];
-        if (hipEventQuery(B.done) != hipSuccess) {
+        const hipError_t q = hipEventQuery(B.done);
+        if (q == hipErrorNotReady) {
             ++k;
             continue;
         }
@@ -376,13 +383,20 @@ bool reap(xc_server *S)
         for (auto &b : B.blocks)
             if (b.first == SIZE_MAX) res_off = b.second;
         const xc_result *hr = (const xc_result *)((char *)slot.pinned + res_off);
+        const auto now = Clock::now();
         for (size_t i = 0; i < B.ids.size(); ++i) {
-            Req &r = S->reqs[B.ids[i]];
-            r.res = hr[i];
-            r.latency_us = us_since(r.t_submit);
-            r.done = true;
-            S->st.completed++;
+            Req &r = S->q.reqs[B.ids[i]];
+            if (q == hipSuccess) {
+                r.res = hr[i];
+                r.latency_us = sched::us_between(r.t_submit, now);
+                r.done = true;
+                S->st.completed++;
+            } else {
+                S->q.complete(B.ids[i], XC_E_DEVICE, now);  // the batch's work failed on the device
+                S->st.failed++;
+            }
         }
+        if (q != hipSuccess) (void)hipGetLastError();
         for (auto &b : B.blocks)
             if (b.first != SIZE_MAX) S->arena.release(b.first, b.second);
         for (auto &key : B.specs) S->specs[key].refs--;
@@ -402,49 +416,44 @@ void run(xc_server *S)
     bool waiting = false;
     for (;;) {
         bool progress = reap(S);
+        const int expired = S->q.expire(Clock::now());
+        if (expired) {
+            S->st.timed_out += expired;
+            progress = true;
+        }
         if (progress) S->cv.notify_all();
-        if (!S->pending.empty()) {
+        if (!S->q.pending.empty()) {
             if (!waiting) {
                 waiting = true;
                 oldest_wait = Clock::now();
             }
-            const bool full = (int)S->pending.size() >= S->cfg.max_batch;
-            const bool aged = us_since(oldest_wait) >= S->cfg.max_wait_us;
+            const bool full = (int)S->q.pending.size() >= S->cfg.max_batch;
+            const bool aged = sched::us_between(oldest_wait, Clock::now()) >= S->cfg.max_wait_us;
             if (full || aged || S->stop) {
-                for (size_t si = 0; si < S->slots.size() && !S->pending.empty(); ++si) {
+                for (size_t si = 0; si < S->slots.size() && !S->q.pending.empty(); ++si) {
                     if (S->slots[si].busy) continue;
                     const int st = launch_batch(S, (int)si);
                     if (st == XC_OK) {
                         progress = true;
+                        S->cv.notify_all();
                     } else if (st == XC_E_NOTREADY && S->inflight.empty()) {
                         // nothing in flight and the head request still does not fit: it never will
-                        Req &r = S->reqs[S->pending.front()];
-                        r.res.status = XC_E_BUDGET;
-                        r.done = true;
-                        S->pending.pop_front();
+                        const uint64_t id = S->q.pending.front();
+                        S->q.pending.pop_front();
+                        S->q.complete(id, XC_E_BUDGET, Clock::now());
                         S->st.rejected++;
                         S->cv.notify_all();
                         progress = true;
-                    } else if (st != XC_E_NOTREADY) {
-                        S->err = st;
-                        // fail the queue rather than hang
-                        while (!S->pending.empty()) {
-                            Req &r = S->reqs[S->pending.front()];
-                            r.res.status = st;
-                            r.done = true;
-                            S->pending.pop_front();
-                        }
-                        S->cv.notify_all();
                     }
                     break;
                 }
-                if (S->pending.empty()) waiting = false;
+                if (S->q.pending.empty()) waiting = false;
                 else oldest_wait = Clock::now();
             }
         } else {
             waiting = false;
         }
-        if (S->stop && S->pending.empty() && S->inflight.empty()) break;
+        if (S->stop && S->q.pending.empty() && S->inflight.empty()) break;
         if (!progress) {
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(S->inflight.empty() ? 50 : 10));
@@ -457,11 +466,14 @@ void run(xc_server *S)
 
 extern "C" int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server **out)
 {
-    if (!ctx || !cfg || !out || cfg->hbm_budget <= 0 || cfg->max_batch <= 0 || cfg->streams <= 0) return XC_E_ARG;
+    if (!ctx || !cfg || !out || cfg->hbm_budget <= 0 || cfg->max_batch <= 0 || cfg->streams <= 0 ||
+        cfg->deadline_us < 0)
+        return XC_E_ARG;
     *out = nullptr;
     auto S = std::make_unique<xc_server>();
     S->cfg = *cfg;
     memset(&S->st, 0, sizeof S->st);
+    if (const char *f = getenv("XC_SERVE_FAIL_PINNED")) S->fail_pinned = atoi(f);
     int st = xc_ctx_create(ctx->device, &S->ctx);
     if (st) return st;
     S->own_stream = S->ctx->stream;
@@ -473,7 +485,14 @@ extern "C" int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server 
     S->arena.init((char *)S->arena_mem, (size_t)cfg->hbm_budget);
     S->slots.resize(cfg->streams);
     for (auto &sl : S->slots) {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) return XC_E_DEVICE;
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) {
+            (void)hipGetLastError();
+            for (auto &o : S->slots)
+                if (o.stream) (void)hipStreamDestroy(o.stream);
+            (void)hipFree(S->arena_mem);
+            xc_ctx_destroy(S->ctx);
+            return XC_E_DEVICE;
+        }
     }
     xc_server *p = S.release();
     p->thr = std::thread(run, p);
@@ -499,23 +518,28 @@ extern "C" int xc_server_submit(xc_server *S, const xc_pair *pair, uint64_t *tic
         S->st.rejected++;
         return XC_E_BUDGET;
     }
-    const uint64_t id = S->next_id++;
-    Req r;
-    r.pair = *pair;
-    r.t_submit = Clock::now();
-    memset(&r.res, 0, sizeof r.res);
-    S->reqs[id] = r;
-    S->pending.push_back(id);
-    *ticket = id;
+    *ticket = S->q.submit(*pair, Clock::now(), S->cfg.deadline_us);
     return XC_OK;
+}
+
+extern "C" int xc_server_cancel(xc_server *S, uint64_t ticket)
+{
+    if (!S) return XC_E_ARG;
+    std::lock_guard<std::mutex> lk(S->mu);
+    const int st = S->q.cancel(ticket, Clock::now());
+    if (st == XC_OK) {
+        S->st.cancelled++;
+        S->cv.notify_all();
+    }
+    return st;
 }
 
 extern "C" int xc_server_poll(xc_server *S, uint64_t ticket, xc_result *res)
 {
     if (!S) return XC_E_ARG;
     std::lock_guard<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(ticket);
-    if (it == S->reqs.end()) return XC_E_ARG;
+    auto it = S->q.reqs.find(ticket);
+    if (it == S->q.reqs.end()) return XC_E_ARG;
     if (!it->second.done) return XC_E_NOTREADY;
     if (res) *res = it->second.res;
     return XC_OK;
@@ -525,13 +549,13 @@ extern "C" int xc_server_wait(xc_server *S, uint64_t ticket, xc_result *res, dou
 {
     if (!S) return XC_E_ARG;
     std::unique_lock<std::mutex> lk(S->mu);
-    auto it = S->reqs.find(ticket);
-    if (it == S->reqs.end()) return XC_E_ARG;
+    auto it = S->q.reqs.find(ticket);
+    if (it == S->q.reqs.end()) return XC_E_ARG;
     S->cv.wait(lk, [&] { return it->second.done; });
     if (res) *res = it->second.res;
     if (latency_us) *latency_us = it->second.latency_us;
     const int st = it->second.res.status;
-    S->reqs.erase(it);
+    S->q.reqs.erase(it);
     return st < 0 ? st : XC_OK;
 }
 

@@ -67,14 +67,20 @@ extern "C" int xc_thmpair_append(const char *path, const xc_thmpair *e)
 {
     if (!path || !e) return XC_E_ARG;
     if (e->atl < 0 || e->btl < 0) return XC_OK;  // "Table writing bypassed if invalid tile ID"
-    char buf[1024];
-    const int n = snprintf(buf, sizeof buf, "%d\t%d\t%d\t%d\t%d\t%f\t%f\t%f\t%f\t%f\t%f\t%f\t%f\n", e->atl, e->acr,
-                           e->btl, e->bcr, e->err, e->A, e->R, e->T[0], e->T[1], e->T[2], e->T[3], e->T[4], e->T[5]);
-    if (n <= 0 || n >= (int)sizeof buf) return XC_E_ARG;
+    // the line at whatever length fprintf would give it (a %f of a diverged entry near 1e300 runs to
+    // ~310 characters), formatted once into a buffer of that size
+    const char *fmt = "%d\t%d\t%d\t%d\t%d\t%f\t%f\t%f\t%f\t%f\t%f\t%f\t%f\n";
+    const int n = snprintf(nullptr, 0, fmt, e->atl, e->acr, e->btl, e->bcr, e->err, e->A, e->R, e->T[0], e->T[1],
+                           e->T[2], e->T[3], e->T[4], e->T[5]);
+    if (n <= 0) return XC_E_ARG;
+    std::vector<char> buf((size_t)n + 1);
+    if (snprintf(buf.data(), buf.size(), fmt, e->atl, e->acr, e->btl, e->bcr, e->err, e->A, e->R, e->T[0], e->T[1],
+                 e->T[2], e->T[3], e->T[4], e->T[5]) != n)
+        return XC_E_ARG;
     // one append write: concurrent writers (one per pair job) do not interleave within a line
     const int fd = open(path, O_WRONLY | O_APPEND | O_CREAT, 0644);
     if (fd < 0) return XC_E_ARG;
-    const ssize_t w = write(fd, buf, (size_t)n);
+    const ssize_t w = write(fd, buf.data(), (size_t)n);
     close(fd);
     return w == n ? XC_OK : XC_E_ARG;
 }

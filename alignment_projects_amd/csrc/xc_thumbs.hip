@@ -422,35 +422,42 @@ extern "C" int xc_make_thumbs_f64(xc_ctx *ctx, xc_point *ap, double *av, int na,
     std::lock_guard<std::mutex> lock(ctx->mu);
     XC_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
+    xc_point *P[2] = {ap, bp};
+    double *V[2] = {av, bv};
     int k[2] = {na, nb};
-    double sd[2] = {0.0, 0.0};
+    DevBuf buf[2], nbuf;
+    double *dv[2] = {nullptr, nullptr};
     int st = XC_OK;
+    auto finish = [&](int code) -> int {
+        (void)hipStreamSynchronize(s);
+        for (DevBuf &b : buf) (void)hipFree(b.p);
+        (void)hipFree(nbuf.p);
+        return code;
+    };
+    // values of side `side` on the device back into the caller's list
+    auto down = [&](int side) -> int {
+        if (k[side] > 0 && (hipMemcpyAsync(V[side], dv[side], sizeof(double) * k[side], hipMemcpyDeviceToHost, s) !=
+                                hipSuccess ||
+                            hipStreamSynchronize(s) != hipSuccess))
+            return set_err(ctx, XC_E_DEVICE, "MakeThumbs: download failed");
+        return XC_OK;
+    };
+    // both sides decimated first (CThmUtil.cpp:483-494); their point lists are final from here on
     for (int side = 0; side < 2 && !st; ++side) {
-        xc_point *p = side ? bp : ap;
-        double *v = side ? bv : av;
         const int n = side ? nb : na, w = side ? bw : aw, h = side ? bh : ah;
-        DevBuf buf, nbuf;
-        double *dv = nullptr;
         if (decfactor > 1) {
             xc_point *dp;
-            st = decimate_points(ctx, p, v, n, w, h, decfactor, buf, &dp, &dv, &k[side]);
-            if (!st && k[side] > 0 && hipMemcpyAsync(p, dp, sizeof(xc_point) * k[side], hipMemcpyDeviceToHost, s))
+            st = decimate_points(ctx, P[side], V[side], n, w, h, decfactor, buf[side], &dp, &dv[side], &k[side]);
+            if (!st && k[side] > 0 &&
+                hipMemcpyAsync(P[side], dp, sizeof(xc_point) * k[side], hipMemcpyDeviceToHost, s) != hipSuccess)
                 st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: download failed");
-        } else if (!(st = ensure_dev(ctx, buf, sizeof(double) * n))) {
-            dv = (double *)buf.p;
-            if (hipMemcpyAsync(dv, v, sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess)
+        } else if (!(st = ensure_dev(ctx, buf[side], sizeof(double) * n))) {
+            dv[side] = (double *)buf[side].p;
+            if (hipMemcpyAsync(dv[side], V[side], sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess)
                 st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: upload failed");
         }
-        if (!st && !(st = ensure_dev(ctx, nbuf, sizeof(double) * (2 * TH_PARTS + 2))))
-            st = normalize_dev(ctx, s, dv, k[side], (double *)nbuf.p, (double *)nbuf.p + 2 * TH_PARTS, &sd[side]);
-        if (!st && k[side] > 0 &&
-            (hipMemcpyAsync(v, dv, sizeof(double) * k[side], hipMemcpyDeviceToHost, s) != hipSuccess ||
-             hipStreamSynchronize(s) != hipSuccess))
-            st = set_err(ctx, XC_E_DEVICE, "MakeThumbs: download failed");
-        (void)hipFree(buf.p);
-        (void)hipFree(nbuf.p);
     }
-    if (st) return st;
+    if (st) return finish(st);
     int64_t req = *req_area;
     int olap = *olap1D;
     if (decfactor > 1) {
@@ -461,11 +468,26 @@ extern "C" int xc_make_thumbs_f64(xc_ctx *ctx, xc_point *ap, double *av, int na,
     *olap1D = olap;
     *na_out = k[0];
     *nb_out = k[1];
-    if (k[0] < req) return set_err(ctx, XC_TOO_SMALL, "MakeThumbs: small intersection %d (required %lld)", k[0],
-                                   (long long)req);
-    if (!(sd[0] != 0.0)) return set_err(ctx, XC_NO_PEAK, "MakeThumbs: image A intersection region: stdev = 0");
-    if (!(sd[1] != 0.0)) return set_err(ctx, XC_NO_PEAK, "MakeThumbs: image B intersection region: stdev = 0");
-    return XC_OK;
+    // the size test comes before any Normalize: the lists stay decimated, not normalised (:496-503)
+    if (k[0] < req) {
+        if (decfactor > 1 && ((st = down(0)) || (st = down(1)))) return finish(st);
+        return finish(set_err(ctx, XC_TOO_SMALL, "MakeThumbs: small intersection %d (required %lld)", k[0],
+                              (long long)req));
+    }
+    if ((st = ensure_dev(ctx, nbuf, sizeof(double) * (2 * TH_PARTS + 2)))) return finish(st);
+    // Normalize A, then B; a stdev of 0 stops there, leaving the other side as it was (:505-519)
+    for (int side = 0; side < 2; ++side) {
+        double sd = 0.0;
+        if ((st = normalize_dev(ctx, s, dv[side], k[side], (double *)nbuf.p, (double *)nbuf.p + 2 * TH_PARTS, &sd)) ||
+            (st = down(side)))
+            return finish(st);
+        if (!(sd != 0.0)) {
+            if (side == 0 && decfactor > 1 && (st = down(1))) return finish(st);
+            return finish(set_err(ctx, XC_NO_PEAK, "MakeThumbs: image %c intersection region: stdev = 0",
+                                  side ? 'B' : 'A'));
+        }
+    }
+    return finish(XC_OK);
 }
 
 extern "C" int xc_make_thumb_raster_dev_f64(xc_ctx *ctx, const void *ras, int dtype, int w, int h, int stride,

@@ -81,6 +81,7 @@ _OPTIONAL = {
     "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
     "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
     "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
+    "xc_server_cancel": ([C.c_void_p, C.c_uint64], C.c_int),
     "xc_server_wait": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult), _dp], C.c_int),
     "xc_server_stats": ([C.c_void_p, C.POINTER(XcServeStats)], C.c_int),
     "xc_server_destroy": ([C.c_void_p], C.c_int),
@@ -695,9 +696,10 @@ class Plan:
 class Server:
     """Serving loop (xc_server): continuous batching of pair requests over resident tiles."""
 
-    def __init__(self, ctx, hbm_budget=8 << 30, max_batch=64, streams=3, max_wait_us=200):
+    def __init__(self, ctx, hbm_budget=8 << 30, max_batch=64, streams=3, max_wait_us=200, deadline_us=0):
         self.ctx = ctx
-        cfg = XcServeCfg(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, max_wait_us=max_wait_us)
+        cfg = XcServeCfg(hbm_budget=hbm_budget, max_batch=max_batch, streams=streams, max_wait_us=max_wait_us,
+                         deadline_us=deadline_us)
         h = C.c_void_p()
         st = ctx.L.xc_server_create(ctx.h, C.byref(cfg), C.byref(h))
         ctx._check(st, "xc_server_create", ok=(abi.XC_OK,))
@@ -710,13 +712,19 @@ class Server:
             raise XcorrError(st, "xc_server_submit")
         return t.value
 
-    def wait(self, ticket):
+    def wait(self, ticket, raise_errors=True):
+        """(XcResult, latency_us); with raise_errors=False an error status (timeout, cancellation, device
+        failure) is returned in the record instead of raised."""
         r = XcResult()
         lat = C.c_double()
         st = self.ctx.L.xc_server_wait(self.h, ticket, C.byref(r), C.byref(lat))
-        if st != abi.XC_OK:
+        if st != abi.XC_OK and raise_errors:
             raise XcorrError(st, "xc_server_wait")
         return r, lat.value
+
+    def cancel(self, ticket):
+        """XC_OK if the queued request was cancelled, XC_E_NOTREADY if it is already on the device or done."""
+        return self.ctx.L.xc_server_cancel(self.h, ticket)
 
     def poll(self, ticket):
         r = XcResult()

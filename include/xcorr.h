@@ -735,13 +735,16 @@ typedef struct xc_serve_cfg {
     int32_t max_batch;      /* requests per launched batch */
     int32_t streams;        /* batches in flight at once */
     int32_t max_wait_us;    /* a partial batch waits at most this long for company (0: launch at once) */
-    int32_t _pad;
+    int32_t deadline_us;    /* a request not launched within this long of its submission completes with
+                               XC_E_TIMEOUT (0: no deadline); once launched it runs to its result */
 } xc_serve_cfg;
 
 typedef struct xc_serve_stats {
     int64_t submitted, completed, rejected, batches;
     int64_t spectra_hits, spectra_misses;
     int64_t arena_bytes, arena_peak;
+    int64_t timed_out, cancelled;
+    int64_t failed;         /* completed with an error of the device path (status < 0) */
 } xc_serve_stats;
 
 int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server **out);
@@ -750,6 +753,10 @@ int xc_server_create(xc_ctx *ctx, const xc_serve_cfg *cfg, xc_server **out);
 int xc_server_submit(xc_server *s, const xc_pair *pair, uint64_t *ticket);
 /* XC_OK with *res filled, or XC_E_NOTREADY. */
 int xc_server_poll(xc_server *s, uint64_t ticket, xc_result *res);
+/* Cancel a request that is still queued: it completes with XC_E_CANCELLED
+ * (XC_OK returned).  XC_E_NOTREADY when it is already on the device or done
+ * (its result stands); XC_E_ARG for an unknown ticket. */
+int xc_server_cancel(xc_server *s, uint64_t ticket);
 /* Block until the request completes; *latency_us (may be NULL) receives
  * submit-to-completion time as measured by the server. */
 int xc_server_wait(xc_server *s, uint64_t ticket, xc_result *res, double *latency_us);

@@ -3,6 +3,7 @@ import ctypes
 import glob
 import os
 import re
+import subprocess
 
 from alignment_projects_amd import abi, xcorr
 
@@ -23,21 +24,46 @@ def test_every_declared_symbol_is_exported():
     assert not missing, missing
 
 
-def test_struct_layouts_match_header():
-    # sizes of the POD structs the header defines (x86-64 / gfx950 share the layout)
-    assert ctypes.sizeof(abi.XcPoint) == 16
-    assert ctypes.sizeof(abi.XcRgnRule) == 56
-    assert ctypes.sizeof(abi.XcCntRule) == 24
-    assert ctypes.sizeof(abi.XcResult) == 72
-    assert ctypes.sizeof(abi.XcBucketInfo) == 32
-    assert ctypes.sizeof(abi.XcServeCfg) == 24
-    assert ctypes.sizeof(abi.XcServeStats) == 64
-    assert ctypes.sizeof(abi.XcAngleRec) == 32
-    assert ctypes.sizeof(abi.XcThmPair) == 6 * 4 + 2 * 8 + 6 * 8
-    assert ctypes.sizeof(abi.XcThmParams) == 8 + 8 * 4 + 8 + 12 * 8
-    assert ctypes.sizeof(abi.XcFourierQa) == 5 * 8 + 6 * 4
-    assert ctypes.sizeof(abi.XcSubimage) == 32
-    assert ctypes.sizeof(abi.XcPair) == 2 * 8 + 6 * 4 + 8 + 56 + 24
+# each POD struct of the header and its ctypes mirror
+MIRRORS = {"xc_point": "XcPoint", "xc_rgn_rule": "XcRgnRule", "xc_cnt_rule": "XcCntRule", "xc_result": "XcResult",
+           "xc_pair": "XcPair", "xc_bucket_info": "XcBucketInfo", "xc_angle_rec": "XcAngleRec",
+           "xc_thm_params": "XcThmParams", "xc_subimage": "XcSubimage", "xc_fourier_qa": "XcFourierQa",
+           "xc_thmpair": "XcThmPair", "xc_serve_cfg": "XcServeCfg", "xc_serve_stats": "XcServeStats"}
+
+
+def _header_layouts(tmp_path):
+    """sizeof and offsetof of every field of the mirrored structs, as the C compiler lays out the header."""
+    hdr = open(os.path.join(ROOT, "include", "xcorr.h")).read()
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "xcorr.h"', "int main(void) {"]
+    for cname, pyname in MIRRORS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in getattr(abi, pyname)._fields_:
+            if re.search(rf"\b{f}\b", hdr):
+                lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    lay = {}
+    for ln in out.split("\n"):
+        if ln:
+            c, f, v = ln.split()
+            lay[(c, f)] = int(v)
+    return lay
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors have the C compiler's size and field offsets for every struct of the header."""
+    lay = _header_layouts(tmp_path)
+    for cname, pyname in MIRRORS.items():
+        cls = getattr(abi, pyname)
+        assert ctypes.sizeof(cls) == lay[(cname, "size")], (cname, ctypes.sizeof(cls), lay[(cname, "size")])
+        for f, _ in cls._fields_:
+            if (cname, f) in lay:
+                assert getattr(cls, f).offset == lay[(cname, f)], (cname, f)
+    assert lay[("xc_serve_stats", "failed")] == 80 and lay[("xc_serve_cfg", "deadline_us")] == 20
 
 
 def test_version_and_sizes_without_device():

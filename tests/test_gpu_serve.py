@@ -93,3 +93,77 @@ def test_budget_rejects_oversized_request(ctx):
     assert srv.stats()["rejected"] == 1
     srv.close()
     a.release()
+
+
+def test_failed_batch_completes_with_error_and_recovers(ctx, monkeypatch):
+    """A batch that fails after its requests left the queue (its pinned staging refused, forced from the
+    host by the server's fault-injection setting) completes them with the error instead of leaving their
+    waiters blocked, and gives back its memory: the spectra it was to produce are dropped, so the same pair
+    submitted again is computed afresh and matches a plan bit for bit."""
+    tiles = _grid_tiles(ctx, n=3, seed=13)
+    rgn = abi.rgn_big_enough(16, 384 * 64 // 8)
+    p = xcorr.make_pair(tiles[0], tiles[1], rgn=rgn)
+    ref = ctx.plan([p]).run()[0]
+    monkeypatch.setenv("XC_SERVE_FAIL_PINNED", "1")
+    srv = xcorr.Server(ctx, hbm_budget=1 << 30, max_batch=4, streams=1, max_wait_us=0)
+    monkeypatch.delenv("XC_SERVE_FAIL_PINNED")
+    r, _ = srv.wait(srv.submit(p), raise_errors=False)
+    assert r.status == abi.XC_E_NOMEM
+    st = srv.stats()
+    assert st["failed"] == 1 and st["arena_bytes"] == 0, st
+    again, _ = srv.wait(srv.submit(p))
+    assert _same(again, ref), (again.as_dict(), ref.as_dict())
+    st = srv.stats()
+    assert st["completed"] == 1 and st["spectra_misses"] == 4, st
+    srv.close()
+    for t in tiles:
+        t.release()
+
+
+def test_deadline_times_out_queued_requests(ctx):
+    """Requests that wait in the queue past the server's deadline complete with XC_E_TIMEOUT (forced from
+    the host: a 1 us deadline against a 200 ms wait for batch company); with a generous deadline the same
+    requests run."""
+    tiles = _grid_tiles(ctx, n=3, seed=14)
+    rgn = abi.rgn_big_enough(16, 384 * 64 // 8)
+    pairs = [xcorr.make_pair(tiles[0], tiles[1], rgn=rgn), xcorr.make_pair(tiles[1], tiles[2], rgn=rgn)]
+    srv = xcorr.Server(ctx, hbm_budget=1 << 30, max_batch=64, streams=1, max_wait_us=200000, deadline_us=1)
+    out = [srv.wait(t, raise_errors=False)[0] for t in [srv.submit(q) for q in pairs]]
+    assert [r.status for r in out] == [abi.XC_E_TIMEOUT] * 2
+    assert srv.stats()["timed_out"] == 2
+    srv.close()
+    ref = ctx.plan(pairs).run()
+    srv = xcorr.Server(ctx, hbm_budget=1 << 30, max_batch=64, streams=1, max_wait_us=1000, deadline_us=10_000_000)
+    out = [srv.wait(t)[0] for t in [srv.submit(q) for q in pairs]]
+    assert all(_same(a, b) for a, b in zip(out, ref))
+    assert srv.stats()["timed_out"] == 0
+    srv.close()
+    for t in tiles:
+        t.release()
+
+
+def test_cancel_queued_request(ctx):
+    """A request cancelled while queued completes with XC_E_CANCELLED; the others of its would-be batch
+    run as usual; a finished request can no longer be cancelled."""
+    tiles = _grid_tiles(ctx, n=4, seed=15)
+    rgn = abi.rgn_big_enough(16, 384 * 64 // 8)
+    pairs = [xcorr.make_pair(tiles[i], tiles[i + 1], rgn=rgn) for i in range(3)]
+    ref = ctx.plan(pairs).run()
+    srv = xcorr.Server(ctx, hbm_budget=1 << 30, max_batch=64, streams=1, max_wait_us=2_000_000)
+    tk = [srv.submit(q) for q in pairs]
+    assert srv.cancel(tk[1]) == abi.XC_OK
+    assert srv.cancel(tk[1]) == abi.XC_E_NOTREADY  # done (cancelled) already
+    r1, _ = srv.wait(tk[1], raise_errors=False)
+    assert r1.status == abi.XC_E_CANCELLED
+    r0, _ = srv.wait(tk[0])
+    while srv.poll(tk[2]) is None:
+        time.sleep(0.001)
+    assert srv.cancel(tk[2]) == abi.XC_E_NOTREADY
+    r2, _ = srv.wait(tk[2])
+    assert _same(r0, ref[0]) and _same(r2, ref[2])
+    assert srv.cancel(tk[0]) == abi.XC_E_ARG  # collected by wait: unknown now
+    st = srv.stats()
+    assert st["cancelled"] == 1 and st["completed"] == 2, st
+    srv.close()
+    for t in tiles:
+        t.release()

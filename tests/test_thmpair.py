@@ -40,6 +40,21 @@ def test_write_format_and_round_trip(tmp_path):
         assert all(abs(g.T[i] - T[i]) <= 5e-7 for i in range(6))
 
 
+def test_long_fields_are_written_whole(tmp_path):
+    """WriteThmPair's fprintf writes any value: a diverged transform entry near 1e300 gives a %f field of
+    ~300 digits, and the line is still written whole in one append."""
+    p = tmp_path / "ThmPair_9^8.txt"
+    xcorr.thmpair_create(p)
+    T = [1.0, -2.5e300, 3.0, 1e300, -0.5, 7.25e250]
+    xcorr.thmpair_append(p, 1, 2, 3, 4, 0, -1e200, 0.25, T)
+    xcorr.thmpair_append(p, 5, 6, 7, 8, 1, 0.5, 0.75, [0.0] * 6)
+    line = FMT % (1, 2, 3, 4, 0, -1e200, 0.25, *T)
+    assert len(line) > 1024
+    assert p.read_text() == HDR + line + FMT % (5, 6, 7, 8, 1, 0.5, 0.75, *([0.0] * 6))
+    got = xcorr.thmpair_read_all(p)
+    assert len(got) == 2 and got[0].T[1] == -2.5e300 and got[0].A == -1e200 and got[1].atl == 5
+
+
 def test_find_and_missing(tmp_path):
     p = tmp_path / "ThmPair_1^0.txt"
     xcorr.thmpair_create(p)

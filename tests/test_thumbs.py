@@ -70,8 +70,32 @@ def test_make_thumbs_matches_oracle(ctx):
         # Normalize's one-pass variance (sum of squares minus n mean^2) turns the different summation
         # order of the device's tree into ~1e-12 here (mean 100, sd 20): 1e-10
         assert np.array_equal(gp, o_p) and np.abs(gv - oracle.normalize(o_v)).max() <= 1e-10
-    st = ctx.make_thumbs(pa, va, 240, 160, pb, vb, 260, 150, 4, 64, 8000 * 16)[0]
-    assert st == abi.XC_TOO_SMALL
+    # too small after decimation: the size test comes before Normalize, so both lists come back decimated
+    # and not normalised (CThmUtil.cpp:496-503)
+    st, ga_p, ga_v, gb_p, gb_v, olap, req = ctx.make_thumbs(pa, va, 240, 160, pb, vb, 260, 150, 4, 64, 8000 * 16)
+    assert st == abi.XC_TOO_SMALL and req == 8000
+    for (gp, gv), (p, v, w, h) in (((ga_p, ga_v), (pa, va, 240, 160)), ((gb_p, gb_v), (pb, vb, 260, 150))):
+        o_p, o_v = oracle.decimate_vector(p, v, w, h, 4)
+        assert np.array_equal(gp, o_p) and np.array_equal(gv, o_v)
+
+
+@pytest.mark.gpu
+def test_make_thumbs_stdev_failures_leave_the_reference_contents(ctx):
+    """Normalize A, then B (CThmUtil.cpp:505-519): a constant A is normalised into 0/0 (NaN) and B is left
+    as given; a constant B comes after A's normalisation, so A is normalised and B turns NaN."""
+    A, B = _raster(9, 40, 48), _raster(10, 40, 48)
+    pa, va = synth.raster_points(A)
+    pb, vb = synth.raster_points(B)
+    const = np.full(len(va), 7.0)
+    st, _, ga_v, _, gb_v, _, _ = ctx.make_thumbs(pa, const, 48, 40, pb, vb, 48, 40, 1, 10, 100)
+    assert st == abi.XC_NO_PEAK and np.isnan(ga_v).all() and np.array_equal(gb_v, vb)
+    st, _, ga_v, _, gb_v, _, _ = ctx.make_thumbs(pa, va, 48, 40, pb, const, 48, 40, 1, 10, 100)
+    assert st == abi.XC_NO_PEAK and np.isnan(gb_v).all()
+    assert np.abs(ga_v - oracle.normalize(va)).max() <= 1e-10
+    # with decimation: a constant A fails after both lists were decimated; B stays decimated, unnormalised
+    st, ga_p, ga_v, gb_p, gb_v, _, _ = ctx.make_thumbs(pa, const, 48, 40, pb, vb, 48, 40, 2, 10, 100)
+    o_p, o_v = oracle.decimate_vector(pb, vb, 48, 40, 2)
+    assert st == abi.XC_NO_PEAK and np.isnan(ga_v).all() and np.array_equal(gb_p, o_p) and np.array_equal(gb_v, o_v)
 
 
 def _hip():
@@ -124,5 +148,15 @@ def test_thumbs_edges(ctx):
     assert np.abs(ga_v - oracle.normalize(va)).max() <= 1e-10
     st = ctx.make_thumbs(pa, np.full(len(va), 7.0), 40, 30, pa, va, 40, 30, 1, 10, 100)[0]
     assert st == abi.XC_NO_PEAK
-    with pytest.raises(XcorrError):
-        ctx.thumb_raster_dev(1, abi.XC_F32, 64, 64, 64, 16, 1)  # refused before any device access
+    # a 16 x 16 bin would wrap the reference's 8-bit cell counter: refused on the host (real device buffers,
+    # so that a regression of the host check could only compute, never touch a bad address)
+    import ctypes as C
+    hip = _hip()
+    d_in, d_out = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(d_in), 64 * 64 * 4) == 0 and hip.hipMalloc(C.byref(d_out), 4 * 4 * 8) == 0
+    try:
+        with pytest.raises(XcorrError):
+            ctx.thumb_raster_dev(d_in, abi.XC_F32, 64, 64, 64, 16, d_out)
+    finally:
+        hip.hipFree(d_in)
+        hip.hipFree(d_out)
