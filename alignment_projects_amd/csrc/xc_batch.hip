@@ -54,23 +54,61 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     if (!es || w <= 0 || h <= 0) return set_err(ctx, XC_E_ARG, "bad tile dtype or extent");
     if (stride <= 0) stride = w;
     if (stride < w) return set_err(ctx, XC_E_ARG, "stride < width");
+    if (w > TP_MAX_W) return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
     std::lock_guard<std::mutex> lock(ctx->mu);
     XC_HIP(ctx, hipSetDevice(ctx->device));
+    const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
+    const bool tab32 = (flags & XC_TILE_F32_TABLES) != 0;
+    // per-row present extents, and the bounding box of the present pixels: the raster the reference builds
+    // from the point list spans that box (BBoxFromPoints, Geometry.cpp:78-106), so the tile's view does too
+    std::vector<int> rx0(h, 0), rx1(h, w - 1);
+    int L = 0, R = w - 1, Bt = 0, T = h - 1;
+    if (mask) {
+        std::vector<uint8_t> hm;
+        const uint8_t *m = mask;
+        if (dev_src) {
+            hm.resize((size_t)w * h);
+            if (hipMemcpy(hm.data(), mask, hm.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return set_err(ctx, XC_E_DEVICE, "tile mask download failed");
+            m = hm.data();
+        }
+        L = w;
+        R = -1;
+        Bt = -1;
+        for (int y = 0; y < h; ++y) {
+            int a = -1, b = -1;
+            for (int x = 0; x < w; ++x)
+                if (m[(size_t)y * w + x]) {
+                    if (a < 0) a = x;
+                    b = x;
+                }
+            rx0[y] = a;
+            rx1[y] = b;
+            if (a >= 0) {
+                L = std::min(L, a);
+                R = std::max(R, b);
+                if (Bt < 0) Bt = y;
+                T = y;
+            }
+        }
+        if (R < 0) return set_err(ctx, XC_E_ARG, "tile has no present pixel");
+    }
+    const int vw = R - L + 1, vh = T - Bt + 1;
     static std::atomic<uint64_t> next_uid{1};
     xc_tile *t = new xc_tile;
     t->uid = next_uid++;
     t->ctx = ctx;
-    t->w = w;
-    t->h = h;
+    t->w = vw;
+    t->h = vh;
+    t->ox = L;
+    t->oy = Bt;
+    t->fw = w;
+    t->fh = h;
     t->dtype = dtype;
     t->flags = flags;
-    const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
-    const bool tab32 = (flags & XC_TILE_F32_TABLES) != 0;
-    const TileSizes ts = tile_sizes(w, h, tab32);
-    if (w > TP_MAX_W) {
-        delete t;
-        return set_err(ctx, XC_E_SHAPE, "tile wider than %d", TP_MAX_W);
-    }
+    t->rx0.swap(rx0);
+    t->rx1.swap(rx1);
+    const TileSizes ts = tile_sizes(vw, vh, tab32);
     Layout lay;
     size_t iS = lay.add(ts.SQ), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
     size_t iM = lay.add(ts.msd), iD = lay.add(sizeof(TileDev)), iL = lay.add(sizeof(int));
@@ -101,17 +139,26 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
             }
         }
     }
+    // the whole picture (rotated copies map the points' own coordinates) ...
+    ImgView &F = t->full;
+    memset(&F, 0, sizeof F);
+    F.data = t->raw;
+    F.mask = t->mask;
+    F.dtype = dtype;
+    F.w = w;
+    F.h = h;
+    F.stride = stride;
+    F.mstride = w;
+    F.msd = (double *)(base + lay.off[iM]);
+    // ... and the bounding-box view the correlations and the tables use
     TileDev &D = t->dev;
     memset(&D, 0, sizeof D);
-    D.v.data = t->raw;
-    D.v.mask = t->mask;
-    D.v.dtype = dtype;
-    D.v.w = w;
-    D.v.h = h;
-    D.v.stride = stride;
-    D.v.mstride = w;
+    D.v = F;
+    D.v.data = (const char *)t->raw + (size_t)es * ((size_t)Bt * stride + L);
+    D.v.mask = t->mask ? t->mask + (size_t)Bt * w + L : nullptr;
+    D.v.w = vw;
+    D.v.h = vh;
     D.msd = (double *)(base + lay.off[iM]);
-    D.v.msd = D.msd;
     D.SQ = base + lay.off[iS];
     D.Z = (int32_t *)(base + lay.off[iZ]);
     D.part = (double *)(base + lay.off[iP]);
@@ -126,37 +173,13 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
         return set_err(ctx, XC_E_DEVICE, "tile descriptor upload failed");
     }
     t->bytes = (int64_t)t->mem.bytes;
-    // per-row present extents (rotated bounding boxes are computed from them)
-    t->rx0.assign(h, 0);
-    t->rx1.assign(h, w - 1);
-    if (mask) {
-        std::vector<uint8_t> hm((size_t)w * h);
-        if (dev_src) {
-            if (hipMemcpy(hm.data(), mask, hm.size(), hipMemcpyDeviceToHost) != hipSuccess) {
-                xc_tile_release(ctx, t);
-                return set_err(ctx, XC_E_DEVICE, "tile mask download failed");
-            }
-        } else {
-            memcpy(hm.data(), mask, hm.size());
-        }
-        for (int y = 0; y < h; ++y) {
-            int a = -1, b = -1;
-            for (int x = 0; x < w; ++x)
-                if (hm[(size_t)y * w + x]) {
-                    if (a < 0) a = x;
-                    b = x;
-                }
-            t->rx0[y] = a;
-            t->rx1[y] = b;
-        }
-    }
     // prepare: statistics + summed-area tables of the FP32 values
     Exec ex;
     ex.ntiles = 1;
     ex.d_tiles = t->d_dev;
     SatGroup sg;
-    sg.cpt = sat_cpt(w);
-    sg.threads = sat_threads(w);
+    sg.cpt = sat_cpt(vw);
+    sg.threads = sat_threads(vw);
     sg.count = 1;
     sg.tab32 = tab32;
     sg.list = (const int *)(base + lay.off[iL]);
@@ -198,7 +221,7 @@ extern "C" int xc_plan_destroy(xc_plan *p);
 static void rotated_bbox(const xc_tile *t, double t0, double t1, double t3, double t4, int &L, int &R, int &B, int &T)
 {
     double xlo = 1e300, xhi = -1e300, ylo = 1e300, yhi = -1e300;
-    for (int y = 0; y < t->h; ++y) {
+    for (int y = 0; y < t->fh; ++y) {
         const int a = t->rx0[y], b = t->rx1[y];
         if (a < 0) continue;
         for (int x : {a, b}) {
@@ -323,7 +346,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
                 tile_idx[t] = (int)tiles.size();
                 tiles.push_back(t);
             }
-        int wa = a->w, ha = a->h, offx = 0, offy = 0;
+        // B2.L - B1.L, B2.B - B1.B: the corners of the two views' bounding boxes
+        int wa = a->w, ha = a->h, offx = b->ox - a->ox, offy = b->oy - a->oy;
         RotKey key{a, {1.0, 0.0, 0.0, 1.0}};
         bool inv_t = true;  // a rotation: its inverse is its transpose
         if (aff4) {
@@ -343,7 +367,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             if (it == rot_idx.end()) {
                 RotDev R;
                 memset(&R, 0, sizeof R);
-                R.src = a->dev.v;
+                R.src = a->full;  // the rotation maps A's points in their own coordinates
                 R.t0 = key.m[0];
                 R.t1 = key.m[1];
                 R.t3 = key.m[2];
@@ -380,8 +404,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             if (q.cnt.enabled) rot_cnt[ri] = true;
             wa = rots[ri].w;
             ha = rots[ri].h;
-            offx = -rots[ri].L;  // B2.L - B1.L with B's corner at 0
-            offy = -rots[ri].B;
+            offx = b->ox - rots[ri].L;  // B2.L - B1.L
+            offy = b->oy - rots[ri].B;
         }
         PairDev &D = P[i];
         memset(&D, 0, sizeof D);

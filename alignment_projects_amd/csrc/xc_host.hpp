@@ -26,7 +26,11 @@ struct DevBuf {
 // A device-resident image (see xc_tile_put).
 struct xc_tile {
     xc_ctx *ctx = nullptr;
-    int w = 0, h = 0, dtype = 0, flags = 0;
+    int w = 0, h = 0;           // the view: bounding box of the present pixels
+    int ox = 0, oy = 0;         // its corner in the picture
+    int fw = 0, fh = 0;         // the whole picture
+    int dtype = 0, flags = 0;
+    xc::ImgView full;           // the whole picture (rotated copies read it)
     void *raw = nullptr;       // device pixels (owned unless adopted)
     bool own_raw = false;
     uint8_t *mask = nullptr;   // device mask (owned unless adopted)
@@ -36,7 +40,7 @@ struct xc_tile {
     xc::TileDev *d_dev = nullptr;  // device copy
     int64_t bytes = 0;
     uint64_t uid = 0;           // process-unique id (caches key on it, never on the address)
-    std::vector<int> rx0, rx1;  // per row: first and last present x (-1: none)
+    std::vector<int> rx0, rx1;  // per picture row: first and last present x (-1: none)
 };
 
 // A tuned column-pass shape (see XC_PLAN_AUTOTUNE).

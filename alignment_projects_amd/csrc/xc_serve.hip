@@ -91,6 +91,8 @@ int make_pair_dev(xc_server *S, const xc_pair &q, PairDev &D)
     D.radius = q.radius;
     D.Rx = q.Rx;
     D.Ry = q.Ry;
+    D.offx = q.b->ox - q.a->ox;  // B2.L - B1.L of the views' bounding boxes
+    D.offy = q.b->oy - q.a->oy;
     D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
     D.need_cnt = D.rule.cnt_enabled;
     D.tab32 = q.a->dev.tab32;

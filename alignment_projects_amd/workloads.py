@@ -145,6 +145,19 @@ def uneven_pairs(n, seed0=4000, lo=128, hi=1024, smax=50, noise=12.0, fsize=6144
     return out
 
 
+def uneven_pair_sized(seed, w, h, smax=50, noise=12.0):
+    """One pair of the uneven batch's kind at a chosen size (w x h), with its own field: (A, mA, B, mB,
+    (sx, sy)) device tensors as uneven_pairs gives them."""
+    g = _gen(seed)
+    mg = smax + 8
+    Fd = field(g, h + 2 * mg, w + 2 * mg, 128.0, 40.0)
+    sx, sy = (int(v) for v in torch.randint(-smax, smax + 1, (2,), generator=g, device=_DEV))
+    a = quantize(Fd[mg:mg + h, mg:mg + w], 255).to(torch.uint8).contiguous()
+    nb = torch.randn((h, w), generator=g, device=_DEV) * noise
+    b = quantize(Fd[mg - sy:mg - sy + h, mg - sx:mg - sx + w] + nb, 255).to(torch.uint8).contiguous()
+    return a, _holes(g, h, w), b, _holes(g, h, w), (sx, sy)
+
+
 # ----------------------------------------------------------------------------
 # Serving stream: tiles on small tori of neighbours (each tile referenced by
 # four requests: its right, left, lower and upper neighbour pairs).

@@ -244,7 +244,11 @@ enum xc_tile_flags {
  * keeps its pixels, (mean, sd) and its summed-area tables resident in HBM:
  * S and S^2 as interleaved FP64 pairs (IntegrateImage's double tables,
  * Correlation.cpp:49-105), or FP32 pairs with XC_TILE_F32_TABLES, plus the
- * int32 non-zero count table.
+ * int32 non-zero count table.  The tile's raster is the bounding box of its
+ * present pixels, as the reference's raster spans the bounding box of the
+ * point list (BBoxFromPoints, Geometry.cpp:78-106): a mask whose edge rows
+ * or columns are empty shrinks it; displacements stay in the picture's
+ * coordinates.  XC_E_ARG for a mask with no present pixel.
  */
 int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int h, int stride,
                 const uint8_t *mask, int flags, xc_tile **out);

@@ -14,7 +14,17 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
-    """One xc_ctx on device 0 for the GPU tests (fails loudly if the library or device is missing)."""
+    """One xc_ctx on device 0 for the GPU tests (fails loudly if the library or device is missing).
+
+    torch (whose ROCm build carries its own HIP runtime) is initialised on the device first, as the
+    benchmark does: the workload generators make their inputs with it, and it does not find the device
+    once the library's runtime has opened it."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception as e:  # the library's own tests do not need torch
+        print(f"torch device init skipped: {e}")
     from alignment_projects_amd import xcorr
     c = xcorr.Context(0)
     yield c

@@ -147,3 +147,56 @@ def test_autotuned_plan_matches_default(ctx):
     for ta, tb in tiles:
         ta.release()
         tb.release()
+
+
+def test_masks_with_empty_edges_match_point_lists(ctx):
+    """A mask whose edge rows / columns are empty: the reference's raster spans the bounding box of the point
+    list, so the batched tile's view shrinks to it (n, overlap widths, offsets); batched = plain FP64 path =
+    oracle on the same point lists, for plain pairs and for rotated copies (RFromAngle)."""
+    from alignment_projects_amd.xcorr import make_pair
+    rng = np.random.default_rng(77)
+    F = synth.bandlimited(rng, 220, 300, 128, 40)
+    A = synth.quantize(F[10:170, 20:260], np.uint8)
+    B = synth.quantize(np.roll(F, (4, -7), axis=(0, 1))[10:170, 20:260] + rng.normal(0, 6, (160, 240)), np.uint8)
+    ma = np.ones(A.shape, np.uint8)
+    mb = np.ones(B.shape, np.uint8)
+    ma[:3, :] = 0
+    ma[:, -5:] = 0
+    ma[40:60, 100:130] = 0
+    mb[:, :2] = 0
+    mb[-4:, :] = 0
+    pa, va = synth.raster_points(A, ma)
+    pb, vb = synth.raster_points(B, mb)
+    va, vb = synth.normalize(va), synth.normalize(vb)
+    rgn, cnt = abi.rgn_big_enough(40, 240 * 160 // 4), abi.cnt_enough_points(240 * 160 // 5)
+    ta, tb = ctx.tile_put(A, mask=ma), ctx.tile_put(B, mask=mb)
+    r = ctx.batch_corr([make_pair(ta, tb, rgn=rgn, cnt=cnt)])[0]
+    g = ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn, cnt=cnt)
+    o = oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn, cnt=cnt)
+    assert (g[4].lag_x, g[4].lag_y) == (o[4].lag_x, o[4].lag_y) and abs(g[1] - o[1]) <= 1e-9
+    assert (g[4].nx, g[4].ny) == (r.nx, r.ny)
+    assert r.status == o[0] == abi.XC_OK
+    assert (r.lag_x, r.lag_y) == (o[4].lag_x, o[4].lag_y), (r.as_dict(), o[4].as_dict())
+    assert abs(r.dx - o[2]) <= SUBPX and abs(r.dy - o[3]) <= SUBPX and abs(r.r - o[1]) <= RTOL32
+    assert abs(r.dx + 7) <= 1 and abs(r.dy - 4) <= 1  # B(p) = A(p + (7, -4)): lag (-7, 4)
+    ta.release()
+    tb.release()
+    # rotated copies of a tile with empty edges: the rotation maps the points' own coordinates
+    Af = A.astype(np.float64)
+    Bf = B.astype(np.float64)
+    ta = ctx.tile_put(Af, mask=ma, normalize=True)
+    tb = ctx.tile_put(Bf, mask=mb, normalize=True)
+    h, w = A.shape
+    req, olap = int(0.2 * w * h), 40
+    angles = [-1.0, 0.0, 1.5]
+    res = ctx.sweep_angles(ta, tb, angles, rgn=abi.rgn_big_enough(olap, req), cnt=abi.cnt_enough_points(req))
+    for ang, rr in zip(angles, res):
+        st, R, X, Y, oo = oracle.rfromangle(ang, pa, va, pb, vb, olap, req)
+        assert rr.status == st, (ang, rr.status, st)
+        if st != abi.XC_OK:
+            continue
+        assert (rr.lag_x, rr.lag_y) == (oo.lag_x, oo.lag_y), (ang, rr.as_dict(), oo.as_dict())
+        assert abs(rr.dx - X) <= SUBPX and abs(rr.dy - Y) <= SUBPX, (ang, rr.dx, rr.dy, X, Y)
+        assert abs(rr.r - R) <= 1e-3 * max(1.0, abs(R)), (ang, rr.r, R)
+    ta.release()
+    tb.release()
