@@ -317,3 +317,73 @@ def xscaled_pair(seed=12, size=96, s=1.04):
     A = synth.normalize(A.ravel()).reshape(A.shape)
     B = synth.normalize(B.ravel()).reshape(B.shape)
     return A, B
+
+
+def p2i_filter(N):
+    """CorrPatchToImage's low-pass over the [N][N/2+1] half spectrum (Correlation.cpp:1255-1283): the first
+    index (the row frequency, signed) and the second (the column frequency, 0..N/2) give
+    exp(-(ky^2 + kx^2) / 1e4), zero where that exponent exceeds 10."""
+    ky = np.arange(N)
+    ky = np.where(ky >= N // 2, ky - N, ky).astype(np.float64)[:, None]
+    kx = np.arange(N // 2 + 1, dtype=np.float64)[None, :]
+    rad2 = (ky * ky + kx * kx) / 10000.0
+    return np.where(rad2 > 10.0, 0.0, np.exp(-rad2))
+
+
+def p2i_surface(A, i2, filt=None):
+    """Un-normalised correlation surface of CorrPatchToImage: N^2 sum_p A(p) i2((p + l) mod N) for the patch A
+    placed at the origin of the N x N grid (the reference's IFT_2D does not divide by N^2).  Without a filter
+    as a direct sum over the patch's pixels (no transform); with one through numpy's rfft2."""
+    N = i2.shape[0]
+    h1, w1 = A.shape
+    if filt is None:
+        c = np.zeros((N, N))
+        for y in range(h1):
+            for x in range(w1):
+                if A[y, x] != 0.0:
+                    c += A[y, x] * np.roll(i2, (-y, -x), axis=(0, 1))
+        return c * (N * N)
+    i1 = np.zeros((N, N))
+    i1[:h1, :w1] = A
+    P = np.fft.rfft2(i2) * np.conj(np.fft.rfft2(i1)) * filt
+    return np.fft.irfft2(P, s=(N, N)) * (N * N)
+
+
+def corr_patch_to_image(A, i2, Ox=0, Oy=0, radius=4096, bfilter=False, filt=None):
+    """CorrPatchToImage (Correlation.cpp:1186-1355) for a patch A (h1 x w1, every pixel a point, at the origin)
+    against the N x N image i2.  The candidates are the lags -(w1-1) <= x <= w2-1, -(h1-1) <= y <= h2-1 (w2, h2
+    from i2's last non-zero column and row) inside the disc, taken in the wrapped raster order of the surface;
+    the best divided by np1 N^2; ParabPeakFFT on the raw surface.  Returns (r, dx, dy, (lag x, lag y))."""
+    N = i2.shape[0]
+    h1, w1 = A.shape
+    ys, xs = np.nonzero(i2)
+    w2, h2 = int(xs.max()) + 1, int(ys.max()) + 1
+    if bfilter and filt is None:
+        filt = p2i_filter(N)
+    S = p2i_surface(A, i2, filt if bfilter else None)
+    cands = []
+    for y in range(-(h1 - 1), h2):
+        for x in range(-(w1 - 1), w2):
+            if (x - Ox) ** 2 + (y - Oy) ** 2 <= radius * radius:
+                cands.append(((y % N) * N + (x % N), x, y))
+    cands.sort()
+    vals = np.array([S[i // N, i % N] for i, _, _ in cands])
+    k = int(np.argmax(vals))
+    _, bx, by = cands[k]
+    best = vals[k] / (float(h1 * w1) * N * N)
+
+    def lk(x, y):
+        return S[y % N, x % N]
+
+    px, py = float(bx), float(by)
+    if bx - 1 >= 0 and bx + 1 < N and lk(bx - 1, by) != 0 and lk(bx + 1, by) != 0:
+        z0, z1, z2 = lk(bx - 1, by), lk(bx, by), lk(bx + 1, by)
+        d = (z0 - z2) / (2 * (z0 + z2 - z1 - z1))
+        if abs(d) <= 8:
+            px += d
+    if by - 1 >= 0 and by + 1 < N and lk(bx, by - 1) != 0 and lk(bx, by + 1) != 0:
+        z0, z1, z2 = lk(bx, by - 1), lk(bx, by), lk(bx, by + 1)
+        d = (z0 - z2) / (2 * (z0 + z2 - z1 - z1))
+        if abs(d) <= 8:
+            py += d
+    return best, px, py, (bx, by)

@@ -221,6 +221,63 @@ def test_bruteforce_detects_transposed_operands():
     assert R.shape != BR.shape or np.abs(R - BR).max() > 1e-3
 
 
+def _p2i_case(seed, N, h1, w1, h2, w2, shift):
+    """A patch cut from a white-noise field (sd 0.5: the un-normalised peak stays below the reference's 1.0001
+    check) and the image holding the field's h2 x w2 corner, embedded in N x N zeros; the patch's content sits
+    at `shift` in the image."""
+    rng = np.random.default_rng(seed)
+    F = 0.5 * rng.standard_normal((h2 + 8, w2 + 8))
+    i2 = np.zeros((N, N))
+    i2[:h2, :w2] = F[:h2, :w2]
+    sx, sy = shift
+    A = F[sy:sy + h1, sx:sx + w1] + 0.05 * rng.standard_normal((h1, w1))
+    return A, i2
+
+
+@pytest.mark.parametrize("seed,N,h1,w1,h2,w2,shift,disc", [
+    (1, 32, 7, 9, 20, 24, (5, 3), (0, 0, 4096)),
+    (2, 32, 9, 6, 14, 19, (9, 2), (8, 2, 3)),
+    (3, 64, 12, 15, 40, 48, (20, 17), (0, 0, 4096)),
+    (4, 16, 5, 5, 11, 9, (2, 4), (1, 3, 2)),
+])
+def test_corr_patch_to_image_vs_direct_sum(seed, N, h1, w1, h2, w2, shift, disc):
+    """CorrPatchToImage without the filter against the direct circular correlation sum (no transform) over the
+    lag box: lag exact, r and the subpixel peak within 1e-9; the planted shift is found without a disc."""
+    A, i2 = _p2i_case(seed, N, h1, w1, h2, w2, shift)
+    pa, va = synth.raster_points(A)
+    Ox, Oy, rad = disc
+    st, r, dx, dy, res = oracle.corr_patch_to_image(pa, va, i2, Ox, Oy, rad, False)
+    br, bx, by, lag = bf.corr_patch_to_image(A, i2, Ox, Oy, rad)
+    assert st == abi.XC_OK
+    assert (res.lag_x, res.lag_y) == lag, (res.as_dict(), lag)
+    assert abs(r - br) <= 1e-9 * max(1.0, abs(br)) and abs(dx - bx) <= 1e-9 and abs(dy - by) <= 1e-9
+    if rad >= 4096:
+        assert lag == shift
+
+
+def test_corr_patch_to_image_filter_vs_numpy():
+    """With bFilter, against numpy's rfft2 with the reference's filter (row frequency as the first index) on a
+    1024 grid, where the filter's cutoff (exponent > 10) is reached; the same filter with its two indices
+    exchanged gives a different surface, so the check tells them apart."""
+    N = 1024
+    A, i2 = _p2i_case(7, N, 40, 56, 300, 420, (150, 90))
+    pa, va = synth.raster_points(A)
+    st, r, dx, dy, res = oracle.corr_patch_to_image(pa, va, i2, 0, 0, 4096, True)
+    filt = bf.p2i_filter(N)
+    assert filt.min() == 0.0 and filt.max() == 1.0
+    br, bx, by, lag = bf.corr_patch_to_image(A, i2, bfilter=True, filt=filt)
+    assert st == abi.XC_OK and (res.lag_x, res.lag_y) == lag == (150, 90), (res.as_dict(), lag)
+    assert abs(r - br) <= 1e-9 * abs(br) and abs(dx - bx) <= 1e-9 and abs(dy - by) <= 1e-9
+    # the transposed filter (column frequency signed, row frequency not) on the same half spectrum
+    ky = np.arange(N, dtype=np.float64)[:, None]
+    kx = np.arange(N // 2 + 1)
+    kx = np.where(kx >= N // 2, kx - N, kx).astype(np.float64)[None, :]
+    rad2 = (ky * ky + kx * kx) / 10000.0
+    swapped = np.where(rad2 > 10.0, 0.0, np.exp(-rad2))
+    sr, _, _, _ = bf.corr_patch_to_image(A, i2, bfilter=True, filt=swapped)
+    assert abs(sr - r) > 1e-3 * abs(r)
+
+
 def test_ncc_formula_equals_pearson():
     rng = np.random.default_rng(6)
     for _ in range(20):

@@ -7,6 +7,8 @@ lstsq residual; at higher orders, the moment form of the sequential removal (its
 last term removed); the DoG against scipy's direct
 convolution, with the sign the kernel's rounding-noise sum gives it; the downsample against a reshape-mean.
 The device path is then held to the oracle on the same rasters."""
+import math
+
 import numpy as np
 import pytest
 from numpy.polynomial import legendre
@@ -76,6 +78,38 @@ def test_dog_kernel_shape_and_normalisation():
         assert K.shape == (6 * r2 + 1, 6 * r2 + 1)
         assert np.array_equal(K, K.T) and np.array_equal(K, K[::-1, ::-1])
         assert abs(K.sum()) <= 1e-14  # two unit-sum Gaussians: the sum is rounding noise
+
+
+@pytest.mark.parametrize("r1,r2", [(3, 6), (1, 2), (2, 5)])
+def test_dog_kernel_values_fixed_by_the_gaussians(r1, r2):
+    """The kernel's values against what the mathematics fixes for a difference of unit-sum Gaussians of
+    standard deviations r1 < r2 on the (6 r2 + 1)^2 grid:
+    - its two normalisers, read back from the kernel (the corner holds the outer Gaussian alone, the centre
+      1/sum1 - 1/sum2), are the squares of the 1-D sums of the sampled Gaussians (the 2-D sums separate);
+    - the reference's own check (CPixPair.cpp:58-61): sum / r^2 = 2 pi, to Poisson summation's 1e-6 for
+      the inner Gaussian (6 sd wide) and, for the outer one cut at 3 sd, 2 pi erf((3 r2 + 1/2) / (r2 sqrt 2))^2
+      (the midpoint sum of the truncated tails) to 1e-3;
+    - its transform is a band-pass whose peak lies at |k|^2 = ln(r2^2 / r1^2) / (2 pi^2 (r2^2 - r1^2)), with
+      the height exp(-2 pi^2 r1^2 k^2) - exp(-2 pi^2 r2^2 k^2) there, to 1 %."""
+    K = oracle.make_dog_kernel(r1, r2)
+    d = 3 * r2
+    x = np.arange(-d, d + 1)
+    sum2 = -np.exp(-(2.0 * d * d) / (2.0 * r2 * r2)) / K[0, 0]
+    sum1 = 1.0 / (K[d, d] + 1.0 / sum2)
+    s1 = np.exp(-x * x / (2.0 * r1 * r1)).sum() ** 2
+    s2 = np.exp(-x * x / (2.0 * r2 * r2)).sum() ** 2
+    assert abs(sum1 / s1 - 1) <= 1e-9 and abs(sum2 / s2 - 1) <= 1e-9
+    assert abs(sum1 / (r1 * r1) / (2 * np.pi) - 1) <= 1e-6
+    tail = math.erf((3 * r2 + 0.5) / (r2 * math.sqrt(2))) ** 2
+    assert abs(sum2 / (r2 * r2) / (2 * np.pi * tail) - 1) <= 1e-3
+    col = K.sum(axis=0)  # the transform along kx at ky = 0 (the kernel is even: a cosine sum)
+    ks = np.linspace(0.0, 0.5, 5001)
+    F = np.array([(col * np.cos(2 * np.pi * k * x)).sum() for k in ks])
+    kstar = math.sqrt(math.log(r2 * r2 / (r1 * r1)) / (2 * math.pi ** 2 * (r2 * r2 - r1 * r1)))
+    height = math.exp(-2 * math.pi ** 2 * r1 * r1 * kstar ** 2) - math.exp(-2 * math.pi ** 2 * r2 * r2 * kstar ** 2)
+    assert abs(ks[int(np.argmax(F))] / kstar - 1) <= 0.01
+    assert abs(F.max() / height - 1) <= 0.01
+    assert abs(F[0]) <= 1e-14  # unit-sum Gaussians: no DC
 
 
 @pytest.mark.parametrize("r1,r2", [(3, 6), (1, 2)])
