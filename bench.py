@@ -26,9 +26,13 @@ Workloads (--workload):
           sustained pairs/s.
 
 Printed: one JSON line per run with the throughput, accuracy against the
-FP64 oracle (and the plain FP64 GPU path for strips), the roofline of the
-dominant kernel (HIP events on the library stream) and a CPU baseline (the
-oracle on the host cores, FFTW3 API where available, on a bounded sample).
+FP64 oracle and the plain FP64 GPU path, the roofline of the dominant kernel
+(HIP events on the library stream) and a CPU baseline (the oracle on the host
+cores, FFTW3 API where available, on a bounded sample: one thread, and one
+pair per thread on all cores).  The default strips run also carries the
+accuracy of the sweep (all 1344 correlations) and of 1024 uneven pairs
+against the plain FP64 GPU path, and a short serving leg (p50 / p99 per
+request at 2000, 8000 and 20000 requests/s).
 """
 import argparse
 import json
@@ -41,6 +45,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
+
+# the CPU baseline's FFT library runs single-threaded per call, as the reference's install notes prescribe
+os.environ.setdefault("MKL_NUM_THREADS", "1")
 
 HBM_PEAK_GBS = 8000.0
 STAGES = ["prep", "row_r2c", "col_fused", "row_c2r_eval", "finalize"]
@@ -76,9 +83,13 @@ def parse():
     ap.add_argument("--height", type=int, default=256)
     ap.add_argument("--oracle-sample", type=int, default=16, help="pairs checked against the FP64 oracle")
     ap.add_argument("--cpu-sample", type=int, default=192, help="pairs timed on the CPU baseline")
-    ap.add_argument("--plain-sample", type=int, default=192, help="pairs run through the plain FP64 GPU path")
+    ap.add_argument("--plain-sample", type=int, default=256, help="pairs run through the plain FP64 GPU path")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-variant", action="store_true", help="skip the FP32-table variant of the strips")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the accuracy legs of the strips run (sweep and uneven against the plain FP64 path)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the strips run")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-core CPU baseline (0: the host's)")
     ap.add_argument("--stage-iters", type=int, default=5)
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="duration of each offered load")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -151,6 +162,53 @@ def accuracy(res, ref):
             "r_err_max": float(rerr.max()), "r_err_p99": float(np.percentile(rerr, 99))}
 
 
+def accuracy_recs(res, recs):
+    """Batched sweep results against the plain FP64 path's RFromAngle records (A, R, X, Y): the integer peak
+    agrees when both displacements are within half a pixel; subpixel error is the larger displacement
+    difference; the score error is relative."""
+    agree, sub, rerr = 0, [], []
+    for r, o in zip(res, recs):
+        dx, dy = abs(r.dx - o.X), abs(r.dy - o.Y)
+        if r.status == 0 and dx < 0.5 and dy < 0.5:
+            agree += 1
+            sub.append(max(dx, dy))
+        rerr.append(abs(r.r - o.R) / max(1.0, abs(o.R)))
+    sub = np.array(sub) if sub else np.array([np.nan])
+    rerr = np.array(rerr)
+    return {"n": len(recs), "peak_agree": agree / max(1, len(recs)), "subpx_max": float(sub.max()),
+            "subpx_p99": float(np.percentile(sub, 99)), "subpx_mean": float(sub.mean()),
+            "r_rel_err_max": float(rerr.max()), "r_rel_err_p99": float(np.percentile(rerr, 99))}
+
+
+def merge_accuracy(parts):
+    """Pooled agreement over several accuracy records (counts add; worst cases are the maxima)."""
+    n = sum(p["n"] for p in parts)
+    agree = sum(p["peak_agree"] * p["n"] for p in parts)
+    return {"n": n, "peak_agree": agree / max(1, n), "peaks_differing": int(round(n - agree)),
+            "subpx_max": max(p["subpx_max"] for p in parts)}
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(args):
+    """Threads of the all-core baseline: the host's cores, at most 16 (one GPU's share of a shared box)."""
+    n = args.cpu_threads or min(16, os.cpu_count() or 1)
+    try:
+        n = min(n, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        pass
+    return max(1, n)
+
+
 def fft_provider(oracle):
     try:
         return oracle.set_fft_provider(1)
@@ -162,6 +220,67 @@ def line_base(metric, value, unit, args, ms_step, config, dtype="f32"):
     return {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+
+
+# ---------------------------------------------------------------------------
+# accuracy legs: the other workloads' batched results against the plain FP64 GPU path
+# ---------------------------------------------------------------------------
+
+def accuracy_legs(ctx, sync):
+    """All 64 x 21 correlations of the thumbnail sweep and the first 1024 pairs of the uneven batch, batched
+    (FP32) against the plain FP64 GPU path (itself bit-exact in its integer peak against the oracle)."""
+    from alignment_projects_amd import abi, synth, workloads, xcorr
+    out = {}
+    t0 = time.time()
+    A, B, _ = workloads.thumb_pairs(ctx, 64)
+    angles = synth.sweep_angles()
+    h, w = A[0].shape
+    req, olap = int(0.25 * w * h), 64
+    rgn, cnt = abi.rgn_big_enough(olap, req), abi.cnt_enough_points(req)
+    A32 = [a.astype(np.float32) for a in A]
+    B32 = [b.astype(np.float32) for b in B]
+    tiles = [(ctx.tile_put(A32[i], normalize=False), ctx.tile_put(B32[i], normalize=False)) for i in range(len(A))]
+    pairs = [xcorr.make_pair(ta, tb, mode=abi.XC_MODE_S, rgn=rgn, cnt=cnt, angle=ang) for ta, tb in tiles
+             for ang in angles]
+    plan = ctx.plan(pairs)
+    res = plan.run(want=True)
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
+    recs = []
+    for i in range(len(A)):
+        pa, va = synth.raster_points(A32[i].astype(np.float64))
+        pb, vb = synth.raster_points(B32[i].astype(np.float64))
+        recs += [ctx.rfromangle(ang, pa, va, pb, vb, olap, req) for ang in angles]
+    out["sweep_vs_plain_fp64_gpu"] = accuracy_recs(res, recs)
+    log(f"accuracy, sweep: {out['sweep_vs_plain_fp64_gpu']} ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    data = workloads.uneven_pairs(1024)
+    sync()
+    tiles, pairs = [], []
+    for a, ma, b, mb, _ in data:
+        ta, tb = _adopt(ctx, a, ma), _adopt(ctx, b, mb)
+        tiles.append((ta, tb))
+        hh, ww = a.shape
+        pairs.append(xcorr.make_pair(ta, tb, cnt=abi.cnt_enough_points(ww * hh // 4)))
+    plan = ctx.plan(pairs, refresh_tiles=True)
+    res = plan.run(want=True)
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
+    ref = []
+    for a, ma, b, mb, _ in data:
+        hh, ww = a.shape
+        pa, va = synth.raster_points(a.cpu().numpy(), ma.cpu().numpy())
+        pb, vb = synth.raster_points(b.cpu().numpy(), mb.cpu().numpy())
+        ref.append(ctx.corr_patches(pa, synth.normalize(va), pb, synth.normalize(vb), 0, 0, 4096,
+                                    cnt=abi.cnt_enough_points(ww * hh // 4)))
+    out["uneven_vs_plain_fp64_gpu"] = accuracy(res, ref)
+    log(f"accuracy, uneven: {out['uneven_vs_plain_fp64_gpu']} ({time.time() - t0:.1f}s)")
+    del data
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -234,6 +353,8 @@ def run_strips(args, sync):
 
     # ---- plain FP64 GPU path on the same pairs (one pair per call) ----
     nplain = min(args.plain_sample, NP)
+    if not args.no_cpu:
+        nplain = NP  # the CPU baselines take their samples from these point lists
     pts = [(synth.raster_points(A[i]), synth.raster_points(B[i])) for i in range(nplain)]
     norm = [((pa, synth.normalize(va)), (pb, synth.normalize(vb))) for (pa, va), (pb, vb) in pts]
     ctx.corr_patches(norm[0][0][0], norm[0][0][1], norm[0][1][0], norm[0][1][1], 0, 0, 4096, rgn=rgn)
@@ -257,15 +378,34 @@ def run_strips(args, sync):
         prov = fft_provider(oracle)
         ncpu = min(args.cpu_sample, nplain)
         oracle.corr_patches(norm[0][0][0], norm[0][0][1], norm[0][1][0], norm[0][1][1], 0, 0, 4096, rgn=rgn)
+        per = []
         t1 = time.perf_counter()
         for (pa, va), (pb, vb) in norm[:ncpu]:
+            t2 = time.perf_counter()
             oracle.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
+            per.append(time.perf_counter() - t2)
         cs = time.perf_counter() - t1
+        single = {"value": round(ncpu / cs, 3), "unit": "pairs/s", "cores": 1, "seconds": round(cs, 2),
+                  "latency_ms": {"p50": round(1e3 * float(np.percentile(per, 50)), 2),
+                                 "p99": round(1e3 * float(np.percentile(per, 99)), 2)}}
+        log(f"cpu baseline, 1 thread: {single['value']} pairs/s")
+        # all cores: one pair per thread (the EZThreads / TCDGet pattern), FFT calls serialised by one mutex as
+        # in the reference's FFTW build; the oracle's calls release the interpreter lock
+        from concurrent.futures import ThreadPoolExecutor
+        nth = cpu_threads(args)
+        nall = min(NP, len(norm))
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=nth) as ex:
+            list(ex.map(lambda q: oracle.corr_patches(q[0][0], q[0][1], q[1][0], q[1][1], 0, 0, 4096, rgn=rgn),
+                        norm[:nall]))
+        ca = time.perf_counter() - t1
         oracle.set_fft_provider(0)
-        cpu = {"value": round(ncpu / cs, 3), "unit": "pairs/s", "cores": 1, "kind": "port",
-               "sample": f"{ncpu} of the {NP} strip pairs, CorrPatches restatement, FFT: {prov}, 1 thread",
-               "seconds": round(cs, 2)}
-        log(f"cpu baseline: {cpu['value']} pairs/s")
+        cpu = {"value": round(nall / ca, 3), "unit": "pairs/s", "cores": nth, "kind": "port",
+               "sample": f"{nall} of the {NP} strip pairs, one pair per thread on {nth} threads, CorrPatches "
+                         f"restatement, FFT: {prov} under one mutex, MKL_NUM_THREADS=1",
+               "seconds": round(ca, 2), "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "single_thread": single}
+        log(f"cpu baseline, {nth} threads: {cpu['value']} pairs/s")
 
     line = line_base("tile-pair correlations/sec (batched FP32, 2048x256 strip pairs)", value, "pairs/s", args,
                      ms_step, {"workload": "batched same-layer strip overlaps", "pairs": NP, "tile": f"{W}x{H} u8",
@@ -284,6 +424,14 @@ def run_strips(args, sync):
     if variant:
         variant["accuracy_vs_plain_fp64_gpu"] = accuracy(vres[:nplain], plain)
         line["variant_f32_tables"] = variant
+    if not args.no_accuracy:
+        acc.update(accuracy_legs(ctx, sync))
+        acc["all_vs_plain_fp64_gpu"] = merge_accuracy([acc["vs_plain_fp64_gpu"], acc["sweep_vs_plain_fp64_gpu"],
+                                                       acc["uneven_vs_plain_fp64_gpu"]])
+    if not args.no_serve:
+        runs, st, cfg = serve_leg(ctx, sync, [2000, 8000, 20000], 0.5, n_uneven=16, n_strips=6, burst=False)
+        line["serving"] = {"config": cfg, "runs": runs, "server_stats": st,
+                           "latency_ms_at_20000_req_s": {"p50": runs[2]["p50_ms"], "p99": runs[2]["p99_ms"]}}
     if cpu:
         line["batched_over_cpu"] = round(value / cpu["value"], 1)
     plan.destroy()
@@ -450,17 +598,18 @@ def run_uneven(args, sync):
 # serve: continuous batching under Poisson arrivals
 # ---------------------------------------------------------------------------
 
-def run_serve(args, sync):
+def serve_leg(ctx, sync, rates, seconds, n_uneven=40, n_strips=12, burst=True):
+    """Poisson arrivals at each rate (and a burst) over the 70/30 mix of uneven tiles and 2048 x 256 strips on
+    small tori of neighbouring tiles.  Returns (runs, server stats, config)."""
     import torch
 
     from alignment_projects_amd import abi, workloads, xcorr
 
     t0 = time.time()
     rng = np.random.default_rng(5000)
-    ctx = xcorr.Context(0)
     keep, tiles, pools = [], [], {"uneven": [], "strips": []}
-    for li in range(40 + 12):
-        kind = "uneven" if li < 40 else "strips"
+    for li in range(n_uneven + n_strips):
+        kind = "uneven" if li < n_uneven else "strips"
         if kind == "uneven":
             w, h = (int(v) for v in rng.integers(128, 1025, size=2))
             tl, reqs = workloads.lattice(6000 + li, w, h, jitter=25)
@@ -532,27 +681,36 @@ def run_serve(args, sync):
 
     offered(2000, 0.3, 1)  # warm-up: kernels loaded, spectra resident
     runs = []
-    for k, rate in enumerate([2000, 8000, 20000, 0]):
-        r = offered(rate, args.serve_seconds, 10 + k)
+    for k, rate in enumerate(list(rates) + ([0] if burst else [])):
+        r = offered(rate, seconds, 10 + k)
         log(json.dumps(r))
         runs.append(r)
     st = srv.stats()
     srv.close()
+    for t in tiles:
+        t.release()
+    del keep
+    cfg = {"mix": "70% w,h U{128..1024} / 30% 2048x256 strips", "tiles": len(tiles),
+           "budget_GB": round(budget / 1e9, 1), "max_batch": 64, "streams": 3, "max_wait_us": 200,
+           "seconds_per_rate": seconds}
+    return runs, st, cfg
+
+
+def run_serve(args, sync):
+    from alignment_projects_amd import xcorr
+
+    ctx = xcorr.Context(0)
+    runs, st, cfg = serve_leg(ctx, sync, [2000, 8000, 20000], args.serve_seconds)
+    ctx.close()
     value = max(r["sustained_pairs_s"] for r in runs)
     line = line_base("served tile-pair correlations/sec (continuous batching, FP32)", value, "pairs/s", args, 0.0,
-                     {"workload": "serving stream", "mix": "70% w,h U{128..1024} / 30% 2048x256 strips",
-                      "tiles": len(tiles), "budget_GB": round(budget / 1e9, 1), "max_batch": 64, "streams": 3,
-                      "max_wait_us": 200})
+                     dict({"workload": "serving stream"}, **cfg))
     line["ms_per_step"] = None
     line["serving"] = runs
     line["server_stats"] = st
     line["latency_ms"] = {"p50": runs[2]["p50_ms"], "p99": runs[2]["p99_ms"], "unit": "per request at 20000 req/s"}
     line["roofline"] = None
     line["cpu_baseline"] = None
-    for t in tiles:
-        t.release()
-    ctx.close()
-    del keep
     return line
 
 
