@@ -78,11 +78,17 @@ class XcAngleRec(C.Structure):
     _fields_ = [("A", C.c_double), ("R", C.c_double), ("X", C.c_double), ("Y", C.c_double)]
 
 
+# xc_ang_proc: void (*)(int *Ox, int *Oy, int *Rx, int *Ry, double deg, void *arg)
+ANG_PROC = C.CFUNCTYPE(None, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                       C.c_double, C.c_void_p)
+
+
 class XcThmParams(C.Structure):
     """CThmScan's matcher state for one pair (CThmScan.h:85-97) as the thm entry points read and update it."""
     _fields_ = [("rthresh", C.c_double), ("const_xy", C.c_int32), ("pretweak", C.c_int32), ("olap1D", C.c_int32),
                 ("Ox", C.c_int32), ("Oy", C.c_int32), ("Rx", C.c_int32), ("Ry", C.c_int32), ("n_evals", C.c_int32),
-                ("req_area", C.c_int64), ("Tdfm", C.c_double * 6), ("Tptwk", C.c_double * 6)]
+                ("req_area", C.c_int64), ("Tdfm", C.c_double * 6), ("Tptwk", C.c_double * 6),
+                ("ang_proc", C.c_void_p), ("ang_arg", C.c_void_p)]
 
     @classmethod
     def default(cls, olap1D=0, req_area=0):

@@ -168,6 +168,45 @@ void IFT_2D(vector<double> &out, const vector<CD> &in, int Nfast, int Nslow, FIL
     if (st != XC_OK) die(flog, "IFT_2D", st);
 }
 
+// Convolve (Correlation.cpp:371-457): src (ws x hs) against the odd-sized kernel K (wk x hk) centred on the
+// origin of the FFTSize canvas, by the device's FP64 transforms; a kfft of the canvas's spectrum size is taken
+// as the kernel's transform from an earlier call, as the reference takes it (stale or not).
+void Convolve(vector<double> &dst, const vector<double> &src, int ws, int hs, const double *K, int wk, int hk,
+              bool kIsSymmetric, bool preNormK, vector<CD> &kfft, FILE *flog)
+{
+    const int Ns = ws * hs, Nk = wk * hk;
+    const int Nx = FFTSize(ws, wk / (1 + kIsSymmetric)), Ny = FFTSize(hs, hk / (1 + kIsSymmetric));
+    const int Nxy = Nx * Ny, M = Ny * (Nx / 2 + 1);
+    if ((int)kfft.size() != M) {
+        vector<double> KK((size_t)Nxy, 0.0);
+        double nrm = 1.0;
+        if (preNormK) {
+            nrm = 0.0;
+            for (int i = 0; i < Nk; ++i) nrm += K[i];
+            if (!nrm) nrm = 1.0;
+        }
+        const int w2 = wk / 2, h2 = hk / 2;
+        for (int i = -h2; i <= h2; ++i) {
+            const int y = i >= 0 ? i : Ny + i;
+            for (int j = -w2; j <= w2; ++j) {
+                const int x = j >= 0 ? j : Nx + j;
+                KK[x + (size_t)Nx * y] = K[j + w2 + (size_t)wk * (i + h2)] / nrm;
+            }
+        }
+        FFT_2D(kfft, KK, Nx, Ny, false, flog);
+    }
+    vector<double> SS((size_t)Nxy, 0.0);
+    for (int y = 0; y < hs; ++y)
+        for (int x = 0; x < ws; ++x) SS[x + (size_t)Nx * y] = src[x + (size_t)ws * y];
+    vector<CD> sfft;
+    FFT_2D(sfft, SS, Nx, Ny, false, flog);
+    for (int i = 0; i < M; ++i) sfft[i] *= kfft[i];
+    IFT_2D(SS, sfft, Nx, Ny, flog);
+    dst.resize(Ns);
+    for (int y = 0; y < hs; ++y)
+        for (int x = 0; x < ws; ++x) dst[x + (size_t)ws * y] = SS[x + (size_t)Nx * y] / Nxy;
+}
+
 void ParabPeakFFT(double &xpk, double &ypk, int d, const double *I, int nx, int ny)
 {
     auto lk = [&](int x, int y) {

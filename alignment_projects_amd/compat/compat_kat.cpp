@@ -1,8 +1,8 @@
 // compat_kat.cpp -- runs the reference's retired harness testnormcorr
 // (Retired_Proj/4_TestNormCorr/testnormcorr.cpp:20-63) through the drop-in
 // signatures, plus a CorrImagesS call with EnoughPoints-style callbacks,
-// FourierMatch and LegPolyFlatten, and prints one machine-readable line per
-// case.
+// FourierMatch, LegPolyFlatten and Convolve (with its kernel-transform
+// cache), and prints one machine-readable line per case.
 #include <stdio.h>
 
 #include "../../include/aln_corr_compat.h"
@@ -64,5 +64,19 @@ int main()
     vector<double> flat;
     LegPolyFlatten(flat, ras.data(), 80, 64, 2);
     printf("KAT legflat %zu %.17g %.17g %.17g\n", flat.size(), flat[0], flat[1234], flat.back());
+
+    // Convolve of the 80 x 64 field with a 7 x 5 kernel (symmetric canvas, prenormalised); the second call
+    // hands back the first call's kernel transform with another kernel: the transform in kfft is used
+    double K[35], K2[35];
+    for (int i = 0; i < 35; ++i) {
+        K[i] = 1.0 + (i % 5);
+        K2[i] = 100.0 - i;
+    }
+    vector<double> cv, cv2;
+    vector<CD> kfft;
+    Convolve(cv, f, 80, 64, K, 7, 5, true, true, kfft, stdout);
+    Convolve(cv2, f, 80, 64, K2, 7, 5, true, true, kfft, stdout);
+    printf("KAT convolve %zu %zu %.17g %.17g %.17g %.17g\n", cv.size(), kfft.size(), cv[0], cv[1234], cv.back(),
+           cv2[1234]);
     return 0;
 }

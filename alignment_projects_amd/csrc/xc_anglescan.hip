@@ -435,11 +435,17 @@ int pretweaks_run(xc_ctx *ctx, const xc_point *ap, const double *av, int na, con
     return pretweaks_core(eval, tw, bR, changed, n_evals);
 }
 
-// RFromAngle at each angle with the deformation D, the pretweak W and the disc (Ox, Oy, Rx, Ry).
-using AngEval = std::function<int(const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx,
-                                  int Ry, std::vector<xc_angle_rec> &out)>;
-// R of each candidate pretweak M(sel[i], mag[i]) * tw at angle deg, with D and the disc.
-using TwkEval = std::function<int(double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+// The search oval of one evaluation: CorrImagesS's (Ox, Oy, Rx, Ry).
+struct Oval {
+    int Ox, Oy, Rx, Ry;
+    bool operator==(const Oval &o) const { return Ox == o.Ox && Oy == o.Oy && Rx == o.Rx && Ry == o.Ry; }
+};
+
+// RFromAngle at each angle with the deformation D, the pretweak W and that angle's oval ov[i].
+using AngEval = std::function<int(const std::vector<double> &deg, const Aff &D, const Aff &W,
+                                  const std::vector<Oval> &ov, std::vector<xc_angle_rec> &out)>;
+// R of each candidate pretweak M(sel[i], mag[i]) * tw at angle deg, with D and the oval.
+using TwkEval = std::function<int(double deg, const Aff &D, const Oval &ov, const std::vector<int> &sel,
                                   const std::vector<double> &mag, const Aff &tw, std::vector<double> &R)>;
 
 // The two evaluators on point lists in FP64: one device correlation per probe.
@@ -455,11 +461,13 @@ struct PointEvals {
     std::vector<xc_point> buf;
     AngEval ang()
     {
-        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx, int Ry,
+        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, const std::vector<Oval> &ov,
                       std::vector<xc_angle_rec> &out) {
             out.resize(deg.size());
             for (size_t i = 0; i < deg.size(); ++i) {
-                const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg[i], &D, &W, ru, Ox, Oy, Rx, Ry, buf, &out[i]);
+                const Oval &o = ov[i];
+                const int st =
+                    rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg[i], &D, &W, ru, o.Ox, o.Oy, o.Rx, o.Ry, buf, &out[i]);
                 if (st) return st;
             }
             return (int)XC_OK;
@@ -467,13 +475,13 @@ struct PointEvals {
     }
     TwkEval twk()
     {
-        return [this](double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+        return [this](double deg, const Aff &D, const Oval &o, const std::vector<int> &sel,
                       const std::vector<double> &mag, const Aff &tw, std::vector<double> &R) {
             R.resize(sel.size());
             for (size_t i = 0; i < sel.size(); ++i) {
                 const Aff W = aff_mul(nu_select(sel[i], mag[i]), tw);  // M * Tptwk (RFromAngle's iaux branch)
                 xc_angle_rec C;
-                const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, Ox, Oy, Rx, Ry, buf, &C);
+                const int st = rfa64_aff(ctx, ap, av, na, bp, bv, nb, deg, &D, &W, ru, o.Ox, o.Oy, o.Rx, o.Ry, buf, &C);
                 if (st) return st;
                 R[i] = C.R;
             }
@@ -490,29 +498,34 @@ struct TileEvals {
     Rules ru;
     std::vector<double> maps;
     std::vector<xc_result> res;
-    int sweep(const std::vector<Aff> &T, int Ox, int Oy, int Rx, int Ry)
+    // one batched sweep of the maps T[i], each with its oval ov[i]
+    int sweep(const std::vector<Aff> &T, const std::vector<Oval> &ov)
     {
         const size_t n = T.size();
         maps.resize(4 * n);
         res.resize(n);
+        std::vector<int> ovals(4 * n);
         for (size_t i = 0; i < n; ++i) {
             maps[4 * i + 0] = T[i].t[0];
             maps[4 * i + 1] = T[i].t[1];
             maps[4 * i + 2] = T[i].t[3];
             maps[4 * i + 3] = T[i].t[4];
+            ovals[4 * i + 0] = ov[i].Ox;
+            ovals[4 * i + 1] = ov[i].Oy;
+            ovals[4 * i + 2] = ov[i].Rx;
+            ovals[4 * i + 3] = ov[i].Ry;
         }
-        const int st = xc_sweep_affine_f32(ctx, a, b, maps.data(), (int)n, &ru.rgn, &ru.cnt, Ox, Oy, Rx, Ry,
-                                           res.data());
+        const int st = sweep_affine_ovals(ctx, a, b, maps.data(), ovals.data(), (int)n, &ru.rgn, &ru.cnt, res.data());
         return st < 0 ? st : (int)XC_OK;
     }
     AngEval ang()
     {
-        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, int Ox, int Oy, int Rx, int Ry,
+        return [this](const std::vector<double> &deg, const Aff &D, const Aff &W, const std::vector<Oval> &ov,
                       std::vector<xc_angle_rec> &out) {
             std::vector<Aff> T(deg.size());
             const Aff DW = aff_mul(D, W);
             for (size_t i = 0; i < deg.size(); ++i) T[i] = aff_mul(nu_select(5, deg[i] * REF_PI / 180.0), DW);
-            const int st = sweep(T, Ox, Oy, Rx, Ry);
+            const int st = sweep(T, ov);
             if (st) return st;
             out.resize(deg.size());
             for (size_t i = 0; i < deg.size(); ++i) {
@@ -527,12 +540,12 @@ struct TileEvals {
     }
     TwkEval twk()
     {
-        return [this](double deg, const Aff &D, int Ox, int Oy, int Rx, int Ry, const std::vector<int> &sel,
+        return [this](double deg, const Aff &D, const Oval &o, const std::vector<int> &sel,
                       const std::vector<double> &mag, const Aff &tw, std::vector<double> &R) {
             const Aff Rot = nu_select(5, deg * REF_PI / 180.0);
             std::vector<Aff> T(sel.size());
             for (size_t i = 0; i < sel.size(); ++i) T[i] = aff_mul(Rot, aff_mul(D, aff_mul(nu_select(sel[i], mag[i]), tw)));
-            const int st = sweep(T, Ox, Oy, Rx, Ry);
+            const int st = sweep(T, std::vector<Oval>(sel.size(), o));
             if (st) return st;
             R.resize(sel.size());
             for (size_t i = 0; i < sel.size(); ++i) R[i] = res[i].status == XC_OK ? res[i].r : 0.0;
@@ -548,11 +561,20 @@ struct Thm {
     AngEval eval_ang;
     TwkEval eval_twk;
 
-    // RFromAngle (CThmScan.cpp:329-366) with the current Tdfm, Tptwk and disc, over a list of angles
+    // The oval at angle deg: newAngProc's when the hook is set, else the disc (CThmScan.cpp:351-355)
+    Oval oval(double deg) const
+    {
+        Oval o{p->Ox, p->Oy, p->Rx, p->Ry};
+        if (p->ang_proc) p->ang_proc(&o.Ox, &o.Oy, &o.Rx, &o.Ry, deg, p->ang_arg);
+        return o;
+    }
+    // RFromAngle (CThmScan.cpp:329-366) with the current Tdfm, Tptwk and ovals, over a list of angles
     int rfa_list(const std::vector<double> &deg, std::vector<xc_angle_rec> &out)
     {
         p->n_evals += (int)deg.size();
-        return eval_ang(deg, aff_of(p->Tdfm), aff_of(p->Tptwk), p->Ox, p->Oy, p->Rx, p->Ry, out);
+        std::vector<Oval> ov(deg.size());
+        for (size_t i = 0; i < deg.size(); ++i) ov[i] = oval(deg[i]);
+        return eval_ang(deg, aff_of(p->Tdfm), aff_of(p->Tptwk), ov, out);
     }
     int rfa(double deg, xc_angle_rec *C)
     {
@@ -579,10 +601,9 @@ struct Thm {
         double bR = bestR;
         int ev = 0;
         const Aff D = aff_of(p->Tdfm);
+        const Oval o = oval(deg);  // every candidate of a round is evaluated at deg
         const PtwEval e = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
-                              std::vector<double> &R) {
-            return eval_twk(deg, D, p->Ox, p->Oy, p->Rx, p->Ry, sel, mag, t, R);
-        };
+                              std::vector<double> &R) { return eval_twk(deg, D, o, sel, mag, t, R); };
         const int st = pretweaks_core(e, tw, bR, changed, &ev);
         p->n_evals += ev;
         if (!st) memcpy(p->Tptwk, tw.t, sizeof tw.t);
@@ -794,7 +815,7 @@ extern "C" int xc_pretweaks_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b,
     double bR = *bestR;
     bool any = false;
     const PtwEval e = [&](const std::vector<int> &sel, const std::vector<double> &mag, const Aff &t,
-                          std::vector<double> &R) { return twk(deg, D, Ox, Oy, Rx, Ry, sel, mag, t, R); };
+                          std::vector<double> &R) { return twk(deg, D, Oval{Ox, Oy, Rx, Ry}, sel, mag, t, R); };
     const int st = pretweaks_core(e, tw, bR, &any, n_evals);
     if (st) return st;
     memcpy(Tptwk, tw.t, sizeof tw.t);
@@ -813,7 +834,8 @@ extern "C" int xc_finish_at_full_res_f32(xc_ctx *ctx, const xc_tile *a, const xc
     const xc_angle_rec b0 = *best;
     std::vector<xc_angle_rec> out;
     // SetDisc( (int)best.X, (int)best.Y, 80, 80 ), then RFromAngle( best, best.A )
-    const int st = ang(std::vector<double>(1, b0.A), aff_of(Tdfm), aff_of(Tptwk), (int)b0.X, (int)b0.Y, 80, 80, out);
+    const int st = ang(std::vector<double>(1, b0.A), aff_of(Tdfm), aff_of(Tptwk),
+                       std::vector<Oval>(1, Oval{(int)b0.X, (int)b0.Y, 80, 80}), out);
     if (st) return st;
     const xc_angle_rec &C = out[0];
     const bool ok = fabs(C.X - b0.X) <= 20 && fabs(C.Y - b0.Y) <= 20;

@@ -861,11 +861,14 @@ extern "C" int xc_sweep_angles_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile 
     return st;
 }
 
-extern "C" int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps, int n_maps,
-                                   const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, int Ox, int Oy, int Rx, int Ry,
-                                   xc_result *results)
+namespace xc {
+
+// One batched sweep of the 2 x 2 maps maps[4i..4i+3] applied to A's pixel grid, each correlation (CorrImagesS)
+// with its own search oval ovals[4i..4i+3] = (Ox, Oy, Rx, Ry).
+int sweep_affine_ovals(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps, const int *ovals,
+                       int n_maps, const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *results)
 {
-    if (!ctx || !a || !b || !maps || n_maps <= 0 || !results) return XC_E_ARG;
+    if (!ctx || !a || !b || !maps || !ovals || n_maps <= 0 || !results) return XC_E_ARG;
     std::vector<xc_pair> pairs(n_maps);
     for (int i = 0; i < n_maps; ++i) {
         xc_pair &q = pairs[i];
@@ -873,10 +876,10 @@ extern "C" int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile 
         q.a = a;
         q.b = b;
         q.mode = XC_MODE_S;
-        q.Ox = Ox;
-        q.Oy = Oy;
-        q.Rx = Rx;
-        q.Ry = Ry;
+        q.Ox = ovals[4 * i + 0];
+        q.Oy = ovals[4 * i + 1];
+        q.Rx = ovals[4 * i + 2];
+        q.Ry = ovals[4 * i + 3];
         if (rgn) q.rgn = *rgn;
         if (cnt) q.cnt = *cnt;
     }
@@ -886,4 +889,21 @@ extern "C" int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile 
     st = xc_plan_run(p, results);
     xc_plan_destroy(p);
     return st;
+}
+
+}  // namespace xc
+
+extern "C" int xc_sweep_affine_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps, int n_maps,
+                                   const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, int Ox, int Oy, int Rx, int Ry,
+                                   xc_result *results)
+{
+    if (!ctx || !a || !b || !maps || n_maps <= 0 || !results) return XC_E_ARG;
+    std::vector<int> ovals(4 * (size_t)n_maps);
+    for (int i = 0; i < n_maps; ++i) {
+        ovals[4 * i + 0] = Ox;
+        ovals[4 * i + 1] = Oy;
+        ovals[4 * i + 2] = Rx;
+        ovals[4 * i + 3] = Ry;
+    }
+    return sweep_affine_ovals(ctx, a, b, maps, ovals.data(), n_maps, rgn, cnt, results);
 }

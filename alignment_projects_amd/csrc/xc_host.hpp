@@ -131,6 +131,11 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
               const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *out, bool fmode, double nbmaxht,
               const FixedWs *fixed);
 
+// One batched FP32 sweep (CorrImagesS) of the 2 x 2 maps maps[4i..4i+3] applied to A's pixel grid against B,
+// each with its own search oval ovals[4i..4i+3] = (Ox, Oy, Rx, Ry) (xc_batch.hip).
+int sweep_affine_ovals(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, const double *maps, const int *ovals,
+                       int n_maps, const xc_rgn_rule *rgn, const xc_cnt_rule *cnt, xc_result *results);
+
 // Looks up the device address of the allocation-free entry point's fixed
 // scratch on ctx->device (current), once per device (xc_noalloc.hip).
 int noalloc_init(xc_ctx *ctx);

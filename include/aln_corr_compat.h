@@ -54,6 +54,10 @@ int FFT_2D(vector<CD> &out, const vector<double> &in, int Nfast, int Nslow, bool
 void IFT_2D(vector<double> &out, const vector<CD> &in, int Nfast, int Nslow, FILE *flog = stderr);
 
 /* ParabPeakFFT (Correlation.cpp:482-517). */
+/* Convolve (Correlation.h:38-50, Correlation.cpp:371-457): by the device's FP64 transforms; a kfft of the
+ * canvas's spectrum size is reused as the kernel's transform, as the reference reuses it. */
+void Convolve(vector<double> &dst, const vector<double> &src, int ws, int hs, const double *K, int wk, int hk,
+              bool kIsSymmetric, bool preNormK, vector<CD> &kfft, FILE *flog);
 void ParabPeakFFT(double &xpk, double &ypk, int d, const double *I, int nx, int ny);
 
 typedef bool (*EvalType)(int sx, int sy, void *v);

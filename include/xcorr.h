@@ -471,14 +471,24 @@ int xc_finish_at_full_res_f32(xc_ctx *ctx, const xc_tile *a, const xc_tile *b, c
                               int *used);
 
 /*
+ * CThmScan's per-angle search-oval hook (T_NewAngProc, CThmScan.h:84,
+ * 143-144): when set, RFromAngle asks it for the oval's centre and
+ * semi-axes at each angle (in degrees) instead of using the disc
+ * (CThmScan.cpp:351-355); 1_Scapeops sets StripAngProc (scapeops.cpp:
+ * 476-491, 542), which centres the oval on the rotated strip.  `arg` is
+ * handed back unchanged (the reference's hook reads globals instead).
+ */
+typedef void (*xc_ang_proc)(int *Ox, int *Oy, int *Rx, int *Ry, double deg, void *arg);
+
+/*
  * CThmScan's matcher state for one pair (CThmScan.h:85-97): thresholds,
- * sweep switches, the disc and the two affines. The entry points below
- * read it and update it as the reference's methods update their object:
- * Pretweaks folds into Tptwk, PostTweaks sets the disc. n_evals counts
- * the RFromAngle evaluations made (the caller sets it; each call adds).
- * xc_thm_params_default gives the constructor's values (CThmScan.cpp:
- * 160-177): rthresh 0.30, ConstXY and Pretweak sweeps on, no disc
- * (0, 0, -1, -1), identity affines.
+ * sweep switches, the disc, the two affines and the per-angle oval hook.
+ * The entry points below read it and update it as the reference's methods
+ * update their object: Pretweaks folds into Tptwk, PostTweaks sets the
+ * disc. n_evals counts the RFromAngle evaluations made (the caller sets
+ * it; each call adds). xc_thm_params_default gives the constructor's
+ * values (CThmScan.cpp:160-177): rthresh 0.30, ConstXY and Pretweak sweeps
+ * on, no disc (0, 0, -1, -1), identity affines, no hook.
  */
 typedef struct xc_thm_params {
     double  rthresh;
@@ -490,6 +500,8 @@ typedef struct xc_thm_params {
     int64_t req_area;
     double  Tdfm[6];
     double  Tptwk[6];
+    xc_ang_proc ang_proc; /* newAngProc (NULL: the disc above for every angle) */
+    void   *ang_arg;
 } xc_thm_params;
 
 void xc_thm_params_default(xc_thm_params *p);

@@ -1526,14 +1526,23 @@ typedef struct {
     xc_thm_params *p;
 } xo_thm;
 
-/* RFromAngle (:329-366) with the block's Tdfm, Tptwk and disc. */
+/* The search oval at angle deg (:351-355): newAngProc's when set, else the disc. */
+static void thm_oval(const xc_thm_params *p, double deg, int *ox, int *oy, int *rx, int *ry)
+{
+    if (p->ang_proc) p->ang_proc(ox, oy, rx, ry, deg, p->ang_arg);
+    else { *ox = p->Ox; *oy = p->Oy; *rx = p->Rx; *ry = p->Ry; }
+}
+
+/* RFromAngle (:329-366) with the block's Tdfm, Tptwk and oval. */
 static void thm_rfa(xo_thm *t, double deg, xc_angle_rec *C)
 {
     xc_thm_params *p = t->p;
+    int ox, oy, rx, ry;
+    thm_oval(p, deg, &ox, &oy, &rx, &ry);
     C->A = deg;
     ++p->n_evals;
     xo_rfromangle_affine(deg, t->ap, t->av, t->na, t->bp, t->bv, t->nb, p->Tdfm, p->Tptwk, p->olap1D, p->req_area,
-                         p->Ox, p->Oy, p->Rx, p->Ry, &C->X, &C->Y, &C->R);
+                         ox, oy, rx, ry, &C->X, &C->Y, &C->R);
 }
 
 /* AngleScanSel (:627-641). */
@@ -1556,9 +1565,10 @@ static double thm_scan_sel(xo_thm *t, xc_angle_rec *best, double center, double 
 static int thm_pretweaks(xo_thm *t, double bestR, double deg)
 {
     xc_thm_params *p = t->p;
-    int changed = 0, ev = 0;
+    int changed = 0, ev = 0, ox, oy, rx, ry;
+    thm_oval(p, deg, &ox, &oy, &rx, &ry);  /* every candidate of a round is evaluated at deg */
     xo_pretweaks(deg, t->ap, t->av, t->na, t->bp, t->bv, t->nb, p->Tdfm, p->Tptwk, &bestR, p->olap1D, p->req_area,
-                 p->Ox, p->Oy, p->Rx, p->Ry, &changed, &ev);
+                 ox, oy, rx, ry, &changed, &ev);
     p->n_evals += ev;
     return changed;
 }

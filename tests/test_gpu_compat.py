@@ -55,3 +55,16 @@ def test_compat_kat_program():
     ras = ((f + 0.5) * 255.0).astype(np.uint8)
     o = oracle.leg_poly_flatten(ras, 2).ravel()
     assert n == 80 * 64 and max(abs(v0 - o[0]), abs(v1234 - o[1234]), abs(vlast - o[-1])) <= 1e-9
+    # Convolve(dst, src, 80, 64, K, 7, 5, sym, prenorm, kfft, flog); the second call reuses kfft
+    n, m, c0, c1234, clast, c2_1234 = lines["convolve"]
+    K = (1.0 + np.arange(35) % 5).reshape(5, 7)
+    o = oracle.convolve(f, K, sym=True, prenorm=True).ravel()
+    # the canvas is FFTSize(80, 3) x FFTSize(64, 2) = 128 x 128: the same circular sum by array shifts
+    canvas = np.zeros((128, 128))
+    canvas[:64, :80] = f
+    d = sum((K[i + 2, j + 3] / K.sum()) * np.roll(canvas, (i, j), axis=(0, 1)) for i in range(-2, 3)
+            for j in range(-3, 4))[:64, :80].ravel()
+    assert np.abs(o - d).max() <= 1e-12
+    assert n == 80 * 64 and m == 128 * 65
+    assert max(abs(c0 - o[0]), abs(c1234 - o[1234]), abs(clast - o[-1])) <= 1e-12
+    assert c2_1234 == c1234  # the cached transform, not the second kernel

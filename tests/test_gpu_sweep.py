@@ -347,3 +347,31 @@ def test_finish_at_full_res_tiles_f32(ctx):
         assert abs(g.X - o.X) <= 0.05 and abs(g.Y - o.Y) <= 0.05
     ta.release()
     tb.release()
+
+
+def test_thm_ang_proc_hook_matches_oracle(ctx):
+    """CThmScan's newAngProc hook (an oval that moves with the angle, as 1_Scapeops' StripAngProc does): the
+    FP64 sequence on point lists matches the oracle exactly, the FP32 sequence over tiles (every scan's angles
+    one batched sweep, each with its own oval) within the batched tolerances."""
+    from tests.test_oracle import ang_hook, _set_hook
+
+    def strip_like(deg):
+        return 2 + int(3 * deg), -1 + int(deg), 10, 10
+
+    pa, va, pb, vb, olap, req = _thm_inputs()
+    hooks = [ang_hook(strip_like) for _ in range(3)]
+    gp, op = abi.XcThmParams.default(olap, req), abi.XcThmParams.default(olap, req)
+    _set_hook(gp, hooks[0])
+    _set_hook(op, hooks[1])
+    g, gok = ctx.thm_denovo_best_angle(pa, va, pb, vb, gp, 0.0, 5.0, 0.5)
+    o, ook = oracle.thm_denovo_best_angle(pa, va, pb, vb, op, 0.0, 5.0, 0.5)
+    assert gok == ook == 1
+    _same_thm(g, o, gp, op)
+    A, B, _, _ = _scan_pair(11, 2.0)
+    ta, tb = ctx.tile_put(A, normalize=False), ctx.tile_put(B, normalize=False)
+    tp = abi.XcThmParams.default(olap, req)
+    _set_hook(tp, hooks[2])
+    t, tok = ctx.thm_tiles("denovo_best_angle", ta, tb, tp, 0.0, 5.0, 0.5)
+    assert tok == 1 and abs(t.A - o.A) <= 0.05 and abs(t.R - o.R) <= 1e-4 * abs(o.R), (t.A, o.A, t.R, o.R)
+    ta.release()
+    tb.release()

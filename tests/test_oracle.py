@@ -602,6 +602,47 @@ def test_thm_use_prior_angles_and_failure_short_circuit():
     assert ok == 0 and p.n_evals == n_scan + n_tw + (n_scan if changed else 0)
 
 
+def ang_hook(fn, calls=None):
+    """An xc_ang_proc over a Python function deg -> (Ox, Oy, Rx, Ry); keep the returned object alive while the
+    parameters use it.  `calls` (a list) receives the angles asked for."""
+    import ctypes as C
+
+    def cb(ox, oy, rx, ry, deg, arg):
+        if calls is not None:
+            calls.append(deg)
+        ox[0], oy[0], rx[0], ry[0] = fn(deg)
+    return abi.ANG_PROC(cb)
+
+
+def _set_hook(p, hook):
+    import ctypes as C
+    p.ang_proc = C.cast(hook, C.c_void_p).value
+
+
+def test_thm_ang_proc_hook():
+    """CThmScan's newAngProc (CThmScan.cpp:351-355): a hook that returns the disc for every angle changes
+    nothing (same record, same evaluations); a hook that confines every angle's oval to a region away from the
+    pair's shift makes the sequence's records land inside it (RFromAngle takes the hook's oval, not the disc);
+    the hook is asked at every angle the sequence evaluates."""
+    pa, va, pb, vb, p = _thm_pair()
+    ref, ok = oracle.thm_denovo_best_angle(pa, va, pb, vb, p, 0.0, 5.0, 0.5)
+    pa, va, pb, vb, q = _thm_pair()
+    calls = []
+    h = ang_hook(lambda deg: (0, 0, -1, -1), calls)
+    _set_hook(q, h)
+    got, ok2 = oracle.thm_denovo_best_angle(pa, va, pb, vb, q, 0.0, 5.0, 0.5)
+    assert (ok2, got.A, got.R, got.X, got.Y, q.n_evals) == (ok, ref.A, ref.R, ref.X, ref.Y, p.n_evals)
+    asked = set(np.round(calls, 9))
+    assert len(calls) >= q.n_evals and 0.0 in asked and round(ref.A, 9) in asked
+    pa, va, pb, vb, q = _thm_pair()
+    h = ang_hook(lambda deg: (-20, 14, 5, 5))
+    _set_hook(q, h)
+    q.rthresh = -1.0
+    far = oracle.thm_scan_with_tweaks(pa, va, pb, vb, q, 0.0, 5.0, 0.5)
+    assert abs(far.X + 20) <= 6 and abs(far.Y - 14) <= 6, (far.X, far.Y)
+    assert abs(ref.X + 20) > 10  # without the hook the pair's own shift wins
+
+
 def test_thm_post_tweaks_finds_x_scale():
     """PostTweaks on a pair stretched 1.04 in x: the disc becomes ((int)X, (int)Y, 40, 40), the composed
     tweak's x scale lands within 0.005 of 1/1.04 and R improves on the untweaked record."""
