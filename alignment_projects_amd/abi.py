@@ -37,6 +37,7 @@ XC_U8, XC_U16, XC_F32, XC_F64 = 0, 1, 2, 3
 XC_TILE_NORMALIZE = 1
 XC_TILE_DEVICE_SRC = 2
 XC_TILE_F32_TABLES = 4
+XC_TILE_F64_TABLES = 8
 
 
 class XcPoint(C.Structure):

@@ -1,7 +1,8 @@
 // xc_batch.hip -- device-resident tiles and batched FP32 plans.
 //
-// A tile is one image kept in HBM with its summed-area tables (FP64 entries
-// unless the tile asks for FP32 ones) and its (mean, sd); it plays the role of the reference's fft2 / ThmRec.ftc cache
+// A tile is one image kept in HBM with its summed-area tables (exact integer
+// sums for unmasked 8-bit tiles of at most 2^20 pixels, else FP64 entries, or
+// FP32 ones when the tile asks for them) and its (mean, sd); it plays the role of the reference's fft2 / ThmRec.ftc cache
 // (Correlation_fft_fftw.cpp:50-51, CThmScan.h:27-34), made explicit.  A plan
 // is a fixed batch of pair correlations laid out once (descriptors, work
 // lists, spectra, lag rows) and replayed as one HIP graph.
@@ -58,7 +59,6 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     std::lock_guard<std::mutex> lock(ctx->mu);
     XC_HIP(ctx, hipSetDevice(ctx->device));
     const bool dev_src = (flags & XC_TILE_DEVICE_SRC) != 0;
-    const bool tab32 = (flags & XC_TILE_F32_TABLES) != 0;
     // per-row present extents, and the bounding box of the present pixels: the raster the reference builds
     // from the point list spans that box (BBoxFromPoints, Geometry.cpp:78-106), so the tile's view does too
     std::vector<int> rx0(h, 0), rx1(h, w - 1);
@@ -94,6 +94,11 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
         if (R < 0) return set_err(ctx, XC_E_ARG, "tile has no present pixel");
     }
     const int vw = R - L + 1, vh = T - Bt + 1;
+    // table form: FP32 on request; exact integer sums for unmasked u8 tiles of at most 2^20 pixels (their box
+    // sums and variance terms are then exact); FP64 pairs otherwise or on request
+    int tabk = TK_F64;
+    if (flags & XC_TILE_F32_TABLES) tabk = TK_F32;
+    else if (dtype == XC_U8 && !mask && (long long)vw * vh <= TABX_MAX_PX && !(flags & XC_TILE_F64_TABLES)) tabk = TK_X;
     static std::atomic<uint64_t> next_uid{1};
     xc_tile *t = new xc_tile;
     t->uid = next_uid++;
@@ -108,13 +113,13 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     t->flags = flags;
     t->rx0.swap(rx0);
     t->rx1.swap(rx1);
-    const TileSizes ts = tile_sizes(vw, vh, tab32);
+    const TileSizes ts = tile_sizes(vw, vh, tabk);
     Layout lay;
     size_t iS = lay.add(ts.SQ), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
     size_t iM = lay.add(ts.msd), iD = lay.add(sizeof(TileDev)), iL = lay.add(sizeof(int));
     size_t iRaw = dev_src ? 0 : lay.add((size_t)es * stride * h);
     size_t iMask = (!dev_src && mask) ? lay.add((size_t)w * h) : 0;
-    int st = ensure_dev(ctx, t->mem, lay.total);
+    int st = ensure_dev(ctx, t->mem, lay.total, false);  // a tile's memory never grows
     if (st) {
         delete t;
         return st;
@@ -164,7 +169,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     D.part = (double *)(base + lay.off[iP]);
     D.nbands = 1;  // swept unbanded, as a plan's batched refresh sweeps it
     D.normalize = (flags & XC_TILE_NORMALIZE) ? 1 : 0;
-    D.tab32 = tab32 ? 1 : 0;
+    D.tabk = tabk;
     t->d_dev = (TileDev *)(base + lay.off[iD]);
     const int zero = 0;
     if (hipMemcpy(t->d_dev, &D, sizeof D, hipMemcpyHostToDevice) != hipSuccess ||
@@ -181,7 +186,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     sg.cpt = sat_cpt(vw);
     sg.threads = sat_threads(vw);
     sg.count = 1;
-    sg.tab32 = tab32;
+    sg.tabk = tabk;
     sg.list = (const int *)(base + lay.off[iL]);
     ex.sat_groups.push_back(sg);
     ex.fp64 = false;
@@ -331,6 +336,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::map<RotKey, int> rot_idx;
     std::vector<int> pair_rot(n, -1);
     std::vector<bool> rot_cnt;  // the rotated copy needs a count table
+    std::map<const xc_tile *, int> der_idx;  // exact tiles that some pair reads through a derived FP64 table
+    std::vector<const xc_tile *> ders;
+    std::vector<int> derA(n, -1), derB(n, -1);
     for (int i = 0; i < n; ++i) {
         const xc_pair &q = pairs[i];
         const xc_tile *a = q.a, *b = q.b;
@@ -339,8 +347,6 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             return set_err(ctx, XC_E_ARG, "pair %d: bad mode", i);
         if (q.rgn.enabled && q.rgn.table)
             return set_err(ctx, XC_E_UNSUPPORTED, "pair %d: tabulated LegalRgn in batched form", i);
-        if (a->dev.tab32 != b->dev.tab32)
-            return set_err(ctx, XC_E_ARG, "pair %d: tiles A and B hold tables of different precision", i);
         for (const xc_tile *t : {a, b})
             if (!tile_idx.count(t)) {
                 tile_idx[t] = (int)tiles.size();
@@ -427,7 +433,26 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         D.offy = offy;
         D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
         D.need_cnt = D.rule.cnt_enabled;
-        D.tab32 = a->dev.tab32;
+        // table forms: both sides read one form; an unrotated exact (TK_X) side beside an FP64 one -- or
+        // beside a rotated copy, whose tables are FP64 -- reads an FP64 table the plan derives from its tile
+        // (S mode reads no (S, S^2) table)
+        const int ka = pair_rot[i] >= 0 ? (a->dev.tabk == TK_F32 ? TK_F32 : TK_F64) : a->dev.tabk, kb = b->dev.tabk;
+        if (mode == PM_S || ka == kb) {
+            D.tabk = ka == kb ? ka : TK_F64;
+        } else if (ka != TK_F32 && kb != TK_F32) {
+            D.tabk = TK_F64;
+            for (int side = 0; side < 2; ++side) {
+                const xc_tile *t = side ? b : a;
+                if ((side ? kb : ka) != TK_X) continue;
+                if (!der_idx.count(t)) {
+                    der_idx[t] = (int)ders.size();
+                    ders.push_back(t);
+                }
+                (side ? derB : derA)[i] = der_idx[t];
+            }
+        } else {
+            return set_err(ctx, XC_E_ARG, "pair %d: FP32 tables pair only with FP32 tables", i);
+        }
         narrow_rows(D);
         D.A = a->dev.v;
         D.B = b->dev.v;
@@ -485,18 +510,22 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     // rotated copies: raster + tables
     std::vector<size_t> i_rimg(rots.size()), i_rS(rots.size()), i_rZ(rots.size());
     std::vector<bool> rot_sq(rots.size(), false);
-    std::vector<char> rots_tab32(rots.size(), 0);  // a rotated copy's tables have its source tile's precision
+    // a rotated copy's tables: FP32 pairs for a source with FP32 tables, else FP64 pairs (its values are
+    // not integers); then the derived FP64 tables, swept in the same launches
+    std::vector<char> dkind(rots.size() + ders.size(), (char)TK_F64);
     for (int i = 0; i < n; ++i)
         if (pair_rot[i] >= 0) {
             if (pairs[i].mode != XC_MODE_S) rot_sq[pair_rot[i]] = true;
-            rots_tab32[pair_rot[i]] = (char)pairs[i].a->dev.tab32;
+            dkind[pair_rot[i]] = (char)(pairs[i].a->dev.tabk == TK_F32 ? TK_F32 : TK_F64);
         }
     for (size_t r = 0; r < rots.size(); ++r) {
         const size_t px = (size_t)rots[r].w * rots[r].h;
         i_rimg[r] = lay.add(sizeof(float) * px);
-        i_rS[r] = lay.add(rot_sq[r] ? (rots_tab32[r] ? sizeof(float2) : sizeof(double2)) * px : 16);
+        i_rS[r] = lay.add(rot_sq[r] ? (dkind[r] == TK_F32 ? sizeof(float2) : sizeof(double2)) * px : 16);
         i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
     }
+    std::vector<size_t> i_derS(ders.size());
+    for (size_t d = 0; d < ders.size(); ++d) i_derS[d] = lay.add(sizeof(double2) * (size_t)ders[d]->w * ders[d]->h);
     size_t i_res = lay.add(sizeof(xc_result) * n);
     // uploaded block: constants, pairs, tiles, rotations, work lists
     size_t i_msd = lay.add(sizeof(double) * 3);
@@ -506,22 +535,23 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::vector<int> sflat, soff, dflat, doff;
     if (refresh) {
         std::vector<int> widths;
-        std::vector<char> t32;
+        std::vector<char> kinds;
         for (const xc_tile *t : tiles) {
             widths.push_back(t->w);
-            t32.push_back((char)t->dev.tab32);
+            kinds.push_back((char)t->dev.tabk);
         }
-        make_sat_groups(widths, t32, sgroups, sflat, soff);
+        make_sat_groups(widths, kinds, sgroups, sflat, soff);
     }
     {
         std::vector<int> widths;
         for (const RotDev &R : rots) widths.push_back(R.w);
-        make_sat_groups(widths, rots_tab32, dgroups, dflat, doff);
+        for (const xc_tile *t : ders) widths.push_back(t->w);
+        make_sat_groups(widths, dkind, dgroups, dflat, doff);
     }
     size_t i_slist = lay.add(sizeof(int) * sflat.size() + 16);
     size_t i_dlist = lay.add(sizeof(int) * dflat.size() + 16);
     size_t i_rot = lay.add(sizeof(RotDev) * rots.size() + 16);
-    size_t i_dtiles = lay.add(sizeof(TileDev) * rots.size() + 16);
+    size_t i_dtiles = lay.add(sizeof(TileDev) * (rots.size() + ders.size()) + 16);
     int nrw = 0;
     for (const RotDev &R : rots) nrw += (int)((rot_splat_units(R.w, R.h) + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
@@ -549,7 +579,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     }
     const size_t up_lo = lay.off[i_msd];
     const size_t up_hi = lay.total;
-    int st = ensure_dev(ctx, p->ws, lay.total);
+    int st = ensure_dev(ctx, p->ws, lay.total, false);  // a plan is laid out once
     if (st) return st;
     char *base = (char *)p->ws.p;
     auto at = [&](size_t i) { return base + lay.off[i]; };
@@ -585,9 +615,19 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         T.SQ = rot_sq[r] ? (void *)at(i_rS[r]) : nullptr;
         T.Z = rot_cnt[r] ? (int32_t *)at(i_rZ[r]) : nullptr;
         T.nbands = 1;
-        T.tab32 = rots_tab32[r];
+        T.tabk = dkind[r];
         hd[r] = T;
         rots[r] = R;
+    }
+    for (size_t d = 0; d < ders.size(); ++d) {
+        // the tile's own view and statistics, an FP64 table of the plan's (its count table is the tile's own)
+        TileDev T = ders[d]->dev;
+        T.msd = nullptr;
+        T.SQ = at(i_derS[d]);
+        T.Z = nullptr;
+        T.nbands = 1;
+        T.tabk = TK_F64;
+        hd[rots.size() + d] = T;
     }
     if (!dflat.empty()) memcpy(hat(i_dlist), dflat.data(), sizeof(int) * dflat.size());
     for (size_t gi = 0; gi < dgroups.size(); ++gi) dgroups[gi].list = (const int *)at(i_dlist) + doff[gi];
@@ -603,6 +643,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             D.SQA = T.SQ;
             D.ZA = T.Z;
         }
+        if (derA[i] >= 0) D.SQA = hd[rots.size() + derA[i]].SQ;
+        if (derB[i] >= 0) D.SQB = hd[rots.size() + derB[i]].SQ;
         D.specA = at(i_spec[specA[i]]);
         D.specB = at(i_spec[specB[i]]);
         D.lag = at(i_lag[k]);
@@ -666,6 +708,11 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     for (int s = 0; s < ST_COUNT; ++s) p->stage_bytes[s] = 0.0;
     std::map<const void *, bool> prepped, specced;
     p->model_bytes = 0.0;
+    for (const xc_tile *t : ders) {  // derived tables: pixels read, FP64 pairs written
+        const double b = (1.0 + sizeof(double2)) * t->w * t->h;
+        p->stage_bytes[ST_PREP] += b;
+        p->model_bytes += b;
+    }
     for (int i = 0; i < n; ++i) {
         const xc_pair &q = pairs[i];
         const PairDev &D = P[i];

@@ -9,7 +9,7 @@
 
 namespace xc {
 
-void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tab32, std::vector<SatGroup> &groups,
+void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, std::vector<SatGroup> &groups,
                      std::vector<int> &flat, std::vector<int> &offsets)
 {
     groups.clear();
@@ -17,14 +17,14 @@ void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &ta
     offsets.clear();
     std::map<std::tuple<int, int, int>, std::vector<int>> m;
     for (int i = 0; i < (int)widths.size(); ++i) {
-        const int t32 = i < (int)tab32.size() && tab32[i] ? 1 : 0;
-        m[std::make_tuple(sat_cpt(widths[i]), sat_threads(widths[i]), t32)].push_back(i);
+        const int k = i < (int)tabk.size() ? tabk[i] : TK_F64;
+        m[std::make_tuple(sat_cpt(widths[i]), sat_threads(widths[i]), k)].push_back(i);
     }
     for (auto &kv : m) {
         SatGroup g;
         g.cpt = std::get<0>(kv.first);
         g.threads = std::get<1>(kv.first);
-        g.tab32 = std::get<2>(kv.first) != 0;
+        g.tabk = std::get<2>(kv.first);
         g.count = (int)kv.second.size();
         offsets.push_back((int)flat.size());
         for (int i : kv.second) flat.push_back(i);
@@ -32,11 +32,11 @@ void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &ta
     }
 }
 
-TileSizes tile_sizes(int w, int h, bool tab32)
+TileSizes tile_sizes(int w, int h, int tabk)
 {
     TileSizes t;
     const size_t n = (size_t)w * h;
-    t.SQ = (tab32 ? sizeof(float2) : sizeof(double2)) * n;
+    t.SQ = (tabk == TK_F32 ? sizeof(float2) : tabk == TK_X ? sizeof(unsigned long long) : sizeof(double2)) * n;
     t.Z = sizeof(int32_t) * n;
     t.part = sizeof(double) * TP_PARTS * 3;
     t.msd = sizeof(double) * 3;
@@ -49,7 +49,7 @@ void make_buckets(const std::vector<PairDev> &pairs, int, std::vector<int> &orde
     std::map<std::tuple<int, int, int, int>, std::vector<int>> m;
     for (int i = 0; i < (int)pairs.size(); ++i) {
         const PairDev &P = pairs[i];
-        m[std::make_tuple(P.Nx / 2, P.Ny, P.mode == PM_S ? 1 : 0, P.tab32 ? 1 : 0)].push_back(i);
+        m[std::make_tuple(P.Nx / 2, P.Ny, P.mode == PM_S ? 1 : 0, (int)P.tabk)].push_back(i);
     }
     order.clear();
     buckets.clear();
@@ -58,7 +58,7 @@ void make_buckets(const std::vector<PairDev> &pairs, int, std::vector<int> &orde
         b.NH = std::get<0>(kv.first);
         b.NY = std::get<1>(kv.first);
         b.whiten = std::get<2>(kv.first) != 0;
-        b.tab32 = std::get<3>(kv.first) != 0;
+        b.tabk = std::get<3>(kv.first);
         b.first = (int)order.size();
         b.count = (int)kv.second.size();
         for (int i : kv.second) {
@@ -74,7 +74,7 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     const double H = P.Nx / 2 + 1;
     const double n1 = (double)P.w1 * P.h1, n2 = (double)P.w2 * P.h2;
     const double zb = P.need_cnt ? 4.0 : 0.0;
-    const double te = P.tab32 ? 8.0 : 16.0;             // one (S, S^2) table entry
+    const double te = P.tabk == TK_F64 ? 16.0 : 8.0;    // one (S, S^2) table entry
     const double sat = (P.mode == PM_S) ? zb : te + zb;  // S mode: count table only
     bytes[ST_PREP] = (prep_a ? ba * n1 + sat * n1 : 0.0) + (prep_b ? bb * n2 + sat * n2 : 0.0);
     bytes[ST_R2C] = ba * n1 + bb * n2 + s * H * (P.h1 + P.h2);
@@ -84,18 +84,21 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
     bytes[ST_FIN] = sizeof(RowRec) * rows;
 }
 
-// Table sweeps of every group: FP64 (S, S^2) entries, or FP32 entries for
-// groups that ask for them (transforms in FP32 only).
+// Table sweeps of every group, by table form: FP64 (S, S^2) entries; FP32
+// entries and exact integer tables (transforms in FP32 only).
 template <typename T>
 static int launch_sats_any(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
 {
-    std::vector<SatGroup> g64, g32;
-    for (const SatGroup &g : groups) (g.tab32 ? g32 : g64).push_back(g);
+    std::vector<SatGroup> g64, g32, gx;
+    for (const SatGroup &g : groups) (g.tabk == TK_F32 ? g32 : g.tabk == TK_X ? gx : g64).push_back(g);
     int st = launch_sats<T, double>(ctx, s, d_tiles, g64);
     if (st) return st;
-    if (g32.empty()) return XC_OK;
-    if constexpr (sizeof(T) == 4) return launch_sats<T, float>(ctx, s, d_tiles, g32);
-    return set_err(ctx, XC_E_ARG, "FP32 tables with FP64 transforms");
+    if (g32.empty() && gx.empty()) return XC_OK;
+    if constexpr (sizeof(T) == 4) {
+        if ((st = launch_sats<T, float>(ctx, s, d_tiles, g32))) return st;
+        return launch_sats<T, TabX>(ctx, s, d_tiles, gx);
+    }
+    return set_err(ctx, XC_E_ARG, "FP32 or exact tables with FP64 transforms");
 }
 
 template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, int s1)
@@ -124,7 +127,7 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
             ss = ctx->side;
         }
         if (ex.ntiles > 0 && (st = launch_sats_any<T>(ctx, ss, ex.d_tiles, ex.sat_groups))) return st;
-        if (ex.n_rot_work > 0 && (st = launch_sats_any<T>(ctx, ss, ex.d_dtiles, ex.dsat_groups))) return st;
+        if (!ex.dsat_groups.empty() && (st = launch_sats_any<T>(ctx, ss, ex.d_dtiles, ex.dsat_groups))) return st;
         XC_HIP(ctx, hipGetLastError());
         if (fork) XC_HIP(ctx, hipEventRecord(ctx->join_ev, ss));
     }
@@ -139,10 +142,10 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     if (fork) XC_HIP(ctx, hipStreamWaitEvent(s, ctx->join_ev, 0));
     if (s0 <= ST_C2R && ST_C2R < s1)
         for (const Bucket &b : ex.buckets)
-            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r, b.cnt, b.tab32))) return st;
+            if ((st = launch_row_c2r<T>(ctx, b.NH, b.n_c2r, ex.d_pairs, b.d_c2r, b.cnt, b.tabk))) return st;
     if (s0 <= ST_FIN && ST_FIN < s1)
         for (const Bucket &b : ex.buckets)
-            if ((st = launch_finalize<T>(ctx, b.count, ex.d_pairs + b.first, b.tab32))) return st;
+            if ((st = launch_finalize<T>(ctx, b.count, ex.d_pairs + b.first, b.tabk))) return st;
     return XC_OK;
 }
 

@@ -26,7 +26,7 @@ struct Bucket {
     const int2 *d_c2r = nullptr;
     int n_c2r = 0;
     bool cnt = false;  // some NCC pair of the bucket has a count rule (c2r variant with the count-table scan)
-    bool tab32 = false;  // the bucket's pairs read FP32 (S, S^2) tables (else FP64)
+    int tabk = TK_F64;  // TabKind of the bucket's pairs' tables
 };
 
 // Device-resident execution description.  All pointers are device pointers
@@ -37,22 +37,23 @@ struct SatGroup {
     int threads = 0;
     int count = 0;
     int nb = 1;  // row bands per image (the largest of the group's images)
-    bool tab32 = false;  // the group's tables are FP32 (S, S^2) pairs (else FP64)
+    int tabk = TK_F64;  // TabKind of the group's tables
     const int *list = nullptr;  // device: tile indices
 };
 
 // Tile indices grouped by sweep shape (sat_cpt(w), sat_threads(w)) and table
-// precision (tab32[i], FP32 entries; empty: all FP64): fills `groups` and the
+// form (tabk[i], a TabKind; missing entries: TK_F64): fills `groups` and the
 // concatenated index list `flat` (upload it, then point each group's list
 // into the device copy at its offset).
-void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tab32, std::vector<SatGroup> &groups,
+void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, std::vector<SatGroup> &groups,
                      std::vector<int> &flat, std::vector<int> &offsets);
 
 struct Exec {
     int ntiles = 0;
     const TileDev *d_tiles = nullptr;
     std::vector<SatGroup> sat_groups;
-    // rotated copies (sweeps): splat, then their tables (no statistics)
+    // rotated copies (sweeps): splat, then their tables (no statistics); the
+    // same sweep launches build the plan's derived FP64 tables of exact tiles
     const RotDev *d_rot = nullptr;
     const int2 *d_rot_work = nullptr;
     int n_rot_work = 0;
@@ -64,7 +65,8 @@ struct Exec {
 };
 
 // Table sweeps of `groups` on stream s: values rounded to T, (S, S^2) entries
-// stored as TB (xc_launch_sat_*.hip instantiate the three combinations).
+// stored as TB (double / float pairs, or TabX exact sums; xc_launch_sat_*.hip
+// instantiate the four combinations).
 template <typename T, typename TB>
 int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups);
 
@@ -77,9 +79,9 @@ int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
 struct TileSizes {
     size_t SQ, Z, part, msd, band;
 };
-TileSizes tile_sizes(int w, int h, bool tab32);  // tab32: FP32 (S, S^2) table entries, else FP64
+TileSizes tile_sizes(int w, int h, int tabk);  // tabk: the TabKind of the tables
 
-// Group pairs into (NH, NY, whiten, table precision) buckets; returns the
+// Group pairs into (NH, NY, whiten, table form) buckets; returns the
 // order of pair indices (bucket-major) and the bucket list (first/count filled).
 void make_buckets(const std::vector<PairDev> &pairs, int rows_g64, std::vector<int> &order,
                   std::vector<Bucket> &buckets);

@@ -94,7 +94,8 @@ struct Layout {
     }
 };
 
-int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes);
+// slack: allocate a quarter more than asked, for buffers that grow with reuse (not for fixed-size ones).
+int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes, bool slack = true);
 int ensure_pinned(xc_ctx *ctx, size_t bytes);
 
 // Host mirror of the reference geometry helpers.
@@ -163,12 +164,14 @@ template <typename T>
 int launch_col(xc_ctx *ctx, int NY, bool whiten, int cwg, int xcd, int nitems, const PairDev *pd, const int2 *work);
 bool col_shape_ok(const xc_ctx *ctx, int NY, int cwg);  // an FP32 column pass of cwg columns fits
 // cnt: some NCC pair of the launch has a count rule (selects the kernel variant that scans the count tables;
-// the FP64 form always uses the general per-lag evaluation for those).  tab32: the pairs' (S, S^2) tables hold
-// FP32 entries (FP32 transforms only; else FP64 entries).
+// the FP64 form always uses the general per-lag evaluation for those).  tabk: the TabKind of the pairs' tables
+// (TK_F32 and TK_X with FP32 transforms only).
 template <typename T>
 int launch_row_c2r(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int2 *work, bool cnt = false,
-                   bool tab32 = false);
-template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd, bool tab32 = false);
+                   int tabk = TK_F64);
+// The TK_X launches (their own translation unit).
+int launch_row_c2r_x(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int2 *work, bool cnt);
+template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd, int tabk = TK_F64);
 template <typename T> int row_group(int NH);   // rows per workgroup of the row kernels
 template <typename T> int r2c_rows(int NH);    // rows per work item of the row r2c
 template <typename T> int col_group(int NY);   // columns per work item of the column kernel

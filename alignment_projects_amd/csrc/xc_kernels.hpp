@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "../../include/xcorr.h"
 #include "xc_fft.hpp"
 #include "xc_types.hpp"
@@ -85,29 +87,93 @@ __host__ __device__ __forceinline__ double2 tab_at(const void *t, long long i)
     return make_double2((double)v.x, (double)v.y);
 }
 
-// The same lookups on the interleaved (S, S^2) table: one load per corner,
-// each component summed as box_sum sums it.
-template <typename FT>
-__host__ __device__ __forceinline__ double2 box_sum2(const void *__restrict__ t, int w, int L, int R, int B, int T)
+// Exact integer tables (TabX, the TK_X form) of unmasked 8-bit images of at
+// most 2^20 pixels: entry (x, y) is the uint64 (Sraw << 36) + Qraw, Sraw the
+// sum of the raw pixel values (< 2^28) and Qraw that of their squares
+// (< 2^36) over [0, x] x [0, y].  Neither field can carry into the other, so
+// a box's four-corner combination in 64-bit (modular) arithmetic is the packed
+// pair of the box's exact sums.
+struct TabX {};
+constexpr int TABX_QB = 36;
+constexpr unsigned long long TABX_QMASK = (1ULL << TABX_QB) - 1;
+constexpr long long TABX_MAX_PX = 1LL << 20;
+template <> struct TabE<TabX> {
+    typedef unsigned long long type;
+};
+template <typename TB> __host__ __device__ constexpr bool is_tabx() { return std::is_same<TB, TabX>::value; }
+
+// A table corner as the box arithmetic carries it: the (S, S^2) pair widened
+// to FP64 for the FP tables, the packed pair for TabX.
+template <typename TB> struct CorT {
+    typedef double2 type;
+};
+template <> struct CorT<TabX> {
+    typedef unsigned long long type;
+};
+template <typename TB> using cor_t = typename CorT<TB>::type;
+
+__host__ __device__ __forceinline__ double2 c_add(double2 a, double2 b)
 {
-    double2 r = tab_at<FT>(t, R + (long long)w * T);
-    if (L > 0) {
-        const double2 u = tab_at<FT>(t, L - 1 + (long long)w * T);
-        r.x -= u.x;
-        r.y -= u.y;
-    }
-    if (B > 0) {
-        const double2 u = tab_at<FT>(t, R + (long long)w * (B - 1));
-        r.x -= u.x;
-        r.y -= u.y;
-    }
-    if (L > 0 && B > 0) {
-        const double2 u = tab_at<FT>(t, (L - 1) + (long long)w * (B - 1));
-        r.x += u.x;
-        r.y += u.y;
-    }
+    a.x += b.x;
+    a.y += b.y;
+    return a;
+}
+__host__ __device__ __forceinline__ double2 c_sub(double2 a, double2 b)
+{
+    a.x -= b.x;
+    a.y -= b.y;
+    return a;
+}
+__host__ __device__ __forceinline__ unsigned long long c_add(unsigned long long a, unsigned long long b) { return a + b; }
+__host__ __device__ __forceinline__ unsigned long long c_sub(unsigned long long a, unsigned long long b) { return a - b; }
+template <typename C> __host__ __device__ __forceinline__ C c_zero();
+template <> __host__ __device__ __forceinline__ double2 c_zero<double2>() { return make_double2(0.0, 0.0); }
+template <> __host__ __device__ __forceinline__ unsigned long long c_zero<unsigned long long>() { return 0ULL; }
+
+template <typename TB>
+__host__ __device__ __forceinline__ cor_t<TB> tab_c(const void *t, long long i)
+{
+    if constexpr (is_tabx<TB>()) return reinterpret_cast<const unsigned long long *>(t)[i];
+    else return tab_at<TB>(t, i);
+}
+
+// The same lookups on an (S, S^2) table: one load per corner, combined in
+// box_sum's order.
+template <typename TB>
+__host__ __device__ __forceinline__ cor_t<TB> box_c(const void *__restrict__ t, int w, int L, int R, int B, int T)
+{
+    cor_t<TB> r = tab_c<TB>(t, R + (long long)w * T);
+    if (L > 0) r = c_sub(r, tab_c<TB>(t, L - 1 + (long long)w * T));
+    if (B > 0) r = c_sub(r, tab_c<TB>(t, R + (long long)w * (B - 1)));
+    if (L > 0 && B > 0) r = c_add(r, tab_c<TB>(t, (L - 1) + (long long)w * (B - 1)));
     return r;
 }
+
+// An image's normalisation, which exact tables apply at lookup: mean and 1/sd.
+struct NmD {
+    double m, i;
+};
+__device__ __forceinline__ NmD nm_of(const ImgView &v) { return NmD{v.msd[0], v.msd[2]}; }
+
+#pragma clang fp contract(off)
+// GetCorr's numerator and variance product (Correlation.cpp:716-754) from two
+// exact box sums.  With v = (raw - mean) / sd over a box of n pixels,
+//   s = (Sraw - n mean) / sd   and   n q - s^2 = (n Qraw - Sraw^2) / sd^2,
+// the bracket formed in 64-bit integers (n Qraw < 2^56): the variance
+// subtraction is exact and only its final scaling rounds.  nraw: n raw / Nxy.
+__device__ __forceinline__ void tabx_num_d(long long n, double nraw, unsigned long long a, unsigned long long b, NmD A,
+                                           NmD B, double *num, double *d)
+{
+    const long long Sa = (long long)(a >> TABX_QB), Qa = (long long)(a & TABX_QMASK);
+    const long long Sb = (long long)(b >> TABX_QB), Qb = (long long)(b & TABX_QMASK);
+    const double s1 = ((double)Sa - (double)n * A.m) * A.i;
+    const double s2 = ((double)Sb - (double)n * B.m) * B.i;
+    const double d1 = (double)(n * Qa - Sa * Sa) * (A.i * A.i);
+    const double d2 = (double)(n * Qb - Sb * Sb) * (B.i * B.i);
+    *num = nraw - s1 * s2;
+    *d = d1 * d2;
+}
+#pragma clang fp contract(on)
 
 // Row part of BoxesFromShifts (Geometry.cpp:150-176) for signed lag y.
 struct RowBox {
@@ -162,13 +228,18 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
     *legal = ok;
     if (P.mode == PM_S) return raw * invNxy;
     double n = (double)(olw * rb.olh);
-    const double2 sqa = box_sum2<FT>(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
-    const double2 sqb = box_sum2<FT>(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
-    double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
-    double num = n * raw * invNxy - s1 * s2;
-    double d1 = n * q1 - s1 * s1;
-    double d2 = n * q2 - s2 * s2;
-    double d = d1 * d2;
+    const cor_t<FT> sqa = box_c<FT>(P.SQA, P.w1, AL, AR, rb.a_b, rb.a_t);
+    const cor_t<FT> sqb = box_c<FT>(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
+    double num, d;
+    if constexpr (is_tabx<FT>()) {
+        tabx_num_d((long long)olw * rb.olh, n * raw * invNxy, sqa, sqb, nm_of(P.A), nm_of(P.B), &num, &d);
+    } else {
+        double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
+        num = n * raw * invNxy - s1 * s2;
+        double d1 = n * q1 - s1 * s1;
+        double d2 = n * q2 - s2 * s2;
+        d = d1 * d2;
+    }
     double r = (d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d));
     if (P.mode == PM_NCC) {
         *odd = fabs(r) > 1.0001;
@@ -819,6 +890,124 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
     }
 }
 
+// Exact integer tables (TabX) of an unmasked u8 tile.  Each thread owns CPT
+// consecutive columns; a row's CPT pixels arrive in one load, issued LA rows
+// ahead so that the store traffic of the rows between does not stall the
+// sweep.  The row's integer prefixes (v, v^2 [, non-zero]) are scanned across
+// the row (DPP wavefront scan + cross-wave exchange, one barrier per row) and
+// each column's packed running sum (Sraw << 36) + Qraw gains the row's prefix;
+// a row prefix fits 32 bits (w <= TP_MAX_W).  WZ: the count table too
+// (pixels whose value rounded to T is non-zero, as the other sweeps count them).
+template <int CPT> struct PixWord {
+    typedef uint32_t type;
+};
+template <> struct PixWord<8> {
+    typedef unsigned long long type;
+};
+
+template <int CPT>
+__device__ __forceinline__ typename PixWord<CPT>::type load_pix(const uint8_t *row, int x0, int w, bool vec)
+{
+    typedef typename PixWord<CPT>::type W;
+    if (vec) {
+        if constexpr (CPT == 2) return *reinterpret_cast<const uint16_t *>(row + x0);
+        else return *reinterpret_cast<const W *>(row + x0);
+    }
+    W v = 0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j)
+        if (x0 + j < w) v |= (W)row[x0 + j] << (8 * j);
+    return v;
+}
+
+template <typename T, int CPT, int NT, bool WZ>
+__device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi)
+{
+    typedef typename PixWord<CPT>::type W;
+    constexpr int NWV = NT / 64;
+    constexpr int NQ = WZ ? 3 : 2;  // S, Q [, non-zero]
+    constexpr int LA = 16;          // rows loaded ahead of the sweep
+    static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
+    static_assert(CPT == 2 || CPT == 4 || CPT == 8, "one load per row");
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = tid * CPT;
+    const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
+    const long long stride = D.v.stride;
+    const bool vec = x0 + CPT <= w && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride) % CPT) == 0;
+    const double mean = D.v.msd[0], inv = D.v.msd[2];
+    unsigned long long *sq = reinterpret_cast<unsigned long long *>(D.SQ);
+    unsigned long long acc[CPT];
+    int az[CPT];
+    W nxt[LA];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        acc[j] = 0;
+        az[j] = 0;
+    }
+#pragma unroll
+    for (int a = 0; a < LA; ++a) nxt[a] = (a < h && x0 < w) ? load_pix<CPT>(src + a * stride, x0, w, vec) : 0;
+    const bool pair_st = (w % 2) == 0;  // 16-byte aligned column pairs
+    for (int y = 0; y < h; ++y) {
+        const W cur = nxt[0];
+#pragma unroll
+        for (int a = 0; a + 1 < LA; ++a) nxt[a] = nxt[a + 1];
+        nxt[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(src + (y + LA) * stride, x0, w, vec) : 0;
+        int pf[NQ][CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int v = (int)((cur >> (8 * j)) & 255);
+            pf[0][j] = (j ? pf[0][j - 1] : 0) + v;
+            pf[1][j] = (j ? pf[1][j - 1] : 0) + v * v;
+            if constexpr (WZ) {
+                const bool nz = x0 + j < w && (T)(((double)v - mean) * inv) != (T)0;
+                pf[2][j] = (j ? pf[2][j - 1] : 0) + (nz ? 1 : 0);
+            }
+        }
+        int base[NQ];
+        const int par = (y & 1) * NQ * NWV;
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+            const int tot = pf[qi][CPT - 1];
+            const int inc = wave_scan_i(tot);
+            if (lane == 63) xi[par + qi * NWV + wv] = inc;
+            base[qi] = inc - tot;
+        }
+        __syncthreads();
+        if constexpr (NWV > 1) {
+            const bool left = lane < wv;
+            const int k = left ? lane : 0;
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi) base[qi] += row16_sum_i(left ? xi[par + qi * NWV + k] : 0);
+        }
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            acc[j] += ((unsigned long long)(unsigned)(base[0] + pf[0][j]) << TABX_QB) +
+                      (unsigned long long)(unsigned)(base[1] + pf[1][j]);
+            if constexpr (WZ) az[j] += base[2] + pf[2][j];
+        }
+        const long long row = (long long)y * w;
+        if (pair_st) {
+#pragma unroll
+            for (int j = 0; j < CPT; j += 2)
+                if (x0 + j < w) {  // w even: x0 + j + 1 < w too
+                    ulonglong2 e;
+                    e.x = acc[j];
+                    e.y = acc[j + 1];
+                    *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
+                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+                if (x0 + j < w) {
+                    sq[row + x0 + j] = acc[j];
+                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                }
+        }
+    }
+}
+
 #define XC_SAT_DISPATCH(CALL)                                                   \
     switch (D.v.dtype) {                                                        \
     case XC_U8:                                                                 \
@@ -864,6 +1053,12 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
+    if constexpr (is_tabx<TB>()) {
+        // the host gives exact tables to unmasked u8 tiles only, swept unbanded
+        if (D.Z) sat_sweep_x<T, CPT, NT, true>(D, xi);
+        else sat_sweep_x<T, CPT, NT, false>(D, xi);
+        return;
+    } else {
     const bool m = D.v.mask != nullptr;
     if (D.v.dtype == XC_U8 && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
         if (m) {
@@ -878,6 +1073,7 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
 #define SW_CALL(InT, MK) sat_sweep<T, TB, CPT, NT, InT, MK>(D, band, xs, xq, xz)
     XC_SAT_DISPATCH(SW_CALL)
 #undef SW_CALL
+    }
 }
 
 // Columns per thread of the sweep for width w: two up to 1024 threads (a
@@ -1310,6 +1506,7 @@ struct NccRow {
     double invN;   // 1 / (Nx Ny), a power of two: raw * invN == raw / Nxy exactly
     const void *sqAt, *sqAb, *sqBt, *sqBb;  // (S, S^2) table rows: band top, row below the band (or null)
     const int32_t *zAt, *zAb, *zBt, *zBb;
+    NmD nA, nB;    // the images' normalisation (exact tables only)
 };
 
 __host__ __device__ __forceinline__ long long isqrt_floor(long long v)
@@ -1361,6 +1558,14 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
     q.zBt = P.ZB ? P.ZB + bt : nullptr;
     q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
     q.zBb = (P.ZB && rb.b_b > 0) ? P.ZB + bb : nullptr;
+    q.nA = NmD{0.0, 0.0};
+    q.nB = NmD{0.0, 0.0};
+#ifdef __HIP_DEVICE_COMPILE__
+    if constexpr (is_tabx<FT>()) {
+        q.nA = nm_of(P.A);
+        q.nB = nm_of(P.B);
+    }
+#endif
     return q;
 }
 
@@ -1378,25 +1583,15 @@ __device__ __forceinline__ TT band_sum(const TT *__restrict__ top, const TT *__r
     return v;
 }
 
-// band_sum on the interleaved (S, S^2) table.
-template <typename FT>
-__device__ __forceinline__ double2 band_sum2(const void *__restrict__ top, const void *__restrict__ bot, int L, int R)
+// band_sum on an (S, S^2) table, in the same order.
+template <typename TB>
+__device__ __forceinline__ cor_t<TB> band_c(const void *__restrict__ top, const void *__restrict__ bot, int L, int R)
 {
-    double2 v = tab_at<FT>(top, R);
-    if (L > 0) {
-        const double2 u = tab_at<FT>(top, L - 1);
-        v.x -= u.x;
-        v.y -= u.y;
-    }
+    cor_t<TB> v = tab_c<TB>(top, R);
+    if (L > 0) v = c_sub(v, tab_c<TB>(top, L - 1));
     if (bot) {
-        const double2 u = tab_at<FT>(bot, R);
-        v.x -= u.x;
-        v.y -= u.y;
-        if (L > 0) {
-            const double2 z = tab_at<FT>(bot, L - 1);
-            v.x += z.x;
-            v.y += z.y;
-        }
+        v = c_sub(v, tab_c<TB>(bot, R));
+        if (L > 0) v = c_add(v, tab_c<TB>(bot, L - 1));
     }
     return v;
 }
@@ -1417,6 +1612,23 @@ __device__ __forceinline__ double ncc_r(int olw, int olh, double invN, double ra
     if constexpr (sizeof(T) == 8) return d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d);
     else return d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d));
 }
+
+// ncc_r from two boxes' table sums in the table's own form (exact tables:
+// tabx_num_d, the same final step).
+template <typename T, typename TB>
+__device__ __forceinline__ double ncc_rc(int olw, int olh, double invN, double raw, cor_t<TB> a, cor_t<TB> b, NmD nA,
+                                         NmD nB)
+{
+    if constexpr (!is_tabx<TB>()) {
+        return ncc_r<T>(olw, olh, invN, raw, a.x, a.y, b.x, b.y);
+    } else {
+        const double n = (double)(olw * olh);
+        double num, d;
+        tabx_num_d((long long)olw * olh, n * raw * invN, a, b, nA, nB, &num, &d);
+        if constexpr (sizeof(T) == 8) return d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d);
+        else return d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d));
+    }
+}
 #pragma clang fp contract(on)
 
 // NCC value of lag x of a row (CLinCorr::CheckDensity/GetCorr, Correlation.cpp:697-754);
@@ -1434,9 +1646,8 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     const int AL = BL - x, AR = BR - x;
     const int olw = BR - BL + 1;
     const RuleDev &g = P.rule;
-    const double2 sqa = band_sum2<TB>(q.sqAt, q.sqAb, AL, AR);
-    const double2 sqb = band_sum2<TB>(q.sqBt, q.sqBb, BL, BR);
-    const double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
+    const cor_t<TB> sqa = band_c<TB>(q.sqAt, q.sqAb, AL, AR);
+    const cor_t<TB> sqb = band_c<TB>(q.sqBt, q.sqBb, BL, BR);
     bool c = true;
     if (g.exact_w > 0 && olw != g.exact_w) c = false;
     if (g.table && (olw > g.table_w || q.olh > g.table_h || !g.table[(long long)(q.olh - 1) * g.table_w + (olw - 1)]))
@@ -1447,7 +1658,7 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
         if (!((long long)c1 > g.k1 && (long long)c2 > g.k2)) c = false;
     }
     *cand = c;
-    const double r = ncc_r<T>(olw, q.olh, q.invN, raw, s1, q1, s2, q2);
+    const double r = ncc_rc<T, TB>(olw, q.olh, q.invN, raw, sqa, sqb, q.nA, q.nB);
     *odd = c && fabs(r) > 1.0001;
     return c ? r : 0.0;
 }
@@ -1472,6 +1683,18 @@ __device__ __forceinline__ double2 ld_sq(const void *p, int i)
     return make_double2((double)v.x, (double)v.y);
 }
 
+// Table corner i of a row, in the table's own form.
+template <typename TB>
+__device__ __forceinline__ cor_t<TB> ld_c(const void *p, int i)
+{
+    if constexpr (is_tabx<TB>()) {
+        typedef __attribute__((address_space(1))) const unsigned long long gu;
+        return ((gu *)p)[i];
+    } else {
+        return ld_sq<TB>(p, i);
+    }
+}
+
 __device__ __forceinline__ int ld_z(const int32_t *p, int i)
 {
     typedef __attribute__((address_space(1))) const int gi;
@@ -1487,17 +1710,18 @@ template <typename T, typename TB, int TPF, bool AB, bool BB, bool CNT, bool NEG
 __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q, const T *R, int t, int NX, int xlo,
                                                int xhi, double &bestv, long long &bestk, int &flags)
 {
+    typedef cor_t<TB> C;
     const void *At = q.sqAt, *Ab = q.sqAb, *Bt = q.sqBt, *Bb = q.sqBb;
     const int w1 = P.w1, w2 = P.w2, olh = q.olh;
     const double invN = q.invN;
     const long long k1 = P.rule.k1, k2 = P.rule.k2;
     const int KC = BC ? w2 - 1 : w1 - 1;  // the constant right edge
-    const double2 ct = ld_sq<TB>(BC ? Bt : At, KC);
-    const double2 cb = (BC ? BB : AB) ? ld_sq<TB>(BC ? Bb : Ab, KC) : make_double2(0.0, 0.0);
+    const C ct = ld_c<TB>(BC ? Bt : At, KC);
+    const C cb = (BC ? BB : AB) ? ld_c<TB>(BC ? Bb : Ab, KC) : c_zero<C>();
     for (int x = xlo + t; x <= xhi; x += TPF) {
         const int BR = BC ? w2 - 1 : x + w1 - 1, AR = BR - x;
         bool cand = true;
-        double2 a, b;
+        C a, b;
         if constexpr (!NEG) {
             // x >= 0: A's band starts at column 0, B's at x; x = 0 has no left term:
             // subtract / add 0 instead (exact), so that every load is unconditional
@@ -1515,22 +1739,15 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
                 }
                 cand = (long long)c1 > k1 && (long long)c2 > k2;
             }
-            a = BC ? ld_sq<TB>(At, AR) : ct;
-            if constexpr (AB) {
-                const double2 u = BC ? ld_sq<TB>(Ab, AR) : cb;
-                a.x -= u.x;
-                a.y -= u.y;
-            }
-            b = BC ? ct : ld_sq<TB>(Bt, BR);
-            const double2 bl = ld_sq<TB>(Bt, BL1);
-            b.x -= l ? bl.x : 0.0;
-            b.y -= l ? bl.y : 0.0;
+            a = BC ? ld_c<TB>(At, AR) : ct;
+            if constexpr (AB) a = c_sub(a, BC ? ld_c<TB>(Ab, AR) : cb);
+            b = BC ? ct : ld_c<TB>(Bt, BR);
+            const C bl = ld_c<TB>(Bt, BL1);
+            b = c_sub(b, l ? bl : c_zero<C>());
             if constexpr (BB) {
-                const double2 u = BC ? cb : ld_sq<TB>(Bb, BR), z = ld_sq<TB>(Bb, BL1);
-                b.x -= u.x;
-                b.y -= u.y;
-                b.x += l ? z.x : 0.0;
-                b.y += l ? z.y : 0.0;
+                const C u = BC ? cb : ld_c<TB>(Bb, BR), z = ld_c<TB>(Bb, BL1);
+                b = c_sub(b, u);
+                b = c_add(b, l ? z : c_zero<C>());
             }
         } else {
             // x < 0: A's band starts at column -x, B's at 0
@@ -1544,26 +1761,18 @@ __device__ __forceinline__ void ncc_scan_range(const PairDev &P, const NccRow &q
                 if constexpr (BB) c2 -= ld_z(q.zBb, BR);
                 cand = (long long)c1 > k1 && (long long)c2 > k2;
             }
-            a = BC ? ld_sq<TB>(At, AR) : ct;
-            const double2 al = ld_sq<TB>(At, AL1);
-            a.x -= al.x;
-            a.y -= al.y;
+            a = BC ? ld_c<TB>(At, AR) : ct;
+            a = c_sub(a, ld_c<TB>(At, AL1));
             if constexpr (AB) {
-                const double2 u = BC ? ld_sq<TB>(Ab, AR) : cb, z = ld_sq<TB>(Ab, AL1);
-                a.x -= u.x;
-                a.y -= u.y;
-                a.x += z.x;
-                a.y += z.y;
+                const C u = BC ? ld_c<TB>(Ab, AR) : cb, z = ld_c<TB>(Ab, AL1);
+                a = c_sub(a, u);
+                a = c_add(a, z);
             }
-            b = BC ? ct : ld_sq<TB>(Bt, BR);
-            if constexpr (BB) {
-                const double2 u = BC ? cb : ld_sq<TB>(Bb, BR);
-                b.x -= u.x;
-                b.y -= u.y;
-            }
+            b = BC ? ct : ld_c<TB>(Bt, BR);
+            if constexpr (BB) b = c_sub(b, BC ? cb : ld_c<TB>(Bb, BR));
         }
         const int ix = NEG ? NX + x : x;
-        const double r = ncc_r<T>(NEG ? BR + 1 : BR - x + 1, olh, invN, (double)R[ix], a.x, a.y, b.x, b.y);
+        const double r = ncc_rc<T, TB>(NEG ? BR + 1 : BR - x + 1, olh, invN, (double)R[ix], a, b, q.nA, q.nB);
         if (cand) {
             flags |= fabs(r) > 1.0001 ? 3 : 1;
             if (r > bestv) {

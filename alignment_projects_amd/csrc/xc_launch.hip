@@ -27,7 +27,7 @@ int set_err(xc_ctx *ctx, int code, const char *fmt, ...)
     return code;
 }
 
-int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes)
+int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes, bool slack)
 {
     if (bytes <= b.bytes) return XC_OK;
     if (b.p) {
@@ -36,7 +36,7 @@ int ensure_dev(xc_ctx *ctx, DevBuf &b, size_t bytes)
         b.p = nullptr;
         b.bytes = 0;
     }
-    size_t want = bytes + bytes / 4;
+    const size_t want = slack ? bytes + bytes / 4 : bytes;
     if (hipMalloc(&b.p, want) != hipSuccess) {
         b.p = nullptr;
         (void)hipGetLastError();

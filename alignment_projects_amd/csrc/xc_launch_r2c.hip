@@ -54,14 +54,16 @@ template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nb, const Pair
     return set_err(ctx, XC_E_SHAPE, "unsupported row length %d", 2 * NH);
 }
 
-template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd, bool tab32)
+template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev *pd, int tabk)
 {
     if (npairs <= 0) return XC_OK;
     if constexpr (sizeof(T) == 8) {
-        if (tab32) return set_err(ctx, XC_E_ARG, "FP32 tables with FP64 transforms");
+        if (tabk != TK_F64) return set_err(ctx, XC_E_ARG, "FP32 or exact tables with FP64 transforms");
         hipLaunchKernelGGL((k_finalize<T, double>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
-    } else if (tab32) {
+    } else if (tabk == TK_F32) {
         hipLaunchKernelGGL((k_finalize<T, float>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
+    } else if (tabk == TK_X) {
+        hipLaunchKernelGGL((k_finalize<T, TabX>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
     } else {
         hipLaunchKernelGGL((k_finalize<T, double>), dim3(npairs), dim3(256), 0, ctx->stream, pd, twiddles<T>(ctx));
     }
@@ -71,8 +73,8 @@ template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev
 
 template int launch_row_r2c<double>(xc_ctx *, int, int, const PairDev *, const int4 *);
 template int launch_row_r2c<float>(xc_ctx *, int, int, const PairDev *, const int4 *);
-template int launch_finalize<double>(xc_ctx *, int, const PairDev *, bool);
-template int launch_finalize<float>(xc_ctx *, int, const PairDev *, bool);
+template int launch_finalize<double>(xc_ctx *, int, const PairDev *, int);
+template int launch_finalize<float>(xc_ctx *, int, const PairDev *, int);
 template int row_group<double>(int);
 template int row_group<float>(int);
 template int r2c_rows<double>(int);

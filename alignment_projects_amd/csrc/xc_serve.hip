@@ -95,7 +95,7 @@ int make_pair_dev(xc_server *S, const xc_pair &q, PairDev &D)
     D.offy = q.b->oy - q.a->oy;
     D.rule = make_rule(&q.rgn, &q.cnt, nullptr);
     D.need_cnt = D.rule.cnt_enabled;
-    D.tab32 = q.a->dev.tab32;
+    D.tabk = q.a->dev.tabk;
     narrow_rows(D);
     D.A = q.a->dev.v;
     D.B = q.b->dev.v;
@@ -507,7 +507,7 @@ extern "C" int xc_server_submit(xc_server *S, const xc_pair *pair, uint64_t *tic
     if (!S || !pair || !ticket || !pair->a || !pair->b) return XC_E_ARG;
     if (pair->angle_deg != 0.0 || (pair->rgn.enabled && pair->rgn.table)) return XC_E_UNSUPPORTED;
     if (pair->mode != XC_MODE_NCC && pair->mode != XC_MODE_R && pair->mode != XC_MODE_S) return XC_E_ARG;
-    if (pair->a->dev.tab32 != pair->b->dev.tab32) return XC_E_ARG;  // one table precision per pair
+    if (pair->a->dev.tabk != pair->b->dev.tabk) return XC_E_ARG;  // one table form per pair
     PairDev D;
     make_pair_dev(S, *pair, D);
     if (D.Nx / 2 > max_nh<float>() || D.Ny > max_ny<float>()) return XC_E_SHAPE;

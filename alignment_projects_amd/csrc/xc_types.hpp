@@ -54,18 +54,24 @@ struct ImgView {
     int32_t pad;
 };
 
+// Forms of a summed-area table of x and x^2 (IntegrateImage, Correlation.cpp:49-105):
+//  TK_F64: interleaved (S, S^2) FP64 pairs, 16 bytes a pixel (the reference keeps double tables);
+//  TK_F32: interleaved FP32 pairs, each the FP64 sum rounded once (tiles that ask for them);
+//  TK_X  : exact integer sums of an unmasked 8-bit image of at most 2^20 pixels, one uint64
+//          (Sraw << 36) + Qraw a pixel, Sraw / Qraw the sums of the raw values / their squares.
+enum TabKind { TK_F64 = 0, TK_F32 = 1, TK_X = 2 };
+
 // Per-image preparation record (statistics and summed-area tables).
 struct TileDev {
     ImgView v;
     double *msd;         // out: (mean, sd, 1/sd)
-    void *SQ;            // out: summed-area tables of x and x^2, interleaved per pixel, w x h (FP64 pairs,
-                         //      or FP32 pairs when tab32 is set)
+    void *SQ;            // out: summed-area tables of x and x^2 per pixel, w x h, in the form tabk names
     int32_t *Z;          // out: non-zero count table, w x h
     double *part;        // scratch: [TP_PARTS][3] statistics partials
     double *bandp;       // scratch: column sums of the table sweep's row bands (nbands > 1)
     int32_t normalize;
     int32_t nbands;      // row bands of the table sweep (1: one workgroup walks every row)
-    int32_t tab32;       // SQ holds FP32 (S, S^2) pairs (else FP64 pairs)
+    int32_t tabk;        // TabKind of SQ
     int32_t pad;
 };
 
@@ -101,12 +107,12 @@ struct PairDev {
     int32_t need_cnt;        // count predicate active (nz tables needed)
     int32_t filter;          // PM_P2I: Gaussian low-pass of the cross power (bFilter)
     int32_t ylo, yhi;        // signed lag rows computed; rows outside hold no legal lag (all 0, no candidate)
-    int32_t tab32;           // SQA / SQB hold FP32 (S, S^2) pairs (else FP64 pairs)
+    int32_t tabk;            // TabKind of SQA and SQB
     double norm;             // PM_P2I: np1 * N^2
     double mincor;
     RuleDev rule;
     ImgView A, B;                     // source images
-    const void *SQA, *SQB;             // summed-area tables (S, S^2) interleaved, FP64 pairs (FP32 pairs if tab32)
+    const void *SQA, *SQB;            // summed-area tables of x and x^2 (form: tabk)
     const int32_t *ZA, *ZB;           // non-zero count tables
     void *specA, *specB;              // cpx<T> [h][Nx/2+1] row spectra
     void *lag;                        // cpx<T> [hR][Nx/2+1] lag rows

@@ -288,13 +288,20 @@ class Context:
 
     # ---- tiles and batched plans ----
 
-    def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0, tables32=False):
+    def tile_put(self, img, mask=None, normalize=True, device_ptr=None, dtype=None, stride=0, tables32=False,
+                 tables=None):
         """Upload (or adopt, with device_ptr) one image as a device-resident tile.
 
-        tables32: keep the summed-area tables as FP32 (S, S^2) entries instead of FP64 ones."""
+        tables: the summed-area tables' form -- None (exact integer sums for an unmasked uint8 tile of at most
+        2^20 pixels, else FP64 (S, S^2) pairs), "f64" (FP64 pairs always) or "f32" (FP32 pairs; also
+        tables32=True)."""
         flags = abi.XC_TILE_NORMALIZE if normalize else 0
-        if tables32:
+        if tables32 or tables == "f32":
             flags |= abi.XC_TILE_F32_TABLES
+        elif tables == "f64":
+            flags |= abi.XC_TILE_F64_TABLES
+        elif tables is not None:
+            raise ValueError(f"tables: {tables!r}")
         if device_ptr is not None:
             flags |= abi.XC_TILE_DEVICE_SRC
             h, w = img

@@ -232,19 +232,31 @@ enum xc_dtype { XC_U8 = 0, XC_U16 = 1, XC_F32 = 2, XC_F64 = 3 };
 enum xc_tile_flags {
     XC_TILE_NORMALIZE = 1,   /* Normalize() the valid pixels: mean 0, sd 1 (n-1), Maths.cpp:192-212 */
     XC_TILE_DEVICE_SRC = 2,  /* `data`/`mask` are device pointers (else host) */
-    XC_TILE_F32_TABLES = 4   /* keep the summed-area tables as FP32 (S, S^2) entries (each the FP64 sum
-                                rounded once) instead of FP64 ones: half the table bytes, for callers that
-                                accept the rounding; every pair of a batch reads tables of one precision */
+    XC_TILE_F32_TABLES = 4,  /* keep the summed-area tables as FP32 (S, S^2) entries (each the FP64 sum
+                                rounded once): half the table bytes, for callers that accept the rounding;
+                                such tiles pair only with tiles of FP32 tables */
+    XC_TILE_F64_TABLES = 8   /* keep FP64 (S, S^2) entries where exact integer tables would apply */
 };
 
 /*
  * Upload (or adopt from device memory) one w x h image with optional
  * validity mask (non-zero = pixel present in the point list; absent pixels
  * are 0 in the raster, as ImageFromValuesAndPoints leaves them).  The tile
- * keeps its pixels, (mean, sd) and its summed-area tables resident in HBM:
- * S and S^2 as interleaved FP64 pairs (IntegrateImage's double tables,
- * Correlation.cpp:49-105), or FP32 pairs with XC_TILE_F32_TABLES, plus the
- * int32 non-zero count table.  The tile's raster is the bounding box of its
+ * keeps its pixels, (mean, sd) and its summed-area tables of S and S^2
+ * (IntegrateImage, Correlation.cpp:49-105) resident in HBM, plus the int32
+ * non-zero count table.  The tables' form:
+ *   - an unmasked XC_U8 tile of at most 2^20 pixels keeps exact integer
+ *     sums, 8 bytes a pixel (the raw values' sum < 2^28 and their squares'
+ *     sum < 2^36 packed in one uint64): every box sum is exact, and GetCorr's
+ *     variance terms n q - s^2 = (n Qraw - Sraw^2) / sd^2 are formed in
+ *     64-bit integers, so they carry no rounding at all until the final
+ *     scaling -- the FP64 tables' content without their rounding;
+ *   - any other tile (or one flagged XC_TILE_F64_TABLES) keeps interleaved
+ *     FP64 pairs, the reference's double tables, 16 bytes a pixel;
+ *   - XC_TILE_F32_TABLES keeps FP32 pairs (each the FP64 sum rounded once).
+ * A plan pairs exact and FP64 tables by deriving an FP64 table of the exact
+ * tile (also for a rotated A, whose copy has FP64 tables); the server needs
+ * both tiles of a request in one form (else XC_E_ARG).  The tile's raster is the bounding box of its
  * present pixels, as the reference's raster spans the bounding box of the
  * point list (BBoxFromPoints, Geometry.cpp:78-106): a mask whose edge rows
  * or columns are empty shrinks it; displacements stay in the picture's
@@ -279,9 +291,10 @@ typedef struct xc_pair {
 } xc_pair;
 
 /*
- * Correlate n pairs in FP32 (FFTs, multiply) with FP64 summed-area tables
- * (FP32 ones for tiles made with XC_TILE_F32_TABLES) and FP64 variance terms;
- * the final num / sqrt(d) of GetCorr is taken in FP32.  Blocks until
+ * Correlate n pairs in FP32 (FFTs, multiply) with the tiles' summed-area
+ * tables (exact integer or FP64 sums; FP32 ones for tiles made with
+ * XC_TILE_F32_TABLES) and FP64 (exact-integer for exact tables) variance
+ * terms; the final num / sqrt(d) of GetCorr is taken in FP32.  Blocks until
  * results[0..n) are written.
  */
 int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *results);

@@ -1,5 +1,6 @@
 """The batched FP32 form at the strip workload's full size: 2048 x 256 u8 strips, FFT 4096 x 512
-(two 2048-point rows per thread in the row r2c, 512-point column pass), FP64 summed-area tables.
+(two 2048-point rows per thread in the row r2c, 512-point column pass), exact integer summed-area
+tables (the default for unmasked u8 tiles) and, on the whole batch, FP64 and FP32 (S, S^2) tables.
 
 Bar (the batched form's): integer peak equal to the reference path's, subpixel error <= 0.05 px,
 correlation value within 1e-4 absolute.  The reference path is the FP64 oracle on a sample and the
@@ -63,29 +64,38 @@ def test_strips_full_size_vs_oracle(ctx, strips):
 
 
 def test_strips_full_batch_vs_plain_fp64(ctx, strips):
-    """The whole 256-pair batch against the plain FP64 GPU path, pair by pair; the FP32-table variant
-    is run on the same batch and must find the same integer peaks."""
+    """The whole 256-pair batch against the plain FP64 GPU path, pair by pair, with exact integer tables
+    (the default) and with FP64 (S, S^2) tables, each to the batched bar; the FP32-table variant is run on
+    the same batch and must find the same integer peaks."""
     A, B, _ = strips
     n = len(A)
     rgn = abi.rgn_min(W // 2, H // 2)
     out = {}
-    for t32 in (False, True):
-        tiles = [(ctx.tile_put(A[i], tables32=t32), ctx.tile_put(B[i], tables32=t32)) for i in range(n)]
+    for form in (None, "f64", "f32"):
+        tiles = [(ctx.tile_put(A[i], tables=form), ctx.tile_put(B[i], tables=form)) for i in range(n)]
+        if form is None:  # exact tables: 8 bytes a pixel against the FP64 pairs' 16
+            t64 = ctx.tile_put(A[0], tables="f64")
+            assert abs((t64.nbytes - tiles[0][0].nbytes) - 8 * W * H) <= 512, (t64.nbytes, tiles[0][0].nbytes)
+            t64.release()
         plan = ctx.plan([make_pair(ta, tb, rgn=rgn) for ta, tb in tiles], refresh_tiles=True)
-        out[t32] = plan.run()
+        out[form] = plan.run()
         plan.destroy()
         for ta, tb in tiles:
             ta.release()
             tb.release()
-    rerr = []
+    rerr = {None: [], "f64": []}
     for i in range(n):
         pa, va = _norm_points(A[i])
         pb, vb = _norm_points(B[i])
         g = ctx.corr_patches(pa, va, pb, vb, 0, 0, 4096, rgn=rgn)
-        _check(out[False][i], g, i)
-        assert (out[True][i].lag_x, out[True][i].lag_y) == (g[4].lag_x, g[4].lag_y), i
-        rerr.append(abs(out[False][i].r - g[1]))
-    print(f"256 strips vs plain FP64: r error max {max(rerr):.3g}", flush=True)
+        for form in (None, "f64"):
+            _check(out[form][i], g, (i, form))
+            rerr[form].append(abs(out[form][i].r - g[1]))
+        assert (out["f32"][i].lag_x, out["f32"][i].lag_y) == (g[4].lag_x, g[4].lag_y), i
+        # exact and FP64 tables differ only in the variance terms' rounding
+        assert abs(out[None][i].r - out["f64"][i].r) <= 1e-6, (i, out[None][i].r, out["f64"][i].r)
+    print(f"256 strips vs plain FP64: r error max {max(rerr[None]):.3g} (exact tables), "
+          f"{max(rerr['f64']):.3g} (FP64 tables)", flush=True)
 
 
 def test_strips_far_corner_small_overlaps(ctx, strips):
