@@ -149,11 +149,15 @@ __host__ __device__ __forceinline__ cor_t<TB> box_c(const void *__restrict__ t, 
     return r;
 }
 
-// An image's normalisation, which exact tables apply at lookup: mean and 1/sd.
+// An image's normalisation, which exact tables apply at lookup: mean, 1/sd and 1/sd^2.
 struct NmD {
-    double m, i;
+    double m, i, i2;
 };
-__device__ __forceinline__ NmD nm_of(const ImgView &v) { return NmD{v.msd[0], v.msd[2]}; }
+__device__ __forceinline__ NmD nm_of(const ImgView &v)
+{
+    const double i = v.msd[2];
+    return NmD{v.msd[0], i, i * i};
+}
 
 #pragma clang fp contract(off)
 // GetCorr's numerator and variance product (Correlation.cpp:716-754) from two
@@ -161,15 +165,26 @@ __device__ __forceinline__ NmD nm_of(const ImgView &v) { return NmD{v.msd[0], v.
 //   s = (Sraw - n mean) / sd   and   n q - s^2 = (n Qraw - Sraw^2) / sd^2,
 // the bracket formed in 64-bit integers (n Qraw < 2^56): the variance
 // subtraction is exact and only its final scaling rounds.  nraw: n raw / Nxy.
-__device__ __forceinline__ void tabx_num_d(long long n, double nraw, unsigned long long a, unsigned long long b, NmD A,
+// n <= 2^20 (the box lies inside a tile of at most 2^20 pixels), so every
+// product is one 32 x 32 -> 64-bit multiply.
+__device__ __forceinline__ unsigned long long tabx_var(unsigned n, unsigned long long a, unsigned *S)
+{
+    *S = (unsigned)(a >> TABX_QB);                                   // < 2^28
+    const unsigned qlo = (unsigned)a, qhi = (unsigned)(a >> 32) & 15u;  // Qraw = qhi 2^32 + qlo
+    const unsigned long long nq = (unsigned long long)n * qlo + ((unsigned long long)(n * qhi) << 32);
+    return nq - (unsigned long long)*S * *S;
+}
+
+__device__ __forceinline__ void tabx_num_d(unsigned n, double nraw, unsigned long long a, unsigned long long b, NmD A,
                                            NmD B, double *num, double *d)
 {
-    const long long Sa = (long long)(a >> TABX_QB), Qa = (long long)(a & TABX_QMASK);
-    const long long Sb = (long long)(b >> TABX_QB), Qb = (long long)(b & TABX_QMASK);
-    const double s1 = ((double)Sa - (double)n * A.m) * A.i;
-    const double s2 = ((double)Sb - (double)n * B.m) * B.i;
-    const double d1 = (double)(n * Qa - Sa * Sa) * (A.i * A.i);
-    const double d2 = (double)(n * Qb - Sb * Sb) * (B.i * B.i);
+    unsigned Sa, Sb;
+    const unsigned long long Da = tabx_var(n, a, &Sa), Db = tabx_var(n, b, &Sb);
+    const double dn = (double)n;
+    const double s1 = ((double)Sa - dn * A.m) * A.i;
+    const double s2 = ((double)Sb - dn * B.m) * B.i;
+    const double d1 = (double)Da * A.i2;
+    const double d2 = (double)Db * B.i2;
     *num = nraw - s1 * s2;
     *d = d1 * d2;
 }
@@ -232,7 +247,7 @@ __device__ __forceinline__ double eval_lag(const PairDev &P, const RowBox &rb, i
     const cor_t<FT> sqb = box_c<FT>(P.SQB, P.w2, BL, BR, rb.b_b, rb.b_t);
     double num, d;
     if constexpr (is_tabx<FT>()) {
-        tabx_num_d((long long)olw * rb.olh, n * raw * invNxy, sqa, sqb, nm_of(P.A), nm_of(P.B), &num, &d);
+        tabx_num_d((unsigned)(olw * rb.olh), n * raw * invNxy, sqa, sqb, nm_of(P.A), nm_of(P.B), &num, &d);
     } else {
         double s1 = sqa.x, q1 = sqa.y, s2 = sqb.x, q2 = sqb.y;
         num = n * raw * invNxy - s1 * s2;
@@ -893,8 +908,9 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
 // Exact integer tables (TabX) of an unmasked u8 tile.  Each thread owns CPT
 // consecutive columns; a row's CPT pixels arrive in one load, issued LA rows
 // ahead so that the store traffic of the rows between does not stall the
-// sweep.  The row's integer prefixes (v, v^2 [, non-zero]) are scanned across
-// the row (DPP wavefront scan + cross-wave exchange, one barrier per row) and
+// sweep.  The rows' integer prefixes (v, v^2 [, non-zero]) are scanned across
+// the row (DPP wavefront scans + a cross-wave exchange of RB rows at a time,
+// one barrier per exchange) and
 // each column's packed running sum (Sraw << 36) + Qraw gains the row's prefix;
 // a row prefix fits 32 bits (w <= TP_MAX_W).  WZ: the count table too
 // (pixels whose value rounded to T is non-zero, as the other sweeps count them).
@@ -920,6 +936,8 @@ __device__ __forceinline__ typename PixWord<CPT>::type load_pix(const uint8_t *r
     return v;
 }
 
+constexpr int SATX_RB = 4;  // rows per barrier of the exact sweep
+
 template <typename T, int CPT, int NT, bool WZ>
 __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi)
 {
@@ -927,6 +945,7 @@ __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi)
     constexpr int NWV = NT / 64;
     constexpr int NQ = WZ ? 3 : 2;  // S, Q [, non-zero]
     constexpr int LA = 16;          // rows loaded ahead of the sweep
+    constexpr int RB = SATX_RB;     // rows scanned per cross-wave exchange (one barrier)
     static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
     static_assert(CPT == 2 || CPT == 4 || CPT == 8, "one load per row");
     const int w = D.v.w, h = D.v.h;
@@ -948,62 +967,77 @@ __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi)
 #pragma unroll
     for (int a = 0; a < LA; ++a) nxt[a] = (a < h && x0 < w) ? load_pix<CPT>(src + a * stride, x0, w, vec) : 0;
     const bool pair_st = (w % 2) == 0;  // 16-byte aligned column pairs
-    for (int y = 0; y < h; ++y) {
-        const W cur = nxt[0];
+    for (int y0 = 0; y0 < h; y0 += RB) {
+        int pf[RB][NQ][CPT];  // per-thread row prefixes of RB rows
 #pragma unroll
-        for (int a = 0; a + 1 < LA; ++a) nxt[a] = nxt[a + 1];
-        nxt[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(src + (y + LA) * stride, x0, w, vec) : 0;
-        int pf[NQ][CPT];
+        for (int r = 0; r < RB; ++r) {
+            const int y = y0 + r;
+            const W cur = nxt[0];
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const int v = (int)((cur >> (8 * j)) & 255);
-            pf[0][j] = (j ? pf[0][j - 1] : 0) + v;
-            pf[1][j] = (j ? pf[1][j - 1] : 0) + v * v;
-            if constexpr (WZ) {
-                const bool nz = x0 + j < w && (T)(((double)v - mean) * inv) != (T)0;
-                pf[2][j] = (j ? pf[2][j - 1] : 0) + (nz ? 1 : 0);
+            for (int a = 0; a + 1 < LA; ++a) nxt[a] = nxt[a + 1];
+            nxt[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(src + (y + LA) * stride, x0, w, vec) : 0;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+                const int v = y < h ? (int)((cur >> (8 * j)) & 255) : 0;
+                pf[r][0][j] = (j ? pf[r][0][j - 1] : 0) + v;
+                pf[r][1][j] = (j ? pf[r][1][j - 1] : 0) + v * v;
+                if constexpr (WZ) {
+                    const bool nz = y < h && x0 + j < w && (T)(((double)v - mean) * inv) != (T)0;
+                    pf[r][2][j] = (j ? pf[r][2][j - 1] : 0) + (nz ? 1 : 0);
+                }
             }
         }
-        int base[NQ];
-        const int par = (y & 1) * NQ * NWV;
+        int base[RB][NQ];
+        const int par = ((y0 / RB) & 1) * RB * NQ * NWV;
 #pragma unroll
-        for (int qi = 0; qi < NQ; ++qi) {
-            const int tot = pf[qi][CPT - 1];
-            const int inc = wave_scan_i(tot);
-            if (lane == 63) xi[par + qi * NWV + wv] = inc;
-            base[qi] = inc - tot;
-        }
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi) {
+                const int tot = pf[r][qi][CPT - 1];
+                const int inc = wave_scan_i(tot);
+                if (lane == 63) xi[par + (r * NQ + qi) * NWV + wv] = inc;
+                base[r][qi] = inc - tot;
+            }
         __syncthreads();
         if constexpr (NWV > 1) {
             const bool left = lane < wv;
             const int k = left ? lane : 0;
 #pragma unroll
-            for (int qi = 0; qi < NQ; ++qi) base[qi] += row16_sum_i(left ? xi[par + qi * NWV + k] : 0);
+            for (int r = 0; r < RB; ++r)
+#pragma unroll
+                for (int qi = 0; qi < NQ; ++qi)
+                    base[r][qi] += row16_sum_i(left ? xi[par + (r * NQ + qi) * NWV + k] : 0);
         }
 #pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            acc[j] += ((unsigned long long)(unsigned)(base[0] + pf[0][j]) << TABX_QB) +
-                      (unsigned long long)(unsigned)(base[1] + pf[1][j]);
-            if constexpr (WZ) az[j] += base[2] + pf[2][j];
-        }
-        const long long row = (long long)y * w;
-        if (pair_st) {
+        for (int r = 0; r < RB; ++r) {
+            const int y = y0 + r;
+            if (y >= h) break;
 #pragma unroll
-            for (int j = 0; j < CPT; j += 2)
-                if (x0 + j < w) {  // w even: x0 + j + 1 < w too
-                    ulonglong2 e;
-                    e.x = acc[j];
-                    e.y = acc[j + 1];
-                    *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
-                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
-                }
-        } else {
+            for (int j = 0; j < CPT; ++j) {
+                acc[j] += ((unsigned long long)(unsigned)(base[r][0] + pf[r][0][j]) << TABX_QB) +
+                          (unsigned long long)(unsigned)(base[r][1] + pf[r][1][j]);
+                if constexpr (WZ) az[j] += base[r][2] + pf[r][2][j];
+            }
+            const long long row = (long long)y * w;
+            if (pair_st) {
 #pragma unroll
-            for (int j = 0; j < CPT; ++j)
-                if (x0 + j < w) {
-                    sq[row + x0 + j] = acc[j];
-                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
-                }
+                for (int j = 0; j < CPT; j += 2)
+                    if (x0 + j < w) {  // w even: x0 + j + 1 < w too
+                        ulonglong2 e;
+                        e.x = acc[j];
+                        e.y = acc[j + 1];
+                        *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
+                        if constexpr (WZ)
+                            *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                    }
+            } else {
+#pragma unroll
+                for (int j = 0; j < CPT; ++j)
+                    if (x0 + j < w) {
+                        sq[row + x0 + j] = acc[j];
+                        if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                    }
+            }
         }
     }
 }
@@ -1049,7 +1083,7 @@ template <typename T, typename TB, int CPT, int NT>
 __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
-    __shared__ int xz[2 * NT / 64], xi[8 * NT / 64];
+    __shared__ int xz[2 * NT / 64], xi[(is_tabx<TB>() ? 6 * SATX_RB : 8) * NT / 64];
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
@@ -1558,8 +1592,8 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
     q.zBt = P.ZB ? P.ZB + bt : nullptr;
     q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
     q.zBb = (P.ZB && rb.b_b > 0) ? P.ZB + bb : nullptr;
-    q.nA = NmD{0.0, 0.0};
-    q.nB = NmD{0.0, 0.0};
+    q.nA = NmD{0.0, 0.0, 0.0};
+    q.nB = NmD{0.0, 0.0, 0.0};
 #ifdef __HIP_DEVICE_COMPILE__
     if constexpr (is_tabx<FT>()) {
         q.nA = nm_of(P.A);
@@ -1624,7 +1658,7 @@ __device__ __forceinline__ double ncc_rc(int olw, int olh, double invN, double r
     } else {
         const double n = (double)(olw * olh);
         double num, d;
-        tabx_num_d((long long)olw * olh, n * raw * invN, a, b, nA, nB, &num, &d);
+        tabx_num_d((unsigned)(olw * olh), n * raw * invN, a, b, nA, nB, &num, &d);
         if constexpr (sizeof(T) == 8) return d < n * n * 1.0E-9 ? 0.0 : num / sqrt(d);
         else return d < n * n * 1.0E-9 ? 0.0 : (double)((float)num * rsqrtf((float)d));
     }

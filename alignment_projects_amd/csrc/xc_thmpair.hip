@@ -10,7 +10,7 @@
 #include <algorithm>
 #include <vector>
 
-#include "xc_host.hpp"
+#include "../../include/xcorr.h"
 
 namespace {
 
@@ -90,7 +90,7 @@ extern "C" int xc_thmpair_read_all(const char *path, xc_thmpair *out, int cap)
     if (!path) return XC_E_ARG;
     std::vector<xc_thmpair> v;
     if (!read_lines(path, v, false, 0, 0, 0, 0, nullptr)) return XC_E_ARG;
-    if (out && cap > 0) memcpy(out, v.data(), sizeof(xc_thmpair) * std::min<size_t>(cap, v.size()));
+    if (out && cap > 0 && !v.empty()) memcpy(out, v.data(), sizeof(xc_thmpair) * std::min<size_t>(cap, v.size()));
     return (int)v.size();
 }
 
