@@ -936,109 +936,177 @@ __device__ __forceinline__ typename PixWord<CPT>::type load_pix(const uint8_t *r
     return v;
 }
 
-constexpr int SATX_RB = 4;  // rows per barrier of the exact sweep
+constexpr int SATX_RB = 4;           // rows per barrier of the exact sweep
+constexpr int SATX_BLK = 32768;      // bytes of pixel rows an exact sweep stages in LDS at a time
+constexpr int SATX_PER = 4;          // 16-byte pieces of a staged block per thread (at most)
 
-template <typename T, int CPT, int NT, bool WZ>
-__device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi)
+// RB rows of the exact sweep from each thread's pixel words pix[r] (rows y0 + r,
+// rows at or past h contribute nothing): integer prefixes across the row, one
+// cross-wave exchange, packed running sums, stores.
+template <typename T, int CPT, int NT, bool WZ, typename W>
+__device__ __forceinline__ void satx_rows(const TileDev &D, int *xi, int y0, const W (&pix)[SATX_RB],
+                                          unsigned long long (&acc)[CPT], int (&az)[CPT], double mean, double inv)
 {
-    typedef typename PixWord<CPT>::type W;
     constexpr int NWV = NT / 64;
     constexpr int NQ = WZ ? 3 : 2;  // S, Q [, non-zero]
-    constexpr int LA = 16;          // rows loaded ahead of the sweep
-    constexpr int RB = SATX_RB;     // rows scanned per cross-wave exchange (one barrier)
-    static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
-    static_assert(CPT == 2 || CPT == 4 || CPT == 8, "one load per row");
+    constexpr int RB = SATX_RB;
     const int w = D.v.w, h = D.v.h;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x0 = tid * CPT;
+    unsigned long long *sq = reinterpret_cast<unsigned long long *>(D.SQ);
+    int pf[RB][NQ][CPT];  // per-thread row prefixes of RB rows
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int y = y0 + r;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            const int v = (y < h && x0 + j < w) ? (int)((pix[r] >> (8 * j)) & 255) : 0;
+            pf[r][0][j] = (j ? pf[r][0][j - 1] : 0) + v;
+            pf[r][1][j] = (j ? pf[r][1][j - 1] : 0) + v * v;
+            if constexpr (WZ) {
+                const bool nz = y < h && x0 + j < w && (T)(((double)v - mean) * inv) != (T)0;
+                pf[r][2][j] = (j ? pf[r][2][j - 1] : 0) + (nz ? 1 : 0);
+            }
+        }
+    }
+    int base[RB][NQ];
+    const int par = ((y0 / RB) & 1) * RB * NQ * NWV;
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+            const int tot = pf[r][qi][CPT - 1];
+            const int inc = wave_scan_i(tot);
+            if (lane == 63) xi[par + (r * NQ + qi) * NWV + wv] = inc;
+            base[r][qi] = inc - tot;
+        }
+    __syncthreads();
+    if constexpr (NWV > 1) {
+        const bool left = lane < wv;
+        const int k = left ? lane : 0;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi) base[r][qi] += row16_sum_i(left ? xi[par + (r * NQ + qi) * NWV + k] : 0);
+    }
+    const bool pair_st = (w % 2) == 0;  // 16-byte aligned column pairs
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int y = y0 + r;
+        if (y >= h) break;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            acc[j] += ((unsigned long long)(unsigned)(base[r][0] + pf[r][0][j]) << TABX_QB) +
+                      (unsigned long long)(unsigned)(base[r][1] + pf[r][1][j]);
+            if constexpr (WZ) az[j] += base[r][2] + pf[r][2][j];
+        }
+        const long long row = (long long)y * w;
+        if (pair_st) {
+#pragma unroll
+            for (int j = 0; j < CPT; j += 2)
+                if (x0 + j < w) {  // w even: x0 + j + 1 < w too
+                    ulonglong2 e;
+                    e.x = acc[j];
+                    e.y = acc[j + 1];
+                    *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
+                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+                if (x0 + j < w) {
+                    sq[row + x0 + j] = acc[j];
+                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                }
+        }
+    }
+}
+
+// Rows of the exact sweep staged through LDS: when rows are 16-byte aligned
+// whole pieces (w, stride and the base address multiples of 16), the
+// workgroup loads a block of BR rows with 16-byte loads, issued while the
+// block before is swept, so that a wait for pixels waits only for the stores
+// of the block before (not for those of the last few rows, as per-thread
+// loads a few rows ahead would); else each thread loads its own pixels LA rows
+// ahead.
+template <typename T, int CPT, int NT, bool WZ>
+__device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi, uint4 *blk)
+{
+    typedef typename PixWord<CPT>::type W;
+    constexpr int LA = 16;        // rows loaded ahead by the per-thread path
+    constexpr int RB = SATX_RB;
+    static_assert(NT / 64 <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
+    static_assert(CPT == 2 || CPT == 4 || CPT == 8, "one load per row");
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x;
+    const int x0 = tid * CPT;
     const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
     const long long stride = D.v.stride;
-    const bool vec = x0 + CPT <= w && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride) % CPT) == 0;
     const double mean = D.v.msd[0], inv = D.v.msd[2];
-    unsigned long long *sq = reinterpret_cast<unsigned long long *>(D.SQ);
     unsigned long long acc[CPT];
     int az[CPT];
-    W nxt[LA];
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
         acc[j] = 0;
         az[j] = 0;
     }
+    const bool staged = ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride | (uintptr_t)w) & 15) == 0;
+    if (staged) {
+        const int cpr = w / 16;  // 16-byte pieces a row
+        int br = SATX_BLK / w;   // rows a block: the LDS block, and at most SATX_PER pieces a thread
+        br = br < (SATX_PER * NT) / cpr ? br : (SATX_PER * NT) / cpr;
+        br = br > 32 ? 32 : br;
+        br = br / RB * RB;  // whole barrier groups (w <= TP_MAX_W = 8192 keeps br >= 4)
+        const int np = br * cpr;
+        uint4 pf[SATX_PER];
+        auto fetch = [&](int yb) {
+#pragma unroll
+            for (int k = 0; k < SATX_PER; ++k) {
+                const int c = tid + NT * k, r = c / cpr;
+                pf[k] = (c < np && yb + r < h)
+                            ? *reinterpret_cast<const uint4 *>(src + (long long)(yb + r) * stride + 16 * (c - r * cpr))
+                            : make_uint4(0, 0, 0, 0);
+            }
+        };
+        fetch(0);
+        for (int yb = 0; yb < h; yb += br) {
+            __syncthreads();  // the block before is consumed
+#pragma unroll
+            for (int k = 0; k < SATX_PER; ++k) {
+                const int c = tid + NT * k;
+                if (c < np) blk[c] = pf[k];
+            }
+            __syncthreads();
+            if (yb + br < h) fetch(yb + br);  // in flight while this block is swept
+            const uint8_t *b = reinterpret_cast<const uint8_t *>(blk);
+            for (int r0 = 0; r0 < br && yb + r0 < h; r0 += RB) {
+                W pix[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    const uint8_t *p = b + (r0 + r) * w + x0;
+                    if constexpr (CPT == 2) pix[r] = x0 < w ? *reinterpret_cast<const uint16_t *>(p) : 0;
+                    else pix[r] = x0 < w ? *reinterpret_cast<const W *>(p) : 0;
+                }
+                satx_rows<T, CPT, NT, WZ, W>(D, xi, yb + r0, pix, acc, az, mean, inv);
+            }
+        }
+        return;
+    }
+    const bool vec = x0 + CPT <= w && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride) % CPT) == 0;
+    W nxt[LA];
 #pragma unroll
     for (int a = 0; a < LA; ++a) nxt[a] = (a < h && x0 < w) ? load_pix<CPT>(src + a * stride, x0, w, vec) : 0;
-    const bool pair_st = (w % 2) == 0;  // 16-byte aligned column pairs
     for (int y0 = 0; y0 < h; y0 += RB) {
-        int pf[RB][NQ][CPT];  // per-thread row prefixes of RB rows
+        W pix[RB];
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
             const int y = y0 + r;
-            const W cur = nxt[0];
+            pix[r] = nxt[0];
 #pragma unroll
             for (int a = 0; a + 1 < LA; ++a) nxt[a] = nxt[a + 1];
             nxt[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(src + (y + LA) * stride, x0, w, vec) : 0;
-#pragma unroll
-            for (int j = 0; j < CPT; ++j) {
-                const int v = y < h ? (int)((cur >> (8 * j)) & 255) : 0;
-                pf[r][0][j] = (j ? pf[r][0][j - 1] : 0) + v;
-                pf[r][1][j] = (j ? pf[r][1][j - 1] : 0) + v * v;
-                if constexpr (WZ) {
-                    const bool nz = y < h && x0 + j < w && (T)(((double)v - mean) * inv) != (T)0;
-                    pf[r][2][j] = (j ? pf[r][2][j - 1] : 0) + (nz ? 1 : 0);
-                }
-            }
         }
-        int base[RB][NQ];
-        const int par = ((y0 / RB) & 1) * RB * NQ * NWV;
-#pragma unroll
-        for (int r = 0; r < RB; ++r)
-#pragma unroll
-            for (int qi = 0; qi < NQ; ++qi) {
-                const int tot = pf[r][qi][CPT - 1];
-                const int inc = wave_scan_i(tot);
-                if (lane == 63) xi[par + (r * NQ + qi) * NWV + wv] = inc;
-                base[r][qi] = inc - tot;
-            }
-        __syncthreads();
-        if constexpr (NWV > 1) {
-            const bool left = lane < wv;
-            const int k = left ? lane : 0;
-#pragma unroll
-            for (int r = 0; r < RB; ++r)
-#pragma unroll
-                for (int qi = 0; qi < NQ; ++qi)
-                    base[r][qi] += row16_sum_i(left ? xi[par + (r * NQ + qi) * NWV + k] : 0);
-        }
-#pragma unroll
-        for (int r = 0; r < RB; ++r) {
-            const int y = y0 + r;
-            if (y >= h) break;
-#pragma unroll
-            for (int j = 0; j < CPT; ++j) {
-                acc[j] += ((unsigned long long)(unsigned)(base[r][0] + pf[r][0][j]) << TABX_QB) +
-                          (unsigned long long)(unsigned)(base[r][1] + pf[r][1][j]);
-                if constexpr (WZ) az[j] += base[r][2] + pf[r][2][j];
-            }
-            const long long row = (long long)y * w;
-            if (pair_st) {
-#pragma unroll
-                for (int j = 0; j < CPT; j += 2)
-                    if (x0 + j < w) {  // w even: x0 + j + 1 < w too
-                        ulonglong2 e;
-                        e.x = acc[j];
-                        e.y = acc[j + 1];
-                        *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
-                        if constexpr (WZ)
-                            *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
-                    }
-            } else {
-#pragma unroll
-                for (int j = 0; j < CPT; ++j)
-                    if (x0 + j < w) {
-                        sq[row + x0 + j] = acc[j];
-                        if constexpr (WZ) D.Z[row + x0 + j] = az[j];
-                    }
-            }
-        }
+        satx_rows<T, CPT, NT, WZ, W>(D, xi, y0, pix, acc, az, mean, inv);
     }
 }
 
@@ -1084,13 +1152,14 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
     __shared__ int xz[2 * NT / 64], xi[(is_tabx<TB>() ? 6 * SATX_RB : 8) * NT / 64];
+    __shared__ uint4 xblk[is_tabx<TB>() ? SATX_BLK / 16 : 1];  // exact sweeps: staged pixel rows
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
     if constexpr (is_tabx<TB>()) {
         // the host gives exact tables to unmasked u8 tiles only, swept unbanded
-        if (D.Z) sat_sweep_x<T, CPT, NT, true>(D, xi);
-        else sat_sweep_x<T, CPT, NT, false>(D, xi);
+        if (D.Z) sat_sweep_x<T, CPT, NT, true>(D, xi, xblk);
+        else sat_sweep_x<T, CPT, NT, false>(D, xi, xblk);
         return;
     } else {
     const bool m = D.v.mask != nullptr;
