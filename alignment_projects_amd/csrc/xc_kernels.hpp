@@ -149,14 +149,30 @@ __host__ __device__ __forceinline__ cor_t<TB> box_c(const void *__restrict__ t, 
     return r;
 }
 
-// An image's normalisation, which exact tables apply at lookup: mean, 1/sd and 1/sd^2.
+// An image's normalisation, which exact tables apply at lookup: mean, 1/sd
+// and 1/sd^2 (FP64); for the FP32 evaluation also the image's exact raw sum
+// T (the table's last entry), its pixel count N, (1/sd) / N and 1/sd^2 in FP32.
 struct NmD {
     double m, i, i2;
+    unsigned long long T;
+    unsigned N;
+    float iN, i2f;
 };
 __device__ __forceinline__ NmD nm_of(const ImgView &v)
 {
     const double i = v.msd[2];
-    return NmD{v.msd[0], i, i * i};
+    return NmD{v.msd[0], i, i * i, 0ULL, 0u, 0.0f, 0.0f};
+}
+// nm_of plus the FP32 evaluation's terms, from the image's exact table (w x h).
+__device__ __forceinline__ NmD nm_of_x(const ImgView &v, const void *tab, int w, int h)
+{
+    NmD r = nm_of(v);
+    const unsigned N = (unsigned)(w * h);
+    r.T = reinterpret_cast<const unsigned long long *>(tab)[N - 1] >> TABX_QB;
+    r.N = N;
+    r.iN = (float)(r.i / (double)N);
+    r.i2f = (float)r.i2;
+    return r;
 }
 
 #pragma clang fp contract(off)
@@ -187,6 +203,34 @@ __device__ __forceinline__ void tabx_num_d(unsigned n, double nraw, unsigned lon
     const double d2 = (double)Db * B.i2;
     *num = nraw - s1 * s2;
     *d = d1 * d2;
+}
+
+// 64-bit integers to FP32 in two halves (each conversion rounds, the fma once more).
+__device__ __forceinline__ float u64_to_f32(unsigned long long v)
+{
+    return fmaf((float)(unsigned)(v >> 32), 4294967296.0f, (float)(unsigned)v);
+}
+__device__ __forceinline__ float i64_to_f32(long long v)
+{
+    return fmaf((float)(int)(v >> 32), 4294967296.0f, (float)(unsigned)v);
+}
+
+// GetCorr's r from exact box sums in the batched form's FP32: every step that
+// cancels is an exact integer -- the variance bracket n Qraw - Sraw^2 and the
+// mean-removed sum Sraw N - n T (= N (Sraw - n mean)) -- and only well-
+// conditioned products, the square root and the quotient are FP32.
+__device__ __forceinline__ double tabx_r32(unsigned n, double raw, double invN, unsigned long long a,
+                                           unsigned long long b, const NmD &A, const NmD &B)
+{
+    unsigned Sa, Sb;
+    const unsigned long long Da = tabx_var(n, a, &Sa), Db = tabx_var(n, b, &Sb);
+    const long long Ea = (long long)((unsigned long long)Sa * A.N) - (long long)((unsigned long long)n * A.T);
+    const long long Eb = (long long)((unsigned long long)Sb * B.N) - (long long)((unsigned long long)n * B.T);
+    const float s1 = i64_to_f32(Ea) * A.iN, s2 = i64_to_f32(Eb) * B.iN;
+    const float d = (u64_to_f32(Da) * A.i2f) * (u64_to_f32(Db) * B.i2f);
+    const float fn = (float)n;
+    const float num = fn * (float)raw * (float)invN - s1 * s2;
+    return d < fn * fn * 1.0E-9f ? 0.0 : (double)(num * rsqrtf(d));
 }
 #pragma clang fp contract(on)
 
@@ -1661,12 +1705,12 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
     q.zBt = P.ZB ? P.ZB + bt : nullptr;
     q.zAb = (P.ZA && rb.a_b > 0) ? P.ZA + ab : nullptr;
     q.zBb = (P.ZB && rb.b_b > 0) ? P.ZB + bb : nullptr;
-    q.nA = NmD{0.0, 0.0, 0.0};
-    q.nB = NmD{0.0, 0.0, 0.0};
+    q.nA = NmD{0.0, 0.0, 0.0, 0ULL, 0u, 0.0f, 0.0f};
+    q.nB = q.nA;
 #ifdef __HIP_DEVICE_COMPILE__
     if constexpr (is_tabx<FT>()) {
-        q.nA = nm_of(P.A);
-        q.nB = nm_of(P.B);
+        q.nA = nm_of_x(P.A, P.SQA, P.w1, P.h1);
+        q.nB = nm_of_x(P.B, P.SQB, P.w2, P.h2);
     }
 #endif
     return q;
@@ -1717,13 +1761,15 @@ __device__ __forceinline__ double ncc_r(int olw, int olh, double invN, double ra
 }
 
 // ncc_r from two boxes' table sums in the table's own form (exact tables:
-// tabx_num_d, the same final step).
+// tabx_r32 in the FP32 form, tabx_num_d and ncc_r's final step in FP64).
 template <typename T, typename TB>
-__device__ __forceinline__ double ncc_rc(int olw, int olh, double invN, double raw, cor_t<TB> a, cor_t<TB> b, NmD nA,
-                                         NmD nB)
+__device__ __forceinline__ double ncc_rc(int olw, int olh, double invN, double raw, cor_t<TB> a, cor_t<TB> b,
+                                         const NmD &nA, const NmD &nB)
 {
     if constexpr (!is_tabx<TB>()) {
         return ncc_r<T>(olw, olh, invN, raw, a.x, a.y, b.x, b.y);
+    } else if constexpr (sizeof(T) == 4) {
+        return tabx_r32((unsigned)(olw * olh), raw, invN, a, b, nA, nB);
     } else {
         const double n = (double)(olw * olh);
         double num, d;
