@@ -27,7 +27,9 @@ namespace xc {
 constexpr int XC_TW_N = 8192;          // twiddle table length (largest Nx)
 constexpr int XC_MAX_NH = 4096;        // largest complex transform length
 
-template <typename T> struct cpx { T x, y; };
+// Complex value; aligned to its full size so that LDS and global accesses move
+// it as one 8-byte (FP32) or 16-byte (FP64) access.
+template <typename T> struct alignas(2 * sizeof(T)) cpx { T x, y; };
 
 template <typename T> __device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T> __device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
