@@ -187,6 +187,7 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     sg.threads = sat_threads(vw);
     sg.count = 1;
     sg.tabk = tabk;
+    sg.stage = dtype == XC_U8;
     sg.list = (const int *)(base + lay.off[iL]);
     ex.sat_groups.push_back(sg);
     ex.fp64 = false;
@@ -535,18 +536,23 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::vector<int> sflat, soff, dflat, doff;
     if (refresh) {
         std::vector<int> widths;
-        std::vector<char> kinds;
+        std::vector<char> kinds, u8;
         for (const xc_tile *t : tiles) {
             widths.push_back(t->w);
             kinds.push_back((char)t->dev.tabk);
+            u8.push_back((char)(t->dtype == XC_U8));
         }
-        make_sat_groups(widths, kinds, sgroups, sflat, soff);
+        make_sat_groups(widths, kinds, u8, sgroups, sflat, soff);
     }
     {
         std::vector<int> widths;
+        std::vector<char> u8(rots.size(), 0);  // rotated copies are FP32 rasters; derived tables read u8 tiles
         for (const RotDev &R : rots) widths.push_back(R.w);
-        for (const xc_tile *t : ders) widths.push_back(t->w);
-        make_sat_groups(widths, dkind, dgroups, dflat, doff);
+        for (const xc_tile *t : ders) {
+            widths.push_back(t->w);
+            u8.push_back((char)(t->dtype == XC_U8));
+        }
+        make_sat_groups(widths, dkind, u8, dgroups, dflat, doff);
     }
     size_t i_slist = lay.add(sizeof(int) * sflat.size() + 16);
     size_t i_dlist = lay.add(sizeof(int) * dflat.size() + 16);

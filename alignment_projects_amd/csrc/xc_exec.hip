@@ -9,22 +9,24 @@
 
 namespace xc {
 
-void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, std::vector<SatGroup> &groups,
-                     std::vector<int> &flat, std::vector<int> &offsets)
+void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, const std::vector<char> &u8,
+                     std::vector<SatGroup> &groups, std::vector<int> &flat, std::vector<int> &offsets)
 {
     groups.clear();
     flat.clear();
     offsets.clear();
-    std::map<std::tuple<int, int, int>, std::vector<int>> m;
+    std::map<std::tuple<int, int, int, int>, std::vector<int>> m;
     for (int i = 0; i < (int)widths.size(); ++i) {
         const int k = i < (int)tabk.size() ? tabk[i] : TK_F64;
-        m[std::make_tuple(sat_cpt(widths[i]), sat_threads(widths[i]), k)].push_back(i);
+        const int b = i < (int)u8.size() && u8[i] ? 1 : 0;
+        m[std::make_tuple(sat_cpt(widths[i]), sat_threads(widths[i]), k, b)].push_back(i);
     }
     for (auto &kv : m) {
         SatGroup g;
         g.cpt = std::get<0>(kv.first);
         g.threads = std::get<1>(kv.first);
         g.tabk = std::get<2>(kv.first);
+        g.stage = std::get<3>(kv.first) != 0;
         g.count = (int)kv.second.size();
         offsets.push_back((int)flat.size());
         for (int i : kv.second) flat.push_back(i);

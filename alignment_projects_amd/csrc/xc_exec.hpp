@@ -38,14 +38,17 @@ struct SatGroup {
     int count = 0;
     int nb = 1;  // row bands per image (the largest of the group's images)
     int tabk = TK_F64;  // TabKind of the group's tables
+    bool stage = false;  // some image is u8: the launch carries the LDS block that stages its rows
     const int *list = nullptr;  // device: tile indices
 };
 
-// Tile indices grouped by sweep shape (sat_cpt(w), sat_threads(w)) and table
-// form (tabk[i], a TabKind; missing entries: TK_F64): fills `groups` and the
+// Tile indices grouped by sweep shape (sat_cpt(w), sat_threads(w)), table
+// form (tabk[i], a TabKind; missing entries: TK_F64) and whether the image is
+// u8 (u8[i]; missing: no): fills `groups` and the
 // concatenated index list `flat` (upload it, then point each group's list
 // into the device copy at its offset).
-void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, std::vector<SatGroup> &groups,
+void make_sat_groups(const std::vector<int> &widths, const std::vector<char> &tabk, const std::vector<char> &u8,
+                     std::vector<SatGroup> &groups,
                      std::vector<int> &flat, std::vector<int> &offsets);
 
 struct Exec {

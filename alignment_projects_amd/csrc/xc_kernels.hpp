@@ -831,123 +831,6 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
     }
 }
 
-// u8 tiles: the rows are scanned in exact integer arithmetic (a row's prefix
-// of v and v^2 fits in 32 bits for w <= TP_MAX_W), the raw sums accumulate
-// down the columns in FP64 (exact integers), and each stored entry is
-// normalised from them over the n present pixels of the box:
-//   S = (Sraw - n mean) / sd,  Q = (Qraw - 2 mean Sraw + n mean^2) / sd^2
-// (unmasked: n = (x+1)(y+1); masked: a third integer scan counts the present
-// pixels).  This is the exact sum of the normalised values; the general sweep
-// sums the values already rounded to T, which differs by the accumulated
-// rounding of T (relative 1e-8 for FP32), far below the transforms' own
-// error.  WZ: the count table (pixels whose value rounded to T is non-zero,
-// as the general sweep counts them) is scanned and written too.  Integer DPP
-// scans are one instruction per step, against three for an FP64 scan.
-template <typename T, typename TB, int CPT, int NT, bool MASK, bool WZ>
-__device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi)
-{
-    constexpr int NWV = NT / 64;
-    constexpr int NQ = 2 + (MASK ? 1 : 0) + (WZ ? 1 : 0);  // S, Q [, present] [, non-zero]
-    static_assert(NWV <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
-    const int w = D.v.w, h = D.v.h;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int x0 = tid * CPT;
-    const double mean = D.v.msd[0], inv = D.v.msd[2];
-    const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
-    const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
-    tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
-    double as[CPT], aq[CPT], an[CPT];
-    int az[CPT];
-    constexpr int LA = 4;  // rows loaded ahead of the sweep
-    int nxt[LA][CPT];      // pixel, or -1 for an absent one
-#pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-        as[j] = 0.0;
-        aq[j] = 0.0;
-        an[j] = 0.0;
-        az[j] = 0;
-        const int x = x0 + j;
-#pragma unroll
-        for (int a = 0; a < LA; ++a) {
-            int v = (x < w && a < h) ? src[(long long)a * D.v.stride + x] : 0;
-            if constexpr (MASK) v = (x < w && a < h && D.v.mask[(long long)a * D.v.mstride + x]) ? v : -1;
-            nxt[a][j] = v;
-        }
-    }
-    const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);
-    for (int y = 0; y < h; ++y) {
-        int pf[NQ][CPT];  // per-thread row prefixes
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            const int raw = nxt[0][j];
-#pragma unroll
-            for (int a = 0; a + 1 < LA; ++a) nxt[a][j] = nxt[a + 1][j];
-            const int x = x0 + j;
-            int nv = (x < w && y + LA < h) ? src[(long long)(y + LA) * D.v.stride + x] : 0;
-            if constexpr (MASK)
-                nv = (x < w && y + LA < h && D.v.mask[(long long)(y + LA) * D.v.mstride + x]) ? nv : -1;
-            nxt[LA - 1][j] = nv;
-            const bool pres = raw >= 0 && x < w;
-            const int v = pres ? raw : 0;
-            pf[0][j] = (j ? pf[0][j - 1] : 0) + v;
-            pf[1][j] = (j ? pf[1][j - 1] : 0) + v * v;
-            if constexpr (MASK) pf[2][j] = (j ? pf[2][j - 1] : 0) + (pres ? 1 : 0);
-            if constexpr (WZ) {
-                const bool nz = pres && (T)(((double)raw - mean) * inv) != (T)0;
-                pf[NQ - 1][j] = (j ? pf[NQ - 1][j - 1] : 0) + (nz ? 1 : 0);
-            }
-        }
-        int base[NQ];
-        const int par = (y & 1) * NQ * NWV;
-#pragma unroll
-        for (int qi = 0; qi < NQ; ++qi) {
-            const int tot = pf[qi][CPT - 1];
-            const int inc = wave_scan_i(tot);
-            if (lane == 63) xi[par + qi * NWV + wv] = inc;
-            base[qi] = inc - tot;
-        }
-        __syncthreads();
-        if constexpr (NWV > 1) {
-            const bool left = lane < wv;
-            const int k = left ? lane : 0;
-#pragma unroll
-            for (int qi = 0; qi < NQ; ++qi) base[qi] += row16_sum_i(left ? xi[par + qi * NWV + k] : 0);
-        }
-        const long long row = (long long)y * w;
-        double sv[CPT], qv[CPT];
-#pragma unroll
-        for (int j = 0; j < CPT; ++j) {
-            as[j] += (double)(base[0] + pf[0][j]);
-            aq[j] += (double)(base[1] + pf[1][j]);
-            double n;
-            if constexpr (MASK) {
-                an[j] += (double)(base[2] + pf[2][j]);
-                n = an[j];
-            } else {
-                n = (double)(x0 + j + 1) * (double)(y + 1);
-            }
-            if constexpr (WZ) az[j] += base[NQ - 1] + pf[NQ - 1][j];
-            sv[j] = (as[j] - n * mean) * inv;
-            qv[j] = ((aq[j] - m2 * as[j]) + n * msq) * inv2;
-        }
-        if (pair_st) {
-#pragma unroll
-            for (int j = 0; j < CPT; j += 2)
-                if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
-                    sq[row + x0 + j + 1] = TabE<TB>::make(sv[j + 1], qv[j + 1]);
-                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
-                }
-        } else {
-#pragma unroll
-            for (int j = 0; j < CPT; ++j)
-                if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
-                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
-                }
-        }
-    }
-}
 
 // Exact integer tables (TabX) of an unmasked u8 tile.  Each thread owns CPT
 // consecutive columns; a row's CPT pixels arrive in one load, issued LA rows
@@ -1094,7 +977,7 @@ __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi, uint4 *bl
         acc[j] = 0;
         az[j] = 0;
     }
-    const bool staged = ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride | (uintptr_t)w) & 15) == 0;
+    const bool staged = blk && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride | (uintptr_t)w) & 15) == 0;
     if (staged) {
         const int cpr = w / 16;  // 16-byte pieces a row
         int br = SATX_BLK / w;   // rows a block: the LDS block, and at most SATX_PER pieces a thread
@@ -1154,6 +1037,227 @@ __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi, uint4 *bl
     }
 }
 
+// FP64 (or FP32) (S, S^2) tables of u8 tiles: the rows are scanned in exact
+// integer arithmetic (a row's prefix of v and v^2 fits in 32 bits for
+// w <= TP_MAX_W), the raw sums accumulate down the columns in FP64 (exact
+// integers), and each stored entry is normalised from them over the n present
+// pixels of the box:
+//   S = (Sraw - n mean) / sd,  Q = (Qraw - 2 mean Sraw + n mean^2) / sd^2
+// (unmasked: n = (x+1)(y+1); masked: a third integer scan counts the present
+// pixels).  This is the exact sum of the normalised values; the general sweep
+// sums the values already rounded to T, which differs by the accumulated
+// rounding of T (relative 1e-8 for FP32), far below the transforms' own
+// error.  WZ: the count table (pixels whose value rounded to T is non-zero,
+// as the general sweep counts them) is scanned and written too.  Integer DPP
+// scans are one instruction per step, against three for an FP64 scan.  RB
+// rows share one cross-wave exchange; pix / pm: the thread's pixel and mask
+// bytes of each row.
+template <typename T, typename TB, int CPT, int NT, bool MASK, bool WZ, typename W>
+__device__ __forceinline__ void satu8_rows(const TileDev &D, int *xi, int y0, const W (&pix)[SATX_RB],
+                                           const W (&pm)[SATX_RB], double (&as)[CPT], double (&aq)[CPT],
+                                           double (&an)[CPT], int (&az)[CPT], double mean, double inv)
+{
+    constexpr int NWV = NT / 64;
+    constexpr int NQ = 2 + (MASK ? 1 : 0) + (WZ ? 1 : 0);  // S, Q [, present] [, non-zero]
+    constexpr int RB = SATX_RB;
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int x0 = tid * CPT;
+    const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
+    tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
+    int pf[RB][NQ][CPT];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int y = y0 + r;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            bool pres = y < h && x0 + j < w;
+            if constexpr (MASK) pres = pres && ((pm[r] >> (8 * j)) & 255) != 0;
+            const int v = pres ? (int)((pix[r] >> (8 * j)) & 255) : 0;
+            pf[r][0][j] = (j ? pf[r][0][j - 1] : 0) + v;
+            pf[r][1][j] = (j ? pf[r][1][j - 1] : 0) + v * v;
+            if constexpr (MASK) pf[r][2][j] = (j ? pf[r][2][j - 1] : 0) + (pres ? 1 : 0);
+            if constexpr (WZ) {
+                const bool nz = pres && (T)(((double)v - mean) * inv) != (T)0;
+                pf[r][NQ - 1][j] = (j ? pf[r][NQ - 1][j - 1] : 0) + (nz ? 1 : 0);
+            }
+        }
+    }
+    int base[RB][NQ];
+    const int par = ((y0 / RB) & 1) * RB * NQ * NWV;
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int qi = 0; qi < NQ; ++qi) {
+            const int tot = pf[r][qi][CPT - 1];
+            const int inc = wave_scan_i(tot);
+            if (lane == 63) xi[par + (r * NQ + qi) * NWV + wv] = inc;
+            base[r][qi] = inc - tot;
+        }
+    __syncthreads();
+    if constexpr (NWV > 1) {
+        const bool left = lane < wv;
+        const int k = left ? lane : 0;
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int qi = 0; qi < NQ; ++qi) base[r][qi] += row16_sum_i(left ? xi[par + (r * NQ + qi) * NWV + k] : 0);
+    }
+    const bool pair_st = (CPT % 2 == 0) && (w % 2 == 0);
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int y = y0 + r;
+        if (y >= h) break;
+        const long long row = (long long)y * w;
+        double sv[CPT], qv[CPT];
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            as[j] += (double)(base[r][0] + pf[r][0][j]);
+            aq[j] += (double)(base[r][1] + pf[r][1][j]);
+            double n;
+            if constexpr (MASK) {
+                an[j] += (double)(base[r][2] + pf[r][2][j]);
+                n = an[j];
+            } else {
+                n = (double)(x0 + j + 1) * (double)(y + 1);
+            }
+            if constexpr (WZ) az[j] += base[r][NQ - 1] + pf[r][NQ - 1][j];
+            sv[j] = (as[j] - n * mean) * inv;
+            qv[j] = ((aq[j] - m2 * as[j]) + n * msq) * inv2;
+        }
+        if (pair_st) {
+#pragma unroll
+            for (int j = 0; j < CPT; j += 2)
+                if (x0 + j < w) {
+                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
+                    sq[row + x0 + j + 1] = TabE<TB>::make(sv[j + 1], qv[j + 1]);
+                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                }
+        } else {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+                if (x0 + j < w) {
+                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
+                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                }
+        }
+    }
+}
+
+// The u8 sweep's pixel (and mask) rows: staged through LDS in blocks when
+// rows are 16-byte aligned whole pieces (as sat_sweep_x stages them; the
+// block holds the pixel rows in its first half and the mask rows in its
+// second), else loaded by each thread LA rows ahead.
+template <typename T, typename TB, int CPT, int NT, bool MASK, bool WZ>
+__device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi, uint4 *blk)
+{
+    typedef typename PixWord<CPT>::type W;
+    constexpr int LA = 16;
+    constexpr int RB = SATX_RB;
+    constexpr int PERH = SATX_PER / 2;  // 16-byte pieces a thread fetches per block, of pixels and of mask
+    static_assert(NT / 64 <= 16, "the cross-wave prefix reads one wave total per lane of a 16-lane row");
+    static_assert(CPT == 2 || CPT == 4 || CPT == 8, "one load per row");
+    const int w = D.v.w, h = D.v.h;
+    const int tid = threadIdx.x;
+    const int x0 = tid * CPT;
+    const uint8_t *src = reinterpret_cast<const uint8_t *>(D.v.data);
+    const uint8_t *msk = D.v.mask;
+    const long long stride = D.v.stride, mstride = D.v.mstride;
+    const double mean = D.v.msd[0], inv = D.v.msd[2];
+    double as[CPT], aq[CPT], an[CPT];
+    int az[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        as[j] = 0.0;
+        aq[j] = 0.0;
+        an[j] = 0.0;
+        az[j] = 0;
+    }
+    uintptr_t al = reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride | (uintptr_t)w;
+    if constexpr (MASK) al |= reinterpret_cast<uintptr_t>(msk) | (uintptr_t)mstride;
+    const int cpr = w / 16;  // 16-byte pieces a row
+    int br = (SATX_BLK / 2) / w;
+    br = cpr > 0 && br > (PERH * NT) / cpr ? (PERH * NT) / cpr : br;
+    br = br > 32 ? 32 : br;
+    br = br / RB * RB;  // whole barrier groups; 0 for rows wider than the half block holds RB of
+    if (blk && (al & 15) == 0 && br > 0) {
+        const int np = br * cpr;
+        uint8_t *bm = reinterpret_cast<uint8_t *>(blk) + SATX_BLK / 2;
+        uint4 pf[PERH], pq[PERH];
+        auto fetch = [&](int yb) {
+#pragma unroll
+            for (int k = 0; k < PERH; ++k) {
+                const int c = tid + NT * k, r = c / cpr, q = c - r * cpr;
+                const bool in = c < np && yb + r < h;
+                pf[k] = in ? *reinterpret_cast<const uint4 *>(src + (long long)(yb + r) * stride + 16 * q)
+                           : make_uint4(0, 0, 0, 0);
+                if constexpr (MASK)
+                    pq[k] = in ? *reinterpret_cast<const uint4 *>(msk + (long long)(yb + r) * mstride + 16 * q)
+                               : make_uint4(0, 0, 0, 0);
+            }
+        };
+        fetch(0);
+        for (int yb = 0; yb < h; yb += br) {
+            __syncthreads();  // the block before is consumed
+#pragma unroll
+            for (int k = 0; k < PERH; ++k) {
+                const int c = tid + NT * k;
+                if (c < np) {
+                    blk[c] = pf[k];
+                    if constexpr (MASK) reinterpret_cast<uint4 *>(bm)[c] = pq[k];
+                }
+            }
+            __syncthreads();
+            if (yb + br < h) fetch(yb + br);  // in flight while this block is swept
+            const uint8_t *b = reinterpret_cast<const uint8_t *>(blk);
+            for (int r0 = 0; r0 < br && yb + r0 < h; r0 += RB) {
+                W pix[RB], pmw[RB];
+#pragma unroll
+                for (int r = 0; r < RB; ++r) {
+                    const int o = (r0 + r) * w + x0;
+                    if constexpr (CPT == 2) {
+                        pix[r] = x0 < w ? *reinterpret_cast<const uint16_t *>(b + o) : 0;
+                        pmw[r] = (MASK && x0 < w) ? *reinterpret_cast<const uint16_t *>(bm + o) : 0;
+                    } else {
+                        pix[r] = x0 < w ? *reinterpret_cast<const W *>(b + o) : 0;
+                        pmw[r] = (MASK && x0 < w) ? *reinterpret_cast<const W *>(bm + o) : 0;
+                    }
+                }
+                satu8_rows<T, TB, CPT, NT, MASK, WZ, W>(D, xi, yb + r0, pix, pmw, as, aq, an, az, mean, inv);
+            }
+        }
+        return;
+    }
+    const bool vec = x0 + CPT <= w && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride) % CPT) == 0;
+    const bool mvec =
+        !MASK || (x0 + CPT <= w && ((reinterpret_cast<uintptr_t>(msk) | (uintptr_t)mstride) % CPT) == 0);
+    W nxt[LA], nxm[LA];
+#pragma unroll
+    for (int a = 0; a < LA; ++a) {
+        nxt[a] = (a < h && x0 < w) ? load_pix<CPT>(src + a * stride, x0, w, vec) : 0;
+        if constexpr (MASK) nxm[a] = (a < h && x0 < w) ? load_pix<CPT>(msk + a * mstride, x0, w, mvec) : 0;
+        else nxm[a] = 0;
+    }
+    for (int y0 = 0; y0 < h; y0 += RB) {
+        W pix[RB], pmw[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int y = y0 + r;
+            pix[r] = nxt[0];
+            pmw[r] = nxm[0];
+#pragma unroll
+            for (int a = 0; a + 1 < LA; ++a) {
+                nxt[a] = nxt[a + 1];
+                nxm[a] = nxm[a + 1];
+            }
+            nxt[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(src + (y + LA) * stride, x0, w, vec) : 0;
+            if constexpr (MASK)
+                nxm[LA - 1] = (y + LA < h && x0 < w) ? load_pix<CPT>(msk + (y + LA) * mstride, x0, w, mvec) : 0;
+        }
+        satu8_rows<T, TB, CPT, NT, MASK, WZ, W>(D, xi, y0, pix, pmw, as, aq, an, az, mean, inv);
+    }
+}
+
 #define XC_SAT_DISPATCH(CALL)                                                   \
     switch (D.v.dtype) {                                                        \
     case XC_U8:                                                                 \
@@ -1192,11 +1296,13 @@ __global__ void __launch_bounds__(NT) k_tp_colsum(const TileDev *__restrict__ ti
 // row parity, one barrier per row) turns column sums into the 2-D prefix.
 // Writes S, Q, Z once.
 template <typename T, typename TB, int CPT, int NT>
-__global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb)
+__global__ void __launch_bounds__(NT)
+    k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb, int stage)
 {
     __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
-    __shared__ int xz[2 * NT / 64], xi[(is_tabx<TB>() ? 6 * SATX_RB : 8) * NT / 64];
-    __shared__ uint4 xblk[is_tabx<TB>() ? SATX_BLK / 16 : 1];  // exact sweeps: staged pixel rows
+    __shared__ int xz[2 * NT / 64], xi[8 * SATX_RB * NT / 64];
+    extern __shared__ uint4 xblk_dyn[];  // SATX_BLK bytes when stage: u8 sweeps' staged pixel (and mask) rows
+    uint4 *xblk = stage ? xblk_dyn : nullptr;
     const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
     const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
@@ -1209,11 +1315,11 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
     const bool m = D.v.mask != nullptr;
     if (D.v.dtype == XC_U8 && D.SQ && tile_bands(D) == 1 && D.v.w <= TP_MAX_W) {
         if (m) {
-            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, true, true>(D, xi);
-            else sat_sweep_u8<T, TB, CPT, NT, true, false>(D, xi);
+            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, true, true>(D, xi, xblk);
+            else sat_sweep_u8<T, TB, CPT, NT, true, false>(D, xi, xblk);
         } else {
-            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, false, true>(D, xi);
-            else sat_sweep_u8<T, TB, CPT, NT, false, false>(D, xi);
+            if (D.Z) sat_sweep_u8<T, TB, CPT, NT, false, true>(D, xi, xblk);
+            else sat_sweep_u8<T, TB, CPT, NT, false, false>(D, xi, xblk);
         }
         return;
     }
@@ -1223,16 +1329,19 @@ __global__ void __launch_bounds__(NT) k_tp_sat(const TileDev *__restrict__ tiles
     }
 }
 
-// Columns per thread of the sweep for width w: two up to 1024 threads (a
+// Columns per thread of the sweep for width w: two up to 2048 columns (a
 // wavefront's row stores are then contiguous 1 KB runs), then wider.  The
-// launch uses max(64, w / cpt) threads rounded up to a wavefront multiple.
+// launch uses the power of two >= w / cpt threads (at least 64): few
+// shapes to compile, and images of similar widths share a launch.
 __host__ __device__ inline int sat_cpt(int w) { return w <= 2048 ? 2 : w <= 4096 ? 4 : 8; }
+
 __host__ __device__ inline int sat_threads(int w)
 {
     const int c = sat_cpt(w);
-    int n = (w + c - 1) / c;
-    n = (n + 63) / 64 * 64;
-    return n < 64 ? 64 : (n > SAT_MAX_THREADS ? SAT_MAX_THREADS : n);
+    const int need = (w + c - 1) / c;
+    int n = 64;
+    while (n < need && n < SAT_MAX_THREADS) n *= 2;
+    return n;
 }
 
 // ------------------------------------------------------------------

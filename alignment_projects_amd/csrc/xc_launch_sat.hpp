@@ -19,15 +19,11 @@ int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::v
     case c * 10000 + n:                                                                                             \
         if (g.nb > 1)                                                                                               \
             hipLaunchKernelGGL((k_tp_colsum<T, c, n>), dim3(g.count * g.nb), dim3(n), 0, s, d_tiles, g.list, g.nb); \
-        hipLaunchKernelGGL((k_tp_sat<T, TB, c, n>), dim3(g.count * g.nb), dim3(n), 0, s, d_tiles, g.list, g.nb);    \
+        hipLaunchKernelGGL((k_tp_sat<T, TB, c, n>), dim3(g.count * g.nb), dim3(n), g.stage ? SATX_BLK : 0, s,      \
+                           d_tiles, g.list, g.nb, g.stage ? 1 : 0);                                                  \
         break;
-                XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 192) XC_SAT(2, 256) XC_SAT(2, 320) XC_SAT(2, 384)
-                XC_SAT(2, 448) XC_SAT(2, 512) XC_SAT(2, 576) XC_SAT(2, 640) XC_SAT(2, 704) XC_SAT(2, 768)
-                XC_SAT(2, 832) XC_SAT(2, 896) XC_SAT(2, 960) XC_SAT(2, 1024)
-                XC_SAT(4, 576) XC_SAT(4, 640) XC_SAT(4, 704) XC_SAT(4, 768) XC_SAT(4, 832) XC_SAT(4, 896)
-                XC_SAT(4, 960) XC_SAT(4, 1024)
-                XC_SAT(8, 576) XC_SAT(8, 640) XC_SAT(8, 704) XC_SAT(8, 768) XC_SAT(8, 832) XC_SAT(8, 896)
-                XC_SAT(8, 960) XC_SAT(8, 1024)
+                XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 256) XC_SAT(2, 512) XC_SAT(2, 1024) XC_SAT(4, 1024)
+                XC_SAT(8, 1024)
 #undef XC_SAT
             default: return set_err(ctx, XC_E_SHAPE, "no table sweep for %d threads x %d columns", g.threads, g.cpt);
             }

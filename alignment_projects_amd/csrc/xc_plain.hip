@@ -248,7 +248,7 @@ int run_plain(xc_ctx *ctx, int mode, const xc_point *ip1, const double *iv1, int
     if (tbl) memcpy(h2p(i_tbl), rgn->table, tbl);
     std::vector<SatGroup> sgroups;
     std::vector<int> sflat, soff;
-    make_sat_groups({w1, w2}, {}, sgroups, sflat, soff);
+    make_sat_groups({w1, w2}, {}, {}, sgroups, sflat, soff);
     for (SatGroup &g : sgroups) g.nb = std::max(sat_bands(h1), sat_bands(h2));
     memcpy(h2p(i_slist), sflat.data(), sizeof(int) * sflat.size());
     for (size_t gi = 0; gi < sgroups.size(); ++gi) sgroups[gi].list = (const int *)at(i_slist) + soff[gi];
