@@ -260,20 +260,42 @@ struct RotKey {  // a tile and the 2 x 2 map of its copy
     }
 };
 
-// Autotuner of the column pass: for each size bucket, time every shape the
-// device accepts (columns per workgroup x XCD order) on the plan's own data
-// and keep the fastest; a context remembers its choices (and XC_TUNE_FILE
-// keeps them across processes).
-static void tune_save(xc_ctx *ctx, const std::tuple<int, int, int> &key, const xc_col_tune &t)
+// Autotuner of the plan's shapes -- the shapes that change speed only, never
+// results: for each size bucket it times every candidate the device accepts on
+// the plan's own data and keeps the fastest.  Column pass: columns per
+// workgroup (1..16, i.e. 64..1024 threads) x XCD-aware workgroup order; row
+// r2c: rows per thread (two rows per thread sharing barriers and twiddles, or
+// one row per thread group with twice the groups per workgroup).  The radix
+// schedule is not a candidate: another order of passes changes the rounding.
+// A context remembers its choices (and XC_TUNE_FILE keeps them across
+// processes).
+static void tune_save(xc_ctx *ctx, const char *tag, int a, int b, int c, const xc_col_tune &t)
 {
     if (ctx->tune_file.empty()) return;
     FILE *f = fopen(ctx->tune_file.c_str(), "a");
     if (!f) return;
-    fprintf(f, "col %d %d %d %d %d %.3f\n", std::get<0>(key), std::get<1>(key), std::get<2>(key), t.c, t.xcd, t.us);
+    fprintf(f, "%s %d %d %d %d %d %.3f\n", tag, a, b, c, t.c, t.xcd, t.us);
     fclose(f);
 }
 
-static int tune_columns(xc_ctx *ctx, xc_plan *p)
+// Smallest of 3 timed launches of `launch` on the context stream, in us (< 0: the device refused it).
+template <typename F> static double time_launch(xc_ctx *ctx, hipEvent_t e0, hipEvent_t e1, F &&launch)
+{
+    float tmin = 1e30f;
+    for (int rep = 0; rep < 3; ++rep) {
+        const bool ok = hipEventRecord(e0, ctx->stream) == hipSuccess && launch() == XC_OK &&
+                        hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+        float t = 0.f;
+        if (!ok) {
+            (void)hipGetLastError();
+            return -1.0;
+        }
+        if (hipEventElapsedTime(&t, e0, e1) == hipSuccess) tmin = std::min(tmin, t);
+    }
+    return 1e3 * tmin;
+}
+
+static int tune_shapes(xc_ctx *ctx, xc_plan *p)
 {
     int st = enqueue(ctx, p->ex, ST_PREP, ST_COL);  // real spectra for the candidates to transform
     if (st) return st;
@@ -282,6 +304,29 @@ static int tune_columns(xc_ctx *ctx, xc_plan *p)
     XC_HIP(ctx, hipEventCreate(&e0));
     XC_HIP(ctx, hipEventCreate(&e1));
     for (Bucket &b : p->ex.buckets) {
+        // row r2c: rows per thread
+        // (the bucket's own rows: one tuning per (NH, NY), like the column pass's)
+        auto ir = ctx->tune_r2c.find(std::make_pair(b.NH, b.NY));
+        if (ir == ctx->tune_r2c.end()) {
+            xc_col_tune best;
+            best.us = 1e300;
+            for (int rpt : {r2c_default_rpt<float>(b.NH), r2c_alt_rpt<float>(b.NH)}) {
+                if (rpt == 0) continue;
+                const double us = time_launch(ctx, e0, e1, [&] {
+                    return launch_row_r2c<float>(ctx, b.NH, b.n_r2c, p->ex.d_pairs, b.d_r2c, rpt);
+                });
+                if (us >= 0.0 && us < best.us) {
+                    best.c = rpt;
+                    best.us = us;
+                }
+            }
+            if (best.c == 0) best.us = 0.0;
+            ir = ctx->tune_r2c.emplace(std::make_pair(b.NH, b.NY), best).first;
+            tune_save(ctx, "r2c", b.NH, b.NY, 0, best);
+        }
+        b.r2c_rpt = ir->second.c;
+        b.r2c_us = ir->second.us;
+        // column pass: columns per workgroup x XCD order
         const auto key = std::make_tuple(b.NH, b.NY, b.whiten ? 1 : 0);
         auto it = ctx->tune.find(key);
         if (it == ctx->tune.end()) {
@@ -290,29 +335,19 @@ static int tune_columns(xc_ctx *ctx, xc_plan *p)
             for (int c : {1, 2, 4, 8, 16}) {
                 if (!col_shape_ok(ctx, b.NY, c)) continue;
                 for (int x : {1, 0}) {
-                    float tmin = 1e30f;
-                    bool ok = true;
-                    for (int rep = 0; rep < 3 && ok; ++rep) {
-                        ok = hipEventRecord(e0, ctx->stream) == hipSuccess &&
-                             launch_col<float>(ctx, b.NY, b.whiten, c, x, b.n_col, p->ex.d_pairs, b.d_col) == XC_OK &&
-                             hipEventRecord(e1, ctx->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
-                        float t = 0.f;
-                        if (ok && hipEventElapsedTime(&t, e0, e1) == hipSuccess) tmin = std::min(tmin, t);
-                    }
-                    if (!ok) {  // a shape this device refuses: skip it
-                        (void)hipGetLastError();
-                        continue;
-                    }
-                    if (1e3 * tmin < best.us) {
+                    const double us = time_launch(ctx, e0, e1, [&] {
+                        return launch_col<float>(ctx, b.NY, b.whiten, c, x, b.n_col, p->ex.d_pairs, b.d_col);
+                    });
+                    if (us >= 0.0 && us < best.us) {
                         best.c = c;
                         best.xcd = x;
-                        best.us = 1e3 * tmin;
+                        best.us = us;
                     }
                 }
             }
             if (best.c == 0) best.us = 0.0;  // nothing ran: keep the default shape
             it = ctx->tune.emplace(key, best).first;
-            tune_save(ctx, key, best);
+            tune_save(ctx, "col", std::get<0>(key), std::get<1>(key), std::get<2>(key), best);
         }
         b.col_c = it->second.c;
         b.col_xcd = it->second.xcd;
@@ -321,6 +356,31 @@ static int tune_columns(xc_ctx *ctx, xc_plan *p)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->err.clear();
+    return XC_OK;
+}
+
+// (Re)record the plan's launches as its graph.
+static int plan_capture(xc_ctx *ctx, xc_plan *p)
+{
+    if (p->exec) {
+        (void)hipGraphExecDestroy(p->exec);
+        p->exec = nullptr;
+    }
+    if (p->graph) {
+        (void)hipGraphDestroy(p->graph);
+        p->graph = nullptr;
+    }
+    XC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int st = enqueue(ctx, p->ex);
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ce != hipSuccess) return set_err(ctx, XC_E_DEVICE, "graph capture: %s", hipGetErrorString(ce));
+    p->graph = g;
+    XC_HIP(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
     return XC_OK;
 }
 
@@ -740,24 +800,13 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     }
 
     if (flags & XC_PLAN_AUTOTUNE) {
-        st = tune_columns(ctx, p);
+        st = tune_shapes(ctx, p);
         if (st) return st;
     }
 
     // ---- pinned results + graph ----
     XC_HIP(ctx, hipHostMalloc((void **)&p->h_res, sizeof(xc_result) * n, hipHostMallocDefault));
-    XC_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    st = enqueue(ctx, p->ex);
-    hipGraph_t g = nullptr;
-    hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
-    if (st) {
-        if (g) (void)hipGraphDestroy(g);
-        return st;
-    }
-    if (ce != hipSuccess) return set_err(ctx, XC_E_DEVICE, "graph capture: %s", hipGetErrorString(ce));
-    p->graph = g;
-    XC_HIP(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
-    return XC_OK;
+    return plan_capture(ctx, p);
 }
 
 static int plan_create(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, const double *aff4, xc_plan **out)
@@ -797,8 +846,31 @@ extern "C" int xc_plan_buckets(const xc_plan *p, xc_bucket_info *out, int cap)
         o.col_c = b.col_c;
         o.col_xcd = b.col_xcd;
         o.col_us = b.col_us;
+        o.r2c_rpt = b.r2c_rpt ? b.r2c_rpt : r2c_default_rpt<float>(b.NH);
+        o._pad = 0;
+        o.r2c_us = b.r2c_us;
     }
     return nb;
+}
+
+extern "C" int xc_plan_set_shapes(xc_plan *p, int bucket, int col_c, int col_xcd, int r2c_rpt)
+{
+    if (!p || bucket < 0 || bucket >= (int)p->ex.buckets.size()) return XC_E_ARG;
+    xc_ctx *ctx = p->ctx;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    Bucket &b = p->ex.buckets[bucket];
+    if (col_c != 0 && !col_shape_ok(ctx, b.NY, col_c))
+        return set_err(ctx, XC_E_SHAPE, "no column pass of %d columns per workgroup for %d-point columns", col_c, b.NY);
+    if (r2c_rpt != 0 && r2c_rpt != r2c_default_rpt<float>(b.NH) && r2c_rpt != r2c_alt_rpt<float>(b.NH))
+        return set_err(ctx, XC_E_SHAPE, "no row r2c with %d rows per thread for %d-point rows", r2c_rpt, 2 * b.NH);
+    XC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a launch in flight keeps its graph
+    b.col_c = col_c;
+    b.col_xcd = col_xcd ? 1 : 0;
+    b.r2c_rpt = r2c_rpt;
+    b.col_us = 0.0;
+    b.r2c_us = 0.0;
+    return plan_capture(ctx, p);
 }
 
 extern "C" int xc_plan_launch(xc_plan *p)

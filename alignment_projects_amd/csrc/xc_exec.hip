@@ -137,7 +137,7 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     // produced before any bucket's column pass reads it
     if (s0 <= ST_R2C && ST_R2C < s1)
         for (const Bucket &b : ex.buckets)
-            if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, ex.d_pairs, b.d_r2c))) return st;
+            if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, ex.d_pairs, b.d_r2c, b.r2c_rpt))) return st;
     if (s0 <= ST_COL && ST_COL < s1)
         for (const Bucket &b : ex.buckets)
             if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.col_c, b.col_xcd, b.n_col, ex.d_pairs, b.d_col))) return st;

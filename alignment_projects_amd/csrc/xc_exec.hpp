@@ -23,6 +23,8 @@ struct Bucket {
     int col_c = 0;     // columns per workgroup of the column pass (0: default)
     int col_xcd = 1;   // XCD-aware workgroup order in the column pass
     double col_us = 0; // tuned time of the column pass (0: not tuned)
+    int r2c_rpt = 0;   // rows per thread of the row r2c (0: default shape)
+    double r2c_us = 0; // tuned time of the row r2c (0: not tuned)
     const int2 *d_c2r = nullptr;
     int n_c2r = 0;
     bool cnt = false;  // some NCC pair of the bucket has a count rule (c2r variant with the count-table scan)

@@ -64,6 +64,7 @@ struct xc_ctx {
     int num_cu = 0;
     size_t lds_max = 65536;  // LDS bytes one workgroup may allocate
     std::map<std::tuple<int, int, int>, xc_col_tune> tune;  // (NH, NY, whiten) -> column-pass shape
+    std::map<std::pair<int, int>, xc_col_tune> tune_r2c;  // (NH, NY) -> FP32 row r2c shape (c: rows per thread)
     std::string tune_file;  // XC_TUNE_FILE: tuned shapes persist here across processes
 };
 
@@ -158,7 +159,11 @@ int live_rows(const PairDev &P);
     F(1) F(2) F(4) F(8) F(16) F(32) F(64) F(128) F(256) F(512) F(1024) F(2048) F(4096)
 
 // Kernel launchers (dispatch on transform length).
-template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work);
+// rpt: rows per thread of the r2c (0: default shape; see r2c_alt_rpt for the other one).
+template <typename T>
+int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work, int rpt = 0);
+template <typename T> int r2c_default_rpt(int NH);  // rows per thread of the default r2c shape
+template <typename T> int r2c_alt_rpt(int NH);      // the other shape's rows per thread (0: none)
 // cwg: columns per workgroup (0: the default shape); xcd: XCD-aware workgroup order.
 template <typename T>
 int launch_col(xc_ctx *ctx, int NY, bool whiten, int cwg, int xcd, int nitems, const PairDev *pd, const int2 *work);

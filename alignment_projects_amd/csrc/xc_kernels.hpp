@@ -1452,17 +1452,25 @@ template <int NH> struct RowCfg {
 
 // Row r2c shape: RowCfg<NH>'s thread groups, each transforming RPT rows at
 // once (FP32 rows of >= 2048 points: two, sharing barriers and twiddles).
+// ROWS rows make one work item; a launch may cover them with RPT rows per
+// thread in ROWS / RPT thread groups (the autotuner's choice, same results:
+// each row's transform is the same arithmetic in either shape).
 template <typename T, int NH> struct R2cCfg {
     static constexpr int RPT = (sizeof(T) == 4 && Plan<NH>::TPF >= 256) ? 2 : 1;
     static constexpr int ROWS = RowCfg<NH>::G * RPT;  // rows per workgroup (per work item)
+    // the other rows-per-thread shape the tuner may pick (0: none): FP32, >= 512-point rows
+    static constexpr int ALT = (sizeof(T) == 4 && NH >= 256 && ROWS >= 2 && Plan<NH>::TPF * ROWS <= 1024)
+                                   ? (RPT == 2 ? 1 : 2)
+                                   : 0;
+    static constexpr int threads(int rpt) { return Plan<NH>::TPF * (ROWS / rpt); }
 };
 
-template <typename T, int NH>
-__global__ void __launch_bounds__(RowCfg<NH>::THREADS)
+template <typename T, int NH, int RPT = R2cCfg<T, NH>::RPT>
+__global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
     k_row_r2c(const PairDev *__restrict__ pairs, const int4 *__restrict__ work, const cpx<T> *__restrict__ tw)
 {
     using PL = Plan<NH>;
-    constexpr int E = PL::E, TPF = PL::TPF, RPT = R2cCfg<T, NH>::RPT;
+    constexpr int E = PL::E, TPF = PL::TPF;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<T> *smem = reinterpret_cast<cpx<T> *>(smem_raw);
     const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;

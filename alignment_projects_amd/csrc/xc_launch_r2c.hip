@@ -29,25 +29,58 @@ template <typename T> int r2c_rows(int NH)
 
 template <typename T> int col_group(int) { return COL_STEP; }
 
-template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
+template <typename T, int NH, int RPT> static int r2c_go(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w)
+{
+    const size_t smem = sizeof(cpx<T>) * R2cCfg<T, NH>::ROWS * lds_len(NH);
+    constexpr int threads = R2cCfg<T, NH>::threads(RPT);
+    hipLaunchKernelGGL((k_row_r2c<T, NH, RPT>), dim3(nb), dim3(threads), smem, ctx->stream, pd, w, twiddles<T>(ctx));
+    XC_HIP(ctx, hipGetLastError());
+    return XC_OK;
+}
+
+// rpt: rows per thread (0: the default shape; else the default or R2cCfg's ALT)
+template <typename T, int NH> static int r2c_n(xc_ctx *ctx, int nb, const PairDev *pd, const int4 *w, int rpt)
 {
     if constexpr (NH > max_nh<T>()) {
         return set_err(ctx, XC_E_SHAPE, "row transform %d too long", 2 * NH);
     } else {
-        const size_t smem = sizeof(cpx<T>) * R2cCfg<T, NH>::ROWS * lds_len(NH);
-        hipLaunchKernelGGL((k_row_r2c<T, NH>), dim3(nb), dim3(RowCfg<NH>::THREADS), smem, ctx->stream, pd, w,
-                           twiddles<T>(ctx));
-        XC_HIP(ctx, hipGetLastError());
-        return XC_OK;
+        constexpr int DEF = R2cCfg<T, NH>::RPT, ALT = R2cCfg<T, NH>::ALT;
+        if (rpt == 0 || rpt == DEF) return r2c_go<T, NH, DEF>(ctx, nb, pd, w);
+        if constexpr (ALT != 0) {
+            if (rpt == ALT) return r2c_go<T, NH, ALT>(ctx, nb, pd, w);
+        }
+        return set_err(ctx, XC_E_ARG, "no row r2c with %d rows per thread for %d-point rows", rpt, 2 * NH);
     }
 }
 
-template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int4 *w)
+template <typename T> int r2c_alt_rpt(int NH)
+{
+    switch (NH) {
+#define F(n) \
+    case n: return R2cCfg<T, n>::ALT;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return 0;
+}
+
+template <typename T> int r2c_default_rpt(int NH)
+{
+    switch (NH) {
+#define F(n) \
+    case n: return R2cCfg<T, n>::RPT;
+        XC_POW2_CASES(F)
+#undef F
+    }
+    return 1;
+}
+
+template <typename T> int launch_row_r2c(xc_ctx *ctx, int NH, int nb, const PairDev *pd, const int4 *w, int rpt)
 {
     if (nb <= 0) return XC_OK;
     switch (NH) {
 #define F(n) \
-    case n: return r2c_n<T, n>(ctx, nb, pd, w);
+    case n: return r2c_n<T, n>(ctx, nb, pd, w, rpt);
         XC_POW2_CASES(F)
 #undef F
     }
@@ -71,8 +104,12 @@ template <typename T> int launch_finalize(xc_ctx *ctx, int npairs, const PairDev
     return XC_OK;
 }
 
-template int launch_row_r2c<double>(xc_ctx *, int, int, const PairDev *, const int4 *);
-template int launch_row_r2c<float>(xc_ctx *, int, int, const PairDev *, const int4 *);
+template int launch_row_r2c<double>(xc_ctx *, int, int, const PairDev *, const int4 *, int);
+template int launch_row_r2c<float>(xc_ctx *, int, int, const PairDev *, const int4 *, int);
+template int r2c_alt_rpt<float>(int);
+template int r2c_alt_rpt<double>(int);
+template int r2c_default_rpt<float>(int);
+template int r2c_default_rpt<double>(int);
 template int launch_finalize<double>(xc_ctx *, int, const PairDev *, int);
 template int launch_finalize<float>(xc_ctx *, int, const PairDev *, int);
 template int row_group<double>(int);

@@ -78,6 +78,7 @@ _OPTIONAL = {
     "xc_plan_model_bytes": ([C.c_void_p], C.c_double),
     "xc_plan_destroy": ([C.c_void_p], C.c_int),
     "xc_plan_buckets": ([C.c_void_p, C.POINTER(XcBucketInfo), C.c_int], C.c_int),
+    "xc_plan_set_shapes": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
     "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
     "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
@@ -682,11 +683,16 @@ class Plan:
         return self.ctx.L.xc_plan_model_bytes(self.h)
 
     def buckets(self):
-        """Size buckets with their column-pass shapes (tuned ones carry col_us)."""
+        """Size buckets with their launch shapes (tuned ones carry col_us / r2c_us)."""
         nb = self.ctx.L.xc_plan_buckets(self.h, None, 0)
         arr = (XcBucketInfo * max(nb, 1))()
         self.ctx.L.xc_plan_buckets(self.h, arr, nb)
-        return [arr[i].as_dict() for i in range(nb)]
+        return [{k: v for k, v in arr[i].as_dict().items() if k != "_pad"} for i in range(nb)]
+
+    def set_shapes(self, bucket, col_c=0, col_xcd=1, r2c_rpt=0):
+        """Set a bucket's launch shapes by hand (xc_plan_set_shapes)."""
+        st = self.ctx.L.xc_plan_set_shapes(self.h, bucket, col_c, col_xcd, r2c_rpt)
+        self.ctx._check(st, "xc_plan_set_shapes", ok=(abi.XC_OK,))
 
     def destroy(self):
         if self.h:

@@ -307,21 +307,25 @@ int xc_batch_corr_f32(xc_ctx *ctx, const xc_pair *pairs, int n, xc_result *resul
  * XC_PLAN_REFRESH_TILES re-runs the tiles' statistics and summed-area tables
  * inside every run (for tiles whose pixels are rewritten between runs).
  * XC_PLAN_AUTOTUNE times the candidate shapes of the column pass (columns
- * per workgroup 1..16, XCD-aware workgroup order on/off) on each of the
- * plan's size buckets and keeps the fastest; choices are cached per context
- * (and in the file named by XC_TUNE_FILE, if set).  Shapes never change
- * results, only speed.  (The reference plans each FFT per call with
+ * per workgroup 1..16, XCD-aware workgroup order on/off) and of the row r2c
+ * (two rows per thread, or one row per thread group with twice the groups)
+ * on each of the plan's size buckets and keeps the fastest; choices are
+ * cached per context (and in the file named by XC_TUNE_FILE, if set).  Shapes
+ * never change results, only speed; the radix schedule is not tuned (another
+ * order of passes would change the rounding).  (The reference plans each FFT per call with
  * FFTW_ESTIMATE, Correlation_fft_fftw.cpp:20-26.)
  */
 typedef struct xc_plan xc_plan;
 
 enum xc_plan_flags { XC_PLAN_REFRESH_TILES = 1, XC_PLAN_AUTOTUNE = 2 };
 
-/* One size bucket of a plan: transform shape, pairs, column-pass shape. */
+/* One size bucket of a plan: transform shape, pairs, the shapes of its launches. */
 typedef struct xc_bucket_info {
     int32_t nh, ny, whiten, pairs;
     int32_t col_c, col_xcd;  /* columns per workgroup (0: default), XCD order */
     double col_us;           /* tuned column-pass time, microseconds (0: not tuned) */
+    int32_t r2c_rpt, _pad;   /* rows per thread of the row r2c */
+    double r2c_us;           /* tuned row r2c time, microseconds (0: not tuned) */
 } xc_bucket_info;
 
 /* Pipeline stages reported by xc_plan_stage_times. */
@@ -345,6 +349,13 @@ int xc_plan_stage_times(xc_plan *plan, int iters, double *ms, double *bytes);
 double xc_plan_model_bytes(const xc_plan *plan);
 /* Copies up to cap bucket records; returns the plan's bucket count. */
 int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);
+/* Set a bucket's launch shapes by hand (the autotuner's candidates): column
+ * pass columns per workgroup (0: default; 1, 2, 4, 8, 16 where the device's
+ * LDS holds them) and XCD order, row r2c rows per thread (0: default; the
+ * bucket's two shapes are listed by the tuner).  Shapes change speed only,
+ * never results.  Waits for a launch in flight, then records the plan's graph
+ * again.  XC_E_SHAPE for a shape the bucket does not have. */
+int xc_plan_set_shapes(xc_plan *plan, int bucket, int col_c, int col_xcd, int r2c_rpt);
 int xc_plan_destroy(xc_plan *plan);
 
 /*

@@ -139,6 +139,7 @@ def test_autotuned_plan_matches_default(ctx):
     r0, r1 = p0.run(), p1.run()
     b1 = p1.buckets()
     assert len(b1) >= 3 and all(b["col_us"] > 0 and b["col_c"] in (1, 2, 4, 8, 16) for b in b1), b1
+    assert all(b["r2c_us"] > 0 and b["r2c_rpt"] in (1, 2) for b in b1), b1
     fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak")
     for a, b in zip(r0, r1):
         assert all(getattr(a, f) == getattr(b, f) for f in fields), (a.as_dict(), b.as_dict())
@@ -200,3 +201,37 @@ def test_masks_with_empty_edges_match_point_lists(ctx):
         assert abs(rr.r - R) <= 1e-3 * max(1.0, abs(R)), (ang, rr.r, R)
     ta.release()
     tb.release()
+
+
+def test_every_tuned_shape_matches_default(ctx):
+    """Each candidate shape the autotuner can choose -- column-pass columns per workgroup and XCD order, row r2c
+    rows per thread -- set by hand on a plan of 512-point and 2048-point rows: every field of every result is
+    bitwise the default shapes' (4096-point rows: two rows per thread or one per thread group)."""
+    from alignment_projects_amd.xcorr import make_pair
+    data = _strips(4, 2048, 64, 20, 950) + _strips(4, 512, 48, 12, 960)
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data]
+    pairs = [make_pair(ta, tb, rgn=abi.rgn_min(64, 16)) for ta, tb in tiles]
+    plan = ctx.plan(pairs)
+    ref = plan.run()
+    fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak")
+    bk = plan.buckets()
+    assert sorted(b["nh"] for b in bk) == [512, 2048], bk
+    tried = 0
+    for bi, b in enumerate(bk):
+        for rpt in (1, 2):
+            for c in (1, 2, 4, 8, 16):
+                for x in (0, 1):
+                    try:
+                        plan.set_shapes(bi, col_c=c, col_xcd=x, r2c_rpt=rpt)
+                    except Exception:  # a shape this bucket or device does not have
+                        continue
+                    got = plan.run()
+                    for i, (a, g) in enumerate(zip(ref, got)):
+                        assert all(getattr(a, f) == getattr(g, f) for f in fields), (bi, rpt, c, x, i)
+                    tried += 1
+        plan.set_shapes(bi)  # back to the defaults
+    assert tried >= 20, tried
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
