@@ -344,17 +344,19 @@ int xc_plan_run(xc_plan *plan, xc_result *results);
  * stage, bytes[XC_STAGE_COUNT] its algorithmic HBM bytes per run. */
 int xc_plan_stage_times(xc_plan *plan, int iters, double *ms, double *bytes);
 /* Algorithmic HBM bytes of one run of the whole plan (inputs once; row
- * spectra, lag rows and the tables, 16 B per FP64 (S, S^2) entry or 8 B per
- * FP32 one, written once and read once). */
+ * spectra, lag rows and the tables, 8 B per exact integer entry, 16 B per
+ * FP64 (S, S^2) entry or 8 B per FP32 one, written once and read once). */
 double xc_plan_model_bytes(const xc_plan *plan);
 /* Copies up to cap bucket records; returns the plan's bucket count. */
 int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);
 /* Set a bucket's launch shapes by hand (the autotuner's candidates): column
  * pass columns per workgroup (0: default; 1, 2, 4, 8, 16 where the device's
- * LDS holds them) and XCD order, row r2c rows per thread (0: default; the
- * bucket's two shapes are listed by the tuner).  Shapes change speed only,
- * never results.  Waits for a launch in flight, then records the plan's graph
- * again.  XC_E_SHAPE for a shape the bucket does not have. */
+ * LDS holds them) and XCD order, row r2c rows per thread (0: default; 1 or 2
+ * for FP32 rows of 512 points or more whose work item holds two rows, else
+ * only the default).  xc_plan_buckets reports the shapes in use.  Shapes
+ * change speed only, never results.  Waits for a launch in flight, then
+ * records the plan's graph again.  XC_E_SHAPE for a shape the bucket does not
+ * have. */
 int xc_plan_set_shapes(xc_plan *plan, int bucket, int col_c, int col_xcd, int r2c_rpt);
 int xc_plan_destroy(xc_plan *plan);
 
