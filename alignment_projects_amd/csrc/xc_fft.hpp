@@ -128,6 +128,16 @@ __device__ __forceinline__ cpx<T> twiddle(const cpx<T> *__restrict__ tw, int m, 
     return w;
 }
 
+// Workgroup barrier that orders LDS only: this wave's LDS accesses complete,
+// then s_barrier.  Unlike __syncthreads() it does not wait for the wave's
+// global loads and stores, so table loads and row stores issued before it
+// stay in flight across it.  Only for barriers whose purpose is an LDS
+// exchange between the workgroup's threads.
+__device__ __forceinline__ void lds_sync()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // LDS view: element pos of this transform lives at base[(pos + pos/8) *
 // stride].  The one-in-eight padding spreads the radix-8 scatter
 // (pos = 8j + r across lanes j) over all banks instead of two.
@@ -183,7 +193,7 @@ __device__ __forceinline__ void tw_load(TwReg<N, T> &R, int t, const cpx<T> *__r
 }
 
 // One length-N transform in the canonical register layout.  Every thread
-// of the workgroup must call it (it contains __syncthreads), including
+// of the workgroup must call it (it contains workgroup barriers), including
 // threads whose data is padding.
 // Twiddle sources: registers preloaded by tw_load (TwReg), or the table
 // read lazily inside each pass (TwLazy).
@@ -217,9 +227,18 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
         constexpr int NB = E / R;
         constexpr int Ns = PL::stride_before(P);
         constexpr int OFF = TwPlan<N>::count(P);
+        // the pass's twiddles first: table loads issued before the barrier arrive during it
+        cpx<T> w[NB][R];
         if constexpr (P > 0) {
+            static_for<0, NB>([&](auto bI) {
+                constexpr int b = decltype(bI)::value;
+                static_for<1, R>([&](auto rI) {
+                    constexpr int r = decltype(rI)::value;
+                    w[b][r] = tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t);
+                });
+            });
             // reload canonical layout written by the previous pass
-            __syncthreads();
+            lds_sync();
 #pragma unroll
             for (int m = 0; m < E; ++m) v[m] = lds[t + TPF * m];
         }
@@ -229,7 +248,7 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
             static_for<0, R>([&](auto rI) {
                 constexpr int r = decltype(rI)::value;
                 u[b][r] = v[b + r * NB];
-                if constexpr (P > 0 && r > 0) u[b][r] = cmul(u[b][r], tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t));
+                if constexpr (P > 0 && r > 0) u[b][r] = cmul(u[b][r], w[b][r]);
             });
             bfly<DIR, R>(u[b]);
         });
@@ -239,7 +258,7 @@ __device__ __forceinline__ void fft_pass(cpx<T> (&v)[Plan<N>::E], int t, LdsView
 #pragma unroll
                 for (int r = 0; r < R; ++r) v[b + r * NB] = u[b][r];
         } else {
-            __syncthreads();  // everyone finished reading the previous contents
+            lds_sync();  // everyone finished reading the previous contents
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int j = t + TPF * b;
@@ -282,8 +301,16 @@ __device__ __forceinline__ void fft_pass_rows(cpx<T> (&v)[RR][Plan<N>::E], int t
         constexpr int NB = E / R;
         constexpr int Ns = PL::stride_before(P);
         constexpr int OFF = TwPlan<N>::count(P);
+        cpx<T> w[NB][R];
         if constexpr (P > 0) {
-            __syncthreads();
+            static_for<0, NB>([&](auto bI) {
+                constexpr int b = decltype(bI)::value;
+                static_for<1, R>([&](auto rI) {
+                    constexpr int r = decltype(rI)::value;
+                    w[b][r] = tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t);
+                });
+            });
+            lds_sync();
 #pragma unroll
             for (int q = 0; q < RR; ++q)
 #pragma unroll
@@ -295,9 +322,8 @@ __device__ __forceinline__ void fft_pass_rows(cpx<T> (&v)[RR][Plan<N>::E], int t
             static_for<0, R>([&](auto rI) {
                 constexpr int r = decltype(rI)::value;
                 if constexpr (P > 0 && r > 0) {
-                    const cpx<T> w = tw_get<DIR, N, T, OFF, b, r, R, Ns>(TW, t);
 #pragma unroll
-                    for (int q = 0; q < RR; ++q) u[q][b][r] = cmul(v[q][b + r * NB], w);
+                    for (int q = 0; q < RR; ++q) u[q][b][r] = cmul(v[q][b + r * NB], w[b][r]);
                 } else {
 #pragma unroll
                     for (int q = 0; q < RR; ++q) u[q][b][r] = v[q][b + r * NB];
@@ -314,7 +340,7 @@ __device__ __forceinline__ void fft_pass_rows(cpx<T> (&v)[RR][Plan<N>::E], int t
 #pragma unroll
                     for (int r = 0; r < R; ++r) v[q][b + r * NB] = u[q][b][r];
         } else {
-            __syncthreads();  // everyone finished reading the previous contents
+            lds_sync();  // everyone finished reading the previous contents
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int j = t + TPF * b;

@@ -1507,12 +1507,16 @@ __global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
     }
 #undef XC_LOAD_ROW
 
-    __syncthreads();
+    // the split's twiddles, loaded before the exchange so that they arrive during it
+    cpx<T> ws[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) ws[m] = twiddle<-1>(tw, t + TPF * m, 2 * NH);
+    lds_sync();
 #pragma unroll
     for (int q = 0; q < RPT; ++q)
 #pragma unroll
         for (int m = 0; m < E; ++m) L[q][t + TPF * m] = v[q][m];
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
         const int y = y0 + q;
@@ -1525,7 +1529,7 @@ __global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
                 const cpx<T> zn = cconj(L[q][(NH - k) & (NH - 1)]);
                 const cpx<T> a = cscale(zk + zn, (T)0.5);
                 const cpx<T> b = rot90<-1>(cscale(zk - zn, (T)0.5));
-                row[k] = a + cmul(twiddle<-1>(tw, k, 2 * NH), b);
+                row[k] = a + cmul(ws[m], b);
             }
             if (t == 0) row[NH] = cpx<T>{v[q][0].x - v[q][0].y, (T)0};
         }
@@ -2120,31 +2124,37 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
     const cpx<T> *LG = reinterpret_cast<const cpx<T> *>(P.lag) + (long long)yl * H;
     cpx<T> *X = smem + g * RS;
 
-    for (int i = t; i < H; i += TPF) X[i] = valid ? LG[i] : cpx<T>{(T)0, (T)0};
-    __syncthreads();
-
-    cpx<T> v[E];
+    // the split's two inputs X[p] and X[NH - p] straight from the row (the mirrored
+    // read is the same cache lines, read by another wavefront of the workgroup),
+    // every load in flight before the first use
+    cpx<T> xk[E], xn[E], ws[E];
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int p = t + TPF * m;
-        const cpx<T> xk = X[p];
-        const cpx<T> xn = cconj(X[NH - p]);
-        const cpx<T> s = xk + xn, d = xk - xn;
-        v[m] = s + rot90<+1>(cmul(twiddle<+1>(tw, p, 2 * NH), d));
+        xk[m] = valid ? LG[p] : cpx<T>{(T)0, (T)0};
+        xn[m] = valid ? LG[NH - p] : cpx<T>{(T)0, (T)0};
+        ws[m] = twiddle<+1>(tw, p, 2 * NH);
+    }
+    cpx<T> v[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+        const cpx<T> xc = cconj(xn[m]);
+        const cpx<T> s = xk[m] + xc, d = xk[m] - xc;
+        v[m] = s + rot90<+1>(cmul(ws[m], d));
     }
     LdsView<T> L{X, 1};
     fft<+1, NH>(v, t, L, tw);
 
     // raw row back to LDS in natural order, then evaluate with unit-stride lanes
     T *R = reinterpret_cast<T *>(X);
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int p = t + TPF * m;
         R[2 * p] = v[m].x;
         R[2 * p + 1] = v[m].y;
     }
-    __syncthreads();
+    lds_sync();
 
     const int y = lag_y_of_row(P, valid ? yl : 0);
     const RowBox rb = row_box(P, y == DEAD_ROW ? 0 : y);
