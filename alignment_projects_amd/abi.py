@@ -189,7 +189,7 @@ XC_PLAN_AUTOTUNE = 2
 class XcBucketInfo(C.Structure):
     _fields_ = [("nh", C.c_int32), ("ny", C.c_int32), ("whiten", C.c_int32), ("pairs", C.c_int32),
                 ("col_c", C.c_int32), ("col_xcd", C.c_int32), ("col_us", C.c_double),
-                ("r2c_rpt", C.c_int32), ("_pad", C.c_int32), ("r2c_us", C.c_double)]
+                ("r2c_rpt", C.c_int32), ("r2c_np", C.c_int32), ("r2c_us", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

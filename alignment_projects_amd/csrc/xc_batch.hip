@@ -304,27 +304,35 @@ static int tune_shapes(xc_ctx *ctx, xc_plan *p)
     XC_HIP(ctx, hipEventCreate(&e0));
     XC_HIP(ctx, hipEventCreate(&e1));
     for (Bucket &b : p->ex.buckets) {
-        // row r2c: rows per thread
-        // (the bucket's own rows: one tuning per (NH, NY), like the column pass's)
-        auto ir = ctx->tune_r2c.find(std::make_pair(b.NH, b.NY));
+        // row r2c: rows per thread, and for the 8-bit kernel work items per workgroup
+        // (the bucket's own rows: one tuning per (NH, NY, kernel), like the column pass's)
+        const auto rkey = std::make_tuple(b.NH, b.NY, b.d_r2c_u8 ? 1 : 0);
+        auto ir = ctx->tune_r2c.find(rkey);
         if (ir == ctx->tune_r2c.end()) {
             xc_col_tune best;
             best.us = 1e300;
+            best.xcd = 0;
             for (int rpt : {r2c_default_rpt<float>(b.NH), r2c_alt_rpt<float>(b.NH)}) {
                 if (rpt == 0) continue;
-                const double us = time_launch(ctx, e0, e1, [&] {
-                    return launch_row_r2c<float>(ctx, b.NH, b.n_r2c, p->ex.d_pairs, b.d_r2c, rpt);
-                });
-                if (us >= 0.0 && us < best.us) {
-                    best.c = rpt;
-                    best.us = us;
+                for (int np : {1, 2, 4}) {
+                    if (!b.d_r2c_u8 && np > 1) break;
+                    const double us = time_launch(ctx, e0, e1, [&] {
+                        return b.d_r2c_u8 ? launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, rpt, np)
+                                          : launch_row_r2c<float>(ctx, b.NH, b.n_r2c, p->ex.d_pairs, b.d_r2c, rpt);
+                    });
+                    if (us >= 0.0 && us < best.us) {
+                        best.c = rpt;
+                        best.xcd = b.d_r2c_u8 ? np : 0;
+                        best.us = us;
+                    }
                 }
             }
             if (best.c == 0) best.us = 0.0;
-            ir = ctx->tune_r2c.emplace(std::make_pair(b.NH, b.NY), best).first;
-            tune_save(ctx, "r2c", b.NH, b.NY, 0, best);
+            ir = ctx->tune_r2c.emplace(rkey, best).first;
+            tune_save(ctx, "r2c", b.NH, b.NY, b.d_r2c_u8 ? 1 : 0, best);
         }
         b.r2c_rpt = ir->second.c;
+        b.r2c_np = b.d_r2c_u8 ? ir->second.xcd : 0;
         b.r2c_us = ir->second.us;
         // column pass: columns per workgroup x XCD order
         const auto key = std::make_tuple(b.NH, b.NY, b.whiten ? 1 : 0);
@@ -621,17 +629,29 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     int nrw = 0;
     for (const RotDev &R : rots) nrw += (int)((rot_splat_units(R.w, R.h) + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
-    std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size());
+    std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size()), i_w8(buckets.size(), 0);
+    std::vector<bool> r2c8(buckets.size(), false);
     std::vector<int> starts;
+    // the 8-bit row r2c takes an image whose rows it can load as pixel pairs
+    auto u8_rows = [](const ImgView &V, int NH) {
+        return V.dtype == XC_U8 && !V.mask && ((uintptr_t)V.data & 1) == 0 && (V.stride & 1) == 0 && V.w <= NH;
+    };
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
         const int G = row_group<float>(B.NH), C = col_group<float>(B.NY), GR = r2c_rows<float>(B.NH);
         int nr = 0, nc = 0, nc2 = 0;
+        bool u8 = r2c_u8_length(B.NH);
         for (int k = B.first; k < B.first + B.count; ++k) {
             const int i = p->order[k];
             const PairDev &D = P[i];
-            if (ownA[i]) nr += (D.h1 + GR - 1) / GR;
-            if (ownB[i]) nr += (D.h2 + GR - 1) / GR;
+            if (ownA[i]) {
+                nr += (D.h1 + GR - 1) / GR;
+                u8 = u8 && pair_rot[i] < 0 && u8_rows(D.A, B.NH);
+            }
+            if (ownB[i]) {
+                nr += (D.h2 + GR - 1) / GR;
+                u8 = u8 && u8_rows(D.B, B.NH);
+            }
             nc += (D.Nx / 2 + 1 + C - 1) / C;
             live_row_groups(D, G, starts);
             nc2 += (int)starts.size();
@@ -639,6 +659,8 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         B.n_r2c = nr;
         B.n_col = nc;
         B.n_c2r = nc2;
+        r2c8[bi] = u8 && nr > 0;
+        if (r2c8[bi]) i_w8[bi] = lay.add(sizeof(R2cU8) * nr + 16);
         i_wr[bi] = lay.add(sizeof(int4) * nr + 16);
         i_wc[bi] = lay.add(sizeof(int2) * nc);
         i_wc2[bi] = lay.add(sizeof(int2) * nc2);
@@ -740,19 +762,39 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         int4 *wr = (int4 *)hat(i_wr[bi]);
         int2 *wc = (int2 *)hat(i_wc[bi]);
         int2 *wc2 = (int2 *)hat(i_wc2[bi]);
+        R2cU8 *w8 = r2c8[bi] ? (R2cU8 *)hat(i_w8[bi]) : nullptr;
+        // an 8-bit work item: rows y .. y + GR - 1 of view V into its spectrum
+        auto item8 = [&](const ImgView &V, void *spec, int nh, int y) {
+            R2cU8 it;
+            it.src = (const uint8_t *)V.data + (long long)y * V.stride;
+            it.dst = (cpx<float> *)spec + (long long)y * (nh + 1);
+            it.msd = V.msd;
+            it.nrows = std::min(GR, V.h - y);
+            it.w = V.w;
+            it.stride = V.stride;
+            it.pad = 0;
+            return it;
+        };
         int a = 0, c = 0, d = 0;
         for (int k = B.first; k < B.first + B.count; ++k) {
             const int i = p->order[k];
             const PairDev &D = hp[k];
             if (ownA[i])
-                for (int y = 0; y < D.h1; y += GR) wr[a++] = make_int4(k, 0, y, 0);
+                for (int y = 0; y < D.h1; y += GR) {
+                    if (w8) w8[a] = item8(D.A, D.specA, B.NH, y);
+                    wr[a++] = make_int4(k, 0, y, 0);
+                }
             if (ownB[i])
-                for (int y = 0; y < D.h2; y += GR) wr[a++] = make_int4(k, 1, y, 0);
+                for (int y = 0; y < D.h2; y += GR) {
+                    if (w8) w8[a] = item8(D.B, D.specB, B.NH, y);
+                    wr[a++] = make_int4(k, 1, y, 0);
+                }
             for (int x = 0; x < D.Nx / 2 + 1; x += C) wc[c++] = make_int2(k, x);
             live_row_groups(D, G, starts);
             for (int y : starts) wc2[d++] = make_int2(k, y);
         }
         B.d_r2c = (const int4 *)at(i_wr[bi]);
+        B.d_r2c_u8 = r2c8[bi] ? (const R2cU8 *)at(i_w8[bi]) : nullptr;
         B.d_col = (const int2 *)at(i_wc[bi]);
         B.d_c2r = (const int2 *)at(i_wc2[bi]);
     }
@@ -847,13 +889,13 @@ extern "C" int xc_plan_buckets(const xc_plan *p, xc_bucket_info *out, int cap)
         o.col_xcd = b.col_xcd;
         o.col_us = b.col_us;
         o.r2c_rpt = b.r2c_rpt ? b.r2c_rpt : r2c_default_rpt<float>(b.NH);
-        o._pad = 0;
+        o.r2c_np = b.d_r2c_u8 ? (b.r2c_np ? b.r2c_np : R2C_U8_NP) : 0;
         o.r2c_us = b.r2c_us;
     }
     return nb;
 }
 
-extern "C" int xc_plan_set_shapes(xc_plan *p, int bucket, int col_c, int col_xcd, int r2c_rpt)
+extern "C" int xc_plan_set_shapes(xc_plan *p, int bucket, int col_c, int col_xcd, int r2c_rpt, int r2c_np)
 {
     if (!p || bucket < 0 || bucket >= (int)p->ex.buckets.size()) return XC_E_ARG;
     xc_ctx *ctx = p->ctx;
@@ -864,10 +906,13 @@ extern "C" int xc_plan_set_shapes(xc_plan *p, int bucket, int col_c, int col_xcd
         return set_err(ctx, XC_E_SHAPE, "no column pass of %d columns per workgroup for %d-point columns", col_c, b.NY);
     if (r2c_rpt != 0 && r2c_rpt != r2c_default_rpt<float>(b.NH) && r2c_rpt != r2c_alt_rpt<float>(b.NH))
         return set_err(ctx, XC_E_SHAPE, "no row r2c with %d rows per thread for %d-point rows", r2c_rpt, 2 * b.NH);
+    if (r2c_np != 0 && r2c_np != 1 && r2c_np != 2 && r2c_np != 4)
+        return set_err(ctx, XC_E_SHAPE, "no 8-bit row r2c with %d work items per workgroup", r2c_np);
     XC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a launch in flight keeps its graph
     b.col_c = col_c;
     b.col_xcd = col_xcd ? 1 : 0;
     b.r2c_rpt = r2c_rpt;
+    b.r2c_np = b.d_r2c_u8 ? r2c_np : 0;
     b.col_us = 0.0;
     b.r2c_us = 0.0;
     return plan_capture(ctx, p);

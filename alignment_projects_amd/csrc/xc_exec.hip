@@ -136,8 +136,15 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
     // stage-major: a row spectrum shared by pairs of several buckets is
     // produced before any bucket's column pass reads it
     if (s0 <= ST_R2C && ST_R2C < s1)
-        for (const Bucket &b : ex.buckets)
+        for (const Bucket &b : ex.buckets) {
+            if constexpr (sizeof(T) == 4) {
+                if (b.d_r2c_u8) {
+                    if ((st = launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, b.r2c_rpt, b.r2c_np))) return st;
+                    continue;
+                }
+            }
             if ((st = launch_row_r2c<T>(ctx, b.NH, b.n_r2c, ex.d_pairs, b.d_r2c, b.r2c_rpt))) return st;
+        }
     if (s0 <= ST_COL && ST_COL < s1)
         for (const Bucket &b : ex.buckets)
             if ((st = launch_col<T>(ctx, b.NY, b.whiten, b.col_c, b.col_xcd, b.n_col, ex.d_pairs, b.d_col))) return st;

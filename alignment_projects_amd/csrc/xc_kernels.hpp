@@ -1536,6 +1536,96 @@ __global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
     }
 }
 
+// Row r2c of unmasked 8-bit rows (FP32 transforms), NP work items per
+// workgroup in turn: the next item's pixels are loaded while this item's rows
+// are transformed, and the split's twiddles are loaded once for all of them.
+// Each row is the same arithmetic as k_row_r2c's (its 8-bit path), so the
+// spectra are bitwise the same.  The host takes this kernel only for items
+// whose rows start at even addresses with an even stride and fit the upper-half
+// padding (w <= NH): two pixels a 16-bit load.
+template <int NH, int RPT, int NP>
+__global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
+    k_row_r2c_u8(const R2cU8 *__restrict__ items, int nitems, const cpx<float> *__restrict__ tw)
+{
+    using PL = Plan<NH>;
+    constexpr int E = PL::E, TPF = PL::TPF, H = NH + 1;
+    static_assert(E == 8, "8-bit row r2c: 8 elements a thread");
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<float> *smem = reinterpret_cast<cpx<float> *>(smem_raw);
+    const int tid = threadIdx.x, g = tid / TPF, t = tid % TPF;
+    LdsView<float> L[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) L[q] = LdsView<float>{smem + (g * RPT + q) * lds_len(NH), 1};
+    cpx<float> ws[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) ws[m] = twiddle<-1>(tw, t + TPF * m, 2 * NH);
+    const int i0 = blockIdx.x * NP;
+    // pixel pairs x = 2p, 2p + 1 (p = t + TPF m, m < E / 2) of this thread's rows, 16 bits each
+    unsigned px[RPT][E / 2];
+    auto fetch = [&](const R2cU8 &it) {
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = g * RPT + q;
+            const uint8_t *row = it.src + (long long)r * it.stride;
+            const uint16_t *r2 = reinterpret_cast<const uint16_t *>(row);
+#pragma unroll
+            for (int m = 0; m < E / 2; ++m) {
+                const int p = t + TPF * m, x = 2 * p;
+                unsigned u = 0;
+                if (r < it.nrows) {
+                    if (x + 1 < it.w) u = r2[p];
+                    else if (x < it.w) u = row[x];
+                }
+                px[q][m] = u;
+            }
+        }
+    };
+    fetch(items[i0]);
+    for (int i = 0; i < NP; ++i) {
+        const int ii = i0 + i;
+        if (ii >= nitems) break;  // uniform over the workgroup
+        const R2cU8 it = items[ii];
+        const double mean = it.msd[0], inv = it.msd[2];
+        const float sc = (float)inv, of = (float)(-mean * inv);
+        cpx<float> v[RPT][E];
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+#pragma unroll
+            for (int m = 0; m < E / 2; ++m) {
+                const int x = 2 * (t + TPF * m);
+                const bool pr = g * RPT + q < it.nrows;
+                v[q][m].x = (pr && x < it.w) ? fmaf((float)(px[q][m] & 255u), sc, of) : 0.0f;
+                v[q][m].y = (pr && x + 1 < it.w) ? fmaf((float)(px[q][m] >> 8), sc, of) : 0.0f;
+            }
+        if (i + 1 < NP && ii + 1 < nitems) fetch(items[ii + 1]);
+        fft_rows<-1, NH, float, RPT, true>(v, t, L, tw);
+        lds_sync();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q)
+#pragma unroll
+            for (int m = 0; m < E; ++m) L[q][t + TPF * m] = v[q][m];
+        lds_sync();
+#pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int r = g * RPT + q;
+            if (r < it.nrows) {
+                cpx<float> *row = reinterpret_cast<cpx<float> *>(it.dst) + (long long)r * H;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const int k = t + TPF * m;
+                    const cpx<float> zk = v[q][m];
+                    const cpx<float> zn = cconj(L[q][(NH - k) & (NH - 1)]);
+                    const cpx<float> a = cscale(zk + zn, 0.5f);
+                    const cpx<float> b = rot90<-1>(cscale(zk - zn, 0.5f));
+                    row[k] = a + cmul(ws[m], b);
+                }
+                if (t == 0) row[NH] = cpx<float>{v[q][0].x - v[q][0].y, 0.0f};
+            }
+        }
+        // (the next item's first exchange waits at its own barrier before overwriting these rows)
+    }
+}
+
 // ------------------------------------------------------------------
 // Column pass, fused: forward Ny-point FFT of the A and B spectrum
 // columns, cross power F_B conj(F_A) (Correlation.cpp:972-986; whitened

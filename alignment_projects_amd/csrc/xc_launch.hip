@@ -246,7 +246,7 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
         ctx->lds_max = prop.sharedMemPerBlock;
     }
     if (const char *tf = getenv("XC_TUNE_FILE")) {
-        // "col NH NY whiten c xcd us" and "r2c NH NY 0 rpt 0 us" lines written by earlier tuning runs
+        // "col NH NY whiten c xcd us" and "r2c NH NY u8 rpt np us" lines written by earlier tuning runs
         ctx->tune_file = tf;
         if (FILE *f = fopen(tf, "r")) {
             char tag[8];
@@ -260,11 +260,13 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
                     t.us = us;
                     ctx->tune[std::make_tuple(nh, ny, wh)] = t;
                 } else if (!strcmp(tag, "r2c") && (c == 0 || c == r2c_default_rpt<float>(nh) ||
-                                                   c == r2c_alt_rpt<float>(nh))) {
+                                                   c == r2c_alt_rpt<float>(nh)) &&
+                           (x == 0 || x == 1 || x == 2 || x == 4)) {
                     xc_col_tune t;
                     t.c = c;
+                    t.xcd = x;
                     t.us = us;
-                    ctx->tune_r2c[std::make_pair(nh, ny)] = t;
+                    ctx->tune_r2c[std::make_tuple(nh, ny, wh ? 1 : 0)] = t;
                 }
             fclose(f);
         }

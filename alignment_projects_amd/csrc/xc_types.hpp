@@ -88,6 +88,17 @@ struct RotDev {
     void *out;
 };
 
+// Work item of the 8-bit row r2c (FP32 transforms of unmasked u8 rows that start
+// at even addresses and fit the upper-half padding): ROWS rows of one image from
+// src (row stride `stride` bytes, w pixels a row) into dst (rows of Nx/2 + 1
+// complex values), nrows of them present; the image's (mean, sd, 1/sd) at msd.
+struct R2cU8 {
+    const uint8_t *src;
+    void *dst;
+    const double *msd;
+    int32_t nrows, w, stride, pad;
+};
+
 constexpr int TP_PARTS = 16;     // statistics partials per image
 constexpr int TP_MAX_W = 8192;   // widest image the table sweep handles
 

@@ -78,7 +78,7 @@ _OPTIONAL = {
     "xc_plan_model_bytes": ([C.c_void_p], C.c_double),
     "xc_plan_destroy": ([C.c_void_p], C.c_int),
     "xc_plan_buckets": ([C.c_void_p, C.POINTER(XcBucketInfo), C.c_int], C.c_int),
-    "xc_plan_set_shapes": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "xc_plan_set_shapes": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
     "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
     "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
     "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
@@ -689,9 +689,9 @@ class Plan:
         self.ctx.L.xc_plan_buckets(self.h, arr, nb)
         return [{k: v for k, v in arr[i].as_dict().items() if k != "_pad"} for i in range(nb)]
 
-    def set_shapes(self, bucket, col_c=0, col_xcd=1, r2c_rpt=0):
+    def set_shapes(self, bucket, col_c=0, col_xcd=1, r2c_rpt=0, r2c_np=0):
         """Set a bucket's launch shapes by hand (xc_plan_set_shapes)."""
-        st = self.ctx.L.xc_plan_set_shapes(self.h, bucket, col_c, col_xcd, r2c_rpt)
+        st = self.ctx.L.xc_plan_set_shapes(self.h, bucket, col_c, col_xcd, r2c_rpt, r2c_np)
         self.ctx._check(st, "xc_plan_set_shapes", ok=(abi.XC_OK,))
 
     def destroy(self):

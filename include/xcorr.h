@@ -324,7 +324,8 @@ typedef struct xc_bucket_info {
     int32_t nh, ny, whiten, pairs;
     int32_t col_c, col_xcd;  /* columns per workgroup (0: default), XCD order */
     double col_us;           /* tuned column-pass time, microseconds (0: not tuned) */
-    int32_t r2c_rpt, _pad;   /* rows per thread of the row r2c */
+    int32_t r2c_rpt;         /* rows per thread of the row r2c */
+    int32_t r2c_np;          /* work items per workgroup of the 8-bit row r2c (0: the general r2c) */
     double r2c_us;           /* tuned row r2c time, microseconds (0: not tuned) */
 } xc_bucket_info;
 
@@ -353,11 +354,12 @@ int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);
  * pass columns per workgroup (0: default; 1, 2, 4, 8, 16 where the device's
  * LDS holds them) and XCD order, row r2c rows per thread (0: default; 1 or 2
  * for FP32 rows of 512 points or more whose work item holds two rows, else
- * only the default).  xc_plan_buckets reports the shapes in use.  Shapes
- * change speed only, never results.  Waits for a launch in flight, then
- * records the plan's graph again.  XC_E_SHAPE for a shape the bucket does not
- * have. */
-int xc_plan_set_shapes(xc_plan *plan, int bucket, int col_c, int col_xcd, int r2c_rpt);
+ * only the default), and for a bucket on the 8-bit row r2c (unmasked u8
+ * images) its work items per workgroup (0: default; 1, 2 or 4; ignored
+ * otherwise).  xc_plan_buckets reports the shapes in use.  Shapes change
+ * speed only, never results.  Waits for a launch in flight, then records the
+ * plan's graph again.  XC_E_SHAPE for a shape the bucket does not have. */
+int xc_plan_set_shapes(xc_plan *plan, int bucket, int col_c, int col_xcd, int r2c_rpt, int r2c_np);
 int xc_plan_destroy(xc_plan *plan);
 
 /*

@@ -205,8 +205,9 @@ def test_masks_with_empty_edges_match_point_lists(ctx):
 
 def test_every_tuned_shape_matches_default(ctx):
     """Each candidate shape the autotuner can choose -- column-pass columns per workgroup and XCD order, row r2c
-    rows per thread -- set by hand on a plan of 512-point and 2048-point rows: every field of every result is
-    bitwise the default shapes' (4096-point rows: two rows per thread or one per thread group)."""
+    rows per thread, and the 8-bit row r2c's work items per workgroup -- set by hand on a plan of 512-point and
+    2048-point rows: every field of every result is bitwise the default shapes' (4096-point rows: two rows per
+    thread or one per thread group; u8 strips, so the 8-bit row r2c)."""
     from alignment_projects_amd.xcorr import make_pair
     data = _strips(4, 2048, 64, 20, 950) + _strips(4, 512, 48, 12, 960)
     tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data]
@@ -216,21 +217,22 @@ def test_every_tuned_shape_matches_default(ctx):
     fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak")
     bk = plan.buckets()
     assert sorted(b["nh"] for b in bk) == [512, 2048], bk
+    assert all(b["r2c_np"] > 0 for b in bk), bk  # u8 strips: the 8-bit row r2c
     tried = 0
+    shapes = [(rpt, c, x, 0) for rpt in (1, 2) for c in (1, 2, 4, 8, 16) for x in (0, 1)]
+    shapes += [(rpt, 0, 1, np_) for rpt in (1, 2) for np_ in (1, 2, 4)]
     for bi, b in enumerate(bk):
-        for rpt in (1, 2):
-            for c in (1, 2, 4, 8, 16):
-                for x in (0, 1):
-                    try:
-                        plan.set_shapes(bi, col_c=c, col_xcd=x, r2c_rpt=rpt)
-                    except Exception:  # a shape this bucket or device does not have
-                        continue
-                    got = plan.run()
-                    for i, (a, g) in enumerate(zip(ref, got)):
-                        assert all(getattr(a, f) == getattr(g, f) for f in fields), (bi, rpt, c, x, i)
-                    tried += 1
+        for rpt, c, x, np_ in shapes:
+            try:
+                plan.set_shapes(bi, col_c=c, col_xcd=x, r2c_rpt=rpt, r2c_np=np_)
+            except Exception:  # a shape this bucket or device does not have
+                continue
+            got = plan.run()
+            for i, (a, g) in enumerate(zip(ref, got)):
+                assert all(getattr(a, f) == getattr(g, f) for f in fields), (bi, rpt, c, x, np_, i)
+            tried += 1
         plan.set_shapes(bi)  # back to the defaults
-    assert tried >= 20, tried
+    assert tried >= 26, tried
     plan.destroy()
     for ta, tb in tiles:
         ta.release()
