@@ -1898,6 +1898,12 @@ __host__ __device__ __forceinline__ NccRow ncc_row(const PairDev &P, int y)
         if (g.min_area >= 0 && g.min_area / rb.olh + 1 > m) m = g.min_area / rb.olh + 1;
         if ((g.or_w_ge > 0 || g.or_h_gt > 0) && !(rb.olh > g.or_h_gt) && g.or_w_ge > m) m = g.or_w_ge;
     }
+    if (g.cnt_enabled) {
+        // LegalCnt needs c1 > k1 and c2 > k2, and a box's present-pixel count is at most its area
+        // olw olh: a lag with olw olh <= max(k1, k2) is no candidate, its R value 0
+        const long long K = g.k1 > g.k2 ? g.k1 : g.k2;
+        if (K >= 0 && K / rb.olh + 1 > m) m = K / rb.olh + 1;
+    }
     // olw(x) >= m  <=>  min(w1, w2) >= m  &&  m - w1 <= x <= w2 - m
     ok = ok && m <= (P.w1 < P.w2 ? P.w1 : P.w2);
     if (m - P.w1 > lo) lo = m - P.w1;
