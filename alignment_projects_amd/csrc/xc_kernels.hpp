@@ -529,6 +529,38 @@ __device__ __forceinline__ void stats_rows_u8x16(const ImgView &V, int y0, int y
     c += (double)n;
 }
 
+// u8 rows stored back to back (stride w, and the mask's too): the band
+// [i0, i1) as one run of bytes, eight pixels (and mask bytes) in flight per
+// thread whatever the row width, summed in integers (exact, as the FP64 sums
+// of integer pixels are).
+__device__ __forceinline__ void stats_flat_u8(const ImgView &V, long long i0, long long i1, int tid, double &a,
+                                              double &b, double &c)
+{
+    const uint8_t *px = reinterpret_cast<const uint8_t *>(V.data), *mk = V.mask;
+    unsigned long long sa = 0, sq = 0, n = 0;
+    for (long long ib = i0; ib < i1; ib += 256 * 8) {
+        unsigned v[8];
+        bool p[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const long long i = ib + tid + 256 * k;
+            p[k] = i < i1;
+            v[k] = p[k] ? px[i] : 0u;
+            if (mk) p[k] = p[k] && mk[i] != 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (p[k]) {
+                sa += v[k];
+                sq += v[k] * v[k];
+                n += 1;
+            }
+    }
+    a += (double)sa;
+    b += (double)sq;
+    c += (double)n;
+}
+
 // Partial sums of present raw values: TP_PARTS workgroups per image, each
 // over a band of rows; threads stride along x (coalesced).
 static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restrict__ tiles)
@@ -541,7 +573,9 @@ static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restri
     double a = 0.0, b = 0.0, c = 0.0;
     const bool wide = D.v.dtype == XC_U8 && !D.v.mask && (reinterpret_cast<uintptr_t>(D.v.data) & 15) == 0 &&
                       (D.v.stride & 15) == 0;
+    const bool flat = D.v.dtype == XC_U8 && D.v.stride == w && (!D.v.mask || D.v.mstride == w);
     if (wide) stats_rows_u8x16(D.v, y0, y1, w, tid, a, b, c);
+    else if (flat) stats_flat_u8(D.v, (long long)y0 * w, (long long)y1 * w, tid, a, b, c);
     else switch (D.v.dtype) {  // uniform per workgroup
     case XC_U8: stats_rows<uint8_t>(D.v, y0, y1, w, tid, a, b, c); break;
     case XC_U16: stats_rows<uint16_t>(D.v, y0, y1, w, tid, a, b, c); break;
