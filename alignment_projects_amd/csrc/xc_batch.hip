@@ -306,7 +306,8 @@ static int tune_shapes(xc_ctx *ctx, xc_plan *p)
     for (Bucket &b : p->ex.buckets) {
         // row r2c: rows per thread, and for the 8-bit kernel work items per workgroup
         // (the bucket's own rows: one tuning per (NH, NY, kernel), like the column pass's)
-        const auto rkey = std::make_tuple(b.NH, b.NY, b.d_r2c_u8 ? 1 : 0);
+        const int kern = b.d_r2c_u8 ? (b.r2c_gen ? 2 : 1) : 0;  // general, 8-bit, 8-bit general rows
+        const auto rkey = std::make_tuple(b.NH, b.NY, kern);
         auto ir = ctx->tune_r2c.find(rkey);
         if (ir == ctx->tune_r2c.end()) {
             xc_col_tune best;
@@ -317,7 +318,7 @@ static int tune_shapes(xc_ctx *ctx, xc_plan *p)
                 for (int np : {1, 2, 4}) {
                     if (!b.d_r2c_u8 && np > 1) break;
                     const double us = time_launch(ctx, e0, e1, [&] {
-                        return b.d_r2c_u8 ? launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, rpt, np)
+                        return b.d_r2c_u8 ? launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, rpt, np, b.r2c_gen)
                                           : launch_row_r2c<float>(ctx, b.NH, b.n_r2c, p->ex.d_pairs, b.d_r2c, rpt);
                     });
                     if (us >= 0.0 && us < best.us) {
@@ -329,7 +330,7 @@ static int tune_shapes(xc_ctx *ctx, xc_plan *p)
             }
             if (best.c == 0) best.us = 0.0;
             ir = ctx->tune_r2c.emplace(rkey, best).first;
-            tune_save(ctx, "r2c", b.NH, b.NY, b.d_r2c_u8 ? 1 : 0, best);
+            tune_save(ctx, "r2c", b.NH, b.NY, kern, best);
         }
         b.r2c_rpt = ir->second.c;
         b.r2c_np = b.d_r2c_u8 ? ir->second.xcd : 0;
@@ -630,12 +631,12 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     for (const RotDev &R : rots) nrw += (int)((rot_splat_units(R.w, R.h) + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size()), i_w8(buckets.size(), 0);
-    std::vector<bool> r2c8(buckets.size(), false);
+    std::vector<bool> r2c8(buckets.size(), false), r2cg(buckets.size(), false);
     std::vector<int> starts;
-    // the 8-bit row r2c takes an image whose rows it can load as pixel pairs
-    auto u8_rows = [](const ImgView &V, int NH) {
-        return V.dtype == XC_U8 && !V.mask && ((uintptr_t)V.data & 1) == 0 && (V.stride & 1) == 0 && V.w <= NH;
-    };
+    // the 8-bit row r2c takes u8 images that fit the upper-half padding; rows it cannot load as pixel
+    // pairs (masked, or starting at odd addresses) need its general form
+    auto u8_rows = [](const ImgView &V, int NH) { return V.dtype == XC_U8 && V.w <= NH; };
+    auto u8_pairs = [](const ImgView &V) { return !V.mask && ((uintptr_t)V.data & 1) == 0 && (V.stride & 1) == 0; };
     for (size_t bi = 0; bi < buckets.size(); ++bi) {
         Bucket &B = buckets[bi];
         const int G = row_group<float>(B.NH), C = col_group<float>(B.NY), GR = r2c_rows<float>(B.NH);
@@ -647,10 +648,12 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             if (ownA[i]) {
                 nr += (D.h1 + GR - 1) / GR;
                 u8 = u8 && pair_rot[i] < 0 && u8_rows(D.A, B.NH);
+                if (!u8_pairs(D.A)) r2cg[bi] = true;
             }
             if (ownB[i]) {
                 nr += (D.h2 + GR - 1) / GR;
                 u8 = u8 && u8_rows(D.B, B.NH);
+                if (!u8_pairs(D.B)) r2cg[bi] = true;
             }
             nc += (D.Nx / 2 + 1 + C - 1) / C;
             live_row_groups(D, G, starts);
@@ -769,10 +772,11 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
             it.src = (const uint8_t *)V.data + (long long)y * V.stride;
             it.dst = (cpx<float> *)spec + (long long)y * (nh + 1);
             it.msd = V.msd;
+            it.mask = V.mask ? V.mask + (long long)y * V.mstride : nullptr;
             it.nrows = std::min(GR, V.h - y);
             it.w = V.w;
             it.stride = V.stride;
-            it.pad = 0;
+            it.mstride = V.mstride;
             return it;
         };
         int a = 0, c = 0, d = 0;
@@ -795,6 +799,7 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         }
         B.d_r2c = (const int4 *)at(i_wr[bi]);
         B.d_r2c_u8 = r2c8[bi] ? (const R2cU8 *)at(i_w8[bi]) : nullptr;
+        B.r2c_gen = r2c8[bi] && r2cg[bi];
         B.d_col = (const int2 *)at(i_wc[bi]);
         B.d_c2r = (const int2 *)at(i_wc2[bi]);
     }

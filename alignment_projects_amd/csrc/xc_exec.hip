@@ -139,7 +139,8 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
         for (const Bucket &b : ex.buckets) {
             if constexpr (sizeof(T) == 4) {
                 if (b.d_r2c_u8) {
-                    if ((st = launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, b.r2c_rpt, b.r2c_np))) return st;
+                    if ((st = launch_row_r2c_u8(ctx, b.NH, b.n_r2c, b.d_r2c_u8, b.r2c_rpt, b.r2c_np, b.r2c_gen)))
+                        return st;
                     continue;
                 }
             }

@@ -25,7 +25,8 @@ struct Bucket {
     double col_us = 0; // tuned time of the column pass (0: not tuned)
     int r2c_rpt = 0;   // rows per thread of the row r2c (0: default shape)
     const R2cU8 *d_r2c_u8 = nullptr;  // the same n_r2c items for the 8-bit row r2c (null: the general kernel)
-    int r2c_np = 0;    // work items per workgroup of the 8-bit row r2c (0: one)
+    int r2c_np = 0;    // work items per workgroup of the 8-bit row r2c (0: the default)
+    bool r2c_gen = false;  // the 8-bit row r2c's general rows (masked, or starting at odd addresses)
     double r2c_us = 0; // tuned time of the row r2c (0: not tuned)
     const int2 *d_c2r = nullptr;
     int n_c2r = 0;

@@ -64,8 +64,8 @@ struct xc_ctx {
     int num_cu = 0;
     size_t lds_max = 65536;  // LDS bytes one workgroup may allocate
     std::map<std::tuple<int, int, int>, xc_col_tune> tune;  // (NH, NY, whiten) -> column-pass shape
-    // (NH, NY, 8-bit kernel) -> FP32 row r2c shape (c: rows per thread, xcd: work items per workgroup of the
-    // 8-bit kernel)
+    // (NH, NY, kernel: 0 general, 1 8-bit, 2 8-bit general rows) -> FP32 row r2c shape (c: rows per thread,
+    // xcd: work items per workgroup of the 8-bit kernel)
     std::map<std::tuple<int, int, int>, xc_col_tune> tune_r2c;
     std::string tune_file;  // XC_TUNE_FILE: tuned shapes persist here across processes
 };
@@ -164,9 +164,10 @@ int live_rows(const PairDev &P);
 // rpt: rows per thread of the r2c (0: default shape; see r2c_alt_rpt for the other one).
 template <typename T>
 int launch_row_r2c(xc_ctx *ctx, int NH, int nblocks, const PairDev *pd, const int4 *work, int rpt = 0);
-// The 8-bit row r2c (FP32, unmasked u8 rows at even addresses, w <= NH): work
-// items of R2cCfg<float, NH>::ROWS rows, np of them per workgroup (1, 2 or 4).
-int launch_row_r2c_u8(xc_ctx *ctx, int NH, int nitems, const xc::R2cU8 *items, int rpt, int np);
+// The 8-bit row r2c (FP32, u8 rows of w <= NH): work items of
+// R2cCfg<float, NH>::ROWS rows, np of them per workgroup (1, 2 or 4); gen: some
+// rows are masked or start at odd addresses (bytewise loads, FP64 normalisation).
+int launch_row_r2c_u8(xc_ctx *ctx, int NH, int nitems, const xc::R2cU8 *items, int rpt, int np, bool gen);
 constexpr int R2C_U8_NP = 1;  // default work items per workgroup of the 8-bit row r2c
 bool r2c_u8_length(int NH);  // row length NH has the 8-bit row r2c
 template <typename T> int r2c_default_rpt(int NH);  // rows per thread of the default r2c shape

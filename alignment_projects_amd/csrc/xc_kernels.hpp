@@ -1570,14 +1570,15 @@ __global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
     }
 }
 
-// Row r2c of unmasked 8-bit rows (FP32 transforms), NP work items per
-// workgroup in turn: the next item's pixels are loaded while this item's rows
-// are transformed, and the split's twiddles are loaded once for all of them.
-// Each row is the same arithmetic as k_row_r2c's (its 8-bit path), so the
-// spectra are bitwise the same.  The host takes this kernel only for items
-// whose rows start at even addresses with an even stride and fit the upper-half
-// padding (w <= NH): two pixels a 16-bit load.
-template <int NH, int RPT, int NP>
+// Row r2c of 8-bit rows (FP32 transforms), NP work items per workgroup in
+// turn: the next item's pixels are loaded while this item's rows are
+// transformed, and the split's twiddles are loaded once for all of them.  Each
+// row is the same arithmetic as k_row_r2c's, so the spectra are bitwise the
+// same: unmasked rows at even addresses (GEN false: two pixels a 16-bit load,
+// normalised with an FP32 fma) or any rows (GEN true: pixel and mask bytes,
+// normalised in FP64 and rounded, absent pixels 0).  The host takes this
+// kernel only for rows that fit the upper-half padding (w <= NH).
+template <int NH, int RPT, int NP, bool GEN>
 __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
     k_row_r2c_u8(const R2cU8 *__restrict__ items, int nitems, const cpx<float> *__restrict__ tw)
 {
@@ -1594,21 +1595,30 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
 #pragma unroll
     for (int m = 0; m < E; ++m) ws[m] = twiddle<-1>(tw, t + TPF * m, 2 * NH);
     const int i0 = blockIdx.x * NP;
-    // pixel pairs x = 2p, 2p + 1 (p = t + TPF m, m < E / 2) of this thread's rows, 16 bits each
+    // pixel pairs x = 2p, 2p + 1 (p = t + TPF m, m < E / 2) of this thread's rows: the two pixels in bits
+    // 0-15; GEN: bits 16 and 17 flag them present (inside the row and the mask), bit 18 marks an unmasked
+    // row at an even address (k_row_r2c normalises those rows with the FP32 fma, the others in FP64)
     unsigned px[RPT][E / 2];
     auto fetch = [&](const R2cU8 &it) {
 #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int r = g * RPT + q;
             const uint8_t *row = it.src + (long long)r * it.stride;
+            const uint8_t *mrow = it.mask ? it.mask + (long long)r * it.mstride : nullptr;
             const uint16_t *r2 = reinterpret_cast<const uint16_t *>(row);
 #pragma unroll
             for (int m = 0; m < E / 2; ++m) {
                 const int p = t + TPF * m, x = 2 * p;
                 unsigned u = 0;
                 if (r < it.nrows) {
-                    if (x + 1 < it.w) u = r2[p];
-                    else if (x < it.w) u = row[x];
+                    if constexpr (GEN) {
+                        if (x < it.w && (!mrow || mrow[x])) u |= row[x] | 0x10000u;
+                        if (x + 1 < it.w && (!mrow || mrow[x + 1])) u |= ((unsigned)row[x + 1] << 8) | 0x20000u;
+                        if (!mrow && (reinterpret_cast<uintptr_t>(row) & 1) == 0) u |= 0x40000u;
+                    } else {
+                        if (x + 1 < it.w) u = r2[p];
+                        else if (x < it.w) u = row[x];
+                    }
                 }
                 px[q][m] = u;
             }
@@ -1626,10 +1636,18 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
         for (int q = 0; q < RPT; ++q)
 #pragma unroll
             for (int m = 0; m < E / 2; ++m) {
-                const int x = 2 * (t + TPF * m);
-                const bool pr = g * RPT + q < it.nrows;
-                v[q][m].x = (pr && x < it.w) ? fmaf((float)(px[q][m] & 255u), sc, of) : 0.0f;
-                v[q][m].y = (pr && x + 1 < it.w) ? fmaf((float)(px[q][m] >> 8), sc, of) : 0.0f;
+                const unsigned u = px[q][m];
+                if constexpr (GEN) {
+                    const float a = (float)(u & 255u), b = (float)((u >> 8) & 255u);
+                    const bool f = (u & 0x40000u) != 0;
+                    v[q][m].x = !(u & 0x10000u) ? 0.0f : f ? fmaf(a, sc, of) : (float)(((double)a - mean) * inv);
+                    v[q][m].y = !(u & 0x20000u) ? 0.0f : f ? fmaf(b, sc, of) : (float)(((double)b - mean) * inv);
+                } else {
+                    const int x = 2 * (t + TPF * m);
+                    const bool pr = g * RPT + q < it.nrows;
+                    v[q][m].x = (pr && x < it.w) ? fmaf((float)(u & 255u), sc, of) : 0.0f;
+                    v[q][m].y = (pr && x + 1 < it.w) ? fmaf((float)(u >> 8), sc, of) : 0.0f;
+                }
             }
         if (i + 1 < NP && ii + 1 < nitems) fetch(items[ii + 1]);
         fft_rows<-1, NH, float, RPT, true>(v, t, L, tw);

@@ -246,7 +246,8 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
         ctx->lds_max = prop.sharedMemPerBlock;
     }
     if (const char *tf = getenv("XC_TUNE_FILE")) {
-        // "col NH NY whiten c xcd us" and "r2c NH NY u8 rpt np us" lines written by earlier tuning runs
+        // "col NH NY whiten c xcd us" and "r2c NH NY kernel rpt np us" lines written by earlier tuning runs
+        // (kernel: 0 the general r2c, 1 the 8-bit one, 2 the 8-bit one's general rows)
         ctx->tune_file = tf;
         if (FILE *f = fopen(tf, "r")) {
             char tag[8];
@@ -266,7 +267,7 @@ extern "C" int xc_ctx_create(int device, xc_ctx **out)
                     t.c = c;
                     t.xcd = x;
                     t.us = us;
-                    ctx->tune_r2c[std::make_tuple(nh, ny, wh ? 1 : 0)] = t;
+                    ctx->tune_r2c[std::make_tuple(nh, ny, wh)] = t;
                 }
             fclose(f);
         }

@@ -88,15 +88,17 @@ struct RotDev {
     void *out;
 };
 
-// Work item of the 8-bit row r2c (FP32 transforms of unmasked u8 rows that start
-// at even addresses and fit the upper-half padding): ROWS rows of one image from
-// src (row stride `stride` bytes, w pixels a row) into dst (rows of Nx/2 + 1
-// complex values), nrows of them present; the image's (mean, sd, 1/sd) at msd.
+// Work item of the 8-bit row r2c (FP32 transforms of u8 rows that fit the
+// upper-half padding): ROWS rows of one image from src (row stride `stride`
+// bytes, w pixels a row; presence mask rows at mask, mstride bytes apart, or
+// null) into dst (rows of Nx/2 + 1 complex values), nrows of them present; the
+// image's (mean, sd, 1/sd) at msd.
 struct R2cU8 {
     const uint8_t *src;
     void *dst;
     const double *msd;
-    int32_t nrows, w, stride, pad;
+    const uint8_t *mask;
+    int32_t nrows, w, stride, mstride;
 };
 
 constexpr int TP_PARTS = 16;     // statistics partials per image

@@ -167,3 +167,36 @@ def test_cancel_queued_request(ctx):
     srv.close()
     for t in tiles:
         t.release()
+
+
+def test_masked_odd_width_tiles_serve_matches_plan(ctx):
+    """Masked tiles and unmasked tiles of odd width (rows at odd addresses): a plan transforms their rows with
+    the 8-bit row r2c's general form, the server with the general r2c; every result bitwise the same."""
+    rng = np.random.default_rng(31)
+    pairs, tiles = [], []
+    for k, (w, h) in enumerate([(301, 97), (257, 130), (384, 64), (199, 211)]):
+        F = synth.quantize(synth.bandlimited(rng, h + 24, w + 24, 128, 40), np.uint8)
+        A = np.ascontiguousarray(F[4:4 + h, 6:6 + w])
+        B = np.ascontiguousarray(F[9:9 + h, 2:2 + w])
+        if k % 2 == 0:  # an elliptical hole in each mask
+            yy, xx = np.mgrid[0:h, 0:w]
+            ma = (((xx - w / 3) / (w / 6)) ** 2 + ((yy - h / 2) / (h / 5)) ** 2 > 1).astype(np.uint8)
+            mb = (((xx - 2 * w / 3) / (w / 7)) ** 2 + ((yy - h / 3) / (h / 6)) ** 2 > 1).astype(np.uint8)
+            ta, tb = ctx.tile_put(A, mask=ma), ctx.tile_put(B, mask=mb)
+        else:
+            ta, tb = ctx.tile_put(A), ctx.tile_put(B)
+        tiles += [ta, tb]
+        pairs.append(xcorr.make_pair(ta, tb, cnt=abi.cnt_enough_points(w * h // 4)))
+        pairs.append(xcorr.make_pair(tb, ta, rgn=abi.rgn_min(w // 2, h // 2)))
+    plan = ctx.plan(pairs)
+    assert any(b["r2c_np"] > 0 for b in plan.buckets())  # the 8-bit row r2c
+    ref = plan.run()
+    srv = xcorr.Server(ctx, hbm_budget=1 << 30, max_batch=4, streams=2, max_wait_us=100)
+    got = [srv.wait(srv.submit(p))[0] for p in pairs]
+    srv.close()
+    for i in range(len(pairs)):
+        assert _same(got[i], ref[i]), (i, got[i].as_dict(), ref[i].as_dict())
+        assert ref[i].status == abi.XC_OK
+    plan.destroy()
+    for t in tiles:
+        t.release()
