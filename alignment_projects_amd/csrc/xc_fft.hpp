@@ -128,6 +128,38 @@ __device__ __forceinline__ cpx<T> twiddle(const cpx<T> *__restrict__ tw, int m, 
     return w;
 }
 
+// Device memory through global-address-space pointers.  The kernels take
+// their buffers out of descriptors as generic pointers, and a generic (flat)
+// access also counts as an LDS operation, so every LDS wait would wait for it
+// too; global loads and stores stay out of the LDS count.  Complex values move
+// as two-element vectors (one 8-byte access for FP32).
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+template <typename T> __device__ __forceinline__ cpx<T> gld(const cpx<T> *p, long long i)
+{
+    typedef __attribute__((address_space(1))) const typename Vec2<T>::type gv;
+    const typename Vec2<T>::type v = ((gv *)(const void *)p)[i];
+    return cpx<T>{v.x, v.y};
+}
+
+template <typename T> __device__ __forceinline__ void gst(cpx<T> *p, long long i, cpx<T> c)
+{
+    typedef __attribute__((address_space(1))) typename Vec2<T>::type gv;
+    typename Vec2<T>::type v;
+    v.x = c.x;
+    v.y = c.y;
+    ((gv *)(void *)p)[i] = v;
+}
+
+// Scalar element i of global memory at p (any arithmetic type).
+template <typename S> __device__ __forceinline__ S gld(const S *p, long long i)
+{
+    typedef __attribute__((address_space(1))) const S gs;
+    return ((gs *)(const void *)p)[i];
+}
+
 // Workgroup barrier that orders LDS only: this wave's LDS accesses complete,
 // then s_barrier.  Unlike __syncthreads() it does not wait for the wave's
 // global loads and stores, so table loads and row stores issued before it

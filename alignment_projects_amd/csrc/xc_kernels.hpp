@@ -436,11 +436,11 @@ __device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, doub
                 const int p = t + TPF * m, x = 2 * p;
                 T a = (T)0, b = (T)0;
                 if (valid && x + 1 < V.w) {
-                    const unsigned int u = r2[p];
+                    const unsigned int u = gld(r2, p);
                     a = fmaf((float)(u & 255u), sc, of);
                     b = fmaf((float)(u >> 8), sc, of);
                 } else if (valid && x < V.w) {
-                    a = fmaf((float)row[x], sc, of);
+                    a = fmaf((float)gld(row, x), sc, of);
                 }
                 v[m].x = a;
                 v[m].y = b;
@@ -452,8 +452,8 @@ __device__ __forceinline__ void load_row_pairs(cpx<T> *v, const ImgView &V, doub
     for (int m = 0; m < ME; ++m) {
         const int x = 2 * (t + TPF * m);
         T a = (T)0, b = (T)0;
-        if (valid && x < V.w && (!mrow || mrow[x])) a = (T)(((double)row[x] - mean) * inv);
-        if (valid && x + 1 < V.w && (!mrow || mrow[x + 1])) b = (T)(((double)row[x + 1] - mean) * inv);
+        if (valid && x < V.w && (!mrow || gld(mrow, x))) a = (T)(((double)gld(row, x) - mean) * inv);
+        if (valid && x + 1 < V.w && (!mrow || gld(mrow, x + 1))) b = (T)(((double)gld(row, x + 1) - mean) * inv);
         v[m].x = a;
         v[m].y = b;
     }
@@ -545,8 +545,8 @@ __device__ __forceinline__ void stats_flat_u8(const ImgView &V, long long i0, lo
         for (int k = 0; k < 8; ++k) {
             const long long i = ib + tid + 256 * k;
             p[k] = i < i1;
-            v[k] = p[k] ? px[i] : 0u;
-            if (mk) p[k] = p[k] && mk[i] != 0;
+            v[k] = p[k] ? gld(px, i) : 0u;
+            if (mk) p[k] = p[k] && gld(mk, i) != 0;
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k)
@@ -1563,9 +1563,9 @@ __global__ void __launch_bounds__((R2cCfg<T, NH>::threads(RPT)))
                 const cpx<T> zn = cconj(L[q][(NH - k) & (NH - 1)]);
                 const cpx<T> a = cscale(zk + zn, (T)0.5);
                 const cpx<T> b = rot90<-1>(cscale(zk - zn, (T)0.5));
-                row[k] = a + cmul(ws[m], b);
+                gst(row, k, a + cmul(ws[m], b));
             }
-            if (t == 0) row[NH] = cpx<T>{v[q][0].x - v[q][0].y, (T)0};
+            if (t == 0) gst(row, NH, cpx<T>{v[q][0].x - v[q][0].y, (T)0});
         }
     }
 }
@@ -1612,12 +1612,12 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
                 unsigned u = 0;
                 if (r < it.nrows) {
                     if constexpr (GEN) {
-                        if (x < it.w && (!mrow || mrow[x])) u |= row[x] | 0x10000u;
-                        if (x + 1 < it.w && (!mrow || mrow[x + 1])) u |= ((unsigned)row[x + 1] << 8) | 0x20000u;
+                        if (x < it.w && (!mrow || gld(mrow, x))) u |= gld(row, x) | 0x10000u;
+                        if (x + 1 < it.w && (!mrow || gld(mrow, x + 1))) u |= ((unsigned)gld(row, x + 1) << 8) | 0x20000u;
                         if (!mrow && (reinterpret_cast<uintptr_t>(row) & 1) == 0) u |= 0x40000u;
                     } else {
-                        if (x + 1 < it.w) u = r2[p];
-                        else if (x < it.w) u = row[x];
+                        if (x + 1 < it.w) u = gld(r2, p);
+                        else if (x < it.w) u = gld(row, x);
                     }
                 }
                 px[q][m] = u;
@@ -1629,7 +1629,7 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
         const int ii = i0 + i;
         if (ii >= nitems) break;  // uniform over the workgroup
         const R2cU8 it = items[ii];
-        const double mean = it.msd[0], inv = it.msd[2];
+        const double mean = gld(it.msd, 0), inv = gld(it.msd, 2);
         const float sc = (float)inv, of = (float)(-mean * inv);
         cpx<float> v[RPT][E];
 #pragma unroll
@@ -1669,9 +1669,9 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
                     const cpx<float> zn = cconj(L[q][(NH - k) & (NH - 1)]);
                     const cpx<float> a = cscale(zk + zn, 0.5f);
                     const cpx<float> b = rot90<-1>(cscale(zk - zn, 0.5f));
-                    row[k] = a + cmul(ws[m], b);
+                    gst(row, k, a + cmul(ws[m], b));
                 }
-                if (t == 0) row[NH] = cpx<float>{v[q][0].x - v[q][0].y, 0.0f};
+                if (t == 0) gst(row, NH, cpx<float>{v[q][0].x - v[q][0].y, 0.0f});
             }
         }
         // (the next item's first exchange waits at its own barrier before overwriting these rows)
@@ -1711,6 +1711,16 @@ template <typename T, int NY> struct ColCfg {
     }
 };
 
+// Spectrum element (y, k) of an h x H row spectrum, or 0 when !ok (padding row or
+// column): the load itself is unconditional, from a clamped address, so that
+// every lane's loads are in flight at once.
+template <typename T>
+__device__ __forceinline__ cpx<T> col_ld(const cpx<T> *S, bool ok, int y, int h, int H, int k)
+{
+    const cpx<T> v = gld(S, (long long)(y < h ? y : h - 1) * H + (k < H ? k : H - 1));
+    return ok ? v : cpx<T>{(T)0, (T)0};
+}
+
 template <typename T, int NY, bool WHITEN, int C>
 __global__ void __launch_bounds__(Plan<NY>::TPF * C)
     k_col_fused(const PairDev *__restrict__ pairs, const int2 *__restrict__ work, const cpx<T> *__restrict__ tw,
@@ -1744,16 +1754,16 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 #pragma unroll
             for (int m = 0; m < E / 2; ++m) {
                 const int y = t + TPF * m;
-                ab[0][m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
-                ab[1][m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+                ab[0][m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
+                ab[1][m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
             }
             fft_rows_tw<-1, NY, T, 2, true>(ab, t, L2, TW);
         } else {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 const int y = t + TPF * m;
-                ab[0][m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
-                ab[1][m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+                ab[0][m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
+                ab[1][m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
             }
             fft_rows_tw<-1, NY, T, 2, false>(ab, t, L2, TW);
         }
@@ -1768,14 +1778,14 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 #pragma unroll
         for (int m = 0; m < E / 2; ++m) {
             const int y = t + TPF * m;
-            a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            a[m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
         }
         fft_upper_zero<-1, NY>(a, t, L, TW);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + TPF * m;
-            a[m] = (kv && y < P.A.h) ? SA[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            a[m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
         }
         fft<-1, NY>(a, t, L, TW);
     }
@@ -1784,14 +1794,14 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 #pragma unroll
         for (int m = 0; m < E / 2; ++m) {
             const int y = t + TPF * m;
-            b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            b[m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
         }
         fft_upper_zero<-1, NY>(b, t, L, TW);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + TPF * m;
-            b[m] = (kv && y < P.B.h) ? SB[(long long)y * H + k] : cpx<T>{(T)0, (T)0};
+            b[m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
         }
         fft<-1, NY>(b, t, L, TW);
     }
@@ -1840,7 +1850,7 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
         for (int m = 0; m < E; ++m) {
             const int iy = t + TPF * m;
             const int yl = lag_row_of_raster(P, iy);
-            if (yl >= 0 && row_live(P, yl)) LG[(long long)yl * H + k] = a[m];
+            if (yl >= 0 && row_live(P, yl)) gst(LG, (long long)yl * H + k, a[m]);
         }
     }
 }
@@ -2279,8 +2289,8 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
 #pragma unroll
     for (int m = 0; m < E; ++m) {
         const int p = t + TPF * m;
-        xk[m] = valid ? LG[p] : cpx<T>{(T)0, (T)0};
-        xn[m] = valid ? LG[NH - p] : cpx<T>{(T)0, (T)0};
+        xk[m] = valid ? gld(LG, p) : cpx<T>{(T)0, (T)0};
+        xn[m] = valid ? gld(LG, NH - p) : cpx<T>{(T)0, (T)0};
         ws[m] = twiddle<+1>(tw, p, 2 * NH);
     }
     cpx<T> v[E];
