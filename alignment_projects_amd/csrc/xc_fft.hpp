@@ -160,6 +160,33 @@ template <typename S> __device__ __forceinline__ S gld(const S *p, long long i)
     return ((gs *)(const void *)p)[i];
 }
 
+// Scalar element i of global memory at p set to v.
+template <typename S> __device__ __forceinline__ void gst(S *p, long long i, S v)
+{
+    typedef __attribute__((address_space(1))) S gs;
+    ((gs *)(void *)p)[i] = v;
+}
+
+// Elements i and i + 1 of global memory at p set to a and b in one access.
+template <typename S> __device__ __forceinline__ void gst2(S *p, long long i, S a, S b)
+{
+    typedef S v2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(1))) v2 gv;
+    v2 v;
+    v.x = a;
+    v.y = b;
+    *((gv *)(void *)(p + i)) = v;
+}
+
+// 16 bytes of global memory at p (16-byte aligned).
+__device__ __forceinline__ uint4 gld16(const void *p)
+{
+    typedef unsigned v4 __attribute__((ext_vector_type(4)));
+    typedef __attribute__((address_space(1))) const v4 gv;
+    const v4 v = *((gv *)p);
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Workgroup barrier that orders LDS only: this wave's LDS accesses complete,
 // then s_barrier.  Unlike __syncthreads() it does not wait for the wave's
 // global loads and stores, so table loads and row stores issued before it

@@ -17,6 +17,7 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 
 #include <type_traits>
 
@@ -59,10 +60,10 @@ __host__ __device__ __forceinline__ bool rgn_ok(const RuleDev &g, int olw, int o
 template <typename TT>
 __device__ __forceinline__ TT box_sum(const TT *__restrict__ t, int w, int L, int R, int B, int T)
 {
-    TT r = t[R + (long long)w * T];
-    if (L > 0) r -= t[L - 1 + (long long)w * T];
-    if (B > 0) r -= t[R + (long long)w * (B - 1)];
-    if (L > 0 && B > 0) r += t[(L - 1) + (long long)w * (B - 1)];
+    TT r = gld(t, R + (long long)w * T);
+    if (L > 0) r -= gld(t, L - 1 + (long long)w * T);
+    if (B > 0) r -= gld(t, R + (long long)w * (B - 1));
+    if (L > 0 && B > 0) r += gld(t, (L - 1) + (long long)w * (B - 1));
     return r;
 }
 
@@ -80,10 +81,17 @@ template <> struct TabE<float> {
 };
 template <typename FT> using tab_t = typename TabE<FT>::type;
 
-template <typename FT>
-__host__ __device__ __forceinline__ double2 tab_at(const void *t, long long i)
+// Table entry i of an FP table (S, S^2) pair, stored through a global pointer.
+template <typename FT> __device__ __forceinline__ void tab_st(void *t, long long i, double a, double b)
 {
-    const tab_t<FT> v = reinterpret_cast<const tab_t<FT> *>(t)[i];
+    typedef typename std::conditional<std::is_same<FT, float>::value, float, double>::type E;
+    gst2(reinterpret_cast<E *>(t), 2 * i, (E)a, (E)b);
+}
+
+template <typename FT>
+__device__ __forceinline__ double2 tab_at(const void *t, long long i)
+{
+    const cpx<FT> v = gld(reinterpret_cast<const cpx<FT> *>(t), i);
     return make_double2((double)v.x, (double)v.y);
 }
 
@@ -131,16 +139,16 @@ template <> __host__ __device__ __forceinline__ double2 c_zero<double2>() { retu
 template <> __host__ __device__ __forceinline__ unsigned long long c_zero<unsigned long long>() { return 0ULL; }
 
 template <typename TB>
-__host__ __device__ __forceinline__ cor_t<TB> tab_c(const void *t, long long i)
+__device__ __forceinline__ cor_t<TB> tab_c(const void *t, long long i)
 {
-    if constexpr (is_tabx<TB>()) return reinterpret_cast<const unsigned long long *>(t)[i];
+    if constexpr (is_tabx<TB>()) return gld(reinterpret_cast<const unsigned long long *>(t), i);
     else return tab_at<TB>(t, i);
 }
 
 // The same lookups on an (S, S^2) table: one load per corner, combined in
 // box_sum's order.
 template <typename TB>
-__host__ __device__ __forceinline__ cor_t<TB> box_c(const void *__restrict__ t, int w, int L, int R, int B, int T)
+__device__ __forceinline__ cor_t<TB> box_c(const void *__restrict__ t, int w, int L, int R, int B, int T)
 {
     cor_t<TB> r = tab_c<TB>(t, R + (long long)w * T);
     if (L > 0) r = c_sub(r, tab_c<TB>(t, L - 1 + (long long)w * T));
@@ -505,7 +513,7 @@ __device__ __forceinline__ void stats_rows_u8x16(const ImgView &V, int y0, int y
         const uint8_t *row = reinterpret_cast<const uint8_t *>(V.data) + (long long)y * V.stride;
         const uint4 *r16 = reinterpret_cast<const uint4 *>(row);
         for (int i = tid; i < w16; i += 256) {
-            const uint4 q = r16[i];
+            const uint4 q = gld16(r16 + i);
             const unsigned int wd[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -846,19 +854,18 @@ __device__ __forceinline__ void sat_sweep(const TileDev &D, int band, double *xs
             for (int j = 0; j < CPT; j += 2) {
                 if (x0 + j < w) {  // w even: x0 + j + 1 < w too
                     if (D.SQ) {
-                        tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
-                        sq[row + x0 + j] = TabE<TB>::make(bs + ps[j], bq + pq[j]);
-                        sq[row + x0 + j + 1] = TabE<TB>::make(bs + ps[j + 1], bq + pq[j + 1]);
+                        tab_st<TB>(D.SQ, row + x0 + j, bs + ps[j], bq + pq[j]);
+                        tab_st<TB>(D.SQ, row + x0 + j + 1, bs + ps[j + 1], bq + pq[j + 1]);
                     }
-                    if (D.Z) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(bz + pz[j], bz + pz[j + 1]);
+                    if (D.Z) gst2(D.Z, row + x0 + j, bz + pz[j], bz + pz[j + 1]);
                 }
             }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j) {
                 if (x0 + j < w) {
-                    if (D.SQ) reinterpret_cast<tab_t<TB> *>(D.SQ)[row + x0 + j] = TabE<TB>::make(bs + ps[j], bq + pq[j]);
-                    if (D.Z) D.Z[row + x0 + j] = bz + pz[j];
+                    if (D.SQ) tab_st<TB>(D.SQ, row + x0 + j, bs + ps[j], bq + pq[j]);
+                    if (D.Z) gst(D.Z, row + x0 + j, bz + pz[j]);
                 }
             }
         }
@@ -887,13 +894,13 @@ __device__ __forceinline__ typename PixWord<CPT>::type load_pix(const uint8_t *r
 {
     typedef typename PixWord<CPT>::type W;
     if (vec) {
-        if constexpr (CPT == 2) return *reinterpret_cast<const uint16_t *>(row + x0);
-        else return *reinterpret_cast<const W *>(row + x0);
+        if constexpr (CPT == 2) return gld(reinterpret_cast<const uint16_t *>(row + x0), 0);
+        else return gld(reinterpret_cast<const W *>(row + x0), 0);
     }
     W v = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j)
-        if (x0 + j < w) v |= (W)row[x0 + j] << (8 * j);
+        if (x0 + j < w) v |= (W)gld(row, x0 + j) << (8 * j);
     return v;
 }
 
@@ -966,18 +973,15 @@ __device__ __forceinline__ void satx_rows(const TileDev &D, int *xi, int y0, con
 #pragma unroll
             for (int j = 0; j < CPT; j += 2)
                 if (x0 + j < w) {  // w even: x0 + j + 1 < w too
-                    ulonglong2 e;
-                    e.x = acc[j];
-                    e.y = acc[j + 1];
-                    *reinterpret_cast<ulonglong2 *>(sq + row + x0 + j) = e;
-                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                    gst2(sq, row + x0 + j, acc[j], acc[j + 1]);
+                    if constexpr (WZ) gst2(D.Z, row + x0 + j, az[j], az[j + 1]);
                 }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j)
                 if (x0 + j < w) {
-                    sq[row + x0 + j] = acc[j];
-                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                    gst(sq, row + x0 + j, acc[j]);
+                    if constexpr (WZ) gst(D.Z, row + x0 + j, az[j]);
                 }
         }
     }
@@ -1025,7 +1029,7 @@ __device__ __forceinline__ void sat_sweep_x(const TileDev &D, int *xi, uint4 *bl
             for (int k = 0; k < SATX_PER; ++k) {
                 const int c = tid + NT * k, r = c / cpr;
                 pf[k] = (c < np && yb + r < h)
-                            ? *reinterpret_cast<const uint4 *>(src + (long long)(yb + r) * stride + 16 * (c - r * cpr))
+                            ? gld16(src + (long long)(yb + r) * stride + 16 * (c - r * cpr))
                             : make_uint4(0, 0, 0, 0);
             }
         };
@@ -1098,7 +1102,6 @@ __device__ __forceinline__ void satu8_rows(const TileDev &D, int *xi, int y0, co
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int x0 = tid * CPT;
     const double inv2 = inv * inv, m2 = 2.0 * mean, msq = mean * mean;
-    tab_t<TB> *sq = reinterpret_cast<tab_t<TB> *>(D.SQ);
     int pf[RB][NQ][CPT];
 #pragma unroll
     for (int r = 0; r < RB; ++r) {
@@ -1163,16 +1166,16 @@ __device__ __forceinline__ void satu8_rows(const TileDev &D, int *xi, int y0, co
 #pragma unroll
             for (int j = 0; j < CPT; j += 2)
                 if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
-                    sq[row + x0 + j + 1] = TabE<TB>::make(sv[j + 1], qv[j + 1]);
-                    if constexpr (WZ) *reinterpret_cast<int2 *>(D.Z + row + x0 + j) = make_int2(az[j], az[j + 1]);
+                    tab_st<TB>(D.SQ, row + x0 + j, sv[j], qv[j]);
+                    tab_st<TB>(D.SQ, row + x0 + j + 1, sv[j + 1], qv[j + 1]);
+                    if constexpr (WZ) gst2(D.Z, row + x0 + j, az[j], az[j + 1]);
                 }
         } else {
 #pragma unroll
             for (int j = 0; j < CPT; ++j)
                 if (x0 + j < w) {
-                    sq[row + x0 + j] = TabE<TB>::make(sv[j], qv[j]);
-                    if constexpr (WZ) D.Z[row + x0 + j] = az[j];
+                    tab_st<TB>(D.SQ, row + x0 + j, sv[j], qv[j]);
+                    if constexpr (WZ) gst(D.Z, row + x0 + j, az[j]);
                 }
         }
     }
@@ -1223,10 +1226,10 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi, uint4 *b
             for (int k = 0; k < PERH; ++k) {
                 const int c = tid + NT * k, r = c / cpr, q = c - r * cpr;
                 const bool in = c < np && yb + r < h;
-                pf[k] = in ? *reinterpret_cast<const uint4 *>(src + (long long)(yb + r) * stride + 16 * q)
+                pf[k] = in ? gld16(src + (long long)(yb + r) * stride + 16 * q)
                            : make_uint4(0, 0, 0, 0);
                 if constexpr (MASK)
-                    pq[k] = in ? *reinterpret_cast<const uint4 *>(msk + (long long)(yb + r) * mstride + 16 * q)
+                    pq[k] = in ? gld16(msk + (long long)(yb + r) * mstride + 16 * q)
                                : make_uint4(0, 0, 0, 0);
             }
         };
@@ -1712,13 +1715,11 @@ template <typename T, int NY> struct ColCfg {
 };
 
 // Spectrum element (y, k) of an h x H row spectrum, or 0 when !ok (padding row or
-// column): the load itself is unconditional, from a clamped address, so that
-// every lane's loads are in flight at once.
+// column, whose load is skipped).
 template <typename T>
-__device__ __forceinline__ cpx<T> col_ld(const cpx<T> *S, bool ok, int y, int h, int H, int k)
+__device__ __forceinline__ cpx<T> col_ld(const cpx<T> *S, bool ok, int y, int, int H, int k)
 {
-    const cpx<T> v = gld(S, (long long)(y < h ? y : h - 1) * H + (k < H ? k : H - 1));
-    return ok ? v : cpx<T>{(T)0, (T)0};
+    return ok ? gld(S, (long long)y * H + k) : cpx<T>{(T)0, (T)0};
 }
 
 template <typename T, int NY, bool WHITEN, int C>
@@ -2078,7 +2079,7 @@ __device__ __forceinline__ double ncc_lag(const PairDev &P, const NccRow &q, int
     const cor_t<TB> sqb = band_c<TB>(q.sqBt, q.sqBb, BL, BR);
     bool c = true;
     if (g.exact_w > 0 && olw != g.exact_w) c = false;
-    if (g.table && (olw > g.table_w || q.olh > g.table_h || !g.table[(long long)(q.olh - 1) * g.table_w + (olw - 1)]))
+    if (g.table && (olw > g.table_w || q.olh > g.table_h || !gld(g.table, (long long)(q.olh - 1) * g.table_w + (olw - 1))))
         c = false;
     if (g.cnt_enabled) {
         const int c1 = band_sum(q.zAt, q.zAb, AL, AR);
@@ -2356,8 +2357,8 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
                     bool dead;
                     const long long ir = (long long)(lag_x_of_raster(P, ix, &dead) + P.nnegx) +
                                          (long long)P.wR * (y + P.nnegy);
-                    P.rimg[ir] = val;
-                    P.cimg[ir] = cand ? 1 : 0;
+                    gst(P.rimg, ir, val);
+                    gst(P.cimg, ir, (uint8_t)(cand ? 1 : 0));
                 }
             }
             if (cand) {
@@ -2443,7 +2444,20 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
         r.sum2 = ssq;
         r.flags = flags;
         r.pad[0] = r.pad[1] = 0;
-        P.rows[yl] = r;
+        // field by field through a global pointer (a generic store would count as an LDS operation)
+        static_assert(offsetof(RowRec, key) == 5 * sizeof(double) && sizeof(RowRec) == 5 * sizeof(double) + 16,
+                      "RowRec: five doubles, then key, flags and two pad words");
+        double *rd = reinterpret_cast<double *>(P.rows + yl);
+        int32_t *ri = reinterpret_cast<int32_t *>(rd + 5);
+        gst(rd, 0, r.best);
+        gst(rd, 1, r.left);
+        gst(rd, 2, r.right);
+        gst(rd, 3, r.sum);
+        gst(rd, 4, r.sum2);
+        gst(ri, 0, r.key);
+        gst(ri, 1, r.flags);
+        gst(ri, 2, 0);
+        gst(ri, 3, 0);
     }
 }
 
