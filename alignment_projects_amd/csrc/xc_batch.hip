@@ -66,9 +66,9 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     if (mask) {
         std::vector<uint8_t> hm;
         const uint8_t *m = mask;
-        if (dev_src) {
+        if (dev_src) {  // a device mask's rows are `stride` bytes apart (the picture's stride)
             hm.resize((size_t)w * h);
-            if (hipMemcpy(hm.data(), mask, hm.size(), hipMemcpyDeviceToHost) != hipSuccess)
+            if (hipMemcpy2D(hm.data(), w, mask, stride, w, h, hipMemcpyDeviceToHost) != hipSuccess)
                 return set_err(ctx, XC_E_DEVICE, "tile mask download failed");
             m = hm.data();
         }
@@ -114,11 +114,15 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     t->rx0.swap(rx0);
     t->rx1.swap(rx1);
     const TileSizes ts = tile_sizes(vw, vh, tabk);
+    // an uploaded 8-bit picture (and any mask) is stored with rows padded to 16-byte multiples, so that the
+    // table sweep can stage whole 16-byte pieces of every row
+    const int pitch = (!dev_src && es == 1) ? (w + 15) / 16 * 16 : stride;
+    const int mpitch = dev_src ? stride : mask ? (w + 15) / 16 * 16 : w;
     Layout lay;
     size_t iS = lay.add(ts.SQ), iZ = lay.add(ts.Z), iP = lay.add(ts.part);
     size_t iM = lay.add(ts.msd), iD = lay.add(sizeof(TileDev)), iL = lay.add(sizeof(int));
-    size_t iRaw = dev_src ? 0 : lay.add((size_t)es * stride * h);
-    size_t iMask = (!dev_src && mask) ? lay.add((size_t)w * h) : 0;
+    size_t iRaw = dev_src ? 0 : lay.add((size_t)es * pitch * h);
+    size_t iMask = (!dev_src && mask) ? lay.add((size_t)mpitch * h) : 0;
     int st = ensure_dev(ctx, t->mem, lay.total, false);  // a tile's memory never grows
     if (st) {
         delete t;
@@ -131,14 +135,15 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     } else {
         t->raw = base + lay.off[iRaw];
         t->own_raw = true;
-        if (hipMemcpy(t->raw, data, (size_t)es * stride * h, hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMemcpy2D(t->raw, (size_t)es * pitch, data, (size_t)es * stride, (size_t)es * w, h,
+                        hipMemcpyHostToDevice) != hipSuccess) {
             xc_tile_release(ctx, t);
             return set_err(ctx, XC_E_DEVICE, "tile upload failed");
         }
         if (mask) {
             t->mask = (uint8_t *)(base + lay.off[iMask]);
             t->own_mask = true;
-            if (hipMemcpy(t->mask, mask, (size_t)w * h, hipMemcpyHostToDevice) != hipSuccess) {
+            if (hipMemcpy2D(t->mask, mpitch, mask, w, w, h, hipMemcpyHostToDevice) != hipSuccess) {
                 xc_tile_release(ctx, t);
                 return set_err(ctx, XC_E_DEVICE, "tile mask upload failed");
             }
@@ -152,15 +157,15 @@ extern "C" int xc_tile_put(xc_ctx *ctx, const void *data, int dtype, int w, int 
     F.dtype = dtype;
     F.w = w;
     F.h = h;
-    F.stride = stride;
-    F.mstride = w;
+    F.stride = dev_src ? stride : pitch;
+    F.mstride = mpitch;
     F.msd = (double *)(base + lay.off[iM]);
     // ... and the bounding-box view the correlations and the tables use
     TileDev &D = t->dev;
     memset(&D, 0, sizeof D);
     D.v = F;
-    D.v.data = (const char *)t->raw + (size_t)es * ((size_t)Bt * stride + L);
-    D.v.mask = t->mask ? t->mask + (size_t)Bt * w + L : nullptr;
+    D.v.data = (const char *)t->raw + (size_t)es * ((size_t)Bt * F.stride + L);
+    D.v.mask = t->mask ? t->mask + (size_t)Bt * F.mstride + L : nullptr;
     D.v.w = vw;
     D.v.h = vh;
     D.msd = (double *)(base + lay.off[iM]);

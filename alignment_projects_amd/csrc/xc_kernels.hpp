@@ -569,6 +569,37 @@ __device__ __forceinline__ void stats_flat_u8(const ImgView &V, long long i0, lo
     c += (double)n;
 }
 
+// u8 rows at any stride (and mask rows at theirs): threads stride along x, each
+// load step takes eight rows at once, summed in integers (exact).
+__device__ __forceinline__ void stats_rows8_u8(const ImgView &V, int y0, int y1, int w, int tid, double &a, double &b,
+                                               double &c)
+{
+    const uint8_t *px = reinterpret_cast<const uint8_t *>(V.data), *mk = V.mask;
+    const long long st = V.stride, ms = V.mstride;
+    unsigned long long sa = 0, sq = 0, n = 0;
+    for (int y = y0; y < y1; y += 8)
+        for (int x = tid; x < w; x += 256) {
+            unsigned v[8];
+            bool p[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                p[k] = y + k < y1;
+                v[k] = p[k] ? gld(px, (long long)(y + k) * st + x) : 0u;
+                if (mk) p[k] = p[k] && gld(mk, (long long)(y + k) * ms + x) != 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (p[k]) {
+                    sa += v[k];
+                    sq += v[k] * v[k];
+                    n += 1;
+                }
+        }
+    a += (double)sa;
+    b += (double)sq;
+    c += (double)n;
+}
+
 // Partial sums of present raw values: TP_PARTS workgroups per image, each
 // over a band of rows; threads stride along x (coalesced).
 static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restrict__ tiles)
@@ -584,6 +615,7 @@ static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restri
     const bool flat = D.v.dtype == XC_U8 && D.v.stride == w && (!D.v.mask || D.v.mstride == w);
     if (wide) stats_rows_u8x16(D.v, y0, y1, w, tid, a, b, c);
     else if (flat) stats_flat_u8(D.v, (long long)y0 * w, (long long)y1 * w, tid, a, b, c);
+    else if (D.v.dtype == XC_U8) stats_rows8_u8(D.v, y0, y1, w, tid, a, b, c);
     else switch (D.v.dtype) {  // uniform per workgroup
     case XC_U8: stats_rows<uint8_t>(D.v, y0, y1, w, tid, a, b, c); break;
     case XC_U16: stats_rows<uint16_t>(D.v, y0, y1, w, tid, a, b, c); break;
@@ -1210,10 +1242,14 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi, uint4 *b
         an[j] = 0.0;
         az[j] = 0;
     }
-    uintptr_t al = reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride | (uintptr_t)w;
+    // rows starting at 16-byte boundaries with 16-byte multiple strides: a row is read as its cpr whole
+    // 16-byte pieces (the last may run into the row's padding, which the stride holds and the sweep
+    // ignores: columns >= w count as absent), staged with an LDS pitch of 16 cpr bytes
+    uintptr_t al = reinterpret_cast<uintptr_t>(src) | (uintptr_t)stride;
     if constexpr (MASK) al |= reinterpret_cast<uintptr_t>(msk) | (uintptr_t)mstride;
-    const int cpr = w / 16;  // 16-byte pieces a row
-    int br = (SATX_BLK / 2) / w;
+    const int cpr = (w + 15) / 16;  // 16-byte pieces a row
+    const int pitch = 16 * cpr;
+    int br = (SATX_BLK / 2) / pitch;
     br = cpr > 0 && br > (PERH * NT) / cpr ? (PERH * NT) / cpr : br;
     br = br > 32 ? 32 : br;
     br = br / RB * RB;  // whole barrier groups; 0 for rows wider than the half block holds RB of
@@ -1251,7 +1287,7 @@ __device__ __forceinline__ void sat_sweep_u8(const TileDev &D, int *xi, uint4 *b
                 W pix[RB], pmw[RB];
 #pragma unroll
                 for (int r = 0; r < RB; ++r) {
-                    const int o = (r0 + r) * w + x0;
+                    const int o = (r0 + r) * pitch + x0;
                     if constexpr (CPT == 2) {
                         pix[r] = x0 < w ? *reinterpret_cast<const uint16_t *>(b + o) : 0;
                         pmw[r] = (MASK && x0 < w) ? *reinterpret_cast<const uint16_t *>(bm + o) : 0;

@@ -125,9 +125,18 @@ def thumb_pairs(ctx, n, seed0=3000, size=2048, dec=8, mean=30000.0, sd=6000.0, m
 # uint8, shift U{-50..50}^2, N(0, 12) on B, 0-3 elliptical holes per mask.
 # ----------------------------------------------------------------------------
 
+def _pitched(t):
+    """t (h x w, uint8 on the device) as a view of an h x P tensor, P the next multiple of 16 bytes: the row
+    pitch the library's table sweep reads fastest (pass t.stride(0) as the tile's stride)."""
+    h, w = t.shape
+    p = torch.zeros((h, (w + 15) // 16 * 16), dtype=t.dtype, device=t.device)
+    p[:, :w] = t
+    return p[:, :w]
+
+
 def uneven_pairs(n, seed0=4000, lo=128, hi=1024, smax=50, noise=12.0, fsize=6144):
-    """Returns a list of (A, mA, B, mB, (sx, sy)) device tensors.  Pairs are cut from shared fields
-    (one per 64 pairs) at random places."""
+    """Returns a list of (A, mA, B, mB, (sx, sy)) device tensors, rows 16-byte pitched (A and its mask share
+    one stride, as do B and its mask).  Pairs are cut from shared fields (one per 64 pairs) at random places."""
     out = []
     g = _gen(seed0)
     Fd = None
@@ -141,7 +150,7 @@ def uneven_pairs(n, seed0=4000, lo=128, hi=1024, smax=50, noise=12.0, fsize=6144
         a = quantize(Fd[y0:y0 + h, x0:x0 + w], 255).to(torch.uint8).contiguous()
         nb = torch.randn((h, w), generator=g, device=_DEV) * noise
         b = quantize(Fd[y0 - sy:y0 - sy + h, x0 - sx:x0 - sx + w] + nb, 255).to(torch.uint8).contiguous()
-        out.append((a, _holes(g, h, w), b, _holes(g, h, w), (sx, sy)))
+        out.append((_pitched(a), _pitched(_holes(g, h, w)), _pitched(b), _pitched(_holes(g, h, w)), (sx, sy)))
     return out
 
 

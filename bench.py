@@ -530,7 +530,8 @@ def run_sweep(args, sync):
 def _adopt(ctx, t, m):
     import torch  # noqa: F401  (tensors own the memory)
     h, w = t.shape
-    return ctx.tile_put((h, w), mask=m.data_ptr(), device_ptr=t.data_ptr(), dtype=0)
+    assert m.stride(0) == t.stride(0)  # a device mask has the picture's row pitch
+    return ctx.tile_put((h, w), mask=m.data_ptr(), device_ptr=t.data_ptr(), dtype=0, stride=t.stride(0))
 
 
 def run_uneven(args, sync):

@@ -241,7 +241,11 @@ enum xc_tile_flags {
 /*
  * Upload (or adopt from device memory) one w x h image with optional
  * validity mask (non-zero = pixel present in the point list; absent pixels
- * are 0 in the raster, as ImageFromValuesAndPoints leaves them).  The tile
+ * are 0 in the raster, as ImageFromValuesAndPoints leaves them).  `stride`
+ * is the picture's row pitch in elements (0: w).  A host mask has rows of w
+ * bytes; a device mask (XC_TILE_DEVICE_SRC) has rows `stride` bytes apart, so
+ * that a caller can pad both to 16-byte multiples, which the table sweep
+ * reads fastest (uploaded 8-bit pictures and masks are stored that way).  The tile
  * keeps its pixels, (mean, sd) and its summed-area tables of S and S^2
  * (IntegrateImage, Correlation.cpp:49-105) resident in HBM, plus the int32
  * non-zero count table.  The tables' form:
