@@ -1750,12 +1750,13 @@ template <typename T, int NY> struct ColCfg {
     }
 };
 
-// Spectrum element (y, k) of an h x H row spectrum, or 0 when !ok (padding row or
+// Spectrum element m of this thread's column: row t + TPF m, from the thread's
+// element pointer e (row t) and the row step TPF H; 0 when !ok (padding row or
 // column, whose load is skipped).
 template <typename T>
-__device__ __forceinline__ cpx<T> col_ld(const cpx<T> *S, bool ok, int y, int, int H, int k)
+__device__ __forceinline__ cpx<T> col_ld(const cpx<T> *e, bool ok, int m, long long step)
 {
-    return ok ? gld(S, (long long)y * H + k) : cpx<T>{(T)0, (T)0};
+    return ok ? gld(e, m * step) : cpx<T>{(T)0, (T)0};
 }
 
 template <typename T, int NY, bool WHITEN, int C>
@@ -1777,6 +1778,9 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
     const cpx<T> *SA = reinterpret_cast<const cpx<T> *>(P.specA);
     const cpx<T> *SB = reinterpret_cast<const cpx<T> *>(P.specB);
     cpx<T> *LG = reinterpret_cast<cpx<T> *>(P.lag);
+    // row t of column k in each spectrum (a padding column reads nothing), then TPF rows per register
+    const long long RS = (long long)TPF * H;
+    const cpx<T> *EA = SA + (long long)t * H + k, *EB = SB + (long long)t * H + k;
     LdsView<T> L{smem + c, C};
     TwReg<NY, T> TW;
     tw_load<NY, T>(TW, t, tw);
@@ -1791,16 +1795,16 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 #pragma unroll
             for (int m = 0; m < E / 2; ++m) {
                 const int y = t + TPF * m;
-                ab[0][m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
-                ab[1][m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
+                ab[0][m] = col_ld(EA, kv && y < P.A.h, m, RS);
+                ab[1][m] = col_ld(EB, kv && y < P.B.h, m, RS);
             }
             fft_rows_tw<-1, NY, T, 2, true>(ab, t, L2, TW);
         } else {
 #pragma unroll
             for (int m = 0; m < E; ++m) {
                 const int y = t + TPF * m;
-                ab[0][m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
-                ab[1][m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
+                ab[0][m] = col_ld(EA, kv && y < P.A.h, m, RS);
+                ab[1][m] = col_ld(EB, kv && y < P.B.h, m, RS);
             }
             fft_rows_tw<-1, NY, T, 2, false>(ab, t, L2, TW);
         }
@@ -1815,30 +1819,30 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
 #pragma unroll
         for (int m = 0; m < E / 2; ++m) {
             const int y = t + TPF * m;
-            a[m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
+            a[m] = col_ld(EA, kv && y < P.A.h, m, RS);
         }
         fft_upper_zero<-1, NY>(a, t, L, TW);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + TPF * m;
-            a[m] = col_ld(SA, kv && y < P.A.h, y, P.A.h, H, k);
+            a[m] = col_ld(EA, kv && y < P.A.h, m, RS);
         }
         fft<-1, NY>(a, t, L, TW);
     }
-    __syncthreads();
+    // (B's transform waits at its first exchange's barrier before it overwrites A's LDS rows)
     if (E == 8 && P.B.h <= NY / 2) {
 #pragma unroll
         for (int m = 0; m < E / 2; ++m) {
             const int y = t + TPF * m;
-            b[m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
+            b[m] = col_ld(EB, kv && y < P.B.h, m, RS);
         }
         fft_upper_zero<-1, NY>(b, t, L, TW);
     } else {
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             const int y = t + TPF * m;
-            b[m] = col_ld(SB, kv && y < P.B.h, y, P.B.h, H, k);
+            b[m] = col_ld(EB, kv && y < P.B.h, m, RS);
         }
         fft<-1, NY>(b, t, L, TW);
     }
@@ -1880,7 +1884,7 @@ __global__ void __launch_bounds__(Plan<NY>::TPF * C)
         }
         a[m] = p;
     }
-    __syncthreads();
+    // (the inverse waits at its first exchange's barrier before it overwrites the forward's LDS rows)
     fft<+1, NY>(a, t, L, TW);
     if (kv) {
 #pragma unroll
