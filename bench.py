@@ -126,7 +126,10 @@ def stage_roofline(plan, iters, ms_step):
     stages = {STAGES[i]: {"ms": round(ms[i], 4), "alg_bytes": by[i],
                           "GBs": round(by[i] / (ms[i] * 1e-3) / 1e9, 1) if ms[i] > 0 else None}
               for i in range(5)}
-    roof = {"bound": "hbm", "kernel": KERNELS[STAGES[dom]], "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+    kern = KERNELS[STAGES[dom]]
+    if STAGES[dom] == "row_r2c" and all(b["r2c_np"] > 0 for b in plan.buckets()):
+        kern = "k_row_r2c_u8"  # every bucket transforms its rows with the 8-bit row r2c
+    roof = {"bound": "hbm", "kernel": kern, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
             "alg_bytes_per_launch": by[dom],
             "pipeline": {"alg_bytes_per_step": model,
