@@ -32,6 +32,7 @@ struct xc_plan {
     hipGraphExec_t exec = nullptr;
     double stage_bytes[ST_COUNT] = {0, 0, 0, 0, 0};
     double model_bytes = 0.0;  // whole-pipeline algorithmic bytes
+    double sweep_us = 0.0;     // tuned whole-plan time with the sweeps' share in use (0: not tuned)
     std::vector<int> order;
 };
 
@@ -395,6 +396,39 @@ static int plan_capture(xc_ctx *ctx, xc_plan *p)
     if (ce != hipSuccess) return set_err(ctx, XC_E_DEVICE, "graph capture: %s", hipGetErrorString(ce));
     p->graph = g;
     XC_HIP(ctx, hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    return XC_OK;
+}
+
+// The table sweeps' share of each CU while they run beside the row
+// transforms (wavefronts per CU, 0: one workgroup per item): each candidate
+// is recorded as the plan's graph and timed over replays of the whole plan,
+// and the fastest stays (the caller records the graph again).  A plan whose
+// sweeps do not run beside the transforms has nothing to tune.
+static int tune_sweep_share(xc_ctx *ctx, xc_plan *p)
+{
+    if (!enqueue_forks(ctx, p->ex)) return XC_OK;
+    hipEvent_t e0, e1;
+    XC_HIP(ctx, hipEventCreate(&e0));
+    XC_HIP(ctx, hipEventCreate(&e1));
+    double best = 1e300;
+    int bw = p->ex.sat_waves, st = XC_OK;
+    for (int w : {0, 16, 8}) {
+        p->ex.sat_waves = w;
+        if ((st = plan_capture(ctx, p))) break;
+        auto replay = [&] { return hipGraphLaunch(p->exec, ctx->stream) == hipSuccess ? XC_OK : XC_E_DEVICE; };
+        if ((st = replay())) break;  // warm-up
+        const double us = time_launch(ctx, e0, e1, replay);
+        if (us >= 0.0 && us < best) {
+            best = us;
+            bw = w;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    p->ex.sat_waves = bw;
+    p->sweep_us = best < 1e300 ? best : 0.0;
+    ctx->err.clear();
     return XC_OK;
 }
 
@@ -858,6 +892,10 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
 
     // ---- pinned results + graph ----
     XC_HIP(ctx, hipHostMalloc((void **)&p->h_res, sizeof(xc_result) * n, hipHostMallocDefault));
+    if (flags & XC_PLAN_AUTOTUNE) {
+        st = tune_sweep_share(ctx, p);
+        if (st) return st;
+    }
     return plan_capture(ctx, p);
 }
 
@@ -925,6 +963,27 @@ extern "C" int xc_plan_set_shapes(xc_plan *p, int bucket, int col_c, int col_xcd
     b.r2c_np = b.d_r2c_u8 ? r2c_np : 0;
     b.col_us = 0.0;
     b.r2c_us = 0.0;
+    return plan_capture(ctx, p);
+}
+
+extern "C" int xc_plan_sweep_waves(const xc_plan *p, int32_t *waves, double *us)
+{
+    if (!p || !waves) return XC_E_ARG;
+    *waves = p->ex.sat_waves;
+    if (us) *us = p->sweep_us;
+    return XC_OK;
+}
+
+extern "C" int xc_plan_set_sweep_waves(xc_plan *p, int waves)
+{
+    if (!p) return XC_E_ARG;
+    xc_ctx *ctx = p->ctx;
+    if (waves < 0 || waves > 32) return set_err(ctx, XC_E_ARG, "sweep share of %d wavefronts per CU (0..32)", waves);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    XC_HIP(ctx, hipSetDevice(ctx->device));
+    XC_HIP(ctx, hipStreamSynchronize(ctx->stream));  // a launch in flight keeps its graph
+    p->ex.sat_waves = waves;
+    p->sweep_us = 0.0;
     return plan_capture(ctx, p);
 }
 

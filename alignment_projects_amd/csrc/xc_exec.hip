@@ -88,11 +88,18 @@ void pair_stage_bytes(const PairDev &P, int s, int ba, int bb, bool prep_a, bool
 
 // Table sweeps of every group, by table form: FP64 (S, S^2) entries; FP32
 // entries and exact integer tables (transforms in FP32 only).
+// side_waves > 0: the sweeps run beside the row transforms, each launch
+// holding at most side_waves wavefronts per CU.
 template <typename T>
-static int launch_sats_any(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
+static int launch_sats_any(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups,
+                           int side_waves)
 {
     std::vector<SatGroup> g64, g32, gx;
-    for (const SatGroup &g : groups) (g.tabk == TK_F32 ? g32 : g.tabk == TK_X ? gx : g64).push_back(g);
+    for (SatGroup g : groups) {
+        const int waves = g.threads >= 64 ? g.threads / 64 : 1;
+        g.grid = side_waves > 0 && ctx->num_cu > 0 ? std::max(1, ctx->num_cu * side_waves / waves) : 0;
+        (g.tabk == TK_F32 ? g32 : g.tabk == TK_X ? gx : g64).push_back(g);
+    }
     int st = launch_sats<T, double>(ctx, s, d_tiles, g64);
     if (st) return st;
     if (g32.empty() && gx.empty()) return XC_OK;
@@ -128,8 +135,10 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
             XC_HIP(ctx, hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
             ss = ctx->side;
         }
-        if (ex.ntiles > 0 && (st = launch_sats_any<T>(ctx, ss, ex.d_tiles, ex.sat_groups))) return st;
-        if (!ex.dsat_groups.empty() && (st = launch_sats_any<T>(ctx, ss, ex.d_dtiles, ex.dsat_groups))) return st;
+        const int sw = fork ? ex.sat_waves : 0;
+        if (ex.ntiles > 0 && (st = launch_sats_any<T>(ctx, ss, ex.d_tiles, ex.sat_groups, sw))) return st;
+        if (!ex.dsat_groups.empty() && (st = launch_sats_any<T>(ctx, ss, ex.d_dtiles, ex.dsat_groups, sw)))
+            return st;
         XC_HIP(ctx, hipGetLastError());
         if (fork) XC_HIP(ctx, hipEventRecord(ctx->join_ev, ss));
     }
@@ -157,6 +166,11 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
         for (const Bucket &b : ex.buckets)
             if ((st = launch_finalize<T>(ctx, b.count, ex.d_pairs + b.first, b.tabk))) return st;
     return XC_OK;
+}
+
+bool enqueue_forks(const xc_ctx *ctx, const Exec &ex)
+{
+    return ctx->side && (!ex.sat_groups.empty() || !ex.dsat_groups.empty());
 }
 
 int enqueue(xc_ctx *ctx, const Exec &ex, int s0, int s1)

@@ -44,8 +44,13 @@ struct SatGroup {
     int nb = 1;  // row bands per image (the largest of the group's images)
     int tabk = TK_F64;  // TabKind of the group's tables
     bool stage = false;  // some image is u8: the launch carries the LDS block that stages its rows
+    int grid = 0;        // workgroups of the launch (0: one per item)
     const int *list = nullptr;  // device: tile indices
 };
+
+// Wavefronts per CU that the table sweeps running beside the row transforms
+// may hold by default (of 32; 0: one workgroup per item, no limit).
+constexpr int SAT_SIDE_WAVES = 16;
 
 // Tile indices grouped by sweep shape (sat_cpt(w), sat_threads(w)), table
 // form (tabk[i], a TabKind; missing entries: TK_F64) and whether the image is
@@ -70,6 +75,7 @@ struct Exec {
     const PairDev *d_pairs = nullptr;
     std::vector<Bucket> buckets;
     bool fp64 = false;
+    int sat_waves = SAT_SIDE_WAVES;  // wavefronts per CU of the sweeps beside the transforms (0: no limit)
 };
 
 // Table sweeps of `groups` on stream s: values rounded to T, (S, S^2) entries
@@ -80,6 +86,8 @@ int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::v
 
 // Enqueue stages [s0, s1) of `ex` on ctx->stream.
 int enqueue(xc_ctx *ctx, const Exec &ex, int s0 = 0, int s1 = ST_COUNT);
+// Whether a whole-pipeline enqueue of ex runs its table sweeps on the side stream.
+bool enqueue_forks(const xc_ctx *ctx, const Exec &ex);
 
 // Host-side helpers that size and fill descriptors.
 

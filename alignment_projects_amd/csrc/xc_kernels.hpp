@@ -1369,15 +1369,9 @@ __global__ void __launch_bounds__(NT) k_tp_colsum(const TileDev *__restrict__ ti
 // row parity, one barrier per row) turns column sums into the 2-D prefix.
 // Writes S, Q, Z once.
 template <typename T, typename TB, int CPT, int NT>
-__global__ void __launch_bounds__(NT)
-    k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb, int stage)
+__device__ __forceinline__ void sat_item(const TileDev &D, int band, double *xs, double *xq, int *xz, int *xi,
+                                         uint4 *xblk)
 {
-    __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
-    __shared__ int xz[2 * NT / 64], xi[8 * SATX_RB * NT / 64];
-    extern __shared__ uint4 xblk_dyn[];  // SATX_BLK bytes when stage: u8 sweeps' staged pixel (and mask) rows
-    uint4 *xblk = stage ? xblk_dyn : nullptr;
-    const TileDev D = tiles[list[blockIdx.x / gnb]];  // by value: the table stores cannot alias it
-    const int band = blockIdx.x % gnb;
     if (band >= tile_bands(D)) return;
     if constexpr (is_tabx<TB>()) {
         // the host gives exact tables to unmasked u8 tiles only, swept unbanded
@@ -1399,6 +1393,25 @@ __global__ void __launch_bounds__(NT)
 #define SW_CALL(InT, MK) sat_sweep<T, TB, CPT, NT, InT, MK>(D, band, xs, xq, xz)
     XC_SAT_DISPATCH(SW_CALL)
 #undef SW_CALL
+    }
+}
+
+// The sweep launch over nitems (tile, band) items: list[item / gnb] is the
+// tile and item % gnb the band.  Workgroup b takes items b, b + gridDim.x,
+// ..., so a launch may hold fewer workgroups than items (a sweep beside the
+// row transforms then leaves them wave slots on every CU).
+template <typename T, typename TB, int CPT, int NT>
+__global__ void __launch_bounds__(NT)
+    k_tp_sat(const TileDev *__restrict__ tiles, const int *__restrict__ list, int gnb, int stage, int nitems)
+{
+    __shared__ double xs[2 * NT / 64], xq[2 * NT / 64];
+    __shared__ int xz[2 * NT / 64], xi[8 * SATX_RB * NT / 64];
+    extern __shared__ uint4 xblk_dyn[];  // SATX_BLK bytes when stage: u8 sweeps' staged pixel (and mask) rows
+    uint4 *xblk = stage ? xblk_dyn : nullptr;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        if (it != (int)blockIdx.x) __syncthreads();  // the item before has finished with the LDS
+        const TileDev D = tiles[list[it / gnb]];  // by value: the table stores cannot alias it
+        sat_item<T, TB, CPT, NT>(D, it % gnb, xs, xq, xz, xi, xblk);
     }
 }
 

@@ -8,6 +8,9 @@
 
 namespace xc {
 
+// Workgroups of a sweep launch of n items: one per item, or at most cap (> 0).
+inline int sat_grid(int n, int cap) { return cap > 0 && cap < n ? cap : n; }
+
 template <typename T, typename TB>
 int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::vector<SatGroup> &groups)
 {
@@ -19,8 +22,8 @@ int launch_sats(xc_ctx *ctx, hipStream_t s, const TileDev *d_tiles, const std::v
     case c * 10000 + n:                                                                                             \
         if (g.nb > 1)                                                                                               \
             hipLaunchKernelGGL((k_tp_colsum<T, c, n>), dim3(g.count * g.nb), dim3(n), 0, s, d_tiles, g.list, g.nb); \
-        hipLaunchKernelGGL((k_tp_sat<T, TB, c, n>), dim3(g.count * g.nb), dim3(n), g.stage ? SATX_BLK : 0, s,      \
-                           d_tiles, g.list, g.nb, g.stage ? 1 : 0);                                                  \
+        hipLaunchKernelGGL((k_tp_sat<T, TB, c, n>), dim3(sat_grid(g.count * g.nb, g.grid)), dim3(n),               \
+                           g.stage ? SATX_BLK : 0, s, d_tiles, g.list, g.nb, g.stage ? 1 : 0, g.count * g.nb);       \
         break;
                 XC_SAT(2, 64) XC_SAT(2, 128) XC_SAT(2, 256) XC_SAT(2, 512) XC_SAT(2, 1024) XC_SAT(4, 1024)
                 XC_SAT(8, 1024)

@@ -79,6 +79,8 @@ _OPTIONAL = {
     "xc_plan_destroy": ([C.c_void_p], C.c_int),
     "xc_plan_buckets": ([C.c_void_p, C.POINTER(XcBucketInfo), C.c_int], C.c_int),
     "xc_plan_set_shapes": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+    "xc_plan_sweep_waves": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)], C.c_int),
+    "xc_plan_set_sweep_waves": ([C.c_void_p, C.c_int], C.c_int),
     "xc_server_create": ([C.c_void_p, C.POINTER(XcServeCfg), C.POINTER(C.c_void_p)], C.c_int),
     "xc_server_submit": ([C.c_void_p, C.POINTER(XcPair), C.POINTER(C.c_uint64)], C.c_int),
     "xc_server_poll": ([C.c_void_p, C.c_uint64, C.POINTER(XcResult)], C.c_int),
@@ -693,6 +695,18 @@ class Plan:
         """Set a bucket's launch shapes by hand (xc_plan_set_shapes)."""
         st = self.ctx.L.xc_plan_set_shapes(self.h, bucket, col_c, col_xcd, r2c_rpt, r2c_np)
         self.ctx._check(st, "xc_plan_set_shapes", ok=(abi.XC_OK,))
+
+    def sweep_waves(self):
+        """(wavefronts per CU of the sweeps beside the row transforms, tuned whole-plan us or 0)."""
+        w, us = C.c_int32(0), C.c_double(0.0)
+        st = self.ctx.L.xc_plan_sweep_waves(self.h, C.byref(w), C.byref(us))
+        self.ctx._check(st, "xc_plan_sweep_waves", ok=(abi.XC_OK,))
+        return w.value, us.value
+
+    def set_sweep_waves(self, waves):
+        """Set the sweeps' share by hand (xc_plan_set_sweep_waves)."""
+        st = self.ctx.L.xc_plan_set_sweep_waves(self.h, waves)
+        self.ctx._check(st, "xc_plan_set_sweep_waves", ok=(abi.XC_OK,))
 
     def destroy(self):
         if self.h:

@@ -427,6 +427,8 @@ def run_strips(args, sync):
                               "batched_over_plain": round(value / plain_rate, 1)}
     line["stages"] = stages
     line["autotune"] = plan.buckets()
+    sw, sw_us = plan.sweep_waves()
+    line["sweep_waves"] = {"waves_per_cu": sw, "tuned_us": round(sw_us, 1)}
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
@@ -515,6 +517,8 @@ def run_sweep(args, sync):
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["stages"] = stages
     line["autotune"] = plan.buckets()
+    sw, sw_us = plan.sweep_waves()
+    line["sweep_waves"] = {"waves_per_cu": sw, "tuned_us": round(sw_us, 1)}
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc
@@ -594,6 +598,8 @@ def run_uneven(args, sync):
                           "p99": round(1e3 * float(np.percentile(lat, 99)), 3), "unit": "per batch"}
     line["stages"] = stages
     line["autotune"] = plan.buckets()
+    sw, sw_us = plan.sweep_waves()
+    line["sweep_waves"] = {"waves_per_cu": sw, "tuned_us": round(sw_us, 1)}
     line["roofline"] = roofline
     line["cpu_baseline"] = cpu
     line["accuracy"] = acc

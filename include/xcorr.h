@@ -364,6 +364,18 @@ int xc_plan_buckets(const xc_plan *plan, xc_bucket_info *out, int cap);
  * speed only, never results.  Waits for a launch in flight, then records the
  * plan's graph again.  XC_E_SHAPE for a shape the bucket does not have. */
 int xc_plan_set_shapes(xc_plan *plan, int bucket, int col_c, int col_xcd, int r2c_rpt, int r2c_np);
+/* The share of each CU that a plan's table sweeps may hold while they run
+ * beside the row transforms (plans that prepare their tiles or sweep rotated
+ * copies in every run): at most `waves` wavefronts per CU, the rest of the
+ * CU's wave slots left to the transforms; 0 = one workgroup per tile, no
+ * limit.  The default is 16; with XC_PLAN_AUTOTUNE the plan times replays of
+ * its whole graph with 0, 16 and 8 and keeps the fastest (*us: that time in
+ * microseconds, 0 when not tuned).  Changes speed only, never results.  (No
+ * counterpart in the reference, which prepares tables per call on the CPU.) */
+int xc_plan_sweep_waves(const xc_plan *plan, int32_t *waves, double *us);
+/* Set the share by hand (0..32, else XC_E_ARG); waits for a launch in
+ * flight, then records the plan's graph again. */
+int xc_plan_set_sweep_waves(xc_plan *plan, int waves);
 int xc_plan_destroy(xc_plan *plan);
 
 /*

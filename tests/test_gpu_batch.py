@@ -237,3 +237,48 @@ def test_every_tuned_shape_matches_default(ctx):
     for ta, tb in tiles:
         ta.release()
         tb.release()
+
+
+def test_every_sweep_share_matches_default(ctx):
+    """The table sweeps' share of each CU beside the row transforms (xc_plan_set_sweep_waves): at every share,
+    including ones small enough that each sweep workgroup walks several tiles in turn (1 wavefront per CU: 16
+    workgroups of 1024 threads for 40 strip tiles; 64 workgroups of 256 threads for the masked tiles 257..400
+    wide, about half of 160), every field of every result is bitwise the default share's.  Exact integer tables on u8 strips;
+    FP64 (S, S^2) pairs and count tables on masked tiles with EnoughPoints.  The autotuned share is one of its
+    candidates, and shares outside 0..32 are refused."""
+    from alignment_projects_amd.xcorr import XcorrError, make_pair
+    fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak", "nx", "ny")
+    strips = _strips(20, 2048, 64, 20, 970)
+    masked = [synth.uneven_pair(2000 + i, lo=100, hi=400, smax=12) for i in range(80)]
+    cases = []
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in strips]
+    cases.append(("strips", tiles, [make_pair(ta, tb, rgn=abi.rgn_min(512, 16)) for ta, tb in tiles]))
+    mt = [(ctx.tile_put(a, mask=ma), ctx.tile_put(b, mask=mb)) for a, ma, b, mb, _ in masked]
+    cases.append(("masked", mt, [make_pair(ta, tb, cnt=abi.cnt_enough_points(a.size // 4))
+                                 for (ta, tb), (a, _, _, _, _) in zip(mt, masked)]))
+    for name, tl, pairs in cases:
+        print(f"sweep shares: {name}, {len(pairs)} pairs", flush=True)
+        plan = ctx.plan(pairs, refresh_tiles=True)
+        assert plan.sweep_waves()[0] == 16  # the default share
+        ref = plan.run()
+        assert sum(r.status == abi.XC_OK for r in ref) >= len(ref) // 2, name
+        for w in (1, 2, 8, 32, 0):
+            plan.set_sweep_waves(w)
+            assert plan.sweep_waves() == (w, 0.0)
+            got = plan.run()
+            for i, (a, g) in enumerate(zip(ref, got)):
+                assert all(getattr(a, f) == getattr(g, f) for f in fields), (name, w, i, a.as_dict(), g.as_dict())
+        for bad in (-1, 33):
+            with pytest.raises(XcorrError):
+                plan.set_sweep_waves(bad)
+        plan.destroy()
+        tuned = ctx.plan(pairs, refresh_tiles=True, autotune=True)
+        w, us = tuned.sweep_waves()
+        assert w in (0, 16, 8) and us > 0.0, (name, w, us)
+        got = tuned.run()
+        for i, (a, g) in enumerate(zip(ref, got)):
+            assert all(getattr(a, f) == getattr(g, f) for f in fields), (name, "tuned", i)
+        tuned.destroy()
+        for ta, tb in tl:
+            ta.release()
+            tb.release()
