@@ -1664,8 +1664,18 @@ __global__ void __launch_bounds__((R2cCfg<float, NH>::threads(RPT)))
                 unsigned u = 0;
                 if (r < it.nrows) {
                     if constexpr (GEN) {
-                        if (x < it.w && (!mrow || gld(mrow, x))) u |= gld(row, x) | 0x10000u;
-                        if (x + 1 < it.w && (!mrow || gld(mrow, x + 1))) u |= ((unsigned)gld(row, x + 1) << 8) | 0x20000u;
+                        const bool even = ((reinterpret_cast<uintptr_t>(row) | reinterpret_cast<uintptr_t>(mrow)) & 1) == 0;
+                        if (even && x + 1 < it.w) {
+                            // both pixels (and both mask bytes) in one 16-bit load each; a pixel whose mask
+                            // byte is 0 is dropped as below
+                            const unsigned pp = gld(r2, p);
+                            const unsigned mm = mrow ? gld(reinterpret_cast<const uint16_t *>(mrow), p) : 0x0101u;
+                            if (mm & 0x00ffu) u |= (pp & 0x00ffu) | 0x10000u;
+                            if (mm & 0xff00u) u |= (pp & 0xff00u) | 0x20000u;
+                        } else {
+                            if (x < it.w && (!mrow || gld(mrow, x))) u |= gld(row, x) | 0x10000u;
+                            if (x + 1 < it.w && (!mrow || gld(mrow, x + 1))) u |= ((unsigned)gld(row, x + 1) << 8) | 0x20000u;
+                        }
                         if (!mrow && (reinterpret_cast<uintptr_t>(row) & 1) == 0) u |= 0x40000u;
                     } else {
                         if (x + 1 < it.w) u = gld(r2, p);
