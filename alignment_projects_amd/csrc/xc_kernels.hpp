@@ -537,6 +537,48 @@ __device__ __forceinline__ void stats_rows_u8x16(const ImgView &V, int y0, int y
     c += (double)n;
 }
 
+// Masked u8 rows whose pixel and mask rows start on 16-byte boundaries: the
+// band's whole 16-byte pieces (row, piece) are dealt out over all threads,
+// 16 pixels and their 16 mask bytes a pair of loads; the columns past the
+// last whole piece byte by byte.  Integer sums (exact).
+__device__ __forceinline__ void stats_rows_u8x16m(const ImgView &V, int y0, int y1, int w, int tid, double &a,
+                                                  double &b, double &c)
+{
+    const uint8_t *px = reinterpret_cast<const uint8_t *>(V.data), *mk = V.mask;
+    const long long st = V.stride, ms = V.mstride;
+    const int w16 = w / 16, tw = w - 16 * w16, rows = y1 - y0;
+    unsigned long long sa = 0, sq = 0, n = 0;
+    if (w16 > 0)
+        for (int i = tid; i < rows * w16; i += 256) {
+            const int y = y0 + i / w16, x = 16 * (i % w16);
+            const uint4 q = gld16(px + (long long)y * st + x), m = gld16(mk + (long long)y * ms + x);
+            const unsigned int wd[4] = {q.x, q.y, q.z, q.w}, wm[4] = {m.x, m.y, m.z, m.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if ((wm[j] >> (8 * k)) & 0xffu) {
+                        const unsigned int v = (wd[j] >> (8 * k)) & 0xffu;
+                        sa += v;
+                        sq += v * v;
+                        n += 1;
+                    }
+        }
+    if (tw > 0)
+        for (int i = tid; i < rows * tw; i += 256) {
+            const int y = y0 + i / tw, x = 16 * w16 + i % tw;
+            if (gld(mk, (long long)y * ms + x)) {
+                const unsigned int v = gld(px, (long long)y * st + x);
+                sa += v;
+                sq += v * v;
+                n += 1;
+            }
+        }
+    a += (double)sa;
+    b += (double)sq;
+    c += (double)n;
+}
+
 // u8 rows stored back to back (stride w, and the mask's too): the band
 // [i0, i1) as one run of bytes, eight pixels (and mask bytes) in flight per
 // thread whatever the row width, summed in integers (exact, as the FP64 sums
@@ -612,8 +654,12 @@ static __global__ void __launch_bounds__(256) k_tp_stats(const TileDev *__restri
     double a = 0.0, b = 0.0, c = 0.0;
     const bool wide = D.v.dtype == XC_U8 && !D.v.mask && (reinterpret_cast<uintptr_t>(D.v.data) & 15) == 0 &&
                       (D.v.stride & 15) == 0;
+    const bool wide_m = D.v.dtype == XC_U8 && D.v.mask &&
+                        ((reinterpret_cast<uintptr_t>(D.v.data) | reinterpret_cast<uintptr_t>(D.v.mask) |
+                          (uintptr_t)D.v.stride | (uintptr_t)D.v.mstride) & 15) == 0;
     const bool flat = D.v.dtype == XC_U8 && D.v.stride == w && (!D.v.mask || D.v.mstride == w);
     if (wide) stats_rows_u8x16(D.v, y0, y1, w, tid, a, b, c);
+    else if (wide_m) stats_rows_u8x16m(D.v, y0, y1, w, tid, a, b, c);
     else if (flat) stats_flat_u8(D.v, (long long)y0 * w, (long long)y1 * w, tid, a, b, c);
     else if (D.v.dtype == XC_U8) stats_rows8_u8(D.v, y0, y1, w, tid, a, b, c);
     else switch (D.v.dtype) {  // uniform per workgroup
