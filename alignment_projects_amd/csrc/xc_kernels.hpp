@@ -502,18 +502,21 @@ __device__ __forceinline__ void stats_rows(const ImgView &V, int y0, int y1, int
     }
 }
 
-// Unmasked u8 rows with 16-byte aligned starts: 16 pixels per load, summed
-// in integers (exact, as the FP64 sums of integer pixels are).
+// Unmasked u8 rows with 16-byte aligned starts: the band's whole 16-byte
+// pieces (row, piece) dealt out over all threads, 16 pixels a load; the
+// columns past the last whole piece byte by byte.  Integer sums (exact, as
+// the FP64 sums of integer pixels are).
 __device__ __forceinline__ void stats_rows_u8x16(const ImgView &V, int y0, int y1, int w, int tid, double &a, double &b,
                                                  double &c)
 {
+    const uint8_t *px = reinterpret_cast<const uint8_t *>(V.data);
+    const long long st = V.stride;
+    const int w16 = w / 16, tw = w - 16 * w16, rows = y1 - y0;
     unsigned long long sa = 0, sq = 0, n = 0;
-    const int w16 = w / 16;
-    for (int y = y0; y < y1; ++y) {
-        const uint8_t *row = reinterpret_cast<const uint8_t *>(V.data) + (long long)y * V.stride;
-        const uint4 *r16 = reinterpret_cast<const uint4 *>(row);
-        for (int i = tid; i < w16; i += 256) {
-            const uint4 q = gld16(r16 + i);
+    if (w16 > 0)
+        for (int i = tid; i < rows * w16; i += 256) {
+            const int y = y0 + i / w16, x = 16 * (i % w16);
+            const uint4 q = gld16(px + (long long)y * st + x);
             const unsigned int wd[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
             for (int j = 0; j < 4; ++j)
@@ -525,13 +528,14 @@ __device__ __forceinline__ void stats_rows_u8x16(const ImgView &V, int y0, int y
                 }
             n += 16;
         }
-        for (int x = 16 * w16 + tid; x < w; x += 256) {
-            const unsigned int v = row[x];
+    if (tw > 0)
+        for (int i = tid; i < rows * tw; i += 256) {
+            const int y = y0 + i / tw, x = 16 * w16 + i % tw;
+            const unsigned int v = gld(px, (long long)y * st + x);
             sa += v;
             sq += v * v;
             n += 1;
         }
-    }
     a += (double)sa;
     b += (double)sq;
     c += (double)n;
