@@ -282,3 +282,29 @@ def test_every_sweep_share_matches_default(ctx):
         for ta, tb in tl:
             ta.release()
             tb.release()
+
+
+def test_long_column_shapes_match_default(ctx):
+    """2048-point columns (tiles 1024 rows tall, the uneven batch's largest buckets): the column pass at 1, 2 and 4
+    columns per workgroup (4: 147 KB of LDS, one workgroup per CU), XCD order on and off.  Every field of every
+    result is bitwise the default shape's."""
+    from alignment_projects_amd.xcorr import make_pair
+    data = _strips(3, 256, 1024, 16, 990)
+    tiles = [(ctx.tile_put(a), ctx.tile_put(b)) for a, b, _ in data]
+    pairs = [make_pair(ta, tb, rgn=abi.rgn_min(64, 256)) for ta, tb in tiles]
+    plan = ctx.plan(pairs)
+    bk = plan.buckets()
+    assert [b["ny"] for b in bk] == [2048], bk
+    ref = plan.run()
+    assert all(r.status == abi.XC_OK for r in ref), [r.as_dict() for r in ref]
+    fields = ("r", "dx", "dy", "status", "lag_x", "lag_y", "sub_x", "sub_y", "peak")
+    for c in (1, 2, 4):
+        for x in (0, 1):
+            plan.set_shapes(0, col_c=c, col_xcd=x)
+            got = plan.run()
+            for i, (a, g) in enumerate(zip(ref, got)):
+                assert all(getattr(a, f) == getattr(g, f) for f in fields), (c, x, i, a.as_dict(), g.as_dict())
+    plan.destroy()
+    for ta, tb in tiles:
+        ta.release()
+        tb.release()
