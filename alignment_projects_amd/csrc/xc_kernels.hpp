@@ -2369,6 +2369,102 @@ __device__ __forceinline__ double ncc_value(const PairDev &P, int x, int y, doub
     return ncc_lag<T, TB>(P, q, x, raw, &cand, &odd);
 }
 
+#pragma clang fp contract(off)
+// S mode's value of a lag, eval_lag's raw / (Nx Ny): a product of its own
+// (never fused into the sums it feeds).
+__device__ __forceinline__ double s_value(double raw, double invNxy) { return raw * invNxy; }
+#pragma clang fp contract(on)
+
+// One S-mode row (CorrImages / CThmScan sweeps; no exact width, no table
+// rule): lag_value's results for each lag, in the same order, with the
+// row's work hoisted.  Every non-dead lag's value feeds the sums; a lag is a
+// candidate only inside the row's legal x interval -- the region rule's
+// terms and the count-area bound, as for NCC rows, without a disc (the lag
+// window is the linear range, so the overlap width never clamps) -- and
+// there its count corners come from the count tables' band rows, then the
+// ellipse (MaskA) and the mincor test.
+// band_sum over a count table's band rows, read through global loads.
+__device__ __forceinline__ int band_z(const int32_t *top, const int32_t *bot, int L, int R)
+{
+    int v = ld_z(top, R);
+    if (L > 0) v -= ld_z(top, L - 1);
+    if (bot) {
+        v -= ld_z(bot, R);
+        if (L > 0) v += ld_z(bot, L - 1);
+    }
+    return v;
+}
+
+template <typename T, int TPF>
+__device__ __forceinline__ void s_scan(const PairDev &P, const RowBox &rb, int y, const T *R, int t, int NX,
+                                       bool centred, double &bestv, long long &bestk, double &ssum, double &ssq,
+                                       int &flags)
+{
+    const RuleDev &g = P.rule;
+    long long m = 1;
+    bool ok = true;
+    if (g.rgn_enabled) {
+        ok = rb.olh >= g.min_h && (g.exact_h <= 0 || rb.olh == g.exact_h);
+        if (g.min_w > m) m = g.min_w;
+        if (g.min_area >= 0 && g.min_area / rb.olh + 1 > m) m = g.min_area / rb.olh + 1;
+        if ((g.or_w_ge > 0 || g.or_h_gt > 0) && !(rb.olh > g.or_h_gt) && g.or_w_ge > m) m = g.or_w_ge;
+    }
+    if (g.cnt_enabled) {
+        const long long K = g.k1 > g.k2 ? g.k1 : g.k2;
+        if (K >= 0 && K / rb.olh + 1 > m) m = K / rb.olh + 1;
+    }
+    ok = ok && rb.olh > 0 && m <= (P.w1 < P.w2 ? P.w1 : P.w2);
+    const int x0 = ok ? (int)(m - P.w1) : 1, x1 = ok ? (int)(P.w2 - m) : 0;
+    const int32_t *zAt = nullptr, *zAb = nullptr, *zBt = nullptr, *zBb = nullptr;
+    if (g.cnt_enabled) {
+        zAt = P.ZA + (long long)rb.a_t * P.w1;
+        zBt = P.ZB + (long long)rb.b_t * P.w2;
+        zAb = rb.a_b > 0 ? P.ZA + (long long)(rb.a_b - 1) * P.w1 : nullptr;
+        zBb = rb.b_b > 0 ? P.ZB + (long long)(rb.b_b - 1) * P.w2 : nullptr;
+    }
+    const double invNxy = 1.0 / ((double)P.Nx * (double)P.Ny);
+    for (int ix = t; ix < NX; ix += TPF) {
+        bool dead;
+        const int x = lag_x_of_raster(P, ix, &dead);
+        if (dead) continue;
+        const double val = s_value((double)R[ix], invNxy);
+        ssum += val;
+        ssq += val * val;
+        bool cand = x >= x0 && x <= x1;
+        if (cand && g.cnt_enabled) {
+            const int BL = x > 0 ? x : 0, BR = (P.w2 - 1 < x + P.w1 - 1) ? P.w2 - 1 : x + P.w1 - 1;
+            const int c1 = band_z(zAt, zAb, BL - x, BR - x), c2 = band_z(zBt, zBb, BL, BR);
+            cand = (long long)c1 > g.k1 && (long long)c2 > g.k2;
+        }
+        if (cand && P.Rx > 0 && P.Ry > 0) {
+            // MaskA (Correlation.cpp:2011-2054) in centred coordinates
+            const int ox = P.Ox - P.offx + P.nnegx, oy = P.Oy - P.offy + P.nnegy;
+            const int cx = x + P.nnegx, cy = y + P.nnegy;
+            if (cy < oy - P.Ry || cy > oy + P.Ry || cx < ox - P.Rx || cx > ox + P.Rx) cand = false;
+            else {
+                const double rx2 = (double)P.Rx * P.Rx, asq = rx2 / ((double)P.Ry * P.Ry);
+                const int xx = cx - ox, yy = cy - oy;
+                if (xx * xx + asq * yy * yy > rx2) cand = false;
+            }
+        }
+        if (cand && P.mincor > 0.0 && !(val > P.mincor)) cand = false;
+        if (P.rimg) {
+            // the centred R image and its candidate flags (CorrImagesF)
+            const long long ir = (long long)(x + P.nnegx) + (long long)P.wR * (y + P.nnegy);
+            gst(P.rimg, ir, val);
+            gst(P.cimg, ir, (uint8_t)(cand ? 1 : 0));
+        }
+        if (cand) {
+            flags |= 1;
+            const long long key = centred ? (long long)(x + P.nnegx) : (long long)ix;
+            if (better(val, key, bestv, bestk)) {
+                bestv = val;
+                bestk = key;
+            }
+        }
+    }
+}
+
 // CNT: the variant for launches with count-rule NCC pairs (scans the count
 // tables in the fast NCC path); without it such rows take the general path.
 template <typename T, int NH, bool CNT, typename TB>
@@ -2457,6 +2553,8 @@ __global__ void __launch_bounds__(RowCfg<NH>::THREADS)
                 }
             }
         }
+    } else if (valid && P.mode == PM_S && P.rule.exact_w <= 0 && !P.rule.table) {
+        s_scan<T, TPF>(P, rb, y, R, t, NX, centred, bestv, bestk, ssum, ssq, flags);
     } else if (valid) {
         for (int ix = t; ix < NX; ix += TPF) {
             bool cand, win, odd;
