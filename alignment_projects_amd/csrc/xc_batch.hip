@@ -442,6 +442,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     std::vector<const xc_tile *> tiles;
     std::map<const xc_tile *, int> tile_idx;
     std::vector<RotDev> rots;
+    std::vector<int> rot_src;                      // per copy: its source (index into nsrcs)
+    std::map<const xc_tile *, int> nsrc_idx;
+    std::vector<const xc_tile *> nsrcs;            // the copies' distinct sources
     std::map<RotKey, int> rot_idx;
     std::vector<int> pair_rot(n, -1);
     std::vector<bool> rot_cnt;  // the rotated copy needs a count table
@@ -510,6 +513,11 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
                 if (R.w > TP_MAX_W) return set_err(ctx, XC_E_SHAPE, "pair %d: rotated A wider than %d", i, TP_MAX_W);
                 ri = (int)rots.size();
                 rots.push_back(R);
+                if (!nsrc_idx.count(a)) {
+                    nsrc_idx[a] = (int)nsrcs.size();
+                    nsrcs.push_back(a);
+                }
+                rot_src.push_back(nsrc_idx[a]);
                 rot_cnt.push_back(false);
                 rot_idx[key] = ri;
             } else {
@@ -633,6 +641,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
         i_rS[r] = lay.add(rot_sq[r] ? (dkind[r] == TK_F32 ? sizeof(float2) : sizeof(double2)) * px : 16);
         i_rZ[r] = lay.add(rot_cnt[r] ? sizeof(int32_t) * px : 8);
     }
+    std::vector<size_t> i_nsrc(nsrcs.size());
+    for (size_t k = 0; k < nsrcs.size(); ++k)
+        i_nsrc[k] = lay.add(sizeof(double) * (size_t)nsrcs[k]->full.w * nsrcs[k]->full.h);
     std::vector<size_t> i_derS(ders.size());
     for (size_t d = 0; d < ders.size(); ++d) i_derS[d] = lay.add(sizeof(double2) * (size_t)ders[d]->w * ders[d]->h);
     size_t i_res = lay.add(sizeof(xc_result) * n);
@@ -669,6 +680,10 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     int nrw = 0;
     for (const RotDev &R : rots) nrw += (int)((rot_splat_units(R.w, R.h) + 255) / 256);
     size_t i_rw = lay.add(sizeof(int2) * nrw + 16);
+    size_t i_rn = lay.add(sizeof(RotNorm) * nsrcs.size() + 16);
+    int nnw = 0;
+    for (const xc_tile *t : nsrcs) nnw += (int)(((long long)t->full.w * t->full.h + 255) / 256);
+    size_t i_rnw = lay.add(sizeof(int2) * nnw + 16);
     std::vector<size_t> i_wr(buckets.size()), i_wc(buckets.size()), i_wc2(buckets.size()), i_w8(buckets.size(), 0);
     std::vector<bool> r2c8(buckets.size(), false), r2cg(buckets.size(), false);
     std::vector<int> starts;
@@ -727,9 +742,21 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     TileDev *hd = (TileDev *)hat(i_dtiles);
     int2 *hrw = (int2 *)hat(i_rw);
     int w = 0;
+    {
+        RotNorm *hn = (RotNorm *)hat(i_rn);
+        int2 *hnw = (int2 *)hat(i_rnw);
+        int wn = 0;
+        for (size_t k = 0; k < nsrcs.size(); ++k) {
+            hn[k].src = nsrcs[k]->full;
+            hn[k].dst = (double *)at(i_nsrc[k]);
+            const long long np = ((long long)nsrcs[k]->full.w * nsrcs[k]->full.h + 255) / 256;
+            for (long long b = 0; b < np; ++b) hnw[wn++] = make_int2((int)k, (int)b);
+        }
+    }
     for (size_t r = 0; r < rots.size(); ++r) {
         RotDev R = rots[r];
         R.out = at(i_rimg[r]);
+        R.nsrc = (const double *)at(i_nsrc[rot_src[r]]);
         hr[r] = R;
         for (int b = 0; b < (int)((rot_splat_units(R.w, R.h) + 255) / 256); ++b) hrw[w++] = make_int2((int)r, b);
         TileDev T;
@@ -853,6 +880,9 @@ static int plan_build(xc_ctx *ctx, const xc_pair *pairs, int n, int flags, xc_pl
     p->ex.d_rot = (const RotDev *)at(i_rot);
     p->ex.d_rot_work = (const int2 *)at(i_rw);
     p->ex.n_rot_work = nrw;
+    p->ex.d_rnorm = (const RotNorm *)at(i_rn);
+    p->ex.d_rnorm_work = (const int2 *)at(i_rnw);
+    p->ex.n_rnorm_work = nnw;
     p->ex.d_dtiles = (const TileDev *)at(i_dtiles);
     p->ex.dsat_groups = dgroups;
 

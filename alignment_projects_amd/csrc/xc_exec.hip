@@ -125,6 +125,7 @@ template <typename T> static int enqueue_t(xc_ctx *ctx, const Exec &ex, int s0, 
         XC_HIP(ctx, hipGetLastError());
     }
     if (prep && ex.n_rot_work > 0) {
+        hipLaunchKernelGGL(k_rot_norm, dim3(ex.n_rnorm_work), dim3(256), 0, s, ex.d_rnorm, ex.d_rnorm_work);
         hipLaunchKernelGGL(k_rot_splat<T>, dim3(ex.n_rot_work), dim3(256), 0, s, ex.d_rot, ex.d_rot_work);
         XC_HIP(ctx, hipGetLastError());
     }

@@ -70,6 +70,9 @@ struct Exec {
     const RotDev *d_rot = nullptr;
     const int2 *d_rot_work = nullptr;
     int n_rot_work = 0;
+    const RotNorm *d_rnorm = nullptr;  // the copies' sources, normalised before the splat
+    const int2 *d_rnorm_work = nullptr;
+    int n_rnorm_work = 0;
     const TileDev *d_dtiles = nullptr;
     std::vector<SatGroup> dsat_groups;
     const PairDev *d_pairs = nullptr;

@@ -1513,6 +1513,21 @@ __device__ __forceinline__ bool rot_weight(int ix, int iy, double alpha, double 
 }
 
 #pragma clang fp contract(off)
+// The normalised values (raw - mean) / sd of a rotated copy's source, the
+// splat's per-point quotient evaluated once per run for every copy and
+// every output pixel that reads the point (absent points too: the splat
+// skips them).
+static __global__ void __launch_bounds__(256) k_rot_norm(const RotNorm *__restrict__ srcs, const int2 *__restrict__ work)
+{
+    const int2 wi = work[blockIdx.x];
+    const RotNorm N = srcs[wi.x];
+    const long long i = (long long)wi.y * 256 + threadIdx.x;
+    if (i >= (long long)N.src.w * N.src.h) return;
+    const int y = (int)(i / N.src.w), x = (int)(i - (long long)y * N.src.w);
+    const double mean = N.src.msd[0], sd = N.src.msd[1];
+    gst(N.dst, i, (load_raw(N.src, x, y) - mean) / sd);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ rots, const int2 *__restrict__ work)
 {
@@ -1523,7 +1538,6 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
     const int wp = (R.w + 1) / 2;
     const int py = (int)(unit / wp), px = 2 * (int)(unit - (long long)py * wp);
     const bool two = px + 1 < R.w;
-    const double mean = R.src.msd[0], sd = R.src.msd[1];
     // grid points whose mapped position lies in [px-1, px+1+two) x [py-1, py+1):
     // they lie in the inverse image of that rectangle (for a rotation, its
     // transpose), so in the integer points of its bounding box, taken with a
@@ -1553,7 +1567,7 @@ __global__ void __launch_bounds__(256) k_rot_splat(const RotDev *__restrict__ ro
             const bool h0 = rot_weight(ix, iy, alpha, beta, px, py, &w0);
             const bool h1 = two && rot_weight(ix, iy, alpha, beta, px + 1, py, &w1);
             if (!h0 && !h1) continue;
-            const double v = (load_raw(R.src, gx, gy) - mean) / sd;
+            const double v = gld(R.nsrc, (long long)gy * R.src.w + gx);
             if (h0) acc0 += w0 * v;
             if (h1) acc1 += w1 * v;
         }

@@ -86,6 +86,13 @@ struct RotDev {
     int32_t L, B;
     int32_t w, h;
     void *out;
+    const double *nsrc;     // src's normalised values (raw - mean) / sd, src.w a row (RotNorm's dst)
+};
+
+// A rotated copy's source, normalised once per run for all of its copies.
+struct RotNorm {
+    ImgView src;
+    double *dst;            // src.w x src.h values, src.w a row
 };
 
 // Work item of the 8-bit row r2c (FP32 transforms of u8 rows that fit the
